@@ -1,0 +1,118 @@
+"""Seeded synthetic halos: NFW profile plus Plummer subhalos in a unit box.
+
+Used to build the workloads of the density stage (one halo, a catalogue of
+halos) with float32 positions/masses.  Every draw comes from numpy's PCG64
+seeded with ``20261203 + config_id`` so all runs see the same particles.
+
+The NFW sample inverts the enclosed-mass fraction
+    m(x) = ln(1 + c x) - c x / (1 + c x),  x = r / r200 in [0, 1]
+by bisection; directions are isotropic.  A Plummer subhalo of scale ``a``
+draws r = a / sqrt(u^(-2/3) - 1) truncated at 10 a.
+"""
+import math
+
+import numpy as np
+
+SEED_BASE = 20261203
+
+
+def rng_for(config_id):
+    return np.random.Generator(np.random.PCG64(SEED_BASE + int(config_id)))
+
+
+def _isotropic(rng, n):
+    u = rng.uniform(-1.0, 1.0, n)
+    phi = rng.uniform(0.0, 2.0 * math.pi, n)
+    s = np.sqrt(np.maximum(0.0, 1.0 - u * u))
+    return np.stack([s * np.cos(phi), s * np.sin(phi), u], axis=1)
+
+
+def _nfw_mass(x, c):
+    return np.log1p(c * x) - c * x / (1.0 + c * x)
+
+
+def nfw_radii(rng, n, c, r200):
+    """Radii of n particles following NFW(c) truncated at r200."""
+    target = rng.uniform(0.0, 1.0, n) * _nfw_mass(1.0, c)
+    lo = np.zeros(n)
+    hi = np.ones(n)
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        below = _nfw_mass(mid, c) < target
+        lo = np.where(below, mid, lo)
+        hi = np.where(below, hi, mid)
+    return 0.5 * (lo + hi) * r200
+
+
+def plummer_radii(rng, n, a, rmax_over_a=10.0):
+    """Plummer radii truncated at rmax_over_a * a (inverse CDF on the
+    truncated mass range)."""
+    q = rmax_over_a
+    umax = q ** 3 / (1.0 + q * q) ** 1.5
+    u = rng.uniform(0.0, umax, n)
+    u = np.maximum(u, 1e-300)
+    return a / np.sqrt(u ** (-2.0 / 3.0) - 1.0)
+
+
+def halo(rng, n, c=10.0, r200=0.4, subs=(), center=(0.5, 0.5, 0.5)):
+    """One halo of n particles: NFW(c, r200) plus Plummer subhalos given as
+    (mass_fraction, scale_a, radius_from_centre) tuples.  Returns float32
+    positions (n, 3) and equal float32 masses 1/n."""
+    counts = [int(round(f * n)) for f, _, _ in subs]
+    n_nfw = n - sum(counts)
+    if n_nfw < 0:
+        raise ValueError("subhalo fractions exceed 1")
+    parts = []
+    r = nfw_radii(rng, n_nfw, c, r200)
+    parts.append(_isotropic(rng, n_nfw) * r[:, None])
+    for (f, a, rad), m in zip(subs, counts):
+        if m == 0:
+            continue
+        off = _isotropic(rng, 1)[0] * rad
+        rr = plummer_radii(rng, m, a)
+        parts.append(off[None, :] + _isotropic(rng, m) * rr[:, None])
+    pos = np.concatenate(parts, axis=0) + np.asarray(center)[None, :]
+    perm = rng.permutation(n)
+    pos = pos[perm].astype(np.float32)
+    mass = np.full(n, 1.0 / n, dtype=np.float32)
+    return pos, mass
+
+
+def single_halo_config(config_id):
+    """The one-halo workloads: returns (pos, mass, spec dict).
+
+    1: 131072 particles, NFW c=10 r200=0.4 (85%) + 3 Plummer (5% each,
+       a=0.01, radii U(0.1,0.3)); grid 64^3, RG = 2 cells, k = 32.
+    2: 4194304 particles, NFW c=10 + 20 Plummer with log-uniform mass
+       fractions in [0.1%, 3%]; grid 256^3, RG = 1.5 cells, k = 32.
+    4: 1048576 particles, NFW c=8 + 10 Plummer (log-uniform 0.1-3%);
+       grid 128^3, RG = 2 cells, k = 32.
+    The grid spans the unit box exactly (origin 0, cellsize 1/n); RG is the
+    reference's gaussian_Smoothing argument in box units (RG cells * 1/n).
+    """
+    rng = rng_for(config_id)
+    if config_id == 1:
+        n, c, ng, rgc, nsub = 131072, 10.0, 64, 2.0, 3
+        fr = [0.05] * 3
+    elif config_id == 2:
+        n, c, ng, rgc, nsub = 4194304, 10.0, 256, 1.5, 20
+        fr = list(np.exp(rng.uniform(math.log(0.001), math.log(0.03), nsub)))
+    elif config_id == 4:
+        n, c, ng, rgc, nsub = 1048576, 8.0, 128, 2.0, 10
+        fr = list(np.exp(rng.uniform(math.log(0.001), math.log(0.03), nsub)))
+    else:
+        raise ValueError("unknown single-halo config %r" % config_id)
+    radii = rng.uniform(0.1, 0.3, nsub)
+    subs = [(f, 0.01, r) for f, r in zip(fr, radii)]
+    pos, mass = halo(rng, n, c=c, r200=0.4, subs=subs)
+    spec = dict(n=n, nx=ng, ny=ng, nz=ng, xmin=0.0, ymin=0.0, zmin=0.0,
+                cellsize=1.0 / ng, RG=rgc / ng, rg_cells=rgc, k=32)
+    return pos, mass, spec
+
+
+def small_halo(n, seed=0, nsub=2):
+    """Small clustered halo for tests (not a named workload)."""
+    rng = np.random.Generator(np.random.PCG64(SEED_BASE + 1000 + seed))
+    radii = rng.uniform(0.1, 0.25, nsub)
+    subs = [(0.05, 0.02, r) for r in radii]
+    return halo(rng, n, c=6.0, r200=0.35, subs=subs)

@@ -1,0 +1,230 @@
+"""ctypes bindings for the CPU oracle (oracle/liboracle.so) and, when built,
+the reference-source build (oracle/_ref/libpgalf_ref.so).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg -- never by the galaxyfinder_amd package.
+
+Functions follow NewGalFinder: tsc_omp2.c (deposit, read-back), gsmooth.c
+(Gaussian smoothing), nnost.c (exact kNN, W4, kNN densities).  Arrays are
+numpy; positions and masses are float64 as in the reference (dptype=double).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_longlong)
+_i64 = ctypes.c_longlong
+_d = ctypes.c_double
+_i = ctypes.c_int
+
+
+def build(ref=False):
+    """Compile the oracle (and the reference build when asked and present)."""
+    subprocess.run(["make", "-s", "-C", _HERE, "liboracle.so"], check=True)
+    if ref and os.path.isdir("/root/reference/NewGalFinder"):
+        subprocess.run(["make", "-s", "-C", _HERE, "ref"], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.orc_tsc_deposit_f32.restype = _i64
+        L.orc_tsc_deposit_f32.argtypes = [_dp, _dp, _dp, _dp, _ip, _i64, _i, _fp,
+                                          _i, _i, _i, _d, _d, _d, _d, _i]
+        L.orc_tsc_deposit_f64.restype = _i64
+        L.orc_tsc_deposit_f64.argtypes = [_dp, _dp, _dp, _dp, _ip, _i64, _i, _dp,
+                                          _i, _i, _i, _d, _d, _d, _d]
+        L.orc_findden.restype = _i64
+        L.orc_findden.argtypes = [_dp, _dp, _dp, _i64, _fp, _d, _d, _d, _d, _fp,
+                                  _i, _i, _i]
+        L.orc_gsmooth.restype = None
+        L.orc_gsmooth.argtypes = [_fp, _i, _i, _i, _d, ctypes.c_float]
+        L.orc_gsmooth_f64.restype = None
+        L.orc_gsmooth_f64.argtypes = [_dp, _i, _i, _i, _d, ctypes.c_float]
+        L.orc_gsmooth_kernel1d.restype = None
+        L.orc_gsmooth_kernel1d.argtypes = [_i, _d, ctypes.c_float, _dp]
+        L.orc_gsmooth_expfac.restype = _d
+        L.orc_gsmooth_expfac.argtypes = [_d, ctypes.c_float]
+        for fn in (L.orc_knn_brute, L.orc_knn_kdtree):
+            fn.restype = _i
+            fn.argtypes = [_dp, _dp, _dp, _i64, _dp, _dp, _dp, _i64, _i, _lp, _dp]
+        L.orc_W4.restype = ctypes.c_float
+        L.orc_W4.argtypes = [ctypes.c_float, ctypes.c_float]
+        L.orc_sph_density.restype = None
+        L.orc_sph_density.argtypes = [_dp, _lp, _i64, _i, _dp, _fp]
+        L.orc_num_threads.restype = _i
+        _LIB = L
+    return _LIB
+
+
+def ref_lib():
+    """The reference-source build, or None when it was not built here."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(_HERE, "_ref", "libpgalf_ref.so")
+        if not os.path.exists(path):
+            return None
+        R = ctypes.CDLL(path)
+        R.ref_tsc.restype = None
+        R.ref_tsc.argtypes = [_dp, _dp, _dp, _dp, _ip, _i64, _fp, _i, _i, _i,
+                              _d, _d, _d, _d, _i, _i]
+        R.ref_findden.restype = None
+        R.ref_findden.argtypes = [_dp, _dp, _dp, _i64, _fp, _d, _d, _d, _d, _fp,
+                                  _i, _i, _i, _i]
+        R.ref_W4.restype = ctypes.c_float
+        R.ref_W4.argtypes = [ctypes.c_float, ctypes.c_float]
+        R.ref_knn_precopied.restype = _i64
+        R.ref_knn_precopied.argtypes = [_dp, _dp, _dp, _i64, _i, _lp, _dp, _i]
+        R.ref_knn_warm.restype = _i64
+        R.ref_knn_warm.argtypes = [_dp, _dp, _dp, _i64, _i, _d, _d, _d, _d, _i, _i,
+                                   _i, _lp, _i]
+        R.ref_findsphdensity.restype = None
+        R.ref_findsphdensity.argtypes = [_dp, _dp, _dp, _dp, _i64, _i, _lp, _fp, _i]
+        R.ref_sizeof_bp.restype = _i
+        R.ref_init.restype = _i
+        R.ref_init.argtypes = [_i64]
+        _REF = R
+    return _REF
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def padded_shape(nx, ny, nz):
+    """(nz, ny, mx) of the in-place r2c layout, mx = 2(nx/2+1) (tsc_omp2.c:198)."""
+    return (nz, ny, 2 * (nx // 2 + 1))
+
+
+def tsc_deposit(pos, mass, grid, ptype=5, types=None, nthreads=1, fp64=False):
+    """assign_density_TSC restated (tsc_omp2.c:166).  grid = (nx,ny,nz,
+    xmin,ymin,zmin,cellsize).  nthreads emulates the OpenMP team size of the
+    reference's summation order.  Returns (den, n_out_of_grid)."""
+    nx, ny, nz, x0, y0, z0, h = grid
+    pos = _f64(pos)
+    x, y, z = (np.ascontiguousarray(pos[:, a]) for a in range(3))
+    m = _f64(mass)
+    n = len(m)
+    t = np.ascontiguousarray(types if types is not None else np.ones(n), dtype=np.int32)
+    if fp64:
+        den = np.zeros(padded_shape(nx, ny, nz), dtype=np.float64)
+        bad = lib().orc_tsc_deposit_f64(_p(x, _dp), _p(y, _dp), _p(z, _dp), _p(m, _dp),
+                                        _p(t, _ip), n, ptype, _p(den, _dp), nx, ny, nz,
+                                        x0, y0, z0, h)
+    else:
+        den = np.zeros(padded_shape(nx, ny, nz), dtype=np.float32)
+        bad = lib().orc_tsc_deposit_f32(_p(x, _dp), _p(y, _dp), _p(z, _dp), _p(m, _dp),
+                                        _p(t, _ip), n, ptype, _p(den, _fp), nx, ny, nz,
+                                        x0, y0, z0, h, nthreads)
+    return den, int(bad)
+
+
+def findden(pos, grid, den):
+    """findDen restated (tsc_omp2.c:40)."""
+    nx, ny, nz, x0, y0, z0, h = grid
+    pos = _f64(pos)
+    x, y, z = (np.ascontiguousarray(pos[:, a]) for a in range(3))
+    den = np.ascontiguousarray(den, dtype=np.float32)
+    out = np.zeros(len(x), dtype=np.float32)
+    bad = lib().orc_findden(_p(x, _dp), _p(y, _dp), _p(z, _dp), len(x), _p(out, _fp),
+                            x0, y0, z0, h, _p(den, _fp), nx, ny, nz)
+    return out, int(bad)
+
+
+def gsmooth(den, nx, ny, nz, cellsize, RG):
+    """gaussian_Smoothing restated in FP64 (gsmooth.c:14); returns a new
+    padded float32 grid with zeroed pad columns."""
+    g = np.array(den, dtype=np.float32, copy=True, order="C")
+    lib().orc_gsmooth(_p(g, _fp), nx, ny, nz, cellsize, RG)
+    return g
+
+
+def gsmooth_f64(grid3, cellsize, RG):
+    """FP64 smoothing of an unpadded (nz,ny,nx) float64 grid."""
+    g = np.array(grid3, dtype=np.float64, copy=True, order="C")
+    nz, ny, nx = g.shape
+    lib().orc_gsmooth_f64(_p(g, _dp), nx, ny, nz, cellsize, RG)
+    return g
+
+
+def gsmooth_kernel1d(n, cellsize, RG):
+    g = np.zeros(n, dtype=np.float64)
+    lib().orc_gsmooth_kernel1d(n, cellsize, RG, _p(g, _dp))
+    return g
+
+
+def gsmooth_expfac(cellsize, RG):
+    return lib().orc_gsmooth_expfac(cellsize, RG)
+
+
+def knn(qpos, k, cpos=None, method="kdtree"):
+    """Exact kNN (self included when cpos is None), sorted by (dist2, index).
+    Returns (idx int64 [nq,k], dist2 float64 [nq,k])."""
+    q = _f64(qpos)
+    c = q if cpos is None else _f64(cpos)
+    qx, qy, qz = (np.ascontiguousarray(q[:, a]) for a in range(3))
+    cx, cy, cz = (np.ascontiguousarray(c[:, a]) for a in range(3))
+    nq, nc = len(qx), len(cx)
+    idx = np.zeros((nq, k), dtype=np.int64)
+    d2 = np.zeros((nq, k), dtype=np.float64)
+    fn = lib().orc_knn_brute if method == "brute" else lib().orc_knn_kdtree
+    rc = fn(_p(qx, _dp), _p(qy, _dp), _p(qz, _dp), nq, _p(cx, _dp), _p(cy, _dp),
+            _p(cz, _dp), nc, k, _p(idx, _lp), _p(d2, _dp))
+    if rc != 0:
+        raise ValueError("kNN needs at least k candidates")
+    return idx, d2
+
+
+def W4(dist, h):
+    return float(lib().orc_W4(dist, h))
+
+
+def sph_density(idx, d2, cmass=None):
+    """kNN W4 density from sorted lists (nnost.c:607-623 / 689-704)."""
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    d2 = np.ascontiguousarray(d2, dtype=np.float64)
+    nq, k = idx.shape
+    out = np.zeros(nq, dtype=np.float32)
+    cm = None if cmass is None else _f64(cmass)
+    lib().orc_sph_density(_p(d2, _dp), _p(idx, _lp), nq, k,
+                          None if cm is None else _p(cm, _dp), _p(out, _fp))
+    return out
+
+
+def findsphdensity(pos, k):
+    """findsphdensity semantics with exact lists: unweighted W4 density."""
+    idx, d2 = knn(pos, k)
+    return idx, sph_density(idx, d2)
+
+
+def starfindsphdensity(pos, mass, types, k, kden=10):
+    """starfindsphdensity semantics (nnost.c:632-744) with exact lists:
+    mass-weighted W4 density over the kden nearest stars, then all-particle
+    index lists of length k."""
+    pos = _f64(pos)
+    stars = np.nonzero(np.asarray(types) == 1)[0]
+    sidx, sd2 = knn(pos, min(kden, len(stars)), cpos=pos[stars])
+    dens = sph_density(sidx, sd2, cmass=_f64(mass)[stars])
+    idx, _ = knn(pos, k)
+    return idx, dens
+
+
+def num_threads():
+    return int(lib().orc_num_threads())

@@ -1,0 +1,695 @@
+/*
+ * pgalf_oracle.c -- CPU restatement of NewGalFinder's per-halo density stage.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity checker for the HIP
+ * library in galaxyfinder_amd/.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg load it; the product path never does.
+ *
+ * Every function restates the reference algorithm with the reference's own
+ * arithmetic (operation order, float/double conversions, round-half-even
+ * node choice), citing the reference file:line it follows.  No reference
+ * source text is copied; the formulas are re-derived and re-written.
+ *
+ * Pins (see tests/test_oracle_*.py):
+ *   orc_tsc_deposit_f32   -- bit-exact vs the reference build (oracle/_ref)
+ *                            on committed golden grids, any thread count;
+ *                            partition of unity; single-particle weights.
+ *   orc_tsc_deposit_f64   -- same contributions, FP64 sums (error reference).
+ *   orc_findden           -- bit-exact vs the reference build; constant and
+ *                            linear fields reproduced; adjoint identity.
+ *   orc_gsmooth           -- reference gsmooth.c is unbuildable (no FFTW in
+ *                            the image): pinned by plane-wave eigenvalues,
+ *                            DC preservation and the delta response written
+ *                            as a closed-form finite cosine sum.
+ *   orc_knn_*             -- brute force vs k-d tree; vs the reference
+ *                            Find_Near_PreCopied lists in golden fixtures.
+ *   orc_W4, orc_sph_*     -- vs the reference W4 (golden table) and closed
+ *                            forms W(0,h)=1/(pi h^3), W(2h)=0.
+ *
+ * Build: oracle/Makefile (gcc -O2 -fopenmp, no FMA contraction).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define TYPE_ALL 5
+
+/* Reference volume factor: (0.001/TSC_CELL_SIZE)^3 with TSC_CELL_SIZE=0.004,
+ * a compile-time constant independent of the cellsize argument
+ * (tsc_omp2.c:188, params.h:92). */
+static double orc_volfact(void) {
+    double r = 0.001 / 0.004;
+    return r * r * r;
+}
+
+/* ------------------------------------------------------------------------ */
+/* TSC weights, deposit flavour (tsc_omp2.c:303-353).                       */
+/* The x weights carry the particle mass; y/z weights do not.               */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    long long near[3];
+    long long sign[3];
+    float w[3][3]; /* w[axis][0]=centre, [1]=near+sign, [2]=near-sign */
+} orc_tscw;
+
+static void orc_tsc_weights_deposit(double x, double y, double z, double mass,
+                                    double xmin, double ymin, double zmin,
+                                    double cellsize, orc_tscw *o) {
+    double p[3];
+    p[0] = (x - xmin) / cellsize;
+    p[1] = (y - ymin) / cellsize;
+    p[2] = (z - zmin) / cellsize;
+    double pm = mass, ph = mass * 0.5;
+    for (int a = 0; a < 3; a++) {
+        long long nr = (long long)rint(p[a]);        /* round half even */
+        float off = (float)(p[a] - (double)nr);       /* FP64 -> FP32 */
+        o->near[a] = nr;
+        o->sign[a] = (long long)copysignf(1.0f, off); /* +1 for +0.0 */
+        float d = (float)fabs((double)off);
+        float d2 = d * d;                              /* float product */
+        if (a == 0) {
+            float w1 = (float)((0.75 - (double)d2) * pm);
+            float w3 = (float)(ph * (0.25 + (double)d * ((double)d - 1.0)));
+            float w2 = (float)((double)w3 + pm * (double)d);
+            o->w[a][0] = w1; o->w[a][1] = w2; o->w[a][2] = w3;
+        } else {
+            float w1 = (float)(0.75 - (double)d2);
+            float w3 = (float)(0.5 * (0.25 + (double)d * ((double)d - 1.0)));
+            float w2 = w3 + d;                         /* float sum */
+            o->w[a][0] = w1; o->w[a][1] = w2; o->w[a][2] = w3;
+        }
+    }
+}
+
+/* Slab classification of tsc_omp2.c:210-270 for a given OpenMP team size.
+ * Returns the pass (-t: lower half-slab of thread t, +t: upper half-slab). */
+static int orc_tsc_mthreads(int nx, int ny, int nz, int nthreads) {
+    int nxyz = nx > ny ? nx : ny;
+    nxyz = nxyz > nz ? nxyz : nz;
+    double xwidth = (double)nxyz / (double)nthreads;
+    double hwidth = 0.5 * xwidth;
+    int mthreads;
+    if (hwidth < 5) mthreads = (int)((double)nxyz / (2 * 5.));
+    else mthreads = nthreads;
+    if (mthreads < 1) mthreads = 1;
+    return mthreads;
+}
+
+/*
+ * orc_tsc_deposit_f32: assign_density_TSC (tsc_omp2.c:166-506) in FP32 with
+ * the reference's per-node summation order for a team of `nthreads` OpenMP
+ * threads.  Per node, the reference adds all pass-A (negative target)
+ * contributions in particle-index order, then all pass-B ones
+ * (tsc_omp2.c:278-493); the half-slabs of one pass never touch a common
+ * node, so the serial emulation below reproduces that order exactly.
+ * With nthreads<=0 the build without OpenMP is emulated (every particle in
+ * pass B, tsc_omp2.c:235-239).
+ * Returns the number of particles whose 27-node stencil left the grid
+ * (skipped here; the reference has no bounds check).
+ */
+long long orc_tsc_deposit_f32(const double *x, const double *y, const double *z,
+                              const double *mass, const int *type, long long np,
+                              int ptype, float *den, int nx, int ny, int nz,
+                              double xmin, double ymin, double zmin,
+                              double cellsize, int nthreads) {
+    long long mx = 2 * (nx / 2 + 1);
+    long long ngrid = mx * (long long)ny * nz;
+    for (long long i = 0; i < ngrid; i++) den[i] = 0.0f;
+    signed char *target = (signed char *)malloc((size_t)(np > 0 ? np : 1));
+    int nxyz = nx > ny ? nx : ny;
+    nxyz = nxyz > nz ? nxyz : nz;
+    int mthreads = 1;
+    for (long long i = 0; i < np; i++) target[i] = 1;
+    if (nthreads > 0) {
+        mthreads = orc_tsc_mthreads(nx, ny, nz, nthreads);
+        double Xwidth = (double)nxyz / (double)mthreads;
+        for (long long i = 0; i < np; i++) {
+            if (ptype != TYPE_ALL && type[i] != ptype) continue;
+            double xp;
+            if (nxyz == nx) xp = (x[i] - xmin) / cellsize;
+            else if (nxyz == ny) xp = (y[i] - ymin) / cellsize;
+            else xp = (z[i] - zmin) / cellsize;
+            int j = (int)(xp / Xwidth) + 1;
+            int k = (int)((long double)(xp / Xwidth) + 0.5L) + 1;
+            if (j >= mthreads + 1) target[i] = (signed char)mthreads;
+            else if (k == j) target[i] = (signed char)(-j);
+            else target[i] = (signed char)j;
+        }
+    }
+    long long bad = 0;
+    for (int pass = 0; pass < 2; pass++) {
+        for (long long i = 0; i < np; i++) {
+            if (ptype != TYPE_ALL && type[i] != ptype) continue;
+            int t = target[i];
+            if (pass == 0 && t >= 0) continue;
+            if (pass == 1 && t <= 0) continue;
+            orc_tscw w;
+            orc_tsc_weights_deposit(x[i], y[i], z[i], mass[i], xmin, ymin, zmin,
+                                    cellsize, &w);
+            long long id[3][3];
+            int ok = 1;
+            for (int a = 0; a < 3; a++) {
+                id[a][0] = w.near[a];
+                id[a][1] = w.near[a] + w.sign[a];
+                id[a][2] = w.near[a] - w.sign[a];
+                long long n = a == 0 ? nx : (a == 1 ? ny : nz);
+                for (int q = 0; q < 3; q++)
+                    if (id[a][q] < 0 || id[a][q] >= n) ok = 0;
+            }
+            if (!ok) { bad++; continue; }
+            /* node order as tsc_omp2.c:354-382: k outer, j middle, i inner */
+            for (int c = 0; c < 3; c++)
+                for (int b = 0; b < 3; b++) {
+                    float wxy[3];
+                    for (int a = 0; a < 3; a++) wxy[a] = w.w[0][a] * w.w[1][b];
+                    for (int a = 0; a < 3; a++) {
+                        long long off = id[0][a] + mx * (id[1][b] + (long long)ny * id[2][c]);
+                        den[off] += wxy[a] * w.w[2][c];
+                    }
+                }
+        }
+    }
+    double vf = orc_volfact();
+    for (long long i = 0; i < ngrid; i++) den[i] = (float)((double)den[i] * vf);
+    free(target);
+    return bad;
+}
+
+/* FP64-accumulated deposit of the same FP32 contributions; output in double
+ * (the exact-sum reference used to state FP32 errors). */
+long long orc_tsc_deposit_f64(const double *x, const double *y, const double *z,
+                              const double *mass, const int *type, long long np,
+                              int ptype, double *den, int nx, int ny, int nz,
+                              double xmin, double ymin, double zmin,
+                              double cellsize) {
+    long long mx = 2 * (nx / 2 + 1);
+    long long ngrid = mx * (long long)ny * nz;
+    for (long long i = 0; i < ngrid; i++) den[i] = 0.0;
+    long long bad = 0;
+    for (long long i = 0; i < np; i++) {
+        if (ptype != TYPE_ALL && type[i] != ptype) continue;
+        orc_tscw w;
+        orc_tsc_weights_deposit(x[i], y[i], z[i], mass[i], xmin, ymin, zmin,
+                                cellsize, &w);
+        long long id[3][3];
+        int ok = 1;
+        for (int a = 0; a < 3; a++) {
+            id[a][0] = w.near[a];
+            id[a][1] = w.near[a] + w.sign[a];
+            id[a][2] = w.near[a] - w.sign[a];
+            long long n = a == 0 ? nx : (a == 1 ? ny : nz);
+            for (int q = 0; q < 3; q++)
+                if (id[a][q] < 0 || id[a][q] >= n) ok = 0;
+        }
+        if (!ok) { bad++; continue; }
+        for (int c = 0; c < 3; c++)
+            for (int b = 0; b < 3; b++)
+                for (int a = 0; a < 3; a++) {
+                    float wxy = w.w[0][a] * w.w[1][b];
+                    float cval = wxy * w.w[2][c];
+                    long long off = id[0][a] + mx * (id[1][b] + (long long)ny * id[2][c]);
+                    den[off] += (double)cval;
+                }
+    }
+    double vf = orc_volfact();
+    for (long long i = 0; i < ngrid; i++) den[i] *= vf;
+    return bad;
+}
+
+/* ------------------------------------------------------------------------ */
+/* TSC read-back: findDen (tsc_omp2.c:40-163).                              */
+/* ------------------------------------------------------------------------ */
+static inline void orc_readback_axis(double p, int *nr, long *sg, float *w1,
+                                     float *w2, float *w3) {
+    int n = (int)rint(p);                       /* int near (tsc_omp2.c:54) */
+    float off = (float)(p - (double)n);
+    *nr = n;
+    *sg = (long)copysignf(1.0f, off);
+    float d = (float)fabs((double)off);
+    float d2 = d * d;
+    *w1 = (float)(0.75 - (double)d2);
+    *w3 = (float)(0.5 * (0.25 + (double)d * ((double)d - 1.0)));
+    *w2 = *w3 + d;
+}
+
+long long orc_findden(const double *x, const double *y, const double *z,
+                      long long np, float *densph, double xmin, double ymin,
+                      double zmin, double cellsize, const float *den, int nx,
+                      int ny, int nz) {
+    long mx = 2 * (nx / 2 + 1);
+    long long bad = 0;
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+    for (long long i = 0; i < np; i++) {
+        int nr[3]; long sg[3]; float w[3][3];
+        double p[3] = {(x[i] - xmin) / cellsize, (y[i] - ymin) / cellsize,
+                       (z[i] - zmin) / cellsize};
+        for (int a = 0; a < 3; a++)
+            orc_readback_axis(p[a], &nr[a], &sg[a], &w[a][0], &w[a][1], &w[a][2]);
+        long id[3][3];
+        int ok = 1;
+        for (int a = 0; a < 3; a++) {
+            id[a][0] = nr[a]; id[a][1] = nr[a] + sg[a]; id[a][2] = nr[a] - sg[a];
+            long n = a == 0 ? nx : (a == 1 ? ny : nz);
+            for (int q = 0; q < 3; q++)
+                if (id[a][q] < 0 || id[a][q] >= n) ok = 0;
+        }
+        if (!ok) { bad++; densph[i] = 0.0f; continue; }
+        float wxy[3][3];
+        for (int b = 0; b < 3; b++)
+            for (int a = 0; a < 3; a++) wxy[b][a] = w[0][a] * w[1][b];
+        float dsum = 0.0f;
+        for (int c = 0; c < 3; c++)
+            for (int b = 0; b < 3; b++)
+                for (int a = 0; a < 3; a++) {
+                    float g = den[id[0][a] + mx * (id[1][b] + (long)ny * id[2][c])];
+                    dsum += (g * wxy[b][a]) * w[2][c];
+                }
+        densph[i] = dsum;
+    }
+    return bad;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Gaussian smoothing: gaussian_Smoothing (gsmooth.c:14-115), FP64.         */
+/* r2c -> multiply by exp(-(pi RG_c)^2 sum (k'/n)^2) -> c2r -> /(nx ny nz). */
+/* Full complex transforms of the real grid are used; the Hermitian half   */
+/* the reference stores maps onto bins k and n-k, whose factors coincide.  */
+/* ------------------------------------------------------------------------ */
+typedef struct { double re, im; } orc_cplx;
+
+static int orc_is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+/* in-place FFT of one strided line; sign=-1 forward. */
+static void orc_fft_line(orc_cplx *buf, orc_cplx *tmp, int n, int sign,
+                         const orc_cplx *tw /* exp(-2 pi i m/n), m<n */) {
+    if (orc_is_pow2(n)) {
+        /* iterative radix-2, bit-reversed input */
+        for (int i = 1, j = 0; i < n; i++) {
+            int bit = n >> 1;
+            for (; j & bit; bit >>= 1) j ^= bit;
+            j ^= bit;
+            if (i < j) { orc_cplx t = buf[i]; buf[i] = buf[j]; buf[j] = t; }
+        }
+        for (int len = 2; len <= n; len <<= 1) {
+            int step = n / len;
+            for (int i = 0; i < n; i += len)
+                for (int k = 0; k < len / 2; k++) {
+                    orc_cplx w = tw[k * step];
+                    if (sign > 0) w.im = -w.im;
+                    orc_cplx u = buf[i + k], v = buf[i + k + len / 2];
+                    orc_cplx vw = {v.re * w.re - v.im * w.im, v.re * w.im + v.im * w.re};
+                    buf[i + k].re = u.re + vw.re; buf[i + k].im = u.im + vw.im;
+                    buf[i + k + len / 2].re = u.re - vw.re;
+                    buf[i + k + len / 2].im = u.im - vw.im;
+                }
+        }
+    } else {
+        /* direct DFT, O(n^2), exact twiddle table */
+        for (int k = 0; k < n; k++) {
+            double sr = 0, si = 0;
+            for (int j = 0; j < n; j++) {
+                long m = ((long)k * j) % n;
+                double wr = tw[m].re, wi = sign > 0 ? -tw[m].im : tw[m].im;
+                sr += buf[j].re * wr - buf[j].im * wi;
+                si += buf[j].re * wi + buf[j].im * wr;
+            }
+            tmp[k].re = sr; tmp[k].im = si;
+        }
+        memcpy(buf, tmp, sizeof(orc_cplx) * (size_t)n);
+    }
+}
+
+static orc_cplx *orc_twiddles(int n) {
+    orc_cplx *tw = (orc_cplx *)malloc(sizeof(orc_cplx) * (size_t)n);
+    for (int m = 0; m < n; m++) {
+        long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)m / (long double)n;
+        tw[m].re = (double)cosl(a);
+        tw[m].im = (double)sinl(a);
+    }
+    return tw;
+}
+
+static void orc_fft3d(orc_cplx *a, int nx, int ny, int nz, int sign) {
+    int nmax = nx > ny ? nx : ny;
+    nmax = nmax > nz ? nmax : nz;
+    orc_cplx *twx = orc_twiddles(nx), *twy = orc_twiddles(ny), *twz = orc_twiddles(nz);
+#pragma omp parallel
+    {
+        orc_cplx *line = (orc_cplx *)malloc(sizeof(orc_cplx) * (size_t)nmax);
+        orc_cplx *tmp = (orc_cplx *)malloc(sizeof(orc_cplx) * (size_t)nmax);
+#pragma omp for schedule(static)
+        for (long r = 0; r < (long)ny * nz; r++) {
+            orc_cplx *row = a + r * nx;
+            memcpy(line, row, sizeof(orc_cplx) * (size_t)nx);
+            orc_fft_line(line, tmp, nx, sign, twx);
+            memcpy(row, line, sizeof(orc_cplx) * (size_t)nx);
+        }
+#pragma omp for schedule(static)
+        for (long r = 0; r < (long)nx * nz; r++) {
+            long i = r % nx, k = r / nx;
+            for (int j = 0; j < ny; j++) line[j] = a[i + (long)nx * (j + (long)ny * k)];
+            orc_fft_line(line, tmp, ny, sign, twy);
+            for (int j = 0; j < ny; j++) a[i + (long)nx * (j + (long)ny * k)] = line[j];
+        }
+#pragma omp for schedule(static)
+        for (long r = 0; r < (long)nx * ny; r++) {
+            for (int k = 0; k < nz; k++) line[k] = a[r + (long)nx * ny * k];
+            orc_fft_line(line, tmp, nz, sign, twz);
+            for (int k = 0; k < nz; k++) a[r + (long)nx * ny * k] = line[k];
+        }
+        free(line); free(tmp);
+    }
+    free(twx); free(twy); free(twz);
+}
+
+/* Per-axis factor exactly as gsmooth.c:67-92 evaluates it (double). */
+static double orc_gfactor(long kk, int n, double expfac) {
+    long h = n / 2;
+    long k1 = kk;
+    if (kk > h) k1 = kk - n; /* negative frequencies (y,z wrap; x Hermitian) */
+    return exp((double)(k1 * k1) * expfac / (double)(n * n));
+}
+
+/* Expfac with the reference's float RG rescale (gsmooth.c:67-69). */
+double orc_gsmooth_expfac(double cellsize, float RG) {
+    float rgc = (float)((double)RG / cellsize);
+    return -(M_PI * (double)rgc) * (M_PI * (double)rgc);
+}
+
+/* FP64 smoothing of the padded float grid (in place; pad columns set to 0). */
+void orc_gsmooth(float *den, int nx, int ny, int nz, double cellsize, float RG) {
+    long mx = 2 * (nx / 2 + 1);
+    size_t nc = (size_t)nx * ny * nz;
+    orc_cplx *a = (orc_cplx *)malloc(sizeof(orc_cplx) * nc);
+    for (long k = 0; k < nz; k++)
+        for (long j = 0; j < ny; j++)
+            for (long i = 0; i < nx; i++) {
+                a[i + nx * (j + ny * k)].re = den[i + mx * (j + ny * k)];
+                a[i + nx * (j + ny * k)].im = 0.0;
+            }
+    orc_fft3d(a, nx, ny, nz, -1);
+    double expfac = orc_gsmooth_expfac(cellsize, RG);
+    double *fx = (double *)malloc(sizeof(double) * nx);
+    double *fy = (double *)malloc(sizeof(double) * ny);
+    double *fz = (double *)malloc(sizeof(double) * nz);
+    for (int i = 0; i < nx; i++) fx[i] = orc_gfactor(i, nx, expfac);
+    for (int j = 0; j < ny; j++) fy[j] = orc_gfactor(j, ny, expfac);
+    for (int k = 0; k < nz; k++) fz[k] = orc_gfactor(k, nz, expfac);
+    for (long k = 0; k < nz; k++)
+        for (long j = 0; j < ny; j++)
+            for (long i = 0; i < nx; i++) {
+                double f = fx[i] * fy[j] * fz[k];
+                a[i + nx * (j + ny * k)].re *= f;
+                a[i + nx * (j + ny * k)].im *= f;
+            }
+    orc_fft3d(a, nx, ny, nz, +1);
+    /* scale 1/(nx ny nz) formed in long double as gsmooth.c:26 */
+    double scale = (double)(1.0L / (long double)((double)nx * (double)ny * (double)nz));
+    for (long k = 0; k < nz; k++)
+        for (long j = 0; j < ny; j++) {
+            for (long i = 0; i < nx; i++)
+                den[i + mx * (j + ny * k)] = (float)(a[i + nx * (j + ny * k)].re * scale);
+            for (long i = nx; i < mx; i++) den[i + mx * (j + ny * k)] = 0.0f;
+        }
+    free(a); free(fx); free(fy); free(fz);
+}
+
+/* FP64 smoothing of a double grid (unpadded, nx*ny*nz, in place). */
+void orc_gsmooth_f64(double *g, int nx, int ny, int nz, double cellsize, float RG) {
+    size_t nc = (size_t)nx * ny * nz;
+    orc_cplx *a = (orc_cplx *)malloc(sizeof(orc_cplx) * nc);
+    for (size_t i = 0; i < nc; i++) { a[i].re = g[i]; a[i].im = 0.0; }
+    orc_fft3d(a, nx, ny, nz, -1);
+    double expfac = orc_gsmooth_expfac(cellsize, RG);
+    for (long k = 0; k < nz; k++)
+        for (long j = 0; j < ny; j++)
+            for (long i = 0; i < nx; i++) {
+                double f = orc_gfactor(i, nx, expfac) * orc_gfactor(j, ny, expfac) *
+                           orc_gfactor(k, nz, expfac);
+                a[i + nx * (j + ny * k)].re *= f;
+                a[i + nx * (j + ny * k)].im *= f;
+            }
+    orc_fft3d(a, nx, ny, nz, +1);
+    double scale = (double)(1.0L / (long double)((double)nx * (double)ny * (double)nz));
+    for (size_t i = 0; i < nc; i++) g[i] = a[i].re * scale;
+    free(a);
+}
+
+/* Closed-form 1-D periodic kernel: g[m] = (1/n) sum_k f(k) cos(2 pi k m / n)
+ * (the delta response of one axis, written without any FFT). */
+void orc_gsmooth_kernel1d(int n, double cellsize, float RG, double *g) {
+    double expfac = orc_gsmooth_expfac(cellsize, RG);
+    for (int m = 0; m < n; m++) {
+        long double s = 0.0L;
+        for (int k = 0; k < n; k++) {
+            long double ang = 2.0L * 3.141592653589793238462643383279502884L *
+                              (long double)(((long)k * m) % n) / (long double)n;
+            s += (long double)orc_gfactor(k, n, expfac) * cosl(ang);
+        }
+        g[m] = (double)(s / (long double)n);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Exact k nearest neighbours (self included), FP64 distances formed as     */
+/* nnost.c:362-365 (query minus candidate, dx*dx+dy*dy+dz*dz, no FMA),      */
+/* sorted ascending by (dist2, particle index).                             */
+/* ------------------------------------------------------------------------ */
+static inline double orc_d2(double qx, double qy, double qz, double cx,
+                            double cy, double cz) {
+    double tx = qx - cx, ty = qy - cy, tz = qz - cz;
+    double a = tx * tx;
+    double b = ty * ty;
+    double c = tz * tz;
+    return (a + b) + c;
+}
+
+typedef struct { double d2; long long idx; } orc_nb;
+
+static inline int orc_nb_lt(double d2a, long long ia, double d2b, long long ib) {
+    return d2a < d2b || (d2a == d2b && ia < ib);
+}
+
+/* insert into sorted list of size cnt (capacity k). */
+static inline void orc_nb_insert(orc_nb *list, int *cnt, int k, double d2,
+                                 long long idx) {
+    int n = *cnt;
+    if (n == k && !orc_nb_lt(d2, idx, list[k - 1].d2, list[k - 1].idx)) return;
+    int pos = n < k ? n : k - 1;
+    while (pos > 0 && orc_nb_lt(d2, idx, list[pos - 1].d2, list[pos - 1].idx)) {
+        list[pos] = list[pos - 1];
+        pos--;
+    }
+    list[pos].d2 = d2;
+    list[pos].idx = idx;
+    if (n < k) *cnt = n + 1;
+}
+
+/* Brute force: queries q[0..nq) vs candidates c[0..nc).  Returns 0, or -1 if
+ * nc < k. */
+int orc_knn_brute(const double *qx, const double *qy, const double *qz,
+                  long long nq, const double *cx, const double *cy,
+                  const double *cz, long long nc, int k, long long *idx,
+                  double *d2out) {
+    if (k <= 0 || nc < k) return -1;
+#pragma omp parallel
+    {
+        orc_nb *list = (orc_nb *)malloc(sizeof(orc_nb) * (size_t)k);
+#pragma omp for schedule(dynamic, 64)
+        for (long long i = 0; i < nq; i++) {
+            int cnt = 0;
+            for (long long j = 0; j < nc; j++) {
+                double d = orc_d2(qx[i], qy[i], qz[i], cx[j], cy[j], cz[j]);
+                orc_nb_insert(list, &cnt, k, d, j);
+            }
+            for (int j = 0; j < k; j++) {
+                idx[i * k + j] = list[j].idx;
+                if (d2out) d2out[i * k + j] = list[j].d2;
+            }
+        }
+        free(list);
+    }
+    return 0;
+}
+
+/* k-d tree (median split on the widest axis, leaf <= 16), exact search.
+ * Node boxes are the min/max of member coordinates, so the box distance
+ * (formed with the same monotone FP64 operations) never exceeds the
+ * computed distance of a member. */
+typedef struct {
+    double lo[3], hi[3];
+    long long begin, end; /* range in perm */
+    int left, right;      /* -1 for leaves */
+} orc_kdnode;
+
+typedef struct {
+    const double *c[3];
+    long long *perm;
+    orc_kdnode *nodes;
+    int nnodes, cap;
+} orc_kdtree;
+
+static int orc_kd_cmp_axis;
+static const double *orc_kd_cmp_arr;
+static int orc_kd_cmp(const void *a, const void *b) {
+    long long ia = *(const long long *)a, ib = *(const long long *)b;
+    double va = orc_kd_cmp_arr[ia], vb = orc_kd_cmp_arr[ib];
+    if (va < vb) return -1;
+    if (va > vb) return 1;
+    return ia < ib ? -1 : (ia > ib);
+}
+
+static int orc_kd_build(orc_kdtree *t, long long b, long long e) {
+    if (t->nnodes == t->cap) {
+        t->cap = t->cap * 2 + 16;
+        t->nodes = (orc_kdnode *)realloc(t->nodes, sizeof(orc_kdnode) * (size_t)t->cap);
+    }
+    int id = t->nnodes++;
+    orc_kdnode nd;
+    for (int a = 0; a < 3; a++) { nd.lo[a] = INFINITY; nd.hi[a] = -INFINITY; }
+    for (long long i = b; i < e; i++) {
+        long long p = t->perm[i];
+        for (int a = 0; a < 3; a++) {
+            double v = t->c[a][p];
+            if (v < nd.lo[a]) nd.lo[a] = v;
+            if (v > nd.hi[a]) nd.hi[a] = v;
+        }
+    }
+    nd.begin = b; nd.end = e; nd.left = nd.right = -1;
+    if (e - b > 16) {
+        int ax = 0;
+        double wmax = -1;
+        for (int a = 0; a < 3; a++)
+            if (nd.hi[a] - nd.lo[a] > wmax) { wmax = nd.hi[a] - nd.lo[a]; ax = a; }
+        orc_kd_cmp_axis = ax;
+        orc_kd_cmp_arr = t->c[ax];
+        qsort(t->perm + b, (size_t)(e - b), sizeof(long long), orc_kd_cmp);
+        long long m = b + (e - b) / 2;
+        t->nodes[id] = nd;
+        int l = orc_kd_build(t, b, m);
+        int r = orc_kd_build(t, m, e);
+        t->nodes[id].left = l;
+        t->nodes[id].right = r;
+        return id;
+    }
+    t->nodes[id] = nd;
+    return id;
+}
+
+static inline double orc_box_d2(const orc_kdnode *nd, double qx, double qy, double qz) {
+    double q[3] = {qx, qy, qz};
+    double g[3];
+    for (int a = 0; a < 3; a++) {
+        if (q[a] < nd->lo[a]) g[a] = nd->lo[a] - q[a];
+        else if (q[a] > nd->hi[a]) g[a] = q[a] - nd->hi[a];
+        else g[a] = 0.0;
+    }
+    double a0 = g[0] * g[0], a1 = g[1] * g[1], a2 = g[2] * g[2];
+    return (a0 + a1) + a2;
+}
+
+static void orc_kd_search(const orc_kdtree *t, int node, double qx, double qy,
+                          double qz, orc_nb *list, int *cnt, int k) {
+    const orc_kdnode *nd = &t->nodes[node];
+    if (*cnt == k) {
+        double bd = orc_box_d2(nd, qx, qy, qz);
+        if (bd > list[k - 1].d2) return;
+    }
+    if (nd->left < 0) {
+        for (long long i = nd->begin; i < nd->end; i++) {
+            long long p = t->perm[i];
+            double d = orc_d2(qx, qy, qz, t->c[0][p], t->c[1][p], t->c[2][p]);
+            orc_nb_insert(list, cnt, k, d, p);
+        }
+        return;
+    }
+    double dl = orc_box_d2(&t->nodes[nd->left], qx, qy, qz);
+    double dr = orc_box_d2(&t->nodes[nd->right], qx, qy, qz);
+    if (dl <= dr) {
+        orc_kd_search(t, nd->left, qx, qy, qz, list, cnt, k);
+        orc_kd_search(t, nd->right, qx, qy, qz, list, cnt, k);
+    } else {
+        orc_kd_search(t, nd->right, qx, qy, qz, list, cnt, k);
+        orc_kd_search(t, nd->left, qx, qy, qz, list, cnt, k);
+    }
+}
+
+int orc_knn_kdtree(const double *qx, const double *qy, const double *qz,
+                   long long nq, const double *cx, const double *cy,
+                   const double *cz, long long nc, int k, long long *idx,
+                   double *d2out) {
+    if (k <= 0 || nc < k) return -1;
+    orc_kdtree t;
+    t.c[0] = cx; t.c[1] = cy; t.c[2] = cz;
+    t.perm = (long long *)malloc(sizeof(long long) * (size_t)nc);
+    for (long long i = 0; i < nc; i++) t.perm[i] = i;
+    t.nodes = NULL; t.nnodes = 0; t.cap = 0;
+    orc_kd_build(&t, 0, nc);
+#pragma omp parallel
+    {
+        orc_nb *list = (orc_nb *)malloc(sizeof(orc_nb) * (size_t)k);
+#pragma omp for schedule(dynamic, 256)
+        for (long long i = 0; i < nq; i++) {
+            int cnt = 0;
+            orc_kd_search(&t, 0, qx[i], qy[i], qz[i], list, &cnt, k);
+            for (int j = 0; j < k; j++) {
+                idx[i * k + j] = list[j].idx;
+                if (d2out) d2out[i * k + j] = list[j].d2;
+            }
+        }
+        free(list);
+    }
+    free(t.perm);
+    free(t.nodes);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* M4 cubic spline W4 (nnost.c:534-555) and kNN densities.                  */
+/* ------------------------------------------------------------------------ */
+float orc_W4(float dist, float h) {
+    float r = dist / h;
+    if (r < 1.) {
+        double t = (2. - (double)r) * (double)r * (double)r - 1.333333;
+        return (float)((-0.2387324) * t / (double)(h * h * h));
+    } else if (r < 2.) {
+        float twomr = (float)((2. - (double)r) / (double)h);
+        return (float)((0.07957747) * (double)twomr * (double)twomr * (double)twomr);
+    }
+    return 0.0f;
+}
+
+/* Density from sorted exact lists: rho_i = sum_j W4(r_ij, h_i) [* m_j], with
+ * h_i = sqrtf(d2 of the last neighbour)/2 and r_ij = sqrtf(d2_ij), summed in
+ * float in list order (nnost.c:607-623 unweighted, :689-704 weighted; the
+ * neighbour mass is the FP32 copy kept in the Neighbor record, nnost.c:287). */
+void orc_sph_density(const double *d2, const long long *idx, long long nq, int k,
+                     const double *cmass /* nullable: unweighted */, float *dens) {
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < nq; i++) {
+        const double *dd = d2 + i * k;
+        float neardist = sqrtf((float)dd[k - 1]);
+        float h = (float)((double)neardist / 2.);
+        float s = 0.0f;
+        for (int j = 0; j < k; j++) {
+            float r = sqrtf((float)dd[j]);
+            float w = orc_W4(r, h);
+            if (cmass) w = w * (float)cmass[idx[i * k + j]];
+            s += w;
+        }
+        dens[i] = s;
+    }
+}
+
+/* OpenMP team size actually used (for reporting CPU baselines). */
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

@@ -1,0 +1,212 @@
+/*
+ * ref_driver.c -- thin C driver around the REFERENCE's own compiled sources
+ * (NewGalFinder/tsc_omp2.c, nnost.c, Memory2.c, built in place from
+ * /root/reference by oracle/Makefile into oracle/_ref/libpgalf_ref.so).
+ *
+ * TEST INFRASTRUCTURE ONLY: used to pin the oracle (tests/) and to write the
+ * golden fixtures (tests/golden/make_golden.py).  It packs plain SoA arrays
+ * into the reference's particle records and calls the reference functions;
+ * the loops that mirror lagFindStellarCore's neighbour-list driver are this
+ * file's own code calling the reference's Find_Near_PreCopied.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <omp.h>
+
+#include "ramses.h"
+#include "tree.h"
+#include "params.h"
+#include "Memory.h"
+
+/* Reference entry points (tsc_omp2.c:40,166; nnost.c:199,330,375,536,560). */
+void assign_density_TSC(SimpleBasicParticleType *, int, float *, int, int, int,
+                        double, double, double, double, int);
+void findDen(SimpleBasicParticleType *, int, float *, double, double, double,
+             double, float *, int, int, int);
+void findsphdensity(SimpleBasicParticleType *, int, long long *, int, float *);
+void new_Make_Tree_Near(TStruct *, size_t, TPtlStruct *, size_t, int);
+float W4(float, float);
+
+/* Record layout of nnost.c:273-277 (file-local there, restated here). */
+typedef struct {
+    dptype dist2;
+    float mass;
+    long long indx;
+} ref_neighbor;
+int Find_Near_PreCopied(particle *, int, TStruct *, TPtlStruct *, ref_neighbor *, int);
+int Find_Near(particle *, int, TStruct *, TPtlStruct *, float *, long long *,
+              dptype *, float *);
+
+static int ref_pool_ready = 0;
+
+int ref_init(long long bytes) {
+    if (ref_pool_ready) return 1;
+    ref_pool_ready = (int)Make_Total_Memory((INT8)bytes);
+    return ref_pool_ready;
+}
+
+static SimpleBasicParticleType *ref_pack(const double *x, const double *y,
+                                         const double *z, const double *m,
+                                         const int *type, long long np) {
+    SimpleBasicParticleType *bp =
+        (SimpleBasicParticleType *)calloc((size_t)np, sizeof(SimpleBasicParticleType));
+    for (long long i = 0; i < np; i++) {
+        bp[i].type = type ? (unsigned int)type[i] : 1u;
+        bp[i].mass = m ? m[i] : 1.0;
+        bp[i].x = x[i];
+        bp[i].y = y[i];
+        bp[i].z = z[i];
+        bp[i].indx = (idtype)i;
+        bp[i].bp = NULL;
+    }
+    return bp;
+}
+
+int ref_sizeof_bp(void) { return (int)sizeof(SimpleBasicParticleType); }
+
+void ref_tsc(const double *x, const double *y, const double *z, const double *m,
+             const int *type, long long np, float *den, int nx, int ny, int nz,
+             double xmin, double ymin, double zmin, double cellsize, int ptype,
+             int nthreads) {
+    ref_init(4LL << 30);
+    SimpleBasicParticleType *bp = ref_pack(x, y, z, m, type, np);
+    omp_set_num_threads(nthreads);
+    assign_density_TSC(bp, (int)np, den, nx, ny, nz, xmin, ymin, zmin, cellsize, ptype);
+    free(bp);
+}
+
+void ref_findden(const double *x, const double *y, const double *z, long long np,
+                 float *densph, double xmin, double ymin, double zmin,
+                 double cellsize, float *den, int nx, int ny, int nz, int nthreads) {
+    SimpleBasicParticleType *bp = ref_pack(x, y, z, NULL, NULL, np);
+    omp_set_num_threads(nthreads);
+    findDen(bp, (int)np, densph, xmin, ymin, zmin, cellsize, den, nx, ny, nz);
+    free(bp);
+}
+
+float ref_W4(float dist, float h) { return W4(dist, h); }
+
+/* Tree set-up as lagFindStellarCore does it (nnost.c:1298-1329): unit masses,
+ * serialized build, MAX(np/2, 650000) nodes. */
+static TStruct *ref_tree(const double *x, const double *y, const double *z,
+                         long long np, TPtlStruct **ptl_out) {
+    TPtlStruct *ptl = (TPtlStruct *)calloc((size_t)np, sizeof(TPtlStruct));
+    for (long long i = 0; i < np; i++) {
+        ptl[i].type = TYPE_PTL;
+        ptl[i].x = x[i];
+        ptl[i].y = y[i];
+        ptl[i].z = z[i];
+        ptl[i].sibling = ptl + (i + 1);
+        ptl[i].mass = 1;
+    }
+    ptl[np - 1].sibling = NULL;
+    size_t nnode = (size_t)(np / 2 > 650000 ? np / 2 : 650000);
+    TStruct *tree = (TStruct *)calloc(nnode, sizeof(TStruct));
+    new_Make_Tree_Near(tree, nnode, ptl, (size_t)np, SERIALIZED);
+    *ptl_out = ptl;
+    return tree;
+}
+
+/* Cold-start Find_Near_PreCopied per particle (exact kNN, first-seen ties).
+ * Returns the number of queries that came back short. */
+long long ref_knn_precopied(const double *x, const double *y, const double *z,
+                            long long np, int k, long long *idx, double *d2,
+                            int nthreads) {
+    TPtlStruct *ptl;
+    TStruct *tree = ref_tree(x, y, z, np, &ptl);
+    long long shortq = 0;
+    omp_set_num_threads(nthreads);
+#pragma omp parallel reduction(+ : shortq)
+    {
+        ref_neighbor nb[MAX_NUM_NEAR];
+#pragma omp for schedule(dynamic, 64)
+        for (long long i = 0; i < np; i++) {
+            particle pp;
+            pp.x = x[i]; pp.y = y[i]; pp.z = z[i];
+            int n = Find_Near_PreCopied(&pp, k, tree, ptl, nb, 0);
+            if (n != k) shortq++;
+            for (int j = 0; j < k; j++) {
+                idx[i * k + j] = j < n ? nb[j].indx : -1;
+                if (d2) d2[i * k + j] = j < n ? nb[j].dist2 : -1.0;
+            }
+        }
+    }
+    free(tree);
+    free(ptl);
+    return shortq;
+}
+
+static int ref_cmp_nb(const void *a, const void *b) {
+    const ref_neighbor *aa = (const ref_neighbor *)a, *bb = (const ref_neighbor *)b;
+    if (aa->dist2 < bb->dist2) return -1;
+    if (aa->dist2 > bb->dist2) return 1;
+    return 0;
+}
+
+/* Warm-started lists on the TSC node cells, the np >= 1e5 branch of
+ * lagFindStellarCore (nnost.c:1361-1461, shallow cells only, one thread per
+ * cell): previous list re-measured, qsort-ed, then Find_Near_PreCopied.
+ * Cells with DEEPSIZE or more members are handled with the same warm start
+ * serially (their OpenMP split changes only the schedule).  Used as the
+ * reference-kind CPU timing and to confirm warm == cold on the fixtures. */
+long long ref_knn_warm(const double *x, const double *y, const double *z,
+                       long long np, int k, double xmin, double ymin, double zmin,
+                       double cellsize, int nx, int ny, int nz, long long *idx,
+                       int nthreads) {
+    TPtlStruct *ptl;
+    TStruct *tree = ref_tree(x, y, z, np, &ptl);
+    long long mx = 2 * (nx / 2 + 1);
+    long long ncells = mx * (long long)ny * nz;
+    long long *head = (long long *)malloc(sizeof(long long) * (size_t)ncells);
+    long long *next = (long long *)malloc(sizeof(long long) * (size_t)np);
+    for (long long c = 0; c < ncells; c++) head[c] = -1;
+    for (long long i = 0; i < np; i++) {
+        long long ir = (long long)rint((x[i] - xmin) / cellsize);
+        long long jr = (long long)rint((y[i] - ymin) / cellsize);
+        long long kr = (long long)rint((z[i] - zmin) / cellsize);
+        long long c = ir + mx * (jr + (long long)ny * kr);
+        next[i] = head[c];
+        head[c] = i;
+    }
+    long long shortq = 0;
+    omp_set_num_threads(nthreads);
+#pragma omp parallel reduction(+ : shortq)
+    {
+        ref_neighbor nb[MAX_NUM_NEAR];
+        int npn = 0;
+#pragma omp for schedule(dynamic, 1)
+        for (long long c = 0; c < ncells; c++) {
+            for (long long i = head[c]; i >= 0; i = next[i]) {
+                particle pp;
+                pp.x = x[i]; pp.y = y[i]; pp.z = z[i];
+                for (int j = 0; j < npn; j++) {
+                    long long q = nb[j].indx;
+                    dptype tx = pp.x - x[q], ty = pp.y - y[q], tz = pp.z - z[q];
+                    nb[j].dist2 = tx * tx + ty * ty + tz * tz;
+                }
+                if (npn) qsort(nb, (size_t)npn, sizeof(ref_neighbor), ref_cmp_nb);
+                npn = Find_Near_PreCopied(&pp, k, tree, ptl, nb, npn);
+                if (npn != k) shortq++;
+                for (int j = 0; j < k; j++) idx[i * k + j] = j < npn ? nb[j].indx : -1;
+            }
+        }
+    }
+    free(head);
+    free(next);
+    free(tree);
+    free(ptl);
+    return shortq;
+}
+
+/* The reference's own findsphdensity (defective Find_Near, nnost.c:375). */
+void ref_findsphdensity(const double *x, const double *y, const double *z,
+                        const double *m, long long np, int k, long long *idx,
+                        float *dens, int nthreads) {
+    ref_init(4LL << 30);
+    SimpleBasicParticleType *bp = ref_pack(x, y, z, m, NULL, np);
+    omp_set_num_threads(nthreads);
+    findsphdensity(bp, (int)np, idx, k, dens);
+    free(bp);
+}
