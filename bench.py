@@ -1,0 +1,189 @@
+"""Benchmark of the MI355X density stage of NewGalFinder.
+
+One step = the whole per-halo density stage on device-resident float32
+inputs: TSC deposit -> FFT Gaussian smoothing -> TSC read-back -> exact
+32-NN lists (int64) + W4 kNN density.  Workload (default): one halo of
+4,194,304 particles (NFW c=10, r200=0.4, plus 20 Plummer subhaloes with
+log-uniform mass fractions 0.1-3%, a=0.01), 256^3 grid over the unit box,
+RG = 1.5 cells, k = 32; synthetic seeded data.
+
+Prints ONE JSON line (rank 0, N=1):
+  value          particles/s through the stage (inputs already in HBM)
+  roofline       dominant kernel (kNN query) algorithmic bytes per launch
+                 / its mean launch time (HIP events on the launch stream)
+  cpu_baseline   the repo's C/OpenMP CPU restatement (oracle/, "port") on
+                 the same halo, timed on this host's cores
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public peak
+
+
+def stage_bytes(n, g_cells, k):
+    """Algorithmic HBM bytes per stage (FP32 positions/masses, int64 ids)."""
+    return {
+        "tsc": 16 * n + 4 * g_cells,
+        "smooth": 8 * g_cells,
+        "findden": 16 * n + 4 * g_cells,
+        "knn": 16 * n + 4 * n + 8 * k * n,
+    }
+
+
+def cpu_baseline(pos, mass, spec, k, knn_sample):
+    """Time the CPU restatement of the stage on the same halo.  Deposit,
+    smoothing and read-back run on the whole halo; the kNN tree is built
+    over every particle and queried for a random sample of knn_sample
+    particles, its query time scaled to all particles."""
+    from oracle import oracle as O
+    n = len(pos)
+    g = (spec["nx"], spec["ny"], spec["nz"], spec["xmin"], spec["ymin"], spec["zmin"],
+         spec["cellsize"])
+    t0 = time.perf_counter()
+    den, _ = O.tsc_deposit(pos, mass, g, nthreads=1)
+    t1 = time.perf_counter()
+    sm = O.gsmooth(den, spec["nx"], spec["ny"], spec["nz"], spec["cellsize"], spec["RG"])
+    t2 = time.perf_counter()
+    O.findden(pos, g, sm)
+    t3 = time.perf_counter()
+    sel = np.random.default_rng(0).choice(n, min(knn_sample, n), replace=False)
+    tb0 = time.perf_counter()
+    O.knn(pos[sel[:64]], k, cpos=pos)  # tree build (+64 queries)
+    tb1 = time.perf_counter()
+    idx, d2 = O.knn(pos[sel], k, cpos=pos)
+    O.sph_density(idx, d2)
+    tb2 = time.perf_counter()
+    build = tb1 - tb0
+    query = max(tb2 - tb1 - build, 0.0) * n / len(sel)
+    total = (t1 - t0) + (t2 - t1) + (t3 - t2) + build + query
+    return {
+        "value": n / total,
+        "unit": "particles/s",
+        "cores": O.num_threads(),
+        "kind": "port",
+        "sample": ("CPU restatement (oracle/pgalf_oracle.c, OpenMP) on the same %d-particle "
+                   "halo: deposit (serial, reference order) + FP64 FFT smoothing + read-back "
+                   "on all particles, k-d tree kNN built on all and queried for %d random "
+                   "particles, query time scaled to all; %.1f s of CPU work"
+                   % (n, len(sel), (t3 - t0) + (tb2 - tb0))),
+        "stages_s": {"tsc": t1 - t0, "smooth": t2 - t1, "findden": t3 - t2,
+                     "knn_build": build, "knn_query_scaled": query},
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 4])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--knn-sample", type=int, default=1000000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build runs the density stage on a "
+              "single MI355X (halos are independent; multi-GPU sharding is a host-side "
+              "split of the halo catalogue, not run here).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from galaxyfinder_amd import density as D
+    from galaxyfinder_amd import halos
+
+    pos, mass, spec = halos.single_halo_config(args.config)
+    n, k = len(pos), spec["k"]
+    print("[bench] workload config %d: N=%d grid=%d^3 RG=%.2f cells k=%d"
+          % (args.config, n, spec["nx"], spec["rg_cells"], k), file=sys.stderr, flush=True)
+    dev = torch.device("cuda")
+    x, y, z = (torch.from_numpy(np.ascontiguousarray(pos[:, a])).to(dev) for a in range(3))
+    m = torch.from_numpy(np.ascontiguousarray(mass)).to(dev)
+    g = D.grid_struct(spec["nx"], spec["ny"], spec["nz"], spec["xmin"], spec["ymin"],
+                      spec["zmin"], spec["cellsize"])
+    den = torch.empty(D.padded_shape(spec["nx"], spec["ny"], spec["nz"]), dtype=torch.float32,
+                      device=dev)
+    rho_t = torch.empty(n, dtype=torch.float32, device=dev)
+    nbr = torch.empty((n, k), dtype=torch.int64, device=dev)
+    rho_k = torch.empty(n, dtype=torch.float32, device=dev)
+
+    def step():
+        D.dev_density_stage(x, y, z, m, g, spec["RG"], k, den, rho_t, nbr, rho_k)
+
+    for _ in range(args.warmup):
+        step()
+    D.dev_status()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    D.dev_status()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    value = n * args.steps / (t1 - t0)
+    print("[bench] timed %d steps: %.3f ms/step" % (args.steps, ms_step), file=sys.stderr,
+          flush=True)
+
+    # per-phase HIP-event timing on the launch stream (separate pass)
+    D.timing(enable=True, reset=True)
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    phases = {}
+    for ph in ("stage", "tsc", "smooth", "findden", "knn", "knn_prep", "knn_query"):
+        tot, cnt = D.timing_get(ph)
+        if cnt:
+            phases[ph] = tot / cnt
+    D.timing(enable=False, reset=True)
+    g_cells = spec["nx"] * spec["ny"] * spec["nz"]
+    bts = stage_bytes(n, g_cells, k)
+    q_ms = phases.get("knn_query", float("nan"))
+    achieved = bts["knn"] / (q_ms * 1e-3) / 1e9
+    roofline = {
+        "bound": "hbm",
+        "kernel": "knn_query_k",
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "bytes_per_launch": bts["knn"],
+        "ms_per_launch": round(q_ms, 4),
+    }
+    stage_alg = sum(bts.values())
+    out = {
+        "metric": "particles/s through density stage (deposit+smooth+read-back+kNN lists+"
+                  "W4 density), one halo",
+        "value": round(value, 1),
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (grid, positions) / f64 (kNN distances)",
+        "data": "synthetic (seeded NFW+Plummer halo)",
+        "config": {"workload": "one halo, N=%d, %d^3 grid, RG=%.1f cells, k=%d"
+                               % (n, spec["nx"], spec["rg_cells"], k),
+                   "particles": n, "grid": [spec["nx"], spec["ny"], spec["nz"]], "k": k},
+        "phase_ms": {k_: round(v, 4) for k_, v in phases.items()},
+        "stage_roofline_frac": round(stage_alg / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(pos.astype(np.float64), mass.astype(np.float64),
+                                           spec, k, args.knn_sample)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

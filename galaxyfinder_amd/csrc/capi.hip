@@ -1,0 +1,601 @@
+// C-ABI of the density-stage library: the reference's entry points
+// (assign_density_TSC, gaussian_Smoothing, findDen, findsphdensity,
+// starfindsphdensity; tsc_omp2.c / gsmooth.c / nnost.c) as synchronous
+// host-array calls, plus the device-resident API used from Python.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "pgalf_internal.h"
+
+namespace pgalf {
+
+// ---- workspace -------------------------------------------------------------
+Workspace::~Workspace() {
+    for (auto &c : chunks_) hipFree(c.ptr);
+    if (status_) hipFree(status_);
+}
+
+hipError_t Workspace::add_chunk(size_t bytes) {
+    Chunk c;
+    c.cap = bytes;
+    c.used = 0;
+    hipError_t e = hipMalloc(&c.ptr, bytes);
+    if (e != hipSuccess) return e;
+    chunks_.push_back(c);
+    return hipSuccess;
+}
+
+hipError_t Workspace::begin(hipStream_t s) {
+    if (!status_) {
+        PGALF_HIP_TRY(hipMalloc(&status_, sizeof(unsigned)));
+        PGALF_HIP_TRY(hipMemsetAsync(status_, 0, sizeof(unsigned), s));
+    }
+    size_t need = 0;
+    for (auto &c : chunks_) need += c.used;
+    if (need > high_water_) high_water_ = need;
+    if (chunks_.size() > 1 || (!chunks_.empty() && chunks_[0].cap < high_water_)) {
+        PGALF_HIP_TRY(hipStreamSynchronize(s));
+        for (auto &c : chunks_) hipFree(c.ptr);
+        chunks_.clear();
+        PGALF_HIP_TRY(add_chunk(high_water_ + (high_water_ >> 3) + (1 << 20)));
+    }
+    for (auto &c : chunks_) c.used = 0;
+    return hipSuccess;
+}
+
+void *Workspace::alloc(size_t bytes) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    if (bytes == 0) bytes = 256;
+    if (chunks_.empty() || chunks_.back().used + bytes > chunks_.back().cap) {
+        size_t cap = bytes < ((size_t)64 << 20) ? ((size_t)64 << 20) : bytes;
+        if (add_chunk(cap) != hipSuccess) return nullptr;
+    }
+    Chunk &c = chunks_.back();
+    void *p = c.ptr + c.used;
+    c.used += bytes;
+    return p;
+}
+
+static std::mutex g_ws_mu;
+static std::map<int, Workspace *> g_ws;
+
+Workspace &workspace_for_current_device() {
+    int dev = 0;
+    hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    auto it = g_ws.find(dev);
+    if (it != g_ws.end()) return *it->second;
+    Workspace *w = new Workspace(dev);
+    g_ws[dev] = w;
+    return *w;
+}
+
+// ---- timing ------------------------------------------------------------------
+static std::mutex g_t_mu;
+static bool g_t_on = false;
+struct PendingEv {
+    std::string name;
+    hipEvent_t a, b;
+    bool closed;
+};
+static std::vector<PendingEv> g_t_pending;
+static std::map<std::string, std::pair<double, int>> g_t_total;
+
+void timing_mark(const char *phase, hipStream_t s, bool end) {
+    std::lock_guard<std::mutex> lk(g_t_mu);
+    if (!g_t_on) return;
+    if (!end) {
+        PendingEv p;
+        p.name = phase;
+        p.closed = false;
+        hipEventCreate(&p.a);
+        hipEventCreate(&p.b);
+        hipEventRecord(p.a, s);
+        g_t_pending.push_back(p);
+    } else {
+        for (auto it = g_t_pending.rbegin(); it != g_t_pending.rend(); ++it) {
+            if (!it->closed && it->name == phase) {
+                hipEventRecord(it->b, s);
+                it->closed = true;
+                break;
+            }
+        }
+    }
+}
+
+static void timing_collect() {
+    for (auto &p : g_t_pending) {
+        if (p.closed) {
+            hipEventSynchronize(p.b);
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, p.a, p.b);
+            auto &t = g_t_total[p.name];
+            t.first += ms;
+            t.second += 1;
+        }
+        hipEventDestroy(p.a);
+        hipEventDestroy(p.b);
+    }
+    g_t_pending.clear();
+}
+
+// ---- AoS unpack ----------------------------------------------------------------
+__global__ void unpack_k(const unsigned char *bp, long long np, double *x, double *y,
+                         double *z, double *m, int *type) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const unsigned char *r = bp + i * sizeof(pgalf_particle_t);
+    type[i] = (int)*reinterpret_cast<const unsigned *>(r + offsetof(pgalf_particle_t, type));
+    m[i] = *reinterpret_cast<const double *>(r + offsetof(pgalf_particle_t, mass));
+    x[i] = *reinterpret_cast<const double *>(r + offsetof(pgalf_particle_t, x));
+    y[i] = *reinterpret_cast<const double *>(r + offsetof(pgalf_particle_t, y));
+    z[i] = *reinterpret_cast<const double *>(r + offsetof(pgalf_particle_t, z));
+}
+
+hipError_t unpack_particles(const void *d_bp, long long np, double *x, double *y, double *z,
+                            double *m, int *type, hipStream_t s) {
+    if (np <= 0) return hipSuccess;
+    unpack_k<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(
+        static_cast<const unsigned char *>(d_bp), np, x, y, z, m, type);
+    return hipGetLastError();
+}
+
+}  // namespace pgalf
+
+using namespace pgalf;
+
+static_assert(sizeof(pgalf_particle_t) == 88, "SimpleBasicParticleType is 88 bytes");
+static_assert(offsetof(pgalf_particle_t, mass) == 8, "layout");
+static_assert(offsetof(pgalf_particle_t, x) == 16, "layout");
+static_assert(offsetof(pgalf_particle_t, indx) == 72, "layout");
+static_assert(offsetof(pgalf_particle_t, bp) == 80, "layout");
+
+// ---- status ----------------------------------------------------------------------
+static thread_local int t_err = PGALF_OK;
+static thread_local char t_msg[256] = "ok";
+static std::mutex g_api_mu;  // host entry points share one workspace per device
+
+static int set_err(int code, const char *fmt, ...) {
+    t_err = code;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(t_msg, sizeof(t_msg), fmt, ap);
+    va_end(ap);
+    return code;
+}
+static int ok() {
+    t_err = PGALF_OK;
+    snprintf(t_msg, sizeof(t_msg), "ok");
+    return PGALF_OK;
+}
+static int hip_err(hipError_t e, const char *where) {
+    if (e == hipErrorOutOfMemory) return set_err(PGALF_ERR_NOMEM, "%s: out of device memory", where);
+    if (e == hipErrorNotReady)
+        return set_err(PGALF_ERR_SHORT_LIST, "%s: fewer candidates than neighbours", where);
+    return set_err(PGALF_ERR_HIP, "%s: %s", where, hipGetErrorString(e));
+}
+
+extern "C" int pgalf_last_error(void) { return t_err; }
+extern "C" const char *pgalf_last_error_string(void) { return t_msg; }
+extern "C" const char *pgalf_version(void) { return "pgalf-density-mi355x 0.1 (gfx950)"; }
+
+extern "C" void pgalf_timing_enable(int on) {
+    std::lock_guard<std::mutex> lk(g_t_mu);
+    g_t_on = on != 0;
+}
+extern "C" void pgalf_timing_reset(void) {
+    std::lock_guard<std::mutex> lk(g_t_mu);
+    timing_collect();
+    g_t_total.clear();
+}
+extern "C" int pgalf_timing_get(const char *phase, double *total_ms, int *count) {
+    std::lock_guard<std::mutex> lk(g_t_mu);
+    timing_collect();
+    auto it = g_t_total.find(phase);
+    if (it == g_t_total.end()) {
+        if (total_ms) *total_ms = 0.0;
+        if (count) *count = 0;
+        return 0;
+    }
+    if (total_ms) *total_ms = it->second.first;
+    if (count) *count = it->second.second;
+    return 1;
+}
+
+// Read and clear the device status word (synchronises s).
+static int consume_status(Workspace &ws, hipStream_t s, const char *where) {
+    unsigned st = 0;
+    hipError_t e = hipMemcpyAsync(&st, ws.status(), sizeof(unsigned), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = hipMemsetAsync(ws.status(), 0, sizeof(unsigned), s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_err(e, where);
+    if (st & DEV_ERR_OUT_OF_GRID)
+        return set_err(PGALF_ERR_OUT_OF_GRID, "%s: particles outside the grid were skipped", where);
+    if (st & DEV_ERR_SHORT_LIST)
+        return set_err(PGALF_ERR_SHORT_LIST, "%s: short neighbour list", where);
+    return ok();
+}
+
+static bool check_grid(int nx, int ny, int nz) {
+    return nx >= 3 && ny >= 3 && nz >= 3 && (long long)nx * ny * nz < (1LL << 31);
+}
+
+// ---- host-array entry points (reference drop-in) ------------------------------------
+namespace {
+
+struct DeviceParticles {
+    double *x, *y, *z, *m;
+    int *type;
+};
+
+hipError_t upload_particles(const pgalf_particle_t *bp, long long np, Workspace &ws,
+                            hipStream_t s, DeviceParticles &d) {
+    void *raw = ws.alloc(sizeof(pgalf_particle_t) * (size_t)np);
+    d.x = ws.alloc_n<double>(np);
+    d.y = ws.alloc_n<double>(np);
+    d.z = ws.alloc_n<double>(np);
+    d.m = ws.alloc_n<double>(np);
+    d.type = ws.alloc_n<int>(np);
+    if (!raw || !d.x || !d.y || !d.z || !d.m || !d.type) return hipErrorOutOfMemory;
+    PGALF_HIP_TRY(hipMemcpyAsync(raw, bp, sizeof(pgalf_particle_t) * (size_t)np,
+                                 hipMemcpyHostToDevice, s));
+    return unpack_particles(raw, np, d.x, d.y, d.z, d.m, d.type, s);
+}
+
+Grid make_grid(int nx, int ny, int nz, double x0, double y0, double z0, double h) {
+    Grid g;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.x0 = x0;
+    g.y0 = y0;
+    g.z0 = z0;
+    g.h = h;
+    return g;
+}
+
+}  // namespace
+
+extern "C" void assign_density_TSC(pgalf_particle_t *bp, int np, float *den, int nx, int ny,
+                                   int nz, double Xmin, double Ymin, double Zmin,
+                                   double cellsize, int ptype) {
+    if (np < 0 || !den || (np > 0 && !bp) || !check_grid(nx, ny, nz) || !(cellsize > 0)) {
+        set_err(PGALF_ERR_ARG, "assign_density_TSC: bad arguments");
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    hipStream_t s = nullptr;
+    Workspace &ws = workspace_for_current_device();
+    hipError_t e = ws.begin(s);
+    Grid g = make_grid(nx, ny, nz, Xmin, Ymin, Zmin, cellsize);
+    DeviceParticles d{};
+    float *dden = nullptr;
+    if (e == hipSuccess) e = upload_particles(bp, np, ws, s, d);
+    if (e == hipSuccess) {
+        dden = ws.alloc_n<float>(g.ncells());
+        if (!dden) e = hipErrorOutOfMemory;
+    }
+    if (e == hipSuccess)
+        e = tsc_deposit<double>(d.x, d.y, d.z, d.m, d.type, np, ptype, g, dden, ws, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(den, dden, sizeof(float) * g.ncells(), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) {
+        hip_err(e, "assign_density_TSC");
+        return;
+    }
+    consume_status(ws, s, "assign_density_TSC");
+}
+
+extern "C" void gaussian_Smoothing(float *denGrid, int nx, int ny, int nz, double cellsize,
+                                   float RG) {
+    if (!denGrid || !check_grid(nx, ny, nz) || !(cellsize > 0)) {
+        set_err(PGALF_ERR_ARG, "gaussian_Smoothing: bad arguments");
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    hipStream_t s = nullptr;
+    Workspace &ws = workspace_for_current_device();
+    hipError_t e = ws.begin(s);
+    Grid g = make_grid(nx, ny, nz, 0, 0, 0, cellsize);
+    float *dden = nullptr;
+    if (e == hipSuccess) {
+        dden = ws.alloc_n<float>(g.ncells());
+        if (!dden) e = hipErrorOutOfMemory;
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dden, denGrid, sizeof(float) * g.ncells(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = gaussian_smooth(dden, g, RG, ws, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(denGrid, dden, sizeof(float) * g.ncells(), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) {
+        hip_err(e, "gaussian_Smoothing");
+        return;
+    }
+    consume_status(ws, s, "gaussian_Smoothing");
+}
+
+extern "C" void findDen(pgalf_particle_t *bp, int np, float *densph, double xmin, double ymin,
+                        double zmin, double cellsize, float *den, int nx, int ny, int nz) {
+    if (np < 0 || !den || (np > 0 && (!bp || !densph)) || !check_grid(nx, ny, nz) ||
+        !(cellsize > 0)) {
+        set_err(PGALF_ERR_ARG, "findDen: bad arguments");
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    hipStream_t s = nullptr;
+    Workspace &ws = workspace_for_current_device();
+    hipError_t e = ws.begin(s);
+    Grid g = make_grid(nx, ny, nz, xmin, ymin, zmin, cellsize);
+    DeviceParticles d{};
+    float *dden = nullptr, *dout = nullptr;
+    if (e == hipSuccess) e = upload_particles(bp, np, ws, s, d);
+    if (e == hipSuccess) {
+        dden = ws.alloc_n<float>(g.ncells());
+        dout = ws.alloc_n<float>(np > 0 ? np : 1);
+        if (!dden || !dout) e = hipErrorOutOfMemory;
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dden, den, sizeof(float) * g.ncells(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = tsc_readback<double>(d.x, d.y, d.z, np, g, dden, dout, ws, s);
+    if (e == hipSuccess && np > 0)
+        e = hipMemcpyAsync(densph, dout, sizeof(float) * np, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) {
+        hip_err(e, "findDen");
+        return;
+    }
+    consume_status(ws, s, "findDen");
+}
+
+// kNN lists (+ optional density) for host particles.
+static int host_knn(pgalf_particle_t *bp, int np, int k, long long *nearindex, float *dens,
+                    int rho_mode, int cand_ptype, const char *where) {
+    if (np <= 0 || !bp || k < 1 || k > 32) return set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
+    if (k > np) return set_err(PGALF_ERR_SHORT_LIST, "%s: Numnear %d > np %d", where, k, np);
+    hipStream_t s = nullptr;
+    Workspace &ws = workspace_for_current_device();
+    hipError_t e = ws.begin(s);
+    DeviceParticles d{};
+    long long *dnbr = nullptr;
+    float *drho = nullptr;
+    if (e == hipSuccess) e = upload_particles(bp, np, ws, s, d);
+    if (e == hipSuccess && nearindex) {
+        dnbr = ws.alloc_n<long long>((size_t)np * k);
+        if (!dnbr) e = hipErrorOutOfMemory;
+    }
+    if (e == hipSuccess && dens) {
+        drho = ws.alloc_n<float>(np);
+        if (!drho) e = hipErrorOutOfMemory;
+    }
+    if (e == hipSuccess)
+        e = knn<double>(d.x, d.y, d.z, d.m, np, cand_ptype == PGALF_TYPE_ALL ? nullptr : d.type,
+                        cand_ptype, k, dnbr, drho, rho_mode, ws, s);
+    if (e == hipSuccess && dnbr)
+        e = hipMemcpyAsync(nearindex, dnbr, sizeof(long long) * (size_t)np * k,
+                           hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && drho)
+        e = hipMemcpyAsync(dens, drho, sizeof(float) * np, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return hip_err(e, where);
+    return consume_status(ws, s, where);
+}
+
+extern "C" void findsphdensity(pgalf_particle_t *bp, int np, long long *nearindex, int Numnear,
+                               float *densph) {
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    host_knn(bp, np, Numnear, nearindex, densph, 1, PGALF_TYPE_ALL, "findsphdensity");
+}
+
+extern "C" void starfindsphdensity(pgalf_particle_t *bp, int np, long long *nearindex,
+                                   int Numnear, float *densph) {
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    if (np <= 0 || !bp) {
+        set_err(PGALF_ERR_ARG, "starfindsphdensity: bad arguments");
+        return;
+    }
+    int nstar = 0;
+    for (int i = 0; i < np; i++) nstar += bp[i].type == PGALF_TYPE_STAR;
+    if (nstar == 0) {
+        set_err(PGALF_ERR_SHORT_LIST, "starfindsphdensity: no star particles");
+        return;
+    }
+    // NUMNEARDEN = 10 nearest stars (params.h:73); fewer when fewer stars exist,
+    // as Find_Near returns them (nnost.c:698-703).
+    int kden = nstar < 10 ? nstar : 10;
+    if (densph &&
+        host_knn(bp, np, kden, nullptr, densph, 2, PGALF_TYPE_STAR, "starfindsphdensity") !=
+            PGALF_OK)
+        return;
+    if (nearindex)
+        host_knn(bp, np, Numnear, nearindex, nullptr, 0, PGALF_TYPE_ALL, "starfindsphdensity");
+}
+
+extern "C" int pgalf_near_lists(pgalf_particle_t *bp, int np, int Numnear, long long *nearindex) {
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    if (!nearindex) return set_err(PGALF_ERR_ARG, "pgalf_near_lists: null output");
+    return host_knn(bp, np, Numnear, nearindex, nullptr, 0, PGALF_TYPE_ALL, "pgalf_near_lists");
+}
+
+extern "C" int pgalf_lag_density_stage(pgalf_particle_t *bp, int np, int Numnear, int ptype,
+                                       float *densph, long long *nearindex, float **grid_out,
+                                       int *nxyz_out, double *xyzmin_out) {
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    const char *where = "pgalf_lag_density_stage";
+    if (np <= 0 || !bp || !densph || Numnear < 1 || Numnear > 32)
+        return set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
+    if (nearindex && Numnear > np)
+        return set_err(PGALF_ERR_SHORT_LIST, "%s: Numnear %d > np %d", where, Numnear, np);
+    // Grid box of nnost.c:1040-1056: min/max of all particles, 1e-4 extent
+    // buffer on the max side, NCELLBUFF = 10 cells, origin shifted by 5 cells.
+    const double cellsize = 0.004;  // TSC_CELL_SIZE (params.h:92)
+    const float RG = 0.008f;        // Gaussian_Smoothing_Length (params.h:104)
+    const int nbuff = 10;           // NCELLBUFF (params.h:115)
+    double xmin = 1e20, ymin = 1e20, zmin = 1e20, xmax = -1e20, ymax = -1e20, zmax = -1e20;
+    for (int i = 0; i < np; i++) {
+        xmin = fmin(xmin, bp[i].x); ymin = fmin(ymin, bp[i].y); zmin = fmin(zmin, bp[i].z);
+        xmax = fmax(xmax, bp[i].x); ymax = fmax(ymax, bp[i].y); zmax = fmax(zmax, bp[i].z);
+    }
+    xmax = xmax + (xmax - xmin) * 0.0001;
+    ymax = ymax + (ymax - ymin) * 0.0001;
+    zmax = zmax + (zmax - zmin) * 0.0001;
+    int nx = (int)((xmax - xmin) / cellsize + nbuff);
+    int ny = (int)((ymax - ymin) / cellsize + nbuff);
+    int nz = (int)((zmax - zmin) / cellsize + nbuff);
+    xmin -= cellsize * nbuff / 2.;
+    ymin -= cellsize * nbuff / 2.;
+    zmin -= cellsize * nbuff / 2.;
+    if (!check_grid(nx, ny, nz)) return set_err(PGALF_ERR_ARG, "%s: grid too large", where);
+    hipStream_t s = nullptr;
+    Workspace &ws = workspace_for_current_device();
+    hipError_t e = ws.begin(s);
+    Grid g = make_grid(nx, ny, nz, xmin, ymin, zmin, cellsize);
+    DeviceParticles d{};
+    float *dden = nullptr, *drho = nullptr;
+    long long *dnbr = nullptr;
+    if (e == hipSuccess) e = upload_particles(bp, np, ws, s, d);
+    if (e == hipSuccess) {
+        dden = ws.alloc_n<float>(g.ncells());
+        drho = ws.alloc_n<float>(np);
+        if (nearindex) dnbr = ws.alloc_n<long long>((size_t)np * Numnear);
+        if (!dden || !drho || (nearindex && !dnbr)) e = hipErrorOutOfMemory;
+    }
+    if (e == hipSuccess) e = tsc_deposit<double>(d.x, d.y, d.z, d.m, d.type, np, ptype, g, dden, ws, s);
+    if (e == hipSuccess) e = gaussian_smooth(dden, g, RG, ws, s);
+    if (e == hipSuccess) e = tsc_readback<double>(d.x, d.y, d.z, np, g, dden, drho, ws, s);
+    if (e == hipSuccess && nearindex)
+        e = knn<double>(d.x, d.y, d.z, d.m, np, nullptr, PGALF_TYPE_ALL, Numnear, dnbr, nullptr, 0,
+                        ws, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(densph, drho, sizeof(float) * np, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && nearindex)
+        e = hipMemcpyAsync(nearindex, dnbr, sizeof(long long) * (size_t)np * Numnear,
+                           hipMemcpyDeviceToHost, s);
+    float *hgrid = nullptr;
+    if (e == hipSuccess && grid_out) {
+        hgrid = (float *)malloc(sizeof(float) * g.ncells());
+        if (!hgrid) return set_err(PGALF_ERR_NOMEM, "%s: host allocation failed", where);
+        e = hipMemcpyAsync(hgrid, dden, sizeof(float) * g.ncells(), hipMemcpyDeviceToHost, s);
+    }
+    if (e != hipSuccess) {
+        free(hgrid);
+        return hip_err(e, where);
+    }
+    int rc = consume_status(ws, s, where);
+    if (grid_out) *grid_out = hgrid;
+    if (nxyz_out) { nxyz_out[0] = nx; nxyz_out[1] = ny; nxyz_out[2] = nz; }
+    if (xyzmin_out) { xyzmin_out[0] = xmin; xyzmin_out[1] = ymin; xyzmin_out[2] = zmin; }
+    return rc;
+}
+
+// ---- device-resident API ---------------------------------------------------------------
+static Grid grid_from(const pgalf_grid_t *g) {
+    return make_grid(g->nx, g->ny, g->nz, g->xmin, g->ymin, g->zmin, g->cellsize);
+}
+
+extern "C" int pgalf_dev_tsc(const float *x, const float *y, const float *z, const float *m,
+                             const int *type, long long np, int ptype, const pgalf_grid_t *g,
+                             float *den, void *stream) {
+    if (!g || !den || np < 0 || !check_grid(g->nx, g->ny, g->nz))
+        return set_err(PGALF_ERR_ARG, "pgalf_dev_tsc: bad arguments");
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    hipStream_t s = (hipStream_t)stream;
+    Workspace &ws = workspace_for_current_device();
+    hipError_t e = ws.begin(s);
+    timing_mark("tsc", s, false);
+    if (e == hipSuccess) e = tsc_deposit<float>(x, y, z, m, type, np, ptype, grid_from(g), den, ws, s);
+    timing_mark("tsc", s, true);
+    return e == hipSuccess ? ok() : hip_err(e, "pgalf_dev_tsc");
+}
+
+extern "C" int pgalf_dev_smooth(float *den, const pgalf_grid_t *g, float RG, void *stream) {
+    if (!g || !den || !check_grid(g->nx, g->ny, g->nz))
+        return set_err(PGALF_ERR_ARG, "pgalf_dev_smooth: bad arguments");
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    hipStream_t s = (hipStream_t)stream;
+    Workspace &ws = workspace_for_current_device();
+    hipError_t e = ws.begin(s);
+    timing_mark("smooth", s, false);
+    if (e == hipSuccess) e = gaussian_smooth(den, grid_from(g), RG, ws, s);
+    timing_mark("smooth", s, true);
+    return e == hipSuccess ? ok() : hip_err(e, "pgalf_dev_smooth");
+}
+
+extern "C" int pgalf_dev_findden(const float *x, const float *y, const float *z, long long np,
+                                 const pgalf_grid_t *g, const float *den, float *densph,
+                                 void *stream) {
+    if (!g || !den || np < 0 || !check_grid(g->nx, g->ny, g->nz))
+        return set_err(PGALF_ERR_ARG, "pgalf_dev_findden: bad arguments");
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    hipStream_t s = (hipStream_t)stream;
+    Workspace &ws = workspace_for_current_device();
+    hipError_t e = ws.begin(s);
+    timing_mark("findden", s, false);
+    if (e == hipSuccess) e = tsc_readback<float>(x, y, z, np, grid_from(g), den, densph, ws, s);
+    timing_mark("findden", s, true);
+    return e == hipSuccess ? ok() : hip_err(e, "pgalf_dev_findden");
+}
+
+extern "C" int pgalf_dev_knn(const float *x, const float *y, const float *z, const float *m,
+                             const int *type, long long np, int cand_ptype, int k,
+                             long long *nbr, float *rho, int rho_mode, void *stream) {
+    if (np < 1 || k < 1 || k > 32 || rho_mode < 0 || rho_mode > 2 || (rho_mode == 2 && !m))
+        return set_err(PGALF_ERR_ARG, "pgalf_dev_knn: bad arguments");
+    if (k > np) return set_err(PGALF_ERR_SHORT_LIST, "pgalf_dev_knn: k > np");
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    hipStream_t s = (hipStream_t)stream;
+    Workspace &ws = workspace_for_current_device();
+    hipError_t e = ws.begin(s);
+    timing_mark("knn", s, false);
+    if (e == hipSuccess)
+        e = knn<float>(x, y, z, m, np, type, type ? cand_ptype : PGALF_TYPE_ALL, k, nbr,
+                       rho_mode ? rho : nullptr, rho_mode, ws, s);
+    timing_mark("knn", s, true);
+    return e == hipSuccess ? ok() : hip_err(e, "pgalf_dev_knn");
+}
+
+extern "C" int pgalf_dev_density_stage(const float *x, const float *y, const float *z,
+                                       const float *m, long long np, const pgalf_grid_t *g,
+                                       float RG, int k, float *den, float *rho_tsc,
+                                       long long *nbr, float *rho_knn, void *stream) {
+    if (!g || !den || np < 1 || k < 1 || k > 32 || !check_grid(g->nx, g->ny, g->nz))
+        return set_err(PGALF_ERR_ARG, "pgalf_dev_density_stage: bad arguments");
+    if (k > np) return set_err(PGALF_ERR_SHORT_LIST, "pgalf_dev_density_stage: k > np");
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    hipStream_t s = (hipStream_t)stream;
+    Workspace &ws = workspace_for_current_device();
+    hipError_t e = ws.begin(s);
+    Grid gg = grid_from(g);
+    timing_mark("stage", s, false);
+    timing_mark("tsc", s, false);
+    if (e == hipSuccess) e = tsc_deposit<float>(x, y, z, m, nullptr, np, PGALF_TYPE_ALL, gg, den, ws, s);
+    timing_mark("tsc", s, true);
+    timing_mark("smooth", s, false);
+    if (e == hipSuccess) e = gaussian_smooth(den, gg, RG, ws, s);
+    timing_mark("smooth", s, true);
+    timing_mark("findden", s, false);
+    if (e == hipSuccess && rho_tsc) e = tsc_readback<float>(x, y, z, np, gg, den, rho_tsc, ws, s);
+    timing_mark("findden", s, true);
+    timing_mark("knn", s, false);
+    if (e == hipSuccess && (nbr || rho_knn))
+        e = knn<float>(x, y, z, m, np, nullptr, PGALF_TYPE_ALL, k, nbr, rho_knn, rho_knn ? 1 : 0, ws, s);
+    timing_mark("knn", s, true);
+    timing_mark("stage", s, true);
+    return e == hipSuccess ? ok() : hip_err(e, "pgalf_dev_density_stage");
+}
+
+extern "C" int pgalf_dev_status(void *stream) {
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    hipStream_t s = (hipStream_t)stream;
+    Workspace &ws = workspace_for_current_device();
+    if (!ws.status()) {
+        hipError_t e = ws.begin(s);
+        if (e != hipSuccess) return hip_err(e, "pgalf_dev_status");
+    }
+    return consume_status(ws, s, "pgalf_dev_status");
+}

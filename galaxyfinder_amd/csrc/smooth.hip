@@ -1,0 +1,304 @@
+// FFT Gaussian smoothing for gfx950 (gaussian_Smoothing,
+// NewGalFinder/gsmooth.c:14-115).
+//
+// The reference transforms the padded grid with a 3-D r2c FFT, multiplies
+// bin (i,j,k) by fx(i)*fy(j)*fz(k), fd(k) = exp(-(pi RG_c)^2 (k'/n_d)^2)
+// with k' the signed frequency, and transforms back with a 1/(nx ny nz)
+// scale.  Because the multiplier is a product of real, even, per-axis
+// factors, the operator is exactly three 1-D circular convolutions, one per
+// axis.  Each axis pass here reads the grid once and writes it once:
+// a workgroup loads a batch of lines into LDS (two real lines packed as one
+// complex line), runs a Stockham radix-4/2 FFT, multiplies by the real
+// spectral factor H, runs the inverse FFT and stores the lines back.
+//   n_d a power of two:  L = n_d and H(k) = fd(k) / n_d.
+//   any other n_d:       L = pow2 >= 2 n_d - 1, the line is zero-padded and
+//                        H = DFT_L(h) / L with h the periodic kernel g_d laid
+//                        out circularly over [-(n_d-1), n_d-1]; the first n_d
+//                        outputs of the length-L linear convolution are the
+//                        length-n_d circular one.
+// H and the twiddles are formed on the host in FP64 and rounded to float.
+// Pad columns [nx, mx) are zeroed by the x pass (the reference leaves FFTW
+// scratch there).
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "pgalf_internal.h"
+
+namespace pgalf {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kLdsComplex = 8192;  // complex elements of line data per block (64 KB)
+constexpr int kMaxL = 4096;
+
+struct AxisPass {
+    int n;              // line length (grid points)
+    int L;              // FFT length (power of two)
+    long long sa;       // stride along the line
+    int nb;             // number of lines along the fast batch dimension
+    long long sb;       // stride of that dimension
+    long long sc;       // stride of the slow batch dimension (one per blockIdx.y)
+    int P;              // complex lines per block (2P real lines)
+    int contiguous;     // 1: lines are contiguous (x pass)
+    int zero_pads;      // x pass: zero columns [n, padw) of each row
+    int padw;           // row width for the pad zeroing (mx)
+};
+
+__device__ inline float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// One Stockham pass over P lines of length L in LDS (in place: all inputs of
+// this thread are read before the barrier, outputs written after it).
+template <int R>
+__device__ inline void stockham_pass(float2 *buf, int L, int P, int Ns, const float2 *tw,
+                                     bool inverse) {
+    // at most kLdsComplex/R butterflies per pass -> kMaxBf per thread; the
+    // per-thread loop is fully unrolled so v[][] stays in registers.
+    constexpr int kMaxBf = kLdsComplex / R / kBlock;
+    const int nbf_line = L / R;
+    const int nbf = P * nbf_line;
+    const int tstep = L / (Ns * R);  // twiddle index step
+    float2 v[kMaxBf][R];
+    int dst[kMaxBf];
+#pragma unroll
+    for (int c = 0; c < kMaxBf; c++) {
+        int b = threadIdx.x + c * kBlock;
+        if (b >= nbf) continue;
+        int p = b / nbf_line, j = b - p * nbf_line;
+        float2 *line = buf + p * L;
+        int k = j % Ns;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            float2 a = line[j + r * nbf_line];
+            if (r) {
+                float2 w = tw[(r * k * tstep) & (L - 1)];
+                if (inverse) w.y = -w.y;
+                a = cmul(a, w);
+            }
+            v[c][r] = a;
+        }
+        if (R == 2) {
+            float2 a0 = v[c][0], a1 = v[c][1];
+            v[c][0] = make_float2(a0.x + a1.x, a0.y + a1.y);
+            v[c][1] = make_float2(a0.x - a1.x, a0.y - a1.y);
+        } else {
+            float2 a0 = v[c][0], a1 = v[c][1], a2 = v[c][2], a3 = v[c][3];
+            float2 s02 = make_float2(a0.x + a2.x, a0.y + a2.y);
+            float2 d02 = make_float2(a0.x - a2.x, a0.y - a2.y);
+            float2 s13 = make_float2(a1.x + a3.x, a1.y + a3.y);
+            float2 d13 = make_float2(a1.x - a3.x, a1.y - a3.y);
+            // forward twiddle W4 = -i: (-i)(x + iy) = y - ix; inverse +i
+            float2 md13 = inverse ? make_float2(-d13.y, d13.x) : make_float2(d13.y, -d13.x);
+            v[c][0] = make_float2(s02.x + s13.x, s02.y + s13.y);
+            v[c][1] = make_float2(d02.x + md13.x, d02.y + md13.y);
+            v[c][2] = make_float2(s02.x - s13.x, s02.y - s13.y);
+            v[c][3] = make_float2(d02.x - md13.x, d02.y - md13.y);
+        }
+        dst[c] = p * L + (j / Ns) * Ns * R + k;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < kMaxBf; c++) {
+        int b = threadIdx.x + c * kBlock;
+        if (b >= nbf) continue;
+#pragma unroll
+        for (int r = 0; r < R; r++) buf[dst[c] + r * Ns] = v[c][r];
+    }
+    __syncthreads();
+}
+
+__device__ inline void fft_lines(float2 *buf, int L, int P, const float2 *tw, bool inverse) {
+    int Ns = 1;
+    while (Ns * 4 <= L) {
+        stockham_pass<4>(buf, L, P, Ns, tw, inverse);
+        Ns *= 4;
+    }
+    if (Ns < L) stockham_pass<2>(buf, L, P, Ns, tw, inverse);
+}
+
+__global__ void __launch_bounds__(kBlock)
+smooth_axis_k(float *g, AxisPass ap, const float2 *tw, const float *H) {
+    extern __shared__ float2 buf[];
+    const int L = ap.L, P = ap.P, n = ap.n;
+    const long long ib0 = (long long)blockIdx.x * 2 * P;
+    float *base = g + (long long)blockIdx.y * ap.sc;
+    const int lines = 2 * P;
+    // load: line lb (0..2P-1) element j -> buf[(lb/2)*L + j].(x|y)
+    const int tot = lines * n;
+    for (int idx = threadIdx.x; idx < tot; idx += kBlock) {
+        int lb, j;
+        if (ap.contiguous) { lb = idx / n; j = idx - lb * n; }
+        else { j = idx / lines; lb = idx - j * lines; }
+        long long ib = ib0 + lb;
+        float v = ib < ap.nb ? base[ib * ap.sb + (long long)j * ap.sa] : 0.0f;
+        float *dst = reinterpret_cast<float *>(&buf[(lb >> 1) * L + j]);
+        dst[lb & 1] = v;
+    }
+    for (int idx = threadIdx.x; idx < P * (L - n); idx += kBlock) {
+        int p = idx / (L - n), j = n + idx % (L - n);
+        buf[p * L + j] = make_float2(0.0f, 0.0f);
+    }
+    __syncthreads();
+    fft_lines(buf, L, P, tw, false);
+    for (int idx = threadIdx.x; idx < P * L; idx += kBlock) {
+        float h = H[idx & (L - 1)];
+        float2 a = buf[idx];
+        buf[idx] = make_float2(a.x * h, a.y * h);
+    }
+    __syncthreads();
+    fft_lines(buf, L, P, tw, true);
+    for (int idx = threadIdx.x; idx < tot; idx += kBlock) {
+        int lb, j;
+        if (ap.contiguous) { lb = idx / n; j = idx - lb * n; }
+        else { j = idx / lines; lb = idx - j * lines; }
+        long long ib = ib0 + lb;
+        if (ib < ap.nb) {
+            const float *src = reinterpret_cast<const float *>(&buf[(lb >> 1) * L + j]);
+            base[ib * ap.sb + (long long)j * ap.sa] = src[lb & 1];
+        }
+    }
+    if (ap.zero_pads) {
+        const int pw = ap.padw - n;
+        for (int idx = threadIdx.x; idx < lines * pw; idx += kBlock) {
+            int lb = idx / pw, j = n + idx % pw;
+            long long ib = ib0 + lb;
+            if (ib < ap.nb) base[ib * ap.sb + j] = 0.0f;
+        }
+    }
+}
+
+// ---- host planning ----------------------------------------------------------
+struct AxisPlan {
+    int n = 0, L = 0;
+    float2 *tw = nullptr;
+    float *H = nullptr;
+};
+
+std::mutex g_plan_mu;
+std::map<std::tuple<int, int, int, unsigned long long>, AxisPlan> g_plans;
+
+int next_pow2(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// gsmooth.c:76-86: exp(k1*k1*expfac/(n*n)), k1 the signed frequency.
+double axis_factor(long k, int n, double expfac) {
+    long k1 = k > n / 2 ? k - n : k;
+    return std::exp((double)(k1 * k1) * expfac / (double)((long)n * n));
+}
+
+hipError_t get_plan(int n, double expfac, AxisPlan &out) {
+    int dev = 0;
+    PGALF_HIP_TRY(hipGetDevice(&dev));
+    unsigned long long eb;
+    std::memcpy(&eb, &expfac, sizeof(eb));
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    auto key = std::make_tuple(dev, n, 0, eb);
+    auto it = g_plans.find(key);
+    if (it != g_plans.end()) {
+        out = it->second;
+        return hipSuccess;
+    }
+    const bool pow2 = (n & (n - 1)) == 0;
+    const int L = pow2 ? n : next_pow2(2 * n - 1);
+    if (L > kMaxL) return hipErrorInvalidValue;
+    const long double kPi = 3.141592653589793238462643383279502884L;
+    std::vector<float2> tw(L);
+    for (int m = 0; m < L; m++) {
+        long double a = -2.0L * kPi * (long double)m / (long double)L;
+        tw[m] = make_float2((float)cosl(a), (float)sinl(a));
+    }
+    std::vector<double> f(n);
+    for (int k = 0; k < n; k++) f[k] = axis_factor(k, n, expfac);
+    std::vector<float> H(L);
+    if (pow2) {
+        for (int k = 0; k < n; k++) H[k] = (float)(f[k] / (double)n);
+    } else {
+        // periodic kernel g[m] = (1/n) sum_k f(k) cos(2 pi k m / n)
+        std::vector<long double> gk(n);
+        for (int m = 0; m < n; m++) {
+            long double s = 0.0L;
+            for (int k = 0; k < n; k++)
+                s += (long double)f[k] *
+                     cosl(2.0L * kPi * (long double)(((long)k * m) % n) / (long double)n);
+            gk[m] = s / (long double)n;
+        }
+        std::vector<long double> h(L, 0.0L);
+        h[0] = gk[0];
+        for (int t = 1; t < n; t++) {
+            h[t] = gk[t];
+            h[L - t] = gk[n - t];
+        }
+        for (int k = 0; k < L; k++) {
+            long double s = 0.0L;
+            for (int t = 0; t < L; t++)
+                if (h[t] != 0.0L)
+                    s += h[t] * cosl(2.0L * kPi * (long double)(((long)k * t) % L) /
+                                     (long double)L);
+            H[k] = (float)(s / (long double)L);
+        }
+    }
+    AxisPlan p;
+    p.n = n;
+    p.L = L;
+    PGALF_HIP_TRY(hipMalloc(&p.tw, sizeof(float2) * L));
+    PGALF_HIP_TRY(hipMalloc(&p.H, sizeof(float) * L));
+    PGALF_HIP_TRY(hipMemcpy(p.tw, tw.data(), sizeof(float2) * L, hipMemcpyHostToDevice));
+    PGALF_HIP_TRY(hipMemcpy(p.H, H.data(), sizeof(float) * L, hipMemcpyHostToDevice));
+    g_plans[key] = p;
+    out = p;
+    return hipSuccess;
+}
+
+hipError_t run_pass(float *g, const AxisPlan &pl, long long sa, int nb, long long sb, int nc,
+                    long long sc, int contiguous, int zero_pads, int padw, hipStream_t s) {
+    AxisPass ap;
+    ap.n = pl.n;
+    ap.L = pl.L;
+    ap.sa = sa;
+    ap.nb = nb;
+    ap.sb = sb;
+    ap.sc = sc;
+    int P = kLdsComplex / pl.L;
+    if (P < 1) P = 1;
+    int maxP = (nb + 1) / 2;
+    if (P > maxP) P = maxP;
+    ap.P = P;
+    ap.contiguous = contiguous;
+    ap.zero_pads = zero_pads;
+    ap.padw = padw;
+    dim3 grid((unsigned)((nb + 2 * P - 1) / (2 * P)), (unsigned)nc);
+    size_t lds = sizeof(float2) * (size_t)P * pl.L;
+    smooth_axis_k<<<grid, kBlock, lds, s>>>(g, ap, pl.tw, pl.H);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t gaussian_smooth(float *den, const Grid &g, float RG, Workspace &ws, hipStream_t s) {
+    (void)ws;
+    // gsmooth.c:67-69: RG rescaled to cells in float, expfac in double.
+    float rgc = (float)((double)RG / g.h);
+    double expfac = -(M_PI * (double)rgc) * (M_PI * (double)rgc);
+    AxisPlan px, py, pz;
+    PGALF_HIP_TRY(get_plan(g.nx, expfac, px));
+    PGALF_HIP_TRY(get_plan(g.ny, expfac, py));
+    PGALF_HIP_TRY(get_plan(g.nz, expfac, pz));
+    const long long mx = g.mx();
+    // x: lines are rows (contiguous), batch over ny*nz rows
+    PGALF_HIP_TRY(run_pass(den, px, 1, g.ny * g.nz, mx, 1, 0, 1, 1, (int)mx, s));
+    // y: lines along y (stride mx), batch over x (stride 1), one z per block row
+    PGALF_HIP_TRY(run_pass(den, py, mx, g.nx, 1, g.nz, mx * g.ny, 0, 0, 0, s));
+    // z: lines along z (stride mx*ny), batch over x, one y per block row
+    PGALF_HIP_TRY(run_pass(den, pz, mx * g.ny, g.nx, 1, g.ny, mx, 0, 0, 0, s));
+    return hipSuccess;
+}
+
+}  // namespace pgalf

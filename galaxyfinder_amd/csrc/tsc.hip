@@ -1,0 +1,486 @@
+// TSC mass deposit and TSC read-back for gfx950.
+//
+// Deposit (assign_density_TSC, NewGalFinder/tsc_omp2.c:166-506):
+//   1. count:   every selected particle lists the 16^3 node tiles its 27-node
+//               stencil touches (1..8 tiles) -> per-tile counts (block-local
+//               LDS histograms, one global add per block and tile);
+//   2. scan:    tile offsets; tiles are cut into work units of <= kChunk
+//               entries so a cusp tile is shared by many workgroups;
+//   3. scatter: particle ids into per-tile bins (ranges reserved per block);
+//   4. deposit: one workgroup per unit accumulates its particles'
+//               contributions that land inside the tile in an LDS int64
+//               fixed-point tile (ds_add_u64).  Integer addition is
+//               associative, so the grid is bit-identical run to run
+//               whatever the order of the atomics; single-unit tiles are
+//               written once with plain coalesced stores, multi-unit tiles
+//               are summed in a global int64 side buffer and converted by a
+//               finalize pass.
+//   The per-contribution value is the reference's FP32 product
+//   (wx*wy)*wz with its float/double conversions (tsc_omp2.c:303-382); the
+//   fixed-point scale 2^S is chosen from max|m| * count so no sum overflows.
+//   Tiles span x in [0, mx), so pad columns are written as zero.
+//
+// Read-back (findDen, tsc_omp2.c:40-163): one thread per particle, 27
+// gathers summed in the reference's order and precision -- bit-identical to
+// the reference on the same grid (no FMA contraction: -ffp-contract=off).
+#include "pgalf_internal.h"
+
+namespace pgalf {
+namespace {
+
+constexpr int kT = 16;               // tile edge (nodes)
+constexpr int kTileNodes = kT * kT * kT;
+constexpr int kChunk = 4096;         // entries per deposit unit
+constexpr int kLdsHistMax = 8192;    // tiles handled with LDS histograms
+constexpr int kBlock = 256;
+constexpr int kPerBlock = 8192;      // particles per count/scatter block
+
+struct TileGeom {
+    int ntx, nty, ntz;
+    __host__ __device__ long long ntiles() const { return (long long)ntx * nty * ntz; }
+};
+
+struct Axis {
+    long long nr;
+    long long sg;
+    float w1, w2, w3;
+};
+
+// Per-axis deposit weights (tsc_omp2.c:303-343).  with_mass: the x axis,
+// whose weights carry the particle mass.
+__device__ inline void deposit_axis(double p, double pm, bool with_mass, Axis &a) {
+    double r = rint(p);
+    a.nr = (long long)r;
+    float off = (float)(p - (double)a.nr);
+    a.sg = (long long)copysignf(1.0f, off);
+    float d = (float)fabs((double)off);
+    float d2 = d * d;
+    if (with_mass) {
+        double ph = pm * 0.5;
+        a.w1 = (float)((0.75 - (double)d2) * pm);
+        a.w3 = (float)(ph * (0.25 + (double)d * ((double)d - 1.0)));
+        a.w2 = (float)((double)a.w3 + pm * (double)d);
+    } else {
+        a.w1 = (float)(0.75 - (double)d2);
+        a.w3 = (float)(0.5 * (0.25 + (double)d * ((double)d - 1.0)));
+        a.w2 = a.w3 + d;
+    }
+}
+
+// Weight of node offset o (relative to the nearest node) along one axis.
+__device__ inline float axis_weight(const Axis &a, long long o) {
+    return o == 0 ? a.w1 : (o == a.sg ? a.w2 : a.w3);
+}
+
+template <class P>
+__device__ inline bool particle_axes(const P *x, const P *y, const P *z, const P *m,
+                                     long long i, const Grid &g, Axis ax[3]) {
+    double pm = (double)m[i];
+    deposit_axis(((double)x[i] - g.x0) / g.h, pm, true, ax[0]);
+    deposit_axis(((double)y[i] - g.y0) / g.h, pm, false, ax[1]);
+    deposit_axis(((double)z[i] - g.z0) / g.h, pm, false, ax[2]);
+    // x nodes may reach the pad columns [nx, mx) like the reference's
+    // unchecked writes do (memory of the padded row); y/z must stay inside.
+    return ax[0].nr >= 1 && ax[0].nr <= g.mx() - 2 && ax[1].nr >= 1 &&
+           ax[1].nr <= g.ny - 2 && ax[2].nr >= 1 && ax[2].nr <= g.nz - 2;
+}
+
+__device__ inline bool selected(const int *type, long long i, int ptype) {
+    return ptype == PGALF_TYPE_ALL || type == nullptr || type[i] == ptype;
+}
+
+// Tiles touched by nodes [nr-1, nr+1] along one axis.
+__device__ inline void tile_span(long long nr, int &t0, int &t1) {
+    t0 = (int)((nr - 1) / kT);
+    t1 = (int)((nr + 1) / kT);
+}
+
+template <class P>
+__global__ void __launch_bounds__(kBlock)
+tsc_count_k(const P *x, const P *y, const P *z, const P *m, const int *type, long long np,
+            int ptype, Grid g, TileGeom tg, unsigned *tcount, unsigned *nsel,
+            unsigned long long *maxm_bits, unsigned *status, int use_lds) {
+    extern __shared__ unsigned hist[];
+    long long ntiles = tg.ntiles();
+    if (use_lds) {
+        for (long long t = threadIdx.x; t < ntiles; t += kBlock) hist[t] = 0;
+        __syncthreads();
+    }
+    long long b0 = (long long)blockIdx.x * kPerBlock;
+    long long b1 = min(b0 + kPerBlock, np);
+    unsigned local_sel = 0;
+    double local_max = 0.0;
+    for (long long i = b0 + threadIdx.x; i < b1; i += kBlock) {
+        if (!selected(type, i, ptype)) continue;
+        Axis ax[3];
+        if (!particle_axes(x, y, z, m, i, g, ax)) {
+            atomicOr(status, (unsigned)DEV_ERR_OUT_OF_GRID);
+            continue;
+        }
+        local_sel++;
+        local_max = fmax(local_max, fabs((double)m[i]));
+        int tx0, tx1, ty0, ty1, tz0, tz1;
+        tile_span(ax[0].nr, tx0, tx1);
+        tile_span(ax[1].nr, ty0, ty1);
+        tile_span(ax[2].nr, tz0, tz1);
+        for (int tz = tz0; tz <= tz1; tz++)
+            for (int ty = ty0; ty <= ty1; ty++)
+                for (int tx = tx0; tx <= tx1; tx++) {
+                    long long t = tx + (long long)tg.ntx * (ty + (long long)tg.nty * tz);
+                    if (use_lds) atomicAdd(&hist[t], 1u);
+                    else atomicAdd(&tcount[t], 1u);
+                }
+    }
+    if (local_sel) atomicAdd(nsel, local_sel);
+    if (local_max > 0.0) atomicMax(maxm_bits, (unsigned long long)__double_as_longlong(local_max));
+    if (use_lds) {
+        __syncthreads();
+        for (long long t = threadIdx.x; t < ntiles; t += kBlock)
+            if (hist[t]) atomicAdd(&tcount[t], hist[t]);
+    }
+}
+
+template <class P>
+__global__ void __launch_bounds__(kBlock)
+tsc_scatter_k(const P *x, const P *y, const P *z, const P *m, const int *type, long long np,
+              int ptype, Grid g, TileGeom tg, const unsigned *tstart, unsigned *tcursor,
+              unsigned *bins, int use_lds) {
+    extern __shared__ unsigned sh[];
+    long long ntiles = tg.ntiles();
+    unsigned *hist = sh;
+    unsigned *base = sh + (use_lds ? ntiles : 0);
+    long long b0 = (long long)blockIdx.x * kPerBlock;
+    long long b1 = min(b0 + kPerBlock, np);
+    if (use_lds) {
+        for (long long t = threadIdx.x; t < ntiles; t += kBlock) hist[t] = 0;
+        __syncthreads();
+        for (long long i = b0 + threadIdx.x; i < b1; i += kBlock) {
+            if (!selected(type, i, ptype)) continue;
+            Axis ax[3];
+            if (!particle_axes(x, y, z, m, i, g, ax)) continue;
+            int tx0, tx1, ty0, ty1, tz0, tz1;
+            tile_span(ax[0].nr, tx0, tx1);
+            tile_span(ax[1].nr, ty0, ty1);
+            tile_span(ax[2].nr, tz0, tz1);
+            for (int tz = tz0; tz <= tz1; tz++)
+                for (int ty = ty0; ty <= ty1; ty++)
+                    for (int tx = tx0; tx <= tx1; tx++)
+                        atomicAdd(&hist[tx + (long long)tg.ntx * (ty + (long long)tg.nty * tz)], 1u);
+        }
+        __syncthreads();
+        for (long long t = threadIdx.x; t < ntiles; t += kBlock) {
+            unsigned c = hist[t];
+            base[t] = c ? tstart[t] + atomicAdd(&tcursor[t], c) : 0u;
+            hist[t] = 0;
+        }
+        __syncthreads();
+    }
+    for (long long i = b0 + threadIdx.x; i < b1; i += kBlock) {
+        if (!selected(type, i, ptype)) continue;
+        Axis ax[3];
+        if (!particle_axes(x, y, z, m, i, g, ax)) continue;
+        int tx0, tx1, ty0, ty1, tz0, tz1;
+        tile_span(ax[0].nr, tx0, tx1);
+        tile_span(ax[1].nr, ty0, ty1);
+        tile_span(ax[2].nr, tz0, tz1);
+        for (int tz = tz0; tz <= tz1; tz++)
+            for (int ty = ty0; ty <= ty1; ty++)
+                for (int tx = tx0; tx <= tx1; tx++) {
+                    long long t = tx + (long long)tg.ntx * (ty + (long long)tg.nty * tz);
+                    unsigned slot;
+                    if (use_lds) slot = base[t] + atomicAdd(&hist[t], 1u);
+                    else slot = tstart[t] + atomicAdd(&tcursor[t], 1u);
+                    bins[slot] = (unsigned)i;
+                }
+    }
+}
+
+// Units per tile (>= 1 so every tile, including empty and pad-only ones,
+// is written) and whether the tile needs the multi-unit side buffer.
+__global__ void tsc_units_k(const unsigned *tcount, long long ntiles, unsigned *nunits,
+                            unsigned *multi) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    unsigned c = tcount[t];
+    unsigned u = c == 0 ? 1u : (c + kChunk - 1) / kChunk;
+    nunits[t] = u;
+    multi[t] = u > 1 ? 1u : 0u;
+}
+
+__global__ void tsc_unit_map_k(const unsigned *ustart, const unsigned *nunits,
+                               long long ntiles, unsigned *unit_tile) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    unsigned s = ustart[t];
+    for (unsigned j = 0; j < nunits[t]; j++) unit_tile[s + j] = (unsigned)t;
+}
+
+// Fixed-point scale from max|m| and the selected count.
+__global__ void tsc_scale_k(const unsigned *nsel, const unsigned long long *maxm_bits,
+                            double *scale) {
+    double maxm = __longlong_as_double((long long)*maxm_bits);
+    double bound = maxm * (double)(*nsel) * (1.0 + 1.0 / 1024.0);
+    int S = 0;
+    if (bound > 0.0) {
+        int e;
+        frexp(bound, &e);  // bound < 2^e
+        S = 62 - e;
+    }
+    double vf = 0.001 / 0.004;  // (0.001/TSC_CELL_SIZE)^3, tsc_omp2.c:188
+    vf = vf * vf * vf;
+    scale[0] = ldexp(1.0, S);
+    scale[1] = ldexp(1.0, -S) * vf;
+}
+
+template <class P>
+__global__ void __launch_bounds__(kBlock)
+tsc_deposit_k(const P *x, const P *y, const P *z, const P *m, Grid g, TileGeom tg,
+              const unsigned *tstart, const unsigned *tcount, const unsigned *ustart,
+              const unsigned *nunits, const unsigned *unit_tile, const unsigned *total_units,
+              const unsigned *multi_slot, const unsigned *bins, const double *scale,
+              float *den, unsigned long long *side) {
+    __shared__ unsigned long long acc[kTileNodes];
+    const long long mx = g.mx();
+    const double sc = scale[0], inv = scale[1];
+    const unsigned nunit_total = *total_units;
+    for (unsigned u = blockIdx.x; u < nunit_total; u += gridDim.x) {
+        unsigned t = unit_tile[u];
+        unsigned chunk = u - ustart[t];
+        unsigned nu = nunits[t];
+        int tx = (int)(t % tg.ntx);
+        int ty = (int)((t / tg.ntx) % tg.nty);
+        int tz = (int)(t / ((unsigned)tg.ntx * tg.nty));
+        long long ox = (long long)tx * kT, oy = (long long)ty * kT, oz = (long long)tz * kT;
+        for (int i = threadIdx.x; i < kTileNodes; i += kBlock) acc[i] = 0ull;
+        __syncthreads();
+        unsigned e0 = tstart[t] + chunk * kChunk;
+        unsigned e1 = min(e0 + kChunk, tstart[t] + tcount[t]);
+        for (unsigned e = e0 + threadIdx.x; e < e1; e += kBlock) {
+            long long i = bins[e];
+            Axis ax[3];
+            particle_axes(x, y, z, m, i, g, ax);
+#pragma unroll
+            for (int c = -1; c <= 1; c++) {
+                long long nzn = ax[2].nr + c - oz;
+                if (nzn < 0 || nzn >= kT) continue;
+                float wz = axis_weight(ax[2], c);
+#pragma unroll
+                for (int b = -1; b <= 1; b++) {
+                    long long nyn = ax[1].nr + b - oy;
+                    if (nyn < 0 || nyn >= kT) continue;
+                    float wy = axis_weight(ax[1], b);
+#pragma unroll
+                    for (int a = -1; a <= 1; a++) {
+                        long long nxn = ax[0].nr + a - ox;
+                        if (nxn < 0 || nxn >= kT) continue;
+                        float wx = axis_weight(ax[0], a);
+                        float wxy = wx * wy;
+                        float cval = wxy * wz;
+                        long long q = (long long)rint((double)cval * sc);
+                        atomicAdd(&acc[nxn + kT * (nyn + kT * nzn)], (unsigned long long)q);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (nu == 1) {
+            for (int l = threadIdx.x; l < kTileNodes; l += kBlock) {
+                long long gx = ox + (l % kT), gy = oy + ((l / kT) % kT), gz = oz + l / (kT * kT);
+                if (gx < mx && gy < g.ny && gz < g.nz)
+                    den[gx + mx * (gy + (long long)g.ny * gz)] =
+                        (float)((double)(long long)acc[l] * inv);
+            }
+        } else {
+            unsigned long long *dst = side + (size_t)multi_slot[t] * kTileNodes;
+            for (int l = threadIdx.x; l < kTileNodes; l += kBlock)
+                if (acc[l]) atomicAdd(&dst[l], acc[l]);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void tsc_finalize_k(Grid g, TileGeom tg, const unsigned *nunits,
+                               const unsigned *multi_slot, const unsigned long long *side,
+                               const double *scale, float *den) {
+    unsigned t = blockIdx.x;
+    if (nunits[t] <= 1) return;
+    const long long mx = g.mx();
+    const double inv = scale[1];
+    int tx = (int)(t % tg.ntx);
+    int ty = (int)((t / tg.ntx) % tg.nty);
+    int tz = (int)(t / ((unsigned)tg.ntx * tg.nty));
+    const unsigned long long *src = side + (size_t)multi_slot[t] * kTileNodes;
+    for (int l = threadIdx.x; l < kTileNodes; l += blockDim.x) {
+        long long gx = (long long)tx * kT + (l % kT), gy = (long long)ty * kT + ((l / kT) % kT),
+                  gz = (long long)tz * kT + l / (kT * kT);
+        if (gx < mx && gy < g.ny && gz < g.nz)
+            den[gx + mx * (gy + (long long)g.ny * gz)] = (float)((double)(long long)src[l] * inv);
+    }
+}
+
+// Zero the side slots actually used (their count is known on device only).
+__global__ void fill_side_k(unsigned long long *p, const unsigned *nmulti) {
+    long long n = (long long)(*nmulti) * kTileNodes;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
+// ---- read-back -------------------------------------------------------------
+struct RAxis {
+    int nr;
+    long sg;
+    float w1, w2, w3;
+};
+
+__device__ inline void readback_axis(double p, RAxis &a) {
+    a.nr = (int)rint(p);
+    float off = (float)(p - (double)a.nr);
+    a.sg = (long)copysignf(1.0f, off);
+    float d = (float)fabs((double)off);
+    float d2 = d * d;
+    a.w1 = (float)(0.75 - (double)d2);
+    a.w3 = (float)(0.5 * (0.25 + (double)d * ((double)d - 1.0)));
+    a.w2 = a.w3 + d;
+}
+
+template <class P>
+__global__ void __launch_bounds__(kBlock)
+tsc_readback_k(const P *x, const P *y, const P *z, long long np, Grid g, const float *den,
+               float *out, unsigned *status) {
+    long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= np) return;
+    RAxis a[3];
+    readback_axis(((double)x[i] - g.x0) / g.h, a[0]);
+    readback_axis(((double)y[i] - g.y0) / g.h, a[1]);
+    readback_axis(((double)z[i] - g.z0) / g.h, a[2]);
+    if (a[0].nr < 1 || a[0].nr > g.mx() - 2 || a[1].nr < 1 || a[1].nr > g.ny - 2 ||
+        a[2].nr < 1 || a[2].nr > g.nz - 2) {
+        atomicOr(status, (unsigned)DEV_ERR_OUT_OF_GRID);
+        out[i] = 0.0f;
+        return;
+    }
+    const long mx = (long)g.mx();
+    long ix[3] = {a[0].nr, a[0].nr + a[0].sg, a[0].nr - a[0].sg};
+    long iy[3] = {a[1].nr, a[1].nr + a[1].sg, a[1].nr - a[1].sg};
+    long iz[3] = {a[2].nr, a[2].nr + a[2].sg, a[2].nr - a[2].sg};
+    float wx[3] = {a[0].w1, a[0].w2, a[0].w3};
+    float wy[3] = {a[1].w1, a[1].w2, a[1].w3};
+    float wz[3] = {a[2].w1, a[2].w2, a[2].w3};
+    float wxy[3][3];
+#pragma unroll
+    for (int b = 0; b < 3; b++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) wxy[b][c] = wx[c] * wy[b];
+    float s = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                float gv = den[ix[q] + mx * (iy[b] + (long)g.ny * iz[c])];
+                s += (gv * wxy[b][q]) * wz[c];
+            }
+    out[i] = s;
+}
+
+inline TileGeom tile_geom(const Grid &g) {
+    TileGeom tg;
+    tg.ntx = (int)((g.mx() + kT - 1) / kT);
+    tg.nty = (g.ny + kT - 1) / kT;
+    tg.ntz = (g.nz + kT - 1) / kT;
+    return tg;
+}
+
+}  // namespace
+
+template <class P>
+hipError_t tsc_deposit(const P *x, const P *y, const P *z, const P *m, const int *type,
+                       long long np, int ptype, const Grid &g, float *den, Workspace &ws,
+                       hipStream_t s) {
+    TileGeom tg = tile_geom(g);
+    long long ntiles = tg.ntiles();
+    int use_lds = ntiles <= kLdsHistMax ? 1 : 0;
+    // counters: tcount, tcursor, nsel, maxm, total_units, nmulti
+    unsigned *tcount = ws.alloc_n<unsigned>(ntiles);
+    unsigned *tcursor = ws.alloc_n<unsigned>(ntiles);
+    unsigned *tstart = ws.alloc_n<unsigned>(ntiles);
+    unsigned *nunits = ws.alloc_n<unsigned>(ntiles);
+    unsigned *ustart = ws.alloc_n<unsigned>(ntiles);
+    unsigned *multi = ws.alloc_n<unsigned>(ntiles);
+    unsigned *misc = ws.alloc_n<unsigned>(8);  // [0]=nsel [1]=total [2]=units [3]=nmulti
+    unsigned long long *maxm = ws.alloc_n<unsigned long long>(1);
+    double *scale = ws.alloc_n<double>(2);
+    if (!tcount || !tcursor || !tstart || !nunits || !ustart || !multi || !misc || !maxm ||
+        !scale)
+        return hipErrorOutOfMemory;
+    PGALF_HIP_TRY(hipMemsetAsync(tcount, 0, sizeof(unsigned) * ntiles, s));
+    PGALF_HIP_TRY(hipMemsetAsync(tcursor, 0, sizeof(unsigned) * ntiles, s));
+    PGALF_HIP_TRY(hipMemsetAsync(misc, 0, sizeof(unsigned) * 8, s));
+    PGALF_HIP_TRY(hipMemsetAsync(maxm, 0, sizeof(unsigned long long), s));
+    long long nblk = (np + kPerBlock - 1) / kPerBlock;
+    size_t lds_count = use_lds ? sizeof(unsigned) * ntiles : 0;
+    if (nblk > 0)
+        tsc_count_k<P><<<dim3((unsigned)nblk), dim3(kBlock), lds_count, s>>>(
+            x, y, z, m, type, np, ptype, g, tg, tcount, &misc[0], maxm, ws.status(), use_lds);
+    PGALF_HIP_TRY(exclusive_scan_u32(tcount, tstart, ntiles, &misc[1], ws, s));
+    // the bin list can hold at most 8 entries per particle
+    unsigned *bins = ws.alloc_n<unsigned>((size_t)np * 8 + 1);
+    if (!bins) return hipErrorOutOfMemory;
+    if (nblk > 0)
+        tsc_scatter_k<P><<<dim3((unsigned)nblk), dim3(kBlock), 2 * lds_count, s>>>(
+            x, y, z, m, type, np, ptype, g, tg, tstart, tcursor, bins, use_lds);
+    unsigned tb = (unsigned)((ntiles + 255) / 256);
+    tsc_units_k<<<tb, 256, 0, s>>>(tcount, ntiles, nunits, multi);
+    PGALF_HIP_TRY(exclusive_scan_u32(nunits, ustart, ntiles, &misc[2], ws, s));
+    PGALF_HIP_TRY(exclusive_scan_u32(multi, multi, ntiles, &misc[3], ws, s));
+    // unit -> tile map: the unit count is bounded by ntiles + 8*np/kChunk
+    size_t max_units = (size_t)ntiles + (size_t)(8 * np) / kChunk + 1;
+    unsigned *unit_tile = ws.alloc_n<unsigned>(max_units);
+    // multi-unit side buffer: a tile with > kChunk entries needs one slot;
+    // there are at most 8*np/kChunk of them.
+    size_t max_multi = (size_t)(8 * np) / kChunk + 1;
+    if ((long long)max_multi > ntiles) max_multi = (size_t)ntiles;
+    unsigned long long *side = ws.alloc_n<unsigned long long>(max_multi * kTileNodes);
+    if (!unit_tile || !side) return hipErrorOutOfMemory;
+    tsc_unit_map_k<<<tb, 256, 0, s>>>(ustart, nunits, ntiles, unit_tile);
+    tsc_scale_k<<<1, 1, 0, s>>>(&misc[0], maxm, scale);
+    static int ncu = 0;
+    if (ncu == 0) {
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ws.device()) !=
+                hipSuccess || ncu <= 0)
+            ncu = 256;
+    }
+    fill_side_k<<<(unsigned)ncu * 4, 256, 0, s>>>(side, &misc[3]);
+    unsigned grid = (unsigned)ncu * 4;
+    tsc_deposit_k<P><<<grid, kBlock, 0, s>>>(x, y, z, m, g, tg, tstart, tcount, ustart, nunits,
+                                            unit_tile, &misc[2], multi, bins, scale, den,
+                                            side);
+    tsc_finalize_k<<<(unsigned)ntiles, 256, 0, s>>>(g, tg, nunits, multi, side, scale, den);
+    return hipGetLastError();
+}
+
+template <class P>
+hipError_t tsc_readback(const P *x, const P *y, const P *z, long long np, const Grid &g,
+                        const float *den, float *out, Workspace &ws, hipStream_t s) {
+    if (np <= 0) return hipSuccess;
+    tsc_readback_k<P><<<(unsigned)((np + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+        x, y, z, np, g, den, out, ws.status());
+    return hipGetLastError();
+}
+
+template hipError_t tsc_deposit<float>(const float *, const float *, const float *,
+                                       const float *, const int *, long long, int,
+                                       const Grid &, float *, Workspace &, hipStream_t);
+template hipError_t tsc_deposit<double>(const double *, const double *, const double *,
+                                        const double *, const int *, long long, int,
+                                        const Grid &, float *, Workspace &, hipStream_t);
+template hipError_t tsc_readback<float>(const float *, const float *, const float *, long long,
+                                        const Grid &, const float *, float *, Workspace &,
+                                        hipStream_t);
+template hipError_t tsc_readback<double>(const double *, const double *, const double *,
+                                         long long, const Grid &, const float *, float *,
+                                         Workspace &, hipStream_t);
+
+}  // namespace pgalf

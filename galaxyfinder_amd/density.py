@@ -1,0 +1,262 @@
+"""Python mirror of NewGalFinder's density-stage interface, backed by the HIP
+library (libpgalf_density.so).
+
+Host functions keep the reference's names and argument meaning
+(tsc_omp2.c:40,166; gsmooth.c:14; nnost.c:560,632) with numpy arrays in
+place of C pointers; outputs are written into the caller's arrays as the C
+functions do.  Failures raise ``PgalfError`` (the C library reports a
+status instead of calling exit()).
+
+Device functions (``dev_*``) take float32 torch tensors resident on the GPU
+and run on torch's current stream.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import PgalfError, pgalf_grid_t
+
+TYPE_STAR = 1
+TYPE_DM = 2
+TYPE_GAS = 3
+TYPE_SINK = 4
+TYPE_ALL = 5
+
+# SimpleBasicParticleType (tree.h:155-162), 88 bytes.
+PARTICLE_DTYPE = np.dtype({
+    "names": ["type", "mass", "x", "y", "z", "vx", "vy", "vz", "link02", "indx", "bp"],
+    "formats": ["<u4", "<f8", "<f8", "<f8", "<f8", "<f8", "<f8", "<f8", "<f8", "<i4", "<u8"],
+    "offsets": [0, 8, 16, 24, 32, 40, 48, 56, 64, 72, 80],
+    "itemsize": 88,
+})
+
+
+def make_particles(pos, mass=None, types=None):
+    """Pack positions (n,3), masses and types into reference particle records."""
+    pos = np.asarray(pos, dtype=np.float64)
+    n = len(pos)
+    bp = np.zeros(n, dtype=PARTICLE_DTYPE)
+    bp["x"], bp["y"], bp["z"] = pos[:, 0], pos[:, 1], pos[:, 2]
+    bp["mass"] = 1.0 if mass is None else np.asarray(mass, dtype=np.float64)
+    bp["type"] = TYPE_STAR if types is None else np.asarray(types, dtype=np.uint32)
+    bp["indx"] = np.arange(n, dtype=np.int32)
+    return bp
+
+
+def padded_shape(nx, ny, nz):
+    """(nz, ny, mx) of the in-place r2c grid, mx = 2(nx/2+1) (tsc_omp2.c:198)."""
+    return (nz, ny, 2 * (nx // 2 + 1))
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _need(a, dtype, n, what):
+    if not isinstance(a, np.ndarray) or a.dtype != dtype or not a.flags.c_contiguous:
+        raise TypeError("%s must be a C-contiguous %s array" % (what, np.dtype(dtype).name))
+    if a.size < n:
+        raise ValueError("%s holds %d elements, needs %d" % (what, a.size, n))
+
+
+def assign_density_TSC(bp, np_, den, nx, ny, nz, Xmin, Ymin, Zmin, cellsize, ptype):
+    """TSC deposit into den (float32, mx*ny*nz) -- tsc_omp2.c:166."""
+    lib = _lib.load()
+    _need(bp, PARTICLE_DTYPE, np_, "bp")
+    _need(den, np.float32, 2 * (nx // 2 + 1) * ny * nz, "den")
+    lib.assign_density_TSC(_ptr(bp), int(np_), _ptr(den), nx, ny, nz, Xmin, Ymin, Zmin,
+                           cellsize, ptype)
+    _lib.check()
+
+
+def gaussian_Smoothing(denGrid, nx, ny, nz, cellsize, RG):
+    """In-place FFT Gaussian smoothing of the padded grid -- gsmooth.c:14."""
+    lib = _lib.load()
+    _need(denGrid, np.float32, 2 * (nx // 2 + 1) * ny * nz, "denGrid")
+    lib.gaussian_Smoothing(_ptr(denGrid), nx, ny, nz, cellsize, RG)
+    _lib.check()
+
+
+def findDen(bp, np_, densph, xmin, ymin, zmin, cellsize, den, nx, ny, nz):
+    """TSC read-back into densph (float32, np) -- tsc_omp2.c:40."""
+    lib = _lib.load()
+    _need(bp, PARTICLE_DTYPE, np_, "bp")
+    _need(densph, np.float32, np_, "densph")
+    _need(den, np.float32, 2 * (nx // 2 + 1) * ny * nz, "den")
+    lib.findDen(_ptr(bp), int(np_), _ptr(densph), xmin, ymin, zmin, cellsize, _ptr(den),
+                nx, ny, nz)
+    _lib.check()
+
+
+def findsphdensity(bp, np_, nearindex, Numnear, densph):
+    """Exact Numnear-NN lists + unweighted W4 density -- nnost.c:560."""
+    lib = _lib.load()
+    _need(bp, PARTICLE_DTYPE, np_, "bp")
+    _need(nearindex, np.int64, np_ * Numnear, "nearindex")
+    _need(densph, np.float32, np_, "densph")
+    lib.findsphdensity(_ptr(bp), int(np_), _ptr(nearindex), int(Numnear), _ptr(densph))
+    _lib.check()
+
+
+def starfindsphdensity(bp, np_, nearindex, Numnear, densph):
+    """Star-weighted W4 density (10 nearest stars) + all-particle lists --
+    nnost.c:632."""
+    lib = _lib.load()
+    _need(bp, PARTICLE_DTYPE, np_, "bp")
+    _need(nearindex, np.int64, np_ * Numnear, "nearindex")
+    _need(densph, np.float32, np_, "densph")
+    lib.starfindsphdensity(_ptr(bp), int(np_), _ptr(nearindex), int(Numnear), _ptr(densph))
+    _lib.check()
+
+
+def near_lists(bp, np_, Numnear, nearindex):
+    """All-particle Numnear-NN lists of lagFindStellarCore (nnost.c:1298-1536)."""
+    lib = _lib.load()
+    _need(bp, PARTICLE_DTYPE, np_, "bp")
+    _need(nearindex, np.int64, np_ * Numnear, "nearindex")
+    _lib.check(lib.pgalf_near_lists(_ptr(bp), int(np_), int(Numnear), _ptr(nearindex)))
+
+
+def lag_density_stage(bp, Numnear, ptype, want_grid=False):
+    """The density stage of lagFindStellarCore (nnost.c:1030-1096,
+    1298-1536): returns (densph, nearindex[np,Numnear], grid or None,
+    (nx,ny,nz), (xmin,ymin,zmin))."""
+    lib = _lib.load()
+    n = len(bp)
+    _need(bp, PARTICLE_DTYPE, n, "bp")
+    densph = np.zeros(n, dtype=np.float32)
+    nbr = np.zeros((n, Numnear), dtype=np.int64)
+    gptr = ctypes.POINTER(ctypes.c_float)()
+    nxyz = (ctypes.c_int * 3)()
+    x0 = (ctypes.c_double * 3)()
+    rc = lib.pgalf_lag_density_stage(_ptr(bp), n, int(Numnear), int(ptype), _ptr(densph),
+                                     _ptr(nbr), ctypes.byref(gptr) if want_grid else None,
+                                     nxyz, x0)
+    grid = None
+    if want_grid and gptr:
+        nx, ny, nz = nxyz[0], nxyz[1], nxyz[2]
+        count = 2 * (nx // 2 + 1) * ny * nz
+        grid = np.ctypeslib.as_array(gptr, shape=(count,)).copy().reshape(padded_shape(nx, ny, nz))
+        libc = ctypes.CDLL(None)
+        libc.free(gptr)
+    _lib.check(rc)
+    return densph, nbr, grid, tuple(nxyz), tuple(x0)
+
+
+# ---------------------------------------------------------------------------
+# device-resident API (torch tensors on the GPU)
+# ---------------------------------------------------------------------------
+def _torch():
+    import torch
+    return torch
+
+
+def _dptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream():
+    torch = _torch()
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _check_dev(t, dtype, n, what):
+    torch = _torch()
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != dtype or \
+            not t.is_contiguous() or t.numel() < n:
+        raise TypeError("%s must be a contiguous %s CUDA tensor with >= %d elements"
+                        % (what, dtype, n))
+
+
+def grid_struct(nx, ny, nz, xmin, ymin, zmin, cellsize):
+    return pgalf_grid_t(int(nx), int(ny), int(nz), float(xmin), float(ymin), float(zmin),
+                        float(cellsize))
+
+
+def dev_tsc(x, y, z, m, grid, den, ptype=TYPE_ALL, types=None):
+    torch = _torch()
+    n = x.numel()
+    for t, w in ((x, "x"), (y, "y"), (z, "z"), (m, "m")):
+        _check_dev(t, torch.float32, n, w)
+    if types is not None:
+        _check_dev(types, torch.int32, n, "types")
+    _check_dev(den, torch.float32, 2 * (grid.nx // 2 + 1) * grid.ny * grid.nz, "den")
+    lib = _lib.load()
+    _lib.check(lib.pgalf_dev_tsc(_dptr(x), _dptr(y), _dptr(z), _dptr(m), _dptr(types), n,
+                                 int(ptype), ctypes.byref(grid), _dptr(den), _stream()))
+
+
+def dev_smooth(den, grid, RG):
+    torch = _torch()
+    _check_dev(den, torch.float32, 2 * (grid.nx // 2 + 1) * grid.ny * grid.nz, "den")
+    lib = _lib.load()
+    _lib.check(lib.pgalf_dev_smooth(_dptr(den), ctypes.byref(grid), float(RG), _stream()))
+
+
+def dev_findden(x, y, z, grid, den, out):
+    torch = _torch()
+    n = x.numel()
+    for t, w in ((x, "x"), (y, "y"), (z, "z"), (out, "out")):
+        _check_dev(t, torch.float32, n, w)
+    lib = _lib.load()
+    _lib.check(lib.pgalf_dev_findden(_dptr(x), _dptr(y), _dptr(z), n, ctypes.byref(grid),
+                                     _dptr(den), _dptr(out), _stream()))
+
+
+def dev_knn(x, y, z, k, m=None, nbr=None, rho=None, rho_mode=0, types=None,
+            cand_ptype=TYPE_ALL):
+    torch = _torch()
+    n = x.numel()
+    for t, w in ((x, "x"), (y, "y"), (z, "z")):
+        _check_dev(t, torch.float32, n, w)
+    if nbr is not None:
+        _check_dev(nbr, torch.int64, n * k, "nbr")
+    if rho is not None:
+        _check_dev(rho, torch.float32, n, "rho")
+    lib = _lib.load()
+    _lib.check(lib.pgalf_dev_knn(_dptr(x), _dptr(y), _dptr(z), _dptr(m), _dptr(types), n,
+                                 int(cand_ptype), int(k), _dptr(nbr), _dptr(rho),
+                                 int(rho_mode), _stream()))
+
+
+def dev_density_stage(x, y, z, m, grid, RG, k, den, rho_tsc=None, nbr=None, rho_knn=None):
+    """deposit -> smooth -> read-back -> kNN lists + W4 density, one call."""
+    torch = _torch()
+    n = x.numel()
+    for t, w in ((x, "x"), (y, "y"), (z, "z"), (m, "m")):
+        _check_dev(t, torch.float32, n, w)
+    lib = _lib.load()
+    _lib.check(lib.pgalf_dev_density_stage(_dptr(x), _dptr(y), _dptr(z), _dptr(m), n,
+                                           ctypes.byref(grid), float(RG), int(k), _dptr(den),
+                                           _dptr(rho_tsc), _dptr(nbr), _dptr(rho_knn),
+                                           _stream()))
+
+
+def dev_status():
+    """Synchronise torch's current stream and raise on kernel-detected errors."""
+    lib = _lib.load()
+    _lib.check(lib.pgalf_dev_status(_stream()))
+
+
+def timing(enable=None, reset=False):
+    lib = _lib.load()
+    if enable is not None:
+        lib.pgalf_timing_enable(1 if enable else 0)
+    if reset:
+        lib.pgalf_timing_reset()
+
+
+def timing_get(phase):
+    lib = _lib.load()
+    ms = ctypes.c_double()
+    cnt = ctypes.c_int()
+    lib.pgalf_timing_get(phase.encode(), ctypes.byref(ms), ctypes.byref(cnt))
+    return ms.value, cnt.value
+
+
+__all__ = ["PgalfError", "PARTICLE_DTYPE", "make_particles", "padded_shape",
+           "assign_density_TSC", "gaussian_Smoothing", "findDen", "findsphdensity",
+           "starfindsphdensity", "near_lists", "lag_density_stage", "grid_struct",
+           "dev_tsc", "dev_smooth", "dev_findden", "dev_knn", "dev_density_stage",
+           "dev_status", "timing", "timing_get"]

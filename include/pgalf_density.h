@@ -1,0 +1,150 @@
+/*
+ * pgalf_density.h -- C-ABI of the MI355X (gfx950) density stage of
+ * NewGalFinder (kjhan0606/GalaxyFinder).
+ *
+ * Part 1 exports the reference's own symbol names with the reference's
+ * argument lists, so the galaxy finder links against this library in place
+ * of tsc_omp2.o, gsmooth.o and the kNN density routines of nnost.o.  The
+ * calls are synchronous: inputs are host arrays, outputs are written to the
+ * caller's host buffers before return.  They never exit(); failures are
+ * reported through pgalf_last_error().
+ *
+ * Part 2 is the device-resident API (float32 SoA device pointers, a HIP
+ * stream) used by the Python/PyTorch wrapper and the benchmarks.
+ */
+#ifndef PGALF_DENSITY_H
+#define PGALF_DENSITY_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same layout as the reference's SimpleBasicParticleType (tree.h:155-162,
+ * 88 bytes under the reference Makefile flags): type@0, mass@8, x,y,z@16,24,
+ * 32, vx,vy,vz@40,48,56, link02@64, indx@72, bp@80. */
+typedef struct pgalf_particle {
+    unsigned int type;
+    double mass, x, y, z;
+    double vx, vy, vz;
+    double link02;
+    int indx;
+    struct pgalf_particle *bp;
+} pgalf_particle_t;
+
+/* Particle types (tree.h:47-48). */
+#define PGALF_TYPE_STAR 1
+#define PGALF_TYPE_DM 2
+#define PGALF_TYPE_GAS 3
+#define PGALF_TYPE_SINK 4
+#define PGALF_TYPE_ALL 5
+
+/* Status codes. */
+#define PGALF_OK 0
+#define PGALF_ERR_ARG 1          /* bad argument (sizes, k > 32, k > np ...) */
+#define PGALF_ERR_HIP 2          /* HIP runtime error (no device, launch ...) */
+#define PGALF_ERR_OUT_OF_GRID 3  /* a TSC stencil left the grid; those
+                                    particles were skipped (the reference
+                                    reads/writes out of bounds there) */
+#define PGALF_ERR_NOMEM 4        /* device allocation failed */
+#define PGALF_ERR_SHORT_LIST 5   /* fewer than k candidates (reference: exit(99)) */
+
+/* Status of the last call on this host thread, and its message. */
+int pgalf_last_error(void);
+const char *pgalf_last_error_string(void);
+
+/* ===================== Part 1: reference entry points ===================== */
+
+/* TSC mass deposit onto the padded float grid den[mx*ny*nz], mx=2*(nx/2+1);
+ * particles with type != ptype are skipped unless ptype == PGALF_TYPE_ALL.
+ * Replaces assign_density_TSC (NewGalFinder/tsc_omp2.c:166). */
+void assign_density_TSC(pgalf_particle_t *bp, int np, float *den, int nx, int ny,
+                        int nz, double Xmin, double Ymin, double Zmin,
+                        double cellsize, int ptype);
+
+/* In-place FFT Gaussian smoothing of the padded grid, kernel exp(-r^2/RG^2).
+ * Replaces gaussian_Smoothing (NewGalFinder/gsmooth.c:14). */
+void gaussian_Smoothing(float *denGrid, int nx, int ny, int nz, double cellsize,
+                        float RG);
+
+/* TSC read-back of the grid at every particle (all types) into densph[np].
+ * Replaces findDen (NewGalFinder/tsc_omp2.c:40). */
+void findDen(pgalf_particle_t *bp, int np, float *densph, double xmin, double ymin,
+             double zmin, double cellsize, float *den, int nx, int ny, int nz);
+
+/* Exact Numnear-NN lists (self included, ascending distance, ties by index)
+ * into nearindex[np*Numnear] and the unweighted W4 kNN density densph[np].
+ * Replaces findsphdensity (NewGalFinder/nnost.c:560). */
+void findsphdensity(pgalf_particle_t *bp, int np, long long *nearindex, int Numnear,
+                    float *densph);
+
+/* Mass-weighted W4 density over the 10 nearest stars (NUMNEARDEN,
+ * params.h:73) into densph[np], and all-particle Numnear-NN lists.
+ * Replaces starfindsphdensity (NewGalFinder/nnost.c:632). */
+void starfindsphdensity(pgalf_particle_t *bp, int np, long long *nearindex,
+                        int Numnear, float *densph);
+
+/* The all-particle Numnear-NN lists that lagFindStellarCore builds inline
+ * (NewGalFinder/nnost.c:1298-1536, its Find_Near / warm-started
+ * Find_Near_PreCopied loops).  Returns a status code. */
+int pgalf_near_lists(pgalf_particle_t *bp, int np, int Numnear, long long *nearindex);
+
+/* Whole density stage of lagFindStellarCore up to the neighbour lists
+ * (nnost.c:1030-1096 and 1298-1536) in one device residency: grid box and
+ * size as nnost.c:1040-1058 (TSC_CELL_SIZE 0.004, 10-cell buffer), deposit
+ * of `ptype` particles, smoothing with RG = 0.008, read-back into densph[np]
+ * and the Numnear-NN lists.  The smoothed grid is returned in *grid_out
+ * (malloc'ed, mx*ny*nz floats; free() it) with its geometry, if non-null.
+ * Returns a status code. */
+int pgalf_lag_density_stage(pgalf_particle_t *bp, int np, int Numnear, int ptype,
+                            float *densph, long long *nearindex, float **grid_out,
+                            int *nxyz_out, double *xyzmin_out);
+
+/* ===================== Part 2: device-resident API ======================== */
+
+typedef struct pgalf_grid {
+    int nx, ny, nz;
+    double xmin, ymin, zmin, cellsize;
+} pgalf_grid_t;
+
+/* All pointers are device pointers; `stream` is a hipStream_t (NULL for the
+ * default stream).  Calls are stream-ordered; their status is that of the
+ * launch.  Kernel-detected conditions (out-of-grid particles, short lists)
+ * are collected with pgalf_dev_status(). */
+int pgalf_dev_tsc(const float *x, const float *y, const float *z, const float *m,
+                  const int *type, long long np, int ptype, const pgalf_grid_t *g,
+                  float *den, void *stream);
+int pgalf_dev_smooth(float *den, const pgalf_grid_t *g, float RG, void *stream);
+int pgalf_dev_findden(const float *x, const float *y, const float *z, long long np,
+                      const pgalf_grid_t *g, const float *den, float *densph,
+                      void *stream);
+/* k <= 32 nearest of each particle among the particles whose type equals
+ * cand_ptype (PGALF_TYPE_ALL or type == NULL: all).  rho_mode 0: none,
+ * 1: unweighted W4 density, 2: mass-weighted.  nbr/rho may be NULL. */
+int pgalf_dev_knn(const float *x, const float *y, const float *z, const float *m,
+                  const int *type, long long np, int cand_ptype, int k,
+                  long long *nbr, float *rho, int rho_mode, void *stream);
+/* deposit -> smooth -> read-back -> kNN lists + W4 density, one call. */
+int pgalf_dev_density_stage(const float *x, const float *y, const float *z,
+                            const float *m, long long np, const pgalf_grid_t *g,
+                            float RG, int k, float *den, float *rho_tsc,
+                            long long *nbr, float *rho_knn, void *stream);
+/* Synchronise `stream` and return the status of the kernels run since the
+ * previous call (PGALF_ERR_OUT_OF_GRID / PGALF_ERR_SHORT_LIST or PGALF_OK). */
+int pgalf_dev_status(void *stream);
+
+/* Phase timing with HIP events on the launch stream (off by default). */
+void pgalf_timing_enable(int on);
+void pgalf_timing_reset(void);
+/* Sum of elapsed ms and number of intervals recorded for `phase`
+ * ("tsc", "smooth", "findden", "knn", "knn_query", ...). */
+int pgalf_timing_get(const char *phase, double *total_ms, int *count);
+
+/* Library version string. */
+const char *pgalf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PGALF_DENSITY_H */

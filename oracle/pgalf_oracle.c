@@ -156,7 +156,9 @@ long long orc_tsc_deposit_f32(const double *x, const double *y, const double *z,
                 id[a][0] = w.near[a];
                 id[a][1] = w.near[a] + w.sign[a];
                 id[a][2] = w.near[a] - w.sign[a];
-                long long n = a == 0 ? nx : (a == 1 ? ny : nz);
+                /* x may reach the pad columns of the padded row, as the
+                 * reference's unchecked stores do */
+                long long n = a == 0 ? mx : (a == 1 ? ny : nz);
                 for (int q = 0; q < 3; q++)
                     if (id[a][q] < 0 || id[a][q] >= n) ok = 0;
             }
@@ -201,7 +203,7 @@ long long orc_tsc_deposit_f64(const double *x, const double *y, const double *z,
             id[a][0] = w.near[a];
             id[a][1] = w.near[a] + w.sign[a];
             id[a][2] = w.near[a] - w.sign[a];
-            long long n = a == 0 ? nx : (a == 1 ? ny : nz);
+            long long n = a == 0 ? mx : (a == 1 ? ny : nz);
             for (int q = 0; q < 3; q++)
                 if (id[a][q] < 0 || id[a][q] >= n) ok = 0;
         }
@@ -253,7 +255,7 @@ long long orc_findden(const double *x, const double *y, const double *z,
         int ok = 1;
         for (int a = 0; a < 3; a++) {
             id[a][0] = nr[a]; id[a][1] = nr[a] + sg[a]; id[a][2] = nr[a] - sg[a];
-            long n = a == 0 ? nx : (a == 1 ? ny : nz);
+            long n = a == 0 ? mx : (a == 1 ? ny : nz);
             for (int q = 0; q < 3; q++)
                 if (id[a][q] < 0 || id[a][q] >= n) ok = 0;
         }
@@ -533,14 +535,39 @@ typedef struct {
     int nnodes, cap;
 } orc_kdtree;
 
-static int orc_kd_cmp_axis;
-static const double *orc_kd_cmp_arr;
-static int orc_kd_cmp(const void *a, const void *b) {
-    long long ia = *(const long long *)a, ib = *(const long long *)b;
-    double va = orc_kd_cmp_arr[ia], vb = orc_kd_cmp_arr[ib];
-    if (va < vb) return -1;
-    if (va > vb) return 1;
-    return ia < ib ? -1 : (ia > ib);
+/* (coordinate, index) order used to split k-d nodes */
+static inline int orc_kd_less(const double *c, long long a, long long b) {
+    return c[a] < c[b] || (c[a] == c[b] && a < b);
+}
+
+/* Quickselect: place the m-th element of perm[b,e) (in (coordinate, index)
+ * order) at position m, smaller ones before it, larger ones after. */
+static void orc_kd_select(long long *perm, long long b, long long e, long long m,
+                          const double *c) {
+    while (e - b > 1) {
+        long long mid = b + (e - b) / 2;
+        /* median of three as pivot */
+        long long i0 = perm[b], i1 = perm[mid], i2 = perm[e - 1], piv;
+        if (orc_kd_less(c, i0, i1)) {
+            if (orc_kd_less(c, i1, i2)) piv = i1;
+            else piv = orc_kd_less(c, i0, i2) ? i2 : i0;
+        } else {
+            if (orc_kd_less(c, i0, i2)) piv = i0;
+            else piv = orc_kd_less(c, i1, i2) ? i2 : i1;
+        }
+        long long lo = b, hi = e - 1;
+        while (lo <= hi) {
+            while (orc_kd_less(c, perm[lo], piv)) lo++;
+            while (orc_kd_less(c, piv, perm[hi])) hi--;
+            if (lo <= hi) {
+                long long t = perm[lo]; perm[lo] = perm[hi]; perm[hi] = t;
+                lo++; hi--;
+            }
+        }
+        if (m <= hi) e = hi + 1;
+        else if (m >= lo) b = lo;
+        else return;
+    }
 }
 
 static int orc_kd_build(orc_kdtree *t, long long b, long long e) {
@@ -565,10 +592,8 @@ static int orc_kd_build(orc_kdtree *t, long long b, long long e) {
         double wmax = -1;
         for (int a = 0; a < 3; a++)
             if (nd.hi[a] - nd.lo[a] > wmax) { wmax = nd.hi[a] - nd.lo[a]; ax = a; }
-        orc_kd_cmp_axis = ax;
-        orc_kd_cmp_arr = t->c[ax];
-        qsort(t->perm + b, (size_t)(e - b), sizeof(long long), orc_kd_cmp);
         long long m = b + (e - b) / 2;
+        orc_kd_select(t->perm, b, e, m, t->c[ax]);
         t->nodes[id] = nd;
         int l = orc_kd_build(t, b, m);
         int r = orc_kd_build(t, m, e);

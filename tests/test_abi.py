@@ -1,0 +1,82 @@
+"""CPU checks of the C-ABI boundary and the host-side logic (no GPU calls):
+the library loads, exports every function include/pgalf_density.h declares,
+the particle record matches the reference's 88-byte layout, and the Python
+wrappers reject malformed arguments before touching the device."""
+import ctypes
+import os
+import re
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+from galaxyfinder_amd import _lib, density as D, halos  # noqa: E402
+
+HEADER = os.path.join(ROOT, "include", "pgalf_density.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text)
+    skip = {"if", "sizeof", "defined"}
+    return sorted({n for n in names if n not in skip and not n.isupper()})
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_functions()
+    assert "assign_density_TSC" in names and "gaussian_Smoothing" in names
+    assert len(names) >= 19
+    for n in names:
+        assert hasattr(lib, n), "missing export %s" % n
+    assert set(names) == set(_lib.SIGNATURES), "ctypes signatures out of date"
+
+
+def test_version_and_error_state_without_gpu_calls():
+    lib = _lib.load()
+    assert b"gfx950" in lib.pgalf_version()
+    assert lib.pgalf_last_error() == 0
+
+
+def test_particle_record_layout():
+    assert D.PARTICLE_DTYPE.itemsize == 88
+    off = {k: v[1] for k, v in D.PARTICLE_DTYPE.fields.items()}
+    assert off["type"] == 0 and off["mass"] == 8 and off["x"] == 16
+    assert off["y"] == 24 and off["z"] == 32 and off["indx"] == 72 and off["bp"] == 80
+    bp = D.make_particles(np.array([[0.1, 0.2, 0.3]]), np.array([2.0]), np.array([2]))
+    raw = bp.tobytes()
+    assert np.frombuffer(raw[16:40], dtype=np.float64).tolist() == [0.1, 0.2, 0.3]
+    assert np.frombuffer(raw[8:16], dtype=np.float64)[0] == 2.0
+
+
+def test_padded_shape():
+    assert D.padded_shape(37, 35, 32) == (32, 35, 38)
+    assert D.padded_shape(64, 64, 64) == (64, 64, 66)
+
+
+def test_argument_checks_raise_before_device():
+    bp = D.make_particles(np.zeros((4, 3)))
+    with pytest.raises(TypeError):
+        D.assign_density_TSC(bp, 4, np.zeros(10, dtype=np.float64), 4, 4, 4, 0, 0, 0, 1.0, 5)
+    with pytest.raises(ValueError):
+        D.findDen(bp, 4, np.zeros(4, dtype=np.float32), 0, 0, 0, 1.0,
+                  np.zeros(5, dtype=np.float32), 4, 4, 4)
+    with pytest.raises(TypeError):
+        D.findsphdensity(bp, 4, np.zeros(4 * 3, dtype=np.int32), 3,
+                         np.zeros(4, dtype=np.float32))
+
+
+def test_halo_generator_seeded_and_inside_box():
+    a, ma = halos.small_halo(5000, seed=1)
+    b, mb = halos.small_halo(5000, seed=1)
+    assert np.array_equal(a, b) and np.array_equal(ma, mb)
+    assert a.dtype == np.float32 and ma.dtype == np.float32
+    assert a.min() >= 0.1 and a.max() <= 0.9
+    p, m, spec = halos.single_halo_config(1)
+    assert len(p) == 131072 and spec["nx"] == 64 and spec["k"] == 32
+    assert p.min() >= 0.1 - 1e-6 and p.max() <= 0.9 + 1e-6
+    assert abs(m.astype(np.float64).sum() - 1.0) < 1e-4

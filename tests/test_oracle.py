@@ -1,0 +1,291 @@
+"""CPU tests that pin the oracle: against the reference's golden fixtures
+(written from the reference sources by tests/golden/make_golden.py), against
+the reference build itself when oracle/_ref is present, and against closed
+forms and invariants that a plausible mistake (a dropped term, a wrong sign,
+a transposed index) would break."""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+from oracle import oracle as O  # noqa: E402
+from galaxyfinder_amd import halos  # noqa: E402
+
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def _grid(f):
+    nx, ny, nz, x0, y0, z0, h = f["grid"]
+    return int(nx), int(ny), int(nz), float(x0), float(y0), float(z0), float(h)
+
+
+# ---- deposit ---------------------------------------------------------------
+@pytest.mark.parametrize("ptype", [5, 1])
+@pytest.mark.parametrize("nth", [1, 4])
+def test_tsc_bitexact_vs_reference_fixture(ptype, nth):
+    f = np.load(os.path.join(GOLD, "tsc_small.npz"))
+    den, bad = O.tsc_deposit(f["pos"], f["mass"], _grid(f), ptype=ptype, types=f["types"],
+                             nthreads=nth)
+    assert bad == 0
+    ref = f["den_p%d_t%d" % (ptype, nth)]
+    assert np.array_equal(den.view(np.int32), ref.view(np.int32))
+
+
+def test_tsc_single_particle_weights():
+    """One unit-mass particle at offsets d from its node: weights are the TSC
+    B-spline 0.75-d^2, (0.5+-d)^2/2, exact in each axis, and the x.5 tie picks
+    the even node (rint, round half even)."""
+    h = 1.0
+    for d in (0.0, 0.25, -0.25, 0.49, -0.49, 0.5):
+        pos = np.array([[8.0 + d, 8.0, 8.0]])
+        den, _ = O.tsc_deposit(pos, np.ones(1), (16, 16, 16, 0.0, 0.0, 0.0, h), fp64=True)
+        den = den * 64.0  # remove volfact
+        near = int(np.rint(8.0 + d))
+        dd = abs(8.0 + d - near)
+        w = {near: 0.75 - dd * dd}
+        s = 1 if (8.0 + d - near) >= 0 else -1
+        w[near + s] = 0.5 * (0.5 + dd) ** 2
+        w[near - s] = 0.5 * (0.5 - dd) ** 2
+        for i, wi in w.items():
+            assert abs(den[8, 8, i] - 0.75 * 0.75 * wi) < 1e-7
+        assert abs(den.sum() - 1.0) < 1e-7
+    # half-way point 8.5 goes to node 8 (even)
+    den, _ = O.tsc_deposit(np.array([[8.5, 8.0, 8.0]]), np.ones(1),
+                           (16, 16, 16, 0.0, 0.0, 0.0, 1.0), fp64=True)
+    assert den[8, 8, 8] * 64 == pytest.approx(0.75 * 0.5 * 0.75, abs=1e-7)
+
+
+def test_tsc_mass_conservation_and_first_moment():
+    pos, mass = halos.small_halo(5000, seed=2)
+    g = (40, 40, 40, 0.0, 0.0, 0.0, 1.0 / 40)
+    den, bad = O.tsc_deposit(pos, mass, g, fp64=True)
+    assert bad == 0
+    vol = (0.001 / 0.004) ** 3
+    assert den.sum() == pytest.approx(vol * mass.astype(np.float64).sum(), rel=1e-6)
+    # TSC reproduces linear functions: sum_i w_i * i = xp per particle
+    xs = np.arange(den.shape[2])
+    mom = (den.sum(axis=(0, 1)) * xs).sum() / vol
+    ex = (mass.astype(np.float64) * (pos[:, 0].astype(np.float64) / (1.0 / 40))).sum()
+    assert mom == pytest.approx(ex, rel=1e-5)
+
+
+def test_tsc_fp32_close_to_fp64():
+    f = np.load(os.path.join(GOLD, "tsc_small.npz"))
+    d32, _ = O.tsc_deposit(f["pos"], f["mass"], _grid(f), nthreads=1)
+    d64, _ = O.tsc_deposit(f["pos"], f["mass"], _grid(f), fp64=True)
+    assert np.abs(d32 - d64).max() <= 1e-6 * np.abs(d64).max()
+
+
+# ---- read-back ---------------------------------------------------------------
+def test_findden_bitexact_vs_reference_fixture():
+    f = np.load(os.path.join(GOLD, "tsc_small.npz"))
+    out, bad = O.findden(f["pos"], _grid(f), f["den_p5_t1"])
+    assert bad == 0
+    assert np.array_equal(out.view(np.int32), f["densph_p5"].view(np.int32))
+
+
+def test_findden_constant_and_linear_fields():
+    pos, _ = halos.small_halo(2000, seed=4)
+    nx = ny = nz = 32
+    g = (nx, ny, nz, 0.0, 0.0, 0.0, 1.0 / 32)
+    den = np.zeros(O.padded_shape(nx, ny, nz), dtype=np.float32)
+    den[..., :nx] = 3.0
+    out, _ = O.findden(pos, g, den)
+    assert np.allclose(out, 3.0, rtol=1e-6)
+    den[..., :nx] = np.arange(nx, dtype=np.float32)[None, None, :]
+    out, _ = O.findden(pos, g, den)
+    assert np.allclose(out, pos[:, 0] * 32, rtol=1e-5, atol=1e-4)
+
+
+def test_deposit_readback_adjoint():
+    """<D m, g> = <m, D^T g>: deposit and read-back use the same weights."""
+    rng = np.random.default_rng(11)
+    pos, _ = halos.small_halo(3000, seed=6)
+    m = rng.random(3000)
+    g = (32, 32, 32, 0.0, 0.0, 0.0, 1.0 / 32)
+    den, _ = O.tsc_deposit(pos, m, g, fp64=True)
+    field = np.zeros(O.padded_shape(32, 32, 32), dtype=np.float32)
+    field[..., :32] = rng.random((32, 32, 32)).astype(np.float32)
+    rb, _ = O.findden(pos, g, field)
+    lhs = (den.astype(np.float64) * field).sum() / ((0.001 / 0.004) ** 3)
+    rhs = (m * rb.astype(np.float64)).sum()
+    assert lhs == pytest.approx(rhs, rel=1e-6)
+
+
+# ---- smoothing ---------------------------------------------------------------
+@pytest.mark.parametrize("n", [(16, 16, 16), (18, 15, 12), (24, 20, 17)])
+def test_gsmooth_plane_wave_eigen(n):
+    """cos(2 pi k.x/n) is an eigenvector with eigenvalue prod_d f_d(k_d)."""
+    nx, ny, nz = n
+    RG, h = 1.7, 1.0
+    expfac = O.gsmooth_expfac(h, RG)
+    kx, ky, kz = 3, 2, 1
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    wave = np.cos(2 * np.pi * (kx * x / nx + ky * y / ny + kz * z / nz))
+    out = O.gsmooth_f64(wave, h, RG)
+    lam = (math.exp(kx * kx * expfac / (nx * nx)) * math.exp(ky * ky * expfac / (ny * ny)) *
+           math.exp(kz * kz * expfac / (nz * nz)))
+    assert np.abs(out - lam * wave).max() < 1e-12
+
+
+def test_gsmooth_dc_and_delta_closed_form():
+    """The delta of NewGalFinder/test.c (32^3, RG = 2 cells): the response is
+    the product of 1-D periodic kernels written as finite cosine sums, and
+    the total is preserved (DC bin untouched)."""
+    n = 32
+    g = np.zeros((n, n, n))
+    g[16, 16, 16] = 1.0
+    out = O.gsmooth_f64(g, 1.0, 2.0)
+    k1 = O.gsmooth_kernel1d(n, 1.0, 2.0)
+    idx = (np.arange(n) - 16) % n
+    ex = k1[idx][:, None, None] * k1[idx][None, :, None] * k1[idx][None, None, :]
+    assert np.abs(out - ex).max() < 1e-14
+    assert out.sum() == pytest.approx(1.0, abs=1e-12)
+    # sigma_eff = RG/sqrt(2): second moment of the 1-D kernel ~ RG^2/2
+    m = np.arange(n) - 16
+    assert (k1[idx] * m * m).sum() == pytest.approx(2.0, rel=1e-4)  # band-limit aliasing ~1e-5
+
+
+def test_gsmooth_padded_matches_unpadded():
+    rng = np.random.default_rng(3)
+    nx, ny, nz = 21, 17, 24
+    g = rng.random((nz, ny, nx))
+    pad = np.zeros(O.padded_shape(nx, ny, nz), dtype=np.float32)
+    pad[..., :nx] = g.astype(np.float32)
+    out = O.gsmooth(pad, nx, ny, nz, 0.5, 0.9)
+    ref = O.gsmooth_f64(g.astype(np.float32).astype(np.float64), 0.5, 0.9)
+    assert np.abs(out[..., :nx] - ref).max() <= 1e-6 * np.abs(ref).max()
+    assert np.all(out[..., nx:] == 0)
+
+
+# ---- kNN -----------------------------------------------------------------------
+def test_knn_vs_reference_fixture():
+    f = np.load(os.path.join(GOLD, "knn_small.npz"))
+    idx, d2 = O.knn(f["pos"], int(f["k"]))
+    assert np.array_equal(idx, f["idx_cold"].astype(np.int64))
+    assert np.array_equal(idx, f["idx_warm"].astype(np.int64))
+    assert np.all(idx[:, 0] == np.arange(len(idx)))  # self first
+    assert np.all(np.diff(d2, axis=1) >= 0)
+
+
+def test_knn_kdtree_equals_brute():
+    pos, _ = halos.small_halo(3000, seed=9)
+    a, da = O.knn(pos, 32, method="brute")
+    b, db = O.knn(pos, 32, method="kdtree")
+    assert np.array_equal(a, b) and np.array_equal(da, db)
+    # subset candidates (stars)
+    c = pos[::3]
+    a, da = O.knn(pos, 10, cpos=c, method="brute")
+    b, db = O.knn(pos, 10, cpos=c, method="kdtree")
+    assert np.array_equal(a, b) and np.array_equal(da, db)
+
+
+def test_knn_lattice_distances_match_reference():
+    f = np.load(os.path.join(GOLD, "knn_lattice.npz"))
+    idx, d2 = O.knn(f["pos"], int(f["k"]), method="brute")
+    assert np.array_equal(np.sort(f["d2"], axis=1), d2)
+    # ties ordered by index in the oracle
+    for i in range(len(idx)):
+        for j in range(1, idx.shape[1]):
+            if d2[i, j] == d2[i, j - 1]:
+                assert idx[i, j] > idx[i, j - 1]
+
+
+def test_knn_distance_is_order_statistic():
+    pos, _ = halos.small_halo(1500, seed=10)
+    idx, d2 = O.knn(pos, 16)
+    p = pos.astype(np.float64)
+    for i in (0, 77, 1499):
+        full = ((p[i] - p) ** 2).sum(axis=1)
+        assert d2[i, -1] == pytest.approx(np.sort(full)[15], rel=1e-15)
+
+
+# ---- W4 --------------------------------------------------------------------------
+def test_w4_table_vs_reference():
+    f = np.load(os.path.join(GOLD, "w4_table.npz"))
+    dist, h, w4 = f["dist"], f["h"], f["w4"]
+    for i in range(dist.shape[0]):
+        for j in range(dist.shape[1]):
+            assert np.float32(O.W4(float(dist[i, j]), float(h[j]))) == w4[i, j]
+
+
+def test_w4_closed_forms():
+    for h in (0.5, 1.0, 2.5):
+        assert O.W4(0.0, h) == pytest.approx(1.0 / (math.pi * h ** 3), rel=1e-6)
+        assert O.W4(2.0 * h, h) == 0.0
+        # normalisation: int W d^3r = 1
+        r = np.linspace(0, 2 * h, 20001)
+        w = np.array([O.W4(float(v), h) for v in r[::20]])
+        integ = np.trapezoid(4 * np.pi * r[::20] ** 2 * w, r[::20])
+        assert integ == pytest.approx(1.0, rel=2e-4)
+
+
+def test_sph_density_uniform_lattice():
+    """On a cubic lattice the W4 kNN density matches the closed-form sum over
+    the lattice shells inside the k-th neighbour radius."""
+    g = np.arange(12) * 1.0
+    lat = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
+    idx, d2 = O.knn(lat, 27, method="brute")
+    rho = O.sph_density(idx, d2)
+    i = np.argmin(((lat - 5.0) ** 2).sum(axis=1))  # interior point
+    # 27-NN of an interior lattice point: self, 6 at 1, 12 at sqrt2, 8 at sqrt3
+    h = math.sqrt(3.0) / 2
+    ex = (O.W4(0.0, h) + 6 * O.W4(1.0, h) + 12 * O.W4(math.sqrt(2.0), h) +
+          8 * O.W4(math.sqrt(3.0), h))
+    assert rho[i] == pytest.approx(ex, rel=1e-6)
+
+
+# ---- against the reference build (built here only) ----------------------------
+ref = O.ref_lib()
+
+
+@pytest.mark.skipif(ref is None, reason="reference build oracle/_ref absent")
+def test_oracle_vs_reference_build_random_halo():
+    R = ref
+    R.ref_init(4 << 30)
+    pos, mass = halos.small_halo(6000, seed=12)
+    pos = pos.astype(np.float64) + 1e-9 * np.arange(6000)[:, None]  # not float-representable
+    m = mass.astype(np.float64) * 3.0
+    types = (np.arange(6000) % 5 == 0).astype(np.int32) + 1
+    g = (41, 39, 44, -0.03, -0.02, -0.04, 0.025)
+    x, y, z = (np.ascontiguousarray(pos[:, a]) for a in range(3))
+    for nth in (1, 3, 8):
+        den = np.zeros(O.padded_shape(*g[:3]), dtype=np.float32)
+        R.ref_tsc(O._p(x, O._dp), O._p(y, O._dp), O._p(z, O._dp), O._p(m, O._dp),
+                  O._p(types, O._ip), 6000, O._p(den, O._fp), *g, 1, nth)
+        mine, _ = O.tsc_deposit(pos, m, g, ptype=1, types=types, nthreads=nth)
+        assert np.array_equal(den.view(np.int32), mine.view(np.int32))
+    rb = np.zeros(6000, dtype=np.float32)
+    R.ref_findden(O._p(x, O._dp), O._p(y, O._dp), O._p(z, O._dp), 6000, O._p(rb, O._fp),
+                  *g[3:], O._p(den, O._fp), *g[:3], 4)
+    mine, _ = O.findden(pos, g, den)
+    assert np.array_equal(rb.view(np.int32), mine.view(np.int32))
+    idx = np.zeros((6000, 32), dtype=np.int64)
+    d2 = np.zeros((6000, 32))
+    assert R.ref_knn_precopied(O._p(x, O._dp), O._p(y, O._dp), O._p(z, O._dp), 6000, 32,
+                               O._p(idx, O._lp), O._p(d2, O._dp), 4) == 0
+    oi, od = O.knn(pos, 32)
+    assert np.array_equal(idx, oi) and np.array_equal(d2, od)
+
+
+@pytest.mark.skipif(ref is None, reason="reference build oracle/_ref absent")
+def test_reference_find_near_is_defective():
+    """The reference findsphdensity (Find_Near, nnost.c:375) does not return
+    the k nearest neighbours; the parity target is the exact kNN."""
+    R = ref
+    R.ref_init(4 << 30)
+    pos, mass = halos.small_halo(600, seed=13)
+    p = pos.astype(np.float64)
+    x, y, z = (np.ascontiguousarray(p[:, a]) for a in range(3))
+    idx = np.zeros((600, 32), dtype=np.int64)
+    dens = np.zeros(600, dtype=np.float32)
+    m = mass.astype(np.float64)
+    R.ref_findsphdensity(O._p(x, O._dp), O._p(y, O._dp), O._p(z, O._dp), O._p(m, O._dp),
+                         600, 32, O._p(idx, O._lp), O._p(dens, O._fp), 1)
+    exact, _ = O.knn(p, 32)
+    assert (idx == exact).all(axis=1).sum() < 600
