@@ -8,18 +8,22 @@
 // to the k-th neighbour (nnost.c:536-555, 607-623, 689-704).
 //
 // Structure:
-//   * queries and candidates are sorted by 30-bit Morton code (stable radix
-//     sort, ties by index) so a 64-particle bucket is spatially compact;
-//   * candidate buckets get FP64 bounding boxes and an implicit binary tree
-//     of boxes over consecutive buckets;
-//   * one wavefront serves one query bucket, one query per lane.  The wave
-//     walks the box tree near-first with a wave-uniform stack, pruning nodes
-//     farther (box to box) than the largest current k-th distance of its
-//     lanes; each surviving leaf bucket is staged in LDS and every lane scans
-//     its 64 candidates.  Accepted candidates go to a per-lane LDS buffer;
-//     when any lane's buffer may overflow, all lanes merge it into their
-//     sorted top-32 list held in registers (bitonic sort of 16 + bitonic
-//     merge), tightening their k-th distance.
+//   * queries and candidates are sorted by their 30-bit Hilbert index (10
+//     bits per axis over the bounding cube; stable radix sort, ties by
+//     index).  Hilbert order has no jumps, so 16 consecutive particles form a
+//     compact bucket;
+//   * candidate buckets of 16 get FP64 bounding boxes and an implicit 8-ary
+//     tree of boxes (8 consecutive nodes per parent);
+//   * a wavefront holds four independent query groups of 16 lanes (16
+//     consecutive sorted queries, one per lane).  Each group first descends
+//     the tree greedily to the bucket nearest its box and scans that bucket
+//     and its two neighbours either side, which gives every lane a k-th
+//     distance; it then walks the tree breadth first, all 16 lanes testing
+//     children in parallel (two parents x eight children per round) against
+//     the group's largest k-th distance, and finally scans the surviving
+//     buckets.  Accepted candidates go to a per-lane LDS buffer; when any
+//     lane's buffer may overflow, all lanes merge it into their sorted top-32
+//     list held in registers (bitonic sort of 16 + bitonic merge).
 //   * keys are (FP64 dist2, uint32 index) compared lexicographically, so the
 //     lists are exact and deterministic regardless of visiting order.
 // Box pruning is conservative: box gaps are formed with the same monotone
@@ -36,22 +40,27 @@ constexpr int kWave = 64;
 constexpr int kTop = 32;     // list capacity (k <= 32, MAX_NUM_NEAR tree.h:20)
 constexpr int kBuf = 16;     // per-lane LDS buffer
 constexpr int kChunkC = 8;   // candidates between overflow checks
-constexpr int kMaxLevels = 32;
-constexpr int kStack = 96;
+constexpr int kMaxLevels = 24;
+constexpr int kB = 16;       // candidates per bucket
+constexpr int kG = 16;       // lanes per query group
+constexpr int kGroups = kWave / kG;
+constexpr int kFan = 8;      // box-tree fan-out
+constexpr int kFCap = 256;   // frontier capacity per group
 
 struct Box {
     double lo[3], hi[3];
 };
 
-// Implicit box tree over nb leaf buckets: level l holds ceil(nb / 2^l)
-// nodes, stored after all lower levels.
+// Implicit 8-ary box tree over nb leaf buckets: level l holds
+// ceil(nb / 8^l) nodes, stored after all lower levels.
 struct Levels {
     int nlev;
     long long nb;
 };
 
 __host__ __device__ inline long long level_count(long long nb, int l) {
-    return (nb + (1LL << l) - 1) >> l;
+    long long d = 1LL << (3 * l);
+    return (nb + d - 1) / d;
 }
 __host__ __device__ inline long long level_offset(long long nb, int l) {
     long long o = 0;
@@ -141,9 +150,11 @@ __device__ inline double unord_d(unsigned long long o) {
     return __longlong_as_double((long long)b);
 }
 
+// Bounding box: per-block reduction, then one atomic per block and value.
 template <class P>
-__global__ void bbox_k(const P *x, const P *y, const P *z, long long np,
-                       unsigned long long *mm /* 3 min, 3 max (ordered) */) {
+__global__ void __launch_bounds__(256) bbox_k(const P *x, const P *y, const P *z, long long np,
+                                              unsigned long long *mm /* 3 min, 3 max */) {
+    __shared__ double s_v[6][4];
     double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < np;
          i += (long long)gridDim.x * blockDim.x) {
@@ -154,6 +165,7 @@ __global__ void bbox_k(const P *x, const P *y, const P *z, long long np,
             hi[a] = fmax(hi[a], v[a]);
         }
     }
+    const int w = threadIdx.x >> 6;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         lo[a] = wave_min_d(lo[a]);
@@ -162,25 +174,60 @@ __global__ void bbox_k(const P *x, const P *y, const P *z, long long np,
     if ((threadIdx.x & 63) == 0) {
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            atomicMin(&mm[a], ord_d(lo[a]));
-            atomicMax(&mm[3 + a], ord_d(hi[a]));
+            s_v[a][w] = lo[a];
+            s_v[3 + a][w] = hi[a];
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        int a = threadIdx.x;
+        double v = s_v[a][0];
+        for (int i = 1; i < (int)(blockDim.x >> 6); i++)
+            v = a < 3 ? fmin(v, s_v[a][i]) : fmax(v, s_v[a][i]);
+        if (a < 3) atomicMin(&mm[a], ord_d(v));
+        else atomicMax(&mm[a], ord_d(v));
     }
 }
 
-__device__ inline unsigned spread3(unsigned v) {
-    v &= 0x3ffu;
-    v = (v | (v << 16)) & 0x030000ffu;
-    v = (v | (v << 8)) & 0x0300f00fu;
-    v = (v | (v << 4)) & 0x030c30c3u;
-    v = (v | (v << 2)) & 0x09249249u;
-    return v;
+// 3-D Hilbert index of 10-bit coordinates (Skilling's axes-to-transpose,
+// then bit interleave with x most significant).
+__device__ inline unsigned hilbert3(unsigned x, unsigned y, unsigned z) {
+    unsigned X[3] = {x, y, z};
+#pragma unroll
+    for (unsigned Q = 1u << 9; Q > 1; Q >>= 1) {
+        const unsigned P = Q - 1;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            if (X[i] & Q) {
+                X[0] ^= P;
+            } else {
+                unsigned t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    X[1] ^= X[0];
+    X[2] ^= X[1];
+    unsigned t = 0;
+#pragma unroll
+    for (unsigned Q = 1u << 9; Q > 1; Q >>= 1)
+        if (X[2] & Q) t ^= Q - 1;
+    X[0] ^= t;
+    X[1] ^= t;
+    X[2] ^= t;
+    unsigned key = 0;
+#pragma unroll
+    for (int b = 9; b >= 0; b--)
+        key = (key << 3) | (((X[0] >> b) & 1u) << 2) | (((X[1] >> b) & 1u) << 1) |
+              ((X[2] >> b) & 1u);
+    return key;
 }
 
-// Morton keys of particles idx[0..n) (idx == nullptr: identity).
+// Hilbert keys of particles idx[0..n) (idx == nullptr: identity).
 template <class P>
-__global__ void morton_k(const P *x, const P *y, const P *z, const unsigned *idx, long long n,
-                         const unsigned long long *mm, unsigned *keys, unsigned *vals) {
+__global__ void hilbert_k(const P *x, const P *y, const P *z, const unsigned *idx, long long n,
+                          const unsigned long long *mm, unsigned *keys, unsigned *vals) {
     long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     double lo[3], ext = 0.0;
@@ -199,7 +246,7 @@ __global__ void morton_k(const P *x, const P *y, const P *z, const unsigned *idx
         int qi = (int)t;
         q[a] = (unsigned)(qi < 0 ? 0 : (qi > 1023 ? 1023 : qi));
     }
-    keys[s] = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
+    keys[s] = hilbert3(q[0], q[1], q[2]);
     vals[s] = i;
 }
 
@@ -215,28 +262,34 @@ __global__ void gather_sorted_k(const P *x, const P *y, const P *z, const unsign
 }
 
 __global__ void leaf_boxes_k(const double *sx, const double *sy, const double *sz, long long n,
-                             Box *boxes) {
-    long long b = blockIdx.x;
-    long long i = b * kWave + threadIdx.x;
-    bool v = i < n;
-    double c[3] = {v ? sx[i] : DBL_MAX, v ? sy[i] : DBL_MAX, v ? sz[i] : DBL_MAX};
-    double d[3] = {v ? sx[i] : -DBL_MAX, v ? sy[i] : -DBL_MAX, v ? sz[i] : -DBL_MAX};
+                             long long nb, Box *boxes) {
+    long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
     Box bx;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        bx.lo[a] = wave_min_d(c[a]);
-        bx.hi[a] = wave_max_d(d[a]);
+        bx.lo[a] = DBL_MAX;
+        bx.hi[a] = -DBL_MAX;
     }
-    if (threadIdx.x == 0) boxes[b] = bx;
+    long long e = min(n, (b + 1) * kB);
+    for (long long i = b * kB; i < e; i++) {
+        double v[3] = {sx[i], sy[i], sz[i]};
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            bx.lo[a] = fmin(bx.lo[a], v[a]);
+            bx.hi[a] = fmax(bx.hi[a], v[a]);
+        }
+    }
+    boxes[b] = bx;
 }
 
 __global__ void parent_boxes_k(const Box *child, long long nchild, Box *parent,
                                long long nparent) {
     long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nparent) return;
-    Box a = child[2 * j];
-    if (2 * j + 1 < nchild) {
-        Box b = child[2 * j + 1];
+    Box a = child[kFan * j];
+    for (long long c = kFan * j + 1; c < min(nchild, kFan * (j + 1)); c++) {
+        Box b = child[c];
 #pragma unroll
         for (int t = 0; t < 3; t++) {
             a.lo[t] = fmin(a.lo[t], b.lo[t]);
@@ -255,6 +308,22 @@ __global__ void select_compact_k(const unsigned *flags, const unsigned *pos, lon
                                  unsigned *out) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < np && flags[i]) out[pos[i]] = (unsigned)i;
+}
+
+__device__ inline double grp_min_d(double v) {
+#pragma unroll
+    for (int o = kG / 2; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline double grp_max_d(double v) {
+#pragma unroll
+    for (int o = kG / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline int wave_max_i(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
 }
 
 // Bitonic networks written as compile-time recursions over split (dist,
@@ -336,20 +405,73 @@ __device__ inline void flush_buffer(double (&td)[kTop], unsigned (&ti)[kTop], in
     merge_top(td, ti, bd, bi);
 }
 
+// Per-lane query state of the kNN kernel.
+struct QState {
+    double td[kTop];
+    unsigned ti[kTop];
+    Key kth;
+    int cnt;
+};
+
+// Scan bucket b (one per group; has == false: this group skips) for every
+// lane whose point-to-box gap does not exceed its k-th distance.  Wave
+// uniform: all lanes call it together; merges are wave-wide.
+__device__ inline void scan_bucket(QState &st, long long b, bool has, bool valid, int k,
+                                   double px, double py, double pz, const double *cx,
+                                   const double *cy, const double *cz, const unsigned *cid,
+                                   long long nc, const Box *boxes, double (*s_bd)[kWave],
+                                   unsigned (*s_bi)[kWave], int lane) {
+    bool need = false;
+    int ncand = 0;
+    long long base = b * kB;
+    if (has) {
+        Box bx = boxes[b];
+        double g0 = gap1(bx.lo[0], bx.hi[0], px, px);
+        double g1 = gap1(bx.lo[1], bx.hi[1], py, py);
+        double g2 = gap1(bx.lo[2], bx.hi[2], pz, pz);
+        double a = g0 * g0;
+        double bb = g1 * g1;
+        double c = g2 * g2;
+        need = valid && !(((a + bb) + c) > st.kth.d);
+        ncand = (int)min((long long)kB, nc - base);
+    }
+    if (__ballot(need) == 0ull) return;
+#pragma unroll
+    for (int h = 0; h < kB; h += kChunkC) {
+#pragma unroll
+        for (int c = h; c < h + kChunkC; c++) {
+            if (need && c < ncand) {
+                Key key;
+                key.d = d2_ref(px, py, pz, cx[base + c], cy[base + c], cz[base + c]);
+                key.i = cid[base + c];
+                if (key_lt(key, st.kth)) {
+                    s_bd[st.cnt][lane] = key.d;
+                    s_bi[st.cnt][lane] = key.i;
+                    st.cnt++;
+                }
+            }
+        }
+        if (__ballot(st.cnt > kBuf - kChunkC) != 0ull) {
+            flush_buffer(st.td, st.ti, st.cnt, s_bd, s_bi, lane);
+            st.cnt = 0;
+            st.kth = kth_of(st.td, st.ti, k);
+        }
+    }
+}
+
 template <class P>
 __global__ void __launch_bounds__(kWave)
 knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned *qid,
             long long nq, const double *cx, const double *cy, const double *cz,
             const unsigned *cid, long long nc, const Box *boxes, Levels lv, int k,
-            long long *nbr, float *rho, int rho_mode, const P *mass) {
-    __shared__ double s_x[kWave], s_y[kWave], s_z[kWave];
-    __shared__ unsigned s_id[kWave];
+            long long *nbr, float *rho, int rho_mode, const P *mass, unsigned *n_overflow) {
     __shared__ double s_bd[kBuf][kWave];
     __shared__ unsigned s_bi[kBuf][kWave];
-    __shared__ int s_stack[kStack];
+    __shared__ int s_fr[kGroups][2][kFCap];
     __shared__ long long s_off[kMaxLevels], s_n[kMaxLevels];
 
     const int lane = threadIdx.x;
+    const int grp = lane / kG, gl = lane % kG, gshift = grp * kG;
     if (lane < lv.nlev) {
         s_off[lane] = level_offset(lv.nb, lane);
         s_n[lane] = level_count(lv.nb, lane);
@@ -363,99 +485,106 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
         pz = qz[qi];
     }
     double qlo[3], qhi[3];
-    qlo[0] = wave_min_d(valid ? px : DBL_MAX);
-    qlo[1] = wave_min_d(valid ? py : DBL_MAX);
-    qlo[2] = wave_min_d(valid ? pz : DBL_MAX);
-    qhi[0] = wave_max_d(valid ? px : -DBL_MAX);
-    qhi[1] = wave_max_d(valid ? py : -DBL_MAX);
-    qhi[2] = wave_max_d(valid ? pz : -DBL_MAX);
+    qlo[0] = grp_min_d(valid ? px : DBL_MAX);
+    qlo[1] = grp_min_d(valid ? py : DBL_MAX);
+    qlo[2] = grp_min_d(valid ? pz : DBL_MAX);
+    qhi[0] = grp_max_d(valid ? px : -DBL_MAX);
+    qhi[1] = grp_max_d(valid ? py : -DBL_MAX);
+    qhi[2] = grp_max_d(valid ? pz : -DBL_MAX);
+    const bool gvalid = ((__ballot(valid) >> gshift) & 0xffffull) != 0ull;
 
-    double td[kTop];
-    unsigned ti[kTop];
+    QState st;
 #pragma unroll
     for (int j = 0; j < kTop; j++) {
-        td[j] = __longlong_as_double(0x7ff0000000000000ll);  // +inf
-        ti[j] = 0xffffffffu;
+        st.td[j] = __longlong_as_double(0x7ff0000000000000ll);  // +inf
+        st.ti[j] = 0xffffffffu;
     }
-    Key kth;
-    kth.d = td[0];
-    kth.i = ti[0];
-    int cnt = 0;
-    double R2 = __longlong_as_double(0x7ff0000000000000ll);
-
-    int sp = 0;
-    if (lane == 0) s_stack[0] = ((lv.nlev - 1) << 26) | 0;
-    sp = 1;
+    st.kth.d = st.td[0];
+    st.kth.i = st.ti[0];
+    st.cnt = 0;
     __syncthreads();
-    while (sp > 0) {
-        int code = s_stack[sp - 1];
-        sp--;
-        int lvl = code >> 26;
-        long long j = code & ((1 << 26) - 1);
-        Box bx = boxes[s_off[lvl] + j];
-        if (box_gap2(bx, qlo, qhi) > R2) continue;
-        if (lvl > 0) {
-            long long c0 = 2 * j, c1 = 2 * j + 1;
-            __syncthreads();
-            if (c1 < s_n[lvl - 1]) {
-                Box b0 = boxes[s_off[lvl - 1] + c0];
-                Box b1 = boxes[s_off[lvl - 1] + c1];
-                double g0 = box_gap2(b0, qlo, qhi), g1 = box_gap2(b1, qlo, qhi);
-                long long nearc = g0 <= g1 ? c0 : c1, farc = g0 <= g1 ? c1 : c0;
-                if (lane == 0) {
-                    s_stack[sp] = ((lvl - 1) << 26) | (int)farc;
-                    s_stack[sp + 1] = ((lvl - 1) << 26) | (int)nearc;
-                }
-                sp += 2;
-            } else {
-                if (lane == 0) s_stack[sp] = ((lvl - 1) << 26) | (int)c0;
-                sp += 1;
-            }
-            __syncthreads();
-            continue;
+    const double inf = __longlong_as_double(0x7ff0000000000000ll);
+    const long long nb = lv.nb;
+
+    // ---- seed: greedy descent to the bucket nearest the group box
+    long long node = 0;
+    for (int l = lv.nlev - 1; l > 0; l--) {
+        long long child = kFan * node + (gl & (kFan - 1));
+        bool ok = gl < kFan && child < s_n[l - 1];
+        double g2 = ok ? box_gap2(boxes[s_off[l - 1] + child], qlo, qhi) : inf;
+        long long cbest = ok ? child : 0x7fffffffffffffffll;
+#pragma unroll
+        for (int o = kG / 2; o > 0; o >>= 1) {
+            double og = __shfl_xor(g2, o, 64);
+            long long oc = __shfl_xor(cbest, o, 64);
+            bool take = (og < g2) | ((og == g2) & (oc < cbest));
+            g2 = take ? og : g2;
+            cbest = take ? oc : cbest;
         }
-        // leaf bucket j: does any lane need it?
-        double lq[3] = {px, py, pz};
-        double lg0 = gap1(bx.lo[0], bx.hi[0], lq[0], lq[0]);
-        double lg1 = gap1(bx.lo[1], bx.hi[1], lq[1], lq[1]);
-        double lg2 = gap1(bx.lo[2], bx.hi[2], lq[2], lq[2]);
-        double lgap = (lg0 * lg0 + lg1 * lg1) + lg2 * lg2;
-        bool need = valid && !(lgap > kth.d);
-        if (__ballot(need) == 0ull) continue;
-        long long base = j * kWave;
-        int ncand = (int)min((long long)kWave, nc - base);
+        node = cbest;
+    }
+    const long long slo = node - 2 < 0 ? 0 : node - 2;
+    const long long shi = node + 2 > nb - 1 ? nb - 1 : node + 2;
+    for (int s = 0; s < 5; s++) {
+        long long b = slo + s;
+        scan_bucket(st, b, gvalid && b <= shi, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes,
+                    s_bd, s_bi, lane);
+    }
+    double R2 = grp_max_d(valid ? st.kth.d : 0.0);
+
+    // ---- breadth-first walk: level l nodes -> level l-1 children
+    int cur = 0, fcnt = 1;
+    bool overflow = false;
+    if (gl == 0) s_fr[grp][0][0] = 0;
+    __syncthreads();
+    for (int l = lv.nlev - 1; l > 0; l--) {
+        int nround = wave_max_i((overflow || !gvalid) ? 0 : (fcnt + 1) / 2);
+        int ncnt = 0;
+        for (int r = 0; r < nround; r++) {
+            int fi = 2 * r + (gl >> 3);
+            bool pass = false;
+            long long child = 0;
+            if (gvalid && !overflow && fi < fcnt) {
+                long long parent = s_fr[grp][cur][fi];
+                child = kFan * parent + (gl & (kFan - 1));
+                if (child < s_n[l - 1])
+                    pass = !(box_gap2(boxes[s_off[l - 1] + child], qlo, qhi) > R2);
+            }
+            unsigned m = (unsigned)((__ballot(pass) >> gshift) & 0xffffull);
+            int pos = ncnt + __popc(m & ((1u << gl) - 1u));
+            if (pass && pos < kFCap) s_fr[grp][cur ^ 1][pos] = (int)child;
+            ncnt += __popc(m);
+        }
+        if (ncnt > kFCap) overflow = true;
+        fcnt = ncnt > kFCap ? kFCap : ncnt;
+        cur ^= 1;
         __syncthreads();
-        if (lane < ncand) {
-            s_x[lane] = cx[base + lane];
-            s_y[lane] = cy[base + lane];
-            s_z[lane] = cz[base + lane];
-            s_id[lane] = cid[base + lane];
+    }
+    if (lv.nlev == 1) {  // a single bucket: the root is the only leaf
+        fcnt = 1;
+    }
+    // ---- scan the surviving leaf buckets (seeds already done)
+    int nproc = wave_max_i((overflow || !gvalid) ? 0 : fcnt);
+    for (int i = 0; i < nproc; i++) {
+        bool has = gvalid && !overflow && i < fcnt;
+        long long b = has ? s_fr[grp][cur][i] : 0;
+        if (has && b >= slo && b <= shi) has = false;
+        scan_bucket(st, b, has, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes, s_bd, s_bi,
+                    lane);
+    }
+    // ---- frontier overflow (a very wide search): scan every bucket
+    if (__ballot(overflow && gl == 0) != 0ull) {
+        if (lane == 0) atomicAdd(n_overflow, 1u);
+        for (long long b = 0; b < nb; b++) {
+            bool has = overflow && !(b >= slo && b <= shi);
+            scan_bucket(st, b, has, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes, s_bd,
+                        s_bi, lane);
         }
-        __syncthreads();
-        for (int c0 = 0; c0 < ncand; c0 += kChunkC) {
-            int c1 = min(c0 + kChunkC, ncand);
-            for (int c = c0; c < c1; c++) {
-                Key key;
-                key.d = d2_ref(px, py, pz, s_x[c], s_y[c], s_z[c]);
-                key.i = s_id[c];
-                if (need && key_lt(key, kth)) {
-                    s_bd[cnt][lane] = key.d;
-                    s_bi[cnt][lane] = key.i;
-                    cnt++;
-                }
-            }
-            if (__ballot(cnt > kBuf - kChunkC) != 0ull) {
-                flush_buffer(td, ti, cnt, s_bd, s_bi, lane);
-                cnt = 0;
-                kth = kth_of(td, ti, k);
-            }
-        }
-        R2 = wave_max_d(valid ? kth.d : 0.0);
     }
     // final merge
-    if (__ballot(cnt > 0) != 0ull) {
-        flush_buffer(td, ti, cnt, s_bd, s_bi, lane);
-        kth = kth_of(td, ti, k);
+    if (__ballot(st.cnt > 0) != 0ull) {
+        flush_buffer(st.td, st.ti, st.cnt, s_bd, s_bi, lane);
+        st.kth = kth_of(st.td, st.ti, k);
     }
     if (!valid) return;
     const long long q = qid[qi];
@@ -463,22 +592,22 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
         long long *row = nbr + q * k;
 #pragma unroll
         for (int t = 0; t < kTop; t++)
-            if (t < k) row[t] = (long long)ti[t];
+            if (t < k) row[t] = (long long)st.ti[t];
     }
     if (rho) {
-        float neardist = sqrtf((float)kth.d);
+        float neardist = sqrtf((float)st.kth.d);
         float h = (float)((double)neardist / 2.0);
-        float s = 0.0f;
+        float sum = 0.0f;
 #pragma unroll
         for (int t = 0; t < kTop; t++) {
             if (t < k) {
-                float r = sqrtf((float)td[t]);
+                float r = sqrtf((float)st.td[t]);
                 float w = W4(r, h);
-                if (rho_mode == 2) w = w * (float)mass[ti[t]];
-                s += w;
+                if (rho_mode == 2) w = w * (float)mass[st.ti[t]];
+                sum += w;
             }
         }
-        rho[q] = s;
+        rho[q] = sum;
     }
 }
 
@@ -495,16 +624,18 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
     const bool subset = type != nullptr && cand_ptype != PGALF_TYPE_ALL;
     // bounding box of every particle (queries include all candidates)
     unsigned long long *mm = ws.alloc_n<unsigned long long>(6);
-    if (!mm) return hipErrorOutOfMemory;
+    unsigned *novf = ws.alloc_n<unsigned>(1);
+    if (!mm || !novf) return hipErrorOutOfMemory;
     PGALF_HIP_TRY(hipMemsetAsync(mm, 0xff, 3 * sizeof(unsigned long long), s));
     PGALF_HIP_TRY(hipMemsetAsync(mm + 3, 0x00, 3 * sizeof(unsigned long long), s));
-    bbox_k<P><<<1024, 256, 0, s>>>(x, y, z, np, mm);
+    PGALF_HIP_TRY(hipMemsetAsync(novf, 0, sizeof(unsigned), s));
+    bbox_k<P><<<256, 256, 0, s>>>(x, y, z, np, mm);
 
-    // queries: all particles, Morton order
+    // queries: all particles, Hilbert order
     unsigned *qk = ws.alloc_n<unsigned>(np), *qv = ws.alloc_n<unsigned>(np);
     unsigned *qk2 = ws.alloc_n<unsigned>(np), *qv2 = ws.alloc_n<unsigned>(np);
     if (!qk || !qv || !qk2 || !qv2) return hipErrorOutOfMemory;
-    morton_k<P><<<nblk(np, 256), 256, 0, s>>>(x, y, z, nullptr, np, mm, qk, qv);
+    hilbert_k<P><<<nblk(np, 256), 256, 0, s>>>(x, y, z, nullptr, np, mm, qk, qv);
     PGALF_HIP_TRY(radix_sort_pairs(&qk, &qv, qk2, qv2, np, 30, ws, s));
     double *qx = ws.alloc_n<double>(np), *qy = ws.alloc_n<double>(np),
            *qz = ws.alloc_n<double>(np);
@@ -532,7 +663,7 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
         double *sx = ws.alloc_n<double>(nc), *sy = ws.alloc_n<double>(nc),
                *sz = ws.alloc_n<double>(nc);
         if (!ck || !cv || !ck2 || !cv2 || !sx || !sy || !sz) return hipErrorOutOfMemory;
-        morton_k<P><<<nblk(nc, 256), 256, 0, s>>>(x, y, z, sel, nc, mm, ck, cv);
+        hilbert_k<P><<<nblk(nc, 256), 256, 0, s>>>(x, y, z, sel, nc, mm, ck, cv);
         PGALF_HIP_TRY(radix_sort_pairs(&ck, &cv, ck2, cv2, nc, 30, ws, s));
         gather_sorted_k<P><<<nblk(nc, 256), 256, 0, s>>>(x, y, z, cv, nc, sx, sy, sz);
         cx = sx;
@@ -543,17 +674,18 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
         return hipErrorNotReady;
     }
 
-    // box tree over candidate buckets
+    // 8-ary box tree over 16-candidate buckets
     Levels lv;
-    long long nb = (nc + kWave - 1) / kWave;
-    if (nb >= (1LL << 26)) return hipErrorInvalidValue;
+    long long nb = (nc + kB - 1) / kB;
+    if (nb >= (1LL << 31)) return hipErrorInvalidValue;
     lv.nb = nb;
     lv.nlev = 1;
     while (level_count(nb, lv.nlev - 1) > 1) lv.nlev++;
+    if (lv.nlev > kMaxLevels) return hipErrorInvalidValue;
     long long total = level_offset(nb, lv.nlev);
     Box *boxes = ws.alloc_n<Box>((size_t)total);
     if (!boxes) return hipErrorOutOfMemory;
-    leaf_boxes_k<<<(unsigned)nb, kWave, 0, s>>>(cx, cy, cz, nc, boxes);
+    leaf_boxes_k<<<nblk(nb, 256), 256, 0, s>>>(cx, cy, cz, nc, nb, boxes);
     for (int l = 1; l < lv.nlev; l++) {
         long long nl = level_count(nb, l);
         parent_boxes_k<<<nblk(nl, 256), 256, 0, s>>>(boxes + level_offset(nb, l - 1),
@@ -564,7 +696,7 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
     timing_mark("knn_query", s, false);
     long long nqb = (np + kWave - 1) / kWave;
     knn_query_k<P><<<(unsigned)nqb, kWave, 0, s>>>(qx, qy, qz, qv, np, cx, cy, cz, cid, nc,
-                                                   boxes, lv, k, nbr, rho, rho_mode, m);
+                                                   boxes, lv, k, nbr, rho, rho_mode, m, novf);
     timing_mark("knn_query", s, true);
     return hipGetLastError();
 }

@@ -92,11 +92,16 @@ def test_smooth_vs_oracle(shape):
     rng = np.random.default_rng(7)
     g = np.zeros(D.padded_shape(nx, ny, nz), dtype=np.float32)
     g[..., :nx] = rng.random((nz, ny, nx), dtype=np.float32) ** 4
+    # white noise: FP32 FFT rounding (~1e-7 of the field scale) sits on every
+    # cell, so the bar is normwise and absolute; the pointwise 1e-5 gate is
+    # applied to halo density fields (test_stage_config1_vs_oracle).
     for RG in (2.0, 1.5, 0.7):
         got = g.copy()
         D.gaussian_Smoothing(got, nx, ny, nz, 1.0, RG)
         ref = O.gsmooth(g, nx, ny, nz, 1.0, RG)
-        _rel_checks(_inner(got, nx), _inner(ref, nx))
+        a, b = _inner(got, nx).astype(np.float64), _inner(ref, nx).astype(np.float64)
+        assert np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(b)
+        assert np.abs(a - b).max() <= 1e-6 * np.abs(b).max()
         assert np.all(got[..., nx:] == 0.0)
 
 
