@@ -532,12 +532,14 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
     }
     double R2 = grp_max_d(valid ? st.kth.d : 0.0);
 
-    // ---- breadth-first walk: level l nodes -> level l-1 children
+    // ---- breadth-first walk over internal levels (frontier in LDS), then
+    // the level-1 nodes are expanded straight into bucket scans, so the
+    // bucket list itself is never stored.
     int cur = 0, fcnt = 1;
     bool overflow = false;
     if (gl == 0) s_fr[grp][0][0] = 0;
     __syncthreads();
-    for (int l = lv.nlev - 1; l > 0; l--) {
+    for (int l = lv.nlev - 1; l > 1; l--) {
         int nround = wave_max_i((overflow || !gvalid) ? 0 : (fcnt + 1) / 2);
         int ncnt = 0;
         for (int r = 0; r < nround; r++) {
@@ -560,17 +562,33 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
         cur ^= 1;
         __syncthreads();
     }
-    if (lv.nlev == 1) {  // a single bucket: the root is the only leaf
-        fcnt = 1;
-    }
-    // ---- scan the surviving leaf buckets (seeds already done)
-    int nproc = wave_max_i((overflow || !gvalid) ? 0 : fcnt);
-    for (int i = 0; i < nproc; i++) {
-        bool has = gvalid && !overflow && i < fcnt;
-        long long b = has ? s_fr[grp][cur][i] : 0;
-        if (has && b >= slo && b <= shi) has = false;
-        scan_bucket(st, b, has, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes, s_bd, s_bi,
-                    lane);
+    if (lv.nlev > 1) {
+        // level-1 nodes (8 buckets each): two per round, children scanned as
+        // they pass; the group bound shrinks as the lists tighten.
+        int nround = wave_max_i((overflow || !gvalid) ? 0 : (fcnt + 1) / 2);
+        for (int r = 0; r < nround; r++) {
+            int fi = 2 * r + (gl >> 3);
+            bool pass = false;
+            long long child = 0;
+            if (gvalid && !overflow && fi < fcnt) {
+                long long parent = s_fr[grp][cur][fi];
+                child = kFan * parent + (gl & (kFan - 1));
+                if (child < nb && !(child >= slo && child <= shi))
+                    pass = !(box_gap2(boxes[child], qlo, qhi) > R2);
+            }
+            unsigned m = (unsigned)((__ballot(pass) >> gshift) & 0xffffull);
+            int npass = wave_max_i(__popc(m));
+            for (int t = 0; t < npass; t++) {
+                // t-th passing child of this group
+                bool has = m != 0u;
+                int bit = has ? __ffs(m) - 1 : 0;
+                long long b = __shfl(child, gshift + bit, 64);
+                scan_bucket(st, b, has, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes, s_bd,
+                            s_bi, lane);
+                if (has) m &= m - 1u;
+            }
+            R2 = grp_max_d(valid ? st.kth.d : 0.0);
+        }
     }
     // ---- frontier overflow (a very wide search): scan every bucket
     if (__ballot(overflow && gl == 0) != 0ull) {

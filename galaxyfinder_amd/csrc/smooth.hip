@@ -16,7 +16,12 @@
 //                        out circularly over [-(n_d-1), n_d-1]; the first n_d
 //                        outputs of the length-L linear convolution are the
 //                        length-n_d circular one.
-// H and the twiddles are formed on the host in FP64 and rounded to float.
+// The grid stays float32 in HBM; each line is transformed in FP64 in LDS
+// (FP64 vector FMA runs at the FP32 rate on this part and the passes are
+// memory-bound), so the only FP32 roundings are the three stores: every
+// cell keeps ~1 ulp relative accuracy even far below the line's peak, which
+// an FP32 FFT (whose error scales with the line norm) cannot give.
+// H and the twiddles are formed on the host in FP64.
 // Pad columns [nx, mx) are zeroed by the x pass (the reference leaves FFTW
 // scratch there).
 #include <cmath>
@@ -32,7 +37,7 @@ namespace pgalf {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kLdsComplex = 8192;  // complex elements of line data per block (64 KB)
+constexpr int kLdsComplex = 4096;  // complex FP64 elements of line data per block (64 KB)
 constexpr int kMaxL = 4096;
 
 struct AxisPass {
@@ -48,14 +53,14 @@ struct AxisPass {
     int padw;           // row width for the pad zeroing (mx)
 };
 
-__device__ inline float2 cmul(float2 a, float2 b) {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+__device__ inline double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
 // One Stockham pass over P lines of length L in LDS (in place: all inputs of
 // this thread are read before the barrier, outputs written after it).
 template <int R>
-__device__ inline void stockham_pass(float2 *buf, int L, int P, int Ns, const float2 *tw,
+__device__ inline void stockham_pass(double2 *buf, int L, int P, int Ns, const double2 *tw,
                                      bool inverse) {
     // at most kLdsComplex/R butterflies per pass -> kMaxBf per thread; the
     // per-thread loop is fully unrolled so v[][] stays in registers.
@@ -63,41 +68,41 @@ __device__ inline void stockham_pass(float2 *buf, int L, int P, int Ns, const fl
     const int nbf_line = L / R;
     const int nbf = P * nbf_line;
     const int tstep = L / (Ns * R);  // twiddle index step
-    float2 v[kMaxBf][R];
+    double2 v[kMaxBf][R];
     int dst[kMaxBf];
 #pragma unroll
     for (int c = 0; c < kMaxBf; c++) {
         int b = threadIdx.x + c * kBlock;
         if (b >= nbf) continue;
         int p = b / nbf_line, j = b - p * nbf_line;
-        float2 *line = buf + p * L;
+        double2 *line = buf + p * L;
         int k = j % Ns;
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            float2 a = line[j + r * nbf_line];
+            double2 a = line[j + r * nbf_line];
             if (r) {
-                float2 w = tw[(r * k * tstep) & (L - 1)];
+                double2 w = tw[(r * k * tstep) & (L - 1)];
                 if (inverse) w.y = -w.y;
                 a = cmul(a, w);
             }
             v[c][r] = a;
         }
         if (R == 2) {
-            float2 a0 = v[c][0], a1 = v[c][1];
-            v[c][0] = make_float2(a0.x + a1.x, a0.y + a1.y);
-            v[c][1] = make_float2(a0.x - a1.x, a0.y - a1.y);
+            double2 a0 = v[c][0], a1 = v[c][1];
+            v[c][0] = make_double2(a0.x + a1.x, a0.y + a1.y);
+            v[c][1] = make_double2(a0.x - a1.x, a0.y - a1.y);
         } else {
-            float2 a0 = v[c][0], a1 = v[c][1], a2 = v[c][2], a3 = v[c][3];
-            float2 s02 = make_float2(a0.x + a2.x, a0.y + a2.y);
-            float2 d02 = make_float2(a0.x - a2.x, a0.y - a2.y);
-            float2 s13 = make_float2(a1.x + a3.x, a1.y + a3.y);
-            float2 d13 = make_float2(a1.x - a3.x, a1.y - a3.y);
+            double2 a0 = v[c][0], a1 = v[c][1], a2 = v[c][2], a3 = v[c][3];
+            double2 s02 = make_double2(a0.x + a2.x, a0.y + a2.y);
+            double2 d02 = make_double2(a0.x - a2.x, a0.y - a2.y);
+            double2 s13 = make_double2(a1.x + a3.x, a1.y + a3.y);
+            double2 d13 = make_double2(a1.x - a3.x, a1.y - a3.y);
             // forward twiddle W4 = -i: (-i)(x + iy) = y - ix; inverse +i
-            float2 md13 = inverse ? make_float2(-d13.y, d13.x) : make_float2(d13.y, -d13.x);
-            v[c][0] = make_float2(s02.x + s13.x, s02.y + s13.y);
-            v[c][1] = make_float2(d02.x + md13.x, d02.y + md13.y);
-            v[c][2] = make_float2(s02.x - s13.x, s02.y - s13.y);
-            v[c][3] = make_float2(d02.x - md13.x, d02.y - md13.y);
+            double2 md13 = inverse ? make_double2(-d13.y, d13.x) : make_double2(d13.y, -d13.x);
+            v[c][0] = make_double2(s02.x + s13.x, s02.y + s13.y);
+            v[c][1] = make_double2(d02.x + md13.x, d02.y + md13.y);
+            v[c][2] = make_double2(s02.x - s13.x, s02.y - s13.y);
+            v[c][3] = make_double2(d02.x - md13.x, d02.y - md13.y);
         }
         dst[c] = p * L + (j / Ns) * Ns * R + k;
     }
@@ -112,7 +117,7 @@ __device__ inline void stockham_pass(float2 *buf, int L, int P, int Ns, const fl
     __syncthreads();
 }
 
-__device__ inline void fft_lines(float2 *buf, int L, int P, const float2 *tw, bool inverse) {
+__device__ inline void fft_lines(double2 *buf, int L, int P, const double2 *tw, bool inverse) {
     int Ns = 1;
     while (Ns * 4 <= L) {
         stockham_pass<4>(buf, L, P, Ns, tw, inverse);
@@ -122,8 +127,8 @@ __device__ inline void fft_lines(float2 *buf, int L, int P, const float2 *tw, bo
 }
 
 __global__ void __launch_bounds__(kBlock)
-smooth_axis_k(float *g, AxisPass ap, const float2 *tw, const float *H) {
-    extern __shared__ float2 buf[];
+smooth_axis_k(float *g, AxisPass ap, const double2 *tw, const double *H) {
+    extern __shared__ double2 buf[];
     const int L = ap.L, P = ap.P, n = ap.n;
     const long long ib0 = (long long)blockIdx.x * 2 * P;
     float *base = g + (long long)blockIdx.y * ap.sc;
@@ -136,19 +141,19 @@ smooth_axis_k(float *g, AxisPass ap, const float2 *tw, const float *H) {
         else { j = idx / lines; lb = idx - j * lines; }
         long long ib = ib0 + lb;
         float v = ib < ap.nb ? base[ib * ap.sb + (long long)j * ap.sa] : 0.0f;
-        float *dst = reinterpret_cast<float *>(&buf[(lb >> 1) * L + j]);
-        dst[lb & 1] = v;
+        double *dst = reinterpret_cast<double *>(&buf[(lb >> 1) * L + j]);
+        dst[lb & 1] = (double)v;
     }
     for (int idx = threadIdx.x; idx < P * (L - n); idx += kBlock) {
         int p = idx / (L - n), j = n + idx % (L - n);
-        buf[p * L + j] = make_float2(0.0f, 0.0f);
+        buf[p * L + j] = make_double2(0.0, 0.0);
     }
     __syncthreads();
     fft_lines(buf, L, P, tw, false);
     for (int idx = threadIdx.x; idx < P * L; idx += kBlock) {
-        float h = H[idx & (L - 1)];
-        float2 a = buf[idx];
-        buf[idx] = make_float2(a.x * h, a.y * h);
+        double h = H[idx & (L - 1)];
+        double2 a = buf[idx];
+        buf[idx] = make_double2(a.x * h, a.y * h);
     }
     __syncthreads();
     fft_lines(buf, L, P, tw, true);
@@ -158,8 +163,8 @@ smooth_axis_k(float *g, AxisPass ap, const float2 *tw, const float *H) {
         else { j = idx / lines; lb = idx - j * lines; }
         long long ib = ib0 + lb;
         if (ib < ap.nb) {
-            const float *src = reinterpret_cast<const float *>(&buf[(lb >> 1) * L + j]);
-            base[ib * ap.sb + (long long)j * ap.sa] = src[lb & 1];
+            const double *src = reinterpret_cast<const double *>(&buf[(lb >> 1) * L + j]);
+            base[ib * ap.sb + (long long)j * ap.sa] = (float)src[lb & 1];
         }
     }
     if (ap.zero_pads) {
@@ -175,8 +180,8 @@ smooth_axis_k(float *g, AxisPass ap, const float2 *tw, const float *H) {
 // ---- host planning ----------------------------------------------------------
 struct AxisPlan {
     int n = 0, L = 0;
-    float2 *tw = nullptr;
-    float *H = nullptr;
+    double2 *tw = nullptr;
+    double *H = nullptr;
 };
 
 std::mutex g_plan_mu;
@@ -210,16 +215,16 @@ hipError_t get_plan(int n, double expfac, AxisPlan &out) {
     const int L = pow2 ? n : next_pow2(2 * n - 1);
     if (L > kMaxL) return hipErrorInvalidValue;
     const long double kPi = 3.141592653589793238462643383279502884L;
-    std::vector<float2> tw(L);
+    std::vector<double2> tw(L);
     for (int m = 0; m < L; m++) {
         long double a = -2.0L * kPi * (long double)m / (long double)L;
-        tw[m] = make_float2((float)cosl(a), (float)sinl(a));
+        tw[m] = make_double2((double)cosl(a), (double)sinl(a));
     }
     std::vector<double> f(n);
     for (int k = 0; k < n; k++) f[k] = axis_factor(k, n, expfac);
-    std::vector<float> H(L);
+    std::vector<double> H(L);
     if (pow2) {
-        for (int k = 0; k < n; k++) H[k] = (float)(f[k] / (double)n);
+        for (int k = 0; k < n; k++) H[k] = f[k] / (double)n;
     } else {
         // periodic kernel g[m] = (1/n) sum_k f(k) cos(2 pi k m / n)
         std::vector<long double> gk(n);
@@ -242,16 +247,16 @@ hipError_t get_plan(int n, double expfac, AxisPlan &out) {
                 if (h[t] != 0.0L)
                     s += h[t] * cosl(2.0L * kPi * (long double)(((long)k * t) % L) /
                                      (long double)L);
-            H[k] = (float)(s / (long double)L);
+            H[k] = (double)(s / (long double)L);
         }
     }
     AxisPlan p;
     p.n = n;
     p.L = L;
-    PGALF_HIP_TRY(hipMalloc(&p.tw, sizeof(float2) * L));
-    PGALF_HIP_TRY(hipMalloc(&p.H, sizeof(float) * L));
-    PGALF_HIP_TRY(hipMemcpy(p.tw, tw.data(), sizeof(float2) * L, hipMemcpyHostToDevice));
-    PGALF_HIP_TRY(hipMemcpy(p.H, H.data(), sizeof(float) * L, hipMemcpyHostToDevice));
+    PGALF_HIP_TRY(hipMalloc(&p.tw, sizeof(double2) * L));
+    PGALF_HIP_TRY(hipMalloc(&p.H, sizeof(double) * L));
+    PGALF_HIP_TRY(hipMemcpy(p.tw, tw.data(), sizeof(double2) * L, hipMemcpyHostToDevice));
+    PGALF_HIP_TRY(hipMemcpy(p.H, H.data(), sizeof(double) * L, hipMemcpyHostToDevice));
     g_plans[key] = p;
     out = p;
     return hipSuccess;
@@ -275,7 +280,7 @@ hipError_t run_pass(float *g, const AxisPlan &pl, long long sa, int nb, long lon
     ap.zero_pads = zero_pads;
     ap.padw = padw;
     dim3 grid((unsigned)((nb + 2 * P - 1) / (2 * P)), (unsigned)nc);
-    size_t lds = sizeof(float2) * (size_t)P * pl.L;
+    size_t lds = sizeof(double2) * (size_t)P * pl.L;
     smooth_axis_k<<<grid, kBlock, lds, s>>>(g, ap, pl.tw, pl.H);
     return hipGetLastError();
 }
