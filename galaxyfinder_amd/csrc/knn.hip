@@ -30,6 +30,8 @@
 // FP64 operations as the distances, from box faces that are member
 // coordinates, so a gap never exceeds a member's computed distance.
 #include <cfloat>
+#include <cstdio>
+#include <cstdlib>
 
 #include "pgalf_internal.h"
 
@@ -464,7 +466,8 @@ __global__ void __launch_bounds__(kWave)
 knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned *qid,
             long long nq, const double *cx, const double *cy, const double *cz,
             const unsigned *cid, long long nc, const Box *boxes, Levels lv, int k,
-            long long *nbr, float *rho, int rho_mode, const P *mass, unsigned *n_overflow) {
+            long long *nbr, float *rho, int rho_mode, const P *mass, unsigned *n_overflow,
+            int self_seed) {
     __shared__ double s_bd[kBuf][kWave];
     __shared__ unsigned s_bi[kBuf][kWave];
     __shared__ int s_fr[kGroups][2][kFCap];
@@ -506,23 +509,42 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
     const double inf = __longlong_as_double(0x7ff0000000000000ll);
     const long long nb = lv.nb;
 
-    // ---- seed: greedy descent to the bucket nearest the group box
+    // ---- seed bucket.  Queries = candidates: the group's own bucket (same
+    // sorted order).  Otherwise descend greedily to the child whose box is
+    // nearest the group's centre point (ties: nearest box centre).
     long long node = 0;
-    for (int l = lv.nlev - 1; l > 0; l--) {
-        long long child = kFan * node + (gl & (kFan - 1));
-        bool ok = gl < kFan && child < s_n[l - 1];
-        double g2 = ok ? box_gap2(boxes[s_off[l - 1] + child], qlo, qhi) : inf;
-        long long cbest = ok ? child : 0x7fffffffffffffffll;
+    if (self_seed) {
+        node = ((long long)blockIdx.x * kWave + grp * kG) / kB;
+    } else {
+        const double gc[3] = {0.5 * (qlo[0] + qhi[0]), 0.5 * (qlo[1] + qhi[1]),
+                              0.5 * (qlo[2] + qhi[2])};
+        for (int l = lv.nlev - 1; l > 0; l--) {
+            long long child = kFan * node + (gl & (kFan - 1));
+            bool ok = gl < kFan && child < s_n[l - 1];
+            double g2 = inf, c2 = inf;
+            if (ok) {
+                Box bx = boxes[s_off[l - 1] + child];
+                g2 = box_gap2(bx, gc, gc);
+                double e0 = 0.5 * (bx.lo[0] + bx.hi[0]) - gc[0];
+                double e1 = 0.5 * (bx.lo[1] + bx.hi[1]) - gc[1];
+                double e2 = 0.5 * (bx.lo[2] + bx.hi[2]) - gc[2];
+                c2 = e0 * e0 + e1 * e1 + e2 * e2;
+            }
+            long long cbest = ok ? child : 0x7fffffffffffffffll;
 #pragma unroll
-        for (int o = kG / 2; o > 0; o >>= 1) {
-            double og = __shfl_xor(g2, o, 64);
-            long long oc = __shfl_xor(cbest, o, 64);
-            bool take = (og < g2) | ((og == g2) & (oc < cbest));
-            g2 = take ? og : g2;
-            cbest = take ? oc : cbest;
+            for (int o = kG / 2; o > 0; o >>= 1) {
+                double og = __shfl_xor(g2, o, 64);
+                double oc2 = __shfl_xor(c2, o, 64);
+                long long oc = __shfl_xor(cbest, o, 64);
+                bool take = (og < g2) | ((og == g2) & ((oc2 < c2) | ((oc2 == c2) & (oc < cbest))));
+                g2 = take ? og : g2;
+                c2 = take ? oc2 : c2;
+                cbest = take ? oc : cbest;
+            }
+            node = cbest;
         }
-        node = cbest;
     }
+    if (node > nb - 1) node = nb - 1;
     const long long slo = node - 2 < 0 ? 0 : node - 2;
     const long long shi = node + 2 > nb - 1 ? nb - 1 : node + 2;
     for (int s = 0; s < 5; s++) {
@@ -714,8 +736,16 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
     timing_mark("knn_query", s, false);
     long long nqb = (np + kWave - 1) / kWave;
     knn_query_k<P><<<(unsigned)nqb, kWave, 0, s>>>(qx, qy, qz, qv, np, cx, cy, cz, cid, nc,
-                                                   boxes, lv, k, nbr, rho, rho_mode, m, novf);
+                                                   boxes, lv, k, nbr, rho, rho_mode, m, novf,
+                                                   subset ? 0 : 1);
     timing_mark("knn_query", s, true);
+    if (getenv("PGALF_DEBUG")) {
+        unsigned h = 0;
+        PGALF_HIP_TRY(hipMemcpyAsync(&h, novf, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        PGALF_HIP_TRY(hipStreamSynchronize(s));
+        fprintf(stderr, "[pgalf] knn: np=%lld nc=%lld buckets=%lld levels=%d frontier "
+                        "overflows=%u\n", np, nc, nb, lv.nlev, h);
+    }
     return hipGetLastError();
 }
 
