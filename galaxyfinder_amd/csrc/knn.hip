@@ -461,6 +461,26 @@ __device__ inline void scan_bucket(QState &st, long long b, bool has, bool valid
     }
 }
 
+// Does box b come within the current k-th distance of any query of group
+// grp (points and k-th distances of the wave in LDS; invalid lanes hold -1)?
+__device__ inline bool union_pass(const Box &b, int grp, const double *s_qx, const double *s_qy,
+                                  const double *s_qz, const double *s_qk) {
+    bool pass = false;
+#pragma unroll
+    for (int j = 0; j < kG; j++) {
+        const int L = grp * kG + j;
+        double x = s_qx[L], y = s_qy[L], z = s_qz[L];
+        double g0 = gap1(b.lo[0], b.hi[0], x, x);
+        double g1 = gap1(b.lo[1], b.hi[1], y, y);
+        double g2 = gap1(b.lo[2], b.hi[2], z, z);
+        double a = g0 * g0;
+        double bb = g1 * g1;
+        double c = g2 * g2;
+        pass |= !(((a + bb) + c) > s_qk[L]);
+    }
+    return pass;
+}
+
 template <class P>
 __global__ void __launch_bounds__(kWave)
 knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned *qid,
@@ -472,6 +492,7 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
     __shared__ unsigned s_bi[kBuf][kWave];
     __shared__ int s_fr[kGroups][2][kFCap];
     __shared__ long long s_off[kMaxLevels], s_n[kMaxLevels];
+    __shared__ double s_qx[kWave], s_qy[kWave], s_qz[kWave], s_qk[kWave];
 
     const int lane = threadIdx.x;
     const int grp = lane / kG, gl = lane % kG, gshift = grp * kG;
@@ -553,6 +574,10 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
                     s_bd, s_bi, lane);
     }
     double R2 = grp_max_d(valid ? st.kth.d : 0.0);
+    s_qx[lane] = px;
+    s_qy[lane] = py;
+    s_qz[lane] = pz;
+    s_qk[lane] = valid ? st.kth.d : -1.0;
 
     // ---- breadth-first walk over internal levels (frontier in LDS), then
     // the level-1 nodes are expanded straight into bucket scans, so the
@@ -571,8 +596,11 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
             if (gvalid && !overflow && fi < fcnt) {
                 long long parent = s_fr[grp][cur][fi];
                 child = kFan * parent + (gl & (kFan - 1));
-                if (child < s_n[l - 1])
-                    pass = !(box_gap2(boxes[s_off[l - 1] + child], qlo, qhi) > R2);
+                if (child < s_n[l - 1]) {
+                    Box cb = boxes[s_off[l - 1] + child];
+                    pass = !(box_gap2(cb, qlo, qhi) > R2) &&
+                           union_pass(cb, grp, s_qx, s_qy, s_qz, s_qk);
+                }
             }
             unsigned m = (unsigned)((__ballot(pass) >> gshift) & 0xffffull);
             int pos = ncnt + __popc(m & ((1u << gl) - 1u));
@@ -595,8 +623,11 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
             if (gvalid && !overflow && fi < fcnt) {
                 long long parent = s_fr[grp][cur][fi];
                 child = kFan * parent + (gl & (kFan - 1));
-                if (child < nb && !(child >= slo && child <= shi))
-                    pass = !(box_gap2(boxes[child], qlo, qhi) > R2);
+                if (child < nb && !(child >= slo && child <= shi)) {
+                    Box cb = boxes[child];
+                    pass = !(box_gap2(cb, qlo, qhi) > R2) &&
+                           union_pass(cb, grp, s_qx, s_qy, s_qz, s_qk);
+                }
             }
             unsigned m = (unsigned)((__ballot(pass) >> gshift) & 0xffffull);
             int npass = wave_max_i(__popc(m));
@@ -610,6 +641,8 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
                 if (has) m &= m - 1u;
             }
             R2 = grp_max_d(valid ? st.kth.d : 0.0);
+            s_qk[lane] = valid ? st.kth.d : -1.0;
+            __syncthreads();
         }
     }
     // ---- frontier overflow (a very wide search): scan every bucket
