@@ -40,7 +40,8 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kTop = 32;     // list capacity (k <= 32, MAX_NUM_NEAR tree.h:20)
-constexpr int kBuf = 16;     // per-lane LDS buffer
+constexpr int kBuf = 8;      // merge chunk (sorted, then merged into the top-32)
+constexpr int kBufCap = 48;  // per-lane LDS append buffer (sorted-candidate indices)
 constexpr int kChunkC = 8;   // candidates between overflow checks
 constexpr int kMaxLevels = 24;
 constexpr int kB = 16;       // candidates per bucket
@@ -358,9 +359,9 @@ __device__ inline void bitonic_clean(double (&d)[N], unsigned (&x)[N]) {
     if constexpr (J > 1) bitonic_clean<N, J / 2>(d, x);
 }
 
-// Merge a buffer of up to 16 keys into the sorted top-32: sort the buffer,
-// take the elementwise minimum against its reverse (padded with +inf) --
-// the 32 smallest, as a bitonic sequence -- and clean.
+// Merge a chunk of kBuf keys into the sorted top-32: sort the chunk, take
+// the elementwise minimum of the top against the chunk reversed and padded
+// with +inf to 32 -- the 32 smallest, as a bitonic sequence -- and clean.
 __device__ inline void merge_top(double (&td)[kTop], unsigned (&ti)[kTop],
                                  double (&bd)[kBuf], unsigned (&bi)[kBuf]) {
     bitonic_sort<kBuf, 2>(bd, bi);
@@ -368,9 +369,10 @@ __device__ inline void merge_top(double (&td)[kTop], unsigned (&ti)[kTop],
     for (int j = 0; j < kBuf; j++) {
         double cd = bd[kBuf - 1 - j];
         unsigned ci = bi[kBuf - 1 - j];
-        bool take = klt(cd, ci, td[kBuf + j], ti[kBuf + j]);
-        td[kBuf + j] = take ? cd : td[kBuf + j];
-        ti[kBuf + j] = take ? ci : ti[kBuf + j];
+        constexpr int o = kTop - kBuf;  // reversed buffer meets the top's tail
+        bool take = klt(cd, ci, td[o + j], ti[o + j]);
+        td[o + j] = take ? cd : td[o + j];
+        ti[o + j] = take ? ci : ti[o + j];
     }
     bitonic_clean<kTop, kTop / 2>(td, ti);
 }
@@ -390,21 +392,31 @@ __device__ inline Key kth_of(const double (&td)[kTop], const unsigned (&ti)[kTop
     return r;
 }
 
-// Load a lane's buffer (cnt entries, rest +inf) and merge it.
+// Merge a lane's buffer of cnt sorted-candidate indices into its top-32:
+// keys are recomputed from the candidate arrays (same arithmetic as the
+// scan, so the same FP64 values), then merged 16 at a time.  Wave uniform.
 __device__ inline void flush_buffer(double (&td)[kTop], unsigned (&ti)[kTop], int cnt,
-                                    const double (*s_bd)[kWave], const unsigned (*s_bi)[kWave],
-                                    int lane) {
-    double bd[kBuf];
-    unsigned bi[kBuf];
+                                    const unsigned (*s_bc)[kWave], int lane, double px,
+                                    double py, double pz, const double *cx, const double *cy,
+                                    const double *cz, const unsigned *cid) {
+#pragma unroll 1
+    for (int ch = 0; ch < kBufCap / kBuf; ch++) {  // not unrolled: bounds live loads
+        if (__ballot(cnt > ch * kBuf) == 0ull) break;
+        double bd[kBuf];
+        unsigned bi[kBuf];
 #pragma unroll
-    for (int t = 0; t < kBuf; t++) {
-        bool v = t < cnt;
-        double dv = s_bd[t][lane];
-        unsigned iv = s_bi[t][lane];
-        bd[t] = v ? dv : __longlong_as_double(0x7ff0000000000000ll);
-        bi[t] = v ? iv : 0xffffffffu;
+        for (int t = 0; t < kBuf; t++) {
+            const int e = ch * kBuf + t;
+            bd[t] = __longlong_as_double(0x7ff0000000000000ll);
+            bi[t] = 0xffffffffu;
+            if (e < cnt) {
+                unsigned sidx = s_bc[e][lane];
+                bd[t] = d2_ref(px, py, pz, cx[sidx], cy[sidx], cz[sidx]);
+                bi[t] = cid[sidx];
+            }
+        }
+        merge_top(td, ti, bd, bi);
     }
-    merge_top(td, ti, bd, bi);
 }
 
 // Per-lane query state of the kNN kernel.
@@ -421,8 +433,8 @@ struct QState {
 __device__ inline void scan_bucket(QState &st, long long b, bool has, bool valid, int k,
                                    double px, double py, double pz, const double *cx,
                                    const double *cy, const double *cz, const unsigned *cid,
-                                   long long nc, const Box *boxes, double (*s_bd)[kWave],
-                                   unsigned (*s_bi)[kWave], int lane) {
+                                   long long nc, const Box *boxes, unsigned (*s_bc)[kWave],
+                                   int lane) {
     bool need = false;
     int ncand = 0;
     long long base = b * kB;
@@ -447,14 +459,13 @@ __device__ inline void scan_bucket(QState &st, long long b, bool has, bool valid
                 key.d = d2_ref(px, py, pz, cx[base + c], cy[base + c], cz[base + c]);
                 key.i = cid[base + c];
                 if (key_lt(key, st.kth)) {
-                    s_bd[st.cnt][lane] = key.d;
-                    s_bi[st.cnt][lane] = key.i;
+                    s_bc[st.cnt][lane] = (unsigned)(base + c);
                     st.cnt++;
                 }
             }
         }
-        if (__ballot(st.cnt > kBuf - kChunkC) != 0ull) {
-            flush_buffer(st.td, st.ti, st.cnt, s_bd, s_bi, lane);
+        if (__ballot(st.cnt > kBufCap - kChunkC) != 0ull) {
+            flush_buffer(st.td, st.ti, st.cnt, s_bc, lane, px, py, pz, cx, cy, cz, cid);
             st.cnt = 0;
             st.kth = kth_of(st.td, st.ti, k);
         }
@@ -466,7 +477,7 @@ __device__ inline void scan_bucket(QState &st, long long b, bool has, bool valid
 __device__ inline bool union_pass(const Box &b, int grp, const double *s_qx, const double *s_qy,
                                   const double *s_qz, const double *s_qk) {
     bool pass = false;
-#pragma unroll
+#pragma unroll 4
     for (int j = 0; j < kG; j++) {
         const int L = grp * kG + j;
         double x = s_qx[L], y = s_qy[L], z = s_qz[L];
@@ -482,14 +493,13 @@ __device__ inline bool union_pass(const Box &b, int grp, const double *s_qx, con
 }
 
 template <class P>
-__global__ void __launch_bounds__(kWave)
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2)))
 knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned *qid,
             long long nq, const double *cx, const double *cy, const double *cz,
             const unsigned *cid, long long nc, const Box *boxes, Levels lv, int k,
             long long *nbr, float *rho, int rho_mode, const P *mass, unsigned *n_overflow,
             int self_seed) {
-    __shared__ double s_bd[kBuf][kWave];
-    __shared__ unsigned s_bi[kBuf][kWave];
+    __shared__ unsigned s_bc[kBufCap][kWave];
     __shared__ int s_fr[kGroups][2][kFCap];
     __shared__ long long s_off[kMaxLevels], s_n[kMaxLevels];
     __shared__ double s_qx[kWave], s_qy[kWave], s_qz[kWave], s_qk[kWave];
@@ -566,12 +576,28 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
         }
     }
     if (node > nb - 1) node = nb - 1;
+    // direct fill: the seed bucket and its successor (predecessor at the
+    // end) give 32 candidates, sorted once; the other seeds go through the
+    // buffer.
+    const long long b1 = node;
+    const long long b2 = node + 1 < nb ? node + 1 : node - 1;
+#pragma unroll
+    for (int c = 0; c < kTop; c++) {
+        const long long bb = c < kB ? b1 : b2;
+        const long long e = bb * kB + (c % kB);
+        if (gvalid && bb >= 0 && e < nc) {
+            st.td[c] = d2_ref(px, py, pz, cx[e], cy[e], cz[e]);
+            st.ti[c] = cid[e];
+        }
+    }
+    bitonic_sort<kTop, 2>(st.td, st.ti);
+    st.kth = kth_of(st.td, st.ti, k);
     const long long slo = node - 2 < 0 ? 0 : node - 2;
     const long long shi = node + 2 > nb - 1 ? nb - 1 : node + 2;
-    for (int s = 0; s < 5; s++) {
-        long long b = slo + s;
-        scan_bucket(st, b, gvalid && b <= shi, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes,
-                    s_bd, s_bi, lane);
+    for (int sd = 0; sd < 5; sd++) {
+        long long b = slo + sd;
+        bool has = gvalid && b <= shi && b != b1 && b != b2;
+        scan_bucket(st, b, has, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes, s_bc, lane);
     }
     double R2 = grp_max_d(valid ? st.kth.d : 0.0);
     s_qx[lane] = px;
@@ -636,8 +662,8 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
                 bool has = m != 0u;
                 int bit = has ? __ffs(m) - 1 : 0;
                 long long b = __shfl(child, gshift + bit, 64);
-                scan_bucket(st, b, has, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes, s_bd,
-                            s_bi, lane);
+                scan_bucket(st, b, has, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes, s_bc,
+                            lane);
                 if (has) m &= m - 1u;
             }
             R2 = grp_max_d(valid ? st.kth.d : 0.0);
@@ -650,13 +676,13 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
         if (lane == 0) atomicAdd(n_overflow, 1u);
         for (long long b = 0; b < nb; b++) {
             bool has = overflow && !(b >= slo && b <= shi);
-            scan_bucket(st, b, has, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes, s_bd,
-                        s_bi, lane);
+            scan_bucket(st, b, has, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes, s_bc,
+                        lane);
         }
     }
     // final merge
     if (__ballot(st.cnt > 0) != 0ull) {
-        flush_buffer(st.td, st.ti, st.cnt, s_bd, s_bi, lane);
+        flush_buffer(st.td, st.ti, st.cnt, s_bc, lane, px, py, pz, cx, cy, cz, cid);
         st.kth = kth_of(st.td, st.ti, k);
     }
     if (!valid) return;
