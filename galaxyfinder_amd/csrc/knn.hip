@@ -434,7 +434,7 @@ __device__ inline void scan_bucket(QState &st, long long b, bool has, bool valid
                                    double px, double py, double pz, const double *cx,
                                    const double *cy, const double *cz, const unsigned *cid,
                                    long long nc, const Box *boxes, unsigned (*s_bc)[kWave],
-                                   int lane) {
+                                   int lane, unsigned long long *dbg) {
     bool need = false;
     int ncand = 0;
     long long base = b * kB;
@@ -450,6 +450,7 @@ __device__ inline void scan_bucket(QState &st, long long b, bool has, bool valid
         ncand = (int)min((long long)kB, nc - base);
     }
     if (__ballot(need) == 0ull) return;
+    if (dbg && lane == 0) atomicAdd(&dbg[0], 1ull);  // buckets scanned (wave)
 #pragma unroll
     for (int h = 0; h < kB; h += kChunkC) {
 #pragma unroll
@@ -465,6 +466,7 @@ __device__ inline void scan_bucket(QState &st, long long b, bool has, bool valid
             }
         }
         if (__ballot(st.cnt > kBufCap - kChunkC) != 0ull) {
+            if (dbg && lane == 0) atomicAdd(&dbg[1], 1ull);  // mid-scan merges
             flush_buffer(st.td, st.ti, st.cnt, s_bc, lane, px, py, pz, cx, cy, cz, cid);
             st.cnt = 0;
             st.kth = kth_of(st.td, st.ti, k);
@@ -498,7 +500,7 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
             long long nq, const double *cx, const double *cy, const double *cz,
             const unsigned *cid, long long nc, const Box *boxes, Levels lv, int k,
             long long *nbr, float *rho, int rho_mode, const P *mass, unsigned *n_overflow,
-            int self_seed) {
+            int self_seed, unsigned long long *dbg) {
     __shared__ unsigned s_bc[kBufCap][kWave];
     __shared__ int s_fr[kGroups][2][kFCap];
     __shared__ long long s_off[kMaxLevels], s_n[kMaxLevels];
@@ -597,7 +599,7 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
     for (int sd = 0; sd < 5; sd++) {
         long long b = slo + sd;
         bool has = gvalid && b <= shi && b != b1 && b != b2;
-        scan_bucket(st, b, has, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes, s_bc, lane);
+        scan_bucket(st, b, has, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes, s_bc, lane, dbg);
     }
     double R2 = grp_max_d(valid ? st.kth.d : 0.0);
     s_qx[lane] = px;
@@ -614,6 +616,7 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
     __syncthreads();
     for (int l = lv.nlev - 1; l > 1; l--) {
         int nround = wave_max_i((overflow || !gvalid) ? 0 : (fcnt + 1) / 2);
+        if (dbg && lane == 0) atomicAdd(&dbg[2], (unsigned long long)nround);
         int ncnt = 0;
         for (int r = 0; r < nround; r++) {
             int fi = 2 * r + (gl >> 3);
@@ -642,6 +645,7 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
         // level-1 nodes (8 buckets each): two per round, children scanned as
         // they pass; the group bound shrinks as the lists tighten.
         int nround = wave_max_i((overflow || !gvalid) ? 0 : (fcnt + 1) / 2);
+        if (dbg && lane == 0) atomicAdd(&dbg[3], (unsigned long long)nround);
         for (int r = 0; r < nround; r++) {
             int fi = 2 * r + (gl >> 3);
             bool pass = false;
@@ -657,13 +661,14 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
             }
             unsigned m = (unsigned)((__ballot(pass) >> gshift) & 0xffffull);
             int npass = wave_max_i(__popc(m));
+            if (dbg && lane == 0) atomicAdd(&dbg[4], (unsigned long long)npass);
             for (int t = 0; t < npass; t++) {
                 // t-th passing child of this group
                 bool has = m != 0u;
                 int bit = has ? __ffs(m) - 1 : 0;
                 long long b = __shfl(child, gshift + bit, 64);
                 scan_bucket(st, b, has, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes, s_bc,
-                            lane);
+                            lane, dbg);
                 if (has) m &= m - 1u;
             }
             R2 = grp_max_d(valid ? st.kth.d : 0.0);
@@ -677,10 +682,11 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
         for (long long b = 0; b < nb; b++) {
             bool has = overflow && !(b >= slo && b <= shi);
             scan_bucket(st, b, has, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes, s_bc,
-                        lane);
+                        lane, dbg);
         }
     }
     // final merge
+    if (dbg && lane == 0) atomicAdd(&dbg[5], 1ull);  // waves
     if (__ballot(st.cnt > 0) != 0ull) {
         flush_buffer(st.td, st.ti, st.cnt, s_bc, lane, px, py, pz, cx, cy, cz, cid);
         st.kth = kth_of(st.td, st.ti, k);
@@ -724,7 +730,10 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
     // bounding box of every particle (queries include all candidates)
     unsigned long long *mm = ws.alloc_n<unsigned long long>(6);
     unsigned *novf = ws.alloc_n<unsigned>(1);
-    if (!mm || !novf) return hipErrorOutOfMemory;
+    const bool debug = getenv("PGALF_DEBUG") != nullptr;
+    unsigned long long *dbg = debug ? ws.alloc_n<unsigned long long>(8) : nullptr;
+    if (!mm || !novf || (debug && !dbg)) return hipErrorOutOfMemory;
+    if (dbg) PGALF_HIP_TRY(hipMemsetAsync(dbg, 0, 8 * sizeof(unsigned long long), s));
     PGALF_HIP_TRY(hipMemsetAsync(mm, 0xff, 3 * sizeof(unsigned long long), s));
     PGALF_HIP_TRY(hipMemsetAsync(mm + 3, 0x00, 3 * sizeof(unsigned long long), s));
     PGALF_HIP_TRY(hipMemsetAsync(novf, 0, sizeof(unsigned), s));
@@ -796,14 +805,19 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
     long long nqb = (np + kWave - 1) / kWave;
     knn_query_k<P><<<(unsigned)nqb, kWave, 0, s>>>(qx, qy, qz, qv, np, cx, cy, cz, cid, nc,
                                                    boxes, lv, k, nbr, rho, rho_mode, m, novf,
-                                                   subset ? 0 : 1);
+                                                   subset ? 0 : 1, dbg);
     timing_mark("knn_query", s, true);
-    if (getenv("PGALF_DEBUG")) {
+    if (debug) {
         unsigned h = 0;
+        unsigned long long d[8];
         PGALF_HIP_TRY(hipMemcpyAsync(&h, novf, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        PGALF_HIP_TRY(hipMemcpyAsync(d, dbg, sizeof(d), hipMemcpyDeviceToHost, s));
         PGALF_HIP_TRY(hipStreamSynchronize(s));
-        fprintf(stderr, "[pgalf] knn: np=%lld nc=%lld buckets=%lld levels=%d frontier "
-                        "overflows=%u\n", np, nc, nb, lv.nlev, h);
+        double w = d[5] ? (double)d[5] : 1.0;
+        fprintf(stderr, "[pgalf] knn: np=%lld nc=%lld buckets=%lld levels=%d overflows=%u | per "
+                        "wave: buckets scanned %.1f, mid-scan merges %.1f, internal rounds "
+                        "%.1f, level-1 rounds %.1f, level-1 bucket slots %.1f\n",
+                np, nc, nb, lv.nlev, h, d[0] / w, d[1] / w, d[2] / w, d[3] / w, d[4] / w);
     }
     return hipGetLastError();
 }
