@@ -48,7 +48,10 @@ constexpr int kB = 16;       // candidates per bucket
 constexpr int kG = 16;       // lanes per query group
 constexpr int kGroups = kWave / kG;
 constexpr int kFan = 8;      // box-tree fan-out
-constexpr int kFCap = 256;   // frontier capacity per group
+constexpr int kFCap = 256;   // frontier / bucket-list capacity per group
+constexpr int kBatch = 4;    // buckets staged in LDS per scan batch
+constexpr int kStage = kBatch * kB;  // staged candidates per group
+constexpr int kStagePad = kStage + 1;  // odd stride in doubles: groups hit other banks
 
 struct Box {
     double lo[3], hi[3];
@@ -427,50 +430,70 @@ struct QState {
     int cnt;
 };
 
-// Scan bucket b (one per group; has == false: this group skips) for every
-// lane whose point-to-box gap does not exceed its k-th distance.  Wave
-// uniform: all lanes call it together; merges are wave-wide.
-__device__ inline void scan_bucket(QState &st, long long b, bool has, bool valid, int k,
-                                   double px, double py, double pz, const double *cx,
-                                   const double *cy, const double *cz, const unsigned *cid,
-                                   long long nc, const Box *boxes, unsigned (*s_bc)[kWave],
-                                   int lane, unsigned long long *dbg) {
-    bool need = false;
-    int ncand = 0;
-    long long base = b * kB;
-    if (has) {
-        Box bx = boxes[b];
-        double g0 = gap1(bx.lo[0], bx.hi[0], px, px);
-        double g1 = gap1(bx.lo[1], bx.hi[1], py, py);
-        double g2 = gap1(bx.lo[2], bx.hi[2], pz, pz);
-        double a = g0 * g0;
-        double bb = g1 * g1;
-        double c = g2 * g2;
-        need = valid && !(((a + bb) + c) > st.kth.d);
-        ncand = (int)min((long long)kB, nc - base);
-    }
-    if (__ballot(need) == 0ull) return;
-    if (dbg && lane == 0) atomicAdd(&dbg[0], 1ull);  // buckets scanned (wave)
+// LDS staging area of one scan batch: kBatch buckets per group.
+struct Stage {
+    double x[kGroups][kStagePad], y[kGroups][kStagePad], z[kGroups][kStagePad];
+    unsigned e[kGroups][kStagePad];   // sorted candidate index, ~0u: empty slot
+    unsigned id[kGroups][kStagePad];  // its particle index (key tie-break, output)
+};
+
+// Scan the buckets listed in list[0..count) of this lane's group (count is
+// group-uniform).  kBatch buckets at a time are staged in LDS -- each lane
+// of the group loads one candidate per bucket, so the global-memory latency
+// is paid once per batch -- then every lane runs over the staged candidates
+// and appends those closer than its k-th key to its buffer.  Wave uniform.
+__device__ inline void scan_list(QState &st, const int *list, int count, bool valid, int k,
+                                 double px, double py, double pz, const double *cx,
+                                 const double *cy, const double *cz, const unsigned *cid,
+                                 long long nc, Stage &sg, unsigned (*s_bc)[kWave], int lane,
+                                 int grp, int gl, unsigned long long *dbg) {
+    const int nmax = wave_max_i(count);
+    for (int i0 = 0; i0 < nmax; i0 += kBatch) {
 #pragma unroll
-    for (int h = 0; h < kB; h += kChunkC) {
-#pragma unroll
-        for (int c = h; c < h + kChunkC; c++) {
-            if (need && c < ncand) {
-                Key key;
-                key.d = d2_ref(px, py, pz, cx[base + c], cy[base + c], cz[base + c]);
-                key.i = cid[base + c];
-                if (key_lt(key, st.kth)) {
-                    s_bc[st.cnt][lane] = (unsigned)(base + c);
-                    st.cnt++;
-                }
+        for (int t = 0; t < kBatch; t++) {
+            const int i = i0 + t;
+            long long e = -1;
+            if (i < count) {
+                long long b = list[i];
+                e = b * kB + gl;
+                if (e >= nc) e = -1;
+            }
+            const int slot = t * kB + gl;
+            if (e >= 0) {
+                sg.x[grp][slot] = cx[e];
+                sg.y[grp][slot] = cy[e];
+                sg.z[grp][slot] = cz[e];
+                sg.e[grp][slot] = (unsigned)e;
+                sg.id[grp][slot] = cid[e];
+            } else {
+                sg.e[grp][slot] = 0xffffffffu;
             }
         }
-        if (__ballot(st.cnt > kBufCap - kChunkC) != 0ull) {
-            if (dbg && lane == 0) atomicAdd(&dbg[1], 1ull);  // mid-scan merges
-            flush_buffer(st.td, st.ti, st.cnt, s_bc, lane, px, py, pz, cx, cy, cz, cid);
-            st.cnt = 0;
-            st.kth = kth_of(st.td, st.ti, k);
+        if (dbg && lane == 0) atomicAdd(&dbg[0], 1ull);  // scan batches (wave)
+        __syncthreads();
+        const int nslots = min(kBatch, nmax - i0) * kB;
+        for (int h = 0; h < nslots; h += kChunkC) {
+#pragma unroll
+            for (int c = h; c < h + kChunkC; c++) {
+                const unsigned e = sg.e[grp][c];
+                if (valid && e != 0xffffffffu) {
+                    Key key;
+                    key.d = d2_ref(px, py, pz, sg.x[grp][c], sg.y[grp][c], sg.z[grp][c]);
+                    key.i = sg.id[grp][c];
+                    if (key_lt(key, st.kth)) {
+                        s_bc[st.cnt][lane] = e;
+                        st.cnt++;
+                    }
+                }
+            }
+            if (__ballot(st.cnt > kBufCap - kChunkC) != 0ull) {
+                if (dbg && lane == 0) atomicAdd(&dbg[1], 1ull);  // mid-scan merges
+                flush_buffer(st.td, st.ti, st.cnt, s_bc, lane, px, py, pz, cx, cy, cz, cid);
+                st.cnt = 0;
+                st.kth = kth_of(st.td, st.ti, k);
+            }
         }
+        __syncthreads();
     }
 }
 
@@ -505,6 +528,7 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
     __shared__ int s_fr[kGroups][2][kFCap];
     __shared__ long long s_off[kMaxLevels], s_n[kMaxLevels];
     __shared__ double s_qx[kWave], s_qy[kWave], s_qz[kWave], s_qk[kWave];
+    __shared__ Stage sg;
 
     const int lane = threadIdx.x;
     const int grp = lane / kG, gl = lane % kG, gshift = grp * kG;
@@ -596,20 +620,25 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
     st.kth = kth_of(st.td, st.ti, k);
     const long long slo = node - 2 < 0 ? 0 : node - 2;
     const long long shi = node + 2 > nb - 1 ? nb - 1 : node + 2;
-    for (int sd = 0; sd < 5; sd++) {
-        long long b = slo + sd;
-        bool has = gvalid && b <= shi && b != b1 && b != b2;
-        scan_bucket(st, b, has, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes, s_bc, lane, dbg);
+    // remaining seed buckets through the batched scanner (list in s_fr[grp][1])
+    int scnt = 0;
+    if (gl == 0) {
+        for (long long b = slo; b <= shi; b++)
+            if (gvalid && b != b1 && b != b2) s_fr[grp][1][scnt++] = (int)b;
     }
+    scnt = __shfl(scnt, gshift, 64);
+    __syncthreads();
+    scan_list(st, s_fr[grp][1], scnt, valid, k, px, py, pz, cx, cy, cz, cid, nc, sg, s_bc, lane,
+              grp, gl, dbg);
     double R2 = grp_max_d(valid ? st.kth.d : 0.0);
     s_qx[lane] = px;
     s_qy[lane] = py;
     s_qz[lane] = pz;
     s_qk[lane] = valid ? st.kth.d : -1.0;
 
-    // ---- breadth-first walk over internal levels (frontier in LDS), then
-    // the level-1 nodes are expanded straight into bucket scans, so the
-    // bucket list itself is never stored.
+    // ---- breadth-first walk over internal levels (frontier in LDS); the
+    // level-1 nodes are then expanded into per-group bucket lists, scanned
+    // in chunks whenever a list nears capacity.
     int cur = 0, fcnt = 1;
     bool overflow = false;
     if (gl == 0) s_fr[grp][0][0] = 0;
@@ -642,10 +671,12 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
         __syncthreads();
     }
     if (lv.nlev > 1) {
-        // level-1 nodes (8 buckets each): two per round, children scanned as
-        // they pass; the group bound shrinks as the lists tighten.
+        // level-1 nodes (8 buckets each), two per round; passing buckets go to
+        // the list in the other frontier half
         int nround = wave_max_i((overflow || !gvalid) ? 0 : (fcnt + 1) / 2);
         if (dbg && lane == 0) atomicAdd(&dbg[3], (unsigned long long)nround);
+        int* list = s_fr[grp][cur ^ 1];
+        int lcnt = 0;
         for (int r = 0; r < nround; r++) {
             int fi = 2 * r + (gl >> 3);
             bool pass = false;
@@ -660,29 +691,36 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
                 }
             }
             unsigned m = (unsigned)((__ballot(pass) >> gshift) & 0xffffull);
-            int npass = wave_max_i(__popc(m));
-            if (dbg && lane == 0) atomicAdd(&dbg[4], (unsigned long long)npass);
-            for (int t = 0; t < npass; t++) {
-                // t-th passing child of this group
-                bool has = m != 0u;
-                int bit = has ? __ffs(m) - 1 : 0;
-                long long b = __shfl(child, gshift + bit, 64);
-                scan_bucket(st, b, has, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes, s_bc,
-                            lane, dbg);
-                if (has) m &= m - 1u;
+            int pos = lcnt + __popc(m & ((1u << gl) - 1u));
+            if (pass) list[pos] = (int)child;
+            lcnt += __popc(m);
+            const bool last = r == nround - 1;
+            if (wave_max_i(lcnt) > kFCap - kG || (last && wave_max_i(lcnt) > 0)) {
+                __syncthreads();
+                if (dbg && lane == 0) atomicAdd(&dbg[4], 1ull);  // list scans
+                scan_list(st, list, lcnt, valid, k, px, py, pz, cx, cy, cz, cid, nc, sg, s_bc,
+                          lane, grp, gl, dbg);
+                lcnt = 0;
+                R2 = grp_max_d(valid ? st.kth.d : 0.0);
+                s_qk[lane] = valid ? st.kth.d : -1.0;
+                __syncthreads();
             }
-            R2 = grp_max_d(valid ? st.kth.d : 0.0);
-            s_qk[lane] = valid ? st.kth.d : -1.0;
-            __syncthreads();
         }
     }
     // ---- frontier overflow (a very wide search): scan every bucket
     if (__ballot(overflow && gl == 0) != 0ull) {
         if (lane == 0) atomicAdd(n_overflow, 1u);
-        for (long long b = 0; b < nb; b++) {
-            bool has = overflow && !(b >= slo && b <= shi);
-            scan_bucket(st, b, has, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes, s_bc,
-                        lane, dbg);
+        int* list = s_fr[grp][0];
+        for (long long b0 = 0; b0 < nb; b0 += kFCap) {
+            int lcnt = 0;
+            if (gl == 0 && overflow) {
+                for (long long b = b0; b < min(nb, b0 + kFCap); b++)
+                    if (!(b >= slo && b <= shi)) list[lcnt++] = (int)b;
+            }
+            lcnt = __shfl(lcnt, gshift, 64);
+            __syncthreads();
+            scan_list(st, list, lcnt, valid, k, px, py, pz, cx, cy, cz, cid, nc, sg, s_bc, lane,
+                      grp, gl, dbg);
         }
     }
     // final merge
@@ -815,8 +853,8 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
         PGALF_HIP_TRY(hipStreamSynchronize(s));
         double w = d[5] ? (double)d[5] : 1.0;
         fprintf(stderr, "[pgalf] knn: np=%lld nc=%lld buckets=%lld levels=%d overflows=%u | per "
-                        "wave: buckets scanned %.1f, mid-scan merges %.1f, internal rounds "
-                        "%.1f, level-1 rounds %.1f, level-1 bucket slots %.1f\n",
+                        "wave: scan batches %.1f, mid-scan merges %.1f, internal rounds "
+                        "%.1f, level-1 rounds %.1f, list scans %.1f\n",
                 np, nc, nb, lv.nlev, h, d[0] / w, d[1] / w, d[2] / w, d[3] / w, d[4] / w);
     }
     return hipGetLastError();
