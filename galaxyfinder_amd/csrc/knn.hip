@@ -52,6 +52,9 @@ constexpr int kFCap = 256;   // frontier / bucket-list capacity per group
 constexpr int kBatch = 4;    // buckets staged in LDS per scan batch
 constexpr int kStage = kBatch * kB;  // staged candidates per group
 constexpr int kStagePad = kStage + 1;  // odd stride in doubles: groups hit other banks
+constexpr int kRings = 8;    // distance classes of the near-first bucket order
+constexpr int kRingShift = 28;  // list entry = bucket | ring << 28
+constexpr int kBucketMask = (1 << kRingShift) - 1;
 
 struct Box {
     double lo[3], hi[3];
@@ -430,7 +433,8 @@ struct QState {
     int cnt;
 };
 
-// LDS staging area of one scan batch: kBatch buckets per group.
+// LDS staging area of one scan batch: kBatch buckets per group.  Between
+// batches it doubles as the ring sort's scratch (kGroups * kFCap ints).
 struct Stage {
     double x[kGroups][kStagePad], y[kGroups][kStagePad], z[kGroups][kStagePad];
     unsigned e[kGroups][kStagePad];   // sorted candidate index, ~0u: empty slot
@@ -445,18 +449,23 @@ struct Stage {
 __device__ inline void scan_list(QState &st, const int *list, int count, bool valid, int k,
                                  double px, double py, double pz, const double *cx,
                                  const double *cy, const double *cz, const unsigned *cid,
-                                 long long nc, Stage &sg, unsigned (*s_bc)[kWave], int lane,
-                                 int grp, int gl, unsigned long long *dbg) {
+                                 long long nc, const Box *boxes, const double *qlo,
+                                 const double *qhi, Stage &sg, unsigned (*s_bc)[kWave],
+                                 int lane, int grp, int gl, unsigned long long *dbg) {
     const int nmax = wave_max_i(count);
     for (int i0 = 0; i0 < nmax; i0 += kBatch) {
+        // buckets whose box is now beyond the group's bound are dropped
+        const double R2 = grp_max_d(valid ? st.kth.d : 0.0);
 #pragma unroll
         for (int t = 0; t < kBatch; t++) {
             const int i = i0 + t;
             long long e = -1;
             if (i < count) {
-                long long b = list[i];
-                e = b * kB + gl;
-                if (e >= nc) e = -1;
+                long long b = list[i] & kBucketMask;
+                if (!(box_gap2(boxes[b], qlo, qhi) > R2)) {
+                    e = b * kB + gl;
+                    if (e >= nc) e = -1;
+                }
             }
             const int slot = t * kB + gl;
             if (e >= 0) {
@@ -495,6 +504,40 @@ __device__ inline void scan_list(QState &st, const int *list, int count, bool va
         }
         __syncthreads();
     }
+}
+
+static_assert(sizeof(int) * kGroups * kFCap <= 3 * sizeof(double) * kGroups * kStagePad,
+              "ring-sort scratch must fit in the staging area");
+
+// Stable counting sort of a group's bucket list by ring (top bits), 16 lanes
+// in parallel; tmp (>= kFCap ints of LDS) is scratch.  Group-uniform count.
+__device__ inline void ring_sort(int *list, int count, int *tmp, int gshift, int gl) {
+    const int per = (count + kG - 1) / kG;
+    const int c0 = min(count, gl * per), c1 = min(count, c0 + per);
+    int cnt[kRings];
+#pragma unroll
+    for (int q = 0; q < kRings; q++) cnt[q] = 0;
+    for (int j = c0; j < c1; j++) {
+        const int q = (unsigned)list[j] >> kRingShift;
+#pragma unroll
+        for (int r = 0; r < kRings; r++) cnt[r] += (q == r);
+    }
+    int base = 0;
+#pragma unroll
+    for (int q = 0; q < kRings; q++) {
+        int incl = cnt[q];
+#pragma unroll
+        for (int o = 1; o < kG; o <<= 1) {
+            int t = __shfl_up(incl, o, kG);
+            if (gl >= o) incl += t;
+        }
+        int off = base + incl - cnt[q];
+        base += __shfl(incl, gshift + kG - 1, 64);
+        for (int j = c0; j < c1; j++)
+            if (((unsigned)list[j] >> kRingShift) == (unsigned)q) tmp[off++] = list[j];
+    }
+    __syncthreads();
+    for (int j = gl; j < count; j += kG) list[j] = tmp[j];
 }
 
 // Does box b come within the current k-th distance of any query of group
@@ -628,8 +671,8 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
     }
     scnt = __shfl(scnt, gshift, 64);
     __syncthreads();
-    scan_list(st, s_fr[grp][1], scnt, valid, k, px, py, pz, cx, cy, cz, cid, nc, sg, s_bc, lane,
-              grp, gl, dbg);
+    scan_list(st, s_fr[grp][1], scnt, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes, qlo,
+              qhi, sg, s_bc, lane, grp, gl, dbg);
     double R2 = grp_max_d(valid ? st.kth.d : 0.0);
     s_qx[lane] = px;
     s_qy[lane] = py;
@@ -677,9 +720,12 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
         if (dbg && lane == 0) atomicAdd(&dbg[3], (unsigned long long)nround);
         int* list = s_fr[grp][cur ^ 1];
         int lcnt = 0;
+        const double gcen[3] = {0.5 * (qlo[0] + qhi[0]), 0.5 * (qlo[1] + qhi[1]),
+                                0.5 * (qlo[2] + qhi[2])};
         for (int r = 0; r < nround; r++) {
             int fi = 2 * r + (gl >> 3);
             bool pass = false;
+            int ring = 0;
             long long child = 0;
             if (gvalid && !overflow && fi < fcnt) {
                 long long parent = s_fr[grp][cur][fi];
@@ -688,18 +734,23 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
                     Box cb = boxes[child];
                     pass = !(box_gap2(cb, qlo, qhi) > R2) &&
                            union_pass(cb, grp, s_qx, s_qy, s_qz, s_qk);
+                    // distance class: gap from the group centre in R/4 steps
+                    double g2c = box_gap2(cb, gcen, gcen);
+                    ring = (int)fmin((double)(kRings - 1), 4.0 * sqrt(g2c / fmax(R2, 1e-300)));
                 }
             }
             unsigned m = (unsigned)((__ballot(pass) >> gshift) & 0xffffull);
             int pos = lcnt + __popc(m & ((1u << gl) - 1u));
-            if (pass) list[pos] = (int)child;
+            if (pass) list[pos] = (int)child | (ring << kRingShift);
             lcnt += __popc(m);
             const bool last = r == nround - 1;
             if (wave_max_i(lcnt) > kFCap - kG || (last && wave_max_i(lcnt) > 0)) {
                 __syncthreads();
+                ring_sort(list, lcnt, reinterpret_cast<int *>(&sg) + grp * kFCap, gshift, gl);
+                __syncthreads();
                 if (dbg && lane == 0) atomicAdd(&dbg[4], 1ull);  // list scans
-                scan_list(st, list, lcnt, valid, k, px, py, pz, cx, cy, cz, cid, nc, sg, s_bc,
-                          lane, grp, gl, dbg);
+                scan_list(st, list, lcnt, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes, qlo,
+                          qhi, sg, s_bc, lane, grp, gl, dbg);
                 lcnt = 0;
                 R2 = grp_max_d(valid ? st.kth.d : 0.0);
                 s_qk[lane] = valid ? st.kth.d : -1.0;
@@ -719,8 +770,8 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
             }
             lcnt = __shfl(lcnt, gshift, 64);
             __syncthreads();
-            scan_list(st, list, lcnt, valid, k, px, py, pz, cx, cy, cz, cid, nc, sg, s_bc, lane,
-                      grp, gl, dbg);
+            scan_list(st, list, lcnt, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes, qlo,
+                      qhi, sg, s_bc, lane, grp, gl, dbg);
         }
     }
     // final merge
@@ -823,7 +874,7 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
     // 8-ary box tree over 16-candidate buckets
     Levels lv;
     long long nb = (nc + kB - 1) / kB;
-    if (nb >= (1LL << 31)) return hipErrorInvalidValue;
+    if (nb >= (1LL << kRingShift)) return hipErrorInvalidValue;
     lv.nb = nb;
     lv.nlev = 1;
     while (level_count(nb, lv.nlev - 1) > 1) lv.nlev++;
