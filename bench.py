@@ -26,6 +26,24 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public peak
+CALIB_COPY_BYTES = 1 << 29  # plain device copy (negation) that scales the HBM byte counters
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "knn_query_k_traffic.txt")
+
+
+def measured_traffic(n, k):
+    """HBM bytes per knn_query_k launch from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the same counter's reading for
+    a plain copy of CALIB_COPY_BYTES), when it was taken on this workload."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            kv = dict(line.split(":", 1) for line in f
+                      if ":" in line and not line.startswith("#"))
+        kv = {a.strip(): b.strip() for a, b in kv.items()}
+        if int(kv["particles"]) != n or int(kv["k"]) != k:
+            return None
+        return int(float(kv["traffic_bytes_per_launch"]))
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def stage_bytes(n, g_cells, k):
@@ -87,6 +105,9 @@ def main():
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 4])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--knn-sample", type=int, default=1000000)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a plain %d-byte device copy (counter calibration)"
+                    % CALIB_COPY_BYTES)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the density stage on a "
@@ -142,6 +163,13 @@ def main():
         if cnt:
             phases[ph] = tot / cnt
     D.timing(enable=False, reset=True)
+    if args.calib_copy:
+        src = torch.ones(CALIB_COPY_BYTES // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            torch.neg(src, out=dst)  # elementwise kernel: reads and writes CALIB_COPY_BYTES
+        torch.cuda.synchronize()
+        del src, dst
     g_cells = spec["nx"] * spec["ny"] * spec["nz"]
     bts = stage_bytes(n, g_cells, k)
     q_ms = phases.get("knn_query", float("nan"))
@@ -153,7 +181,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": measured_traffic(n, k),
         "bytes_per_launch": bts["knn"],
         "ms_per_launch": round(q_ms, 4),
     }
