@@ -44,17 +44,33 @@ constexpr int kBuf = 8;      // merge chunk (sorted, then merged into the top-32
 constexpr int kBufCap = 48;  // per-lane LDS append buffer (sorted-candidate indices)
 constexpr int kChunkC = 8;   // candidates between overflow checks
 constexpr int kMaxLevels = 24;
-constexpr int kB = 16;       // candidates per bucket
-constexpr int kG = 16;       // lanes per query group
+constexpr int kB = 8;        // candidates per bucket
+constexpr int kG = 8;        // lanes per query group (one candidate of a bucket each)
 constexpr int kGroups = kWave / kG;
+constexpr unsigned long long kGMask = (1ull << kG) - 1ull;
 constexpr int kFan = 8;      // box-tree fan-out
-constexpr int kFCap = 256;   // frontier / bucket-list capacity per group
+constexpr int kPar = kG / kFan;  // parents expanded per walk round
+constexpr int kSeedFill = kTop / kB;  // buckets of the direct seed fill
+constexpr int kSeedSpan = 2 * kSeedFill;  // seed buckets in all
+constexpr int kFCap = 128;   // bucket-list capacity per group
+constexpr int kStackCap = 96;  // depth-first walk stack per group
+constexpr int kFirstScan = 16;  // the first list is scanned early to tighten the bounds
+constexpr int kLevelShift = 5;  // stack entry = node << 5 | level
 constexpr int kBatch = 4;    // buckets staged in LDS per scan batch
 constexpr int kStage = kBatch * kB;  // staged candidates per group
-constexpr int kStagePad = kStage + 1;  // odd stride in doubles: groups hit other banks
+constexpr int kStageStride = 2 * kStage + 1;  // 16-byte units per group: groups hit other banks
 constexpr int kRings = 8;    // distance classes of the near-first bucket order
 constexpr int kRingShift = 28;  // list entry = bucket | ring << 28
 constexpr int kBucketMask = (1 << kRingShift) - 1;
+static_assert(kG == kB, "a scan batch loads one candidate of each bucket per group lane");
+static_assert(kG % kFan == 0 && kPar >= 1, "a walk round expands whole parents");
+static_assert(kTop % kB == 0, "the direct seed fill takes whole buckets");
+static_assert(kPar == 1, "the depth-first walk expands one parent per round");
+// The walk keeps at most kFan entries of the deepest level and kFan-1 of every
+// level above it on the stack.
+constexpr int stack_need(int nlev) { return kFan + (kFan - 1) * (nlev > 3 ? nlev - 3 : 0); }
+constexpr int kMaxWalkLevels = 14;
+static_assert(stack_need(kMaxWalkLevels) <= kStackCap, "walk stack too small");
 
 struct Box {
     double lo[3], hi[3];
@@ -335,6 +351,15 @@ __device__ inline int wave_max_i(int v) {
     return v;
 }
 
+// Debug-only per-wave counters (PGALF_DEBUG): events and shader-clock cycles.
+constexpr int kDbgN = 16;
+__device__ inline unsigned long long dbg_clock(const unsigned long long *dbg) {
+    return dbg ? (unsigned long long)clock64() : 0ull;
+}
+__device__ inline void dbg_cycles(unsigned long long *dbg, int i, unsigned long long t0, int lane) {
+    if (dbg && lane == 0) atomicAdd(&dbg[i], (unsigned long long)clock64() - t0);
+}
+
 // Bitonic networks written as compile-time recursions over split (dist,
 // index) arrays so every index is a constant and the lists stay in
 // registers.
@@ -433,13 +458,46 @@ struct QState {
     int cnt;
 };
 
-// LDS staging area of one scan batch: kBatch buckets per group.  Between
-// batches it doubles as the ring sort's scratch (kGroups * kFCap ints).
+// LDS staging area of one scan batch: kBatch buckets per group, one 32-byte
+// record per candidate in two 16-byte units -- {x, y} and {z, sorted index,
+// particle index} -- read with two 128-bit loads.  An empty slot has a NaN x,
+// so its distance never compares below a bound.  Between batches the area
+// doubles as the ring sort's scratch (kGroups * kFCap ints).
 struct Stage {
-    double x[kGroups][kStagePad], y[kGroups][kStagePad], z[kGroups][kStagePad];
-    unsigned e[kGroups][kStagePad];   // sorted candidate index, ~0u: empty slot
-    unsigned id[kGroups][kStagePad];  // its particle index (key tie-break, output)
+    uint4 u[kGroups * kStageStride];
 };
+__device__ inline void stage_put(Stage &sg, int grp, int slot, double x, double y, double z,
+                                 unsigned e, unsigned id) {
+    const int b = grp * kStageStride + 2 * slot;
+    const unsigned long long bx = (unsigned long long)__double_as_longlong(x);
+    const unsigned long long by = (unsigned long long)__double_as_longlong(y);
+    const unsigned long long bz = (unsigned long long)__double_as_longlong(z);
+    sg.u[b] = make_uint4((unsigned)bx, (unsigned)(bx >> 32), (unsigned)by, (unsigned)(by >> 32));
+    sg.u[b + 1] = make_uint4((unsigned)bz, (unsigned)(bz >> 32), e, id);
+}
+__device__ inline double hilo_d(unsigned lo, unsigned hi) {
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Does box b come within the current k-th distance of any query of group
+// grp (points and k-th distances of the wave in LDS; invalid lanes hold -1)?
+__device__ inline bool union_pass(const Box &b, int grp, const double *s_qx, const double *s_qy,
+                                  const double *s_qz, const double *s_qk) {
+    bool pass = false;
+#pragma unroll 4
+    for (int j = 0; j < kG; j++) {
+        const int L = grp * kG + j;
+        double x = s_qx[L], y = s_qy[L], z = s_qz[L];
+        double g0 = gap1(b.lo[0], b.hi[0], x, x);
+        double g1 = gap1(b.lo[1], b.hi[1], y, y);
+        double g2 = gap1(b.lo[2], b.hi[2], z, z);
+        double a = g0 * g0;
+        double bb = g1 * g1;
+        double c = g2 * g2;
+        pass |= !(((a + bb) + c) > s_qk[L]);
+    }
+    return pass;
+}
 
 // Scan the buckets listed in list[0..count) of this lane's group (count is
 // group-uniform).  kBatch buckets at a time are staged in LDS -- each lane
@@ -449,64 +507,78 @@ struct Stage {
 __device__ inline void scan_list(QState &st, const int *list, int count, bool valid, int k,
                                  double px, double py, double pz, const double *cx,
                                  const double *cy, const double *cz, const unsigned *cid,
-                                 long long nc, const Box *boxes, const double *qlo,
-                                 const double *qhi, Stage &sg, unsigned (*s_bc)[kWave],
-                                 int lane, int grp, int gl, unsigned long long *dbg) {
+                                 long long nc, const Box *boxes, Stage &sg,
+                                 unsigned (*s_bc)[kWave], double *s_qx, double *s_qy,
+                                 double *s_qz, double *s_qk, int lane, int grp, int gl,
+                                 unsigned long long *dbg) {
+    const unsigned long long t_scan = dbg_clock(dbg);
     const int nmax = wave_max_i(count);
     for (int i0 = 0; i0 < nmax; i0 += kBatch) {
-        // buckets whose box is now beyond the group's bound are dropped
-        const double R2 = grp_max_d(valid ? st.kth.d : 0.0);
+        // buckets that no lane's current ball reaches are dropped (group lane
+        // t tests bucket i0 + t)
+        bool bpass = false;
+        if (gl < kBatch && i0 + gl < count)
+            bpass = union_pass(boxes[list[i0 + gl] & kBucketMask], grp, s_qx, s_qy, s_qz, s_qk);
+        const unsigned bm = (unsigned)((__ballot(bpass) >> (grp * kG)) & kGMask);
 #pragma unroll
         for (int t = 0; t < kBatch; t++) {
             const int i = i0 + t;
             long long e = -1;
-            if (i < count) {
-                long long b = list[i] & kBucketMask;
-                if (!(box_gap2(boxes[b], qlo, qhi) > R2)) {
-                    e = b * kB + gl;
-                    if (e >= nc) e = -1;
-                }
+            if (i < count && ((bm >> t) & 1u)) {
+                e = (long long)(list[i] & kBucketMask) * kB + gl;
+                if (e >= nc) e = -1;
             }
             const int slot = t * kB + gl;
-            if (e >= 0) {
-                sg.x[grp][slot] = cx[e];
-                sg.y[grp][slot] = cy[e];
-                sg.z[grp][slot] = cz[e];
-                sg.e[grp][slot] = (unsigned)e;
-                sg.id[grp][slot] = cid[e];
-            } else {
-                sg.e[grp][slot] = 0xffffffffu;
-            }
+            if (e >= 0)
+                stage_put(sg, grp, slot, cx[e], cy[e], cz[e], (unsigned)e, cid[e]);
+            else
+                stage_put(sg, grp, slot, __longlong_as_double(0x7ff8000000000000ll), 0.0, 0.0,
+                          0xffffffffu, 0xffffffffu);
         }
         if (dbg && lane == 0) atomicAdd(&dbg[0], 1ull);  // scan batches (wave)
         __syncthreads();
         const int nslots = min(kBatch, nmax - i0) * kB;
+        const uint4 *su = sg.u + grp * kStageStride;
         for (int h = 0; h < nslots; h += kChunkC) {
+            // branch-free: every candidate's index is written at the lane's
+            // count, which advances only when the key is below the bound
+            // (the count is at most kBufCap - kChunkC on entry)
 #pragma unroll
             for (int c = h; c < h + kChunkC; c++) {
-                const unsigned e = sg.e[grp][c];
-                if (valid && e != 0xffffffffu) {
-                    Key key;
-                    key.d = d2_ref(px, py, pz, sg.x[grp][c], sg.y[grp][c], sg.z[grp][c]);
-                    key.i = sg.id[grp][c];
-                    if (key_lt(key, st.kth)) {
-                        s_bc[st.cnt][lane] = e;
-                        st.cnt++;
-                    }
-                }
+                const uint4 a = su[2 * c], b = su[2 * c + 1];
+                const double d = d2_ref(px, py, pz, hilo_d(a.x, a.y), hilo_d(a.z, a.w),
+                                        hilo_d(b.x, b.y));
+                const bool acc = valid & klt(d, b.w, st.kth.d, st.kth.i);
+                s_bc[st.cnt][lane] = b.z;
+                st.cnt += acc ? 1 : 0;
             }
             if (__ballot(st.cnt > kBufCap - kChunkC) != 0ull) {
                 if (dbg && lane == 0) atomicAdd(&dbg[1], 1ull);  // mid-scan merges
+                const unsigned long long t_m = dbg_clock(dbg);
                 flush_buffer(st.td, st.ti, st.cnt, s_bc, lane, px, py, pz, cx, cy, cz, cid);
+                dbg_cycles(dbg, 11, t_m, lane);
                 st.cnt = 0;
                 st.kth = kth_of(st.td, st.ti, k);
+                s_qk[lane] = valid ? st.kth.d : -1.0;
             }
         }
         __syncthreads();
     }
+    // merge what is left, so the caller's bounds are exact for what was seen
+    if (__ballot(st.cnt > 0) != 0ull) {
+        if (dbg && lane == 0) atomicAdd(&dbg[6], 1ull);  // end-of-list merges
+        const unsigned long long t_m = dbg_clock(dbg);
+        flush_buffer(st.td, st.ti, st.cnt, s_bc, lane, px, py, pz, cx, cy, cz, cid);
+        dbg_cycles(dbg, 11, t_m, lane);
+        st.cnt = 0;
+        st.kth = kth_of(st.td, st.ti, k);
+    }
+    s_qk[lane] = valid ? st.kth.d : -1.0;
+    __syncthreads();
+    dbg_cycles(dbg, 10, t_scan, lane);
 }
 
-static_assert(sizeof(int) * kGroups * kFCap <= 3 * sizeof(double) * kGroups * kStagePad,
+static_assert(sizeof(int) * kGroups * kFCap <= sizeof(Stage),
               "ring-sort scratch must fit in the staging area");
 
 // Stable counting sort of a group's bucket list by ring (top bits), 16 lanes
@@ -540,39 +612,21 @@ __device__ inline void ring_sort(int *list, int count, int *tmp, int gshift, int
     for (int j = gl; j < count; j += kG) list[j] = tmp[j];
 }
 
-// Does box b come within the current k-th distance of any query of group
-// grp (points and k-th distances of the wave in LDS; invalid lanes hold -1)?
-__device__ inline bool union_pass(const Box &b, int grp, const double *s_qx, const double *s_qy,
-                                  const double *s_qz, const double *s_qk) {
-    bool pass = false;
-#pragma unroll 4
-    for (int j = 0; j < kG; j++) {
-        const int L = grp * kG + j;
-        double x = s_qx[L], y = s_qy[L], z = s_qz[L];
-        double g0 = gap1(b.lo[0], b.hi[0], x, x);
-        double g1 = gap1(b.lo[1], b.hi[1], y, y);
-        double g2 = gap1(b.lo[2], b.hi[2], z, z);
-        double a = g0 * g0;
-        double bb = g1 * g1;
-        double c = g2 * g2;
-        pass |= !(((a + bb) + c) > s_qk[L]);
-    }
-    return pass;
-}
-
 template <class P>
 __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2)))
 knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned *qid,
             long long nq, const double *cx, const double *cy, const double *cz,
             const unsigned *cid, long long nc, const Box *boxes, Levels lv, int k,
-            long long *nbr, float *rho, int rho_mode, const P *mass, unsigned *n_overflow,
+            long long *nbr, float *rho, int rho_mode, const P *mass,
             int self_seed, unsigned long long *dbg) {
     __shared__ unsigned s_bc[kBufCap][kWave];
-    __shared__ int s_fr[kGroups][2][kFCap];
+    __shared__ int s_list[kGroups][kFCap];
+    __shared__ int s_stk[kGroups][kStackCap];  // also the seed list before the walk
     __shared__ long long s_off[kMaxLevels], s_n[kMaxLevels];
     __shared__ double s_qx[kWave], s_qy[kWave], s_qz[kWave], s_qk[kWave];
     __shared__ Stage sg;
 
+    const unsigned long long t_start = dbg_clock(dbg);
     const int lane = threadIdx.x;
     const int grp = lane / kG, gl = lane % kG, gshift = grp * kG;
     if (lane < lv.nlev) {
@@ -594,7 +648,7 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
     qhi[0] = grp_max_d(valid ? px : -DBL_MAX);
     qhi[1] = grp_max_d(valid ? py : -DBL_MAX);
     qhi[2] = grp_max_d(valid ? pz : -DBL_MAX);
-    const bool gvalid = ((__ballot(valid) >> gshift) & 0xffffull) != 0ull;
+    const bool gvalid = ((__ballot(valid) >> gshift) & kGMask) != 0ull;
 
     QState st;
 #pragma unroll
@@ -645,141 +699,115 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
         }
     }
     if (node > nb - 1) node = nb - 1;
-    // direct fill: the seed bucket and its successor (predecessor at the
-    // end) give 32 candidates, sorted once; the other seeds go through the
-    // buffer.
-    const long long b1 = node;
-    const long long b2 = node + 1 < nb ? node + 1 : node - 1;
+    // direct fill: kSeedFill consecutive buckets from the one before the seed
+    // bucket give 32 candidates, sorted once; the other seeds -- the window
+    // [slo, shi] of kSeedSpan buckets around them -- go through the buffer.
+    long long f0 = node - 1;
+    if (f0 > nb - kSeedFill) f0 = nb - kSeedFill;
+    if (f0 < 0) f0 = 0;
 #pragma unroll
     for (int c = 0; c < kTop; c++) {
-        const long long bb = c < kB ? b1 : b2;
-        const long long e = bb * kB + (c % kB);
-        if (gvalid && bb >= 0 && e < nc) {
+        const long long e = f0 * kB + c;
+        if (gvalid && e < nc) {
             st.td[c] = d2_ref(px, py, pz, cx[e], cy[e], cz[e]);
             st.ti[c] = cid[e];
         }
     }
     bitonic_sort<kTop, 2>(st.td, st.ti);
     st.kth = kth_of(st.td, st.ti, k);
-    const long long slo = node - 2 < 0 ? 0 : node - 2;
-    const long long shi = node + 2 > nb - 1 ? nb - 1 : node + 2;
-    // remaining seed buckets through the batched scanner (list in s_fr[grp][1])
+    const long long slo = f0 - kSeedFill / 2 < 0 ? 0 : f0 - kSeedFill / 2;
+    const long long shi = slo + kSeedSpan - 1 > nb - 1 ? nb - 1 : slo + kSeedSpan - 1;
+    // remaining seed buckets through the batched scanner (list in the stack)
+    static_assert(kSeedSpan <= kStackCap, "seed list lives in the walk stack");
     int scnt = 0;
     if (gl == 0) {
         for (long long b = slo; b <= shi; b++)
-            if (gvalid && b != b1 && b != b2) s_fr[grp][1][scnt++] = (int)b;
+            if (gvalid && (b < f0 || b >= f0 + kSeedFill)) s_stk[grp][scnt++] = (int)b;
     }
     scnt = __shfl(scnt, gshift, 64);
     __syncthreads();
-    scan_list(st, s_fr[grp][1], scnt, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes, qlo,
-              qhi, sg, s_bc, lane, grp, gl, dbg);
-    double R2 = grp_max_d(valid ? st.kth.d : 0.0);
     s_qx[lane] = px;
     s_qy[lane] = py;
     s_qz[lane] = pz;
     s_qk[lane] = valid ? st.kth.d : -1.0;
-
-    // ---- breadth-first walk over internal levels (frontier in LDS); the
-    // level-1 nodes are then expanded into per-group bucket lists, scanned
-    // in chunks whenever a list nears capacity.
-    int cur = 0, fcnt = 1;
-    bool overflow = false;
-    if (gl == 0) s_fr[grp][0][0] = 0;
     __syncthreads();
-    for (int l = lv.nlev - 1; l > 1; l--) {
-        int nround = wave_max_i((overflow || !gvalid) ? 0 : (fcnt + 1) / 2);
-        if (dbg && lane == 0) atomicAdd(&dbg[2], (unsigned long long)nround);
-        int ncnt = 0;
-        for (int r = 0; r < nround; r++) {
-            int fi = 2 * r + (gl >> 3);
-            bool pass = false;
-            long long child = 0;
-            if (gvalid && !overflow && fi < fcnt) {
-                long long parent = s_fr[grp][cur][fi];
-                child = kFan * parent + (gl & (kFan - 1));
-                if (child < s_n[l - 1]) {
-                    Box cb = boxes[s_off[l - 1] + child];
-                    pass = !(box_gap2(cb, qlo, qhi) > R2) &&
-                           union_pass(cb, grp, s_qx, s_qy, s_qz, s_qk);
-                }
-            }
-            unsigned m = (unsigned)((__ballot(pass) >> gshift) & 0xffffull);
-            int pos = ncnt + __popc(m & ((1u << gl) - 1u));
-            if (pass && pos < kFCap) s_fr[grp][cur ^ 1][pos] = (int)child;
-            ncnt += __popc(m);
-        }
-        if (ncnt > kFCap) overflow = true;
-        fcnt = ncnt > kFCap ? kFCap : ncnt;
-        cur ^= 1;
-        __syncthreads();
+    scan_list(st, s_stk[grp], scnt, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes, sg, s_bc,
+              s_qx, s_qy, s_qz, s_qk, lane, grp, gl, dbg);
+    double R2 = grp_max_d(valid ? st.kth.d : 0.0);
+    dbg_cycles(dbg, 8, t_start, lane);
+
+    // ---- depth-first walk, nearest child first: a round pops one node per
+    // group and its kG lanes test the node's children against the group box
+    // and the union of the lanes' current balls.  Passing internal children
+    // are pushed farthest first; passing buckets are appended to the group's
+    // list, which is ring-sorted and scanned whenever some list nears
+    // capacity, tightening the bounds for the rest of the walk.
+    const double gcen[3] = {0.5 * (qlo[0] + qhi[0]), 0.5 * (qlo[1] + qhi[1]),
+                            0.5 * (qlo[2] + qhi[2])};
+    int *list = s_list[grp];
+    int *stk = s_stk[grp];
+    int sp = 0, lcnt = 0, nscan = 0;
+    if (gvalid && lv.nlev > 1) {
+        if (gl == 0) stk[0] = lv.nlev - 1;  // root: node 0 of the top level
+        sp = 1;
     }
-    if (lv.nlev > 1) {
-        // level-1 nodes (8 buckets each), two per round; passing buckets go to
-        // the list in the other frontier half
-        int nround = wave_max_i((overflow || !gvalid) ? 0 : (fcnt + 1) / 2);
-        if (dbg && lane == 0) atomicAdd(&dbg[3], (unsigned long long)nround);
-        int* list = s_fr[grp][cur ^ 1];
-        int lcnt = 0;
-        const double gcen[3] = {0.5 * (qlo[0] + qhi[0]), 0.5 * (qlo[1] + qhi[1]),
-                                0.5 * (qlo[2] + qhi[2])};
-        for (int r = 0; r < nround; r++) {
-            int fi = 2 * r + (gl >> 3);
+    __syncthreads();
+    for (;;) {
+        const bool more = sp > 0;
+        const bool any = __ballot(more) != 0ull;
+        if (any) {
+            const unsigned long long t_r = dbg_clock(dbg);
+            int ent = 0;
+            if (more) ent = stk[sp - 1];
+            sp -= more ? 1 : 0;
+            const int cl = more ? (ent & ((1 << kLevelShift) - 1)) - 1 : 0;  // children's level
+            const long long child = (long long)kFan * (ent >> kLevelShift) + gl;
             bool pass = false;
-            int ring = 0;
-            long long child = 0;
-            if (gvalid && !overflow && fi < fcnt) {
-                long long parent = s_fr[grp][cur][fi];
-                child = kFan * parent + (gl & (kFan - 1));
-                if (child < nb && !(child >= slo && child <= shi)) {
-                    Box cb = boxes[child];
-                    pass = !(box_gap2(cb, qlo, qhi) > R2) &&
-                           union_pass(cb, grp, s_qx, s_qy, s_qz, s_qk);
-                    // distance class: gap from the group centre in R/4 steps
-                    double g2c = box_gap2(cb, gcen, gcen);
-                    ring = (int)fmin((double)(kRings - 1), 4.0 * sqrt(g2c / fmax(R2, 1e-300)));
+            double cg = inf;
+            if (more && child < s_n[cl] && !(cl == 0 && child >= slo && child <= shi)) {
+                Box cb = boxes[s_off[cl] + child];
+                pass = !(box_gap2(cb, qlo, qhi) > R2) &&
+                       union_pass(cb, grp, s_qx, s_qy, s_qz, s_qk);
+                cg = box_gap2(cb, gcen, gcen);
+            }
+            const unsigned m = (unsigned)((__ballot(pass) >> gshift) & kGMask);
+            const int npass = __popc(m);
+            if (dbg && lane == 0) atomicAdd(&dbg[2], 1ull);  // walk rounds
+            if (cl == 0) {
+                int pos = lcnt + __popc(m & ((1u << gl) - 1u));
+                // distance class: gap from the group centre in R/4 steps
+                int ring = (int)fmin((double)(kRings - 1), 4.0 * sqrt(cg / fmax(R2, 1e-300)));
+                if (pass) list[pos] = (int)child | (ring << kRingShift);
+                lcnt += npass;
+            } else {
+                // rank among the passing children by (gap, lane): nearer ones on top
+                int nearer = 0;
+#pragma unroll
+                for (int j = 0; j < kG; j++) {
+                    const double og = __shfl(cg, gshift + j, 64);
+                    nearer += (((m >> j) & 1u) != 0u) & ((og < cg) | ((og == cg) & (j < gl)));
                 }
+                if (pass) stk[sp + npass - 1 - nearer] = (int)(child << kLevelShift) | cl;
+                sp += npass;
             }
-            unsigned m = (unsigned)((__ballot(pass) >> gshift) & 0xffffull);
-            int pos = lcnt + __popc(m & ((1u << gl) - 1u));
-            if (pass) list[pos] = (int)child | (ring << kRingShift);
-            lcnt += __popc(m);
-            const bool last = r == nround - 1;
-            if (wave_max_i(lcnt) > kFCap - kG || (last && wave_max_i(lcnt) > 0)) {
-                __syncthreads();
-                ring_sort(list, lcnt, reinterpret_cast<int *>(&sg) + grp * kFCap, gshift, gl);
-                __syncthreads();
-                if (dbg && lane == 0) atomicAdd(&dbg[4], 1ull);  // list scans
-                scan_list(st, list, lcnt, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes, qlo,
-                          qhi, sg, s_bc, lane, grp, gl, dbg);
-                lcnt = 0;
-                R2 = grp_max_d(valid ? st.kth.d : 0.0);
-                s_qk[lane] = valid ? st.kth.d : -1.0;
-                __syncthreads();
-            }
-        }
-    }
-    // ---- frontier overflow (a very wide search): scan every bucket
-    if (__ballot(overflow && gl == 0) != 0ull) {
-        if (lane == 0) atomicAdd(n_overflow, 1u);
-        int* list = s_fr[grp][0];
-        for (long long b0 = 0; b0 < nb; b0 += kFCap) {
-            int lcnt = 0;
-            if (gl == 0 && overflow) {
-                for (long long b = b0; b < min(nb, b0 + kFCap); b++)
-                    if (!(b >= slo && b <= shi)) list[lcnt++] = (int)b;
-            }
-            lcnt = __shfl(lcnt, gshift, 64);
             __syncthreads();
-            scan_list(st, list, lcnt, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes, qlo,
-                      qhi, sg, s_bc, lane, grp, gl, dbg);
+            dbg_cycles(dbg, 9, t_r, lane);
         }
+        const int lmax = wave_max_i(lcnt);
+        if (lmax > kFCap - kG || (!any && lmax > 0) || (nscan == 0 && lmax >= kFirstScan)) {
+            nscan++;
+            ring_sort(list, lcnt, reinterpret_cast<int *>(&sg) + grp * kFCap, gshift, gl);
+            __syncthreads();
+            if (dbg && lane == 0) atomicAdd(&dbg[4], 1ull);  // list scans
+            scan_list(st, list, lcnt, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes, sg, s_bc,
+                      s_qx, s_qy, s_qz, s_qk, lane, grp, gl, dbg);
+            lcnt = 0;
+            R2 = grp_max_d(valid ? st.kth.d : 0.0);
+        }
+        if (!any) break;
     }
-    // final merge
     if (dbg && lane == 0) atomicAdd(&dbg[5], 1ull);  // waves
-    if (__ballot(st.cnt > 0) != 0ull) {
-        flush_buffer(st.td, st.ti, st.cnt, s_bc, lane, px, py, pz, cx, cy, cz, cid);
-        st.kth = kth_of(st.td, st.ti, k);
-    }
+    dbg_cycles(dbg, 12, t_start, lane);
     if (!valid) return;
     const long long q = qid[qi];
     if (nbr) {
@@ -818,14 +846,12 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
     const bool subset = type != nullptr && cand_ptype != PGALF_TYPE_ALL;
     // bounding box of every particle (queries include all candidates)
     unsigned long long *mm = ws.alloc_n<unsigned long long>(6);
-    unsigned *novf = ws.alloc_n<unsigned>(1);
     const bool debug = getenv("PGALF_DEBUG") != nullptr;
-    unsigned long long *dbg = debug ? ws.alloc_n<unsigned long long>(8) : nullptr;
-    if (!mm || !novf || (debug && !dbg)) return hipErrorOutOfMemory;
-    if (dbg) PGALF_HIP_TRY(hipMemsetAsync(dbg, 0, 8 * sizeof(unsigned long long), s));
+    unsigned long long *dbg = debug ? ws.alloc_n<unsigned long long>(kDbgN) : nullptr;
+    if (!mm || (debug && !dbg)) return hipErrorOutOfMemory;
+    if (dbg) PGALF_HIP_TRY(hipMemsetAsync(dbg, 0, kDbgN * sizeof(unsigned long long), s));
     PGALF_HIP_TRY(hipMemsetAsync(mm, 0xff, 3 * sizeof(unsigned long long), s));
     PGALF_HIP_TRY(hipMemsetAsync(mm + 3, 0x00, 3 * sizeof(unsigned long long), s));
-    PGALF_HIP_TRY(hipMemsetAsync(novf, 0, sizeof(unsigned), s));
     bbox_k<P><<<256, 256, 0, s>>>(x, y, z, np, mm);
 
     // queries: all particles, Hilbert order
@@ -878,7 +904,7 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
     lv.nb = nb;
     lv.nlev = 1;
     while (level_count(nb, lv.nlev - 1) > 1) lv.nlev++;
-    if (lv.nlev > kMaxLevels) return hipErrorInvalidValue;
+    if (lv.nlev > kMaxLevels || lv.nlev > kMaxWalkLevels) return hipErrorInvalidValue;
     long long total = level_offset(nb, lv.nlev);
     Box *boxes = ws.alloc_n<Box>((size_t)total);
     if (!boxes) return hipErrorOutOfMemory;
@@ -893,20 +919,21 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
     timing_mark("knn_query", s, false);
     long long nqb = (np + kWave - 1) / kWave;
     knn_query_k<P><<<(unsigned)nqb, kWave, 0, s>>>(qx, qy, qz, qv, np, cx, cy, cz, cid, nc,
-                                                   boxes, lv, k, nbr, rho, rho_mode, m, novf,
+                                                   boxes, lv, k, nbr, rho, rho_mode, m,
                                                    subset ? 0 : 1, dbg);
     timing_mark("knn_query", s, true);
     if (debug) {
-        unsigned h = 0;
-        unsigned long long d[8];
-        PGALF_HIP_TRY(hipMemcpyAsync(&h, novf, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        unsigned long long d[kDbgN];
         PGALF_HIP_TRY(hipMemcpyAsync(d, dbg, sizeof(d), hipMemcpyDeviceToHost, s));
         PGALF_HIP_TRY(hipStreamSynchronize(s));
         double w = d[5] ? (double)d[5] : 1.0;
-        fprintf(stderr, "[pgalf] knn: np=%lld nc=%lld buckets=%lld levels=%d overflows=%u | per "
-                        "wave: scan batches %.1f, mid-scan merges %.1f, internal rounds "
-                        "%.1f, level-1 rounds %.1f, list scans %.1f\n",
-                np, nc, nb, lv.nlev, h, d[0] / w, d[1] / w, d[2] / w, d[3] / w, d[4] / w);
+        fprintf(stderr, "[pgalf] knn: np=%lld nc=%lld buckets=%lld levels=%d | per "
+                        "wave: scan batches %.1f, mid-scan merges %.1f, walk rounds %.1f, "
+                        "list scans %.1f, end merges %.1f | kcycles per wave: seed %.1f, walk %.1f, "
+                        "scan %.1f (merges %.1f), total %.1f\n",
+                np, nc, nb, lv.nlev, d[0] / w, d[1] / w, d[2] / w, d[4] / w, d[6] / w,
+                d[8] / w / 1e3, d[9] / w / 1e3, d[10] / w / 1e3, d[11] / w / 1e3,
+                d[12] / w / 1e3);
     }
     return hipGetLastError();
 }
