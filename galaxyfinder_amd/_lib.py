@@ -7,7 +7,9 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libpgalf_density.so")
+# PGALF_LIB selects another build of the library (e.g. one with tuned
+# compile-time capacities); the default is the in-tree build.
+LIB_PATH = os.environ.get("PGALF_LIB") or os.path.join(_HERE, "libpgalf_density.so")
 
 _lib = None
 

@@ -29,29 +29,32 @@ CFLAGS = [
 ]
 
 
-def _compile(src, out_dir):
+def _compile(src, out_dir, extra):
     obj = os.path.join(out_dir, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + CFLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC] + CFLAGS + list(extra) + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (src, " ".join(cmd), r.stderr))
     return obj
 
 
-def build(verbose=False):
-    out_dir = os.path.join(HERE, "_build")
+def build(verbose=False, extra_flags=(), lib=LIB):
+    """Compile every source and link `lib`.  `extra_flags` go to every hipcc
+    compile (e.g. -DPGALF_KNN_BUFCAP=32 for the kNN capacities)."""
+    tag = "" if lib == LIB else "_" + os.path.splitext(os.path.basename(lib))[0]
+    out_dir = os.path.join(HERE, "_build" + tag)
     os.makedirs(out_dir, exist_ok=True)
     with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
-        objs = list(ex.map(lambda s: _compile(s, out_dir), SOURCES))
-    tmp = LIB + ".tmp"
+        objs = list(ex.map(lambda s: _compile(s, out_dir, extra_flags), SOURCES))
+    tmp = lib + ".tmp"
     cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr))
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
     if verbose:
-        print("built", LIB)
-    return LIB
+        print("built", lib)
+    return lib
 
 
 if __name__ == "__main__":

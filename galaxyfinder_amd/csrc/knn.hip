@@ -41,7 +41,18 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kTop = 32;     // list capacity (k <= 32, MAX_NUM_NEAR tree.h:20)
 constexpr int kBuf = 8;      // merge chunk (sorted, then merged into the top-32)
-constexpr int kBufCap = 48;  // per-lane LDS append buffer (sorted-candidate indices)
+// Capacities below can be set at compile time (-DPGALF_KNN_...): they trade
+// LDS per wavefront (occupancy) against merge and scan frequency.
+#ifndef PGALF_KNN_BUFCAP
+#define PGALF_KNN_BUFCAP 32
+#endif
+#ifndef PGALF_KNN_LISTCAP
+#define PGALF_KNN_LISTCAP 64
+#endif
+#ifndef PGALF_KNN_BATCH
+#define PGALF_KNN_BATCH 2
+#endif
+constexpr int kBufCap = PGALF_KNN_BUFCAP;  // per-lane LDS append buffer (sorted-candidate indices)
 constexpr int kChunkC = 8;   // candidates between overflow checks
 constexpr int kMaxLevels = 24;
 constexpr int kB = 8;        // candidates per bucket
@@ -52,11 +63,11 @@ constexpr int kFan = 8;      // box-tree fan-out
 constexpr int kPar = kG / kFan;  // parents expanded per walk round
 constexpr int kSeedFill = kTop / kB;  // buckets of the direct seed fill
 constexpr int kSeedSpan = 2 * kSeedFill;  // seed buckets in all
-constexpr int kFCap = 128;   // bucket-list capacity per group
+constexpr int kFCap = PGALF_KNN_LISTCAP;  // bucket-list capacity per group
 constexpr int kStackCap = 96;  // depth-first walk stack per group
 constexpr int kFirstScan = 16;  // the first list is scanned early to tighten the bounds
 constexpr int kLevelShift = 5;  // stack entry = node << 5 | level
-constexpr int kBatch = 4;    // buckets staged in LDS per scan batch
+constexpr int kBatch = PGALF_KNN_BATCH;  // buckets staged in LDS per scan batch
 constexpr int kStage = kBatch * kB;  // staged candidates per group
 constexpr int kStageStride = 2 * kStage + 1;  // 16-byte units per group: groups hit other banks
 constexpr int kRings = 8;    // distance classes of the near-first bucket order
@@ -66,6 +77,9 @@ static_assert(kG == kB, "a scan batch loads one candidate of each bucket per gro
 static_assert(kG % kFan == 0 && kPar >= 1, "a walk round expands whole parents");
 static_assert(kTop % kB == 0, "the direct seed fill takes whole buckets");
 static_assert(kPar == 1, "the depth-first walk expands one parent per round");
+static_assert(kBufCap % 8 == 0 && kBufCap >= 24, "buffer holds whole merge chunks");
+static_assert(kBatch >= 1 && kBatch <= kG, "one group lane tests each bucket of a batch");
+static_assert(kFCap >= 32 + 8 && kFCap % 8 == 0, "list must take the first scan and a round");
 // The walk keeps at most kFan entries of the deepest level and kFan-1 of every
 // level above it on the stack.
 constexpr int stack_need(int nlev) { return kFan + (kFan - 1) * (nlev > 3 ? nlev - 3 : 0); }
