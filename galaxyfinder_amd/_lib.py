@@ -50,6 +50,12 @@ SIGNATURES = {
                                        ctypes.POINTER(pgalf_grid_t), _c_f, _c_i, _c_p, _c_p,
                                        _c_p, _c_p, _c_p]),
     "pgalf_dev_status": (_c_i, [_c_p]),
+    "pgalf_batch_create": (_c_p, [_c_i, _c_p, _c_p, _c_p, _c_i]),
+    "pgalf_batch_grid_offset": (_c_ll, [_c_p, _c_i]),
+    "pgalf_batch_run": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i,
+                               _c_p]),
+    "pgalf_batch_status": (_c_i, [_c_p, _c_p]),
+    "pgalf_batch_destroy": (None, [_c_p]),
     "pgalf_timing_enable": (None, [_c_i]),
     "pgalf_timing_reset": (None, []),
     "pgalf_timing_get": (_c_i, [ctypes.c_char_p, ctypes.POINTER(_c_d), ctypes.POINTER(_c_i)]),

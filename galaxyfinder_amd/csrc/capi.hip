@@ -88,9 +88,12 @@ struct PendingEv {
 static std::vector<PendingEv> g_t_pending;
 static std::map<std::string, std::pair<double, int>> g_t_total;
 
+static thread_local bool t_t_suspended = false;
+void timing_suspend(bool on) { t_t_suspended = on; }
+
 void timing_mark(const char *phase, hipStream_t s, bool end) {
     std::lock_guard<std::mutex> lk(g_t_mu);
-    if (!g_t_on) return;
+    if (!g_t_on || t_t_suspended) return;
     if (!end) {
         PendingEv p;
         p.name = phase;
@@ -160,9 +163,9 @@ static_assert(offsetof(pgalf_particle_t, bp) == 80, "layout");
 // ---- status ----------------------------------------------------------------------
 static thread_local int t_err = PGALF_OK;
 static thread_local char t_msg[256] = "ok";
-static std::mutex g_api_mu;  // host entry points share one workspace per device
+std::mutex g_api_mu;  // host entry points share one workspace per device
 
-static int set_err(int code, const char *fmt, ...) {
+int set_err(int code, const char *fmt, ...) {
     t_err = code;
     va_list ap;
     va_start(ap, fmt);
@@ -170,12 +173,12 @@ static int set_err(int code, const char *fmt, ...) {
     va_end(ap);
     return code;
 }
-static int ok() {
+int ok() {
     t_err = PGALF_OK;
     snprintf(t_msg, sizeof(t_msg), "ok");
     return PGALF_OK;
 }
-static int hip_err(hipError_t e, const char *where) {
+int hip_err(hipError_t e, const char *where) {
     if (e == hipErrorOutOfMemory) return set_err(PGALF_ERR_NOMEM, "%s: out of device memory", where);
     if (e == hipErrorNotReady)
         return set_err(PGALF_ERR_SHORT_LIST, "%s: fewer candidates than neighbours", where);
@@ -210,7 +213,7 @@ extern "C" int pgalf_timing_get(const char *phase, double *total_ms, int *count)
 }
 
 // Read and clear the device status word (synchronises s).
-static int consume_status(Workspace &ws, hipStream_t s, const char *where) {
+int consume_status(Workspace &ws, hipStream_t s, const char *where) {
     unsigned st = 0;
     hipError_t e = hipMemcpyAsync(&st, ws.status(), sizeof(unsigned), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);

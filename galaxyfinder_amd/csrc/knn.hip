@@ -32,6 +32,8 @@
 #include <cfloat>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
+#include <vector>
 
 #include "pgalf_internal.h"
 
@@ -91,10 +93,21 @@ struct Box {
 };
 
 // Implicit 8-ary box tree over nb leaf buckets: level l holds
-// ceil(nb / 8^l) nodes, stored after all lower levels.
-struct Levels {
+// ceil(nb / 8^l) nodes.  One tree per halo; the box of node j of level l is
+// boxes[loff[l] + j].
+struct KHalo {
+    long long c0;        // first candidate slot (a multiple of kB)
+    long long nc;        // candidates
+    long long q0, nq;    // query slots [q0, q0 + nq)
+    long long idx_base;  // subtracted from candidate ids in the output lists
+    long long nb;        // leaf buckets
     int nlev;
-    long long nb;
+    long long loff[kMaxWalkLevels];
+};
+// One wavefront of queries: kWave consecutive query slots of one halo.
+struct KWave {
+    int halo;
+    long long q0;
 };
 
 __host__ __device__ inline long long level_count(long long nb, int l) {
@@ -263,12 +276,9 @@ __device__ inline unsigned hilbert3(unsigned x, unsigned y, unsigned z) {
     return key;
 }
 
-// Hilbert keys of particles idx[0..n) (idx == nullptr: identity).
-template <class P>
-__global__ void hilbert_k(const P *x, const P *y, const P *z, const unsigned *idx, long long n,
-                          const unsigned long long *mm, unsigned *keys, unsigned *vals) {
-    long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
+// Hilbert key of point v on the 1024^3 lattice over the cube of the box mm
+// (3 ordered minima, 3 ordered maxima).
+__device__ inline unsigned hilbert_of(const double v[3], const unsigned long long *mm) {
     double lo[3], ext = 0.0;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
@@ -276,8 +286,6 @@ __global__ void hilbert_k(const P *x, const P *y, const P *z, const unsigned *id
         ext = fmax(ext, unord_d(mm[3 + a]) - lo[a]);
     }
     double sc = ext > 0.0 ? 1024.0 / ext : 0.0;
-    unsigned i = idx ? idx[s] : (unsigned)s;
-    double v[3] = {(double)x[i], (double)y[i], (double)z[i]};
     unsigned q[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
@@ -285,8 +293,140 @@ __global__ void hilbert_k(const P *x, const P *y, const P *z, const unsigned *id
         int qi = (int)t;
         q[a] = (unsigned)(qi < 0 ? 0 : (qi > 1023 ? 1023 : qi));
     }
-    keys[s] = hilbert3(q[0], q[1], q[2]);
+    return hilbert3(q[0], q[1], q[2]);
+}
+
+// Hilbert keys of particles idx[0..n) (idx == nullptr: identity).
+template <class P>
+__global__ void hilbert_k(const P *x, const P *y, const P *z, const unsigned *idx, long long n,
+                          const unsigned long long *mm, unsigned *keys, unsigned *vals) {
+    long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    unsigned i = idx ? idx[s] : (unsigned)s;
+    double v[3] = {(double)x[i], (double)y[i], (double)z[i]};
+    keys[s] = hilbert_of(v, mm);
     vals[s] = i;
+}
+
+// ---- batched preparation (one block per particle chunk of one halo) -------
+template <class P>
+__global__ void __launch_bounds__(256) bbox_batch_k(const P *x, const P *y, const P *z,
+                                                    const PChunk *chunks,
+                                                    unsigned long long *mm /* 6 per halo */) {
+    __shared__ double s_v[6][4];
+    const PChunk c = chunks[blockIdx.x];
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (long long i = c.b0 + threadIdx.x; i < c.b1; i += blockDim.x) {
+        double v[3] = {(double)x[i], (double)y[i], (double)z[i]};
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            lo[a] = fmin(lo[a], v[a]);
+            hi[a] = fmax(hi[a], v[a]);
+        }
+    }
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        lo[a] = wave_min_d(lo[a]);
+        hi[a] = wave_max_d(hi[a]);
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            s_v[a][w] = lo[a];
+            s_v[3 + a][w] = hi[a];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        int a = threadIdx.x;
+        double v = s_v[a][0];
+        for (int i = 1; i < (int)(blockDim.x >> 6); i++)
+            v = a < 3 ? fmin(v, s_v[a][i]) : fmax(v, s_v[a][i]);
+        unsigned long long *hm = mm + 6 * (long long)c.halo;
+        if (a < 3) atomicMin(&hm[a], ord_d(v));
+        else atomicMax(&hm[a], ord_d(v));
+    }
+}
+
+// Minima start at the largest ordered value, maxima at the smallest.
+__global__ void bbox_init_k(unsigned long long *mm, int nhalo) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 6LL * nhalo) mm[i] = (i % 6) < 3 ? ~0ull : 0ull;
+}
+
+// Hilbert key of every particle within its halo's own box, and its halo.
+template <class P>
+__global__ void __launch_bounds__(256) hilbert_batch_k(const P *x, const P *y, const P *z,
+                                                       const PChunk *chunks,
+                                                       const unsigned long long *mm,
+                                                       unsigned *keys, unsigned *vals,
+                                                       unsigned *halo_of) {
+    const PChunk c = chunks[blockIdx.x];
+    const unsigned long long *hm = mm + 6 * (long long)c.halo;
+    for (long long i = c.b0 + threadIdx.x; i < c.b1; i += blockDim.x) {
+        double v[3] = {(double)x[i], (double)y[i], (double)z[i]};
+        keys[i] = hilbert_of(v, hm);
+        vals[i] = (unsigned)i;
+        halo_of[i] = (unsigned)c.halo;
+    }
+}
+
+__global__ void halo_keys_k(const unsigned *vals, const unsigned *halo_of, long long n,
+                            unsigned *hk) {
+    long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) hk[s] = halo_of[vals[s]];
+}
+
+// (halo, Hilbert)-sorted particles into the padded candidate layout: halo
+// h's s-th particle goes to slot c0 + s; the pad slots up to the next
+// multiple of kB hold NaN coordinates and id 0xffffffff.
+template <class P>
+__global__ void gather_padded_k(const P *x, const P *y, const P *z, const unsigned *perm,
+                                const unsigned *hsorted, const KHalo *halos, long long n,
+                                double *sx, double *sy, double *sz, unsigned *sid) {
+    long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const KHalo &hd = halos[hsorted[s]];
+    const long long p = hd.c0 + (s - hd.idx_base);
+    const unsigned i = perm[s];
+    sx[p] = (double)x[i];
+    sy[p] = (double)y[i];
+    sz[p] = (double)z[i];
+    sid[p] = i;
+}
+
+__global__ void pad_slots_k(const KHalo *halos, int nhalo, double *sx, double *sy, double *sz,
+                            unsigned *sid) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nhalo * kB) return;
+    const KHalo &hd = halos[t / kB];
+    const long long j = hd.nc + t % kB;
+    if (j >= hd.nb * kB) return;
+    const double nan = __longlong_as_double(0x7ff8000000000000ll);
+    sx[hd.c0 + j] = nan;
+    sy[hd.c0 + j] = nan;
+    sz[hd.c0 + j] = nan;
+    sid[hd.c0 + j] = 0xffffffffu;
+}
+
+// Internal nodes [b0, b1) of one level: union of their children's boxes.
+__global__ void node_boxes_k(Box *boxes, long long b0, long long b1, long long nleaf,
+                             const long long *child_begin, const unsigned char *child_cnt) {
+    long long b = b0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= b1) return;
+    const long long t = b - nleaf;
+    const long long c0 = child_begin[t];
+    Box a = boxes[c0];
+    for (long long c = c0 + 1; c < c0 + child_cnt[t]; c++) {
+        Box q = boxes[c];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            a.lo[d] = fmin(a.lo[d], q.lo[d]);
+            a.hi[d] = fmax(a.hi[d], q.hi[d]);
+        }
+    }
+    boxes[b] = a;
 }
 
 template <class P>
@@ -521,7 +661,7 @@ __device__ inline bool union_pass(const Box &b, int grp, const double *s_qx, con
 __device__ inline void scan_list(QState &st, const int *list, int count, bool valid, int k,
                                  double px, double py, double pz, const double *cx,
                                  const double *cy, const double *cz, const unsigned *cid,
-                                 long long nc, const Box *boxes, Stage &sg,
+                                 long long nc, const Box *leaves, Stage &sg,
                                  unsigned (*s_bc)[kWave], double *s_qx, double *s_qy,
                                  double *s_qz, double *s_qk, int lane, int grp, int gl,
                                  unsigned long long *dbg) {
@@ -532,7 +672,7 @@ __device__ inline void scan_list(QState &st, const int *list, int count, bool va
         // t tests bucket i0 + t)
         bool bpass = false;
         if (gl < kBatch && i0 + gl < count)
-            bpass = union_pass(boxes[list[i0 + gl] & kBucketMask], grp, s_qx, s_qy, s_qz, s_qk);
+            bpass = union_pass(leaves[list[i0 + gl] & kBucketMask], grp, s_qx, s_qy, s_qz, s_qk);
         const unsigned bm = (unsigned)((__ballot(bpass) >> (grp * kG)) & kGMask);
 #pragma unroll
         for (int t = 0; t < kBatch; t++) {
@@ -629,8 +769,8 @@ __device__ inline void ring_sort(int *list, int count, int *tmp, int gshift, int
 template <class P>
 __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2)))
 knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned *qid,
-            long long nq, const double *cx, const double *cy, const double *cz,
-            const unsigned *cid, long long nc, const Box *boxes, Levels lv, int k,
+            const double *cxa, const double *cya, const double *cza, const unsigned *cida,
+            const Box *boxes, const KHalo *halos, KHalo halo0, const KWave *waves, int k,
             long long *nbr, float *rho, int rho_mode, const P *mass,
             int self_seed, unsigned long long *dbg) {
     __shared__ unsigned s_bc[kBufCap][kWave];
@@ -643,12 +783,27 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
     const unsigned long long t_start = dbg_clock(dbg);
     const int lane = threadIdx.x;
     const int grp = lane / kG, gl = lane % kG, gshift = grp * kG;
-    if (lane < lv.nlev) {
-        s_off[lane] = level_offset(lv.nb, lane);
-        s_n[lane] = level_count(lv.nb, lane);
+    // this wave's halo: its candidates, queries and tree
+    KWave wv;
+    if (waves) {
+        wv = waves[blockIdx.x];
+    } else {
+        wv.halo = 0;
+        wv.q0 = (long long)blockIdx.x * kWave;
     }
-    const long long qi = (long long)blockIdx.x * kWave + lane;
-    const bool valid = qi < nq;
+    const KHalo &hd = halos ? halos[wv.halo] : halo0;
+    const long long nc = hd.nc, nb = hd.nb, hq0 = hd.q0, hq1 = hd.q0 + hd.nq;
+    const long long idx_base = hd.idx_base;
+    const int nlev = hd.nlev;
+    const double *cx = cxa + hd.c0, *cy = cya + hd.c0, *cz = cza + hd.c0;
+    const unsigned *cid = cida + hd.c0;
+    if (lane < nlev) {
+        s_off[lane] = hd.loff[lane];
+        s_n[lane] = level_count(nb, lane);
+    }
+    const Box *leaves = boxes + hd.loff[0];
+    const long long qi = wv.q0 + lane;
+    const bool valid = qi < hq1;
     double px = 0, py = 0, pz = 0;
     if (valid) {
         px = qx[qi];
@@ -675,18 +830,17 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
     st.cnt = 0;
     __syncthreads();
     const double inf = __longlong_as_double(0x7ff0000000000000ll);
-    const long long nb = lv.nb;
 
     // ---- seed bucket.  Queries = candidates: the group's own bucket (same
     // sorted order).  Otherwise descend greedily to the child whose box is
     // nearest the group's centre point (ties: nearest box centre).
     long long node = 0;
     if (self_seed) {
-        node = ((long long)blockIdx.x * kWave + grp * kG) / kB;
+        node = (wv.q0 - hq0 + grp * kG) / kB;
     } else {
         const double gc[3] = {0.5 * (qlo[0] + qhi[0]), 0.5 * (qlo[1] + qhi[1]),
                               0.5 * (qlo[2] + qhi[2])};
-        for (int l = lv.nlev - 1; l > 0; l--) {
+        for (int l = nlev - 1; l > 0; l--) {
             long long child = kFan * node + (gl & (kFan - 1));
             bool ok = gl < kFan && child < s_n[l - 1];
             double g2 = inf, c2 = inf;
@@ -745,7 +899,7 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
     s_qz[lane] = pz;
     s_qk[lane] = valid ? st.kth.d : -1.0;
     __syncthreads();
-    scan_list(st, s_stk[grp], scnt, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes, sg, s_bc,
+    scan_list(st, s_stk[grp], scnt, valid, k, px, py, pz, cx, cy, cz, cid, nc, leaves, sg, s_bc,
               s_qx, s_qy, s_qz, s_qk, lane, grp, gl, dbg);
     double R2 = grp_max_d(valid ? st.kth.d : 0.0);
     dbg_cycles(dbg, 8, t_start, lane);
@@ -761,8 +915,8 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
     int *list = s_list[grp];
     int *stk = s_stk[grp];
     int sp = 0, lcnt = 0, nscan = 0;
-    if (gvalid && lv.nlev > 1) {
-        if (gl == 0) stk[0] = lv.nlev - 1;  // root: node 0 of the top level
+    if (gvalid && nlev > 1) {
+        if (gl == 0) stk[0] = nlev - 1;  // root: node 0 of the top level
         sp = 1;
     }
     __syncthreads();
@@ -813,7 +967,7 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
             ring_sort(list, lcnt, reinterpret_cast<int *>(&sg) + grp * kFCap, gshift, gl);
             __syncthreads();
             if (dbg && lane == 0) atomicAdd(&dbg[4], 1ull);  // list scans
-            scan_list(st, list, lcnt, valid, k, px, py, pz, cx, cy, cz, cid, nc, boxes, sg, s_bc,
+            scan_list(st, list, lcnt, valid, k, px, py, pz, cx, cy, cz, cid, nc, leaves, sg, s_bc,
                       s_qx, s_qy, s_qz, s_qk, lane, grp, gl, dbg);
             lcnt = 0;
             R2 = grp_max_d(valid ? st.kth.d : 0.0);
@@ -828,7 +982,7 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
         long long *row = nbr + q * k;
 #pragma unroll
         for (int t = 0; t < kTop; t++)
-            if (t < k) row[t] = (long long)st.ti[t];
+            if (t < k) row[t] = (long long)st.ti[t] - idx_base;
     }
     if (rho) {
         float neardist = sqrtf((float)st.kth.d);
@@ -911,19 +1065,25 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
         return hipErrorNotReady;
     }
 
-    // 8-ary box tree over 16-candidate buckets
-    Levels lv;
+    // 8-ary box tree over kB-candidate buckets
+    KHalo h0{};
     long long nb = (nc + kB - 1) / kB;
     if (nb >= (1LL << kRingShift)) return hipErrorInvalidValue;
-    lv.nb = nb;
-    lv.nlev = 1;
-    while (level_count(nb, lv.nlev - 1) > 1) lv.nlev++;
-    if (lv.nlev > kMaxLevels || lv.nlev > kMaxWalkLevels) return hipErrorInvalidValue;
-    long long total = level_offset(nb, lv.nlev);
+    h0.c0 = 0;
+    h0.nc = nc;
+    h0.q0 = 0;
+    h0.nq = np;
+    h0.idx_base = 0;
+    h0.nb = nb;
+    h0.nlev = 1;
+    while (level_count(nb, h0.nlev - 1) > 1) h0.nlev++;
+    if (h0.nlev > kMaxLevels || h0.nlev > kMaxWalkLevels) return hipErrorInvalidValue;
+    for (int l = 0; l < h0.nlev; l++) h0.loff[l] = level_offset(nb, l);
+    long long total = level_offset(nb, h0.nlev);
     Box *boxes = ws.alloc_n<Box>((size_t)total);
     if (!boxes) return hipErrorOutOfMemory;
     leaf_boxes_k<<<nblk(nb, 256), 256, 0, s>>>(cx, cy, cz, nc, nb, boxes);
-    for (int l = 1; l < lv.nlev; l++) {
+    for (int l = 1; l < h0.nlev; l++) {
         long long nl = level_count(nb, l);
         parent_boxes_k<<<nblk(nl, 256), 256, 0, s>>>(boxes + level_offset(nb, l - 1),
                                                      level_count(nb, l - 1),
@@ -932,9 +1092,9 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
     timing_mark("knn_prep", s, true);
     timing_mark("knn_query", s, false);
     long long nqb = (np + kWave - 1) / kWave;
-    knn_query_k<P><<<(unsigned)nqb, kWave, 0, s>>>(qx, qy, qz, qv, np, cx, cy, cz, cid, nc,
-                                                   boxes, lv, k, nbr, rho, rho_mode, m,
-                                                   subset ? 0 : 1, dbg);
+    knn_query_k<P><<<(unsigned)nqb, kWave, 0, s>>>(qx, qy, qz, qv, cx, cy, cz, cid, boxes,
+                                                   nullptr, h0, nullptr, k, nbr, rho, rho_mode,
+                                                   m, subset ? 0 : 1, dbg);
     timing_mark("knn_query", s, true);
     if (debug) {
         unsigned long long d[kDbgN];
@@ -945,12 +1105,166 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
                         "wave: scan batches %.1f, mid-scan merges %.1f, walk rounds %.1f, "
                         "list scans %.1f, end merges %.1f | kcycles per wave: seed %.1f, walk %.1f, "
                         "scan %.1f (merges %.1f), total %.1f\n",
-                np, nc, nb, lv.nlev, d[0] / w, d[1] / w, d[2] / w, d[4] / w, d[6] / w,
+                np, nc, nb, h0.nlev, d[0] / w, d[1] / w, d[2] / w, d[4] / w, d[6] / w,
                 d[8] / w / 1e3, d[9] / w / 1e3, d[10] / w / 1e3, d[11] / w / 1e3,
                 d[12] / w / 1e3);
     }
     return hipGetLastError();
 }
+
+hipError_t knn_batch_plan(const long long *off, int nhalo, int k, KnnBatchPlan &plan) {
+    if (nhalo < 1 || k < 1 || k > kTop || off[0] != 0) return hipErrorInvalidValue;
+    KnnBatchPlan p;
+    p.nhalo = nhalo;
+    p.k = k;
+    p.np = off[nhalo];
+    if (p.np >= (1LL << 32)) return hipErrorInvalidValue;  // u32 particle ids
+    p.hbits = 0;
+    while ((1LL << p.hbits) < nhalo) p.hbits++;
+    std::vector<KHalo> hs((size_t)nhalo);
+    // leaves in padded candidate order
+    long long c0 = 0, maxlev = 1;
+    for (int h = 0; h < nhalo; h++) {
+        const long long n = off[h + 1] - off[h];
+        if (n < k) return hipErrorNotReady;  // mapped to PGALF_ERR_SHORT_LIST
+        KHalo &hd = hs[h];
+        hd.c0 = c0;
+        hd.nc = n;
+        hd.q0 = c0;
+        hd.nq = n;
+        hd.idx_base = off[h];
+        hd.nb = (n + kB - 1) / kB;
+        if (hd.nb >= (1LL << kRingShift)) return hipErrorInvalidValue;
+        hd.nlev = 1;
+        while (level_count(hd.nb, hd.nlev - 1) > 1) hd.nlev++;
+        if (hd.nlev > kMaxWalkLevels) return hipErrorInvalidValue;
+        hd.loff[0] = c0 / kB;
+        c0 += hd.nb * kB;
+        if (hd.nlev > maxlev) maxlev = hd.nlev;
+    }
+    p.npad = c0;
+    p.nb_total = c0 / kB;
+    // internal levels, level-major
+    std::vector<long long> cbeg;
+    std::vector<unsigned char> ccnt;
+    long long b = p.nb_total;
+    for (int l = 1; l < maxlev; l++) {
+        for (int h = 0; h < nhalo; h++) {
+            KHalo &hd = hs[h];
+            if (l >= hd.nlev) continue;
+            hd.loff[l] = b;
+            const long long nl = level_count(hd.nb, l), nchild = level_count(hd.nb, l - 1);
+            for (long long j = 0; j < nl; j++) {
+                const long long cfirst = kFan * j;
+                const long long clast = cfirst + kFan < nchild ? cfirst + kFan : nchild;
+                cbeg.push_back(hd.loff[l - 1] + cfirst);
+                ccnt.push_back((unsigned char)(clast - cfirst));
+            }
+            b += nl;
+        }
+        p.level_begin.push_back(b);
+    }
+    p.nboxes = b;
+    // waves, largest halos first (their waves run longest)
+    std::vector<int> order((size_t)nhalo);
+    for (int h = 0; h < nhalo; h++) order[h] = h;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int c) { return hs[a].nc > hs[c].nc; });
+    std::vector<KWave> wv;
+    for (int h : order)
+        for (long long q = 0; q < hs[h].nc; q += kWave) {
+            KWave w;
+            w.halo = h;
+            w.q0 = hs[h].c0 + q;
+            wv.push_back(w);
+        }
+    p.nwaves = (long long)wv.size();
+    hipError_t e = hipMalloc(&p.halos, sizeof(KHalo) * hs.size());
+    if (e == hipSuccess) e = hipMalloc(&p.waves, sizeof(KWave) * wv.size());
+    const size_t ni = cbeg.size() ? cbeg.size() : 1;
+    if (e == hipSuccess) e = hipMalloc(&p.child_begin, sizeof(long long) * ni);
+    if (e == hipSuccess) e = hipMalloc(&p.child_cnt, ni);
+    if (e == hipSuccess)
+        e = hipMemcpy(p.halos, hs.data(), sizeof(KHalo) * hs.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(p.waves, wv.data(), sizeof(KWave) * wv.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !cbeg.empty())
+        e = hipMemcpy(p.child_begin, cbeg.data(), sizeof(long long) * cbeg.size(),
+                      hipMemcpyHostToDevice);
+    if (e == hipSuccess && !ccnt.empty())
+        e = hipMemcpy(p.child_cnt, ccnt.data(), ccnt.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        knn_batch_plan_free(p);
+        return e;
+    }
+    plan = p;
+    return hipSuccess;
+}
+
+void knn_batch_plan_free(KnnBatchPlan &plan) {
+    if (plan.halos) (void)hipFree(plan.halos);
+    if (plan.waves) (void)hipFree(plan.waves);
+    if (plan.child_begin) (void)hipFree(plan.child_begin);
+    if (plan.child_cnt) (void)hipFree(plan.child_cnt);
+    plan = KnnBatchPlan();
+}
+
+template <class P>
+hipError_t knn_batch(const P *x, const P *y, const P *z, const P *m, const BatchDev &b,
+                     const KnnBatchPlan &plan, long long *nbr, float *rho, int rho_mode,
+                     Workspace &ws, hipStream_t s) {
+    const long long np = plan.np, npad = plan.npad;
+    if (np != b.np || plan.nhalo != b.nhalo) return hipErrorInvalidValue;
+    timing_mark("knn_prep", s, false);
+    const KHalo *halos = static_cast<const KHalo *>(plan.halos);
+    unsigned long long *mm = ws.alloc_n<unsigned long long>(6 * (size_t)plan.nhalo);
+    unsigned *hk = ws.alloc_n<unsigned>(np), *hv = ws.alloc_n<unsigned>(np);
+    unsigned *hk2 = ws.alloc_n<unsigned>(np), *hv2 = ws.alloc_n<unsigned>(np);
+    unsigned *halo_of = ws.alloc_n<unsigned>(np);
+    double *sx = ws.alloc_n<double>(npad), *sy = ws.alloc_n<double>(npad),
+           *sz = ws.alloc_n<double>(npad);
+    unsigned *sid = ws.alloc_n<unsigned>(npad);
+    Box *boxes = ws.alloc_n<Box>((size_t)plan.nboxes);
+    if (!mm || !hk || !hv || !hk2 || !hv2 || !halo_of || !sx || !sy || !sz || !sid || !boxes)
+        return hipErrorOutOfMemory;
+    bbox_init_k<<<nblk(6LL * plan.nhalo, 256), 256, 0, s>>>(mm, plan.nhalo);
+    bbox_batch_k<P><<<(unsigned)b.nchunks, 256, 0, s>>>(x, y, z, b.chunks, mm);
+    hilbert_batch_k<P><<<(unsigned)b.nchunks, 256, 0, s>>>(x, y, z, b.chunks, mm, hk, hv,
+                                                          halo_of);
+    // stable sorts: by Hilbert key, then by halo -> (halo, Hilbert, index) order
+    PGALF_HIP_TRY(radix_sort_pairs(&hk, &hv, hk2, hv2, np, 30, ws, s));
+    unsigned *k2 = hk == hk2 ? ws.alloc_n<unsigned>(np) : hk2;
+    unsigned *k2t = ws.alloc_n<unsigned>(np), *v2t = hv == hv2 ? ws.alloc_n<unsigned>(np) : hv2;
+    if (!k2 || !k2t || !v2t) return hipErrorOutOfMemory;
+    halo_keys_k<<<nblk(np, 256), 256, 0, s>>>(hv, halo_of, np, k2);
+    if (plan.hbits > 0) PGALF_HIP_TRY(radix_sort_pairs(&k2, &hv, k2t, v2t, np, plan.hbits, ws, s));
+    gather_padded_k<P><<<nblk(np, 256), 256, 0, s>>>(x, y, z, hv, k2, halos, np, sx, sy, sz, sid);
+    pad_slots_k<<<nblk((long long)plan.nhalo * kB, 256), 256, 0, s>>>(halos, plan.nhalo, sx, sy,
+                                                                      sz, sid);
+    leaf_boxes_k<<<nblk(plan.nb_total, 256), 256, 0, s>>>(sx, sy, sz, npad, plan.nb_total,
+                                                          boxes);
+    long long lb = plan.nb_total;
+    for (long long le : plan.level_begin) {
+        node_boxes_k<<<nblk(le - lb, 256), 256, 0, s>>>(boxes, lb, le, plan.nb_total,
+                                                        plan.child_begin, plan.child_cnt);
+        lb = le;
+    }
+    timing_mark("knn_prep", s, true);
+    timing_mark("knn_query", s, false);
+    KHalo h0{};
+    knn_query_k<P><<<(unsigned)plan.nwaves, kWave, 0, s>>>(
+        sx, sy, sz, sid, sx, sy, sz, sid, boxes, halos, h0,
+        static_cast<const KWave *>(plan.waves), plan.k, nbr, rho, rho_mode, m, 1, nullptr);
+    timing_mark("knn_query", s, true);
+    return hipGetLastError();
+}
+
+template hipError_t knn_batch<float>(const float *, const float *, const float *, const float *,
+                                     const BatchDev &, const KnnBatchPlan &, long long *,
+                                     float *, int, Workspace &, hipStream_t);
+template hipError_t knn_batch<double>(const double *, const double *, const double *,
+                                      const double *, const BatchDev &, const KnnBatchPlan &,
+                                      long long *, float *, int, Workspace &, hipStream_t);
 
 template hipError_t knn<float>(const float *, const float *, const float *, const float *,
                                long long, const int *, int, int, long long *, float *, int,

@@ -22,6 +22,37 @@ struct Grid {
     __host__ __device__ long long ncells() const { return mx() * (long long)ny * nz; }
 };
 
+// One halo of a batch (a catalogue processed in one pass).  Its particles
+// are [p0, p0 + np) of the concatenated SoA arrays, its padded grid starts at
+// float goff of the batch grid buffer and its deposit tiles at toff of the
+// batch tile space.  The single-halo entry points use one HaloSeg with
+// p0 = goff = toff = 0.
+struct HaloSeg {
+    long long p0, np;
+    long long goff;
+    long long toff;
+    Grid g;
+};
+
+// A block of at most kChunkParticles particles that all belong to one halo.
+struct PChunk {
+    int halo;
+    long long b0, b1;
+};
+constexpr int kChunkParticles = 8192;
+
+// Device view of a batch: descriptors built on the host once per catalogue.
+struct BatchDev {
+    int nhalo = 0;
+    long long np = 0;               // particles in all halos
+    const HaloSeg *segs = nullptr;  // [nhalo]
+    const PChunk *chunks = nullptr; // [nchunks]
+    long long nchunks = 0;
+    const unsigned *tile_halo = nullptr;  // [ntiles] halo of each deposit tile
+    long long ntiles = 0;
+    long long max_halo_tiles = 0;
+};
+
 // Device error bits raised by kernels (read back with the status word).
 enum : unsigned {
     DEV_ERR_OUT_OF_GRID = 1u,
@@ -63,6 +94,9 @@ hipError_t exclusive_scan_u32(const unsigned *in, unsigned *out, long long n,
 hipError_t radix_sort_pairs(unsigned **keys, unsigned **vals, unsigned *keys_tmp,
                             unsigned *vals_tmp, long long n, int bits,
                             Workspace &ws, hipStream_t s);
+// Fill n words with v (a kernel, not hipMemsetAsync: graph-capture safe).
+hipError_t fill_u32(unsigned *p, long long n, unsigned v, hipStream_t s);
+hipError_t fill_u64(unsigned long long *p, long long n, unsigned long long v, hipStream_t s);
 
 // ---- TSC deposit / read-back (tsc.hip) ----------------------------------
 template <class P>
@@ -73,10 +107,36 @@ template <class P>
 hipError_t tsc_readback(const P *x, const P *y, const P *z, long long np,
                         const Grid &g, const float *den, float *out,
                         Workspace &ws, hipStream_t s);
+// Batched forms: every halo of `b` deposited into / read back from its own
+// grid of the batch grid buffer `den` (HaloSeg::goff).  No host
+// synchronisation, no memset: safe to capture in a HIP graph.
+template <class P>
+hipError_t tsc_deposit_batch(const P *x, const P *y, const P *z, const P *m,
+                             const BatchDev &b, float *den, Workspace &ws,
+                             hipStream_t s);
+template <class P>
+hipError_t tsc_readback_batch(const P *x, const P *y, const P *z, const BatchDev &b,
+                              const float *den, float *out, Workspace &ws,
+                              hipStream_t s);
+// Deposit tiles of one grid (16^3 nodes over the padded mx x ny x nz box).
+long long tsc_tile_count(const Grid &g);
 
 // ---- Gaussian smoothing (smooth.hip) ------------------------------------
 hipError_t gaussian_smooth(float *den, const Grid &g, float RG, Workspace &ws,
                            hipStream_t s);
+// -(pi RG/h)^2 with RG/h rounded to float as gsmooth.c:67-69 does.
+double smooth_expfac(float RG, double cellsize);
+// Grids of one shape and one smoothing factor, smoothed together.
+struct SmoothGroup {
+    int nx, ny, nz;
+    double expfac;
+    const long long *goffs;  // device [count]: first float of each grid
+    int count;
+};
+// Builds the group's FFT plans (host work and uploads; call before capture).
+hipError_t smooth_group_prepare(const SmoothGroup &grp);
+hipError_t gaussian_smooth_batch(float *den, const SmoothGroup *groups, int ngroups,
+                                 hipStream_t s);
 
 // ---- kNN (knn.hip) -------------------------------------------------------
 // Exact k nearest candidates of every query (self included when the query
@@ -90,14 +150,48 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np,
                const int *type, int cand_ptype, int k, long long *nbr,
                float *rho, int rho_mode, Workspace &ws, hipStream_t s);
 
+// Device tables of a batched kNN, built once per catalogue (knn_batch_plan):
+// per-halo trees in one box array (leaves of every halo first, in padded
+// candidate order, then the internal levels level by level) and the wave
+// table, largest halos first.
+struct KnnBatchPlan {
+    int nhalo = 0, k = 0, hbits = 0;
+    long long np = 0, npad = 0, nb_total = 0, nboxes = 0, nwaves = 0;
+    void *halos = nullptr;  // KHalo[nhalo] (knn.hip)
+    void *waves = nullptr;  // KWave[nwaves] (knn.hip)
+    long long *child_begin = nullptr;      // [nboxes - nb_total] first child box
+    unsigned char *child_cnt = nullptr;    // [nboxes - nb_total] children (1..8)
+    std::vector<long long> level_begin;    // internal level l: boxes [level_begin[l-1], level_begin[l])
+};
+// off: host [nhalo+1] particle offsets (off[0] = 0, every halo >= k particles).
+hipError_t knn_batch_plan(const long long *off, int nhalo, int k, KnnBatchPlan &plan);
+void knn_batch_plan_free(KnnBatchPlan &plan);
+// Exact k-NN lists (indices local to each halo) and W4 densities of every
+// particle among the particles of its own halo; no host synchronisation.
+template <class P>
+hipError_t knn_batch(const P *x, const P *y, const P *z, const P *m, const BatchDev &b,
+                     const KnnBatchPlan &plan, long long *nbr, float *rho, int rho_mode,
+                     Workspace &ws, hipStream_t s);
+
 // AoS SimpleBasicParticleType (88 B) -> SoA double arrays on device.
 hipError_t unpack_particles(const void *d_bp, long long np, double *x, double *y,
                             double *z, double *m, int *type, hipStream_t s);
 
 // ---- phase timing (capi.hip) ---------------------------------------------
 void timing_mark(const char *phase, hipStream_t s, bool end);
+// Phase timing is suspended on this thread while a graph is being captured.
+void timing_suspend(bool on);
 
 }  // namespace pgalf
+
+// ---- status of the C-ABI entry points (capi.hip) -----------------------------
+#include <mutex>
+extern std::mutex g_api_mu;  // serialises entry points sharing a device workspace
+int set_err(int code, const char *fmt, ...);
+int ok();
+int hip_err(hipError_t e, const char *where);
+// Read and clear the workspace's device status word (synchronises s).
+int consume_status(pgalf::Workspace &ws, hipStream_t s, const char *where);
 
 #define PGALF_HIP_TRY(expr)                       \
     do {                                          \

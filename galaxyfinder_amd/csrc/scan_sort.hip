@@ -184,14 +184,34 @@ __global__ void __launch_bounds__(kThreads) rs_scatter_k(const unsigned *keys,
     }
 }
 
+template <class T>
+__global__ void fill_k(T *p, long long n, T v) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+template <class T>
+hipError_t fill_any(T *p, long long n, T v, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    long long nb = (n + kThreads - 1) / kThreads;
+    if (nb > 4096) nb = 4096;
+    fill_k<T><<<dim3((unsigned)nb), dim3(kThreads), 0, s>>>(p, n, v);
+    return hipGetLastError();
+}
+
 }  // namespace
+
+hipError_t fill_u32(unsigned *p, long long n, unsigned v, hipStream_t s) {
+    return fill_any<unsigned>(p, n, v, s);
+}
+hipError_t fill_u64(unsigned long long *p, long long n, unsigned long long v, hipStream_t s) {
+    return fill_any<unsigned long long>(p, n, v, s);
+}
 
 hipError_t exclusive_scan_u32(const unsigned *in, unsigned *out, long long n,
                               unsigned *total, Workspace &ws, hipStream_t s) {
-    if (n <= 0) {
-        if (total) return hipMemsetAsync(total, 0, sizeof(unsigned), s);
-        return hipSuccess;
-    }
+    if (n <= 0) return total ? fill_u32(total, 1, 0u, s) : hipSuccess;
     long long nb = (n + kTile - 1) / kTile;
     unsigned *bsum = ws.alloc_n<unsigned>((size_t)nb);
     if (!bsum) return hipErrorOutOfMemory;

@@ -126,12 +126,15 @@ __device__ inline void fft_lines(double2 *buf, int L, int P, const double2 *tw, 
     if (Ns < L) stockham_pass<2>(buf, L, P, Ns, tw, inverse);
 }
 
+// blockIdx.x: batch of 2P lines; blockIdx.y: slow batch index; blockIdx.z:
+// grid of a batch of same-shape grids at offsets goffs (nullptr: one grid).
 __global__ void __launch_bounds__(kBlock)
-smooth_axis_k(float *g, AxisPass ap, const double2 *tw, const double *H) {
+smooth_axis_k(float *g, AxisPass ap, const double2 *tw, const double *H,
+              const long long *goffs) {
     extern __shared__ double2 buf[];
     const int L = ap.L, P = ap.P, n = ap.n;
     const long long ib0 = (long long)blockIdx.x * 2 * P;
-    float *base = g + (long long)blockIdx.y * ap.sc;
+    float *base = g + (goffs ? goffs[blockIdx.z] : 0LL) + (long long)blockIdx.y * ap.sc;
     const int lines = 2 * P;
     // load: line lb (0..2P-1) element j -> buf[(lb/2)*L + j].(x|y)
     const int tot = lines * n;
@@ -263,7 +266,8 @@ hipError_t get_plan(int n, double expfac, AxisPlan &out) {
 }
 
 hipError_t run_pass(float *g, const AxisPlan &pl, long long sa, int nb, long long sb, int nc,
-                    long long sc, int contiguous, int zero_pads, int padw, hipStream_t s) {
+                    long long sc, int contiguous, int zero_pads, int padw, hipStream_t s,
+                    const long long *goffs = nullptr, int ngrids = 1) {
     AxisPass ap;
     ap.n = pl.n;
     ap.L = pl.L;
@@ -279,30 +283,61 @@ hipError_t run_pass(float *g, const AxisPlan &pl, long long sa, int nb, long lon
     ap.contiguous = contiguous;
     ap.zero_pads = zero_pads;
     ap.padw = padw;
-    dim3 grid((unsigned)((nb + 2 * P - 1) / (2 * P)), (unsigned)nc);
     size_t lds = sizeof(double2) * (size_t)P * pl.L;
-    smooth_axis_k<<<grid, kBlock, lds, s>>>(g, ap, pl.tw, pl.H);
+    constexpr int kMaxZ = 65535;  // grid dimension limit: larger batches in several launches
+    for (int z0 = 0; z0 < ngrids; z0 += kMaxZ) {
+        const int nz = ngrids - z0 < kMaxZ ? ngrids - z0 : kMaxZ;
+        dim3 grid((unsigned)((nb + 2 * P - 1) / (2 * P)), (unsigned)nc, (unsigned)nz);
+        smooth_axis_k<<<grid, kBlock, lds, s>>>(g, ap, pl.tw, pl.H, goffs ? goffs + z0 : nullptr);
+    }
     return hipGetLastError();
+}
+
+// The three axis passes of one grid shape over `ngrids` grids.
+hipError_t smooth_passes(float *den, int nx, int ny, int nz, double expfac, hipStream_t s,
+                         const long long *goffs, int ngrids) {
+    AxisPlan px, py, pz;
+    PGALF_HIP_TRY(get_plan(nx, expfac, px));
+    PGALF_HIP_TRY(get_plan(ny, expfac, py));
+    PGALF_HIP_TRY(get_plan(nz, expfac, pz));
+    const long long mx = 2LL * (nx / 2 + 1);
+    // x: lines are rows (contiguous), batch over ny*nz rows
+    PGALF_HIP_TRY(run_pass(den, px, 1, ny * nz, mx, 1, 0, 1, 1, (int)mx, s, goffs, ngrids));
+    // y: lines along y (stride mx), batch over x (stride 1), one z per block row
+    PGALF_HIP_TRY(run_pass(den, py, mx, nx, 1, nz, mx * ny, 0, 0, 0, s, goffs, ngrids));
+    // z: lines along z (stride mx*ny), batch over x, one y per block row
+    PGALF_HIP_TRY(run_pass(den, pz, mx * ny, nx, 1, ny, mx, 0, 0, 0, s, goffs, ngrids));
+    return hipSuccess;
 }
 
 }  // namespace
 
+double smooth_expfac(float RG, double cellsize) {
+    // gsmooth.c:67-69: RG rescaled to cells in float, expfac in double.
+    float rgc = (float)((double)RG / cellsize);
+    return -(M_PI * (double)rgc) * (M_PI * (double)rgc);
+}
+
 hipError_t gaussian_smooth(float *den, const Grid &g, float RG, Workspace &ws, hipStream_t s) {
     (void)ws;
-    // gsmooth.c:67-69: RG rescaled to cells in float, expfac in double.
-    float rgc = (float)((double)RG / g.h);
-    double expfac = -(M_PI * (double)rgc) * (M_PI * (double)rgc);
-    AxisPlan px, py, pz;
-    PGALF_HIP_TRY(get_plan(g.nx, expfac, px));
-    PGALF_HIP_TRY(get_plan(g.ny, expfac, py));
-    PGALF_HIP_TRY(get_plan(g.nz, expfac, pz));
-    const long long mx = g.mx();
-    // x: lines are rows (contiguous), batch over ny*nz rows
-    PGALF_HIP_TRY(run_pass(den, px, 1, g.ny * g.nz, mx, 1, 0, 1, 1, (int)mx, s));
-    // y: lines along y (stride mx), batch over x (stride 1), one z per block row
-    PGALF_HIP_TRY(run_pass(den, py, mx, g.nx, 1, g.nz, mx * g.ny, 0, 0, 0, s));
-    // z: lines along z (stride mx*ny), batch over x, one y per block row
-    PGALF_HIP_TRY(run_pass(den, pz, mx * g.ny, g.nx, 1, g.ny, mx, 0, 0, 0, s));
+    return smooth_passes(den, g.nx, g.ny, g.nz, smooth_expfac(RG, g.h), s, nullptr, 1);
+}
+
+hipError_t smooth_group_prepare(const SmoothGroup &grp) {
+    AxisPlan p;
+    PGALF_HIP_TRY(get_plan(grp.nx, grp.expfac, p));
+    PGALF_HIP_TRY(get_plan(grp.ny, grp.expfac, p));
+    return get_plan(grp.nz, grp.expfac, p);
+}
+
+hipError_t gaussian_smooth_batch(float *den, const SmoothGroup *groups, int ngroups,
+                                 hipStream_t s) {
+    for (int i = 0; i < ngroups; i++) {
+        const SmoothGroup &grp = groups[i];
+        if (grp.count > 0)
+            PGALF_HIP_TRY(smooth_passes(den, grp.nx, grp.ny, grp.nz, grp.expfac, s, grp.goffs,
+                                        grp.count));
+    }
     return hipSuccess;
 }
 

@@ -95,22 +95,50 @@ __device__ inline void tile_span(long long nr, int &t0, int &t1) {
     t1 = (int)((nr + 1) / kT);
 }
 
+// Halo `h` of a batch, or the single halo `seg0` of a one-halo call.
+__device__ inline HaloSeg seg_at(const HaloSeg *segs, const HaloSeg &seg0, int h) {
+    return segs ? segs[h] : seg0;
+}
+// This block's particle chunk: from the batch's chunk list, or block
+// `blockIdx.x` of `per` particles of the single halo.
+__device__ inline PChunk chunk_at(const PChunk *chunks, const HaloSeg &seg0, int per) {
+    if (chunks) return chunks[blockIdx.x];
+    PChunk c;
+    c.halo = 0;
+    c.b0 = (long long)blockIdx.x * per;
+    c.b1 = min(c.b0 + per, seg0.np);
+    return c;
+}
+
+__host__ __device__ inline TileGeom tile_geom(const Grid &g) {
+    TileGeom tg;
+    tg.ntx = (int)((g.mx() + kT - 1) / kT);
+    tg.nty = (g.ny + kT - 1) / kT;
+    tg.ntz = (g.nz + kT - 1) / kT;
+    return tg;
+}
+
+// Per-block tile histogram of the chunk's halo (LDS when the halo has at
+// most kLdsHistMax tiles), then global per-tile counts; per-halo selected
+// count and max |m| for the fixed-point scale.
 template <class P>
 __global__ void __launch_bounds__(kBlock)
-tsc_count_k(const P *x, const P *y, const P *z, const P *m, const int *type, long long np,
-            int ptype, Grid g, TileGeom tg, unsigned *tcount, unsigned *nsel,
-            unsigned long long *maxm_bits, unsigned *status, int use_lds) {
+tsc_count_k(const P *x, const P *y, const P *z, const P *m, const int *type, int ptype,
+            const HaloSeg *segs, HaloSeg seg0, const PChunk *chunks, unsigned *tcount,
+            unsigned *nsel, unsigned long long *maxm_bits, unsigned *status, int use_lds) {
     extern __shared__ unsigned hist[];
-    long long ntiles = tg.ntiles();
+    const PChunk c = chunk_at(chunks, seg0, kPerBlock);
+    const HaloSeg sg = seg_at(segs, seg0, c.halo);
+    const Grid g = sg.g;
+    const TileGeom tg = tile_geom(g);
+    const long long ntiles = tg.ntiles();
     if (use_lds) {
         for (long long t = threadIdx.x; t < ntiles; t += kBlock) hist[t] = 0;
         __syncthreads();
     }
-    long long b0 = (long long)blockIdx.x * kPerBlock;
-    long long b1 = min(b0 + kPerBlock, np);
     unsigned local_sel = 0;
     double local_max = 0.0;
-    for (long long i = b0 + threadIdx.x; i < b1; i += kBlock) {
+    for (long long i = c.b0 + threadIdx.x; i < c.b1; i += kBlock) {
         if (!selected(type, i, ptype)) continue;
         Axis ax[3];
         if (!particle_axes(x, y, z, m, i, g, ax)) {
@@ -128,33 +156,36 @@ tsc_count_k(const P *x, const P *y, const P *z, const P *m, const int *type, lon
                 for (int tx = tx0; tx <= tx1; tx++) {
                     long long t = tx + (long long)tg.ntx * (ty + (long long)tg.nty * tz);
                     if (use_lds) atomicAdd(&hist[t], 1u);
-                    else atomicAdd(&tcount[t], 1u);
+                    else atomicAdd(&tcount[sg.toff + t], 1u);
                 }
     }
-    if (local_sel) atomicAdd(nsel, local_sel);
-    if (local_max > 0.0) atomicMax(maxm_bits, (unsigned long long)__double_as_longlong(local_max));
+    if (local_sel) atomicAdd(&nsel[c.halo], local_sel);
+    if (local_max > 0.0)
+        atomicMax(&maxm_bits[c.halo], (unsigned long long)__double_as_longlong(local_max));
     if (use_lds) {
         __syncthreads();
         for (long long t = threadIdx.x; t < ntiles; t += kBlock)
-            if (hist[t]) atomicAdd(&tcount[t], hist[t]);
+            if (hist[t]) atomicAdd(&tcount[sg.toff + t], hist[t]);
     }
 }
 
 template <class P>
 __global__ void __launch_bounds__(kBlock)
-tsc_scatter_k(const P *x, const P *y, const P *z, const P *m, const int *type, long long np,
-              int ptype, Grid g, TileGeom tg, const unsigned *tstart, unsigned *tcursor,
-              unsigned *bins, int use_lds) {
+tsc_scatter_k(const P *x, const P *y, const P *z, const P *m, const int *type, int ptype,
+              const HaloSeg *segs, HaloSeg seg0, const PChunk *chunks, const unsigned *tstart,
+              unsigned *tcursor, unsigned *bins, int use_lds) {
     extern __shared__ unsigned sh[];
-    long long ntiles = tg.ntiles();
+    const PChunk c = chunk_at(chunks, seg0, kPerBlock);
+    const HaloSeg sg = seg_at(segs, seg0, c.halo);
+    const Grid g = sg.g;
+    const TileGeom tg = tile_geom(g);
+    const long long ntiles = tg.ntiles();
     unsigned *hist = sh;
     unsigned *base = sh + (use_lds ? ntiles : 0);
-    long long b0 = (long long)blockIdx.x * kPerBlock;
-    long long b1 = min(b0 + kPerBlock, np);
     if (use_lds) {
         for (long long t = threadIdx.x; t < ntiles; t += kBlock) hist[t] = 0;
         __syncthreads();
-        for (long long i = b0 + threadIdx.x; i < b1; i += kBlock) {
+        for (long long i = c.b0 + threadIdx.x; i < c.b1; i += kBlock) {
             if (!selected(type, i, ptype)) continue;
             Axis ax[3];
             if (!particle_axes(x, y, z, m, i, g, ax)) continue;
@@ -169,13 +200,13 @@ tsc_scatter_k(const P *x, const P *y, const P *z, const P *m, const int *type, l
         }
         __syncthreads();
         for (long long t = threadIdx.x; t < ntiles; t += kBlock) {
-            unsigned c = hist[t];
-            base[t] = c ? tstart[t] + atomicAdd(&tcursor[t], c) : 0u;
+            unsigned cnt = hist[t];
+            base[t] = cnt ? tstart[sg.toff + t] + atomicAdd(&tcursor[sg.toff + t], cnt) : 0u;
             hist[t] = 0;
         }
         __syncthreads();
     }
-    for (long long i = b0 + threadIdx.x; i < b1; i += kBlock) {
+    for (long long i = c.b0 + threadIdx.x; i < c.b1; i += kBlock) {
         if (!selected(type, i, ptype)) continue;
         Axis ax[3];
         if (!particle_axes(x, y, z, m, i, g, ax)) continue;
@@ -189,7 +220,7 @@ tsc_scatter_k(const P *x, const P *y, const P *z, const P *m, const int *type, l
                     long long t = tx + (long long)tg.ntx * (ty + (long long)tg.nty * tz);
                     unsigned slot;
                     if (use_lds) slot = base[t] + atomicAdd(&hist[t], 1u);
-                    else slot = tstart[t] + atomicAdd(&tcursor[t], 1u);
+                    else slot = tstart[sg.toff + t] + atomicAdd(&tcursor[sg.toff + t], 1u);
                     bins[slot] = (unsigned)i;
                 }
     }
@@ -215,11 +246,13 @@ __global__ void tsc_unit_map_k(const unsigned *ustart, const unsigned *nunits,
     for (unsigned j = 0; j < nunits[t]; j++) unit_tile[s + j] = (unsigned)t;
 }
 
-// Fixed-point scale from max|m| and the selected count.
-__global__ void tsc_scale_k(const unsigned *nsel, const unsigned long long *maxm_bits,
+// Fixed-point scale of each halo from its max|m| and selected count.
+__global__ void tsc_scale_k(const unsigned *nsel, const unsigned long long *maxm_bits, int nhalo,
                             double *scale) {
-    double maxm = __longlong_as_double((long long)*maxm_bits);
-    double bound = maxm * (double)(*nsel) * (1.0 + 1.0 / 1024.0);
+    int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nhalo) return;
+    double maxm = __longlong_as_double((long long)maxm_bits[h]);
+    double bound = maxm * (double)nsel[h] * (1.0 + 1.0 / 1024.0);
     int S = 0;
     if (bound > 0.0) {
         int e;
@@ -228,28 +261,35 @@ __global__ void tsc_scale_k(const unsigned *nsel, const unsigned long long *maxm
     }
     double vf = 0.001 / 0.004;  // (0.001/TSC_CELL_SIZE)^3, tsc_omp2.c:188
     vf = vf * vf * vf;
-    scale[0] = ldexp(1.0, S);
-    scale[1] = ldexp(1.0, -S) * vf;
+    scale[2 * h] = ldexp(1.0, S);
+    scale[2 * h + 1] = ldexp(1.0, -S) * vf;
 }
 
 template <class P>
 __global__ void __launch_bounds__(kBlock)
-tsc_deposit_k(const P *x, const P *y, const P *z, const P *m, Grid g, TileGeom tg,
-              const unsigned *tstart, const unsigned *tcount, const unsigned *ustart,
-              const unsigned *nunits, const unsigned *unit_tile, const unsigned *total_units,
+tsc_deposit_k(const P *x, const P *y, const P *z, const P *m, const HaloSeg *segs,
+              HaloSeg seg0, const unsigned *tile_halo, const unsigned *tstart,
+              const unsigned *tcount, const unsigned *ustart, const unsigned *nunits,
+              const unsigned *unit_tile, const unsigned *total_units,
               const unsigned *multi_slot, const unsigned *bins, const double *scale,
               float *den, unsigned long long *side) {
     __shared__ unsigned long long acc[kTileNodes];
-    const long long mx = g.mx();
-    const double sc = scale[0], inv = scale[1];
     const unsigned nunit_total = *total_units;
     for (unsigned u = blockIdx.x; u < nunit_total; u += gridDim.x) {
         unsigned t = unit_tile[u];
+        const int h = tile_halo ? (int)tile_halo[t] : 0;
+        const HaloSeg sg = seg_at(segs, seg0, h);
+        const Grid g = sg.g;
+        const TileGeom tg = tile_geom(g);
+        const long long mx = g.mx();
+        const double sc = scale[2 * h], inv = scale[2 * h + 1];
+        float *hden = den + sg.goff;
         unsigned chunk = u - ustart[t];
         unsigned nu = nunits[t];
-        int tx = (int)(t % tg.ntx);
-        int ty = (int)((t / tg.ntx) % tg.nty);
-        int tz = (int)(t / ((unsigned)tg.ntx * tg.nty));
+        const unsigned tl = (unsigned)(t - sg.toff);
+        int tx = (int)(tl % tg.ntx);
+        int ty = (int)((tl / tg.ntx) % tg.nty);
+        int tz = (int)(tl / ((unsigned)tg.ntx * tg.nty));
         long long ox = (long long)tx * kT, oy = (long long)ty * kT, oz = (long long)tz * kT;
         for (int i = threadIdx.x; i < kTileNodes; i += kBlock) acc[i] = 0ull;
         __syncthreads();
@@ -287,7 +327,7 @@ tsc_deposit_k(const P *x, const P *y, const P *z, const P *m, Grid g, TileGeom t
             for (int l = threadIdx.x; l < kTileNodes; l += kBlock) {
                 long long gx = ox + (l % kT), gy = oy + ((l / kT) % kT), gz = oz + l / (kT * kT);
                 if (gx < mx && gy < g.ny && gz < g.nz)
-                    den[gx + mx * (gy + (long long)g.ny * gz)] =
+                    hden[gx + mx * (gy + (long long)g.ny * gz)] =
                         (float)((double)(long long)acc[l] * inv);
             }
         } else {
@@ -299,22 +339,29 @@ tsc_deposit_k(const P *x, const P *y, const P *z, const P *m, Grid g, TileGeom t
     }
 }
 
-__global__ void tsc_finalize_k(Grid g, TileGeom tg, const unsigned *nunits,
-                               const unsigned *multi_slot, const unsigned long long *side,
-                               const double *scale, float *den) {
-    unsigned t = blockIdx.x;
+// One block per (global) tile; only multi-unit tiles are written here.
+__global__ void tsc_finalize_k(const HaloSeg *segs, HaloSeg seg0, const unsigned *tile_halo,
+                               const unsigned *nunits, const unsigned *multi_slot,
+                               const unsigned long long *side, const double *scale, float *den) {
+    const unsigned t = blockIdx.x;
     if (nunits[t] <= 1) return;
+    const int h = tile_halo ? (int)tile_halo[t] : 0;
+    const HaloSeg sg = seg_at(segs, seg0, h);
+    const Grid g = sg.g;
+    const TileGeom tg = tile_geom(g);
     const long long mx = g.mx();
-    const double inv = scale[1];
-    int tx = (int)(t % tg.ntx);
-    int ty = (int)((t / tg.ntx) % tg.nty);
-    int tz = (int)(t / ((unsigned)tg.ntx * tg.nty));
+    const double inv = scale[2 * h + 1];
+    float *hden = den + sg.goff;
+    const unsigned tl = (unsigned)(t - sg.toff);
+    int tx = (int)(tl % tg.ntx);
+    int ty = (int)((tl / tg.ntx) % tg.nty);
+    int tz = (int)(tl / ((unsigned)tg.ntx * tg.nty));
     const unsigned long long *src = side + (size_t)multi_slot[t] * kTileNodes;
     for (int l = threadIdx.x; l < kTileNodes; l += blockDim.x) {
         long long gx = (long long)tx * kT + (l % kT), gy = (long long)ty * kT + ((l / kT) % kT),
                   gz = (long long)tz * kT + l / (kT * kT);
         if (gx < mx && gy < g.ny && gz < g.nz)
-            den[gx + mx * (gy + (long long)g.ny * gz)] = (float)((double)(long long)src[l] * inv);
+            hden[gx + mx * (gy + (long long)g.ny * gz)] = (float)((double)(long long)src[l] * inv);
     }
 }
 
@@ -346,91 +393,109 @@ __device__ inline void readback_axis(double p, RAxis &a) {
 
 template <class P>
 __global__ void __launch_bounds__(kBlock)
-tsc_readback_k(const P *x, const P *y, const P *z, long long np, Grid g, const float *den,
-               float *out, unsigned *status) {
-    long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= np) return;
-    RAxis a[3];
-    readback_axis(((double)x[i] - g.x0) / g.h, a[0]);
-    readback_axis(((double)y[i] - g.y0) / g.h, a[1]);
-    readback_axis(((double)z[i] - g.z0) / g.h, a[2]);
-    if (a[0].nr < 1 || a[0].nr > g.mx() - 2 || a[1].nr < 1 || a[1].nr > g.ny - 2 ||
-        a[2].nr < 1 || a[2].nr > g.nz - 2) {
-        atomicOr(status, (unsigned)DEV_ERR_OUT_OF_GRID);
-        out[i] = 0.0f;
-        return;
-    }
+tsc_readback_k(const P *x, const P *y, const P *z, const HaloSeg *segs, HaloSeg seg0,
+               const PChunk *chunks, int per, const float *den, float *out, unsigned *status) {
+    const PChunk ch = chunk_at(chunks, seg0, per);
+    const HaloSeg sg = seg_at(segs, seg0, ch.halo);
+    const Grid g = sg.g;
+    const float *hden = den + sg.goff;
     const long mx = (long)g.mx();
-    long ix[3] = {a[0].nr, a[0].nr + a[0].sg, a[0].nr - a[0].sg};
-    long iy[3] = {a[1].nr, a[1].nr + a[1].sg, a[1].nr - a[1].sg};
-    long iz[3] = {a[2].nr, a[2].nr + a[2].sg, a[2].nr - a[2].sg};
-    float wx[3] = {a[0].w1, a[0].w2, a[0].w3};
-    float wy[3] = {a[1].w1, a[1].w2, a[1].w3};
-    float wz[3] = {a[2].w1, a[2].w2, a[2].w3};
-    float wxy[3][3];
-#pragma unroll
-    for (int b = 0; b < 3; b++)
-#pragma unroll
-        for (int c = 0; c < 3; c++) wxy[b][c] = wx[c] * wy[b];
-    float s = 0.0f;
-#pragma unroll
-    for (int c = 0; c < 3; c++)
+    for (long long i = ch.b0 + threadIdx.x; i < ch.b1; i += kBlock) {
+        RAxis a[3];
+        readback_axis(((double)x[i] - g.x0) / g.h, a[0]);
+        readback_axis(((double)y[i] - g.y0) / g.h, a[1]);
+        readback_axis(((double)z[i] - g.z0) / g.h, a[2]);
+        if (a[0].nr < 1 || a[0].nr > g.mx() - 2 || a[1].nr < 1 || a[1].nr > g.ny - 2 ||
+            a[2].nr < 1 || a[2].nr > g.nz - 2) {
+            atomicOr(status, (unsigned)DEV_ERR_OUT_OF_GRID);
+            out[i] = 0.0f;
+            continue;
+        }
+        long ix[3] = {a[0].nr, a[0].nr + a[0].sg, a[0].nr - a[0].sg};
+        long iy[3] = {a[1].nr, a[1].nr + a[1].sg, a[1].nr - a[1].sg};
+        long iz[3] = {a[2].nr, a[2].nr + a[2].sg, a[2].nr - a[2].sg};
+        float wx[3] = {a[0].w1, a[0].w2, a[0].w3};
+        float wy[3] = {a[1].w1, a[1].w2, a[1].w3};
+        float wz[3] = {a[2].w1, a[2].w2, a[2].w3};
+        float wxy[3][3];
 #pragma unroll
         for (int b = 0; b < 3; b++)
 #pragma unroll
-            for (int q = 0; q < 3; q++) {
-                float gv = den[ix[q] + mx * (iy[b] + (long)g.ny * iz[c])];
-                s += (gv * wxy[b][q]) * wz[c];
-            }
-    out[i] = s;
+            for (int c = 0; c < 3; c++) wxy[b][c] = wx[c] * wy[b];
+        float s = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int b = 0; b < 3; b++)
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    float gv = hden[ix[q] + mx * (iy[b] + (long)g.ny * iz[c])];
+                    s += (gv * wxy[b][q]) * wz[c];
+                }
+        out[i] = s;
+    }
 }
 
-inline TileGeom tile_geom(const Grid &g) {
-    TileGeom tg;
-    tg.ntx = (int)((g.mx() + kT - 1) / kT);
-    tg.nty = (g.ny + kT - 1) / kT;
-    tg.ntz = (g.nz + kT - 1) / kT;
-    return tg;
+HaloSeg single_seg(const Grid &g, long long np) {
+    HaloSeg sg;
+    sg.p0 = 0;
+    sg.np = np;
+    sg.goff = 0;
+    sg.toff = 0;
+    sg.g = g;
+    return sg;
 }
 
-}  // namespace
+int cu_count(int device) {
+    static int ncu = 0;
+    if (ncu == 0) {
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) !=
+                hipSuccess || ncu <= 0)
+            ncu = 256;
+    }
+    return ncu;
+}
 
+// Deposit of the halos of `segs` (or the single halo `seg0`) through the
+// tile bins.  ntiles, the chunk count and max_halo_tiles describe the tile
+// space and the particle blocks; tile_halo maps a tile to its halo.
 template <class P>
-hipError_t tsc_deposit(const P *x, const P *y, const P *z, const P *m, const int *type,
-                       long long np, int ptype, const Grid &g, float *den, Workspace &ws,
-                       hipStream_t s) {
-    TileGeom tg = tile_geom(g);
-    long long ntiles = tg.ntiles();
-    int use_lds = ntiles <= kLdsHistMax ? 1 : 0;
-    // counters: tcount, tcursor, nsel, maxm, total_units, nmulti
+hipError_t deposit_impl(const P *x, const P *y, const P *z, const P *m, const int *type,
+                        int ptype, int nhalo, const HaloSeg *segs, const HaloSeg &seg0,
+                        const PChunk *chunks, long long nchunks, const unsigned *tile_halo,
+                        long long ntiles, long long max_halo_tiles, long long np, float *den,
+                        Workspace &ws, hipStream_t s) {
+    int use_lds = max_halo_tiles <= kLdsHistMax ? 1 : 0;
     unsigned *tcount = ws.alloc_n<unsigned>(ntiles);
     unsigned *tcursor = ws.alloc_n<unsigned>(ntiles);
     unsigned *tstart = ws.alloc_n<unsigned>(ntiles);
     unsigned *nunits = ws.alloc_n<unsigned>(ntiles);
     unsigned *ustart = ws.alloc_n<unsigned>(ntiles);
     unsigned *multi = ws.alloc_n<unsigned>(ntiles);
-    unsigned *misc = ws.alloc_n<unsigned>(8);  // [0]=nsel [1]=total [2]=units [3]=nmulti
-    unsigned long long *maxm = ws.alloc_n<unsigned long long>(1);
-    double *scale = ws.alloc_n<double>(2);
-    if (!tcount || !tcursor || !tstart || !nunits || !ustart || !multi || !misc || !maxm ||
-        !scale)
+    unsigned *misc = ws.alloc_n<unsigned>(8);  // [1]=total entries [2]=units [3]=nmulti
+    unsigned *nsel = ws.alloc_n<unsigned>(nhalo);
+    unsigned long long *maxm = ws.alloc_n<unsigned long long>(nhalo);
+    double *scale = ws.alloc_n<double>(2 * (size_t)nhalo);
+    if (!tcount || !tcursor || !tstart || !nunits || !ustart || !multi || !misc || !nsel ||
+        !maxm || !scale)
         return hipErrorOutOfMemory;
-    PGALF_HIP_TRY(hipMemsetAsync(tcount, 0, sizeof(unsigned) * ntiles, s));
-    PGALF_HIP_TRY(hipMemsetAsync(tcursor, 0, sizeof(unsigned) * ntiles, s));
-    PGALF_HIP_TRY(hipMemsetAsync(misc, 0, sizeof(unsigned) * 8, s));
-    PGALF_HIP_TRY(hipMemsetAsync(maxm, 0, sizeof(unsigned long long), s));
-    long long nblk = (np + kPerBlock - 1) / kPerBlock;
-    size_t lds_count = use_lds ? sizeof(unsigned) * ntiles : 0;
-    if (nblk > 0)
-        tsc_count_k<P><<<dim3((unsigned)nblk), dim3(kBlock), lds_count, s>>>(
-            x, y, z, m, type, np, ptype, g, tg, tcount, &misc[0], maxm, ws.status(), use_lds);
+    PGALF_HIP_TRY(fill_u32(tcount, ntiles, 0u, s));
+    PGALF_HIP_TRY(fill_u32(tcursor, ntiles, 0u, s));
+    PGALF_HIP_TRY(fill_u32(misc, 8, 0u, s));
+    PGALF_HIP_TRY(fill_u32(nsel, nhalo, 0u, s));
+    PGALF_HIP_TRY(fill_u64(maxm, nhalo, 0ull, s));
+    size_t lds_count = use_lds ? sizeof(unsigned) * max_halo_tiles : 0;
+    if (nchunks > 0)
+        tsc_count_k<P><<<dim3((unsigned)nchunks), dim3(kBlock), lds_count, s>>>(
+            x, y, z, m, type, ptype, segs, seg0, chunks, tcount, nsel, maxm, ws.status(),
+            use_lds);
     PGALF_HIP_TRY(exclusive_scan_u32(tcount, tstart, ntiles, &misc[1], ws, s));
     // the bin list can hold at most 8 entries per particle
     unsigned *bins = ws.alloc_n<unsigned>((size_t)np * 8 + 1);
     if (!bins) return hipErrorOutOfMemory;
-    if (nblk > 0)
-        tsc_scatter_k<P><<<dim3((unsigned)nblk), dim3(kBlock), 2 * lds_count, s>>>(
-            x, y, z, m, type, np, ptype, g, tg, tstart, tcursor, bins, use_lds);
+    if (nchunks > 0)
+        tsc_scatter_k<P><<<dim3((unsigned)nchunks), dim3(kBlock), 2 * lds_count, s>>>(
+            x, y, z, m, type, ptype, segs, seg0, chunks, tstart, tcursor, bins, use_lds);
     unsigned tb = (unsigned)((ntiles + 255) / 256);
     tsc_units_k<<<tb, 256, 0, s>>>(tcount, ntiles, nunits, multi);
     PGALF_HIP_TRY(exclusive_scan_u32(nunits, ustart, ntiles, &misc[2], ws, s));
@@ -445,42 +510,77 @@ hipError_t tsc_deposit(const P *x, const P *y, const P *z, const P *m, const int
     unsigned long long *side = ws.alloc_n<unsigned long long>(max_multi * kTileNodes);
     if (!unit_tile || !side) return hipErrorOutOfMemory;
     tsc_unit_map_k<<<tb, 256, 0, s>>>(ustart, nunits, ntiles, unit_tile);
-    tsc_scale_k<<<1, 1, 0, s>>>(&misc[0], maxm, scale);
-    static int ncu = 0;
-    if (ncu == 0) {
-        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ws.device()) !=
-                hipSuccess || ncu <= 0)
-            ncu = 256;
-    }
+    tsc_scale_k<<<(unsigned)((nhalo + 255) / 256), 256, 0, s>>>(nsel, maxm, nhalo, scale);
+    const int ncu = cu_count(ws.device());
     fill_side_k<<<(unsigned)ncu * 4, 256, 0, s>>>(side, &misc[3]);
-    unsigned grid = (unsigned)ncu * 4;
-    tsc_deposit_k<P><<<grid, kBlock, 0, s>>>(x, y, z, m, g, tg, tstart, tcount, ustart, nunits,
-                                            unit_tile, &misc[2], multi, bins, scale, den,
-                                            side);
-    tsc_finalize_k<<<(unsigned)ntiles, 256, 0, s>>>(g, tg, nunits, multi, side, scale, den);
+    tsc_deposit_k<P><<<(unsigned)ncu * 4, kBlock, 0, s>>>(
+        x, y, z, m, segs, seg0, tile_halo, tstart, tcount, ustart, nunits, unit_tile, &misc[2],
+        multi, bins, scale, den, side);
+    tsc_finalize_k<<<(unsigned)ntiles, 256, 0, s>>>(segs, seg0, tile_halo, nunits, multi, side,
+                                                    scale, den);
     return hipGetLastError();
+}
+
+}  // namespace
+
+long long tsc_tile_count(const Grid &g) { return tile_geom(g).ntiles(); }
+
+template <class P>
+hipError_t tsc_deposit(const P *x, const P *y, const P *z, const P *m, const int *type,
+                       long long np, int ptype, const Grid &g, float *den, Workspace &ws,
+                       hipStream_t s) {
+    const HaloSeg seg0 = single_seg(g, np);
+    const long long ntiles = tile_geom(g).ntiles();
+    const long long nchunks = (np + kPerBlock - 1) / kPerBlock;
+    return deposit_impl<P>(x, y, z, m, type, ptype, 1, nullptr, seg0, nullptr, nchunks, nullptr,
+                           ntiles, ntiles, np, den, ws, s);
+}
+
+template <class P>
+hipError_t tsc_deposit_batch(const P *x, const P *y, const P *z, const P *m, const BatchDev &b,
+                             float *den, Workspace &ws, hipStream_t s) {
+    static_assert(kPerBlock == kChunkParticles, "batch chunks are deposit blocks");
+    const HaloSeg seg0{};
+    return deposit_impl<P>(x, y, z, m, nullptr, PGALF_TYPE_ALL, b.nhalo, b.segs, seg0, b.chunks,
+                           b.nchunks, b.tile_halo, b.ntiles, b.max_halo_tiles, b.np, den, ws, s);
 }
 
 template <class P>
 hipError_t tsc_readback(const P *x, const P *y, const P *z, long long np, const Grid &g,
                         const float *den, float *out, Workspace &ws, hipStream_t s) {
     if (np <= 0) return hipSuccess;
+    const HaloSeg seg0 = single_seg(g, np);
     tsc_readback_k<P><<<(unsigned)((np + kBlock - 1) / kBlock), kBlock, 0, s>>>(
-        x, y, z, np, g, den, out, ws.status());
+        x, y, z, nullptr, seg0, nullptr, kBlock, den, out, ws.status());
     return hipGetLastError();
 }
 
-template hipError_t tsc_deposit<float>(const float *, const float *, const float *,
-                                       const float *, const int *, long long, int,
-                                       const Grid &, float *, Workspace &, hipStream_t);
-template hipError_t tsc_deposit<double>(const double *, const double *, const double *,
-                                        const double *, const int *, long long, int,
-                                        const Grid &, float *, Workspace &, hipStream_t);
-template hipError_t tsc_readback<float>(const float *, const float *, const float *, long long,
-                                        const Grid &, const float *, float *, Workspace &,
-                                        hipStream_t);
-template hipError_t tsc_readback<double>(const double *, const double *, const double *,
-                                         long long, const Grid &, const float *, float *,
-                                         Workspace &, hipStream_t);
+template <class P>
+hipError_t tsc_readback_batch(const P *x, const P *y, const P *z, const BatchDev &b,
+                              const float *den, float *out, Workspace &ws, hipStream_t s) {
+    if (b.nchunks <= 0) return hipSuccess;
+    const HaloSeg seg0{};
+    tsc_readback_k<P><<<(unsigned)b.nchunks, kBlock, 0, s>>>(x, y, z, b.segs, seg0, b.chunks,
+                                                             kChunkParticles, den, out,
+                                                             ws.status());
+    return hipGetLastError();
+}
+
+#define PGALF_TSC_INST(P)                                                                      \
+    template hipError_t tsc_deposit<P>(const P *, const P *, const P *, const P *, const int *, \
+                                       long long, int, const Grid &, float *, Workspace &,      \
+                                       hipStream_t);                                           \
+    template hipError_t tsc_readback<P>(const P *, const P *, const P *, long long,            \
+                                        const Grid &, const float *, float *, Workspace &,     \
+                                        hipStream_t);                                          \
+    template hipError_t tsc_deposit_batch<P>(const P *, const P *, const P *, const P *,       \
+                                             const BatchDev &, float *, Workspace &,           \
+                                             hipStream_t);                                     \
+    template hipError_t tsc_readback_batch<P>(const P *, const P *, const P *,                 \
+                                              const BatchDev &, const float *, float *,        \
+                                              Workspace &, hipStream_t);
+PGALF_TSC_INST(float)
+PGALF_TSC_INST(double)
+#undef PGALF_TSC_INST
 
 }  // namespace pgalf

@@ -233,6 +233,85 @@ def dev_density_stage(x, y, z, m, grid, RG, k, den, rho_tsc=None, nbr=None, rho_
                                            _stream()))
 
 
+class Batch:
+    """A catalogue of halos run through the density stage together
+    (pgalf_batch_* of the C-ABI) -- the batched counterpart of calling the
+    stage once per halo as the reference's workers do (gfind.c:394-549).
+
+    offsets: int64 [nhalo+1], halo h owns particles [offsets[h], offsets[h+1])
+    of the concatenated arrays (offsets[0] == 0).  grids: nhalo grid specs
+    (pgalf_grid_t or (nx, ny, nz, xmin, ymin, zmin, cellsize) tuples).  RG:
+    per-halo smoothing lengths in box units.  k: neighbours per particle.
+    The grid buffer holds every halo's padded grid; halo h's starts at float
+    ``grid_offsets[h]`` and the buffer has ``grid_offsets[-1]`` floats.
+    Neighbour indices are local to each halo.
+    """
+
+    def __init__(self, offsets, grids, RG, k):
+        lib = _lib.load()
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        nhalo = len(off) - 1
+        garr = (pgalf_grid_t * nhalo)()
+        for h, g in enumerate(grids):
+            if isinstance(g, pgalf_grid_t):
+                garr[h] = g
+            else:
+                garr[h] = grid_struct(*g)
+        rg = np.ascontiguousarray(np.broadcast_to(np.asarray(RG, dtype=np.float32), (nhalo,)))
+        self.nhalo, self.k = nhalo, int(k)
+        self.offsets = off
+        self._b = lib.pgalf_batch_create(nhalo, _ptr(off), garr, _ptr(rg), int(k))
+        if not self._b:
+            _lib.check()
+            raise PgalfError(-1, "pgalf_batch_create failed")
+        self.grid_offsets = np.array([lib.pgalf_batch_grid_offset(self._b, h)
+                                      for h in range(nhalo + 1)], dtype=np.int64)
+        self.grids = [(garr[h].nx, garr[h].ny, garr[h].nz) for h in range(nhalo)]
+
+    @property
+    def grid_floats(self):
+        return int(self.grid_offsets[-1])
+
+    def grid_view(self, den, h):
+        """Halo h's padded (nz, ny, mx) grid inside the batch grid buffer."""
+        nx, ny, nz = self.grids[h]
+        a, b = self.grid_offsets[h], self.grid_offsets[h + 1]
+        return den[a:b].reshape(padded_shape(nx, ny, nz))
+
+    def run(self, x, y, z, m, den, rho_tsc=None, nbr=None, rho_knn=None, graph=False):
+        torch = _torch()
+        n = int(self.offsets[-1])
+        for t, w in ((x, "x"), (y, "y"), (z, "z"), (m, "m")):
+            _check_dev(t, torch.float32, n, w)
+        _check_dev(den, torch.float32, self.grid_floats, "den")
+        if rho_tsc is not None:
+            _check_dev(rho_tsc, torch.float32, n, "rho_tsc")
+        if nbr is not None:
+            _check_dev(nbr, torch.int64, n * self.k, "nbr")
+        if rho_knn is not None:
+            _check_dev(rho_knn, torch.float32, n, "rho_knn")
+        lib = _lib.load()
+        _lib.check(lib.pgalf_batch_run(self._b, _dptr(x), _dptr(y), _dptr(z), _dptr(m),
+                                       _dptr(den), _dptr(rho_tsc), _dptr(nbr), _dptr(rho_knn),
+                                       1 if graph else 0, _stream()))
+
+    def status(self):
+        """Synchronise torch's current stream; raise on kernel-detected errors."""
+        lib = _lib.load()
+        _lib.check(lib.pgalf_batch_status(self._b, _stream()))
+
+    def close(self):
+        if getattr(self, "_b", None):
+            _lib.load().pgalf_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def dev_status():
     """Synchronise torch's current stream and raise on kernel-detected errors."""
     lib = _lib.load()
@@ -259,4 +338,4 @@ __all__ = ["PgalfError", "PARTICLE_DTYPE", "make_particles", "padded_shape",
            "assign_density_TSC", "gaussian_Smoothing", "findDen", "findsphdensity",
            "starfindsphdensity", "near_lists", "lag_density_stage", "grid_struct",
            "dev_tsc", "dev_smooth", "dev_findden", "dev_knn", "dev_density_stage",
-           "dev_status", "timing", "timing_get"]
+           "dev_status", "Batch", "timing", "timing_get"]

@@ -110,6 +110,58 @@ def single_halo_config(config_id):
     return pos, mass, spec
 
 
+def powerlaw_counts(rng, nhalo, alpha, nmin, nmax):
+    """Halo particle counts with dN/dn proportional to n^-alpha on
+    [nmin, nmax] (inverse CDF, rounded down, clipped to the range)."""
+    u = rng.uniform(0.0, 1.0, nhalo)
+    a1 = 1.0 - alpha
+    lo, hi = float(nmin) ** a1, float(nmax) ** a1
+    n = (lo + u * (hi - lo)) ** (1.0 / a1)
+    return np.clip(np.floor(n).astype(np.int64), nmin, nmax)
+
+
+def grid_for(n, gmin=32, gmax=256):
+    """Per-halo grid size: the power of two >= 4 n^(1/3), clamped to
+    [gmin, gmax]."""
+    t = 4.0 * float(n) ** (1.0 / 3.0)
+    g = 1
+    while g < t:
+        g *= 2
+    return int(min(max(g, gmin), gmax))
+
+
+def catalogue(config_id=3, nhalo=10000, alpha=2.0, nmin=100, nmax=1048576, rg_cells=2.0,
+              gmin=32, gmax=256):
+    """A catalogue of halos for the batched stage, each in its own unit box.
+
+    Counts follow dN/dn ~ n^-alpha on [nmin, nmax]; every halo is NFW with c
+    uniform in [4, 15] (r200 = 0.4) plus Poisson(5) Plummer subhalos with
+    log-uniform mass fractions in [0.1%, 3%], scale 0.01 and centres at radii
+    U(0.1, 0.3); masses are 1/n.  Its grid spans the unit box with
+    n_g = grid_for(n) cells per side, RG = rg_cells / n_g.
+    Returns (pos (N,3) float32, mass (N,) float32, offsets int64 [nhalo+1],
+    grids [(nx,ny,nz,xmin,ymin,zmin,cellsize)], RG float32 [nhalo])."""
+    rng = rng_for(config_id)
+    counts = powerlaw_counts(rng, nhalo, alpha, nmin, nmax)
+    pos_l, mass_l, grids, rg = [], [], [], []
+    for n in counts:
+        c = rng.uniform(4.0, 15.0)
+        nsub = rng.poisson(5.0)
+        fr = np.exp(rng.uniform(math.log(0.001), math.log(0.03), nsub))
+        radii = rng.uniform(0.1, 0.3, nsub)
+        p, m = halo(rng, int(n), c=c, r200=0.4,
+                    subs=[(f, 0.01, r) for f, r in zip(fr, radii)])
+        pos_l.append(p)
+        mass_l.append(m)
+        ng = grid_for(n, gmin, gmax)
+        grids.append((ng, ng, ng, 0.0, 0.0, 0.0, 1.0 / ng))
+        rg.append(rg_cells / ng)
+    off = np.zeros(nhalo + 1, dtype=np.int64)
+    off[1:] = np.cumsum(counts)
+    return (np.concatenate(pos_l), np.concatenate(mass_l), off, grids,
+            np.asarray(rg, dtype=np.float32))
+
+
 def small_halo(n, seed=0, nsub=2):
     """Small clustered halo for tests (not a named workload)."""
     rng = np.random.Generator(np.random.PCG64(SEED_BASE + 1000 + seed))

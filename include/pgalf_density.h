@@ -134,6 +134,37 @@ int pgalf_dev_density_stage(const float *x, const float *y, const float *z,
  * previous call (PGALF_ERR_OUT_OF_GRID / PGALF_ERR_SHORT_LIST or PGALF_OK). */
 int pgalf_dev_status(void *stream);
 
+/* ===================== Part 3: batched catalogue API ====================== */
+
+/* A catalogue of halos run through the density stage together, in place of
+ * the one-halo-per-call loop of the reference's MPI workers
+ * (NewGalFinder/gfind.c:394-549 dispatching subhalo_den per halo).
+ * The particles of halo h are [off[h], off[h+1]) of the concatenated float32
+ * SoA device arrays (off[0] == 0, each halo >= k particles).  Halo h is
+ * deposited onto its own padded grid grids[h] (at float offset
+ * pgalf_batch_grid_offset(b, h) of one batch grid buffer of
+ * pgalf_batch_grid_offset(b, nhalo) floats), smoothed with RG[h] (box
+ * units, the reference's gaussian_Smoothing argument), read back, and gets
+ * its exact k-NN lists (indices local to the halo, as nearindex is in
+ * lagFindStellarCore) and unweighted W4 densities.  The plan (descriptors,
+ * FFT plans, tree layout, wave table) is built once per catalogue shape.
+ * Returns NULL on failure (see pgalf_last_error()). */
+typedef struct pgalf_batch pgalf_batch_t;
+pgalf_batch_t *pgalf_batch_create(int nhalo, const long long *off, const pgalf_grid_t *grids,
+                                  const float *RG, int k);
+long long pgalf_batch_grid_offset(const pgalf_batch_t *b, int h);
+/* One stage pass over the catalogue, stream-ordered on `stream`.  rho_tsc,
+ * nbr and rho_knn may be NULL.  use_graph != 0: the launch sequence is
+ * captured into a HIP graph on first use (per set of pointers) and replayed
+ * afterwards. */
+int pgalf_batch_run(pgalf_batch_t *b, const float *x, const float *y, const float *z,
+                    const float *m, float *den, float *rho_tsc, long long *nbr,
+                    float *rho_knn, int use_graph, void *stream);
+/* Synchronise `stream` and return the status of the batch's kernels since
+ * the previous call (PGALF_ERR_OUT_OF_GRID or PGALF_OK). */
+int pgalf_batch_status(pgalf_batch_t *b, void *stream);
+void pgalf_batch_destroy(pgalf_batch_t *b);
+
 /* Phase timing with HIP events on the launch stream (off by default). */
 void pgalf_timing_enable(int on);
 void pgalf_timing_reset(void);

@@ -80,3 +80,34 @@ def test_halo_generator_seeded_and_inside_box():
     assert len(p) == 131072 and spec["nx"] == 64 and spec["k"] == 32
     assert p.min() >= 0.1 - 1e-6 and p.max() <= 0.9 + 1e-6
     assert abs(m.astype(np.float64).sum() - 1.0) < 1e-4
+
+
+def test_batch_create_validates_before_device():
+    """pgalf_batch_create checks the catalogue on the host: a halo with fewer
+    than k particles is a short list (the reference exits with 99 there),
+    a malformed offset table a bad argument."""
+    grids = [(32, 32, 32, 0.0, 0.0, 0.0, 1.0 / 32)] * 2
+    with pytest.raises(D.PgalfError) as ei:
+        D.Batch(np.array([0, 100, 120], dtype=np.int64), grids, [2.0 / 32] * 2, 32)
+    assert ei.value.code == 5
+    with pytest.raises(D.PgalfError) as ei:
+        D.Batch(np.array([5, 100, 220], dtype=np.int64), grids, [2.0 / 32] * 2, 32)
+    assert ei.value.code == 1
+    with pytest.raises(D.PgalfError) as ei:
+        D.Batch(np.array([0, 100, 220], dtype=np.int64), grids, [2.0 / 32] * 2, 33)
+    assert ei.value.code == 1
+
+
+def test_catalogue_generator():
+    p, m, off, grids, rg = halos.catalogue(config_id=103, nhalo=50, nmax=20000)
+    c = np.diff(off)
+    assert off[0] == 0 and off[-1] == len(p) == len(m) and len(grids) == len(rg) == 50
+    assert c.min() >= 100 and c.max() <= 20000
+    assert p.min() >= 0.1 - 1e-6 and p.max() <= 0.9 + 1e-6
+    for n, g, r in zip(c, grids, rg):
+        assert g[0] == halos.grid_for(n) and abs(r * g[0] - 2.0) < 1e-6
+    assert halos.grid_for(100) == 32 and halos.grid_for(1048576) == 256
+    assert halos.grid_for(32768) == 128
+    # dN/dn ~ n^-2 on [100, 1e6]: the median is ~2 nmin
+    big = halos.powerlaw_counts(halos.rng_for(0), 20000, 2.0, 100, 1048576)
+    assert 180 < np.median(big) < 220
