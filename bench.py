@@ -56,44 +56,212 @@ def stage_bytes(n, g_cells, k):
     }
 
 
-def cpu_baseline(pos, mass, spec, k, knn_sample):
-    """Time the CPU restatement of the stage on the same halo.  Deposit,
-    smoothing and read-back run on the whole halo; the kNN tree is built
-    over every particle and queried for a random sample of knn_sample
-    particles, its query time scaled to all particles."""
+def cpu_baseline(pos, mass, spec, k, knn_sample, reps=2):
+    """Time the CPU restatement of the stage on the same halo, `reps` times
+    (mean reported).  Deposit, smoothing and read-back run on the whole
+    halo; the kNN tree is built over every particle and queried for
+    knn_sample random particles (all by default), the query time scaled to
+    all particles."""
     from oracle import oracle as O
     n = len(pos)
     g = (spec["nx"], spec["ny"], spec["nz"], spec["xmin"], spec["ymin"], spec["zmin"],
          spec["cellsize"])
-    t0 = time.perf_counter()
-    den, _ = O.tsc_deposit(pos, mass, g, nthreads=1)
-    t1 = time.perf_counter()
-    sm = O.gsmooth(den, spec["nx"], spec["ny"], spec["nz"], spec["cellsize"], spec["RG"])
-    t2 = time.perf_counter()
-    O.findden(pos, g, sm)
-    t3 = time.perf_counter()
     sel = np.random.default_rng(0).choice(n, min(knn_sample, n), replace=False)
-    tb0 = time.perf_counter()
-    O.knn(pos[sel[:64]], k, cpos=pos)  # tree build (+64 queries)
-    tb1 = time.perf_counter()
-    idx, d2 = O.knn(pos[sel], k, cpos=pos)
-    O.sph_density(idx, d2)
-    tb2 = time.perf_counter()
-    build = tb1 - tb0
-    query = max(tb2 - tb1 - build, 0.0) * n / len(sel)
-    total = (t1 - t0) + (t2 - t1) + (t3 - t2) + build + query
+    st = {"tsc": 0.0, "smooth": 0.0, "findden": 0.0, "knn_build": 0.0, "knn_query_scaled": 0.0}
+    work = 0.0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        den, _ = O.tsc_deposit(pos, mass, g, nthreads=1)
+        t1 = time.perf_counter()
+        sm = O.gsmooth(den, spec["nx"], spec["ny"], spec["nz"], spec["cellsize"], spec["RG"])
+        t2 = time.perf_counter()
+        O.findden(pos, g, sm)
+        t3 = time.perf_counter()
+        tb0 = time.perf_counter()
+        O.knn(pos[sel[:64]], k, cpos=pos)  # tree build (+64 queries)
+        tb1 = time.perf_counter()
+        idx, d2 = O.knn(pos[sel], k, cpos=pos)
+        O.sph_density(idx, d2)
+        tb2 = time.perf_counter()
+        build = tb1 - tb0
+        st["tsc"] += (t1 - t0) / reps
+        st["smooth"] += (t2 - t1) / reps
+        st["findden"] += (t3 - t2) / reps
+        st["knn_build"] += build / reps
+        st["knn_query_scaled"] += max(tb2 - tb1 - build, 0.0) * n / len(sel) / reps
+        work += (t3 - t0) + (tb2 - tb0)
+    total = sum(st.values())
     return {
         "value": n / total,
         "unit": "particles/s",
         "cores": O.num_threads(),
         "kind": "port",
         "sample": ("CPU restatement (oracle/pgalf_oracle.c, OpenMP) on the same %d-particle "
-                   "halo: deposit (serial, reference order) + FP64 FFT smoothing + read-back "
-                   "on all particles, k-d tree kNN built on all and queried for %d random "
-                   "particles, query time scaled to all; %.1f s of CPU work"
-                   % (n, len(sel), (t3 - t0) + (tb2 - tb0))),
-        "stages_s": {"tsc": t1 - t0, "smooth": t2 - t1, "findden": t3 - t2,
-                     "knn_build": build, "knn_query_scaled": query},
+                   "halo, mean of %d runs: deposit (serial, reference order) + FP64 FFT "
+                   "smoothing + read-back on all particles, k-d tree kNN built on all and "
+                   "queried for %d particles (query time scaled to all); %.1f s of CPU work"
+                   % (n, reps, len(sel), work)),
+        "stages_s": st,
+    }
+
+
+def batch_bench(steps, warmup, plain_pass=True):
+    """Batched forward over the 10000-halo catalogue (dN/dn ~ n^-2 on
+    [100, 1048576], grids 32^3-256^3 from n, RG = 2 cells, k = 32): eager
+    and HIP-graph runs of the batch, and one pass of the single-halo stage
+    called once per halo on the same device-resident particles."""
+    import torch
+    from galaxyfinder_amd import density as D
+    from galaxyfinder_amd import halos
+
+    t0 = time.perf_counter()
+    pos, mass, off, grids, rg = halos.catalogue(config_id=3)
+    gen_s = time.perf_counter() - t0
+    n, nh, k = len(pos), len(off) - 1, 32
+    print("[bench] batch workload: %d halos, %d particles (generated in %.1f s)"
+          % (nh, n, gen_s), file=sys.stderr, flush=True)
+    dev = torch.device("cuda")
+    x, y, z = (torch.from_numpy(np.ascontiguousarray(pos[:, a])).to(dev) for a in range(3))
+    m = torch.from_numpy(np.ascontiguousarray(mass)).to(dev)
+    b = D.Batch(off, grids, rg, k)
+    den = torch.empty(b.grid_floats, dtype=torch.float32, device=dev)
+    rho_t = torch.empty(n, dtype=torch.float32, device=dev)
+    nbr = torch.empty((n, k), dtype=torch.int64, device=dev)
+    rho_k = torch.empty(n, dtype=torch.float32, device=dev)
+
+    def timed(graph):
+        for _ in range(warmup):
+            b.run(x, y, z, m, den, rho_t, nbr, rho_k, graph=graph)
+        b.status()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.run(x, y, z, m, den, rho_t, nbr, rho_k, graph=graph)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        b.status()
+        return (t1 - t0) * 1e3 / steps
+
+    eager_ms = timed(False)
+    graph_ms = timed(True)
+    D.timing(enable=True, reset=True)
+    for _ in range(steps):
+        b.run(x, y, z, m, den, rho_t, nbr, rho_k, graph=False)
+    torch.cuda.synchronize()
+    phases = {}
+    for ph in ("batch_stage", "batch_tsc", "batch_smooth", "batch_findden", "batch_knn",
+               "knn_prep", "knn_query"):
+        tot, cnt = D.timing_get(ph)
+        if cnt:
+            phases[ph] = round(tot / cnt, 4)
+    D.timing(enable=False, reset=True)
+    print("[bench] batch: eager %.3f ms, graph %.3f ms" % (eager_ms, graph_ms),
+          file=sys.stderr, flush=True)
+    out = {
+        "workload": "%d halos, %d particles, dN/dn ~ n^-2 on [100, 1048576], grids "
+                    "32^3-256^3 (pow2 >= 4 n^1/3), RG = 2 cells, k = %d" % (nh, n, k),
+        "halos": nh, "particles": n,
+        "value": round(n / (min(eager_ms, graph_ms) * 1e-3), 1),
+        "unit": "particles/s",
+        "ms_per_step_eager": round(eager_ms, 4),
+        "ms_per_step_graph": round(graph_ms, 4),
+        "phase_ms": phases,
+    }
+    if plain_pass:
+        # the single-halo stage called once per halo (views into the same
+        # device arrays; one pass, after a few warm-up halos)
+        views = []
+        for h in range(nh):
+            a, e = int(off[h]), int(off[h + 1])
+            g = D.grid_struct(*grids[h])
+            views.append((x[a:e], y[a:e], z[a:e], m[a:e], g, float(rg[h]),
+                          b.grid_view(den, h), rho_t[a:e], nbr[a:e], rho_k[a:e]))
+        for v in views[:20]:
+            D.dev_density_stage(v[0], v[1], v[2], v[3], v[4], v[5], k, v[6], v[7], v[8], v[9])
+        D.dev_status()
+        t0 = time.perf_counter()
+        for v in views:
+            D.dev_density_stage(v[0], v[1], v[2], v[3], v[4], v[5], k, v[6], v[7], v[8], v[9])
+        torch.cuda.synchronize()
+        plain_ms = (time.perf_counter() - t0) * 1e3
+        D.dev_status()
+        out["plain_per_halo_ms"] = round(plain_ms, 2)
+        out["plain_per_halo_value"] = round(n / (plain_ms * 1e-3), 1)
+        out["speedup_vs_per_halo"] = round(plain_ms / min(eager_ms, graph_ms), 2)
+        print("[bench] per-halo plain pass: %.1f ms (%.1fx)" % (plain_ms, out["speedup_vs_per_halo"]),
+              file=sys.stderr, flush=True)
+    b.close()
+    return out
+
+
+def grad_bench(steps, warmup):
+    """Forward and forward+backward of the differentiable stage on one halo
+    of 1048576 particles (NFW c=8 + 10 subhalos), 128^3 grid, RG = 2 cells,
+    k = 32, mass-weighted W4 density; loss = sum w_t log rho_tsc + sum w_k
+    log rho_knn with seeded Gaussian weights."""
+    import torch
+    from galaxyfinder_amd import density as D
+    from galaxyfinder_amd import halos
+    from galaxyfinder_amd.autograd import density_stage
+
+    pos, mass, spec = halos.single_halo_config(4)
+    n, k = len(pos), spec["k"]
+    dev = torch.device("cuda")
+    xyz = [torch.from_numpy(np.ascontiguousarray(pos[:, a])).to(dev) for a in range(3)]
+    m = torch.from_numpy(np.ascontiguousarray(mass)).to(dev)
+    g = D.grid_struct(spec["nx"], spec["ny"], spec["nz"], spec["xmin"], spec["ymin"],
+                      spec["zmin"], spec["cellsize"])
+    rng = np.random.default_rng(20261203 + 4)
+    wt = torch.from_numpy(rng.normal(size=n).astype(np.float32)).to(dev)
+    wk = torch.from_numpy(rng.normal(size=n).astype(np.float32)).to(dev)
+
+    def fwd():
+        with torch.no_grad():
+            return density_stage(xyz[0], xyz[1], xyz[2], m, g, spec["RG"], k, True)
+
+    def fwd_bwd():
+        leaves = [t.detach().requires_grad_() for t in xyz + [m]]
+        rt, rk, _, _ = density_stage(leaves[0], leaves[1], leaves[2], leaves[3], g,
+                                     spec["RG"], k, True)
+        loss = (wt * torch.log(rt)).sum() + (wk * torch.log(rk)).sum()
+        loss.backward()
+        return leaves
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        D.dev_status()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        D.dev_status()
+        return (t1 - t0) * 1e3 / steps
+
+    f_ms = timed(fwd)
+    fb_ms = timed(fwd_bwd)
+    D.timing(enable=True, reset=True)
+    for _ in range(steps):
+        fwd_bwd()
+    torch.cuda.synchronize()
+    phases = {}
+    for ph in ("bwd", "bwd_tsc", "bwd_knn"):
+        tot, cnt = D.timing_get(ph)
+        if cnt:
+            phases[ph] = round(tot / cnt, 4)
+    D.timing(enable=False, reset=True)
+    print("[bench] grad: fwd %.3f ms, fwd+bwd %.3f ms" % (f_ms, fb_ms), file=sys.stderr,
+          flush=True)
+    return {
+        "workload": "one halo, N=%d, %d^3 grid, RG=%.1f cells, k=%d, mass-weighted W4; "
+                    "loss sum w log rho over both densities" % (n, spec["nx"], spec["rg_cells"], k),
+        "particles": n,
+        "fwd_ms": round(f_ms, 4),
+        "fwd_bwd_ms": round(fb_ms, 4),
+        "fwd_bwd_over_fwd": round(fb_ms / f_ms, 3),
+        "value": round(n / (fb_ms * 1e-3), 1),
+        "unit": "particles/s (fwd+bwd)",
+        "phase_ms": phases,
     }
 
 
@@ -104,7 +272,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 4])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--knn-sample", type=int, default=1000000)
+    ap.add_argument("--knn-sample", type=int, default=1 << 30)
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched 10000-halo catalogue measurement")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the batched catalogue measurement (no per-halo pass)")
+    ap.add_argument("--no-grad", action="store_true",
+                    help="skip the forward+backward measurement")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain %d-byte device copy (counter calibration)"
                     % CALIB_COPY_BYTES)
@@ -119,6 +293,10 @@ def main():
     from galaxyfinder_amd import density as D
     from galaxyfinder_amd import halos
 
+    if args.batch_only:
+        print(json.dumps({"batch": batch_bench(args.steps, args.warmup, plain_pass=False)}),
+              flush=True)
+        return
     pos, mass, spec = halos.single_halo_config(args.config)
     n, k = len(pos), spec["k"]
     print("[bench] workload config %d: N=%d grid=%d^3 RG=%.2f cells k=%d"
@@ -207,6 +385,10 @@ def main():
         "stage_roofline_frac": round(stage_alg / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": roofline,
     }
+    if not args.no_batch:
+        out["batch"] = batch_bench(max(3, args.steps // 2), 2)
+    if not args.no_grad:
+        out["grad"] = grad_bench(max(3, args.steps // 2), 2)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(pos.astype(np.float64), mass.astype(np.float64),
                                            spec, k, args.knn_sample)

@@ -49,6 +49,10 @@ SIGNATURES = {
     "pgalf_dev_density_stage": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_ll,
                                        ctypes.POINTER(pgalf_grid_t), _c_f, _c_i, _c_p, _c_p,
                                        _c_p, _c_p, _c_p]),
+    "pgalf_dev_density_stage_backward": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_ll,
+                                                ctypes.POINTER(pgalf_grid_t), _c_f, _c_i,
+                                                _c_p, _c_p, _c_i, _c_p, _c_p, _c_p, _c_p,
+                                                _c_p, _c_p, _c_p]),
     "pgalf_dev_status": (_c_i, [_c_p]),
     "pgalf_batch_create": (_c_p, [_c_i, _c_p, _c_p, _c_p, _c_i]),
     "pgalf_batch_grid_offset": (_c_ll, [_c_p, _c_i]),

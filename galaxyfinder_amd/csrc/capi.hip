@@ -592,6 +592,47 @@ extern "C" int pgalf_dev_density_stage(const float *x, const float *y, const flo
     return e == hipSuccess ? ok() : hip_err(e, "pgalf_dev_density_stage");
 }
 
+extern "C" int pgalf_dev_density_stage_backward(
+    const float *x, const float *y, const float *z, const float *m, long long np,
+    const pgalf_grid_t *g, float RG, int k, const float *den, const long long *nbr,
+    int rho_mode, const float *g_tsc, const float *g_knn, float *gx, float *gy, float *gz,
+    float *gm, void *stream) {
+    const char *where = "pgalf_dev_density_stage_backward";
+    if (!g || np < 1 || !x || !y || !z || !m || !check_grid(g->nx, g->ny, g->nz) ||
+        (g_tsc && !den) || (g_knn && (!nbr || k < 1 || k > 32 || k > np)) ||
+        (g_knn && rho_mode != 1 && rho_mode != 2))
+        return set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    hipStream_t s = (hipStream_t)stream;
+    Workspace &ws = workspace_for_current_device();
+    hipError_t e = ws.begin(s);
+    const Grid gg = grid_from(g);
+    double *acc = ws.alloc_n<double>(4 * (size_t)np);
+    if (!acc) e = hipErrorOutOfMemory;
+    timing_mark("bwd", s, false);
+    if (e == hipSuccess) e = fill_u64(reinterpret_cast<unsigned long long *>(acc), 4 * np, 0ull, s);
+    if (e == hipSuccess && g_tsc) {
+        // adjoint grid A = S D_g (deposit of the read-back cotangent, smoothed:
+        // the Gaussian filter is real and even, so S is self-adjoint)
+        timing_mark("bwd_tsc", s, false);
+        float *A = ws.alloc_n<float>(gg.ncells());
+        if (!A) e = hipErrorOutOfMemory;
+        if (e == hipSuccess)
+            e = tsc_deposit<float>(x, y, z, g_tsc, nullptr, np, PGALF_TYPE_ALL, gg, A, ws, s);
+        if (e == hipSuccess) e = gaussian_smooth(A, gg, RG, ws, s);
+        if (e == hipSuccess) e = tsc_grad_readback<float>(x, y, z, m, g_tsc, np, gg, den, A, acc, ws, s);
+        timing_mark("bwd_tsc", s, true);
+    }
+    if (e == hipSuccess && g_knn) {
+        timing_mark("bwd_knn", s, false);
+        e = knn_density_backward<float>(x, y, z, m, np, nbr, k, rho_mode == 2 ? 1 : 0, g_knn, acc, s);
+        timing_mark("bwd_knn", s, true);
+    }
+    if (e == hipSuccess) e = grad_finalize(acc, np, gx, gy, gz, gm, s);
+    timing_mark("bwd", s, true);
+    return e == hipSuccess ? ok() : hip_err(e, where);
+}
+
 extern "C" int pgalf_dev_status(void *stream) {
     std::lock_guard<std::mutex> lk(g_api_mu);
     hipStream_t s = (hipStream_t)stream;

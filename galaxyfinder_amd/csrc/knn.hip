@@ -1112,6 +1112,113 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
     return hipGetLastError();
 }
 
+// ---- adjoint of the W4 kNN density (no counterpart in the reference) -------
+namespace {
+
+// M4 spline of W(r, h) = f(q) / (pi h^3), q = r/h, and f'(q) (nnost.c:536-555).
+__device__ inline void m4_f(double q, double &f, double &fp) {
+    if (q < 1.0) {
+        f = 1.0 - 1.5 * q * q + 0.75 * q * q * q;
+        fp = -3.0 * q + 2.25 * q * q;
+    } else if (q < 2.0) {
+        const double t = 2.0 - q;
+        f = 0.25 * t * t * t;
+        fp = -0.75 * t * t;
+    } else {
+        f = 0.0;
+        fp = 0.0;
+    }
+}
+
+// One thread per particle i over its forward list (FP64):
+//   rho_i = sum_t m_t W(r_it, h_i), h_i = r_i,(k-1) / 2,
+//   dW/dr = f'(q) / (pi h^4), dW/dh = -(3 f + q f') / (pi h^4);
+// i's own terms and its neighbours' (opposite sign) go to acc by FP64
+// atomic adds (acc = [gx | gy | gz | gm]).
+template <class P>
+__global__ void __launch_bounds__(256)
+knn_backward_k(const P *x, const P *y, const P *z, const P *m, long long np,
+               const long long *nbr, int k, int weighted, const float *gin, double *acc) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const double g = (double)gin[i];
+    if (g == 0.0) return;
+    const double xi = (double)x[i], yi = (double)y[i], zi = (double)z[i];
+    const long long *li = nbr + i * k;
+    const long long K = li[k - 1];
+    const double kx = xi - (double)x[K], ky = yi - (double)y[K], kz = zi - (double)z[K];
+    const double rK = sqrt(kx * kx + ky * ky + kz * kz), hh = 0.5 * rK;
+    if (!(hh > 0.0)) return;  // k-th neighbour at zero distance: no finite density
+    const double h3 = M_PI * hh * hh * hh, h4 = h3 * hh;
+    double ox = 0.0, oy = 0.0, oz = 0.0, sh = 0.0;
+    for (int t = 0; t < k; t++) {
+        const long long j = li[t];
+        const double dx = xi - (double)x[j], dy = yi - (double)y[j], dz = zi - (double)z[j];
+        const double r = sqrt(dx * dx + dy * dy + dz * dz), q = r / hh;
+        double f, fp;
+        m4_f(q, f, fp);
+        const double mt = weighted ? (double)m[j] : 1.0;
+        sh += mt * (-(3.0 * f + q * fp) / h4);
+        if (weighted && f != 0.0) unsafeAtomicAdd(&acc[3 * np + j], g * f / h3);
+        if (r > 0.0 && fp != 0.0) {
+            const double c = g * mt * (fp / h4) / r;
+            ox += c * dx;
+            oy += c * dy;
+            oz += c * dz;
+            unsafeAtomicAdd(&acc[j], -c * dx);
+            unsafeAtomicAdd(&acc[np + j], -c * dy);
+            unsafeAtomicAdd(&acc[2 * np + j], -c * dz);
+        }
+    }
+    const double c = g * sh * 0.5 / rK;
+    ox += c * kx;
+    oy += c * ky;
+    oz += c * kz;
+    unsafeAtomicAdd(&acc[K], -c * kx);
+    unsafeAtomicAdd(&acc[np + K], -c * ky);
+    unsafeAtomicAdd(&acc[2 * np + K], -c * kz);
+    unsafeAtomicAdd(&acc[i], ox);
+    unsafeAtomicAdd(&acc[np + i], oy);
+    unsafeAtomicAdd(&acc[2 * np + i], oz);
+}
+
+__global__ void grad_finalize_k(const double *acc, long long np, float *gx, float *gy,
+                                float *gz, float *gm) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    if (gx) gx[i] = (float)acc[i];
+    if (gy) gy[i] = (float)acc[np + i];
+    if (gz) gz[i] = (float)acc[2 * np + i];
+    if (gm) gm[i] = (float)acc[3 * np + i];
+}
+
+}  // namespace
+
+template <class P>
+hipError_t knn_density_backward(const P *x, const P *y, const P *z, const P *m, long long np,
+                                const long long *nbr, int k, int weighted, const float *gin,
+                                double *acc, hipStream_t s) {
+    if (np <= 0) return hipSuccess;
+    if (k < 1 || k > kTop) return hipErrorInvalidValue;
+    knn_backward_k<P><<<nblk(np, 256), 256, 0, s>>>(x, y, z, m, np, nbr, k, weighted, gin, acc);
+    return hipGetLastError();
+}
+
+hipError_t grad_finalize(const double *acc, long long np, float *gx, float *gy, float *gz,
+                         float *gm, hipStream_t s) {
+    if (np <= 0) return hipSuccess;
+    grad_finalize_k<<<nblk(np, 256), 256, 0, s>>>(acc, np, gx, gy, gz, gm);
+    return hipGetLastError();
+}
+
+template hipError_t knn_density_backward<float>(const float *, const float *, const float *,
+                                                const float *, long long, const long long *,
+                                                int, int, const float *, double *, hipStream_t);
+template hipError_t knn_density_backward<double>(const double *, const double *,
+                                                 const double *, const double *, long long,
+                                                 const long long *, int, int, const float *,
+                                                 double *, hipStream_t);
+
 hipError_t knn_batch_plan(const long long *off, int nhalo, int k, KnnBatchPlan &plan) {
     if (nhalo < 1 || k < 1 || k > kTop || off[0] != 0) return hipErrorInvalidValue;
     KnnBatchPlan p;

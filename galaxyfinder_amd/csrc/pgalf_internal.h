@@ -120,6 +120,13 @@ hipError_t tsc_readback_batch(const P *x, const P *y, const P *z, const BatchDev
                               hipStream_t s);
 // Deposit tiles of one grid (16^3 nodes over the padded mx x ny x nz box).
 long long tsc_tile_count(const Grid &g);
+// Adjoint of read-back and deposit (see tsc.hip): adds dL/dx, dL/dy, dL/dz,
+// dL/dm of L(rho_tsc) to acc = [gx | gy | gz | gm] (FP64, np each) from
+// gin = dL/drho_tsc, the forward smoothed grid F and the adjoint grid A.
+template <class P>
+hipError_t tsc_grad_readback(const P *x, const P *y, const P *z, const P *m, const float *gin,
+                             long long np, const Grid &g, const float *F, const float *A,
+                             double *acc, Workspace &ws, hipStream_t s);
 
 // ---- Gaussian smoothing (smooth.hip) ------------------------------------
 hipError_t gaussian_smooth(float *den, const Grid &g, float RG, Workspace &ws,
@@ -172,6 +179,17 @@ template <class P>
 hipError_t knn_batch(const P *x, const P *y, const P *z, const P *m, const BatchDev &b,
                      const KnnBatchPlan &plan, long long *nbr, float *rho, int rho_mode,
                      Workspace &ws, hipStream_t s);
+
+// Adjoint of the W4 kNN density over the forward lists nbr (np x k, indices
+// into the same arrays): adds dL/dx.. and, when weighted, dL/dm of
+// L(rho_knn) to acc = [gx | gy | gz | gm] (FP64 atomics), gin = dL/drho_knn.
+template <class P>
+hipError_t knn_density_backward(const P *x, const P *y, const P *z, const P *m, long long np,
+                                const long long *nbr, int k, int weighted, const float *gin,
+                                double *acc, hipStream_t s);
+// out[i] = (float)acc[i] for the four gradient arrays (any may be NULL).
+hipError_t grad_finalize(const double *acc, long long np, float *gx, float *gy, float *gz,
+                         float *gm, hipStream_t s);
 
 // AoS SimpleBasicParticleType (88 B) -> SoA double arrays on device.
 hipError_t unpack_particles(const void *d_bp, long long np, double *x, double *y,

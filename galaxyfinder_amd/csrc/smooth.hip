@@ -37,7 +37,11 @@ namespace pgalf {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kLdsComplex = 4096;  // complex FP64 elements of line data per block (64 KB)
+// Complex FP64 elements of line data per block (P * L <= LC): it sizes the
+// per-thread butterfly registers of a pass.  LC = 2048 keeps the kernel at
+// ~60 VGPRs; the longest lines (L = 4096) use the LC = 4096 instance.
+constexpr int kLdsComplex = 2048;
+constexpr int kLdsComplexMax = 4096;
 constexpr int kMaxL = 4096;
 
 struct AxisPass {
@@ -51,6 +55,8 @@ struct AxisPass {
     int contiguous;     // 1: lines are contiguous (x pass)
     int zero_pads;      // x pass: zero columns [n, padw) of each row
     int padw;           // row width for the pad zeroing (mx)
+    int LS;             // line stride in LDS (complex elements)
+    int tables;         // 1: twiddles and spectral factor staged in LDS
 };
 
 __device__ inline double2 cmul(double2 a, double2 b) {
@@ -59,12 +65,12 @@ __device__ inline double2 cmul(double2 a, double2 b) {
 
 // One Stockham pass over P lines of length L in LDS (in place: all inputs of
 // this thread are read before the barrier, outputs written after it).
-template <int R>
-__device__ inline void stockham_pass(double2 *buf, int L, int P, int Ns, const double2 *tw,
-                                     bool inverse) {
-    // at most kLdsComplex/R butterflies per pass -> kMaxBf per thread; the
+template <int R, int LC>
+__device__ inline void stockham_pass(double2 *buf, int L, int LS, int P, int Ns,
+                                     const double2 *tw, bool inverse) {
+    // at most LC/R butterflies per pass -> kMaxBf per thread; the
     // per-thread loop is fully unrolled so v[][] stays in registers.
-    constexpr int kMaxBf = kLdsComplex / R / kBlock;
+    constexpr int kMaxBf = LC / R / kBlock;
     const int nbf_line = L / R;
     const int nbf = P * nbf_line;
     const int tstep = L / (Ns * R);  // twiddle index step
@@ -75,7 +81,7 @@ __device__ inline void stockham_pass(double2 *buf, int L, int P, int Ns, const d
         int b = threadIdx.x + c * kBlock;
         if (b >= nbf) continue;
         int p = b / nbf_line, j = b - p * nbf_line;
-        double2 *line = buf + p * L;
+        double2 *line = buf + p * LS;
         int k = j % Ns;
 #pragma unroll
         for (int r = 0; r < R; r++) {
@@ -104,7 +110,7 @@ __device__ inline void stockham_pass(double2 *buf, int L, int P, int Ns, const d
             v[c][2] = make_double2(s02.x - s13.x, s02.y - s13.y);
             v[c][3] = make_double2(d02.x - md13.x, d02.y - md13.y);
         }
-        dst[c] = p * L + (j / Ns) * Ns * R + k;
+        dst[c] = p * LS + (j / Ns) * Ns * R + k;
     }
     __syncthreads();
 #pragma unroll
@@ -117,56 +123,95 @@ __device__ inline void stockham_pass(double2 *buf, int L, int P, int Ns, const d
     __syncthreads();
 }
 
-__device__ inline void fft_lines(double2 *buf, int L, int P, const double2 *tw, bool inverse) {
+template <int LC>
+__device__ inline void fft_lines(double2 *buf, int L, int LS, int P, const double2 *tw,
+                                 bool inverse) {
     int Ns = 1;
     while (Ns * 4 <= L) {
-        stockham_pass<4>(buf, L, P, Ns, tw, inverse);
+        stockham_pass<4, LC>(buf, L, LS, P, Ns, tw, inverse);
         Ns *= 4;
     }
-    if (Ns < L) stockham_pass<2>(buf, L, P, Ns, tw, inverse);
+    if (Ns < L) stockham_pass<2, LC>(buf, L, LS, P, Ns, tw, inverse);
 }
+
+// LDS per block: line data (P lines of LS complex), plus the twiddles and
+// the spectral factor when L <= kLdsTables.  LS = L + 1: one complex element
+// of padding per line, so the strided (y/z) passes, whose consecutive
+// threads write consecutive lines, hit different banks.
+constexpr int kLdsBytes = 65536;
+constexpr int kLdsTables = 1024;
+constexpr int kLoadBatch = 16;  // global loads in flight per thread in the load phase
 
 // blockIdx.x: batch of 2P lines; blockIdx.y: slow batch index; blockIdx.z:
 // grid of a batch of same-shape grids at offsets goffs (nullptr: one grid).
+template <int LC>
 __global__ void __launch_bounds__(kBlock)
-smooth_axis_k(float *g, AxisPass ap, const double2 *tw, const double *H,
+smooth_axis_k(float *g, AxisPass ap, const double2 *tw_g, const double *H_g,
               const long long *goffs) {
     extern __shared__ double2 buf[];
-    const int L = ap.L, P = ap.P, n = ap.n;
+    const int L = ap.L, P = ap.P, n = ap.n, LS = ap.LS;
     const long long ib0 = (long long)blockIdx.x * 2 * P;
     float *base = g + (goffs ? goffs[blockIdx.z] : 0LL) + (long long)blockIdx.y * ap.sc;
+    const bool tables = ap.tables != 0;
+    double2 *s_tw = buf + P * LS;
+    double *s_H = reinterpret_cast<double *>(s_tw + L);
+    if (tables) {
+        for (int i = threadIdx.x; i < L; i += kBlock) {
+            s_tw[i] = tw_g[i];
+            s_H[i] = H_g[i];
+        }
+    }
+    const double2 *tw = tables ? s_tw : tw_g;
+    const double *H = tables ? s_H : H_g;
     const int lines = 2 * P;
-    // load: line lb (0..2P-1) element j -> buf[(lb/2)*L + j].(x|y)
+    // load: line lb (0..2P-1) element j -> buf[(lb/2)*LS + j].(x|y); kLoadBatch
+    // independent loads per thread are issued before their LDS stores so
+    // their latencies overlap
     const int tot = lines * n;
-    for (int idx = threadIdx.x; idx < tot; idx += kBlock) {
-        int lb, j;
-        if (ap.contiguous) { lb = idx / n; j = idx - lb * n; }
-        else { j = idx / lines; lb = idx - j * lines; }
-        long long ib = ib0 + lb;
-        float v = ib < ap.nb ? base[ib * ap.sb + (long long)j * ap.sa] : 0.0f;
-        double *dst = reinterpret_cast<double *>(&buf[(lb >> 1) * L + j]);
-        dst[lb & 1] = (double)v;
+    for (int i0 = threadIdx.x; i0 < tot; i0 += kBlock * kLoadBatch) {
+        float v[kLoadBatch];
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; u++) {
+            const int idx = i0 + u * kBlock;
+            int lb = 0, j = 0;
+            if (ap.contiguous) { lb = idx / n; j = idx - lb * n; }
+            else { j = idx / lines; lb = idx - j * lines; }
+            const long long ib = ib0 + lb;
+            v[u] = (idx < tot && ib < ap.nb) ? base[ib * ap.sb + (long long)j * ap.sa] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; u++) {
+            const int idx = i0 + u * kBlock;
+            if (idx < tot) {
+                int lb, j;
+                if (ap.contiguous) { lb = idx / n; j = idx - lb * n; }
+                else { j = idx / lines; lb = idx - j * lines; }
+                double *dst = reinterpret_cast<double *>(&buf[(lb >> 1) * LS + j]);
+                dst[lb & 1] = (double)v[u];
+            }
+        }
     }
     for (int idx = threadIdx.x; idx < P * (L - n); idx += kBlock) {
         int p = idx / (L - n), j = n + idx % (L - n);
-        buf[p * L + j] = make_double2(0.0, 0.0);
+        buf[p * LS + j] = make_double2(0.0, 0.0);
     }
     __syncthreads();
-    fft_lines(buf, L, P, tw, false);
+    fft_lines<LC>(buf, L, LS, P, tw, false);
     for (int idx = threadIdx.x; idx < P * L; idx += kBlock) {
-        double h = H[idx & (L - 1)];
-        double2 a = buf[idx];
-        buf[idx] = make_double2(a.x * h, a.y * h);
+        const int p = idx / L, j = idx - p * L;
+        double h = H[j];
+        double2 a = buf[p * LS + j];
+        buf[p * LS + j] = make_double2(a.x * h, a.y * h);
     }
     __syncthreads();
-    fft_lines(buf, L, P, tw, true);
+    fft_lines<LC>(buf, L, LS, P, tw, true);
     for (int idx = threadIdx.x; idx < tot; idx += kBlock) {
         int lb, j;
         if (ap.contiguous) { lb = idx / n; j = idx - lb * n; }
         else { j = idx / lines; lb = idx - j * lines; }
         long long ib = ib0 + lb;
         if (ib < ap.nb) {
-            const double *src = reinterpret_cast<const double *>(&buf[(lb >> 1) * L + j]);
+            const double *src = reinterpret_cast<const double *>(&buf[(lb >> 1) * LS + j]);
             base[ib * ap.sb + (long long)j * ap.sa] = (float)src[lb & 1];
         }
     }
@@ -275,20 +320,36 @@ hipError_t run_pass(float *g, const AxisPlan &pl, long long sa, int nb, long lon
     ap.nb = nb;
     ap.sb = sb;
     ap.sc = sc;
-    int P = kLdsComplex / pl.L;
-    if (P < 1) P = 1;
+    // P * L <= LC bounds the butterflies per thread of a pass
+    const int LC = pl.L > kLdsComplex ? kLdsComplexMax : kLdsComplex;
+    ap.tables = pl.L <= kLdsTables ? 1 : 0;
+    const size_t tbytes = ap.tables ? (sizeof(double2) + sizeof(double)) * (size_t)pl.L : 0;
+    ap.LS = pl.L + 1;
+    int P = (int)((kLdsBytes - tbytes) / (sizeof(double2) * (size_t)ap.LS));
+    if (P > LC / pl.L) P = LC / pl.L;
+    if (P < 1) {  // one line of the longest length fills the block unpadded
+        P = 1;
+        ap.LS = pl.L;
+    }
+    int p2 = 1;  // a power of two: line counts (grid sides) divide evenly
+    while (2 * p2 <= P) p2 *= 2;
+    P = p2;
     int maxP = (nb + 1) / 2;
     if (P > maxP) P = maxP;
     ap.P = P;
     ap.contiguous = contiguous;
     ap.zero_pads = zero_pads;
     ap.padw = padw;
-    size_t lds = sizeof(double2) * (size_t)P * pl.L;
+    const size_t lds = sizeof(double2) * (size_t)P * ap.LS + tbytes;
     constexpr int kMaxZ = 65535;  // grid dimension limit: larger batches in several launches
     for (int z0 = 0; z0 < ngrids; z0 += kMaxZ) {
         const int nz = ngrids - z0 < kMaxZ ? ngrids - z0 : kMaxZ;
         dim3 grid((unsigned)((nb + 2 * P - 1) / (2 * P)), (unsigned)nc, (unsigned)nz);
-        smooth_axis_k<<<grid, kBlock, lds, s>>>(g, ap, pl.tw, pl.H, goffs ? goffs + z0 : nullptr);
+        const long long *gz = goffs ? goffs + z0 : nullptr;
+        if (LC == kLdsComplex)
+            smooth_axis_k<kLdsComplex><<<grid, kBlock, lds, s>>>(g, ap, pl.tw, pl.H, gz);
+        else
+            smooth_axis_k<kLdsComplexMax><<<grid, kBlock, lds, s>>>(g, ap, pl.tw, pl.H, gz);
     }
     return hipGetLastError();
 }

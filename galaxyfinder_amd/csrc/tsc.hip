@@ -291,6 +291,15 @@ tsc_deposit_k(const P *x, const P *y, const P *z, const P *m, const HaloSeg *seg
         int ty = (int)((tl / tg.ntx) % tg.nty);
         int tz = (int)(tl / ((unsigned)tg.ntx * tg.nty));
         long long ox = (long long)tx * kT, oy = (long long)ty * kT, oz = (long long)tz * kT;
+        if (tcount[t] == 0) {
+            // empty tile (block-uniform): zeros, no accumulator, no barrier
+            for (int l = threadIdx.x; l < kTileNodes; l += kBlock) {
+                long long gx = ox + (l % kT), gy = oy + ((l / kT) % kT), gz = oz + l / (kT * kT);
+                if (gx < mx && gy < g.ny && gz < g.nz)
+                    hden[gx + mx * (gy + (long long)g.ny * gz)] = 0.0f;
+            }
+            continue;
+        }
         for (int i = threadIdx.x; i < kTileNodes; i += kBlock) acc[i] = 0ull;
         __syncthreads();
         unsigned e0 = tstart[t] + chunk * kChunk;
@@ -436,6 +445,73 @@ tsc_readback_k(const P *x, const P *y, const P *z, const HaloSeg *segs, HaloSeg 
     }
 }
 
+// ---- adjoint (no counterpart in the reference) -----------------------------
+// One axis in FP64: nodes (nearest rint(p), the offset's side, the other
+// side), weights w(u) = 0.75 - u^2 (|u| <= 0.5), 0.5 (1.5 - |u|)^2 (0.5 <
+// |u| < 1.5) of u = p - node, and their derivatives dw/dp.
+__device__ inline void grad_axis(double p, long id[3], double w[3], double dw[3]) {
+    const long nr = (long)rint(p);
+    const double off = p - (double)nr;
+    const double sg = off < 0.0 ? -1.0 : 1.0;
+    const double d = fabs(off);
+    id[0] = nr;
+    id[1] = nr + (long)sg;
+    id[2] = nr - (long)sg;
+    w[0] = 0.75 - d * d;
+    dw[0] = -2.0 * off;
+    w[1] = 0.5 * (0.5 + d) * (0.5 + d);
+    dw[1] = (0.5 + d) * sg;
+    w[2] = 0.5 * (0.5 - d) * (0.5 - d);
+    dw[2] = -(0.5 - d) * sg;
+}
+
+// Gradient read-back of L(rho_tsc), rho_tsc = R S D: with gin = dL/drho,
+// F the forward smoothed grid and A = S D_gin the adjoint grid,
+//   dL/dm_p = sum_nodes W A,   dL/dx_p = (1/h) sum_nodes dW/dx (g_p F + m_p A).
+// acc holds [gx | gy | gz | gm] (np each, FP64); this kernel adds its part.
+template <class P>
+__global__ void __launch_bounds__(kBlock)
+tsc_gradback_k(const P *x, const P *y, const P *z, const P *m, const float *gin, long long np,
+               Grid g, const float *F, const float *A, double *acc, unsigned *status) {
+    const long long p = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= np) return;
+    long id[3][3];
+    double w[3][3], dw[3][3];
+    grad_axis(((double)x[p] - g.x0) / g.h, id[0], w[0], dw[0]);
+    grad_axis(((double)y[p] - g.y0) / g.h, id[1], w[1], dw[1]);
+    grad_axis(((double)z[p] - g.z0) / g.h, id[2], w[2], dw[2]);
+    const long mx = (long)g.mx();
+    const long n[3] = {mx, g.ny, g.nz};
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+            if (id[a][q] < 0 || id[a][q] >= n[a]) {
+                atomicOr(status, (unsigned)DEV_ERR_OUT_OF_GRID);
+                return;
+            }
+    const double gp = (double)gin[p], mp = (double)m[p];
+    double sx = 0.0, sy = 0.0, sz = 0.0, sm = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const long o = id[0][a] + mx * (id[1][b] + (long)g.ny * id[2][c]);
+                const double fa = (double)A[o];
+                const double v = gp * (double)F[o] + mp * fa;
+                sx += dw[0][a] * w[1][b] * w[2][c] * v;
+                sy += w[0][a] * dw[1][b] * w[2][c] * v;
+                sz += w[0][a] * w[1][b] * dw[2][c] * v;
+                sm += w[0][a] * w[1][b] * w[2][c] * fa;
+            }
+    acc[p] += sx / g.h;
+    acc[np + p] += sy / g.h;
+    acc[2 * np + p] += sz / g.h;
+    acc[3 * np + p] += sm;
+}
+
 HaloSeg single_seg(const Grid &g, long long np) {
     HaloSeg sg;
     sg.p0 = 0;
@@ -566,7 +642,21 @@ hipError_t tsc_readback_batch(const P *x, const P *y, const P *z, const BatchDev
     return hipGetLastError();
 }
 
+template <class P>
+hipError_t tsc_grad_readback(const P *x, const P *y, const P *z, const P *m, const float *gin,
+                             long long np, const Grid &g, const float *F, const float *A,
+                             double *acc, Workspace &ws, hipStream_t s) {
+    if (np <= 0) return hipSuccess;
+    tsc_gradback_k<P><<<(unsigned)((np + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+        x, y, z, m, gin, np, g, F, A, acc, ws.status());
+    return hipGetLastError();
+}
+
 #define PGALF_TSC_INST(P)                                                                      \
+    template hipError_t tsc_grad_readback<P>(const P *, const P *, const P *, const P *,       \
+                                             const float *, long long, const Grid &,           \
+                                             const float *, const float *, double *,           \
+                                             Workspace &, hipStream_t);                        \
     template hipError_t tsc_deposit<P>(const P *, const P *, const P *, const P *, const int *, \
                                        long long, int, const Grid &, float *, Workspace &,      \
                                        hipStream_t);                                           \

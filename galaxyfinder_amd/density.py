@@ -233,6 +233,31 @@ def dev_density_stage(x, y, z, m, grid, RG, k, den, rho_tsc=None, nbr=None, rho_
                                            _stream()))
 
 
+def dev_density_stage_backward(x, y, z, m, grid, RG, k, den, nbr, rho_mode, g_tsc=None,
+                               g_knn=None):
+    """Adjoint of the device stage: returns (dL/dx, dL/dy, dL/dz, dL/dm)
+    float32 tensors from the cotangents of the read-back density (g_tsc)
+    and of the W4 kNN density (g_knn, rho_mode 1 unweighted / 2 weighted)."""
+    torch = _torch()
+    n = x.numel()
+    for t, w in ((x, "x"), (y, "y"), (z, "z"), (m, "m")):
+        _check_dev(t, torch.float32, n, w)
+    if g_tsc is not None:
+        _check_dev(g_tsc, torch.float32, n, "g_tsc")
+        _check_dev(den, torch.float32, 2 * (grid.nx // 2 + 1) * grid.ny * grid.nz, "den")
+    if g_knn is not None:
+        _check_dev(g_knn, torch.float32, n, "g_knn")
+        _check_dev(nbr, torch.int64, n * k, "nbr")
+    out = [torch.empty(n, dtype=torch.float32, device=x.device) for _ in range(4)]
+    lib = _lib.load()
+    _lib.check(lib.pgalf_dev_density_stage_backward(
+        _dptr(x), _dptr(y), _dptr(z), _dptr(m), n, ctypes.byref(grid), float(RG), int(k),
+        _dptr(den) if g_tsc is not None else None, _dptr(nbr) if g_knn is not None else None,
+        int(rho_mode), _dptr(g_tsc), _dptr(g_knn), _dptr(out[0]), _dptr(out[1]),
+        _dptr(out[2]), _dptr(out[3]), _stream()))
+    return tuple(out)
+
+
 class Batch:
     """A catalogue of halos run through the density stage together
     (pgalf_batch_* of the C-ABI) -- the batched counterpart of calling the
@@ -338,4 +363,4 @@ __all__ = ["PgalfError", "PARTICLE_DTYPE", "make_particles", "padded_shape",
            "assign_density_TSC", "gaussian_Smoothing", "findDen", "findsphdensity",
            "starfindsphdensity", "near_lists", "lag_density_stage", "grid_struct",
            "dev_tsc", "dev_smooth", "dev_findden", "dev_knn", "dev_density_stage",
-           "dev_status", "Batch", "timing", "timing_get"]
+           "dev_density_stage_backward", "dev_status", "Batch", "timing", "timing_get"]

@@ -130,6 +130,22 @@ int pgalf_dev_density_stage(const float *x, const float *y, const float *z,
                             const float *m, long long np, const pgalf_grid_t *g,
                             float RG, int k, float *den, float *rho_tsc,
                             long long *nbr, float *rho_knn, void *stream);
+/* Adjoint of the stage (no counterpart in the reference; added for
+ * differentiable density estimates).  With the forward's smoothed grid `den`
+ * (from pgalf_dev_tsc + pgalf_dev_smooth with the same grid and RG) and
+ * neighbour lists `nbr` (pgalf_dev_knn, indices into the same arrays), and
+ * the cotangents g_tsc = dL/drho_tsc (read-back density; NULL: none) and
+ * g_knn = dL/drho_knn (W4 kNN density of rho_mode 1 unweighted / 2
+ * mass-weighted; NULL: none), writes dL/dx, dL/dy, dL/dz, dL/dm (float32,
+ * overwritten; any may be NULL).  Derivatives of the FP64 mathematical
+ * forward (TSC weights, W4 with h = r_k/2) with the lists held fixed;
+ * accumulated in FP64. */
+int pgalf_dev_density_stage_backward(const float *x, const float *y, const float *z,
+                                     const float *m, long long np, const pgalf_grid_t *g,
+                                     float RG, int k, const float *den,
+                                     const long long *nbr, int rho_mode, const float *g_tsc,
+                                     const float *g_knn, float *gx, float *gy, float *gz,
+                                     float *gm, void *stream);
 /* Synchronise `stream` and return the status of the kernels run since the
  * previous call (PGALF_ERR_OUT_OF_GRID / PGALF_ERR_SHORT_LIST or PGALF_OK). */
 int pgalf_dev_status(void *stream);

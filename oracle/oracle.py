@@ -66,6 +66,19 @@ def lib():
         L.orc_sph_density.restype = None
         L.orc_sph_density.argtypes = [_dp, _lp, _i64, _i, _dp, _fp]
         L.orc_num_threads.restype = _i
+        L.orc_tsc_deposit_d.restype = _i64
+        L.orc_tsc_deposit_d.argtypes = [_dp, _dp, _dp, _dp, _i64, _dp, _i, _i, _i,
+                                        _d, _d, _d, _d]
+        L.orc_readback_d.restype = _i64
+        L.orc_readback_d.argtypes = [_dp, _dp, _dp, _i64, _dp, _i, _i, _i, _d, _d, _d, _d, _dp]
+        L.orc_tsc_grad_d.restype = None
+        L.orc_tsc_grad_d.argtypes = [_dp, _dp, _dp, _dp, _dp, _i64, _dp, _dp, _i, _i, _i,
+                                     _d, _d, _d, _d, _dp, _dp, _dp, _dp]
+        L.orc_sph_density_d.restype = None
+        L.orc_sph_density_d.argtypes = [_dp, _dp, _dp, _dp, _lp, _i64, _i, _dp]
+        L.orc_sph_grad_d.restype = None
+        L.orc_sph_grad_d.argtypes = [_dp, _dp, _dp, _dp, _lp, _i64, _i, _dp, _dp, _dp, _dp,
+                                     _dp]
         _LIB = L
     return _LIB
 
@@ -228,3 +241,84 @@ def starfindsphdensity(pos, mass, types, k, kden=10):
 
 def num_threads():
     return int(lib().orc_num_threads())
+
+
+# ---------------------------------------------------------------------------
+# FP64 forward and reference gradients of the stage (the adjoint's checker)
+# ---------------------------------------------------------------------------
+def _xyz(pos):
+    pos = _f64(pos)
+    return tuple(np.ascontiguousarray(pos[:, a]) for a in range(3))
+
+
+def tsc_deposit_d(pos, mass, grid):
+    """FP64 TSC deposit (FP64 weights) onto an unpadded (nz,ny,nx) grid."""
+    nx, ny, nz, x0, y0, z0, h = grid
+    x, y, z = _xyz(pos)
+    m = _f64(mass)
+    den = np.zeros((nz, ny, nx), dtype=np.float64)
+    bad = lib().orc_tsc_deposit_d(_p(x, _dp), _p(y, _dp), _p(z, _dp), _p(m, _dp), len(x),
+                                  _p(den, _dp), nx, ny, nz, x0, y0, z0, h)
+    if bad:
+        raise ValueError("%d particles outside the grid" % bad)
+    return den
+
+
+def readback_d(pos, grid, den3):
+    nx, ny, nz, x0, y0, z0, h = grid
+    x, y, z = _xyz(pos)
+    d = np.ascontiguousarray(den3, dtype=np.float64)
+    out = np.zeros(len(x))
+    lib().orc_readback_d(_p(x, _dp), _p(y, _dp), _p(z, _dp), len(x), _p(d, _dp), nx, ny, nz,
+                         x0, y0, z0, h, _p(out, _dp))
+    return out
+
+
+def sph_density_d(pos, idx, mass=None):
+    """FP64 W4 density over fixed lists (mass-weighted when mass is given)."""
+    x, y, z = _xyz(pos)
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    n, k = idx.shape
+    m = None if mass is None else _f64(mass)
+    out = np.zeros(n)
+    lib().orc_sph_density_d(_p(x, _dp), _p(y, _dp), _p(z, _dp),
+                            None if m is None else _p(m, _dp), _p(idx, _lp), n, k,
+                            _p(out, _dp))
+    return out
+
+
+def stage_forward_d(pos, mass, grid, RG, idx, weighted=True):
+    """FP64 (rho_tsc, rho_knn, F): smoothed-grid read-back density, W4 kNN
+    density over the fixed lists idx, and the smoothed grid F."""
+    h = grid[6]
+    F = gsmooth_f64(tsc_deposit_d(pos, mass, grid), h, RG)
+    return readback_d(pos, grid, F), sph_density_d(pos, idx, mass if weighted else None), F
+
+
+def stage_loss_d(pos, mass, grid, RG, idx, w_tsc, w_knn, weighted=True):
+    """L = sum w_tsc log rho_tsc + sum w_knn log rho_knn (FP64)."""
+    rt, rk, _ = stage_forward_d(pos, mass, grid, RG, idx, weighted)
+    return float(np.sum(w_tsc * np.log(rt)) + np.sum(w_knn * np.log(rk)))
+
+
+def stage_grad_d(pos, mass, grid, RG, idx, w_tsc, w_knn, weighted=True):
+    """Analytic FP64 gradient of stage_loss_d: (dL/dpos (n,3), dL/dm (n,))."""
+    nx, ny, nz, x0, y0, z0, h = grid
+    rt, rk, F = stage_forward_d(pos, mass, grid, RG, idx, weighted)
+    g_t = _f64(w_tsc / rt)
+    g_k = _f64(w_knn / rk)
+    x, y, z = _xyz(pos)
+    m = _f64(mass)
+    n = len(x)
+    gx, gy, gz, gm = (np.zeros(n) for _ in range(4))
+    A = np.ascontiguousarray(gsmooth_f64(tsc_deposit_d(pos, g_t, grid), h, RG))
+    F = np.ascontiguousarray(F)
+    lib().orc_tsc_grad_d(_p(x, _dp), _p(y, _dp), _p(z, _dp), _p(m, _dp), _p(g_t, _dp), n,
+                         _p(F, _dp), _p(A, _dp), nx, ny, nz, x0, y0, z0, h, _p(gx, _dp),
+                         _p(gy, _dp), _p(gz, _dp), _p(gm, _dp))
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    lib().orc_sph_grad_d(_p(x, _dp), _p(y, _dp), _p(z, _dp), _p(m, _dp) if weighted else None,
+                         _p(idx, _lp), n, idx.shape[1], _p(g_k, _dp), _p(gx, _dp),
+                         _p(gy, _dp), _p(gz, _dp), _p(gm, _dp))
+    return np.stack([gx, gy, gz], axis=1), gm
+

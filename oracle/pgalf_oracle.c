@@ -718,3 +718,196 @@ int orc_num_threads(void) {
     return 1;
 #endif
 }
+
+/* ======================================================================== */
+/* FP64 forward and reference gradients of the density stage (no           */
+/* counterpart in the reference: the adjoint is this build's addition).     */
+/* The forward here is the mathematical one in FP64 -- TSC weights of       */
+/* tsc_omp2.c:303-353 / findDen tsc_omp2.c:48-161, FFT smoothing of         */
+/* gsmooth.c:67-104 (orc_gsmooth_f64), W4 of nnost.c:536-555 -- so central  */
+/* finite differences of it pin the analytic gradients below.  Grids are   */
+/* unpadded (nz, ny, nx) doubles; every stencil must lie inside the grid.  */
+/* ======================================================================== */
+
+/* One axis: nodes (nearest rint(p), the node on the offset's side, the
+ * other side), weights w(u) = 0.75 - u^2 (|u| <= 0.5), 0.5 (1.5 - |u|)^2
+ * (0.5 < |u| < 1.5) for u = p - node, and dw/dp. */
+static void orc_tsc_axis_d(double p, long *id, double *w, double *dw) {
+    long nr = (long)rint(p);
+    double off = p - (double)nr;
+    long sg = off < 0.0 ? -1 : 1;
+    double d = fabs(off);
+    id[0] = nr; id[1] = nr + sg; id[2] = nr - sg;
+    w[0] = 0.75 - d * d;            dw[0] = -2.0 * off;
+    w[1] = 0.5 * (0.5 + d) * (0.5 + d); dw[1] = (0.5 + d) * (double)sg;
+    w[2] = 0.5 * (0.5 - d) * (0.5 - d); dw[2] = -(0.5 - d) * (double)sg;
+}
+
+static int orc_stencil_d(double x, double y, double z, int nx, int ny, int nz,
+                         double xmin, double ymin, double zmin, double h,
+                         long id[3][3], double w[3][3], double dw[3][3]) {
+    double p[3] = {(x - xmin) / h, (y - ymin) / h, (z - zmin) / h};
+    long n[3] = {nx, ny, nz};
+    for (int a = 0; a < 3; a++) {
+        orc_tsc_axis_d(p[a], id[a], w[a], dw[a]);
+        for (int q = 0; q < 3; q++)
+            if (id[a][q] < 0 || id[a][q] >= n[a]) return 0;
+    }
+    return 1;
+}
+
+/* den[(k*ny + j)*nx + i] = volfact * sum_p m_p W(x_p - node); returns the
+ * number of particles whose stencil left the grid (skipped). */
+long long orc_tsc_deposit_d(const double *x, const double *y, const double *z,
+                            const double *m, long long np, double *den, int nx, int ny,
+                            int nz, double xmin, double ymin, double zmin, double h) {
+    long long ng = (long long)nx * ny * nz, bad = 0;
+    for (long long i = 0; i < ng; i++) den[i] = 0.0;
+    double vf = orc_volfact();
+    for (long long p = 0; p < np; p++) {
+        long id[3][3]; double w[3][3], dw[3][3];
+        if (!orc_stencil_d(x[p], y[p], z[p], nx, ny, nz, xmin, ymin, zmin, h, id, w, dw)) {
+            bad++;
+            continue;
+        }
+        for (int c = 0; c < 3; c++)
+            for (int b = 0; b < 3; b++)
+                for (int a = 0; a < 3; a++)
+                    den[(id[2][c] * ny + id[1][b]) * nx + id[0][a]] +=
+                        vf * m[p] * w[0][a] * w[1][b] * w[2][c];
+    }
+    return bad;
+}
+
+/* out[p] = sum_nodes W(x_p - node) den(node). */
+long long orc_readback_d(const double *x, const double *y, const double *z, long long np,
+                         const double *den, int nx, int ny, int nz, double xmin,
+                         double ymin, double zmin, double h, double *out) {
+    long long bad = 0;
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+    for (long long p = 0; p < np; p++) {
+        long id[3][3]; double w[3][3], dw[3][3];
+        if (!orc_stencil_d(x[p], y[p], z[p], nx, ny, nz, xmin, ymin, zmin, h, id, w, dw)) {
+            bad++;
+            out[p] = 0.0;
+            continue;
+        }
+        double s = 0.0;
+        for (int c = 0; c < 3; c++)
+            for (int b = 0; b < 3; b++)
+                for (int a = 0; a < 3; a++)
+                    s += w[0][a] * w[1][b] * w[2][c] *
+                         den[(id[2][c] * ny + id[1][b]) * nx + id[0][a]];
+        out[p] = s;
+    }
+    return bad;
+}
+
+/* Gradient of L(rho_tsc) for rho_tsc = R S D: with g = dL/drho_tsc, the
+ * forward smoothed grid F = S D and the adjoint grid A = S D_g (D_g the
+ * deposit of masses g, volfact included):
+ *   dL/dm_p = R_p A
+ *   dL/dx_p = (1/h) sum_nodes dW/dx (g_p F + m_p A)   (likewise y, z).
+ * Results are added to gx, gy, gz, gm. */
+void orc_tsc_grad_d(const double *x, const double *y, const double *z, const double *m,
+                    const double *g, long long np, const double *F, const double *A, int nx,
+                    int ny, int nz, double xmin, double ymin, double zmin, double h,
+                    double *gx, double *gy, double *gz, double *gm) {
+#pragma omp parallel for schedule(static)
+    for (long long p = 0; p < np; p++) {
+        long id[3][3]; double w[3][3], dw[3][3];
+        if (!orc_stencil_d(x[p], y[p], z[p], nx, ny, nz, xmin, ymin, zmin, h, id, w, dw))
+            continue;
+        double sx = 0, sy = 0, sz = 0, sm = 0;
+        for (int c = 0; c < 3; c++)
+            for (int b = 0; b < 3; b++)
+                for (int a = 0; a < 3; a++) {
+                    long long o = (id[2][c] * ny + id[1][b]) * nx + id[0][a];
+                    double v = g[p] * F[o] + m[p] * A[o];
+                    sx += dw[0][a] * w[1][b] * w[2][c] * v;
+                    sy += w[0][a] * dw[1][b] * w[2][c] * v;
+                    sz += w[0][a] * w[1][b] * dw[2][c] * v;
+                    sm += w[0][a] * w[1][b] * w[2][c] * A[o];
+                }
+        gx[p] += sx / h;
+        gy[p] += sy / h;
+        gz[p] += sz / h;
+        gm[p] += sm;
+    }
+}
+
+/* M4 spline f(q) of W(r, h) = f(r/h) / (pi h^3) and f'(q). */
+static void orc_m4(double q, double *f, double *fp) {
+    if (q < 1.0) {
+        *f = 1.0 - 1.5 * q * q + 0.75 * q * q * q;
+        *fp = -3.0 * q + 2.25 * q * q;
+    } else if (q < 2.0) {
+        double t = 2.0 - q;
+        *f = 0.25 * t * t * t;
+        *fp = -0.75 * t * t;
+    } else {
+        *f = 0.0;
+        *fp = 0.0;
+    }
+}
+
+static double orc_dist_d(const double *x, const double *y, const double *z, long long i,
+                         long long j) {
+    double dx = x[i] - x[j], dy = y[i] - y[j], dz = z[i] - z[j];
+    return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+/* rho_i = sum_t m_t W(r_it, h_i), h_i = r_i,(k-1) / 2, over the fixed lists
+ * idx[i*k + t] (m == NULL: unweighted). */
+void orc_sph_density_d(const double *x, const double *y, const double *z, const double *m,
+                       const long long *idx, long long n, int k, double *rho) {
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < n; i++) {
+        const long long *li = idx + i * k;
+        double hh = 0.5 * orc_dist_d(x, y, z, i, li[k - 1]);
+        double s = 0.0;
+        for (int t = 0; t < k; t++) {
+            double f, fp;
+            orc_m4(orc_dist_d(x, y, z, i, li[t]) / hh, &f, &fp);
+            s += (m ? m[li[t]] : 1.0) * f / (M_PI * hh * hh * hh);
+        }
+        rho[i] = s;
+    }
+}
+
+/* Gradient of L(rho_knn) over fixed lists, g = dL/drho_knn:
+ *   dW/dr = f'(q) / (pi h^4),  dW/dh = -(3 f(q) + q f'(q)) / (pi h^4),
+ *   drho_i = sum_t [m_t dW/dr dr_it + m_t dW/dh dh_i + W dm_t],
+ *   dr_it = (x_i - x_t).(dx_i - dx_t) / r_it,  dh_i = dr_i,(k-1) / 2.
+ * Serial (neighbours receive contributions).  Added to gx, gy, gz, gm. */
+void orc_sph_grad_d(const double *x, const double *y, const double *z, const double *m,
+                    const long long *idx, long long n, int k, const double *g, double *gx,
+                    double *gy, double *gz, double *gm) {
+    for (long long i = 0; i < n; i++) {
+        const long long *li = idx + i * k;
+        const long long K = li[k - 1];
+        double rK = orc_dist_d(x, y, z, i, K), hh = 0.5 * rK;
+        double h4 = M_PI * hh * hh * hh * hh;
+        double sh = 0.0;
+        for (int t = 0; t < k; t++) {
+            long long j = li[t];
+            double r = orc_dist_d(x, y, z, i, j), q = r / hh, f, fp;
+            orc_m4(q, &f, &fp);
+            double mt = m ? m[j] : 1.0;
+            sh += mt * (-(3.0 * f + q * fp) / h4);
+            if (m) gm[j] += g[i] * f / (M_PI * hh * hh * hh);
+            if (r > 0.0) {
+                double c = g[i] * mt * (fp / h4) / r;
+                double ex = c * (x[i] - x[j]), ey = c * (y[i] - y[j]), ez = c * (z[i] - z[j]);
+                gx[i] += ex; gy[i] += ey; gz[i] += ez;
+                gx[j] -= ex; gy[j] -= ey; gz[j] -= ez;
+            }
+        }
+        if (rK > 0.0) {
+            double c = g[i] * sh * 0.5 / rK;
+            double ex = c * (x[i] - x[K]), ey = c * (y[i] - y[K]), ez = c * (z[i] - z[K]);
+            gx[i] += ex; gy[i] += ey; gz[i] += ez;
+            gx[K] -= ex; gy[K] -= ey; gz[K] -= ez;
+        }
+    }
+}
