@@ -225,11 +225,108 @@ smooth_axis_k(float *g, AxisPass ap, const double2 *tw_g, const double *H_g,
     }
 }
 
+// ---- direct circular convolution for short lines ------------------------------
+// For n a multiple of 8 up to kDirectMax, an axis pass is the circular
+// convolution out[i] = sum_j g[(i - j) mod n] in[j] with the exact periodic
+// kernel g[m] = (1/n) sum_k f(k) cos(2 pi k m / n) of the spectral factor f
+// (identical in exact arithmetic to the FFT -> multiply -> inverse FFT of the
+// other path).  One thread owns one line: it loads the line into registers
+// (FP64), forms the n outputs with n FMAs each (taps from the kernel
+// arguments, i.e. scalar registers; g is even, so g[m] = g[n - m]) and
+// stores them over the line.  No LDS, no barriers: the pass streams at
+// memory speed, and its n FMAs per point match the FFT's flop count at
+// these lengths.
+constexpr int kDirectMax = 64;
+struct Taps {
+    double g[kDirectMax / 2 + 1];
+};
+struct DirectPass {
+    long long nlines;  // lines of one grid
+    long long na;      // lines along the fast batch dimension
+    long long sa, sb;  // start of line l: (l % na) * sa + (l / na) * sb
+    long long step;    // stride along the line
+    int zero_pads;     // x pass: zero columns [n, padw) of each row
+    int padw;
+};
+
+// out[i * step] = sum_j g[(i - j) mod N] v[j], j ascending, for every i;
+// eight outputs at a time so eight independent FMA chains interleave.
+// The line is held in FP32 (as stored) and each value is widened to FP64
+// once per block of outputs (exact), halving the registers of an FP64 copy.
+template <int N>
+__device__ inline void conv_line(const float (&v)[N], const Taps &taps, float *out,
+                                 long long step) {
+    constexpr int kB = 8;
+    static_assert(N % kB == 0, "whole blocks of outputs");
+#pragma unroll
+    for (int i0 = 0; i0 < N; i0 += kB) {
+        double acc[kB];
+#pragma unroll
+        for (int b = 0; b < kB; b++) acc[b] = 0.0;
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            const double vj = (double)v[j];
+#pragma unroll
+            for (int b = 0; b < kB; b++) {
+                const int d = (i0 + b - j + N) % N;
+                acc[b] = fma(taps.g[d <= N / 2 ? d : N - d], vj, acc[b]);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < kB; b++) out[(long long)(i0 + b) * step] = (float)acc[b];
+    }
+}
+
+// Strided lines (y and z passes): consecutive threads own lines at
+// consecutive x, so every load and store of the wave is coalesced.
+template <int N>
+__global__ void __launch_bounds__(256)
+smooth_direct_k(float *grid, DirectPass dp, Taps taps, const long long *goffs) {
+    const long long line = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (line >= dp.nlines) return;
+    float *p = grid + (goffs ? goffs[blockIdx.y] : 0LL) + (line % dp.na) * dp.sa +
+               (line / dp.na) * dp.sb;
+    float v[N];
+#pragma unroll
+    for (int j = 0; j < N; j++) v[j] = p[(long long)j * dp.step];
+    conv_line<N>(v, taps, p, dp.step);
+}
+
+// Contiguous rows (x pass): the block's BLK consecutive rows (BLK * padw
+// floats, contiguous in the grid) are staged through LDS with a row stride
+// of padw + 1 (odd: the rows a wave reads start in different banks), so the
+// global loads and stores are coalesced; pad columns are written as zero.
+template <int N, int BLK>
+__global__ void __launch_bounds__(BLK)
+smooth_direct_x_k(float *grid, DirectPass dp, Taps taps, const long long *goffs) {
+    extern __shared__ float tile[];
+    // row width mx = 2 (n/2 + 1) = N + 2 for even N (checked by the host)
+    constexpr int W = N + 2, WS = N + 3;
+    const long long row0 = (long long)blockIdx.x * BLK;
+    const int nrows = (int)(dp.nlines - row0 < BLK ? dp.nlines - row0 : BLK);
+    float *base = grid + (goffs ? goffs[blockIdx.y] : 0LL) + row0 * W;
+    const int tot = nrows * W;
+    for (int e = threadIdx.x; e < tot; e += BLK) tile[(e / W) * WS + e % W] = base[e];
+    __syncthreads();
+    if (threadIdx.x < nrows) {
+        float *r = tile + threadIdx.x * WS;
+        float v[N];
+#pragma unroll
+        for (int j = 0; j < N; j++) v[j] = r[j];
+        conv_line<N>(v, taps, r, 1);
+        for (int j = N; j < W; j++) r[j] = 0.0f;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < tot; e += BLK) base[e] = tile[(e / W) * WS + e % W];
+}
+
 // ---- host planning ----------------------------------------------------------
 struct AxisPlan {
     int n = 0, L = 0;
     double2 *tw = nullptr;
     double *H = nullptr;
+    bool direct = false;  // n a multiple of 8 <= kDirectMax: direct convolution
+    Taps taps{};
 };
 
 std::mutex g_plan_mu;
@@ -271,11 +368,11 @@ hipError_t get_plan(int n, double expfac, AxisPlan &out) {
     std::vector<double> f(n);
     for (int k = 0; k < n; k++) f[k] = axis_factor(k, n, expfac);
     std::vector<double> H(L);
-    if (pow2) {
-        for (int k = 0; k < n; k++) H[k] = f[k] / (double)n;
-    } else {
-        // periodic kernel g[m] = (1/n) sum_k f(k) cos(2 pi k m / n)
-        std::vector<long double> gk(n);
+    // periodic kernel g[m] = (1/n) sum_k f(k) cos(2 pi k m / n)
+    const bool direct = n % 8 == 0 && n <= kDirectMax;
+    std::vector<long double> gk;
+    if (direct || !pow2) {
+        gk.resize(n);
         for (int m = 0; m < n; m++) {
             long double s = 0.0L;
             for (int k = 0; k < n; k++)
@@ -283,6 +380,10 @@ hipError_t get_plan(int n, double expfac, AxisPlan &out) {
                      cosl(2.0L * kPi * (long double)(((long)k * m) % n) / (long double)n);
             gk[m] = s / (long double)n;
         }
+    }
+    if (pow2) {
+        for (int k = 0; k < n; k++) H[k] = f[k] / (double)n;
+    } else {
         std::vector<long double> h(L, 0.0L);
         h[0] = gk[0];
         for (int t = 1; t < n; t++) {
@@ -301,6 +402,9 @@ hipError_t get_plan(int n, double expfac, AxisPlan &out) {
     AxisPlan p;
     p.n = n;
     p.L = L;
+    p.direct = direct;
+    if (direct)
+        for (int m = 0; m <= n / 2; m++) p.taps.g[m] = (double)gk[m];
     PGALF_HIP_TRY(hipMalloc(&p.tw, sizeof(double2) * L));
     PGALF_HIP_TRY(hipMalloc(&p.H, sizeof(double) * L));
     PGALF_HIP_TRY(hipMemcpy(p.tw, tw.data(), sizeof(double2) * L, hipMemcpyHostToDevice));
@@ -313,6 +417,55 @@ hipError_t get_plan(int n, double expfac, AxisPlan &out) {
 hipError_t run_pass(float *g, const AxisPlan &pl, long long sa, int nb, long long sb, int nc,
                     long long sc, int contiguous, int zero_pads, int padw, hipStream_t s,
                     const long long *goffs = nullptr, int ngrids = 1) {
+    // direct convolution: strided passes up to kDirectMax, contiguous rows
+    // up to 32 (longer rows need ~255 VGPRs there: the FFT pass is faster)
+    if (pl.direct && !(sa == 1 && pl.n > 32)) {
+        DirectPass dp;
+        dp.na = nb;
+        dp.nlines = (long long)nb * nc;
+        dp.sa = sb;
+        dp.sb = sc;
+        dp.step = sa;
+        dp.zero_pads = zero_pads;
+        dp.padw = padw;
+        constexpr int kMaxY = 65535;
+        // x pass: rows staged through LDS, 256 rows per block
+        const bool rows = sa == 1;
+        if (rows && padw != pl.n + 2) return hipErrorInvalidValue;
+        const int blk = 256;
+        const unsigned gx = (unsigned)((dp.nlines + blk - 1) / blk);
+        const size_t lds = rows ? sizeof(float) * (size_t)blk * (padw + 1) : 0;
+        for (int z0 = 0; z0 < ngrids; z0 += kMaxY) {
+            const int nz = ngrids - z0 < kMaxY ? ngrids - z0 : kMaxY;
+            const long long *gz = goffs ? goffs + z0 : nullptr;
+            const dim3 grid(gx, (unsigned)nz);
+#define PGALF_DIRECT_ROWS(NN)                                                                  \
+    case NN:                                                                                   \
+        if (rows)                                                                              \
+            smooth_direct_x_k<NN, 256><<<grid, 256, lds, s>>>(g, dp, pl.taps, gz);             \
+        else                                                                                   \
+            smooth_direct_k<NN><<<grid, 256, 0, s>>>(g, dp, pl.taps, gz);                      \
+        break;
+#define PGALF_DIRECT_STRIDED(NN)                                                               \
+    case NN:                                                                                   \
+        smooth_direct_k<NN><<<grid, 256, 0, s>>>(g, dp, pl.taps, gz);                          \
+        break;
+            switch (pl.n) {
+                PGALF_DIRECT_ROWS(8)
+                PGALF_DIRECT_ROWS(16)
+                PGALF_DIRECT_ROWS(24)
+                PGALF_DIRECT_ROWS(32)
+                PGALF_DIRECT_STRIDED(40)
+                PGALF_DIRECT_STRIDED(48)
+                PGALF_DIRECT_STRIDED(56)
+                PGALF_DIRECT_STRIDED(64)
+            default: return hipErrorInvalidValue;
+            }
+#undef PGALF_DIRECT_ROWS
+#undef PGALF_DIRECT_STRIDED
+        }
+        return hipGetLastError();
+    }
     AxisPass ap;
     ap.n = pl.n;
     ap.L = pl.L;
