@@ -133,13 +133,16 @@ __device__ inline bool klt(double ad, unsigned ai, double bd, unsigned bi) {
 }
 __device__ inline bool key_lt(const Key &a, const Key &b) { return klt(a.d, a.i, b.d, b.i); }
 
+// Compare-exchange of two keys (a <= b afterwards).  The distances are
+// ordered by min/max (equal distances are interchangeable, and list keys are
+// never NaN); only the indices need the lexicographic select.
 __device__ inline void cas2(double &ad, unsigned &ai, double &bd, unsigned &bi) {
-    bool sw = klt(bd, bi, ad, ai);  // a <= b afterwards
-    double t0 = ad, t2 = bd;
-    unsigned t1 = ai, t3 = bi;
-    ad = sw ? t2 : t0;
+    const bool sw = klt(bd, bi, ad, ai);
+    const double lo = fmin(ad, bd), hi = fmax(ad, bd);
+    const unsigned t1 = ai, t3 = bi;
+    ad = lo;
+    bd = hi;
     ai = sw ? t3 : t1;
-    bd = sw ? t0 : t2;
     bi = sw ? t1 : t3;
 }
 
@@ -566,6 +569,11 @@ __device__ inline void merge_top(double (&td)[kTop], unsigned (&ti)[kTop],
 // t == k-1 would be folded into a dynamic index).
 __device__ inline Key kth_of(const double (&td)[kTop], const unsigned (&ti)[kTop], int k) {
     Key r;
+    if (k == kTop) {  // the common case: the last slot
+        r.d = td[kTop - 1];
+        r.i = ti[kTop - 1];
+        return r;
+    }
     r.d = td[0];
     r.i = ti[0];
 #pragma unroll
