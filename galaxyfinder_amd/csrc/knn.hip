@@ -676,26 +676,45 @@ __device__ inline void scan_list(QState &st, const int *list, int count, bool va
     const unsigned long long t_scan = dbg_clock(dbg);
     const int nmax = wave_max_i(count);
     for (int i0 = 0; i0 < nmax; i0 += kBatch) {
-        // buckets that no lane's current ball reaches are dropped (group lane
-        // t tests bucket i0 + t)
+        // the batch's candidates are loaded together with the boxes of its
+        // buckets (one memory latency per batch); buckets that no lane's
+        // current ball reaches (group lane t tests bucket i0 + t) are then
+        // staged as empty
+        const double nan = __longlong_as_double(0x7ff8000000000000ll);
+        double vx[kBatch], vy[kBatch], vz[kBatch];
+        unsigned ve[kBatch], vid[kBatch];
+#pragma unroll
+        for (int t = 0; t < kBatch; t++) {
+            const int i = i0 + t;
+            long long e = -1;
+            if (i < count) {
+                e = (long long)(list[i] & kBucketMask) * kB + gl;
+                if (e >= nc) e = -1;
+            }
+            vx[t] = nan;
+            vy[t] = 0.0;
+            vz[t] = 0.0;
+            ve[t] = 0xffffffffu;
+            vid[t] = 0xffffffffu;
+            if (e >= 0) {
+                vx[t] = cx[e];
+                vy[t] = cy[e];
+                vz[t] = cz[e];
+                ve[t] = (unsigned)e;
+                vid[t] = cid[e];
+            }
+        }
         bool bpass = false;
         if (gl < kBatch && i0 + gl < count)
             bpass = union_pass(leaves[list[i0 + gl] & kBucketMask], grp, s_qx, s_qy, s_qz, s_qk);
         const unsigned bm = (unsigned)((__ballot(bpass) >> (grp * kG)) & kGMask);
 #pragma unroll
         for (int t = 0; t < kBatch; t++) {
-            const int i = i0 + t;
-            long long e = -1;
-            if (i < count && ((bm >> t) & 1u)) {
-                e = (long long)(list[i] & kBucketMask) * kB + gl;
-                if (e >= nc) e = -1;
-            }
             const int slot = t * kB + gl;
-            if (e >= 0)
-                stage_put(sg, grp, slot, cx[e], cy[e], cz[e], (unsigned)e, cid[e]);
+            if ((bm >> t) & 1u)
+                stage_put(sg, grp, slot, vx[t], vy[t], vz[t], ve[t], vid[t]);
             else
-                stage_put(sg, grp, slot, __longlong_as_double(0x7ff8000000000000ll), 0.0, 0.0,
-                          0xffffffffu, 0xffffffffu);
+                stage_put(sg, grp, slot, nan, 0.0, 0.0, 0xffffffffu, 0xffffffffu);
         }
         if (dbg && lane == 0) atomicAdd(&dbg[0], 1ull);  // scan batches (wave)
         __syncthreads();
