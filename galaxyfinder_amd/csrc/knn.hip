@@ -63,11 +63,17 @@ constexpr int kGroups = kWave / kG;
 constexpr unsigned long long kGMask = (1ull << kG) - 1ull;
 constexpr int kFan = 8;      // box-tree fan-out
 constexpr int kPar = kG / kFan;  // parents expanded per walk round
+#ifndef PGALF_KNN_SEEDSPAN
+#define PGALF_KNN_SEEDSPAN 12
+#endif
+#ifndef PGALF_KNN_FIRSTSCAN
+#define PGALF_KNN_FIRSTSCAN 16
+#endif
 constexpr int kSeedFill = kTop / kB;  // buckets of the direct seed fill
-constexpr int kSeedSpan = 2 * kSeedFill;  // seed buckets in all
+constexpr int kSeedSpan = PGALF_KNN_SEEDSPAN;  // seed buckets in all (a window around the fill)
 constexpr int kFCap = PGALF_KNN_LISTCAP;  // bucket-list capacity per group
 constexpr int kStackCap = 96;  // depth-first walk stack per group
-constexpr int kFirstScan = 16;  // the first list is scanned early to tighten the bounds
+constexpr int kFirstScan = PGALF_KNN_FIRSTSCAN;  // the first list is scanned early to tighten the bounds
 constexpr int kLevelShift = 5;  // stack entry = node << 5 | level
 constexpr int kBatch = PGALF_KNN_BATCH;  // buckets staged in LDS per scan batch
 constexpr int kStage = kBatch * kB;  // staged candidates per group
@@ -82,6 +88,7 @@ static_assert(kPar == 1, "the depth-first walk expands one parent per round");
 static_assert(kBufCap % 8 == 0 && kBufCap >= 24, "buffer holds whole merge chunks");
 static_assert(kBatch >= 1 && kBatch <= kG, "one group lane tests each bucket of a batch");
 static_assert(kFCap >= 32 + 8 && kFCap % 8 == 0, "list must take the first scan and a round");
+static_assert(kSeedSpan >= kSeedFill && kFirstScan + kG <= kFCap, "seed window and first scan");
 // The walk keeps at most kFan entries of the deepest level and kFan-1 of every
 // level above it on the stack.
 constexpr int stack_need(int nlev) { return kFan + (kFan - 1) * (nlev > 3 ? nlev - 3 : 0); }
@@ -587,7 +594,7 @@ __device__ inline Key kth_of(const double (&td)[kTop], const unsigned (&ti)[kTop
 
 // Merge a lane's buffer of cnt sorted-candidate indices into its top-32:
 // keys are recomputed from the candidate arrays (same arithmetic as the
-// scan, so the same FP64 values), then merged 16 at a time.  Wave uniform.
+// scan, so the same FP64 values), then merged kBuf at a time.  Wave uniform.
 __device__ inline void flush_buffer(double (&td)[kTop], unsigned (&ti)[kTop], int cnt,
                                     const unsigned (*s_bc)[kWave], int lane, double px,
                                     double py, double pz, const double *cx, const double *cy,
@@ -910,7 +917,8 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
     }
     bitonic_sort<kTop, 2>(st.td, st.ti);
     st.kth = kth_of(st.td, st.ti, k);
-    const long long slo = f0 - kSeedFill / 2 < 0 ? 0 : f0 - kSeedFill / 2;
+    constexpr int kSeedBefore = (kSeedSpan - kSeedFill) / 2;
+    const long long slo = f0 - kSeedBefore < 0 ? 0 : f0 - kSeedBefore;
     const long long shi = slo + kSeedSpan - 1 > nb - 1 ? nb - 1 : slo + kSeedSpan - 1;
     // remaining seed buckets through the batched scanner (list in the stack)
     static_assert(kSeedSpan <= kStackCap, "seed list lives in the walk stack");
