@@ -625,7 +625,8 @@ extern "C" int pgalf_dev_density_stage_backward(
     }
     if (e == hipSuccess && g_knn) {
         timing_mark("bwd_knn", s, false);
-        e = knn_density_backward<float>(x, y, z, m, np, nbr, k, rho_mode == 2 ? 1 : 0, g_knn, acc, s);
+        e = knn_density_backward<float>(x, y, z, m, np, nbr, k, rho_mode == 2 ? 1 : 0, g_knn, acc,
+                                        ws, s);
         timing_mark("bwd_knn", s, true);
     }
     if (e == hipSuccess) e = grad_finalize(acc, np, gx, gy, gz, gm, s);

@@ -1168,53 +1168,91 @@ __device__ inline void m4_f(double q, double &f, double &fp) {
 // One thread per particle i over its forward list (FP64):
 //   rho_i = sum_t m_t W(r_it, h_i), h_i = r_i,(k-1) / 2,
 //   dW/dr = f'(q) / (pi h^4), dW/dh = -(3 f + q f') / (pi h^4);
-// i's own terms and its neighbours' (opposite sign) go to acc by FP64
-// atomic adds (acc = [gx | gy | gz | gm]).
+// i's own terms and its neighbours' (opposite sign) are accumulated in FP64.
+// A workgroup takes kBwdBlock consecutive particles of the Hilbert order
+// (order[]; rank[] is its inverse), so most neighbours lie within
+// kBwdHalo sorted positions of the block: their terms go to an LDS window
+// (LDS atomics), flushed to acc once per block; the rest go to acc by
+// global FP64 atomics (acc = [gx | gy | gz | gm]).
+constexpr int kBwdBlock = 256;
+constexpr int kBwdHalo = 512;
+constexpr int kBwdWin = kBwdBlock + 2 * kBwdHalo;
+
 template <class P>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kBwdBlock)
 knn_backward_k(const P *x, const P *y, const P *z, const P *m, long long np,
-               const long long *nbr, int k, int weighted, const float *gin, double *acc) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
-    const double g = (double)gin[i];
-    if (g == 0.0) return;
-    const double xi = (double)x[i], yi = (double)y[i], zi = (double)z[i];
-    const long long *li = nbr + i * k;
-    const long long K = li[k - 1];
-    const double kx = xi - (double)x[K], ky = yi - (double)y[K], kz = zi - (double)z[K];
-    const double rK = sqrt(kx * kx + ky * ky + kz * kz), hh = 0.5 * rK;
-    if (!(hh > 0.0)) return;  // k-th neighbour at zero distance: no finite density
-    const double h3 = M_PI * hh * hh * hh, h4 = h3 * hh;
-    double ox = 0.0, oy = 0.0, oz = 0.0, sh = 0.0;
-    for (int t = 0; t < k; t++) {
-        const long long j = li[t];
-        const double dx = xi - (double)x[j], dy = yi - (double)y[j], dz = zi - (double)z[j];
-        const double r = sqrt(dx * dx + dy * dy + dz * dz), q = r / hh;
-        double f, fp;
-        m4_f(q, f, fp);
-        const double mt = weighted ? (double)m[j] : 1.0;
-        sh += mt * (-(3.0 * f + q * fp) / h4);
-        if (weighted && f != 0.0) unsafeAtomicAdd(&acc[3 * np + j], g * f / h3);
-        if (r > 0.0 && fp != 0.0) {
-            const double c = g * mt * (fp / h4) / r;
-            ox += c * dx;
-            oy += c * dy;
-            oz += c * dz;
-            unsafeAtomicAdd(&acc[j], -c * dx);
-            unsafeAtomicAdd(&acc[np + j], -c * dy);
-            unsafeAtomicAdd(&acc[2 * np + j], -c * dz);
+               const long long *nbr, int k, int weighted, const float *gin, double *acc,
+               const unsigned *order, const unsigned *rank) {
+    __shared__ double win[4][kBwdWin];
+    for (int c = threadIdx.x; c < 4 * kBwdWin; c += kBwdBlock) (&win[0][0])[c] = 0.0;
+    __syncthreads();
+    const long long s0 = (long long)blockIdx.x * kBwdBlock;
+    const long long wb = s0 - kBwdHalo;  // sorted position of window slot 0
+    const long long s = s0 + threadIdx.x;
+    // adds v to component c of particle j (sorted position rank[j])
+    auto add = [&](long long j, int c, double v) {
+        const long long w = (long long)rank[j] - wb;
+        if (w >= 0 && w < kBwdWin) unsafeAtomicAdd(&win[c][w], v);
+        else unsafeAtomicAdd(&acc[c * np + j], v);
+    };
+    if (s < np) {
+        const long long i = (long long)order[s];
+        const double g = (double)gin[i];
+        const long long *li = nbr + i * k;
+        const long long K = li[k - 1];
+        const double xi = (double)x[i], yi = (double)y[i], zi = (double)z[i];
+        const double kx = xi - (double)x[K], ky = yi - (double)y[K], kz = zi - (double)z[K];
+        const double rK = sqrt(kx * kx + ky * ky + kz * kz), hh = 0.5 * rK;
+        // g == 0: no terms; k-th neighbour at zero distance: no finite density
+        if (g != 0.0 && hh > 0.0) {
+            const double h3 = M_PI * hh * hh * hh, h4 = h3 * hh;
+            double ox = 0.0, oy = 0.0, oz = 0.0, sh = 0.0;
+            for (int t = 0; t < k; t++) {
+                const long long j = li[t];
+                const double dx = xi - (double)x[j], dy = yi - (double)y[j],
+                             dz = zi - (double)z[j];
+                const double r = sqrt(dx * dx + dy * dy + dz * dz), q = r / hh;
+                double f, fp;
+                m4_f(q, f, fp);
+                const double mt = weighted ? (double)m[j] : 1.0;
+                sh += mt * (-(3.0 * f + q * fp) / h4);
+                if (weighted && f != 0.0) add(j, 3, g * f / h3);
+                if (r > 0.0 && fp != 0.0) {
+                    const double c = g * mt * (fp / h4) / r;
+                    ox += c * dx;
+                    oy += c * dy;
+                    oz += c * dz;
+                    add(j, 0, -c * dx);
+                    add(j, 1, -c * dy);
+                    add(j, 2, -c * dz);
+                }
+            }
+            const double c = g * sh * 0.5 / rK;
+            ox += c * kx;
+            oy += c * ky;
+            oz += c * kz;
+            add(K, 0, -c * kx);
+            add(K, 1, -c * ky);
+            add(K, 2, -c * kz);
+            add(i, 0, ox);
+            add(i, 1, oy);
+            add(i, 2, oz);
         }
     }
-    const double c = g * sh * 0.5 / rK;
-    ox += c * kx;
-    oy += c * ky;
-    oz += c * kz;
-    unsafeAtomicAdd(&acc[K], -c * kx);
-    unsafeAtomicAdd(&acc[np + K], -c * ky);
-    unsafeAtomicAdd(&acc[2 * np + K], -c * kz);
-    unsafeAtomicAdd(&acc[i], ox);
-    unsafeAtomicAdd(&acc[np + i], oy);
-    unsafeAtomicAdd(&acc[2 * np + i], oz);
+    __syncthreads();
+    for (int w = threadIdx.x; w < kBwdWin; w += kBwdBlock) {
+        const long long sp = wb + w;
+        if (sp < 0 || sp >= np) continue;
+        const long long j = (long long)order[sp];
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            if (win[c][w] != 0.0) unsafeAtomicAdd(&acc[c * np + j], win[c][w]);
+    }
+}
+
+__global__ void inverse_perm_k(const unsigned *order, long long n, unsigned *rank) {
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) rank[order[s]] = (unsigned)s;
 }
 
 __global__ void grad_finalize_k(const double *acc, long long np, float *gx, float *gy,
@@ -1232,10 +1270,22 @@ __global__ void grad_finalize_k(const double *acc, long long np, float *gx, floa
 template <class P>
 hipError_t knn_density_backward(const P *x, const P *y, const P *z, const P *m, long long np,
                                 const long long *nbr, int k, int weighted, const float *gin,
-                                double *acc, hipStream_t s) {
+                                double *acc, Workspace &ws, hipStream_t s) {
     if (np <= 0) return hipSuccess;
     if (k < 1 || k > kTop) return hipErrorInvalidValue;
-    knn_backward_k<P><<<nblk(np, 256), 256, 0, s>>>(x, y, z, m, np, nbr, k, weighted, gin, acc);
+    // Hilbert order of the particles (as the forward's preparation) and its inverse
+    unsigned long long *mm = ws.alloc_n<unsigned long long>(6);
+    unsigned *hk = ws.alloc_n<unsigned>(np), *hv = ws.alloc_n<unsigned>(np);
+    unsigned *hk2 = ws.alloc_n<unsigned>(np), *hv2 = ws.alloc_n<unsigned>(np);
+    unsigned *rank = ws.alloc_n<unsigned>(np);
+    if (!mm || !hk || !hv || !hk2 || !hv2 || !rank) return hipErrorOutOfMemory;
+    bbox_init_k<<<1, 64, 0, s>>>(mm, 1);
+    bbox_k<P><<<256, 256, 0, s>>>(x, y, z, np, mm);
+    hilbert_k<P><<<nblk(np, 256), 256, 0, s>>>(x, y, z, nullptr, np, mm, hk, hv);
+    PGALF_HIP_TRY(radix_sort_pairs(&hk, &hv, hk2, hv2, np, 30, ws, s));
+    inverse_perm_k<<<nblk(np, 256), 256, 0, s>>>(hv, np, rank);
+    knn_backward_k<P><<<nblk(np, kBwdBlock), kBwdBlock, 0, s>>>(x, y, z, m, np, nbr, k, weighted,
+                                                                gin, acc, hv, rank);
     return hipGetLastError();
 }
 
@@ -1248,11 +1298,12 @@ hipError_t grad_finalize(const double *acc, long long np, float *gx, float *gy, 
 
 template hipError_t knn_density_backward<float>(const float *, const float *, const float *,
                                                 const float *, long long, const long long *,
-                                                int, int, const float *, double *, hipStream_t);
+                                                int, int, const float *, double *, Workspace &,
+                                                hipStream_t);
 template hipError_t knn_density_backward<double>(const double *, const double *,
                                                  const double *, const double *, long long,
                                                  const long long *, int, int, const float *,
-                                                 double *, hipStream_t);
+                                                 double *, Workspace &, hipStream_t);
 
 hipError_t knn_batch_plan(const long long *off, int nhalo, int k, KnnBatchPlan &plan) {
     if (nhalo < 1 || k < 1 || k > kTop || off[0] != 0) return hipErrorInvalidValue;

@@ -186,7 +186,7 @@ hipError_t knn_batch(const P *x, const P *y, const P *z, const P *m, const Batch
 template <class P>
 hipError_t knn_density_backward(const P *x, const P *y, const P *z, const P *m, long long np,
                                 const long long *nbr, int k, int weighted, const float *gin,
-                                double *acc, hipStream_t s);
+                                double *acc, Workspace &ws, hipStream_t s);
 // out[i] = (float)acc[i] for the four gradient arrays (any may be NULL).
 hipError_t grad_finalize(const double *acc, long long np, float *gx, float *gy, float *gz,
                          float *gm, hipStream_t s);
