@@ -59,6 +59,11 @@ SIGNATURES = {
     "pgalf_batch_run": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i,
                                _c_p]),
     "pgalf_batch_status": (_c_i, [_c_p, _c_p]),
+    "pgalf_batch_run_grad": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i,
+                                    _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p,
+                                    _c_i, _c_f, _c_p, _c_p, _c_p, _c_p, _c_i, _c_p]),
+    "pgalf_dev_peak_climb": (_c_i, [_c_p, _c_p, _c_p, _c_ll, ctypes.POINTER(pgalf_grid_t),
+                                    _c_p, _c_i, _c_f, _c_p, _c_p, _c_p, _c_p, _c_p]),
     "pgalf_batch_destroy": (None, [_c_p]),
     "pgalf_timing_enable": (None, [_c_i]),
     "pgalf_timing_reset": (None, []),

@@ -72,10 +72,16 @@ hipError_t upload(T **dst, const std::vector<T> &src) {
     return hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice);
 }
 
+// g[i] = w[i] / rho[i]: the cotangent of sum w log rho (0 where rho is 0).
+__global__ void log_cotangent_k(const float *w, const float *rho, long long n, float *g) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) g[i] = rho[i] != 0.0f ? w[i] / rho[i] : 0.0f;
+}
+
 // The launch sequence of one stage pass (stream-ordered, no synchronisation).
 hipError_t run_sequence(pgalf_batch *b, const float *x, const float *y, const float *z,
                         const float *m, float *den, float *rho_tsc, long long *nbr,
-                        float *rho_knn, hipStream_t s) {
+                        float *rho_knn, hipStream_t s, int rho_mode = 1) {
     Workspace &ws = *b->ws;
     timing_mark("batch_stage", s, false);
     timing_mark("batch_tsc", s, false);
@@ -90,10 +96,117 @@ hipError_t run_sequence(pgalf_batch *b, const float *x, const float *y, const fl
     timing_mark("batch_knn", s, false);
     if (nbr || rho_knn)
         PGALF_HIP_TRY(knn_batch<float>(x, y, z, m, b->bd, b->kplan, nbr, rho_knn,
-                                       rho_knn ? 1 : 0, ws, s));
+                                       rho_knn ? rho_mode : 0, ws, s));
     timing_mark("batch_knn", s, true);
     timing_mark("batch_stage", s, true);
     return hipGetLastError();
+}
+
+struct GradArgs {
+    const float *x, *y, *z, *m, *w_tsc, *w_knn;
+    int rho_mode;
+    float *den, *rho_tsc, *rho_knn;
+    long long *nbr;
+    float *gx, *gy, *gz, *gm;
+    int climb_iters;
+    float climb_step;
+    float *cx, *cy, *cz;
+    int *peak;
+};
+
+// Forward, cotangents of L = sum w_tsc log rho_tsc + sum w_knn log rho_knn,
+// adjoint and peak climbing: one stream-ordered sequence.
+hipError_t run_grad_sequence(pgalf_batch *b, const GradArgs &a, hipStream_t s) {
+    Workspace &ws = *b->ws;
+    const long long np = b->bd.np;
+    PGALF_HIP_TRY(run_sequence(b, a.x, a.y, a.z, a.m, a.den, a.rho_tsc, a.nbr, a.rho_knn, s,
+                               a.rho_mode));
+    timing_mark("batch_bwd", s, false);
+    double *acc = ws.alloc_n<double>(4 * (size_t)np);
+    float *g_t = a.w_tsc ? ws.alloc_n<float>(np) : nullptr;
+    float *g_k = a.w_knn ? ws.alloc_n<float>(np) : nullptr;
+    if (!acc || (a.w_tsc && !g_t) || (a.w_knn && !g_k)) return hipErrorOutOfMemory;
+    const unsigned nb = (unsigned)((np + 255) / 256);
+    PGALF_HIP_TRY(fill_u64(reinterpret_cast<unsigned long long *>(acc), 4 * np, 0ull, s));
+    if (g_t) {
+        log_cotangent_k<<<nb, 256, 0, s>>>(a.w_tsc, a.rho_tsc, np, g_t);
+        float *A = ws.alloc_n<float>(b->goff[b->nhalo]);
+        if (!A) return hipErrorOutOfMemory;
+        PGALF_HIP_TRY(tsc_deposit_batch<float>(a.x, a.y, a.z, g_t, b->bd, A, ws, s));
+        PGALF_HIP_TRY(gaussian_smooth_batch(A, b->groups.data(), (int)b->groups.size(), s));
+        PGALF_HIP_TRY(tsc_grad_readback_batch<float>(a.x, a.y, a.z, a.m, g_t, b->bd, a.den, A,
+                                                     acc, ws, s));
+    }
+    if (g_k) {
+        log_cotangent_k<<<nb, 256, 0, s>>>(a.w_knn, a.rho_knn, np, g_k);
+        PGALF_HIP_TRY(knn_batch_backward<float>(a.x, a.y, a.z, a.m, b->bd, b->kplan, a.nbr,
+                                                a.rho_mode == 2 ? 1 : 0, g_k, acc, ws, s));
+    }
+    PGALF_HIP_TRY(grad_finalize(acc, np, a.gx, a.gy, a.gz, a.gm, s));
+    timing_mark("batch_bwd", s, true);
+    timing_mark("batch_climb", s, false);
+    PGALF_HIP_TRY(peak_climb_batch<float>(a.x, a.y, a.z, b->bd, a.den, a.climb_iters,
+                                          (double)a.climb_step, a.cx, a.cy, a.cz, a.peak, s));
+    timing_mark("batch_climb", s, true);
+    return hipGetLastError();
+}
+
+// Runs `seq` eagerly on the caller's stream, or (use_graph) replays the HIP
+// graph captured for the pointer set `key` on the batch's own stream,
+// capturing it first: an eager pass sizes the arena, the arena settles into
+// one block, then the sequence is captured.  The batch stream is ordered
+// after the caller's stream and back by events.
+template <class Seq>
+int run_maybe_graph(pgalf_batch *b, const std::vector<const void *> &key, int use_graph,
+                    hipStream_t s, const char *where, Seq seq) {
+    hipError_t e = hipSuccess;
+    if (!use_graph) {
+        if (b->gexec) {  // an eager run may resize the arena under the graph
+            (void)hipGraphExecDestroy(b->gexec);
+            b->gexec = nullptr;
+        }
+        e = b->ws->begin(s);
+        if (e == hipSuccess) e = seq(s);
+        return e == hipSuccess ? ok() : hip_err(e, where);
+    }
+    if (!b->gstream) {
+        e = hipStreamCreateWithFlags(&b->gstream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming);
+        if (e != hipSuccess) return hip_err(e, where);
+    }
+    e = hipEventRecord(b->ev_in, s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(b->gstream, b->ev_in, 0);
+    if (e == hipSuccess && (!b->gexec || key != b->gkey)) {
+        if (b->gexec) {
+            (void)hipGraphExecDestroy(b->gexec);
+            b->gexec = nullptr;
+        }
+        e = b->ws->begin(b->gstream);
+        if (e == hipSuccess) e = seq(b->gstream);
+        if (e == hipSuccess) e = b->ws->begin(b->gstream);
+        if (e == hipSuccess) e = hipStreamSynchronize(b->gstream);
+        hipGraph_t graph = nullptr;
+        if (e == hipSuccess) e = hipStreamBeginCapture(b->gstream, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            timing_suspend(true);
+            hipError_t er = seq(b->gstream);
+            timing_suspend(false);
+            e = hipStreamEndCapture(b->gstream, &graph);
+            if (er != hipSuccess) e = er;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            b->gexec = nullptr;
+            return hip_err(e, where);
+        }
+        b->gkey = key;
+    }
+    if (e == hipSuccess) e = hipGraphLaunch(b->gexec, b->gstream);
+    if (e == hipSuccess) e = hipEventRecord(b->ev_out, b->gstream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, b->ev_out, 0);
+    return e == hipSuccess ? ok() : hip_err(e, where);
 }
 
 }  // namespace
@@ -219,60 +332,35 @@ extern "C" int pgalf_batch_run(pgalf_batch_t *b, const float *x, const float *y,
     const char *where = "pgalf_batch_run";
     if (!b || !x || !y || !z || !m || !den) return set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
     std::lock_guard<std::mutex> lk(g_api_mu);
-    hipStream_t s = (hipStream_t)stream;
-    hipError_t e = hipSuccess;
-    if (!use_graph) {
-        if (b->gexec) {  // an eager run may resize the arena under the graph
-            (void)hipGraphExecDestroy(b->gexec);
-            b->gexec = nullptr;
-        }
-        e = b->ws->begin(s);
-        if (e == hipSuccess) e = run_sequence(b, x, y, z, m, den, rho_tsc, nbr, rho_knn, s);
-        return e == hipSuccess ? ok() : hip_err(e, where);
-    }
-    // graph mode: the sequence runs on the batch's own stream, ordered after
-    // the caller's stream by events (capture is not possible on the null
-    // stream, which callers commonly use)
     const std::vector<const void *> key = {x, y, z, m, den, rho_tsc, nbr, rho_knn};
-    if (!b->gstream) {
-        e = hipStreamCreateWithFlags(&b->gstream, hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming);
-        if (e != hipSuccess) return hip_err(e, where);
-    }
-    e = hipEventRecord(b->ev_in, s);
-    if (e == hipSuccess) e = hipStreamWaitEvent(b->gstream, b->ev_in, 0);
-    if (e == hipSuccess && (!b->gexec || key != b->gkey)) {
-        if (b->gexec) {
-            (void)hipGraphExecDestroy(b->gexec);
-            b->gexec = nullptr;
-        }
-        // size the arena with an eager pass, settle it into one block, then capture
-        e = b->ws->begin(b->gstream);
-        if (e == hipSuccess) e = run_sequence(b, x, y, z, m, den, rho_tsc, nbr, rho_knn, b->gstream);
-        if (e == hipSuccess) e = b->ws->begin(b->gstream);
-        if (e == hipSuccess) e = hipStreamSynchronize(b->gstream);
-        hipGraph_t graph = nullptr;
-        if (e == hipSuccess) e = hipStreamBeginCapture(b->gstream, hipStreamCaptureModeThreadLocal);
-        if (e == hipSuccess) {
-            timing_suspend(true);
-            hipError_t er = run_sequence(b, x, y, z, m, den, rho_tsc, nbr, rho_knn, b->gstream);
-            timing_suspend(false);
-            e = hipStreamEndCapture(b->gstream, &graph);
-            if (er != hipSuccess) e = er;
-        }
-        if (e == hipSuccess) e = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
-        if (graph) (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) {
-            b->gexec = nullptr;
-            return hip_err(e, where);
-        }
-        b->gkey = key;
-    }
-    if (e == hipSuccess) e = hipGraphLaunch(b->gexec, b->gstream);
-    if (e == hipSuccess) e = hipEventRecord(b->ev_out, b->gstream);
-    if (e == hipSuccess) e = hipStreamWaitEvent(s, b->ev_out, 0);
-    return e == hipSuccess ? ok() : hip_err(e, where);
+    return run_maybe_graph(b, key, use_graph, (hipStream_t)stream, where, [&](hipStream_t s) {
+        return run_sequence(b, x, y, z, m, den, rho_tsc, nbr, rho_knn, s);
+    });
+}
+
+extern "C" int pgalf_batch_run_grad(pgalf_batch_t *b, const float *x, const float *y,
+                                    const float *z, const float *m, const float *w_tsc,
+                                    const float *w_knn, int rho_mode, float *den,
+                                    float *rho_tsc, long long *nbr, float *rho_knn, float *gx,
+                                    float *gy, float *gz, float *gm, int climb_iters,
+                                    float climb_step, float *cx, float *cy, float *cz,
+                                    int *peak, int use_graph, void *stream) {
+    const char *where = "pgalf_batch_run_grad";
+    if (!b || !x || !y || !z || !m || !den || !rho_tsc || !nbr || !rho_knn || !gx || !gy ||
+        !gz || !gm || !cx || !cy || !cz || !peak || (rho_mode != 1 && rho_mode != 2) ||
+        climb_iters < 0 || !(climb_step >= 0.0f))
+        return set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    GradArgs a{x, y, z, m, w_tsc, w_knn, rho_mode, den, rho_tsc, rho_knn, nbr,
+               gx, gy, gz, gm, climb_iters, climb_step, cx, cy, cz, peak};
+    // the iteration count and step are baked into a captured graph: part of the key
+    const std::vector<const void *> key = {
+        x, y, z, m, w_tsc, w_knn, den, rho_tsc, nbr, rho_knn, gx, gy, gz, gm, cx, cy, cz, peak,
+        reinterpret_cast<const void *>((long long)rho_mode),
+        reinterpret_cast<const void *>((long long)climb_iters),
+        reinterpret_cast<const void *>((long long)((double)climb_step * 1e6))};
+    return run_maybe_graph(b, key, use_graph, (hipStream_t)stream, where,
+                           [&](hipStream_t s) { return run_grad_sequence(b, a, s); });
 }
 
 extern "C" int pgalf_batch_status(pgalf_batch_t *b, void *stream) {

@@ -634,6 +634,24 @@ extern "C" int pgalf_dev_density_stage_backward(
     return e == hipSuccess ? ok() : hip_err(e, where);
 }
 
+extern "C" int pgalf_dev_peak_climb(const float *x, const float *y, const float *z,
+                                    long long np, const pgalf_grid_t *g, const float *den,
+                                    int niter, float step, float *ox, float *oy, float *oz,
+                                    int *peak, void *stream) {
+    const char *where = "pgalf_dev_peak_climb";
+    if (!g || np < 0 || !den || !ox || !oy || !oz || !peak || niter < 0 || !(step >= 0.0f) ||
+        (np > 0 && (!x || !y || !z)) || !check_grid(g->nx, g->ny, g->nz))
+        return set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    hipStream_t s = (hipStream_t)stream;
+    Workspace &ws = workspace_for_current_device();
+    hipError_t e = ws.begin(s);
+    if (e == hipSuccess)
+        e = peak_climb<float>(x, y, z, np, grid_from(g), den, niter, (double)step, ox, oy, oz,
+                              peak, s);
+    return e == hipSuccess ? ok() : hip_err(e, where);
+}
+
 extern "C" int pgalf_dev_status(void *stream) {
     std::lock_guard<std::mutex> lk(g_api_mu);
     hipStream_t s = (hipStream_t)stream;

@@ -1182,7 +1182,8 @@ template <class P>
 __global__ void __launch_bounds__(kBwdBlock)
 knn_backward_k(const P *x, const P *y, const P *z, const P *m, long long np,
                const long long *nbr, int k, int weighted, const float *gin, double *acc,
-               const unsigned *order, const unsigned *rank) {
+               const unsigned *order, const unsigned *rank, const unsigned *halo_of,
+               const HaloSeg *segs) {
     __shared__ double win[4][kBwdWin];
     for (int c = threadIdx.x; c < 4 * kBwdWin; c += kBwdBlock) (&win[0][0])[c] = 0.0;
     __syncthreads();
@@ -1199,7 +1200,9 @@ knn_backward_k(const P *x, const P *y, const P *z, const P *m, long long np,
         const long long i = (long long)order[s];
         const double g = (double)gin[i];
         const long long *li = nbr + i * k;
-        const long long K = li[k - 1];
+        // batch lists hold indices local to the particle's halo
+        const long long base = halo_of ? segs[halo_of[i]].p0 : 0;
+        const long long K = li[k - 1] + base;
         const double xi = (double)x[i], yi = (double)y[i], zi = (double)z[i];
         const double kx = xi - (double)x[K], ky = yi - (double)y[K], kz = zi - (double)z[K];
         const double rK = sqrt(kx * kx + ky * ky + kz * kz), hh = 0.5 * rK;
@@ -1208,7 +1211,7 @@ knn_backward_k(const P *x, const P *y, const P *z, const P *m, long long np,
             const double h3 = M_PI * hh * hh * hh, h4 = h3 * hh;
             double ox = 0.0, oy = 0.0, oz = 0.0, sh = 0.0;
             for (int t = 0; t < k; t++) {
-                const long long j = li[t];
+                const long long j = li[t] + base;
                 const double dx = xi - (double)x[j], dy = yi - (double)y[j],
                              dz = zi - (double)z[j];
                 const double r = sqrt(dx * dx + dy * dy + dz * dz), q = r / hh;
@@ -1285,7 +1288,8 @@ hipError_t knn_density_backward(const P *x, const P *y, const P *z, const P *m, 
     PGALF_HIP_TRY(radix_sort_pairs(&hk, &hv, hk2, hv2, np, 30, ws, s));
     inverse_perm_k<<<nblk(np, 256), 256, 0, s>>>(hv, np, rank);
     knn_backward_k<P><<<nblk(np, kBwdBlock), kBwdBlock, 0, s>>>(x, y, z, m, np, nbr, k, weighted,
-                                                                gin, acc, hv, rank);
+                                                                gin, acc, hv, rank, nullptr,
+                                                                nullptr);
     return hipGetLastError();
 }
 
@@ -1402,6 +1406,35 @@ void knn_batch_plan_free(KnnBatchPlan &plan) {
     plan = KnnBatchPlan();
 }
 
+// The batch's particles in (halo, Hilbert within the halo's own box, index)
+// order: *order the sorted particle ids, *hsorted their halos, *halo_of the
+// halo of every particle.  Two stable sorts: Hilbert key, then halo.
+template <class P>
+hipError_t batch_order(const P *x, const P *y, const P *z, const BatchDev &b,
+                       const KnnBatchPlan &plan, Workspace &ws, hipStream_t s, unsigned **order,
+                       unsigned **hsorted, unsigned **halo_of_out) {
+    const long long np = plan.np;
+    unsigned long long *mm = ws.alloc_n<unsigned long long>(6 * (size_t)plan.nhalo);
+    unsigned *hk = ws.alloc_n<unsigned>(np), *hv = ws.alloc_n<unsigned>(np);
+    unsigned *hk2 = ws.alloc_n<unsigned>(np), *hv2 = ws.alloc_n<unsigned>(np);
+    unsigned *halo_of = ws.alloc_n<unsigned>(np);
+    if (!mm || !hk || !hv || !hk2 || !hv2 || !halo_of) return hipErrorOutOfMemory;
+    bbox_init_k<<<nblk(6LL * plan.nhalo, 256), 256, 0, s>>>(mm, plan.nhalo);
+    bbox_batch_k<P><<<(unsigned)b.nchunks, 256, 0, s>>>(x, y, z, b.chunks, mm);
+    hilbert_batch_k<P><<<(unsigned)b.nchunks, 256, 0, s>>>(x, y, z, b.chunks, mm, hk, hv,
+                                                          halo_of);
+    PGALF_HIP_TRY(radix_sort_pairs(&hk, &hv, hk2, hv2, np, 30, ws, s));
+    unsigned *k2 = hk == hk2 ? ws.alloc_n<unsigned>(np) : hk2;
+    unsigned *k2t = ws.alloc_n<unsigned>(np), *v2t = hv == hv2 ? ws.alloc_n<unsigned>(np) : hv2;
+    if (!k2 || !k2t || !v2t) return hipErrorOutOfMemory;
+    halo_keys_k<<<nblk(np, 256), 256, 0, s>>>(hv, halo_of, np, k2);
+    if (plan.hbits > 0) PGALF_HIP_TRY(radix_sort_pairs(&k2, &hv, k2t, v2t, np, plan.hbits, ws, s));
+    *order = hv;
+    *hsorted = k2;
+    *halo_of_out = halo_of;
+    return hipGetLastError();
+}
+
 template <class P>
 hipError_t knn_batch(const P *x, const P *y, const P *z, const P *m, const BatchDev &b,
                      const KnnBatchPlan &plan, long long *nbr, float *rho, int rho_mode,
@@ -1410,27 +1443,13 @@ hipError_t knn_batch(const P *x, const P *y, const P *z, const P *m, const Batch
     if (np != b.np || plan.nhalo != b.nhalo) return hipErrorInvalidValue;
     timing_mark("knn_prep", s, false);
     const KHalo *halos = static_cast<const KHalo *>(plan.halos);
-    unsigned long long *mm = ws.alloc_n<unsigned long long>(6 * (size_t)plan.nhalo);
-    unsigned *hk = ws.alloc_n<unsigned>(np), *hv = ws.alloc_n<unsigned>(np);
-    unsigned *hk2 = ws.alloc_n<unsigned>(np), *hv2 = ws.alloc_n<unsigned>(np);
-    unsigned *halo_of = ws.alloc_n<unsigned>(np);
+    unsigned *hv = nullptr, *k2 = nullptr, *halo_of = nullptr;
+    PGALF_HIP_TRY(batch_order<P>(x, y, z, b, plan, ws, s, &hv, &k2, &halo_of));
     double *sx = ws.alloc_n<double>(npad), *sy = ws.alloc_n<double>(npad),
            *sz = ws.alloc_n<double>(npad);
     unsigned *sid = ws.alloc_n<unsigned>(npad);
     Box *boxes = ws.alloc_n<Box>((size_t)plan.nboxes);
-    if (!mm || !hk || !hv || !hk2 || !hv2 || !halo_of || !sx || !sy || !sz || !sid || !boxes)
-        return hipErrorOutOfMemory;
-    bbox_init_k<<<nblk(6LL * plan.nhalo, 256), 256, 0, s>>>(mm, plan.nhalo);
-    bbox_batch_k<P><<<(unsigned)b.nchunks, 256, 0, s>>>(x, y, z, b.chunks, mm);
-    hilbert_batch_k<P><<<(unsigned)b.nchunks, 256, 0, s>>>(x, y, z, b.chunks, mm, hk, hv,
-                                                          halo_of);
-    // stable sorts: by Hilbert key, then by halo -> (halo, Hilbert, index) order
-    PGALF_HIP_TRY(radix_sort_pairs(&hk, &hv, hk2, hv2, np, 30, ws, s));
-    unsigned *k2 = hk == hk2 ? ws.alloc_n<unsigned>(np) : hk2;
-    unsigned *k2t = ws.alloc_n<unsigned>(np), *v2t = hv == hv2 ? ws.alloc_n<unsigned>(np) : hv2;
-    if (!k2 || !k2t || !v2t) return hipErrorOutOfMemory;
-    halo_keys_k<<<nblk(np, 256), 256, 0, s>>>(hv, halo_of, np, k2);
-    if (plan.hbits > 0) PGALF_HIP_TRY(radix_sort_pairs(&k2, &hv, k2t, v2t, np, plan.hbits, ws, s));
+    if (!sx || !sy || !sz || !sid || !boxes) return hipErrorOutOfMemory;
     gather_padded_k<P><<<nblk(np, 256), 256, 0, s>>>(x, y, z, hv, k2, halos, np, sx, sy, sz, sid);
     pad_slots_k<<<nblk((long long)plan.nhalo * kB, 256), 256, 0, s>>>(halos, plan.nhalo, sx, sy,
                                                                       sz, sid);
@@ -1451,6 +1470,27 @@ hipError_t knn_batch(const P *x, const P *y, const P *z, const P *m, const Batch
     timing_mark("knn_query", s, true);
     return hipGetLastError();
 }
+
+template <class P>
+hipError_t knn_batch_backward(const P *x, const P *y, const P *z, const P *m, const BatchDev &b,
+                              const KnnBatchPlan &plan, const long long *nbr, int weighted,
+                              const float *gin, double *acc, Workspace &ws, hipStream_t s) {
+    const long long np = plan.np;
+    if (np != b.np || plan.nhalo != b.nhalo) return hipErrorInvalidValue;
+    unsigned *order = nullptr, *hsorted = nullptr, *halo_of = nullptr;
+    PGALF_HIP_TRY(batch_order<P>(x, y, z, b, plan, ws, s, &order, &hsorted, &halo_of));
+    unsigned *rank = ws.alloc_n<unsigned>(np);
+    if (!rank) return hipErrorOutOfMemory;
+    inverse_perm_k<<<nblk(np, 256), 256, 0, s>>>(order, np, rank);
+    knn_backward_k<P><<<nblk(np, kBwdBlock), kBwdBlock, 0, s>>>(
+        x, y, z, m, np, nbr, plan.k, weighted, gin, acc, order, rank, halo_of, b.segs);
+    return hipGetLastError();
+}
+
+template hipError_t knn_batch_backward<float>(const float *, const float *, const float *,
+                                              const float *, const BatchDev &,
+                                              const KnnBatchPlan &, const long long *, int,
+                                              const float *, double *, Workspace &, hipStream_t);
 
 template hipError_t knn_batch<float>(const float *, const float *, const float *, const float *,
                                      const BatchDev &, const KnnBatchPlan &, long long *,

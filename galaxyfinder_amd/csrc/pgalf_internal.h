@@ -127,6 +127,23 @@ template <class P>
 hipError_t tsc_grad_readback(const P *x, const P *y, const P *z, const P *m, const float *gin,
                              long long np, const Grid &g, const float *F, const float *A,
                              double *acc, Workspace &ws, hipStream_t s);
+template <class P>
+hipError_t tsc_grad_readback_batch(const P *x, const P *y, const P *z, const P *m,
+                                   const float *gin, const BatchDev &b, const float *F,
+                                   const float *A, double *acc, Workspace &ws, hipStream_t s);
+// Peak climbing (see tsc.hip): niter normalised gradient-ascent steps of
+// `step` cells on the read-back field of the smoothed grid den, climbed
+// positions to ox/oy/oz, and each particle's peak (the local maximum of den
+// reached by integer steepest ascent from its climbed node; unpadded flat
+// index within its halo's grid) to peak.
+template <class P>
+hipError_t peak_climb(const P *x, const P *y, const P *z, long long np, const Grid &g,
+                      const float *den, int niter, double step, float *ox, float *oy,
+                      float *oz, int *peak, hipStream_t s);
+template <class P>
+hipError_t peak_climb_batch(const P *x, const P *y, const P *z, const BatchDev &b,
+                            const float *den, int niter, double step, float *ox, float *oy,
+                            float *oz, int *peak, hipStream_t s);
 
 // ---- Gaussian smoothing (smooth.hip) ------------------------------------
 hipError_t gaussian_smooth(float *den, const Grid &g, float RG, Workspace &ws,
@@ -187,6 +204,11 @@ template <class P>
 hipError_t knn_density_backward(const P *x, const P *y, const P *z, const P *m, long long np,
                                 const long long *nbr, int k, int weighted, const float *gin,
                                 double *acc, Workspace &ws, hipStream_t s);
+// Batched form: lists of the batch (indices local to each halo).
+template <class P>
+hipError_t knn_batch_backward(const P *x, const P *y, const P *z, const P *m, const BatchDev &b,
+                              const KnnBatchPlan &plan, const long long *nbr, int weighted,
+                              const float *gin, double *acc, Workspace &ws, hipStream_t s);
 // out[i] = (float)acc[i] for the four gradient arrays (any may be NULL).
 hipError_t grad_finalize(const double *acc, long long np, float *gx, float *gy, float *gz,
                          float *gm, hipStream_t s);

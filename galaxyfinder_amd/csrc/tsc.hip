@@ -468,48 +468,160 @@ __device__ inline void grad_axis(double p, long id[3], double w[3], double dw[3]
 // Gradient read-back of L(rho_tsc), rho_tsc = R S D: with gin = dL/drho,
 // F the forward smoothed grid and A = S D_gin the adjoint grid,
 //   dL/dm_p = sum_nodes W A,   dL/dx_p = (1/h) sum_nodes dW/dx (g_p F + m_p A).
-// acc holds [gx | gy | gz | gm] (np each, FP64); this kernel adds its part.
+// acc holds [gx | gy | gz | gm] (nacc each, FP64); this kernel adds its part.
+// One block per particle chunk (batch) or per `per` particles (one halo).
 template <class P>
 __global__ void __launch_bounds__(kBlock)
-tsc_gradback_k(const P *x, const P *y, const P *z, const P *m, const float *gin, long long np,
-               Grid g, const float *F, const float *A, double *acc, unsigned *status) {
-    const long long p = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (p >= np) return;
-    long id[3][3];
-    double w[3][3], dw[3][3];
-    grad_axis(((double)x[p] - g.x0) / g.h, id[0], w[0], dw[0]);
-    grad_axis(((double)y[p] - g.y0) / g.h, id[1], w[1], dw[1]);
-    grad_axis(((double)z[p] - g.z0) / g.h, id[2], w[2], dw[2]);
+tsc_gradback_k(const P *x, const P *y, const P *z, const P *m, const float *gin,
+               const HaloSeg *segs, HaloSeg seg0, const PChunk *chunks, int per,
+               const float *F, const float *A, double *acc, long long nacc,
+               unsigned *status) {
+    const PChunk ch = chunk_at(chunks, seg0, per);
+    const HaloSeg sg = seg_at(segs, seg0, ch.halo);
+    const Grid g = sg.g;
+    const float *hF = F + sg.goff, *hA = A + sg.goff;
     const long mx = (long)g.mx();
     const long n[3] = {mx, g.ny, g.nz};
+    for (long long p = ch.b0 + threadIdx.x; p < ch.b1; p += kBlock) {
+        long id[3][3];
+        double w[3][3], dw[3][3];
+        grad_axis(((double)x[p] - g.x0) / g.h, id[0], w[0], dw[0]);
+        grad_axis(((double)y[p] - g.y0) / g.h, id[1], w[1], dw[1]);
+        grad_axis(((double)z[p] - g.z0) / g.h, id[2], w[2], dw[2]);
+        bool inside = true;
 #pragma unroll
-    for (int a = 0; a < 3; a++)
+        for (int a = 0; a < 3; a++)
 #pragma unroll
-        for (int q = 0; q < 3; q++)
-            if (id[a][q] < 0 || id[a][q] >= n[a]) {
-                atomicOr(status, (unsigned)DEV_ERR_OUT_OF_GRID);
-                return;
-            }
-    const double gp = (double)gin[p], mp = (double)m[p];
-    double sx = 0.0, sy = 0.0, sz = 0.0, sm = 0.0;
+            for (int q = 0; q < 3; q++) inside &= id[a][q] >= 0 && id[a][q] < n[a];
+        if (!inside) {
+            atomicOr(status, (unsigned)DEV_ERR_OUT_OF_GRID);
+            continue;
+        }
+        const double gp = (double)gin[p], mp = (double)m[p];
+        double sx = 0.0, sy = 0.0, sz = 0.0, sm = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int b = 0; b < 3; b++)
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    const long o = id[0][a] + mx * (id[1][b] + (long)g.ny * id[2][c]);
+                    const double fa = (double)hA[o];
+                    const double v = gp * (double)hF[o] + mp * fa;
+                    sx += dw[0][a] * w[1][b] * w[2][c] * v;
+                    sy += w[0][a] * dw[1][b] * w[2][c] * v;
+                    sz += w[0][a] * w[1][b] * dw[2][c] * v;
+                    sm += w[0][a] * w[1][b] * w[2][c] * fa;
+                }
+        acc[p] += sx / g.h;
+        acc[nacc + p] += sy / g.h;
+        acc[2 * nacc + p] += sz / g.h;
+        acc[3 * nacc + p] += sm;
+    }
+}
+
+// ---- peak climbing (no counterpart in the reference) -------------------------
+// Gradient (cell units) of the read-back field of the smoothed grid F at cell
+// coordinate q, FP64 TSC weights; false when the stencil leaves the grid.
+__device__ inline bool field_grad(const double q[3], const Grid &g, const float *F,
+                                  double G[3]) {
+    long id[3][3];
+    double w[3][3], dw[3][3];
+    const long n[3] = {(long)g.nx, g.ny, g.nz};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        grad_axis(q[a], id[a], w[a], dw[a]);
+#pragma unroll
+        for (int t = 0; t < 3; t++)
+            if (id[a][t] < 0 || id[a][t] >= n[a]) return false;
+    }
+    const long mx = (long)g.mx();
+    double gx = 0.0, gy = 0.0, gz = 0.0;
 #pragma unroll
     for (int c = 0; c < 3; c++)
 #pragma unroll
         for (int b = 0; b < 3; b++)
 #pragma unroll
             for (int a = 0; a < 3; a++) {
-                const long o = id[0][a] + mx * (id[1][b] + (long)g.ny * id[2][c]);
-                const double fa = (double)A[o];
-                const double v = gp * (double)F[o] + mp * fa;
-                sx += dw[0][a] * w[1][b] * w[2][c] * v;
-                sy += w[0][a] * dw[1][b] * w[2][c] * v;
-                sz += w[0][a] * w[1][b] * dw[2][c] * v;
-                sm += w[0][a] * w[1][b] * w[2][c] * fa;
+                const double f = (double)F[id[0][a] + mx * (id[1][b] + (long)g.ny * id[2][c])];
+                gx += dw[0][a] * w[1][b] * w[2][c] * f;
+                gy += w[0][a] * dw[1][b] * w[2][c] * f;
+                gz += w[0][a] * w[1][b] * dw[2][c] * f;
             }
-    acc[p] += sx / g.h;
-    acc[np + p] += sy / g.h;
-    acc[2 * np + p] += sz / g.h;
-    acc[3 * np + p] += sm;
+    G[0] = gx;
+    G[1] = gy;
+    G[2] = gz;
+    return true;
+}
+
+// Peak climbing: niter normalised gradient-ascent steps of `step` cells on
+// the read-back field of the smoothed grid (positions clamped to [1, n-2]
+// cells so every stencil stays inside), then membership: from the nearest
+// node (rint) an exact integer steepest ascent over the 26-neighbourhood of
+// F (move to the largest neighbour while it is strictly larger than the
+// current node; among equal neighbours the lowest index) to a local maximum,
+// whose unpadded flat index i + nx (j + ny k) is the particle's peak.  Each
+// move strictly increases F, so the walk ends; kClimbMax bounds it.
+constexpr int kClimbMax = 4096;
+
+template <class P>
+__global__ void __launch_bounds__(kBlock)
+peak_climb_k(const P *x, const P *y, const P *z, const HaloSeg *segs, HaloSeg seg0,
+             const PChunk *chunks, int per, const float *den, int niter, double step,
+             float *ox, float *oy, float *oz, int *peak) {
+    const PChunk ch = chunk_at(chunks, seg0, per);
+    const HaloSeg sg = seg_at(segs, seg0, ch.halo);
+    const Grid g = sg.g;
+    const float *F = den + sg.goff;
+    const long long mx = g.mx();
+    const double lim[3] = {(double)(g.nx - 2), (double)(g.ny - 2), (double)(g.nz - 2)};
+    for (long long p = ch.b0 + threadIdx.x; p < ch.b1; p += kBlock) {
+        double q[3] = {((double)x[p] - g.x0) / g.h, ((double)y[p] - g.y0) / g.h,
+                       ((double)z[p] - g.z0) / g.h};
+#pragma unroll
+        for (int a = 0; a < 3; a++) q[a] = fmin(fmax(q[a], 1.0), lim[a]);
+        for (int it = 0; it < niter; it++) {
+            double G[3];
+            if (!field_grad(q, g, F, G)) break;
+            const double nrm = sqrt(G[0] * G[0] + G[1] * G[1] + G[2] * G[2]);
+            if (!(nrm > 0.0)) break;
+#pragma unroll
+            for (int a = 0; a < 3; a++) q[a] = fmin(fmax(q[a] + step * G[a] / nrm, 1.0), lim[a]);
+        }
+        ox[p] = (float)(g.x0 + q[0] * g.h);
+        oy[p] = (float)(g.y0 + q[1] * g.h);
+        oz[p] = (float)(g.z0 + q[2] * g.h);
+        long c[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) c[a] = (long)rint(q[a]);
+        float fc = F[c[0] + mx * (c[1] + (long long)g.ny * c[2])];
+        for (int step_i = 0; step_i < kClimbMax; step_i++) {
+            long best[3] = {c[0], c[1], c[2]};
+            float fb = fc;
+            for (int dz = -1; dz <= 1; dz++)
+                for (int dy = -1; dy <= 1; dy++)
+                    for (int dx = -1; dx <= 1; dx++) {
+                        const long i = c[0] + dx, j = c[1] + dy, k = c[2] + dz;
+                        if (i < 0 || i >= g.nx || j < 0 || j >= g.ny || k < 0 || k >= g.nz)
+                            continue;
+                        const float f = F[i + mx * (j + (long long)g.ny * k)];
+                        // loop order visits neighbours by increasing flat index:
+                        // a strictly larger value wins, so ties keep the lowest
+                        if (f > fb) {
+                            fb = f;
+                            best[0] = i;
+                            best[1] = j;
+                            best[2] = k;
+                        }
+                    }
+            if (!(fb > fc)) break;
+            c[0] = best[0];
+            c[1] = best[1];
+            c[2] = best[2];
+            fc = fb;
+        }
+        peak[p] = (int)(c[0] + (long long)g.nx * (c[1] + (long long)g.ny * c[2]));
+    }
 }
 
 HaloSeg single_seg(const Grid &g, long long np) {
@@ -647,8 +759,44 @@ hipError_t tsc_grad_readback(const P *x, const P *y, const P *z, const P *m, con
                              long long np, const Grid &g, const float *F, const float *A,
                              double *acc, Workspace &ws, hipStream_t s) {
     if (np <= 0) return hipSuccess;
+    const HaloSeg seg0 = single_seg(g, np);
     tsc_gradback_k<P><<<(unsigned)((np + kBlock - 1) / kBlock), kBlock, 0, s>>>(
-        x, y, z, m, gin, np, g, F, A, acc, ws.status());
+        x, y, z, m, gin, nullptr, seg0, nullptr, kBlock, F, A, acc, np, ws.status());
+    return hipGetLastError();
+}
+
+template <class P>
+hipError_t tsc_grad_readback_batch(const P *x, const P *y, const P *z, const P *m,
+                                   const float *gin, const BatchDev &b, const float *F,
+                                   const float *A, double *acc, Workspace &ws, hipStream_t s) {
+    if (b.nchunks <= 0) return hipSuccess;
+    const HaloSeg seg0{};
+    tsc_gradback_k<P><<<(unsigned)b.nchunks, kBlock, 0, s>>>(
+        x, y, z, m, gin, b.segs, seg0, b.chunks, kChunkParticles, F, A, acc, b.np, ws.status());
+    return hipGetLastError();
+}
+
+template <class P>
+hipError_t peak_climb(const P *x, const P *y, const P *z, long long np, const Grid &g,
+                      const float *den, int niter, double step, float *ox, float *oy,
+                      float *oz, int *peak, hipStream_t s) {
+    if (np <= 0) return hipSuccess;
+    if (g.nx < 3 || g.ny < 3 || g.nz < 3) return hipErrorInvalidValue;
+    const HaloSeg seg0 = single_seg(g, np);
+    peak_climb_k<P><<<(unsigned)((np + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+        x, y, z, nullptr, seg0, nullptr, kBlock, den, niter, step, ox, oy, oz, peak);
+    return hipGetLastError();
+}
+
+template <class P>
+hipError_t peak_climb_batch(const P *x, const P *y, const P *z, const BatchDev &b,
+                            const float *den, int niter, double step, float *ox, float *oy,
+                            float *oz, int *peak, hipStream_t s) {
+    if (b.nchunks <= 0) return hipSuccess;
+    const HaloSeg seg0{};
+    peak_climb_k<P><<<(unsigned)b.nchunks, kBlock, 0, s>>>(x, y, z, b.segs, seg0, b.chunks,
+                                                          kChunkParticles, den, niter, step, ox,
+                                                          oy, oz, peak);
     return hipGetLastError();
 }
 
@@ -657,6 +805,16 @@ hipError_t tsc_grad_readback(const P *x, const P *y, const P *z, const P *m, con
                                              const float *, long long, const Grid &,           \
                                              const float *, const float *, double *,           \
                                              Workspace &, hipStream_t);                        \
+    template hipError_t tsc_grad_readback_batch<P>(const P *, const P *, const P *, const P *, \
+                                                   const float *, const BatchDev &,            \
+                                                   const float *, const float *, double *,     \
+                                                   Workspace &, hipStream_t);                  \
+    template hipError_t peak_climb<P>(const P *, const P *, const P *, long long,              \
+                                      const Grid &, const float *, int, double, float *,       \
+                                      float *, float *, int *, hipStream_t);                   \
+    template hipError_t peak_climb_batch<P>(const P *, const P *, const P *, const BatchDev &, \
+                                            const float *, int, double, float *, float *,      \
+                                            float *, int *, hipStream_t);                      \
     template hipError_t tsc_deposit<P>(const P *, const P *, const P *, const P *, const int *, \
                                        long long, int, const Grid &, float *, Workspace &,      \
                                        hipStream_t);                                           \

@@ -320,6 +320,40 @@ class Batch:
                                        _dptr(den), _dptr(rho_tsc), _dptr(nbr), _dptr(rho_knn),
                                        1 if graph else 0, _stream()))
 
+    def run_grad(self, x, y, z, m, w_tsc, w_knn, rho_mode=2, climb_iters=5, climb_step=0.5,
+                 graph=False, out=None):
+        """Forward, cotangents of sum w_tsc log rho_tsc + sum w_knn log rho_knn,
+        adjoint and peak climbing in one launch sequence (pgalf_batch_run_grad).
+        Returns a dict of device tensors: den, rho_tsc, nbr, rho_knn, gx, gy,
+        gz, gm, cx, cy, cz, peak (``out`` may supply them to reuse buffers)."""
+        torch = _torch()
+        n = int(self.offsets[-1])
+        dev = x.device
+        for t, w in ((x, "x"), (y, "y"), (z, "z"), (m, "m")):
+            _check_dev(t, torch.float32, n, w)
+        for t, w in ((w_tsc, "w_tsc"), (w_knn, "w_knn")):
+            if t is not None:
+                _check_dev(t, torch.float32, n, w)
+        if out is None:
+            f32 = dict(dtype=torch.float32, device=dev)
+            out = dict(den=torch.empty(self.grid_floats, **f32),
+                       rho_tsc=torch.empty(n, **f32), rho_knn=torch.empty(n, **f32),
+                       nbr=torch.empty((n, self.k), dtype=torch.int64, device=dev),
+                       gx=torch.empty(n, **f32), gy=torch.empty(n, **f32),
+                       gz=torch.empty(n, **f32), gm=torch.empty(n, **f32),
+                       cx=torch.empty(n, **f32), cy=torch.empty(n, **f32),
+                       cz=torch.empty(n, **f32),
+                       peak=torch.empty(n, dtype=torch.int32, device=dev))
+        o = out
+        lib = _lib.load()
+        _lib.check(lib.pgalf_batch_run_grad(
+            self._b, _dptr(x), _dptr(y), _dptr(z), _dptr(m), _dptr(w_tsc), _dptr(w_knn),
+            int(rho_mode), _dptr(o["den"]), _dptr(o["rho_tsc"]), _dptr(o["nbr"]),
+            _dptr(o["rho_knn"]), _dptr(o["gx"]), _dptr(o["gy"]), _dptr(o["gz"]),
+            _dptr(o["gm"]), int(climb_iters), float(climb_step), _dptr(o["cx"]),
+            _dptr(o["cy"]), _dptr(o["cz"]), _dptr(o["peak"]), 1 if graph else 0, _stream()))
+        return out
+
     def status(self):
         """Synchronise torch's current stream; raise on kernel-detected errors."""
         lib = _lib.load()
@@ -335,6 +369,23 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def dev_peak_climb(x, y, z, grid, den, niter=5, step=0.5):
+    """Peak climbing on the smoothed grid den (pgalf_dev_peak_climb): returns
+    (climbed x, y, z, peak flat index int32)."""
+    torch = _torch()
+    n = x.numel()
+    for t, w in ((x, "x"), (y, "y"), (z, "z")):
+        _check_dev(t, torch.float32, n, w)
+    _check_dev(den, torch.float32, 2 * (grid.nx // 2 + 1) * grid.ny * grid.nz, "den")
+    ox, oy, oz = (torch.empty(n, dtype=torch.float32, device=x.device) for _ in range(3))
+    peak = torch.empty(n, dtype=torch.int32, device=x.device)
+    lib = _lib.load()
+    _lib.check(lib.pgalf_dev_peak_climb(_dptr(x), _dptr(y), _dptr(z), n, ctypes.byref(grid),
+                                        _dptr(den), int(niter), float(step), _dptr(ox),
+                                        _dptr(oy), _dptr(oz), _dptr(peak), _stream()))
+    return ox, oy, oz, peak
 
 
 def dev_status():
@@ -363,4 +414,5 @@ __all__ = ["PgalfError", "PARTICLE_DTYPE", "make_particles", "padded_shape",
            "assign_density_TSC", "gaussian_Smoothing", "findDen", "findsphdensity",
            "starfindsphdensity", "near_lists", "lag_density_stage", "grid_struct",
            "dev_tsc", "dev_smooth", "dev_findden", "dev_knn", "dev_density_stage",
-           "dev_density_stage_backward", "dev_status", "Batch", "timing", "timing_get"]
+           "dev_density_stage_backward", "dev_peak_climb", "dev_status", "Batch", "timing",
+           "timing_get"]

@@ -146,6 +146,16 @@ int pgalf_dev_density_stage_backward(const float *x, const float *y, const float
                                      const long long *nbr, int rho_mode, const float *g_tsc,
                                      const float *g_knn, float *gx, float *gy, float *gz,
                                      float *gm, void *stream);
+/* Peak climbing on a smoothed grid `den` (no counterpart in the reference;
+ * uses the density gradients of the adjoint machinery): niter normalised
+ * gradient-ascent steps of `step` cells on the TSC read-back field of den
+ * (positions kept within [1, n-2] cells), climbed positions to ox, oy, oz,
+ * then an exact integer steepest ascent over the 26-neighbourhood of den
+ * from the nearest node to a local maximum: its unpadded flat index
+ * i + nx*(j + ny*k) is the particle's peak (membership). */
+int pgalf_dev_peak_climb(const float *x, const float *y, const float *z, long long np,
+                         const pgalf_grid_t *g, const float *den, int niter, float step,
+                         float *ox, float *oy, float *oz, int *peak, void *stream);
 /* Synchronise `stream` and return the status of the kernels run since the
  * previous call (PGALF_ERR_OUT_OF_GRID / PGALF_ERR_SHORT_LIST or PGALF_OK). */
 int pgalf_dev_status(void *stream);
@@ -176,6 +186,24 @@ long long pgalf_batch_grid_offset(const pgalf_batch_t *b, int h);
 int pgalf_batch_run(pgalf_batch_t *b, const float *x, const float *y, const float *z,
                     const float *m, float *den, float *rho_tsc, long long *nbr,
                     float *rho_knn, int use_graph, void *stream);
+/* One differentiable pass over the catalogue with peak climbing (no
+ * counterpart in the reference), stream-ordered and, with use_graph != 0,
+ * replayed from one HIP graph:
+ *  - forward as pgalf_batch_run, with the W4 kNN density of rho_mode
+ *    1 (unweighted) or 2 (mass-weighted);
+ *  - the cotangents of L = sum w_tsc log rho_tsc + sum w_knn log rho_knn
+ *    (w_tsc / w_knn may be NULL: that term is absent);
+ *  - the adjoint (as pgalf_dev_density_stage_backward) into gx, gy, gz, gm;
+ *  - peak climbing on each halo's smoothed grid (as pgalf_dev_peak_climb):
+ *    climbed positions in cx, cy, cz and each particle's peak cell (its
+ *    membership) in peak.
+ * All other pointers are required. */
+int pgalf_batch_run_grad(pgalf_batch_t *b, const float *x, const float *y, const float *z,
+                         const float *m, const float *w_tsc, const float *w_knn, int rho_mode,
+                         float *den, float *rho_tsc, long long *nearindex, float *rho_knn,
+                         float *gx, float *gy, float *gz, float *gm, int climb_iters,
+                         float climb_step, float *cx, float *cy, float *cz, int *peak,
+                         int use_graph, void *stream);
 /* Synchronise `stream` and return the status of the batch's kernels since
  * the previous call (PGALF_ERR_OUT_OF_GRID or PGALF_OK). */
 int pgalf_batch_status(pgalf_batch_t *b, void *stream);

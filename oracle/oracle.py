@@ -322,3 +322,65 @@ def stage_grad_d(pos, mass, grid, RG, idx, w_tsc, w_knn, weighted=True):
                          _p(gy, _dp), _p(gz, _dp), _p(gm, _dp))
     return np.stack([gx, gy, gz], axis=1), gm
 
+
+
+def peak_climb(pos, grid, den_padded, niter=5, step=0.5):
+    """Peak climbing restated (the GPU's pgalf_dev_peak_climb): niter
+    normalised gradient-ascent steps of `step` cells on the FP64 TSC read-back
+    field of the smoothed grid (orc_tsc_grad_d in cell units: unit cotangent,
+    zero mass), positions clamped to [1, n-2] cells; then integer steepest
+    ascent over the 26-neighbourhood of the FP32 grid to a local maximum
+    (strictly larger values only, ties to the lowest flat index).
+    Returns (climbed positions float32 (n,3), peak flat index int32)."""
+    nx, ny, nz, x0, y0, z0, h = grid
+    F3 = np.ascontiguousarray(np.asarray(den_padded, dtype=np.float32)[..., :nx],
+                              dtype=np.float64)
+    pos = _f64(pos)
+    lo = np.array([x0, y0, z0])
+    lim = np.array([nx - 2, ny - 2, nz - 2], dtype=np.float64)
+    q = np.minimum(np.maximum((pos - lo[None, :]) / h, 1.0), lim[None, :])
+    n = len(q)
+    one, zero = np.ones(n), np.zeros(n)
+    zgrid = np.zeros_like(F3)
+    cgrid = (nx, ny, nz, 0.0, 0.0, 0.0, 1.0)
+    active = np.ones(n, dtype=bool)
+    for _ in range(niter):
+        gx, gy, gz = (np.zeros(n) for _ in range(3))
+        qx, qy, qz = (np.ascontiguousarray(q[:, a]) for a in range(3))
+        gm = np.zeros(n)
+        lib().orc_tsc_grad_d(_p(qx, _dp), _p(qy, _dp), _p(qz, _dp), _p(zero, _dp),
+                             _p(one, _dp), n, _p(F3, _dp), _p(zgrid, _dp), nx, ny, nz,
+                             0.0, 0.0, 0.0, 1.0, _p(gx, _dp), _p(gy, _dp), _p(gz, _dp),
+                             _p(gm, _dp))
+        G = np.stack([gx, gy, gz], axis=1)
+        nrm = np.sqrt(G[:, 0] * G[:, 0] + G[:, 1] * G[:, 1] + G[:, 2] * G[:, 2])
+        active &= nrm > 0.0
+        safe = np.where(active, nrm, 1.0)
+        step_v = step * G / safe[:, None]
+        qn = np.minimum(np.maximum(q + step_v, 1.0), lim[None, :])
+        q = np.where(active[:, None], qn, q)
+    out = (lo[None, :] + q * h).astype(np.float32)
+    Ff = np.asarray(den_padded, dtype=np.float32)[..., :nx]
+    c = np.rint(q).astype(np.int64)
+    offs = [(dx, dy, dz) for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1)]
+    fc = Ff[c[:, 2], c[:, 1], c[:, 0]]
+    moving = np.ones(n, dtype=bool)
+    for _ in range(4096):
+        if not moving.any():
+            break
+        best = c.copy()
+        fb = fc.copy()
+        for dx, dy, dz in offs:
+            i, j, k = c[:, 0] + dx, c[:, 1] + dy, c[:, 2] + dz
+            ok = (i >= 0) & (i < nx) & (j >= 0) & (j < ny) & (k >= 0) & (k < nz)
+            f = np.full(n, -np.inf, dtype=np.float32)
+            f[ok] = Ff[k[ok], j[ok], i[ok]]
+            take = moving & (f > fb)
+            fb = np.where(take, f, fb)
+            best[take] = np.stack([i, j, k], axis=1)[take]
+        moved = moving & (fb > fc)
+        c = np.where(moved[:, None], best, c)
+        fc = np.where(moved, fb, fc)
+        moving = moved
+    peak = (c[:, 0] + nx * (c[:, 1] + ny * c[:, 2])).astype(np.int32)
+    return out, peak
