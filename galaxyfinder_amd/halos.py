@@ -162,6 +162,70 @@ def catalogue(config_id=3, nhalo=10000, alpha=2.0, nmin=100, nmax=1048576, rg_ce
             np.asarray(rg, dtype=np.float32))
 
 
+def catalogue_vec(config_id=5, nhalo=50000, alpha=1.9, nmin=64, nmax=2097152, rg_cells=2.0,
+                  gmin=32, gmax=256, nfw_iters=30):
+    """Large catalogues with the distributions of `catalogue` (NFW c ~ U(4,
+    15), r200 = 0.4, Poisson(5) Plummer subhalos of log-uniform mass fraction
+    0.1-3%, scale 0.01, centres at radii U(0.1, 0.3); masses 1/n; one unit
+    box per halo), drawn for all halos at once: NFW radii by a vectorised
+    bisection of nfw_iters steps (2^-30 of r200 at the default), particles of
+    a halo ordered NFW first, then its subhalos.  Returns the same tuple as
+    `catalogue`."""
+    rng = rng_for(config_id)
+    counts = powerlaw_counts(rng, nhalo, alpha, nmin, nmax)
+    conc = rng.uniform(4.0, 15.0, nhalo)
+    nsub = rng.poisson(5.0, nhalo)
+    hs = np.repeat(np.arange(nhalo), nsub)                 # halo of each subhalo
+    fr = np.exp(rng.uniform(math.log(0.001), math.log(0.03), len(hs)))
+    srad = rng.uniform(0.1, 0.3, len(hs))
+    scen = _isotropic(rng, len(hs)) * srad[:, None]
+    scnt = np.rint(fr * counts[hs]).astype(np.int64)
+    # keep each halo's subhalo total below its count
+    tot = np.bincount(hs, weights=scnt, minlength=nhalo).astype(np.int64)
+    over = tot > counts
+    if over.any():
+        scale = np.where(over, counts / np.maximum(tot, 1), 1.0)
+        scnt = np.floor(scnt * scale[hs]).astype(np.int64)
+        tot = np.bincount(hs, weights=scnt, minlength=nhalo).astype(np.int64)
+    n_nfw = counts - tot
+    # NFW part of every halo
+    cp = np.repeat(conc, n_nfw)
+    target = rng.uniform(0.0, 1.0, len(cp)) * _nfw_mass(1.0, cp)
+    lo = np.zeros(len(cp))
+    hi = np.ones(len(cp))
+    for _ in range(nfw_iters):
+        mid = 0.5 * (lo + hi)
+        below = _nfw_mass(mid, cp) < target
+        lo = np.where(below, mid, lo)
+        hi = np.where(below, hi, mid)
+    del cp, target
+    r = 0.5 * (lo + hi) * 0.4
+    nfw = _isotropic(rng, len(r)) * r[:, None]
+    del lo, hi, r
+    # Plummer parts
+    sp = np.repeat(np.arange(len(hs)), scnt)
+    rr = plummer_radii(rng, len(sp), 0.01)
+    sub = scen[sp] + _isotropic(rng, len(sp)) * rr[:, None]
+    del rr
+    # assemble halo by halo: NFW block then subhalo block
+    off = np.zeros(nhalo + 1, dtype=np.int64)
+    off[1:] = np.cumsum(counts)
+    pos = np.empty((int(off[-1]), 3), dtype=np.float32)
+    nfw_off = np.concatenate([[0], np.cumsum(n_nfw)])
+    sub_off = np.concatenate([[0], np.cumsum(tot)])
+    dst_nfw = np.repeat(off[:-1] - nfw_off[:-1], n_nfw) + np.arange(len(nfw))
+    pos[dst_nfw] = (nfw + 0.5).astype(np.float32)
+    dst_sub = np.repeat(off[:-1] + n_nfw - sub_off[:-1], tot) + np.arange(len(sub))
+    pos[dst_sub] = (sub + 0.5).astype(np.float32)
+    mass = np.repeat((1.0 / counts).astype(np.float32), counts)
+    grids, rg = [], []
+    for n in counts:
+        ng = grid_for(n, gmin, gmax)
+        grids.append((ng, ng, ng, 0.0, 0.0, 0.0, 1.0 / ng))
+        rg.append(rg_cells / ng)
+    return pos, mass, off, grids, np.asarray(rg, dtype=np.float32)
+
+
 def small_halo(n, seed=0, nsub=2):
     """Small clustered halo for tests (not a named workload)."""
     rng = np.random.Generator(np.random.PCG64(SEED_BASE + 1000 + seed))
