@@ -111,3 +111,15 @@ def test_catalogue_generator():
     # dN/dn ~ n^-2 on [100, 1e6]: the median is ~2 nmin
     big = halos.powerlaw_counts(halos.rng_for(0), 20000, 2.0, 100, 1048576)
     assert 180 < np.median(big) < 220
+
+
+def test_vectorised_catalogue():
+    p, m, off, grids, rg = halos.catalogue_vec(config_id=105, nhalo=300, nmax=50000)
+    c = np.diff(off)
+    assert off[0] == 0 and off[-1] == len(p) == len(m) and len(grids) == len(rg) == 300
+    assert c.min() >= 64 and c.max() <= 50000
+    assert p.min() >= 0.1 - 1e-6 and p.max() <= 0.9 + 1e-6
+    for h in (0, int(np.argmax(c))):
+        assert np.allclose(m[off[h]:off[h + 1]], 1.0 / c[h])
+        # each halo is centred in its own box
+        assert np.abs(p[off[h]:off[h + 1]].mean(axis=0) - 0.5).max() < 0.15

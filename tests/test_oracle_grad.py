@@ -99,3 +99,28 @@ def test_euler_mass_identity_and_translation(problem):
     assert abs(lhs - rhs) <= 1e-9 * max(1.0, np.abs(wt).sum() + np.abs(wk).sum())
     gk, _ = O.stage_grad_d(pos, mass, grid, RG, idx, np.zeros(N), wk, True)
     assert np.abs(gk.sum(axis=0)).max() <= 1e-9 * np.abs(gk).sum()
+
+
+def test_peak_climb_restatement_properties():
+    """The oracle's climb (the GPU climb's checker): peaks are local maxima
+    of the smoothed grid; with no ascent steps the peak is reached from the
+    nearest node; positions stay within [1, n-2] cells."""
+    pos, mass = halos.small_halo(3000, seed=12)
+    ng = 32
+    grid = (ng, ng, ng, 0.0, 0.0, 0.0, 1.0 / ng)
+    den, _ = O.tsc_deposit(pos, mass, grid)
+    sm = O.gsmooth(den, ng, ng, ng, 1.0 / ng, 2.0 / ng)
+    F = sm[..., :ng]
+    for niter in (0, 5):
+        out, peak = O.peak_climb(pos, grid, sm, niter, 0.5)
+        q = out.astype(np.float64) * ng
+        assert q.min() >= 1.0 - 1e-5 and q.max() <= ng - 2 + 1e-5
+        k, j, i = peak // (ng * ng), (peak // ng) % ng, peak % ng
+        for dz in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    ii, jj, kk = i + dx, j + dy, k + dz
+                    ok = (ii >= 0) & (ii < ng) & (jj >= 0) & (jj < ng) & (kk >= 0) & (kk < ng)
+                    assert np.all(F[kk[ok], jj[ok], ii[ok]] <= F[k[ok], j[ok], i[ok]])
+    out0, _ = O.peak_climb(pos, grid, sm, 0, 0.5)
+    assert np.allclose(out0, pos, atol=1e-6)
