@@ -57,6 +57,8 @@ struct AxisPass {
     int padw;           // row width for the pad zeroing (mx)
     int LS;             // line stride in LDS (complex elements)
     int tables;         // 1: twiddles and spectral factor staged in LDS
+    int pairs;          // 1: adjacent lines may be read as one float2 (8-byte aligned)
+    long long gcells;   // floats of one grid (register FFT: byte offsets fit 31 bits)
 };
 
 __device__ inline double2 cmul(double2 a, double2 b) {
@@ -222,6 +224,281 @@ smooth_axis_k(float *g, AxisPass ap, const double2 *tw_g, const double *H_g,
             long long ib = ib0 + lb;
             if (ib < ap.nb) base[ib * ap.sb + j] = 0.0f;
         }
+    }
+}
+
+// ---- register-resident four-step FFT for lines of 64, 128 and 256 -------------
+// L = RA * RB (64 = 8 x 8, 128 = 8 x 16, 256 = 16 x 16).  With j = j1 + RB j2
+// and k = k2 + RA k1,
+//   X[k2 + RA k1] = sum_j1 W_RB^{j1 k1} W_L^{j1 k2} sum_j2 W_RA^{j2 k2} x[j1 + RB j2].
+// Step 1: thread j1 loads x[j1 + RB j2] (j2 < RA) of its two real lines
+// straight from the grid, runs a DFT_RA in registers and applies W_L^{j1 k2};
+// an LDS transpose gives thread k2 the row Y[k2][j1] (j1 < RB).  Step 2:
+// DFT_RB -> X[k2 + RA k1], times the real factor H, inverse DFT_RB and
+// W_L^{-j1 k2}; the inverse transform keeps the same element order, so
+// nothing moves between the two.  A second transpose returns column j1 to
+// thread j1, whose inverse DFT_RA yields x'[j1 + RB j2] -- the elements it
+// loaded -- and it stores them straight back.  Two LDS round trips per line
+// in all, every DFT in FP64 registers, one FP32 rounding per output.
+
+// cos and sin of 2 pi m / 16 (m after unrolling is a constant)
+__device__ constexpr double kCos1 = 0x1.d906bcf328d46p-1;  // cos(pi/8)
+__device__ constexpr double kCos2 = 0x1.6a09e667f3bcdp-1;  // cos(pi/4)
+__device__ constexpr double kCos3 = 0x1.87de2a6aea963p-2;  // cos(3pi/8)
+__device__ constexpr double cos16(int m) {
+    return m == 0 ? 1.0 : m == 1 ? kCos1 : m == 2 ? kCos2 : m == 3 ? kCos3 : m == 4 ? 0.0
+         : m == 5 ? -kCos3 : m == 6 ? -kCos2 : -kCos1;  // m < 8
+}
+__device__ constexpr double sin16(int m) {
+    return m == 0 ? 0.0 : m == 1 ? kCos3 : m == 2 ? kCos2 : m == 3 ? kCos1 : m == 4 ? 1.0
+         : m == 5 ? kCos1 : m == 6 ? kCos2 : kCos3;  // m < 8
+}
+// b * exp(-+2 pi i m / 16): forward takes the minus sign, INV the plus
+// FMA contraction is allowed in this FFT (it follows no reference rounding;
+// the parity bar is against the FP64 smoothing).
+__device__ inline double2 cmul_c(double2 a, double2 b) {
+#pragma clang fp contract(fast)
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <bool INV>
+__device__ inline double2 tw16(double2 b, int m) {
+#pragma clang fp contract(fast)
+    if (m == 0) return b;
+    if (m == 4) return INV ? make_double2(-b.y, b.x) : make_double2(b.y, -b.x);
+    const double c = cos16(m), s = INV ? sin16(m) : -sin16(m);
+    return make_double2(b.x * c - b.y * s, b.x * s + b.y * c);
+}
+template <int R>
+__device__ constexpr int bitrev(int i) {
+    int r = 0;
+    for (int b = 1; b < R; b <<= 1) {
+        r = (r << 1) | (i & 1);
+        i >>= 1;
+    }
+    return r;
+}
+// radix-2 decimation-in-time stages (SPAN = 1, 2, ..., R/2)
+template <int R, bool INV, int SPAN>
+__device__ inline void dit_stages(double2 (&v)[R]) {
+    if constexpr (SPAN < R) {
+#pragma unroll
+        for (int s0 = 0; s0 < R; s0 += 2 * SPAN) {
+#pragma unroll
+            for (int k = 0; k < SPAN; k++) {
+                const double2 a = v[s0 + k];
+                const double2 b = tw16<INV>(v[s0 + k + SPAN], k * (16 / (2 * SPAN)));
+                v[s0 + k] = make_double2(a.x + b.x, a.y + b.y);
+                v[s0 + k + SPAN] = make_double2(a.x - b.x, a.y - b.y);
+            }
+        }
+        dit_stages<R, INV, 2 * SPAN>(v);
+    }
+}
+// In-register DFT of size R <= 16, natural order in and out (unscaled).
+template <int R, bool INV>
+__device__ inline void dft_reg(double2 (&v)[R]) {
+    static_assert(R >= 2 && R <= 16 && (R & (R - 1)) == 0, "radix");
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        const int j = bitrev<R>(i);
+        if (j > i) {
+            const double2 t = v[i];
+            v[i] = v[j];
+            v[j] = t;
+        }
+    }
+    dit_stages<R, INV, 1>(v);
+}
+// Twiddle bases of one thread: p[b] = W_L^{b u} (b = 1..3) and
+// q[a] = W_L^{4 a u} (a = 1..RQ) from the table tw[m] = exp(-2 pi i m / L).
+template <int RQ>
+struct TwBase {
+    double2 p[3], q[RQ];
+};
+template <int RQ>
+__device__ inline TwBase<RQ> twiddle_base(const double2 *tw, int u, int L) {
+    TwBase<RQ> t;
+#pragma unroll
+    for (int b = 1; b < 4; b++) t.p[b - 1] = tw[(b * u) & (L - 1)];
+#pragma unroll
+    for (int a = 1; a <= RQ; a++) t.q[a - 1] = tw[(4 * a * u) & (L - 1)];
+    return t;
+}
+// v[k] *= W_L^{+-u k} (k < R): the powers 1..3 and the multiples of 4 are
+// table entries, the rest one product of two.
+template <int R, bool INV, int RQ>
+__device__ inline void twiddle_row(double2 (&v)[R], const TwBase<RQ> &t) {
+    static_assert(R % 4 == 0 && R / 4 - 1 <= RQ, "whole quads");
+#pragma unroll
+    for (int k = 1; k < R; k++) {
+        const int a = k / 4, b = k % 4;
+        double2 w = a == 0 ? t.p[b - 1] : b == 0 ? t.q[a - 1] : cmul_c(t.q[a - 1], t.p[b - 1]);
+        if (INV) w.y = -w.y;
+        v[k] = cmul_c(v[k], w);
+    }
+}
+
+// Line groups: one group is NL complex lines (2 NL real lines, consecutive
+// in the batch dimension) at one slow index of one grid; RB threads per
+// line.  A block walks the groups with a grid stride and loads the next
+// group's elements into registers before transforming the current one, so
+// the memory latency overlaps the arithmetic.  CONTIG (rows): lanes run along
+// the line, so loads are coalesced within a row; otherwise lanes run across
+// lines (consecutive x) and a line pair is one float2.  LDS rows are padded
+// (RB + 1 per k2 row, odd line stride) so both transposes spread over the
+// banks.
+struct Fft2Groups {
+    int nbx;           // groups along the batch dimension
+    int nc;            // slow batch indices
+    unsigned ngroups;  // nbx * nc * grids (< 2^31)
+};
+
+// Grid memory is read and written with buffer instructions: a wave-uniform
+// descriptor per group (its slice of the grid), one 32-bit byte offset per
+// lane and a uniform offset per element, so no per-element 64-bit address
+// is held in registers.
+typedef unsigned int u32x2 __attribute__((__vector_size__(8)));
+constexpr int kRsrcWord3 = 0x00020000;  // raw dword access (gfx9 family)
+
+template <int RA, int RB, int NL, bool CONTIG>
+__global__ void __launch_bounds__(NL * RB) __attribute__((amdgpu_waves_per_eu(2)))
+smooth_fft2_k(float *g, AxisPass ap, const double2 *tw, const double *H,
+              const long long *goffs, Fft2Groups gr) {
+    static_assert(RB >= RA, "step-1 threads cover every line element");
+    constexpr int L = RA * RB;
+    constexpr int NT = NL * RB;
+    constexpr int ROW = RB + 1;
+    constexpr int LSTR = (RA * ROW) | 1;
+    constexpr int RQ = RB / 4 - 1;
+    __shared__ double2 buf[NL * LSTR];
+    __shared__ double s_H[L];
+    const int tid = threadIdx.x;
+    const int line = CONTIG ? tid / RB : tid % NL;
+    const int u = CONTIG ? tid % RB : tid / NL;
+    for (int i = tid; i < L; i += NT) s_H[i] = H[i];
+    const TwBase<RQ> twb = twiddle_base<RQ>(tw, u, L);
+    double2 *row = buf + line * LSTR;
+    // byte strides: along the line (per element j2 of this thread) and
+    // between the two lines of a pair
+    const int step = RB * (int)ap.sa * 4;
+    const int bstep = (int)ap.sb * 4;
+
+    // group gi: its slice of the grid (wave-uniform) and this lane's lines
+    struct Grp {
+        float *base;  // the group's slice
+        int bytes;    // bytes from base to the end of the grid
+        int off;      // byte offset of element u of line a
+        bool va, vb;  // lines a, b present
+    };
+    auto locate = [&](unsigned gi) {
+        Grp q;
+        const unsigned rest = gi / (unsigned)gr.nbx;
+        const unsigned bx = gi - rest * (unsigned)gr.nbx;
+        const unsigned c = rest % (unsigned)gr.nc, z = rest / (unsigned)gr.nc;
+        const long long c0 = (long long)c * ap.sc;
+        q.base = g + (goffs ? goffs[z] : 0LL) + c0;
+        q.bytes = (int)((ap.gcells - c0) * 4);
+        const int la = (int)bx * 2 * NL + 2 * line;
+        q.va = la < ap.nb;
+        q.vb = la + 1 < ap.nb;
+        q.off = (int)(((long long)la * ap.sb + (long long)u * ap.sa) * 4);
+        return q;
+    };
+    auto rsrc = [](const Grp &q) {
+        return __builtin_amdgcn_make_buffer_rsrc(q.base, (short)0, q.bytes, kRsrcWord3);
+    };
+    auto load = [&](const Grp &q, float2 (&x)[RA]) {
+        const auto r = rsrc(q);
+        const bool pair = !CONTIG && ap.pairs && q.vb;
+#pragma unroll
+        for (int j2 = 0; j2 < RA; j2++) {
+            float a = 0.0f, b = 0.0f;
+            if (pair) {
+                const u32x2 t = __builtin_amdgcn_raw_buffer_load_b64(r, q.off, j2 * step, 0);
+                a = __uint_as_float(t[0]);
+                b = __uint_as_float(t[1]);
+            } else {
+                if (q.va) a = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, q.off, j2 * step, 0));
+                if (q.vb) b = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, q.off, j2 * step + bstep, 0));
+            }
+            x[j2] = make_float2(a, b);
+        }
+    };
+
+    unsigned gi = blockIdx.x;
+    Grp cg{};
+    float2 cur[RA];
+    if (gi < gr.ngroups) {
+        cg = locate(gi);
+        load(cg, cur);
+    }
+    __syncthreads();  // s_H
+    for (; gi < gr.ngroups; gi += gridDim.x) {
+        // next group's elements in flight during this group's transform
+        const unsigned gn = gi + gridDim.x;
+        Grp ng{};
+        float2 nxt[RA];
+        if (gn < gr.ngroups) {
+            ng = locate(gn);
+            load(ng, nxt);
+        }
+        // step 1: DFT_RA over j2 and W_L^{j1 k2}, j1 = u
+        double2 v[RA];
+#pragma unroll
+        for (int j2 = 0; j2 < RA; j2++) v[j2] = make_double2((double)cur[j2].x, (double)cur[j2].y);
+        dft_reg<RA, false>(v);
+        twiddle_row<RA, false>(v, twb);
+        __syncthreads();  // the previous group's last reads of buf are done
+#pragma unroll
+        for (int k2 = 0; k2 < RA; k2++) row[k2 * ROW + u] = v[k2];
+        __syncthreads();
+        // step 2 (k2 = u): each thread reads and rewrites only its own row
+        if (u < RA) {
+            double2 w[RB];
+#pragma unroll
+            for (int j1 = 0; j1 < RB; j1++) w[j1] = row[u * ROW + j1];
+            dft_reg<RB, false>(w);
+#pragma unroll
+            for (int k1 = 0; k1 < RB; k1++) {
+                const double h = s_H[u + RA * k1];
+                w[k1] = make_double2(w[k1].x * h, w[k1].y * h);
+            }
+            dft_reg<RB, true>(w);
+            twiddle_row<RB, true>(w, twb);
+#pragma unroll
+            for (int j1 = 0; j1 < RB; j1++) row[u * ROW + j1] = w[j1];
+        }
+        __syncthreads();
+        // step 3: inverse DFT_RA over k2 gives x'[u + RB j2]
+#pragma unroll
+        for (int k2 = 0; k2 < RA; k2++) v[k2] = row[k2 * ROW + u];
+        dft_reg<RA, true>(v);
+        const bool pair = !CONTIG && ap.pairs && cg.vb;
+        const auto cr = rsrc(cg);
+#pragma unroll
+        for (int j2 = 0; j2 < RA; j2++) {
+            const unsigned a = __float_as_uint((float)v[j2].x);
+            const unsigned b = __float_as_uint((float)v[j2].y);
+            if (pair) {
+                u32x2 t;
+                t[0] = a;
+                t[1] = b;
+                __builtin_amdgcn_raw_buffer_store_b64(t, cr, cg.off, j2 * step, 0);
+            } else {
+                if (cg.va) __builtin_amdgcn_raw_buffer_store_b32(a, cr, cg.off, j2 * step, 0);
+                if (cg.vb) __builtin_amdgcn_raw_buffer_store_b32(b, cr, cg.off, j2 * step + bstep, 0);
+            }
+        }
+        if (ap.zero_pads && u == 0) {
+            // rows: element j of line a at off + 4 j (u = 0)
+            for (int j = ap.n; j < ap.padw; j++) {
+                if (cg.va) __builtin_amdgcn_raw_buffer_store_b32(0u, cr, cg.off, 4 * j, 0);
+                if (cg.vb) __builtin_amdgcn_raw_buffer_store_b32(0u, cr, cg.off, 4 * j + bstep, 0);
+            }
+        }
+        cg = ng;
+#pragma unroll
+        for (int j2 = 0; j2 < RA; j2++) cur[j2] = nxt[j2];
     }
 }
 
@@ -414,11 +691,93 @@ hipError_t get_plan(int n, double expfac, AxisPlan &out) {
     return hipSuccess;
 }
 
+// Kernel choice per line length (compile-time switches for A/B timing):
+// the register FFT for n = 64, 128, 256 unless the direct convolution takes
+// the pass (strided lines up to PGALF_DIRECT_STRIDED_MAX, rows up to 32).
+#ifndef PGALF_SMOOTH_FFT2
+#define PGALF_SMOOTH_FFT2 1
+#endif
+#ifndef PGALF_DIRECT_STRIDED_MAX
+#define PGALF_DIRECT_STRIDED_MAX 32  // 40..56 direct; 64 takes the register FFT
+#endif
+#ifndef PGALF_FFT2_NL256
+#define PGALF_FFT2_NL256 16  // complex lines per block at L = 256 (16: 128-byte rows)
+#endif
+
+// Resident blocks of one kernel instance on the current device (cached).
+template <class K>
+int resident_blocks(K kernel, int threads) {
+    static std::mutex mu;
+    static std::map<int, int> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 1024;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(dev);
+    if (it != cache.end()) return it->second;
+    int cus = 0, per = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess ||
+        cus < 1 || per < 1)
+        return 1024;
+    cache[dev] = cus * per;
+    return cus * per;
+}
+
+hipError_t run_fft2(float *g, const AxisPlan &pl, const AxisPass &ap, int nc, hipStream_t s,
+                    const long long *goffs, int ngrids) {
+#define PGALF_FFT2(RA, RB, NL)                                                                 \
+    {                                                                                          \
+        Fft2Groups gr;                                                                         \
+        gr.nbx = (ap.nb + 2 * (NL) - 1) / (2 * (NL));                                          \
+        gr.nc = nc;                                                                            \
+        const long long ng = (long long)gr.nbx * nc * ngrids;                                 \
+        if (ng >= (1LL << 31)) return hipErrorInvalidValue;                                    \
+        gr.ngroups = (unsigned)ng;                                                             \
+        constexpr int kThreads = (NL) * (RB);                                                  \
+        auto kern = ap.contiguous ? smooth_fft2_k<RA, RB, NL, true>                            \
+                                  : smooth_fft2_k<RA, RB, NL, false>;                          \
+        long long blocks = resident_blocks(kern, kThreads);                                    \
+        if (blocks > (long long)gr.ngroups) blocks = gr.ngroups;                               \
+        kern<<<(unsigned)blocks, kThreads, 0, s>>>(g, ap, pl.tw, pl.H, goffs, gr);             \
+    }
+    switch (pl.L) {
+    case 64: PGALF_FFT2(8, 8, 32) break;
+    case 128: PGALF_FFT2(8, 16, 16) break;
+    case 256: PGALF_FFT2(16, 16, PGALF_FFT2_NL256) break;
+    default: return hipErrorInvalidValue;
+    }
+#undef PGALF_FFT2
+    return hipGetLastError();
+}
+
 hipError_t run_pass(float *g, const AxisPlan &pl, long long sa, int nb, long long sb, int nc,
-                    long long sc, int contiguous, int zero_pads, int padw, hipStream_t s,
-                    const long long *goffs = nullptr, int ngrids = 1) {
-    // direct convolution: strided passes up to kDirectMax, contiguous rows
-    // up to 32 (longer rows need ~255 VGPRs there: the FFT pass is faster)
+                    long long sc, int contiguous, int zero_pads, int padw, long long gcells,
+                    hipStream_t s, const long long *goffs = nullptr, int ngrids = 1) {
+    const bool fft2 = PGALF_SMOOTH_FFT2 && pl.L == pl.n &&
+                      (pl.n == 64 || pl.n == 128 || pl.n == 256) && gcells > 0 &&
+                      gcells < (1LL << 29);
+    // direct convolution: strided passes up to PGALF_DIRECT_STRIDED_MAX,
+    // contiguous rows up to 32 (longer rows need ~255 VGPRs there)
+    const bool direct = pl.direct && (sa == 1 ? pl.n <= 32
+                                              : pl.n <= PGALF_DIRECT_STRIDED_MAX || !fft2);
+    if (fft2 && !direct) {
+        AxisPass ap{};
+        ap.n = pl.n;
+        ap.L = pl.L;
+        ap.sa = sa;
+        ap.nb = nb;
+        ap.sb = sb;
+        ap.sc = sc;
+        ap.contiguous = contiguous;
+        ap.zero_pads = zero_pads;
+        ap.padw = padw;
+        // grid offsets and slow strides are even (mx is), so line pairs
+        // starting at even x are 8-byte aligned when the grid is
+        ap.pairs = !contiguous && sb == 1 && (sc % 2 == 0) &&
+                   ((reinterpret_cast<unsigned long long>(g) & 7ull) == 0);
+        ap.gcells = gcells;
+        return run_fft2(g, pl, ap, nc, s, goffs, ngrids);
+    }
     if (pl.direct && !(sa == 1 && pl.n > 32)) {
         DirectPass dp;
         dp.na = nb;
@@ -466,7 +825,7 @@ hipError_t run_pass(float *g, const AxisPlan &pl, long long sa, int nb, long lon
         }
         return hipGetLastError();
     }
-    AxisPass ap;
+    AxisPass ap{};
     ap.n = pl.n;
     ap.L = pl.L;
     ap.sa = sa;
@@ -515,12 +874,13 @@ hipError_t smooth_passes(float *den, int nx, int ny, int nz, double expfac, hipS
     PGALF_HIP_TRY(get_plan(ny, expfac, py));
     PGALF_HIP_TRY(get_plan(nz, expfac, pz));
     const long long mx = 2LL * (nx / 2 + 1);
+    const long long gc = mx * ny * nz;
     // x: lines are rows (contiguous), batch over ny*nz rows
-    PGALF_HIP_TRY(run_pass(den, px, 1, ny * nz, mx, 1, 0, 1, 1, (int)mx, s, goffs, ngrids));
+    PGALF_HIP_TRY(run_pass(den, px, 1, ny * nz, mx, 1, 0, 1, 1, (int)mx, gc, s, goffs, ngrids));
     // y: lines along y (stride mx), batch over x (stride 1), one z per block row
-    PGALF_HIP_TRY(run_pass(den, py, mx, nx, 1, nz, mx * ny, 0, 0, 0, s, goffs, ngrids));
+    PGALF_HIP_TRY(run_pass(den, py, mx, nx, 1, nz, mx * ny, 0, 0, 0, gc, s, goffs, ngrids));
     // z: lines along z (stride mx*ny), batch over x, one y per block row
-    PGALF_HIP_TRY(run_pass(den, pz, mx * ny, nx, 1, ny, mx, 0, 0, 0, s, goffs, ngrids));
+    PGALF_HIP_TRY(run_pass(den, pz, mx * ny, nx, 1, ny, mx, 0, 0, 0, gc, s, goffs, ngrids));
     return hipSuccess;
 }
 

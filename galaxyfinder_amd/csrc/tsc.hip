@@ -226,14 +226,14 @@ tsc_scatter_k(const P *x, const P *y, const P *z, const P *m, const int *type, i
     }
 }
 
-// Units per tile (>= 1 so every tile, including empty and pad-only ones,
-// is written) and whether the tile needs the multi-unit side buffer.
+// Units per tile (none for a tile without bin entries: tsc_zero_k writes
+// those) and whether the tile needs the multi-unit side buffer.
 __global__ void tsc_units_k(const unsigned *tcount, long long ntiles, unsigned *nunits,
                             unsigned *multi) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntiles) return;
     unsigned c = tcount[t];
-    unsigned u = c == 0 ? 1u : (c + kChunk - 1) / kChunk;
+    unsigned u = (c + kChunk - 1) / kChunk;
     nunits[t] = u;
     multi[t] = u > 1 ? 1u : 0u;
 }
@@ -291,15 +291,6 @@ tsc_deposit_k(const P *x, const P *y, const P *z, const P *m, const HaloSeg *seg
         int ty = (int)((tl / tg.ntx) % tg.nty);
         int tz = (int)(tl / ((unsigned)tg.ntx * tg.nty));
         long long ox = (long long)tx * kT, oy = (long long)ty * kT, oz = (long long)tz * kT;
-        if (tcount[t] == 0) {
-            // empty tile (block-uniform): zeros, no accumulator, no barrier
-            for (int l = threadIdx.x; l < kTileNodes; l += kBlock) {
-                long long gx = ox + (l % kT), gy = oy + ((l / kT) % kT), gz = oz + l / (kT * kT);
-                if (gx < mx && gy < g.ny && gz < g.nz)
-                    hden[gx + mx * (gy + (long long)g.ny * gz)] = 0.0f;
-            }
-            continue;
-        }
         for (int i = threadIdx.x; i < kTileNodes; i += kBlock) acc[i] = 0ull;
         __syncthreads();
         unsigned e0 = tstart[t] + chunk * kChunk;
@@ -345,6 +336,36 @@ tsc_deposit_k(const P *x, const P *y, const P *z, const P *m, const HaloSeg *seg
                 if (acc[l]) atomicAdd(&dst[l], acc[l]);
         }
         __syncthreads();
+    }
+}
+
+// Tiles without bin entries (empty, or pad columns only): zeros.  One wave
+// per tile, four rows of 16 nodes per store instruction; the kernel is
+// light so many waves keep stores in flight.
+__global__ void __launch_bounds__(256)
+tsc_zero_k(const HaloSeg *segs, HaloSeg seg0, const unsigned *tile_halo,
+           const unsigned *tcount, long long ntiles, float *den) {
+    const int lane = threadIdx.x & 63;
+    const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+    for (long long t = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; t < ntiles;
+         t += nw) {
+        if (tcount[t] != 0) continue;
+        const int h = tile_halo ? (int)tile_halo[t] : 0;
+        const HaloSeg sg = seg_at(segs, seg0, h);
+        const Grid g = sg.g;
+        const TileGeom tg = tile_geom(g);
+        const long long mx = g.mx();
+        float *hden = den + sg.goff;
+        const unsigned tl = (unsigned)(t - sg.toff);
+        const long long ox = (long long)(tl % tg.ntx) * kT;
+        const long long oy = (long long)((tl / tg.ntx) % tg.nty) * kT;
+        const long long oz = (long long)(tl / ((unsigned)tg.ntx * tg.nty)) * kT;
+        const long long gx = ox + (lane & (kT - 1));
+        if (gx >= mx) continue;
+        for (int r = lane / kT; r < kT * kT; r += 64 / kT) {
+            const long long gy = oy + (r % kT), gz = oz + r / kT;
+            if (gy < g.ny && gz < g.nz) hden[gx + mx * (gy + (long long)g.ny * gz)] = 0.0f;
+        }
     }
 }
 
@@ -701,6 +722,11 @@ hipError_t deposit_impl(const P *x, const P *y, const P *z, const P *m, const in
     tsc_scale_k<<<(unsigned)((nhalo + 255) / 256), 256, 0, s>>>(nsel, maxm, nhalo, scale);
     const int ncu = cu_count(ws.device());
     fill_side_k<<<(unsigned)ncu * 4, 256, 0, s>>>(side, &misc[3]);
+    {
+        long long zb = (ntiles + 3) / 4;  // four tiles (waves) per block
+        if (zb > (long long)ncu * 32) zb = (long long)ncu * 32;
+        tsc_zero_k<<<(unsigned)zb, 256, 0, s>>>(segs, seg0, tile_halo, tcount, ntiles, den);
+    }
     tsc_deposit_k<P><<<(unsigned)ncu * 4, kBlock, 0, s>>>(
         x, y, z, m, segs, seg0, tile_halo, tstart, tcount, ustart, nunits, unit_tile, &misc[2],
         multi, bins, scale, den, side);

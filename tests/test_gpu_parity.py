@@ -85,7 +85,9 @@ def test_findden_bitexact_fixture():
     assert np.array_equal(out.view(np.int32), f["densph_p5"].view(np.int32))
 
 
-@pytest.mark.parametrize("shape", [(32, 32, 32), (37, 35, 32), (64, 48, 40), (21, 17, 24)])
+@pytest.mark.parametrize("shape", [(32, 32, 32), (37, 35, 32), (64, 48, 40), (21, 17, 24),
+                                   (64, 64, 64), (128, 37, 64), (37, 256, 128),
+                                   (256, 24, 128)])
 def test_smooth_vs_oracle(shape):
     nx, ny, nz = shape
     say("smooth %s" % (shape,))
