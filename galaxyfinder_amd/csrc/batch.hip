@@ -36,6 +36,7 @@ struct pgalf_batch {
     HaloSeg *d_segs = nullptr;
     PChunk *d_chunks = nullptr;
     unsigned *d_tile_halo = nullptr;
+    unsigned *d_slab_tile = nullptr;
     long long *d_goffs = nullptr;  // smoothing groups' grid offsets, group after group
     std::vector<SmoothGroup> groups;
     KnnBatchPlan kplan;
@@ -58,6 +59,7 @@ void free_batch(pgalf_batch *b) {
     if (b->d_segs) (void)hipFree(b->d_segs);
     if (b->d_chunks) (void)hipFree(b->d_chunks);
     if (b->d_tile_halo) (void)hipFree(b->d_tile_halo);
+    if (b->d_slab_tile) (void)hipFree(b->d_slab_tile);
     if (b->d_goffs) (void)hipFree(b->d_goffs);
     knn_batch_plan_free(b->kplan);
     delete b->ws;
@@ -243,8 +245,9 @@ extern "C" pgalf_batch_t *pgalf_batch_create(int nhalo, const long long *off,
     b->k = k;
     std::vector<HaloSeg> segs((size_t)nhalo);
     std::vector<PChunk> chunks;
-    std::vector<unsigned> tile_halo;
+    std::vector<unsigned> tile_halo, slab_tile;
     b->goff.assign((size_t)nhalo + 1, 0);
+    bool too_wide = false;
     long long toff = 0, max_tiles = 0;
     for (int h = 0; h < nhalo; h++) {
         const pgalf_grid_t &gg = grids[h];
@@ -263,6 +266,8 @@ extern "C" pgalf_batch_t *pgalf_batch_create(int nhalo, const long long *off,
         b->goff[h + 1] = b->goff[h] + sg.g.ncells();
         const long long nt = tsc_tile_count(sg.g);
         tile_halo.insert(tile_halo.end(), (size_t)nt, (unsigned)h);
+        tsc_slab_tiles(sg.g, toff, slab_tile);
+        too_wide |= tsc_tiles_x(sg.g) > tsc_max_tiles_x();
         toff += nt;
         max_tiles = std::max(max_tiles, nt);
         for (long long c = off[h]; c < off[h + 1]; c += kChunkParticles) {
@@ -294,9 +299,11 @@ extern "C" pgalf_batch_t *pgalf_batch_create(int nhalo, const long long *off,
         goffs.insert(goffs.end(), kv.second.begin(), kv.second.end());
         b->groups.push_back(grp);
     }
-    hipError_t e = upload(&b->d_segs, segs);
+    hipError_t e = too_wide ? hipErrorInvalidValue : hipSuccess;
+    if (e == hipSuccess) e = upload(&b->d_segs, segs);
     if (e == hipSuccess) e = upload(&b->d_chunks, chunks);
     if (e == hipSuccess) e = upload(&b->d_tile_halo, tile_halo);
+    if (e == hipSuccess) e = upload(&b->d_slab_tile, slab_tile);
     if (e == hipSuccess) e = upload(&b->d_goffs, goffs);
     for (size_t i = 0; i < b->groups.size(); i++) {
         b->groups[i].goffs = b->d_goffs + gfirst[i];
@@ -316,6 +323,8 @@ extern "C" pgalf_batch_t *pgalf_batch_create(int nhalo, const long long *off,
     b->bd.tile_halo = b->d_tile_halo;
     b->bd.ntiles = toff;
     b->bd.max_halo_tiles = max_tiles;
+    b->bd.slab_tile = b->d_slab_tile;
+    b->bd.nslabs = (long long)slab_tile.size();
     b->ws = new Workspace(b->device);
     ok();
     return b;

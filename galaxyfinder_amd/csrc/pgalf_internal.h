@@ -51,6 +51,8 @@ struct BatchDev {
     const unsigned *tile_halo = nullptr;  // [ntiles] halo of each deposit tile
     long long ntiles = 0;
     long long max_halo_tiles = 0;
+    const unsigned *slab_tile = nullptr;  // [nslabs] first tile of each (halo, ty, tz) slab
+    long long nslabs = 0;
 };
 
 // Device error bits raised by kernels (read back with the status word).
@@ -120,6 +122,11 @@ hipError_t tsc_readback_batch(const P *x, const P *y, const P *z, const BatchDev
                               hipStream_t s);
 // Deposit tiles of one grid (16^3 nodes over the padded mx x ny x nz box).
 long long tsc_tile_count(const Grid &g);
+// Appends the first tile id (tx = 0) of every (ty, tz) slab of a grid whose
+// tiles start at toff; tiles along x of a grid and their limit.
+void tsc_slab_tiles(const Grid &g, long long toff, std::vector<unsigned> &out);
+int tsc_tiles_x(const Grid &g);
+int tsc_max_tiles_x();
 // Adjoint of read-back and deposit (see tsc.hip): adds dL/dx, dL/dy, dL/dz,
 // dL/dm of L(rho_tsc) to acc = [gx | gy | gz | gm] (FP64, np each) from
 // gin = dL/drho_tsc, the forward smoothed grid F and the adjoint grid A.

@@ -12,9 +12,16 @@
 //               fixed-point tile (ds_add_u64).  Integer addition is
 //               associative, so the grid is bit-identical run to run
 //               whatever the order of the atomics; single-unit tiles are
-//               written once with plain coalesced stores, multi-unit tiles
-//               are summed in a global int64 side buffer and converted by a
-//               finalize pass.
+//               written once, whole, into a tile-major buffer (one aligned
+//               16 KB slot per tile with entries), multi-unit tiles are
+//               summed in a global int64 side buffer and converted into
+//               their slot by a finalize pass;
+//   5. de-tile: the padded row-major grid is written row by row from the
+//               slots, zeros for tiles without entries.  Every row is one
+//               run of consecutive stores: tile-shaped stores into the
+//               padded grid (rows of mx floats) leave partly written lines
+//               at every tile edge, which the memory system completes at a
+//               fraction of its streaming rate.
 //   The per-contribution value is the reference's FP32 product
 //   (wx*wy)*wz with its float/double conversions (tsc_omp2.c:303-382); the
 //   fixed-point scale 2^S is chosen from max|m| * count so no sum overflows.
@@ -226,8 +233,8 @@ tsc_scatter_k(const P *x, const P *y, const P *z, const P *m, const int *type, i
     }
 }
 
-// Units per tile (none for a tile without bin entries: tsc_zero_k writes
-// those) and whether the tile needs the multi-unit side buffer.
+// Units per tile (none for a tile without bin entries: the de-tiling pass
+// writes its zeros) and whether the tile needs the multi-unit side buffer.
 __global__ void tsc_units_k(const unsigned *tcount, long long ntiles, unsigned *nunits,
                             unsigned *multi) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -272,7 +279,7 @@ tsc_deposit_k(const P *x, const P *y, const P *z, const P *m, const HaloSeg *seg
               const unsigned *tcount, const unsigned *ustart, const unsigned *nunits,
               const unsigned *unit_tile, const unsigned *total_units,
               const unsigned *multi_slot, const unsigned *bins, const double *scale,
-              float *den, unsigned long long *side) {
+              const unsigned *tslot, float *tbuf, unsigned long long *side) {
     __shared__ unsigned long long acc[kTileNodes];
     const unsigned nunit_total = *total_units;
     for (unsigned u = blockIdx.x; u < nunit_total; u += gridDim.x) {
@@ -281,9 +288,7 @@ tsc_deposit_k(const P *x, const P *y, const P *z, const P *m, const HaloSeg *seg
         const HaloSeg sg = seg_at(segs, seg0, h);
         const Grid g = sg.g;
         const TileGeom tg = tile_geom(g);
-        const long long mx = g.mx();
         const double sc = scale[2 * h], inv = scale[2 * h + 1];
-        float *hden = den + sg.goff;
         unsigned chunk = u - ustart[t];
         unsigned nu = nunits[t];
         const unsigned tl = (unsigned)(t - sg.toff);
@@ -324,12 +329,10 @@ tsc_deposit_k(const P *x, const P *y, const P *z, const P *m, const HaloSeg *seg
         }
         __syncthreads();
         if (nu == 1) {
-            for (int l = threadIdx.x; l < kTileNodes; l += kBlock) {
-                long long gx = ox + (l % kT), gy = oy + ((l / kT) % kT), gz = oz + l / (kT * kT);
-                if (gx < mx && gy < g.ny && gz < g.nz)
-                    hden[gx + mx * (gy + (long long)g.ny * gz)] =
-                        (float)((double)(long long)acc[l] * inv);
-            }
+            // the whole tile, tile-major: aligned full-line stores
+            float *dst = tbuf + (size_t)tslot[t] * kTileNodes;
+            for (int l = threadIdx.x; l < kTileNodes; l += kBlock)
+                dst[l] = (float)((double)(long long)acc[l] * inv);
         } else {
             unsigned long long *dst = side + (size_t)multi_slot[t] * kTileNodes;
             for (int l = threadIdx.x; l < kTileNodes; l += kBlock)
@@ -339,59 +342,89 @@ tsc_deposit_k(const P *x, const P *y, const P *z, const P *m, const HaloSeg *seg
     }
 }
 
-// Tiles without bin entries (empty, or pad columns only): zeros.  One wave
-// per tile, four rows of 16 nodes per store instruction; the kernel is
-// light so many waves keep stores in flight.
-__global__ void __launch_bounds__(256)
-tsc_zero_k(const HaloSeg *segs, HaloSeg seg0, const unsigned *tile_halo,
-           const unsigned *tcount, long long ntiles, float *den) {
-    const int lane = threadIdx.x & 63;
-    const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
-    for (long long t = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; t < ntiles;
-         t += nw) {
-        if (tcount[t] != 0) continue;
-        const int h = tile_halo ? (int)tile_halo[t] : 0;
-        const HaloSeg sg = seg_at(segs, seg0, h);
-        const Grid g = sg.g;
-        const TileGeom tg = tile_geom(g);
-        const long long mx = g.mx();
-        float *hden = den + sg.goff;
-        const unsigned tl = (unsigned)(t - sg.toff);
-        const long long ox = (long long)(tl % tg.ntx) * kT;
-        const long long oy = (long long)((tl / tg.ntx) % tg.nty) * kT;
-        const long long oz = (long long)(tl / ((unsigned)tg.ntx * tg.nty)) * kT;
-        const long long gx = ox + (lane & (kT - 1));
-        if (gx >= mx) continue;
-        for (int r = lane / kT; r < kT * kT; r += 64 / kT) {
-            const long long gy = oy + (r % kT), gz = oz + r / kT;
-            if (gy < g.ny && gz < g.nz) hden[gx + mx * (gy + (long long)g.ny * gz)] = 0.0f;
-        }
-    }
-}
-
-// One block per (global) tile; only multi-unit tiles are written here.
-__global__ void tsc_finalize_k(const HaloSeg *segs, HaloSeg seg0, const unsigned *tile_halo,
-                               const unsigned *nunits, const unsigned *multi_slot,
-                               const unsigned long long *side, const double *scale, float *den) {
+// Multi-unit tiles: the side-buffer sums into the tile-major buffer.
+__global__ void tsc_finalize_k(const unsigned *tile_halo, const unsigned *nunits,
+                               const unsigned *multi_slot, const unsigned *tslot,
+                               const unsigned long long *side, const double *scale,
+                               float *tbuf) {
     const unsigned t = blockIdx.x;
     if (nunits[t] <= 1) return;
     const int h = tile_halo ? (int)tile_halo[t] : 0;
-    const HaloSeg sg = seg_at(segs, seg0, h);
-    const Grid g = sg.g;
-    const TileGeom tg = tile_geom(g);
-    const long long mx = g.mx();
     const double inv = scale[2 * h + 1];
-    float *hden = den + sg.goff;
-    const unsigned tl = (unsigned)(t - sg.toff);
-    int tx = (int)(tl % tg.ntx);
-    int ty = (int)((tl / tg.ntx) % tg.nty);
-    int tz = (int)(tl / ((unsigned)tg.ntx * tg.nty));
     const unsigned long long *src = side + (size_t)multi_slot[t] * kTileNodes;
-    for (int l = threadIdx.x; l < kTileNodes; l += blockDim.x) {
-        long long gx = (long long)tx * kT + (l % kT), gy = (long long)ty * kT + ((l / kT) % kT),
-                  gz = (long long)tz * kT + l / (kT * kT);
-        if (gx < mx && gy < g.ny && gz < g.nz)
-            hden[gx + mx * (gy + (long long)g.ny * gz)] = (float)((double)(long long)src[l] * inv);
+    float *dst = tbuf + (size_t)tslot[t] * kTileNodes;
+    for (int l = threadIdx.x; l < kTileNodes; l += blockDim.x)
+        dst[l] = (float)((double)(long long)src[l] * inv);
+}
+
+// Tiles with bin entries: slot[t] = 1 (ranked by the exclusive scan after).
+__global__ void tsc_slot_flags_k(const unsigned *tcount, long long ntiles, unsigned *slot) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ntiles) slot[t] = tcount[t] ? 1u : 0u;
+}
+
+// The padded row-major grid from the tile-major buffer.  A block takes four
+// layers z of one slab (the tile row ty, layer tz of one halo); each wave
+// writes the rows y in [kT ty, kT ty + kT) of its layer, which are one
+// contiguous run of floats, with consecutive stores (zeros where the tile
+// has no entries).  Tile-shaped stores into the padded grid (rows of mx
+// floats) would leave partly written lines at every tile edge, which the
+// memory system completes at a fraction of its streaming rate; the reads
+// here are aligned 16-float pieces of the tiles, four per lane in flight.
+constexpr int kMaxTilesX = 1024;  // tiles along x (mx <= 16384)
+__global__ void __launch_bounds__(256)
+tsc_detile_k(const HaloSeg *segs, HaloSeg seg0, const unsigned *tile_halo,
+             const unsigned *slab_tile, long long nslabs, const unsigned *tcount,
+             const unsigned *tslot, const float *tbuf, float *den) {
+    __shared__ int s_slot[kMaxTilesX];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int kU = 4;  // loads in flight per lane
+    for (long long bu = blockIdx.x; bu < nslabs * 4; bu += gridDim.x) {
+        const long long sl = bu >> 2;
+        const int h0 = slab_tile ? (int)tile_halo[slab_tile[sl]] : 0;
+        const HaloSeg sg = seg_at(segs, seg0, h0);
+        const Grid g = sg.g;
+        const TileGeom tg = tile_geom(g);
+        const long long mx = g.mx();
+        const long long t0 = slab_tile ? (long long)slab_tile[sl] : sl * tg.ntx;
+        const long long tl = t0 - sg.toff;
+        const long long ty = (tl / tg.ntx) % tg.nty, tz = tl / ((long long)tg.ntx * tg.nty);
+        __syncthreads();  // the previous unit's reads of s_slot are done
+        for (int i = threadIdx.x; i < tg.ntx; i += blockDim.x)
+            s_slot[i] = tcount[t0 + i] ? (int)tslot[t0 + i] : -1;
+        __syncthreads();
+        const int lz = (int)(bu & 3) * 4 + wave;
+        const long long gz = tz * kT + lz;
+        if (gz >= g.nz) continue;
+        const long long nrows = g.ny - ty * kT < kT ? g.ny - ty * kT : kT;
+        float *base = den + sg.goff + mx * (ty * kT + (long long)g.ny * gz);
+        const long long n = nrows * mx;
+        // element e = lane + 64 i of the run: row ly, column x (tracked)
+        long long x = lane % mx;
+        int ly = (int)(lane / mx);
+        for (long long e0 = 0; e0 < n; e0 += 64 * kU) {
+            float v[kU];
+#pragma unroll
+            for (int j = 0; j < kU; j++) {
+                const long long e = e0 + lane + 64 * j;
+                v[j] = 0.0f;
+                if (e < n) {
+                    const int slot = s_slot[x / kT];
+                    if (slot >= 0)
+                        v[j] = tbuf[(size_t)slot * kTileNodes + (x % kT) + kT * ly + kT * kT * lz];
+                }
+                x += 64;
+                while (x >= mx) {
+                    x -= mx;
+                    ly++;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kU; j++) {
+                const long long e = e0 + lane + 64 * j;
+                if (e < n) base[e] = v[j];
+            }
+        }
     }
 }
 
@@ -672,8 +705,9 @@ template <class P>
 hipError_t deposit_impl(const P *x, const P *y, const P *z, const P *m, const int *type,
                         int ptype, int nhalo, const HaloSeg *segs, const HaloSeg &seg0,
                         const PChunk *chunks, long long nchunks, const unsigned *tile_halo,
-                        long long ntiles, long long max_halo_tiles, long long np, float *den,
-                        Workspace &ws, hipStream_t s) {
+                        long long ntiles, long long max_halo_tiles, const unsigned *slab_tile,
+                        long long nslabs, long long np, float *den, Workspace &ws,
+                        hipStream_t s) {
     int use_lds = max_halo_tiles <= kLdsHistMax ? 1 : 0;
     unsigned *tcount = ws.alloc_n<unsigned>(ntiles);
     unsigned *tcursor = ws.alloc_n<unsigned>(ntiles);
@@ -722,22 +756,39 @@ hipError_t deposit_impl(const P *x, const P *y, const P *z, const P *m, const in
     tsc_scale_k<<<(unsigned)((nhalo + 255) / 256), 256, 0, s>>>(nsel, maxm, nhalo, scale);
     const int ncu = cu_count(ws.device());
     fill_side_k<<<(unsigned)ncu * 4, 256, 0, s>>>(side, &misc[3]);
-    {
-        long long zb = (ntiles + 3) / 4;  // four tiles (waves) per block
-        if (zb > (long long)ncu * 32) zb = (long long)ncu * 32;
-        tsc_zero_k<<<(unsigned)zb, 256, 0, s>>>(segs, seg0, tile_halo, tcount, ntiles, den);
-    }
+    // tile-major buffer: one kTileNodes slot per tile with entries (at most
+    // one per bin entry)
+    unsigned *tslot = ws.alloc_n<unsigned>(ntiles);
+    const long long ncap = std::max(1LL, std::min(ntiles, 8 * np));
+    float *tbuf = ws.alloc_n<float>((size_t)ncap * kTileNodes);
+    if (!tslot || !tbuf) return hipErrorOutOfMemory;
+    tsc_slot_flags_k<<<tb, 256, 0, s>>>(tcount, ntiles, tslot);
+    PGALF_HIP_TRY(exclusive_scan_u32(tslot, tslot, ntiles, &misc[4], ws, s));
     tsc_deposit_k<P><<<(unsigned)ncu * 4, kBlock, 0, s>>>(
         x, y, z, m, segs, seg0, tile_halo, tstart, tcount, ustart, nunits, unit_tile, &misc[2],
-        multi, bins, scale, den, side);
-    tsc_finalize_k<<<(unsigned)ntiles, 256, 0, s>>>(segs, seg0, tile_halo, nunits, multi, side,
-                                                    scale, den);
+        multi, bins, scale, tslot, tbuf, side);
+    tsc_finalize_k<<<(unsigned)ntiles, 256, 0, s>>>(tile_halo, nunits, multi, tslot, side, scale,
+                                                    tbuf);
+    long long db = 4 * nslabs < (long long)ncu * 8 ? 4 * nslabs : (long long)ncu * 8;
+    if (db > 0)
+        tsc_detile_k<<<(unsigned)db, 256, 0, s>>>(segs, seg0, tile_halo, slab_tile, nslabs, tcount,
+                                                  tslot, tbuf, den);
     return hipGetLastError();
 }
 
 }  // namespace
 
 long long tsc_tile_count(const Grid &g) { return tile_geom(g).ntiles(); }
+
+void tsc_slab_tiles(const Grid &g, long long toff, std::vector<unsigned> &out) {
+    const TileGeom tg = tile_geom(g);
+    for (int tz = 0; tz < tg.ntz; tz++)
+        for (int ty = 0; ty < tg.nty; ty++)
+            out.push_back((unsigned)(toff + (long long)tg.ntx * (ty + (long long)tg.nty * tz)));
+}
+
+int tsc_tiles_x(const Grid &g) { return tile_geom(g).ntx; }
+int tsc_max_tiles_x() { return kMaxTilesX; }
 
 template <class P>
 hipError_t tsc_deposit(const P *x, const P *y, const P *z, const P *m, const int *type,
@@ -746,8 +797,10 @@ hipError_t tsc_deposit(const P *x, const P *y, const P *z, const P *m, const int
     const HaloSeg seg0 = single_seg(g, np);
     const long long ntiles = tile_geom(g).ntiles();
     const long long nchunks = (np + kPerBlock - 1) / kPerBlock;
+    const TileGeom tg = tile_geom(g);
+    if (tg.ntx > kMaxTilesX) return hipErrorInvalidValue;
     return deposit_impl<P>(x, y, z, m, type, ptype, 1, nullptr, seg0, nullptr, nchunks, nullptr,
-                           ntiles, ntiles, np, den, ws, s);
+                           ntiles, ntiles, nullptr, (long long)tg.nty * tg.ntz, np, den, ws, s);
 }
 
 template <class P>
@@ -756,7 +809,8 @@ hipError_t tsc_deposit_batch(const P *x, const P *y, const P *z, const P *m, con
     static_assert(kPerBlock == kChunkParticles, "batch chunks are deposit blocks");
     const HaloSeg seg0{};
     return deposit_impl<P>(x, y, z, m, nullptr, PGALF_TYPE_ALL, b.nhalo, b.segs, seg0, b.chunks,
-                           b.nchunks, b.tile_halo, b.ntiles, b.max_halo_tiles, b.np, den, ws, s);
+                           b.nchunks, b.tile_halo, b.ntiles, b.max_halo_tiles, b.slab_tile,
+                           b.nslabs, b.np, den, ws, s);
 }
 
 template <class P>
