@@ -55,7 +55,10 @@ constexpr int kBuf = 8;      // merge chunk (sorted, then merged into the top-32
 #define PGALF_KNN_BATCH 2
 #endif
 constexpr int kBufCap = PGALF_KNN_BUFCAP;  // per-lane LDS append buffer (sorted-candidate indices)
-constexpr int kChunkC = 8;   // candidates between overflow checks
+#ifndef PGALF_KNN_CHUNK
+#define PGALF_KNN_CHUNK 8
+#endif
+constexpr int kChunkC = PGALF_KNN_CHUNK;  // candidates between overflow checks
 constexpr int kMaxLevels = 24;
 constexpr int kB = 8;        // candidates per bucket
 constexpr int kG = 8;        // lanes per query group (one candidate of a bucket each)
@@ -673,7 +676,7 @@ __device__ inline bool union_pass(const Box &b, int grp, const double *s_qx, con
 // of the group loads one candidate per bucket, so the global-memory latency
 // is paid once per batch -- then every lane runs over the staged candidates
 // and appends those closer than its k-th key to its buffer.  Wave uniform.
-__device__ inline void scan_list(QState &st, const int *list, int count, bool valid, int k,
+__device__ inline void scan_list(QState &st, int *list, int count, bool valid, int k,
                                  double px, double py, double pz, const double *cx,
                                  const double *cy, const double *cz, const unsigned *cid,
                                  long long nc, const Box *leaves, Stage &sg,
@@ -681,23 +684,38 @@ __device__ inline void scan_list(QState &st, const int *list, int count, bool va
                                  double *s_qz, double *s_qk, int lane, int grp, int gl,
                                  unsigned long long *dbg) {
     const unsigned long long t_scan = dbg_clock(dbg);
-    const int nmax = wave_max_i(count);
-    for (int i0 = 0; i0 < nmax; i0 += kBatch) {
-        // the batch's candidates are loaded together with the boxes of its
-        // buckets (one memory latency per batch); buckets that no lane's
-        // current ball reaches (group lane t tests bucket i0 + t) are then
-        // staged as empty
-        const double nan = __longlong_as_double(0x7ff8000000000000ll);
+    const double nan = __longlong_as_double(0x7ff8000000000000ll);
+    int ipos = 0;  // the group's read position in its list
+    const int nit = wave_max_i((count + kBatch - 1) / kBatch);  // a bound on the batches
+    for (int it = 0; it < nit; it++) {
+        if (__ballot(ipos < count) == 0ull) break;
+        // the batch: the first kBatch entries of the window [ipos, ipos + kG)
+        // that some lane's current ball reaches (group lane t tests entry
+        // ipos + t); the window then advances past the last one taken, or
+        // past the whole window when fewer pass -- entries that no ball
+        // reaches any more are dropped before their candidates are loaded
+        const int wend = ipos + kG < count ? ipos + kG : count;
+        int ent = 0;
+        bool pass = false;
+        if (ipos + gl < count) {
+            ent = list[ipos + gl];
+            pass = union_pass(leaves[ent & kBucketMask], grp, s_qx, s_qy, s_qz, s_qk);
+        }
+        const unsigned mm = (unsigned)((__ballot(pass) >> (grp * kG)) & kGMask);
+        const int npass = __popc(mm);
+        // passing entries packed, in order, at the end of the window; the
+        // batch takes the first kBatch of them and the next window starts
+        // right after (every lane has read its entry before any writes)
+        const int first = wend - npass;
+        if (pass) list[first + __popc(mm & ((1u << gl) - 1u))] = ent;
+        const int nsel = npass < kBatch ? npass : kBatch;
+        if (ipos < count) ipos = first + nsel;
         double vx[kBatch], vy[kBatch], vz[kBatch];
         unsigned ve[kBatch], vid[kBatch];
 #pragma unroll
         for (int t = 0; t < kBatch; t++) {
-            const int i = i0 + t;
-            long long e = -1;
-            if (i < count) {
-                e = (long long)(list[i] & kBucketMask) * kB + gl;
-                if (e >= nc) e = -1;
-            }
+            long long e = t < nsel ? (long long)(list[first + t] & kBucketMask) * kB + gl : -1;
+            if (e >= nc) e = -1;
             vx[t] = nan;
             vy[t] = 0.0;
             vz[t] = 0.0;
@@ -711,21 +729,18 @@ __device__ inline void scan_list(QState &st, const int *list, int count, bool va
                 vid[t] = cid[e];
             }
         }
-        bool bpass = false;
-        if (gl < kBatch && i0 + gl < count)
-            bpass = union_pass(leaves[list[i0 + gl] & kBucketMask], grp, s_qx, s_qy, s_qz, s_qk);
-        const unsigned bm = (unsigned)((__ballot(bpass) >> (grp * kG)) & kGMask);
-#pragma unroll
-        for (int t = 0; t < kBatch; t++) {
-            const int slot = t * kB + gl;
-            if ((bm >> t) & 1u)
-                stage_put(sg, grp, slot, vx[t], vy[t], vz[t], ve[t], vid[t]);
-            else
-                stage_put(sg, grp, slot, nan, 0.0, 0.0, 0xffffffffu, 0xffffffffu);
+        const int wsel = wave_max_i(nsel);
+        if (dbg && gl == 0) {
+            // staged buckets that some lane's ball reaches, and bucket slots scanned
+            atomicAdd(&dbg[3], (unsigned long long)nsel);
+            atomicAdd(&dbg[7], (unsigned long long)wsel);
         }
+#pragma unroll
+        for (int t = 0; t < kBatch; t++)
+            stage_put(sg, grp, t * kB + gl, vx[t], vy[t], vz[t], ve[t], vid[t]);
         if (dbg && lane == 0) atomicAdd(&dbg[0], 1ull);  // scan batches (wave)
         __syncthreads();
-        const int nslots = min(kBatch, nmax - i0) * kB;
+        const int nslots = wsel * kB;
         const uint4 *su = sg.u + grp * kStageStride;
         for (int h = 0; h < nslots; h += kChunkC) {
             // branch-free: every candidate's index is written at the lane's
@@ -1138,9 +1153,11 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
         double w = d[5] ? (double)d[5] : 1.0;
         fprintf(stderr, "[pgalf] knn: np=%lld nc=%lld buckets=%lld levels=%d | per "
                         "wave: scan batches %.1f, mid-scan merges %.1f, walk rounds %.1f, "
-                        "list scans %.1f, end merges %.1f | kcycles per wave: seed %.1f, walk %.1f, "
+                        "list scans %.1f, end merges %.1f, passing/scanned buckets per group "
+                        "%.1f/%.1f | kcycles per wave: seed %.1f, walk %.1f, "
                         "scan %.1f (merges %.1f), total %.1f\n",
                 np, nc, nb, h0.nlev, d[0] / w, d[1] / w, d[2] / w, d[4] / w, d[6] / w,
+                d[3] / w / kGroups, d[7] / w / kGroups,
                 d[8] / w / 1e3, d[9] / w / 1e3, d[10] / w / 1e3, d[11] / w / 1e3,
                 d[12] / w / 1e3);
     }
