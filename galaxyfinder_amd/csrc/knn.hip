@@ -42,7 +42,10 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kTop = 32;     // list capacity (k <= 32, MAX_NUM_NEAR tree.h:20)
-constexpr int kBuf = 8;      // merge chunk (sorted, then merged into the top-32)
+#ifndef PGALF_KNN_MERGE
+#define PGALF_KNN_MERGE 16
+#endif
+constexpr int kBuf = PGALF_KNN_MERGE;  // merge chunk (sorted, then merged into the top-32)
 // Capacities below can be set at compile time (-DPGALF_KNN_...): they trade
 // LDS per wavefront (occupancy) against merge and scan frequency.
 #ifndef PGALF_KNN_BUFCAP
