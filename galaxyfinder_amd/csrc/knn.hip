@@ -91,7 +91,8 @@ static_assert(kG == kB, "a scan batch loads one candidate of each bucket per gro
 static_assert(kG % kFan == 0 && kPar >= 1, "a walk round expands whole parents");
 static_assert(kTop % kB == 0, "the direct seed fill takes whole buckets");
 static_assert(kPar == 1, "the depth-first walk expands one parent per round");
-static_assert(kBufCap % 8 == 0 && kBufCap >= 24, "buffer holds whole merge chunks");
+static_assert(kBufCap % kBuf == 0 && kBufCap % kChunkC == 0 && kBufCap >= 24,
+              "buffer holds whole merge chunks");
 static_assert(kBatch >= 1 && kBatch <= kG, "one group lane tests each bucket of a batch");
 static_assert(kFCap >= 32 + 8 && kFCap % 8 == 0, "list must take the first scan and a round");
 static_assert(kSeedSpan >= kSeedFill && kFirstScan + kG <= kFCap, "seed window and first scan");
@@ -996,8 +997,12 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
             if (dbg && lane == 0) atomicAdd(&dbg[2], 1ull);  // walk rounds
             if (cl == 0) {
                 int pos = lcnt + __popc(m & ((1u << gl) - 1u));
-                // distance class: gap from the group centre in R/4 steps
-                int ring = (int)fmin((double)(kRings - 1), 4.0 * sqrt(cg / fmax(R2, 1e-300)));
+                // distance class: gap from the group centre in R/4 steps,
+                // floor(4 sqrt(cg / R2)) capped at kRings - 1, by comparisons
+                const double c16 = 16.0 * cg;
+                int ring = 0;
+#pragma unroll
+                for (int j = 1; j < kRings; j++) ring += c16 >= (double)(j * j) * R2 ? 1 : 0;
                 if (pass) list[pos] = (int)child | (ring << kRingShift);
                 lcnt += npass;
             } else {

@@ -363,9 +363,9 @@ __global__ void tsc_slot_flags_k(const unsigned *tcount, long long ntiles, unsig
     if (t < ntiles) slot[t] = tcount[t] ? 1u : 0u;
 }
 
-// The padded row-major grid from the tile-major buffer.  A block takes four
-// layers z of one slab (the tile row ty, layer tz of one halo); each wave
-// writes the rows y in [kT ty, kT ty + kT) of its layer, which are one
+// The padded row-major grid from the tile-major buffer.  A block takes one
+// slab (the tile row ty, layer tz of one halo); each wave writes, for four of
+// its kT layers z, the rows y in [kT ty, kT ty + kT), which are one
 // contiguous run of floats, with consecutive stores (zeros where the tile
 // has no entries).  Tile-shaped stores into the padded grid (rows of mx
 // floats) would leave partly written lines at every tile edge, which the
@@ -379,8 +379,7 @@ tsc_detile_k(const HaloSeg *segs, HaloSeg seg0, const unsigned *tile_halo,
     __shared__ int s_slot[kMaxTilesX];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     constexpr int kU = 4;  // loads in flight per lane
-    for (long long bu = blockIdx.x; bu < nslabs * 4; bu += gridDim.x) {
-        const long long sl = bu >> 2;
+    for (long long sl = blockIdx.x; sl < nslabs; sl += gridDim.x) {
         const int h0 = slab_tile ? (int)tile_halo[slab_tile[sl]] : 0;
         const HaloSeg sg = seg_at(segs, seg0, h0);
         const Grid g = sg.g;
@@ -393,36 +392,38 @@ tsc_detile_k(const HaloSeg *segs, HaloSeg seg0, const unsigned *tile_halo,
         for (int i = threadIdx.x; i < tg.ntx; i += blockDim.x)
             s_slot[i] = tcount[t0 + i] ? (int)tslot[t0 + i] : -1;
         __syncthreads();
-        const int lz = (int)(bu & 3) * 4 + wave;
-        const long long gz = tz * kT + lz;
-        if (gz >= g.nz) continue;
         const long long nrows = g.ny - ty * kT < kT ? g.ny - ty * kT : kT;
-        float *base = den + sg.goff + mx * (ty * kT + (long long)g.ny * gz);
         const long long n = nrows * mx;
-        // element e = lane + 64 i of the run: row ly, column x (tracked)
-        long long x = lane % mx;
-        int ly = (int)(lane / mx);
-        for (long long e0 = 0; e0 < n; e0 += 64 * kU) {
-            float v[kU];
+        for (int lz = wave; lz < kT; lz += 4) {
+            const long long gz = tz * kT + lz;
+            if (gz >= g.nz) break;
+            float *base = den + sg.goff + mx * (ty * kT + (long long)g.ny * gz);
+            // element e = lane + 64 i of the run: row ly, column x (tracked)
+            long long x = lane % mx;
+            int ly = (int)(lane / mx);
+            for (long long e0 = 0; e0 < n; e0 += 64 * kU) {
+                float v[kU];
 #pragma unroll
-            for (int j = 0; j < kU; j++) {
-                const long long e = e0 + lane + 64 * j;
-                v[j] = 0.0f;
-                if (e < n) {
-                    const int slot = s_slot[x / kT];
-                    if (slot >= 0)
-                        v[j] = tbuf[(size_t)slot * kTileNodes + (x % kT) + kT * ly + kT * kT * lz];
+                for (int j = 0; j < kU; j++) {
+                    const long long e = e0 + lane + 64 * j;
+                    v[j] = 0.0f;
+                    if (e < n) {
+                        const int slot = s_slot[x / kT];
+                        if (slot >= 0)
+                            v[j] = tbuf[(size_t)slot * kTileNodes + (x % kT) + kT * ly +
+                                        kT * kT * lz];
+                    }
+                    x += 64;
+                    while (x >= mx) {
+                        x -= mx;
+                        ly++;
+                    }
                 }
-                x += 64;
-                while (x >= mx) {
-                    x -= mx;
-                    ly++;
-                }
-            }
 #pragma unroll
-            for (int j = 0; j < kU; j++) {
-                const long long e = e0 + lane + 64 * j;
-                if (e < n) base[e] = v[j];
+                for (int j = 0; j < kU; j++) {
+                    const long long e = e0 + lane + 64 * j;
+                    if (e < n) base[e] = v[j];
+                }
             }
         }
     }
@@ -769,7 +770,7 @@ hipError_t deposit_impl(const P *x, const P *y, const P *z, const P *m, const in
         multi, bins, scale, tslot, tbuf, side);
     tsc_finalize_k<<<(unsigned)ntiles, 256, 0, s>>>(tile_halo, nunits, multi, tslot, side, scale,
                                                     tbuf);
-    long long db = 4 * nslabs < (long long)ncu * 8 ? 4 * nslabs : (long long)ncu * 8;
+    long long db = nslabs < (long long)ncu * 8 ? nslabs : (long long)ncu * 8;
     if (db > 0)
         tsc_detile_k<<<(unsigned)db, 256, 0, s>>>(segs, seg0, tile_halo, slab_tile, nslabs, tcount,
                                                   tslot, tbuf, den);
