@@ -37,9 +37,10 @@ hipError_t Workspace::begin(hipStream_t s) {
         PGALF_HIP_TRY(hipMalloc(&status_, sizeof(unsigned)));
         PGALF_HIP_TRY(hipMemsetAsync(status_, 0, sizeof(unsigned), s));
     }
-    size_t need = 0;
-    for (auto &c : chunks_) need += c.used;
-    if (need > high_water_) high_water_ = need;
+    if (peak_ > high_water_) high_water_ = peak_;
+    peak_ = 0;
+    // several chunks (the last call grew) or one too small: one chunk of the
+    // high-water size, allocated after the old ones are freed
     if (chunks_.size() > 1 || (!chunks_.empty() && chunks_[0].cap < high_water_)) {
         PGALF_HIP_TRY(hipStreamSynchronize(s));
         for (auto &c : chunks_) hipFree(c.ptr);
@@ -47,20 +48,36 @@ hipError_t Workspace::begin(hipStream_t s) {
         PGALF_HIP_TRY(add_chunk(high_water_ + (high_water_ >> 3) + (1 << 20)));
     }
     for (auto &c : chunks_) c.used = 0;
+    cur_ = 0;
     return hipSuccess;
 }
 
 void *Workspace::alloc(size_t bytes) {
     bytes = (bytes + 255) & ~(size_t)255;
     if (bytes == 0) bytes = 256;
-    if (chunks_.empty() || chunks_.back().used + bytes > chunks_.back().cap) {
+    // the bump chunk, else a later one (emptied by a release), else a new one
+    while (!chunks_.empty() && chunks_[cur_].used + bytes > chunks_[cur_].cap &&
+           cur_ + 1 < chunks_.size())
+        cur_++;
+    if (chunks_.empty() || chunks_[cur_].used + bytes > chunks_[cur_].cap) {
         size_t cap = bytes < ((size_t)64 << 20) ? ((size_t)64 << 20) : bytes;
         if (add_chunk(cap) != hipSuccess) return nullptr;
+        cur_ = chunks_.size() - 1;
     }
-    Chunk &c = chunks_.back();
+    Chunk &c = chunks_[cur_];
     void *p = c.ptr + c.used;
     c.used += bytes;
+    size_t live = 0;
+    for (size_t i = 0; i <= cur_; i++) live += chunks_[i].used;
+    if (live > peak_) peak_ = live;
     return p;
+}
+
+void Workspace::release(const Mark &m) {
+    if (chunks_.empty()) return;
+    for (size_t i = m.chunk + 1; i < chunks_.size(); i++) chunks_[i].used = 0;
+    chunks_[m.chunk].used = m.used;
+    cur_ = m.chunk;
 }
 
 static std::mutex g_ws_mu;

@@ -1068,6 +1068,7 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
                int cand_ptype, int k, long long *nbr, float *rho, int rho_mode, Workspace &ws,
                hipStream_t s) {
     if (k < 1 || k > kTop || np < 1) return hipErrorInvalidValue;
+    WsScope scope(ws);
     timing_mark("knn_prep", s, false);
     const bool subset = type != nullptr && cand_ptype != PGALF_TYPE_ALL;
     // bounding box of every particle (queries include all candidates)
@@ -1202,6 +1203,7 @@ __device__ inline void m4_f(double q, double &f, double &fp) {
 constexpr int kBwdBlock = 256;
 constexpr int kBwdHalo = 512;
 constexpr int kBwdWin = kBwdBlock + 2 * kBwdHalo;
+constexpr int kBwdChunk = 8;  // neighbours gathered before their adds
 
 template <class P>
 __global__ void __launch_bounds__(kBwdBlock)
@@ -1233,26 +1235,57 @@ knn_backward_k(const P *x, const P *y, const P *z, const P *m, long long np,
         const double rK = sqrt(kx * kx + ky * ky + kz * kz), hh = 0.5 * rK;
         // g == 0: no terms; k-th neighbour at zero distance: no finite density
         if (g != 0.0 && hh > 0.0) {
-            const double h3 = M_PI * hh * hh * hh, h4 = h3 * hh;
+            // 1/h, 1/(pi h^3), 1/(pi h^4) once per particle: one square root
+            // and one division per neighbour
+            const double ih = 1.0 / hh, ih3 = 1.0 / (M_PI * hh * hh * hh), ih4 = ih3 * ih;
             double ox = 0.0, oy = 0.0, oz = 0.0, sh = 0.0;
-            for (int t = 0; t < k; t++) {
-                const long long j = li[t] + base;
-                const double dx = xi - (double)x[j], dy = yi - (double)y[j],
-                             dz = zi - (double)z[j];
-                const double r = sqrt(dx * dx + dy * dy + dz * dz), q = r / hh;
-                double f, fp;
-                m4_f(q, f, fp);
-                const double mt = weighted ? (double)m[j] : 1.0;
-                sh += mt * (-(3.0 * f + q * fp) / h4);
-                if (weighted && f != 0.0) add(j, 3, g * f / h3);
-                if (r > 0.0 && fp != 0.0) {
-                    const double c = g * mt * (fp / h4) / r;
-                    ox += c * dx;
-                    oy += c * dy;
-                    oz += c * dz;
-                    add(j, 0, -c * dx);
-                    add(j, 1, -c * dy);
-                    add(j, 2, -c * dz);
+            // neighbours in chunks: every gather of a chunk is issued before
+            // any of its atomics, so the loads overlap instead of waiting
+            // behind the previous neighbour's adds
+            for (int t0 = 0; t0 < k; t0 += kBwdChunk) {
+                long long jj[kBwdChunk];
+                double px[kBwdChunk], py[kBwdChunk], pz[kBwdChunk], pm[kBwdChunk];
+#pragma unroll
+                for (int u = 0; u < kBwdChunk; u++)
+                    jj[u] = t0 + u < k ? li[t0 + u] + base : i;
+#pragma unroll
+                for (int u = 0; u < kBwdChunk; u++) {
+                    px[u] = (double)x[jj[u]];
+                    py[u] = (double)y[jj[u]];
+                    pz[u] = (double)z[jj[u]];
+                    pm[u] = weighted ? (double)m[jj[u]] : 1.0;
+                }
+                double ax[kBwdChunk], ay[kBwdChunk], az[kBwdChunk], am[kBwdChunk];
+#pragma unroll
+                for (int u = 0; u < kBwdChunk; u++) {
+                    ax[u] = ay[u] = az[u] = am[u] = 0.0;
+                    if (t0 + u >= k) continue;
+                    const double dx = xi - px[u], dy = yi - py[u], dz = zi - pz[u];
+                    const double r = sqrt(dx * dx + dy * dy + dz * dz), q = r * ih;
+                    double f, fp;
+                    m4_f(q, f, fp);
+                    const double mt = pm[u];
+                    sh -= mt * (3.0 * f + q * fp) * ih4;
+                    if (weighted) am[u] = g * f * ih3;
+                    if (r > 0.0 && fp != 0.0) {
+                        const double c = g * mt * fp * ih4 / r;
+                        ox += c * dx;
+                        oy += c * dy;
+                        oz += c * dz;
+                        ax[u] = -c * dx;
+                        ay[u] = -c * dy;
+                        az[u] = -c * dz;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kBwdChunk; u++) {
+                    if (t0 + u >= k) continue;
+                    if (am[u] != 0.0) add(jj[u], 3, am[u]);
+                    if (ax[u] != 0.0 || ay[u] != 0.0 || az[u] != 0.0) {
+                        add(jj[u], 0, ax[u]);
+                        add(jj[u], 1, ay[u]);
+                        add(jj[u], 2, az[u]);
+                    }
                 }
             }
             const double c = g * sh * 0.5 / rK;
@@ -1301,6 +1334,7 @@ hipError_t knn_density_backward(const P *x, const P *y, const P *z, const P *m, 
                                 double *acc, Workspace &ws, hipStream_t s) {
     if (np <= 0) return hipSuccess;
     if (k < 1 || k > kTop) return hipErrorInvalidValue;
+    WsScope scope(ws);
     // Hilbert order of the particles (as the forward's preparation) and its inverse
     unsigned long long *mm = ws.alloc_n<unsigned long long>(6);
     unsigned *hk = ws.alloc_n<unsigned>(np), *hv = ws.alloc_n<unsigned>(np);
@@ -1466,6 +1500,7 @@ hipError_t knn_batch(const P *x, const P *y, const P *z, const P *m, const Batch
                      Workspace &ws, hipStream_t s) {
     const long long np = plan.np, npad = plan.npad;
     if (np != b.np || plan.nhalo != b.nhalo) return hipErrorInvalidValue;
+    WsScope scope(ws);
     timing_mark("knn_prep", s, false);
     const KHalo *halos = static_cast<const KHalo *>(plan.halos);
     unsigned *hv = nullptr, *k2 = nullptr, *halo_of = nullptr;
@@ -1502,6 +1537,7 @@ hipError_t knn_batch_backward(const P *x, const P *y, const P *z, const P *m, co
                               const float *gin, double *acc, Workspace &ws, hipStream_t s) {
     const long long np = plan.np;
     if (np != b.np || plan.nhalo != b.nhalo) return hipErrorInvalidValue;
+    WsScope scope(ws);
     unsigned *order = nullptr, *hsorted = nullptr, *halo_of = nullptr;
     PGALF_HIP_TRY(batch_order<P>(x, y, z, b, plan, ws, s, &order, &hsorted, &halo_of));
     unsigned *rank = ws.alloc_n<unsigned>(np);

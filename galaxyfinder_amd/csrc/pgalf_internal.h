@@ -66,21 +66,43 @@ enum : unsigned {
 // begin(), after the stream has drained).
 class Workspace {
   public:
+    // A bump position; release() returns the arena to it, so a stage's
+    // scratch is reused by the stages after it.  All users run on one stream
+    // (or one captured graph), so later kernels that reuse the memory run
+    // after the earlier ones have finished with it.
+    struct Mark {
+        size_t chunk, used;
+    };
     explicit Workspace(int device) : device_(device) {}
     ~Workspace();
     hipError_t begin(hipStream_t s);
     void *alloc(size_t bytes);
     template <class T> T *alloc_n(size_t n) { return static_cast<T *>(alloc(n * sizeof(T))); }
+    Mark mark() const { return chunks_.empty() ? Mark{0, 0} : Mark{cur_, chunks_[cur_].used}; }
+    void release(const Mark &m);
     unsigned *status() { return status_; }  // device word of DEV_ERR_* bits
     int device() const { return device_; }
 
   private:
     struct Chunk { char *ptr; size_t cap, used; };
     std::vector<Chunk> chunks_;
-    size_t high_water_ = 0;
+    size_t cur_ = 0;         // chunk of the bump position
+    size_t peak_ = 0;        // largest live total since begin()
+    size_t high_water_ = 0;  // largest peak of any call
     unsigned *status_ = nullptr;
     int device_;
     hipError_t add_chunk(size_t bytes);
+};
+
+// Releases a stage's workspace scratch when the stage returns (its kernels
+// are queued; later stages on the same stream reuse the memory after them).
+struct WsScope {
+    Workspace &ws;
+    Workspace::Mark m;
+    explicit WsScope(Workspace &w) : ws(w), m(w.mark()) {}
+    ~WsScope() { ws.release(m); }
+    WsScope(const WsScope &) = delete;
+    WsScope &operator=(const WsScope &) = delete;
 };
 
 Workspace &workspace_for_current_device();

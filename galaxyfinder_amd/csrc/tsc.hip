@@ -709,6 +709,7 @@ hipError_t deposit_impl(const P *x, const P *y, const P *z, const P *m, const in
                         long long ntiles, long long max_halo_tiles, const unsigned *slab_tile,
                         long long nslabs, long long np, float *den, Workspace &ws,
                         hipStream_t s) {
+    WsScope scope(ws);  // bins, tile-major buffer and side buffer end with the deposit
     int use_lds = max_halo_tiles <= kLdsHistMax ? 1 : 0;
     unsigned *tcount = ws.alloc_n<unsigned>(ntiles);
     unsigned *tcursor = ws.alloc_n<unsigned>(ntiles);
