@@ -1203,7 +1203,10 @@ __device__ inline void m4_f(double q, double &f, double &fp) {
 constexpr int kBwdBlock = 256;
 constexpr int kBwdHalo = 512;
 constexpr int kBwdWin = kBwdBlock + 2 * kBwdHalo;
-constexpr int kBwdChunk = 8;  // neighbours gathered before their adds
+#ifndef PGALF_KNN_BWD_CHUNK
+#define PGALF_KNN_BWD_CHUNK 8
+#endif
+constexpr int kBwdChunk = PGALF_KNN_BWD_CHUNK;  // neighbours gathered before their adds
 
 template <class P>
 __global__ void __launch_bounds__(kBwdBlock)
