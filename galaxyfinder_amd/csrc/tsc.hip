@@ -392,36 +392,39 @@ tsc_detile_k(const HaloSeg *segs, HaloSeg seg0, const unsigned *tile_halo,
         for (int i = threadIdx.x; i < tg.ntx; i += blockDim.x)
             s_slot[i] = tcount[t0 + i] ? (int)tslot[t0 + i] : -1;
         __syncthreads();
-        const long long nrows = g.ny - ty * kT < kT ? g.ny - ty * kT : kT;
-        const long long n = nrows * mx;
+        const int mxi = (int)mx;
+        const int nrows = (int)(g.ny - ty * kT < kT ? g.ny - ty * kT : kT);
+        const int n = nrows * mxi;  // floats of one layer's run (< 2^18)
+        const float rmx = 1.0f / (float)mxi;
         for (int lz = wave; lz < kT; lz += 4) {
             const long long gz = tz * kT + lz;
             if (gz >= g.nz) break;
             float *base = den + sg.goff + mx * (ty * kT + (long long)g.ny * gz);
-            // element e = lane + 64 i of the run: row ly, column x (tracked)
-            long long x = lane % mx;
-            int ly = (int)(lane / mx);
-            for (long long e0 = 0; e0 < n; e0 += 64 * kU) {
+            const int lzo = kT * kT * lz;
+            for (int e0 = 0; e0 < n; e0 += 64 * kU) {
                 float v[kU];
 #pragma unroll
                 for (int j = 0; j < kU; j++) {
-                    const long long e = e0 + lane + 64 * j;
-                    v[j] = 0.0f;
-                    if (e < n) {
-                        const int slot = s_slot[x / kT];
-                        if (slot >= 0)
-                            v[j] = tbuf[(size_t)slot * kTileNodes + (x % kT) + kT * ly +
-                                        kT * kT * lz];
-                    }
-                    x += 64;
-                    while (x >= mx) {
-                        x -= mx;
+                    // element e of the run: row ly = e / mx (float estimate,
+                    // exact after one correction), column x
+                    const int e = e0 + lane + 64 * j;
+                    int ly = (int)((float)e * rmx);
+                    int x = e - ly * mxi;
+                    if (x < 0) {
+                        x += mxi;
+                        ly--;
+                    } else if (x >= mxi) {
+                        x -= mxi;
                         ly++;
                     }
+                    const int slot = e < n ? s_slot[x >> 4] : -1;
+                    v[j] = slot >= 0 ? tbuf[(size_t)slot * kTileNodes +
+                                            (unsigned)((x & (kT - 1)) + kT * ly + lzo)]
+                                     : 0.0f;
                 }
 #pragma unroll
                 for (int j = 0; j < kU; j++) {
-                    const long long e = e0 + lane + 64 * j;
+                    const int e = e0 + lane + 64 * j;
                     if (e < n) base[e] = v[j];
                 }
             }
