@@ -134,43 +134,66 @@ __global__ void __launch_bounds__(kThreads) rs_hist_k(const unsigned *keys, long
     counts[(long long)threadIdx.x * nb + blockIdx.x] = hist[threadIdx.x];
 }
 
+// One LSD pass over the block's tile: every item is ranked within its digit
+// (stable: by round, wave and lane), placed in LDS in digit order, and the
+// tile is then written out digit run by digit run, so consecutive threads
+// store consecutive positions of one run.
 __global__ void __launch_bounds__(kThreads) rs_scatter_k(const unsigned *keys,
                                                          const unsigned *vals, long long n,
                                                          int shift, long long nb,
                                                          const unsigned *offsets,
                                                          unsigned *okeys, unsigned *ovals) {
+    __shared__ unsigned s_key[kTile], s_val[kTile];
     __shared__ unsigned gbase[256];
+    __shared__ unsigned lstart[256];
     __shared__ unsigned run[256];
     __shared__ unsigned wcnt[kWaves][256];
+    __shared__ unsigned wsum[kWaves];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    gbase[tid] = offsets[(long long)tid * nb + blockIdx.x];
+    const long long base = (long long)blockIdx.x * kTile;
+    const int tcount = (int)(n - base < kTile ? n - base : kTile);
+    // this block's count of digit tid from the scanned (digit-major) counts
+    const long long ci = (long long)tid * nb + blockIdx.x;
+    const unsigned gb = offsets[ci];
+    const unsigned cnt = (ci + 1 < 256 * nb ? offsets[ci + 1] : (unsigned)n) - gb;
+    gbase[tid] = gb;
+    // exclusive scan of the block's digit counts: local start of each run
+    unsigned incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) wsum[w] = incl;
     run[tid] = 0;
 #pragma unroll
     for (int i = 0; i < kWaves; i++) wcnt[i][tid] = 0;
     __syncthreads();
+    unsigned wbase = 0;
+    for (int i = 0; i < w; i++) wbase += wsum[i];
+    lstart[tid] = wbase + incl - cnt;
+    __syncthreads();
     const unsigned long long lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-    long long base = (long long)blockIdx.x * kTile;
     for (int r = 0; r < kItems; r++) {
-        long long i = base + r * kThreads + tid;
-        bool valid = i < n;
-        unsigned key = valid ? keys[i] : 0u;
-        unsigned val = valid ? vals[i] : 0u;
-        unsigned d = (key >> shift) & 255u;
+        const int li = r * kThreads + tid;
+        const bool valid = li < tcount;
+        const unsigned key = valid ? keys[base + li] : 0u;
+        const unsigned val = valid ? vals[base + li] : 0u;
+        const unsigned d = (key >> shift) & 255u;
         unsigned long long peers = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
-            unsigned long long bb = __ballot((d >> b) & 1u);
-            peers &= ((d >> b) & 1u) ? bb : ~bb;
+        for (int bt = 0; bt < 8; bt++) {
+            const unsigned long long bb = __ballot((d >> bt) & 1u);
+            peers &= ((d >> bt) & 1u) ? bb : ~bb;
         }
-        unsigned rank = __popcll(peers & lt_mask);
+        const unsigned rank = __popcll(peers & lt_mask);
         if (valid && rank == 0) wcnt[w][d] = __popcll(peers);
         __syncthreads();
         if (valid) {
-            unsigned off = run[d] + rank;
+            unsigned off = lstart[d] + run[d] + rank;
             for (int ww = 0; ww < w; ww++) off += wcnt[ww][d];
-            unsigned dst = gbase[d] + off;
-            okeys[dst] = key;
-            ovals[dst] = val;
+            s_key[off] = key;
+            s_val[off] = val;
         }
         __syncthreads();
         unsigned add = 0;
@@ -181,6 +204,13 @@ __global__ void __launch_bounds__(kThreads) rs_scatter_k(const unsigned *keys,
         }
         run[tid] += add;
         __syncthreads();
+    }
+    for (int i = tid; i < tcount; i += kThreads) {
+        const unsigned key = s_key[i];
+        const unsigned d = (key >> shift) & 255u;
+        const unsigned dst = gbase[d] + (unsigned)i - lstart[d];
+        okeys[dst] = key;
+        ovals[dst] = s_val[i];
     }
 }
 
