@@ -10,20 +10,24 @@
 // Structure:
 //   * queries and candidates are sorted by their 30-bit Hilbert index (10
 //     bits per axis over the bounding cube; stable radix sort, ties by
-//     index).  Hilbert order has no jumps, so 16 consecutive particles form a
+//     index).  Hilbert order has no jumps, so 8 consecutive particles form a
 //     compact bucket;
-//   * candidate buckets of 16 get FP64 bounding boxes and an implicit 8-ary
+//   * candidate buckets of 8 get FP64 bounding boxes and an implicit 8-ary
 //     tree of boxes (8 consecutive nodes per parent);
-//   * a wavefront holds four independent query groups of 16 lanes (16
-//     consecutive sorted queries, one per lane).  Each group first descends
-//     the tree greedily to the bucket nearest its box and scans that bucket
-//     and its two neighbours either side, which gives every lane a k-th
-//     distance; it then walks the tree breadth first, all 16 lanes testing
-//     children in parallel (two parents x eight children per round) against
-//     the group's largest k-th distance, and finally scans the surviving
-//     buckets.  Accepted candidates go to a per-lane LDS buffer; when any
-//     lane's buffer may overflow, all lanes merge it into their sorted top-32
-//     list held in registers (bitonic sort of 16 + bitonic merge).
+//   * a wavefront holds eight independent query groups of 8 lanes (8
+//     consecutive sorted queries -- one bucket -- one per lane).  Each group
+//     fills every lane's list from the 4 buckets around its own (a 32-key
+//     bitonic sort) and scans a window of seed buckets, which gives every
+//     lane a k-th distance.  It then walks the tree depth first, nearest
+//     child first: a round pops one node and its 8 lanes test the 8 children
+//     against the group's box and the union of the lanes' current balls.
+//     Passing buckets go to a list, ordered by distance class and scanned in
+//     batches: each batch takes the first two buckets of an 8-entry window
+//     that some lane's ball still reaches, stages their candidates in LDS,
+//     and every lane tests them against its own k-th key.  Accepted
+//     candidates go to a per-lane LDS buffer; when any lane's buffer may
+//     overflow, all lanes merge it into their sorted top-32 list held in
+//     registers, 16 keys at a time (bitonic sort of 16 + bitonic merge);
 //   * keys are (FP64 dist2, uint32 index) compared lexicographically, so the
 //     lists are exact and deterministic regardless of visiting order.
 // Box pruning is conservative: box gaps are formed with the same monotone
@@ -788,8 +792,8 @@ __device__ inline void scan_list(QState &st, int *list, int count, bool valid, i
 static_assert(sizeof(int) * kGroups * kFCap <= sizeof(Stage),
               "ring-sort scratch must fit in the staging area");
 
-// Stable counting sort of a group's bucket list by ring (top bits), 16 lanes
-// in parallel; tmp (>= kFCap ints of LDS) is scratch.  Group-uniform count.
+// Stable counting sort of a group's bucket list by ring (top bits), the
+// group's kG lanes in parallel; tmp (>= kFCap ints of LDS) is scratch.  Group-uniform count.
 __device__ inline void ring_sort(int *list, int count, int *tmp, int gshift, int gl) {
     const int per = (count + kG - 1) / kG;
     const int c0 = min(count, gl * per), c1 = min(count, c0 + per);

@@ -6,18 +6,24 @@
 // with k' the signed frequency, and transforms back with a 1/(nx ny nz)
 // scale.  Because the multiplier is a product of real, even, per-axis
 // factors, the operator is exactly three 1-D circular convolutions, one per
-// axis.  Each axis pass here reads the grid once and writes it once:
-// a workgroup loads a batch of lines into LDS (two real lines packed as one
-// complex line), runs a Stockham radix-4/2 FFT, multiplies by the real
-// spectral factor H, runs the inverse FFT and stores the lines back.
+// axis.  Each axis pass reads the grid once and writes it once, two real
+// lines packed as one complex line, with one of three kernels:
+//   * lines of 64, 128, 256: a four-step FFT with each thread's DFT-8/16 in
+//     FP64 registers and two LDS transposes per line (smooth_fft2_k);
+//   * short lines (multiples of 8 up to 56 strided, 32 as rows): direct
+//     circular convolution with the periodic kernel (smooth_direct_k);
+//   * other lengths: a workgroup loads a batch of lines into LDS, runs a
+//     Stockham radix-4/2 FFT, multiplies by the real spectral factor H, runs
+//     the inverse FFT and stores the lines back (smooth_axis_k).
+// FFT lengths:
 //   n_d a power of two:  L = n_d and H(k) = fd(k) / n_d.
 //   any other n_d:       L = pow2 >= 2 n_d - 1, the line is zero-padded and
 //                        H = DFT_L(h) / L with h the periodic kernel g_d laid
 //                        out circularly over [-(n_d-1), n_d-1]; the first n_d
 //                        outputs of the length-L linear convolution are the
 //                        length-n_d circular one.
-// The grid stays float32 in HBM; each line is transformed in FP64 in LDS
-// (FP64 vector FMA runs at the FP32 rate on this part and the passes are
+// The grid stays float32 in HBM; each line is transformed in FP64 (FP64
+// vector FMA runs at the FP32 rate on this part and the passes are
 // memory-bound), so the only FP32 roundings are the three stores: every
 // cell keeps ~1 ulp relative accuracy even far below the line's peak, which
 // an FP32 FFT (whose error scales with the line norm) cannot give.
