@@ -589,7 +589,21 @@ smooth_direct_x_k(float *grid, DirectPass dp, Taps taps, const long long *goffs)
     const int nrows = (int)(dp.nlines - row0 < BLK ? dp.nlines - row0 : BLK);
     float *base = grid + (goffs ? goffs[blockIdx.y] : 0LL) + row0 * W;
     const int tot = nrows * W;
-    for (int e = threadIdx.x; e < tot; e += BLK) tile[(e / W) * WS + e % W] = base[e];
+    // kLB independent loads per thread in flight before their LDS stores
+    constexpr int kLB = 12;
+    for (int e0 = threadIdx.x; e0 < tot; e0 += BLK * kLB) {
+        float v[kLB];
+#pragma unroll
+        for (int u = 0; u < kLB; u++) {
+            const int e = e0 + u * BLK;
+            v[u] = e < tot ? base[e] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kLB; u++) {
+            const int e = e0 + u * BLK;
+            if (e < tot) tile[(e / W) * WS + e % W] = v[u];
+        }
+    }
     __syncthreads();
     if (threadIdx.x < nrows) {
         float *r = tile + threadIdx.x * WS;
@@ -600,7 +614,19 @@ smooth_direct_x_k(float *grid, DirectPass dp, Taps taps, const long long *goffs)
         for (int j = N; j < W; j++) r[j] = 0.0f;
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < tot; e += BLK) base[e] = tile[(e / W) * WS + e % W];
+    for (int e0 = threadIdx.x; e0 < tot; e0 += BLK * kLB) {
+        float v[kLB];
+#pragma unroll
+        for (int u = 0; u < kLB; u++) {
+            const int e = e0 + u * BLK;
+            v[u] = e < tot ? tile[(e / W) * WS + e % W] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kLB; u++) {
+            const int e = e0 + u * BLK;
+            if (e < tot) base[e] = v[u];
+        }
+    }
 }
 
 // ---- host planning ----------------------------------------------------------
