@@ -520,6 +520,13 @@ __device__ inline double grp_max_d(double v) {
     for (int o = kG / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
     return v;
 }
+#ifdef PGALF_KNN_STATS
+__device__ inline int grp_max_i(int v) {
+#pragma unroll
+    for (int o = kG / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+#endif
 __device__ inline int wave_max_i(int v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
@@ -527,7 +534,11 @@ __device__ inline int wave_max_i(int v) {
 }
 
 // Debug-only per-wave counters (PGALF_DEBUG): events and shader-clock cycles.
+#ifdef PGALF_KNN_STATS
+constexpr int kDbgN = 64;
+#else
 constexpr int kDbgN = 16;
+#endif
 __device__ inline unsigned long long dbg_clock(const unsigned long long *dbg) {
     return dbg ? (unsigned long long)clock64() : 0ull;
 }
@@ -636,6 +647,9 @@ struct QState {
     unsigned ti[kTop];
     Key kth;
     int cnt;
+#ifdef PGALF_KNN_STATS
+    int acc;  // candidates accepted into the buffer so far
+#endif
 };
 
 // LDS staging area of one scan batch: kBatch buckets per group, one 32-byte
@@ -765,6 +779,12 @@ __device__ inline void scan_list(QState &st, int *list, int count, bool valid, i
             }
             if (__ballot(st.cnt > kBufCap - kChunkC) != 0ull) {
                 if (dbg && lane == 0) atomicAdd(&dbg[1], 1ull);  // mid-scan merges
+                if (dbg && valid) atomicAdd(&dbg[13], (unsigned long long)st.cnt);
+#ifdef PGALF_KNN_STATS
+                st.acc += valid ? st.cnt : 0;
+#endif
+                const int wm = dbg ? wave_max_i(st.cnt) : 0;
+                if (dbg && lane == 0) atomicAdd(&dbg[14], (unsigned long long)((wm + kBuf - 1) / kBuf));
                 const unsigned long long t_m = dbg_clock(dbg);
                 flush_buffer(st.td, st.ti, st.cnt, s_bc, lane, px, py, pz, cx, cy, cz, cid);
                 dbg_cycles(dbg, 11, t_m, lane);
@@ -778,6 +798,12 @@ __device__ inline void scan_list(QState &st, int *list, int count, bool valid, i
     // merge what is left, so the caller's bounds are exact for what was seen
     if (__ballot(st.cnt > 0) != 0ull) {
         if (dbg && lane == 0) atomicAdd(&dbg[6], 1ull);  // end-of-list merges
+        if (dbg && valid) atomicAdd(&dbg[13], (unsigned long long)st.cnt);
+#ifdef PGALF_KNN_STATS
+        st.acc += valid ? st.cnt : 0;
+#endif
+        const int wm = dbg ? wave_max_i(st.cnt) : 0;
+        if (dbg && lane == 0) atomicAdd(&dbg[14], (unsigned long long)((wm + kBuf - 1) / kBuf));
         const unsigned long long t_m = dbg_clock(dbg);
         flush_buffer(st.td, st.ti, st.cnt, s_bc, lane, px, py, pz, cx, cy, cz, cid);
         dbg_cycles(dbg, 11, t_m, lane);
@@ -885,6 +911,9 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
     st.kth.d = st.td[0];
     st.kth.i = st.ti[0];
     st.cnt = 0;
+#ifdef PGALF_KNN_STATS
+    st.acc = 0;
+#endif
     __syncthreads();
     const double inf = __longlong_as_double(0x7ff0000000000000ll);
 
@@ -961,6 +990,9 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
               s_qx, s_qy, s_qz, s_qk, lane, grp, gl, dbg);
     double R2 = grp_max_d(valid ? st.kth.d : 0.0);
     dbg_cycles(dbg, 8, t_start, lane);
+#ifdef PGALF_KNN_STATS
+    const int acc_seed = st.acc;
+#endif
 
     // ---- depth-first walk, nearest child first: a round pops one node per
     // group and its kG lanes test the node's children against the group box
@@ -1037,6 +1069,22 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
         if (!any) break;
     }
     if (dbg && lane == 0) atomicAdd(&dbg[5], 1ull);  // waves
+#ifdef PGALF_KNN_STATS
+    if (dbg) {
+        const int mx = wave_max_i(st.acc), mxs = wave_max_i(acc_seed);
+        const int gmx = grp_max_i(st.acc);
+        if (valid) {
+            atomicAdd(&dbg[16], (unsigned long long)acc_seed);
+            atomicAdd(&dbg[17], (unsigned long long)(st.acc - acc_seed));
+            atomicAdd(&dbg[24 + min(st.acc / 16, 31)], 1ull);
+        }
+        if (lane == 0) {
+            atomicAdd(&dbg[18], (unsigned long long)mx);
+            atomicAdd(&dbg[19], (unsigned long long)mxs);
+        }
+        if (gl == 0) atomicAdd(&dbg[20], (unsigned long long)gmx);
+    }
+#endif
     dbg_cycles(dbg, 12, t_start, lane);
     if (!valid) return;
     const long long q = qid[qi];
@@ -1168,11 +1216,20 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
                         "wave: scan batches %.1f, mid-scan merges %.1f, walk rounds %.1f, "
                         "list scans %.1f, end merges %.1f, passing/scanned buckets per group "
                         "%.1f/%.1f | kcycles per wave: seed %.1f, walk %.1f, "
-                        "scan %.1f (merges %.1f), total %.1f\n",
+                        "scan %.1f (merges %.1f), total %.1f | accepted per query %.1f, chunk "
+                        "merges per wave %.1f\n",
                 np, nc, nb, h0.nlev, d[0] / w, d[1] / w, d[2] / w, d[4] / w, d[6] / w,
                 d[3] / w / kGroups, d[7] / w / kGroups,
                 d[8] / w / 1e3, d[9] / w / 1e3, d[10] / w / 1e3, d[11] / w / 1e3,
-                d[12] / w / 1e3);
+                d[12] / w / 1e3, d[13] / (double)(np > 0 ? np : 1), d[14] / w);
+#ifdef PGALF_KNN_STATS
+        const double nq = (double)(np > 0 ? np : 1);
+        fprintf(stderr, "[pgalf] knn stats: accepted per query seed %.1f walk %.1f | wave max "
+                        "%.1f (seed %.1f) | group max %.1f | histogram (16s):",
+                d[16] / nq, d[17] / nq, d[18] / w, d[19] / w, d[20] / w / kGroups);
+        for (int b = 0; b < 32; b++) fprintf(stderr, " %llu", d[24 + b]);
+        fprintf(stderr, "\n");
+#endif
     }
     return hipGetLastError();
 }
