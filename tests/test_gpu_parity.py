@@ -87,7 +87,7 @@ def test_findden_bitexact_fixture():
 
 @pytest.mark.parametrize("shape", [(32, 32, 32), (37, 35, 32), (64, 48, 40), (21, 17, 24),
                                    (64, 64, 64), (128, 37, 64), (37, 256, 128),
-                                   (256, 24, 128)])
+                                   (256, 24, 128), (16, 16, 16), (20, 20, 20), (8, 12, 16)])
 def test_smooth_vs_oracle(shape):
     nx, ny, nz = shape
     say("smooth %s" % (shape,))
