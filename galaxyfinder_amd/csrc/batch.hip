@@ -339,7 +339,8 @@ extern "C" int pgalf_batch_run(pgalf_batch_t *b, const float *x, const float *y,
                                const float *z, const float *m, float *den, float *rho_tsc,
                                long long *nbr, float *rho_knn, int use_graph, void *stream) {
     const char *where = "pgalf_batch_run";
-    if (!b || !x || !y || !z || !m || !den) return set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
+    if (!b || !x || !y || !z || !m || !den || !aligned8(den))
+        return set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
     std::lock_guard<std::mutex> lk(g_api_mu);
     const std::vector<const void *> key = {x, y, z, m, den, rho_tsc, nbr, rho_knn};
     return run_maybe_graph(b, key, use_graph, (hipStream_t)stream, where, [&](hipStream_t s) {
@@ -355,7 +356,8 @@ extern "C" int pgalf_batch_run_grad(pgalf_batch_t *b, const float *x, const floa
                                     float climb_step, float *cx, float *cy, float *cz,
                                     int *peak, int use_graph, void *stream) {
     const char *where = "pgalf_batch_run_grad";
-    if (!b || !x || !y || !z || !m || !den || !rho_tsc || !nbr || !rho_knn || !gx || !gy ||
+    if (!b || !x || !y || !z || !m || !den || !aligned8(den) || !rho_tsc || !nbr || !rho_knn ||
+        !gx || !gy ||
         !gz || !gm || !cx || !cy || !cz || !peak || (rho_mode != 1 && rho_mode != 2) ||
         climb_iters < 0 || !(climb_step >= 0.0f))
         return set_err(PGALF_ERR_ARG, "%s: bad arguments", where);

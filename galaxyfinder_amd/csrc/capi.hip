@@ -521,7 +521,7 @@ static Grid grid_from(const pgalf_grid_t *g) {
 extern "C" int pgalf_dev_tsc(const float *x, const float *y, const float *z, const float *m,
                              const int *type, long long np, int ptype, const pgalf_grid_t *g,
                              float *den, void *stream) {
-    if (!g || !den || np < 0 || !check_grid(g->nx, g->ny, g->nz))
+    if (!g || !den || !aligned8(den) || np < 0 || !check_grid(g->nx, g->ny, g->nz))
         return set_err(PGALF_ERR_ARG, "pgalf_dev_tsc: bad arguments");
     std::lock_guard<std::mutex> lk(g_api_mu);
     hipStream_t s = (hipStream_t)stream;
@@ -534,7 +534,7 @@ extern "C" int pgalf_dev_tsc(const float *x, const float *y, const float *z, con
 }
 
 extern "C" int pgalf_dev_smooth(float *den, const pgalf_grid_t *g, float RG, void *stream) {
-    if (!g || !den || !check_grid(g->nx, g->ny, g->nz))
+    if (!g || !den || !aligned8(den) || !check_grid(g->nx, g->ny, g->nz))
         return set_err(PGALF_ERR_ARG, "pgalf_dev_smooth: bad arguments");
     std::lock_guard<std::mutex> lk(g_api_mu);
     hipStream_t s = (hipStream_t)stream;
@@ -583,7 +583,8 @@ extern "C" int pgalf_dev_density_stage(const float *x, const float *y, const flo
                                        const float *m, long long np, const pgalf_grid_t *g,
                                        float RG, int k, float *den, float *rho_tsc,
                                        long long *nbr, float *rho_knn, void *stream) {
-    if (!g || !den || np < 1 || k < 1 || k > 32 || !check_grid(g->nx, g->ny, g->nz))
+    if (!g || !den || !aligned8(den) || np < 1 || k < 1 || k > 32 ||
+        !check_grid(g->nx, g->ny, g->nz))
         return set_err(PGALF_ERR_ARG, "pgalf_dev_density_stage: bad arguments");
     if (k > np) return set_err(PGALF_ERR_SHORT_LIST, "pgalf_dev_density_stage: k > np");
     std::lock_guard<std::mutex> lk(g_api_mu);

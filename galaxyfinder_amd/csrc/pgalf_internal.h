@@ -13,6 +13,10 @@
 
 namespace pgalf {
 
+// Grids are read and written as float pairs (mx and every grid offset are
+// even): device grid pointers must be 8-byte aligned.
+inline bool aligned8(const void *p) { return ((uintptr_t)p & 7u) == 0; }
+
 // Grid geometry of the reference's padded in-place r2c layout:
 // den(i,j,k) = den[i + mx*(j + ny*k)], mx = 2*(nx/2+1) (tsc_omp2.c:16,198).
 struct Grid {

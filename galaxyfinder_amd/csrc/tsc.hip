@@ -401,13 +401,15 @@ tsc_detile_k(const HaloSeg *segs, HaloSeg seg0, const unsigned *tile_halo,
             if (gz >= g.nz) break;
             float *base = den + sg.goff + mx * (ty * kT + (long long)g.ny * gz);
             const int lzo = kT * kT * lz;
-            for (int e0 = 0; e0 < n; e0 += 64 * kU) {
-                float v[kU];
+            // pairs of floats: mx and every grid offset are even, so an even
+            // element e and e + 1 share a row and a 16-float tile piece
+            for (int e0 = 0; e0 < n; e0 += 128 * kU) {
+                float2 v[kU];
 #pragma unroll
                 for (int j = 0; j < kU; j++) {
                     // element e of the run: row ly = e / mx (float estimate,
                     // exact after one correction), column x
-                    const int e = e0 + lane + 64 * j;
+                    const int e = e0 + 2 * (lane + 64 * j);
                     int ly = (int)((float)e * rmx);
                     int x = e - ly * mxi;
                     if (x < 0) {
@@ -418,14 +420,15 @@ tsc_detile_k(const HaloSeg *segs, HaloSeg seg0, const unsigned *tile_halo,
                         ly++;
                     }
                     const int slot = e < n ? s_slot[x >> 4] : -1;
-                    v[j] = slot >= 0 ? tbuf[(size_t)slot * kTileNodes +
-                                            (unsigned)((x & (kT - 1)) + kT * ly + lzo)]
-                                     : 0.0f;
+                    v[j] = slot >= 0 ? *reinterpret_cast<const float2 *>(
+                                           tbuf + (size_t)slot * kTileNodes +
+                                           (unsigned)((x & (kT - 1)) + kT * ly + lzo))
+                                     : make_float2(0.0f, 0.0f);
                 }
 #pragma unroll
                 for (int j = 0; j < kU; j++) {
-                    const int e = e0 + lane + 64 * j;
-                    if (e < n) base[e] = v[j];
+                    const int e = e0 + 2 * (lane + 64 * j);
+                    if (e < n) *reinterpret_cast<float2 *>(base + e) = v[j];
                 }
             }
         }

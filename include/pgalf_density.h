@@ -111,7 +111,10 @@ typedef struct pgalf_grid {
 /* All pointers are device pointers; `stream` is a hipStream_t (NULL for the
  * default stream).  Calls are stream-ordered; their status is that of the
  * launch.  Kernel-detected conditions (out-of-grid particles, short lists)
- * are collected with pgalf_dev_status(). */
+ * are collected with pgalf_dev_status().  Grids that a call writes (`den` of
+ * pgalf_dev_tsc, pgalf_dev_smooth, pgalf_dev_density_stage and the batch
+ * runs) must be 8-byte aligned: they are written as float pairs
+ * (PGALF_ERR_ARG otherwise). */
 int pgalf_dev_tsc(const float *x, const float *y, const float *z, const float *m,
                   const int *type, long long np, int ptype, const pgalf_grid_t *g,
                   float *den, void *stream);

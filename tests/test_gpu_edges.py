@@ -7,7 +7,8 @@ The cases:
   * coincident particles (distance ties, broken by index);
   * too few candidates (status 5, where the reference exits);
   * empty deposits and read-backs;
-  * a batch of tiny ragged halos.
+  * a batch of tiny ragged halos;
+  * a device grid that is not 8-byte aligned (refused).
 
 Expected results:
   * lists come from the oracle's brute-force exact search and are bit-exact;
@@ -137,3 +138,26 @@ def test_batch_of_tiny_ragged_halos(k):
         idx, _ = O.knn(pos[a:e].astype(np.float64), k, method="brute")
         assert np.array_equal(nbr_h[a:e], idx), "oracle lists, halo %d" % h
     b.close()
+
+
+def test_unaligned_grid_is_rejected():
+    """Grids are written as float pairs: a device grid at an odd float offset
+    is refused before any launch (PGALF_ERR_ARG)."""
+    import torch
+    say("unaligned device grid")
+    pos = _cloud(64, 32, 9).astype(np.float32)
+    x, y, z = (_dev(pos[:, a]) for a in range(3))
+    m = _dev(np.ones(64, dtype=np.float32))
+    g = D.grid_struct(16, 16, 16, 0.0, 0.0, 0.0, 1.0 / 16)
+    nfl = int(np.prod(D.padded_shape(16, 16, 16)))
+    big = torch.zeros(nfl + 1, dtype=torch.float32, device="cuda")
+    with pytest.raises(D.PgalfError) as ei:
+        D.dev_tsc(x, y, z, m, g, big[1:].view(D.padded_shape(16, 16, 16)))
+    assert ei.value.code == 1
+    D.dev_tsc(x, y, z, m, g, big[:nfl].view(D.padded_shape(16, 16, 16)))
+    D.dev_status()
+    unit, bad = O.tsc_deposit(np.array([[0.5, 0.5, 0.5]]), np.array([1.0]),
+                              (16, 16, 16, 0.0, 0.0, 0.0, 1.0 / 16), fp64=True)
+    assert bad == 0
+    tot, want = float(big[:nfl].double().sum()), 64.0 * float(unit.sum())
+    assert abs(tot - want) <= 1e-5 * want
