@@ -442,6 +442,22 @@ extern "C" int pgalf_near_lists(pgalf_particle_t *bp, int np, int Numnear, long 
     return host_knn(bp, np, Numnear, nearindex, nullptr, 0, PGALF_TYPE_ALL, "pgalf_near_lists");
 }
 
+// The density stage's read-back, run by the kNN once its queries are in
+// Hilbert order: over the sorted FP64 coordinates (the same values as the
+// caller's, so the same bits), writing each result to its particle.
+// Neighbouring threads then gather neighbouring grid nodes.
+static SortedHook sorted_readback(const Grid &g, const float *den, float *out, Workspace &ws,
+                                  hipStream_t s) {
+    Workspace *w = &ws;
+    return [g, den, out, w, s](const double *sx, const double *sy, const double *sz,
+                               const unsigned *order, long long n) {
+        timing_mark("findden", s, false);
+        const hipError_t e = tsc_readback<double>(sx, sy, sz, n, g, den, out, *w, s, order);
+        timing_mark("findden", s, true);
+        return e;
+    };
+}
+
 extern "C" int pgalf_lag_density_stage(pgalf_particle_t *bp, int np, int Numnear, int ptype,
                                        float *densph, long long *nearindex, float **grid_out,
                                        int *nxyz_out, double *xyzmin_out) {
@@ -487,10 +503,13 @@ extern "C" int pgalf_lag_density_stage(pgalf_particle_t *bp, int np, int Numnear
     }
     if (e == hipSuccess) e = tsc_deposit<double>(d.x, d.y, d.z, d.m, d.type, np, ptype, g, dden, ws, s);
     if (e == hipSuccess) e = gaussian_smooth(dden, g, RG, ws, s);
-    if (e == hipSuccess) e = tsc_readback<double>(d.x, d.y, d.z, np, g, dden, drho, ws, s);
-    if (e == hipSuccess && nearindex)
+    if (e == hipSuccess && nearindex) {
+        const SortedHook rb = sorted_readback(g, dden, drho, ws, s);
         e = knn<double>(d.x, d.y, d.z, d.m, np, nullptr, PGALF_TYPE_ALL, Numnear, dnbr, nullptr, 0,
-                        ws, s);
+                        ws, s, &rb);
+    } else if (e == hipSuccess) {
+        e = tsc_readback<double>(d.x, d.y, d.z, np, g, dden, drho, ws, s);
+    }
     if (e == hipSuccess)
         e = hipMemcpyAsync(densph, drho, sizeof(float) * np, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess && nearindex)
@@ -599,12 +618,20 @@ extern "C" int pgalf_dev_density_stage(const float *x, const float *y, const flo
     timing_mark("smooth", s, false);
     if (e == hipSuccess) e = gaussian_smooth(den, gg, RG, ws, s);
     timing_mark("smooth", s, true);
-    timing_mark("findden", s, false);
-    if (e == hipSuccess && rho_tsc) e = tsc_readback<float>(x, y, z, np, gg, den, rho_tsc, ws, s);
-    timing_mark("findden", s, true);
+    // with a kNN to run, the read-back waits for its Hilbert order (and is
+    // timed inside the "knn" interval)
+    const bool with_knn = nbr || rho_knn;
+    if (e == hipSuccess && rho_tsc && !with_knn) {
+        timing_mark("findden", s, false);
+        e = tsc_readback<float>(x, y, z, np, gg, den, rho_tsc, ws, s);
+        timing_mark("findden", s, true);
+    }
     timing_mark("knn", s, false);
-    if (e == hipSuccess && (nbr || rho_knn))
-        e = knn<float>(x, y, z, m, np, nullptr, PGALF_TYPE_ALL, k, nbr, rho_knn, rho_knn ? 1 : 0, ws, s);
+    if (e == hipSuccess && with_knn) {
+        const SortedHook rb = rho_tsc ? sorted_readback(gg, den, rho_tsc, ws, s) : SortedHook();
+        e = knn<float>(x, y, z, m, np, nullptr, PGALF_TYPE_ALL, k, nbr, rho_knn, rho_knn ? 1 : 0,
+                       ws, s, &rb);
+    }
     timing_mark("knn", s, true);
     timing_mark("stage", s, true);
     return e == hipSuccess ? ok() : hip_err(e, "pgalf_dev_density_stage");

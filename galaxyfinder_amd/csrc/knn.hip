@@ -1127,7 +1127,7 @@ inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 template <class P>
 hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, const int *type,
                int cand_ptype, int k, long long *nbr, float *rho, int rho_mode, Workspace &ws,
-               hipStream_t s) {
+               hipStream_t s, const SortedHook *sorted_hook) {
     if (k < 1 || k > kTop || np < 1) return hipErrorInvalidValue;
     WsScope scope(ws);
     timing_mark("knn_prep", s, false);
@@ -1210,6 +1210,7 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
                                                      boxes + level_offset(nb, l), nl);
     }
     timing_mark("knn_prep", s, true);
+    if (sorted_hook && *sorted_hook) PGALF_HIP_TRY((*sorted_hook)(qx, qy, qz, qv, np));
     timing_mark("knn_query", s, false);
     long long nqb = (np + kWave - 1) / kWave;
     knn_query_k<P><<<(unsigned)nqb, kWave, 0, s>>>(qx, qy, qz, qv, cx, cy, cz, cid, boxes,
@@ -1635,9 +1636,9 @@ template hipError_t knn_batch<double>(const double *, const double *, const doub
 
 template hipError_t knn<float>(const float *, const float *, const float *, const float *,
                                long long, const int *, int, int, long long *, float *, int,
-                               Workspace &, hipStream_t);
+                               Workspace &, hipStream_t, const SortedHook *);
 template hipError_t knn<double>(const double *, const double *, const double *,
                                 const double *, long long, const int *, int, int, long long *,
-                                float *, int, Workspace &, hipStream_t);
+                                float *, int, Workspace &, hipStream_t, const SortedHook *);
 
 }  // namespace pgalf

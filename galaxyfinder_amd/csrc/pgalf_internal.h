@@ -7,6 +7,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <vector>
 
 #include "../../include/pgalf_density.h"
@@ -131,10 +132,12 @@ template <class P>
 hipError_t tsc_deposit(const P *x, const P *y, const P *z, const P *m,
                        const int *type, long long np, int ptype, const Grid &g,
                        float *den, Workspace &ws, hipStream_t s);
+// order (optional): particle i of x, y, z is particle order[i] of out --
+// the read-back then runs over coordinates already in a spatial order.
 template <class P>
 hipError_t tsc_readback(const P *x, const P *y, const P *z, long long np,
                         const Grid &g, const float *den, float *out,
-                        Workspace &ws, hipStream_t s);
+                        Workspace &ws, hipStream_t s, const unsigned *order = nullptr);
 // Batched forms: every halo of `b` deposited into / read back from its own
 // grid of the batch grid buffer `den` (HaloSeg::goff).  No host
 // synchronisation, no memset: safe to capture in a HIP graph.
@@ -202,10 +205,16 @@ hipError_t gaussian_smooth_batch(float *den, const SmoothGroup *groups, int ngro
 // rho_mode: 0 none, 1 unweighted W4 sum, 2 W4 weighted by candidate mass.
 // cand_sel: optional int type array + ptype selecting the candidates among
 // the same particle arrays (nullptr: every particle is a candidate).
+// sorted_hook (optional) runs on the stream once the queries are in
+// Hilbert order: FP64 copies of their coordinates and each sorted query's
+// particle index (used by the density stage for a coherent read-back).
+using SortedHook = std::function<hipError_t(const double *, const double *, const double *,
+                                            const unsigned *, long long)>;
 template <class P>
 hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np,
                const int *type, int cand_ptype, int k, long long *nbr,
-               float *rho, int rho_mode, Workspace &ws, hipStream_t s);
+               float *rho, int rho_mode, Workspace &ws, hipStream_t s,
+               const SortedHook *sorted_hook = nullptr);
 
 // Device tables of a batched kNN, built once per catalogue (knn_batch_plan):
 // per-halo trees in one box array (leaves of every halo first, in padded

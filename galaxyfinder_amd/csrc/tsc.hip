@@ -464,7 +464,8 @@ __device__ inline void readback_axis(double p, RAxis &a) {
 template <class P>
 __global__ void __launch_bounds__(kBlock)
 tsc_readback_k(const P *x, const P *y, const P *z, const HaloSeg *segs, HaloSeg seg0,
-               const PChunk *chunks, int per, const float *den, float *out, unsigned *status) {
+               const PChunk *chunks, int per, const float *den, float *out, unsigned *status,
+               const unsigned *order) {
     const PChunk ch = chunk_at(chunks, seg0, per);
     const HaloSeg sg = seg_at(segs, seg0, ch.halo);
     const Grid g = sg.g;
@@ -480,7 +481,7 @@ tsc_readback_k(const P *x, const P *y, const P *z, const HaloSeg *segs, HaloSeg 
         if (a[0].nr < 1 || a[0].nr > g.mx() - 2 || a[1].nr < 1 || a[1].nr > g.ny - 2 ||
             a[2].nr < 1 || a[2].nr > g.nz - 2) {
             atomicOr(status, (unsigned)DEV_ERR_OUT_OF_GRID);
-            out[i] = 0.0f;
+            out[order ? (long long)order[i] : i] = 0.0f;
             continue;
         }
         long ix[3] = {a[0].nr, a[0].nr + a[0].sg, a[0].nr - a[0].sg};
@@ -504,7 +505,7 @@ tsc_readback_k(const P *x, const P *y, const P *z, const HaloSeg *segs, HaloSeg 
                     float gv = hden[ix[q] + mx * (iy[b] + (long)g.ny * iz[c])];
                     s += (gv * wxy[b][q]) * wz[c];
                 }
-        out[i] = s;
+        out[order ? (long long)order[i] : i] = s;
     }
 }
 
@@ -830,11 +831,12 @@ hipError_t tsc_deposit_batch(const P *x, const P *y, const P *z, const P *m, con
 
 template <class P>
 hipError_t tsc_readback(const P *x, const P *y, const P *z, long long np, const Grid &g,
-                        const float *den, float *out, Workspace &ws, hipStream_t s) {
+                        const float *den, float *out, Workspace &ws, hipStream_t s,
+                        const unsigned *order) {
     if (np <= 0) return hipSuccess;
     const HaloSeg seg0 = single_seg(g, np);
     tsc_readback_k<P><<<(unsigned)((np + kBlock - 1) / kBlock), kBlock, 0, s>>>(
-        x, y, z, nullptr, seg0, nullptr, kBlock, den, out, ws.status());
+        x, y, z, nullptr, seg0, nullptr, kBlock, den, out, ws.status(), order);
     return hipGetLastError();
 }
 
@@ -844,9 +846,8 @@ hipError_t tsc_readback_batch(const P *x, const P *y, const P *z, const BatchDev
     if (b.nchunks <= 0) return hipSuccess;
     const HaloSeg seg0{};
     const dim3 grid((unsigned)b.nchunks, kChunkParticles / kBlock);
-    tsc_readback_k<P><<<grid, kBlock, 0, s>>>(x, y, z, b.segs, seg0, b.chunks,
-                                                             kChunkParticles, den, out,
-                                                             ws.status());
+    tsc_readback_k<P><<<grid, kBlock, 0, s>>>(x, y, z, b.segs, seg0, b.chunks, kChunkParticles,
+                                               den, out, ws.status(), nullptr);
     return hipGetLastError();
 }
 
@@ -918,7 +919,7 @@ hipError_t peak_climb_batch(const P *x, const P *y, const P *z, const BatchDev &
                                        hipStream_t);                                           \
     template hipError_t tsc_readback<P>(const P *, const P *, const P *, long long,            \
                                         const Grid &, const float *, float *, Workspace &,     \
-                                        hipStream_t);                                          \
+                                        hipStream_t, const unsigned *);                        \
     template hipError_t tsc_deposit_batch<P>(const P *, const P *, const P *, const P *,       \
                                              const BatchDev &, float *, Workspace &,           \
                                              hipStream_t);                                     \
