@@ -92,13 +92,32 @@ hipError_t run_sequence(pgalf_batch *b, const float *x, const float *y, const fl
     timing_mark("batch_smooth", s, false);
     PGALF_HIP_TRY(gaussian_smooth_batch(den, b->groups.data(), (int)b->groups.size(), s));
     timing_mark("batch_smooth", s, true);
-    timing_mark("batch_findden", s, false);
-    if (rho_tsc) PGALF_HIP_TRY(tsc_readback_batch<float>(x, y, z, b->bd, den, rho_tsc, ws, s));
-    timing_mark("batch_findden", s, true);
+    // with a kNN to run, the read-back goes over its sorted slots (coherent
+    // gathers, the same FP64 coordinates) and is timed inside "batch_knn"
+    const bool with_knn = nbr || rho_knn;
+    if (rho_tsc && !with_knn) {
+        timing_mark("batch_findden", s, false);
+        PGALF_HIP_TRY(tsc_readback_batch<float>(x, y, z, b->bd, den, rho_tsc, ws, s));
+        timing_mark("batch_findden", s, true);
+    }
     timing_mark("batch_knn", s, false);
-    if (nbr || rho_knn)
+    if (with_knn) {
+        const long long *shift = b->kplan.slot_shift;
+        const BatchDev &bd = b->bd;
+        const SortedHook rb =
+            rho_tsc ? SortedHook([&bd, den, rho_tsc, &ws, s, shift](
+                                     const double *sx, const double *sy, const double *sz,
+                                     const unsigned *sid, long long) {
+                timing_mark("batch_findden", s, false);
+                const hipError_t e =
+                    tsc_readback_batch<double>(sx, sy, sz, bd, den, rho_tsc, ws, s, sid, shift);
+                timing_mark("batch_findden", s, true);
+                return e;
+            })
+                    : SortedHook();
         PGALF_HIP_TRY(knn_batch<float>(x, y, z, m, b->bd, b->kplan, nbr, rho_knn,
-                                       rho_knn ? rho_mode : 0, ws, s));
+                                       rho_knn ? rho_mode : 0, ws, s, &rb));
+    }
     timing_mark("batch_knn", s, true);
     timing_mark("batch_stage", s, true);
     return hipGetLastError();

@@ -1514,8 +1514,15 @@ hipError_t knn_batch_plan(const long long *off, int nhalo, int k, KnnBatchPlan &
     const size_t ni = cbeg.size() ? cbeg.size() : 1;
     if (e == hipSuccess) e = hipMalloc(&p.child_begin, sizeof(long long) * ni);
     if (e == hipSuccess) e = hipMalloc(&p.child_cnt, ni);
+    if (e == hipSuccess) e = hipMalloc(&p.slot_shift, sizeof(long long) * hs.size());
     if (e == hipSuccess)
         e = hipMemcpy(p.halos, hs.data(), sizeof(KHalo) * hs.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        std::vector<long long> shift(hs.size());
+        for (size_t h = 0; h < hs.size(); h++) shift[h] = hs[h].c0 - hs[h].idx_base;
+        e = hipMemcpy(p.slot_shift, shift.data(), sizeof(long long) * shift.size(),
+                      hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess)
         e = hipMemcpy(p.waves, wv.data(), sizeof(KWave) * wv.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && !cbeg.empty())
@@ -1536,6 +1543,7 @@ void knn_batch_plan_free(KnnBatchPlan &plan) {
     if (plan.waves) (void)hipFree(plan.waves);
     if (plan.child_begin) (void)hipFree(plan.child_begin);
     if (plan.child_cnt) (void)hipFree(plan.child_cnt);
+    if (plan.slot_shift) (void)hipFree(plan.slot_shift);
     plan = KnnBatchPlan();
 }
 
@@ -1571,7 +1579,7 @@ hipError_t batch_order(const P *x, const P *y, const P *z, const BatchDev &b,
 template <class P>
 hipError_t knn_batch(const P *x, const P *y, const P *z, const P *m, const BatchDev &b,
                      const KnnBatchPlan &plan, long long *nbr, float *rho, int rho_mode,
-                     Workspace &ws, hipStream_t s) {
+                     Workspace &ws, hipStream_t s, const SortedHook *sorted_hook) {
     const long long np = plan.np, npad = plan.npad;
     if (np != b.np || plan.nhalo != b.nhalo) return hipErrorInvalidValue;
     WsScope scope(ws);
@@ -1596,6 +1604,7 @@ hipError_t knn_batch(const P *x, const P *y, const P *z, const P *m, const Batch
         lb = le;
     }
     timing_mark("knn_prep", s, true);
+    if (sorted_hook && *sorted_hook) PGALF_HIP_TRY((*sorted_hook)(sx, sy, sz, sid, npad));
     timing_mark("knn_query", s, false);
     KHalo h0{};
     knn_query_k<P><<<(unsigned)plan.nwaves, kWave, 0, s>>>(
@@ -1629,10 +1638,11 @@ template hipError_t knn_batch_backward<float>(const float *, const float *, cons
 
 template hipError_t knn_batch<float>(const float *, const float *, const float *, const float *,
                                      const BatchDev &, const KnnBatchPlan &, long long *,
-                                     float *, int, Workspace &, hipStream_t);
+                                     float *, int, Workspace &, hipStream_t, const SortedHook *);
 template hipError_t knn_batch<double>(const double *, const double *, const double *,
                                       const double *, const BatchDev &, const KnnBatchPlan &,
-                                      long long *, float *, int, Workspace &, hipStream_t);
+                                      long long *, float *, int, Workspace &, hipStream_t,
+                                      const SortedHook *);
 
 template hipError_t knn<float>(const float *, const float *, const float *, const float *,
                                long long, const int *, int, int, long long *, float *, int,

@@ -145,10 +145,13 @@ template <class P>
 hipError_t tsc_deposit_batch(const P *x, const P *y, const P *z, const P *m,
                              const BatchDev &b, float *den, Workspace &ws,
                              hipStream_t s);
+// order/shift (optional): the coordinates are in sorted slots -- particle
+// i of halo h at slot i + shift[h] -- and slot p is particle order[p] of out.
 template <class P>
 hipError_t tsc_readback_batch(const P *x, const P *y, const P *z, const BatchDev &b,
                               const float *den, float *out, Workspace &ws,
-                              hipStream_t s);
+                              hipStream_t s, const unsigned *order = nullptr,
+                              const long long *shift = nullptr);
 // Deposit tiles of one grid (16^3 nodes over the padded mx x ny x nz box).
 long long tsc_tile_count(const Grid &g);
 // Appends the first tile id (tx = 0) of every (ty, tz) slab of a grid whose
@@ -227,6 +230,7 @@ struct KnnBatchPlan {
     void *waves = nullptr;  // KWave[nwaves] (knn.hip)
     long long *child_begin = nullptr;      // [nboxes - nb_total] first child box
     unsigned char *child_cnt = nullptr;    // [nboxes - nb_total] children (1..8)
+    long long *slot_shift = nullptr;       // [nhalo] sorted slot of a halo's particle i: i + shift
     std::vector<long long> level_begin;    // internal level l: boxes [level_begin[l-1], level_begin[l])
 };
 // off: host [nhalo+1] particle offsets (off[0] = 0, every halo >= k particles).
@@ -234,10 +238,12 @@ hipError_t knn_batch_plan(const long long *off, int nhalo, int k, KnnBatchPlan &
 void knn_batch_plan_free(KnnBatchPlan &plan);
 // Exact k-NN lists (indices local to each halo) and W4 densities of every
 // particle among the particles of its own halo; no host synchronisation.
+// sorted_hook (optional): as for knn(), with the padded sorted slots
+// (KnnBatchPlan::slot_shift maps particles to them).
 template <class P>
 hipError_t knn_batch(const P *x, const P *y, const P *z, const P *m, const BatchDev &b,
                      const KnnBatchPlan &plan, long long *nbr, float *rho, int rho_mode,
-                     Workspace &ws, hipStream_t s);
+                     Workspace &ws, hipStream_t s, const SortedHook *sorted_hook = nullptr);
 
 // Adjoint of the W4 kNN density over the forward lists nbr (np x k, indices
 // into the same arrays): adds dL/dx.. and, when weighted, dL/dm of

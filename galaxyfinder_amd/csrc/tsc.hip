@@ -465,23 +465,27 @@ template <class P>
 __global__ void __launch_bounds__(kBlock)
 tsc_readback_k(const P *x, const P *y, const P *z, const HaloSeg *segs, HaloSeg seg0,
                const PChunk *chunks, int per, const float *den, float *out, unsigned *status,
-               const unsigned *order) {
+               const unsigned *order, const long long *shift) {
     const PChunk ch = chunk_at(chunks, seg0, per);
     const HaloSeg sg = seg_at(segs, seg0, ch.halo);
     const Grid g = sg.g;
     const float *hden = den + sg.goff;
     const long mx = (long)g.mx();
+    // particle i's coordinates sit at p = i + sh, its result at order[p]
+    const long long sh = shift ? shift[ch.halo] : 0;
     // blockIdx.y: a share of the chunk (batch chunks are split over gridDim.y blocks)
     for (long long i = ch.b0 + (long long)blockIdx.y * kBlock + threadIdx.x; i < ch.b1;
          i += (long long)kBlock * gridDim.y) {
+        const long long p = i + sh;
+        const long long o = order ? (long long)order[p] : i;
         RAxis a[3];
-        readback_axis(((double)x[i] - g.x0) / g.h, a[0]);
-        readback_axis(((double)y[i] - g.y0) / g.h, a[1]);
-        readback_axis(((double)z[i] - g.z0) / g.h, a[2]);
+        readback_axis(((double)x[p] - g.x0) / g.h, a[0]);
+        readback_axis(((double)y[p] - g.y0) / g.h, a[1]);
+        readback_axis(((double)z[p] - g.z0) / g.h, a[2]);
         if (a[0].nr < 1 || a[0].nr > g.mx() - 2 || a[1].nr < 1 || a[1].nr > g.ny - 2 ||
             a[2].nr < 1 || a[2].nr > g.nz - 2) {
             atomicOr(status, (unsigned)DEV_ERR_OUT_OF_GRID);
-            out[order ? (long long)order[i] : i] = 0.0f;
+            out[o] = 0.0f;
             continue;
         }
         long ix[3] = {a[0].nr, a[0].nr + a[0].sg, a[0].nr - a[0].sg};
@@ -505,7 +509,7 @@ tsc_readback_k(const P *x, const P *y, const P *z, const HaloSeg *segs, HaloSeg 
                     float gv = hden[ix[q] + mx * (iy[b] + (long)g.ny * iz[c])];
                     s += (gv * wxy[b][q]) * wz[c];
                 }
-        out[order ? (long long)order[i] : i] = s;
+        out[o] = s;
     }
 }
 
@@ -836,18 +840,19 @@ hipError_t tsc_readback(const P *x, const P *y, const P *z, long long np, const 
     if (np <= 0) return hipSuccess;
     const HaloSeg seg0 = single_seg(g, np);
     tsc_readback_k<P><<<(unsigned)((np + kBlock - 1) / kBlock), kBlock, 0, s>>>(
-        x, y, z, nullptr, seg0, nullptr, kBlock, den, out, ws.status(), order);
+        x, y, z, nullptr, seg0, nullptr, kBlock, den, out, ws.status(), order, nullptr);
     return hipGetLastError();
 }
 
 template <class P>
 hipError_t tsc_readback_batch(const P *x, const P *y, const P *z, const BatchDev &b,
-                              const float *den, float *out, Workspace &ws, hipStream_t s) {
+                              const float *den, float *out, Workspace &ws, hipStream_t s,
+                              const unsigned *order, const long long *shift) {
     if (b.nchunks <= 0) return hipSuccess;
     const HaloSeg seg0{};
     const dim3 grid((unsigned)b.nchunks, kChunkParticles / kBlock);
     tsc_readback_k<P><<<grid, kBlock, 0, s>>>(x, y, z, b.segs, seg0, b.chunks, kChunkParticles,
-                                               den, out, ws.status(), nullptr);
+                                               den, out, ws.status(), order, shift);
     return hipGetLastError();
 }
 
@@ -925,7 +930,8 @@ hipError_t peak_climb_batch(const P *x, const P *y, const P *z, const BatchDev &
                                              hipStream_t);                                     \
     template hipError_t tsc_readback_batch<P>(const P *, const P *, const P *,                 \
                                               const BatchDev &, const float *, float *,        \
-                                              Workspace &, hipStream_t);
+                                              Workspace &, hipStream_t, const unsigned *,      \
+                                              const long long *);
 PGALF_TSC_INST(float)
 PGALF_TSC_INST(double)
 #undef PGALF_TSC_INST
