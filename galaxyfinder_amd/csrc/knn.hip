@@ -17,7 +17,9 @@
 //   * a wavefront holds eight independent query groups of 8 lanes (8
 //     consecutive sorted queries -- one bucket -- one per lane).  Each group
 //     fills every lane's list from the 4 buckets around its own (a 32-key
-//     odd-even merge sort) and scans a window of seed buckets, which gives every
+//     odd-even merge sort) and merges the 8 buckets on either side of them
+//     straight in, 16 keys at a time (about half of them would pass a bound
+//     test, so testing and buffering them costs more), which gives every
 //     lane a k-th distance.  It then walks the tree depth first, nearest
 //     child first: a round pops one node and its 8 lanes test the 8 children
 //     against the group's box and the union of the lanes' current balls.
@@ -81,6 +83,10 @@ constexpr int kPar = kG / kFan;  // parents expanded per walk round
 #endif
 constexpr int kSeedFill = kTop / kB;  // buckets of the direct seed fill
 constexpr int kSeedSpan = PGALF_KNN_SEEDSPAN;  // seed buckets in all (a window around the fill)
+#ifndef PGALF_KNN_DIRECT
+#define PGALF_KNN_DIRECT 8
+#endif
+constexpr int kDirect = PGALF_KNN_DIRECT;  // window buckets merged without a bound test
 constexpr int kFCap = PGALF_KNN_LISTCAP;  // bucket-list capacity per group
 constexpr int kStackCap = 96;  // depth-first walk stack per group
 constexpr int kFirstScan = PGALF_KNN_FIRSTSCAN;  // the first list is scanned early to tighten the bounds
@@ -100,6 +106,8 @@ static_assert(kBufCap % kBuf == 0 && kBufCap % kChunkC == 0 && kBufCap >= 24,
 static_assert(kBatch >= 1 && kBatch <= kG, "one group lane tests each bucket of a batch");
 static_assert(kFCap >= 32 + 8 && kFCap % 8 == 0, "list must take the first scan and a round");
 static_assert(kSeedSpan >= kSeedFill && kFirstScan + kG <= kFCap, "seed window and first scan");
+static_assert(kDirect % 2 == 0 && kDirect <= kSeedSpan - kSeedFill && kBuf == 2 * kB,
+              "direct merges take two buckets per 16-key chunk");
 // The walk keeps at most kFan entries of the deepest level and kFan-1 of every
 // level above it on the stack.
 constexpr int stack_need(int nlev) { return kFan + (kFan - 1) * (nlev > 3 ? nlev - 3 : 0); }
@@ -981,12 +989,40 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
     constexpr int kSeedBefore = (kSeedSpan - kSeedFill) / 2;
     const long long slo = f0 - kSeedBefore < 0 ? 0 : f0 - kSeedBefore;
     const long long shi = slo + kSeedSpan - 1 > nb - 1 ? nb - 1 : slo + kSeedSpan - 1;
+    // the window's buckets next to the fill, [d0, f0) and [f0 + kSeedFill, d1),
+    // merged straight into the sorted lists (no bound test, no buffer)
+    const long long d0 = f0 - kDirect / 2 > slo ? f0 - kDirect / 2 : slo;
+    const long long d1 = f0 + kSeedFill + kDirect / 2 < shi + 1 ? f0 + kSeedFill + kDirect / 2
+                                                               : shi + 1;
+    if constexpr (kDirect > 0) {
+        const int nlo = (int)(f0 - d0);
+        const int ndir = nlo + (d1 > f0 + kSeedFill ? (int)(d1 - f0 - kSeedFill) : 0);
+#pragma unroll 1
+        for (int c = 0; c < kDirect / 2; c++) {
+            double bd[kBuf];
+            unsigned bi[kBuf];
+#pragma unroll
+            for (int t = 0; t < kBuf; t++) {
+                const int j = 2 * c + t / kB;  // j-th direct bucket
+                const long long b = j < nlo ? d0 + j : f0 + kSeedFill + (j - nlo);
+                const long long e = b * kB + t % kB;
+                bd[t] = inf;
+                bi[t] = 0xffffffffu;
+                if (gvalid && j < ndir && e < nc) {
+                    bd[t] = d2_ref(px, py, pz, cx[e], cy[e], cz[e]);
+                    bi[t] = cid[e];
+                }
+            }
+            merge_top(st.td, st.ti, bd, bi);
+        }
+        st.kth = kth_of(st.td, st.ti, k);
+    }
     // remaining seed buckets through the batched scanner (list in the stack)
     static_assert(kSeedSpan <= kStackCap, "seed list lives in the walk stack");
     int scnt = 0;
     if (gl == 0) {
         for (long long b = slo; b <= shi; b++)
-            if (gvalid && (b < f0 || b >= f0 + kSeedFill)) s_stk[grp][scnt++] = (int)b;
+            if (gvalid && (b < d0 || b >= d1)) s_stk[grp][scnt++] = (int)b;
     }
     scnt = __shfl(scnt, gshift, 64);
     __syncthreads();
