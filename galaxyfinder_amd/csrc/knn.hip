@@ -17,8 +17,8 @@
 //   * a wavefront holds eight independent query groups of 8 lanes (8
 //     consecutive sorted queries -- one bucket -- one per lane).  Each group
 //     fills every lane's list from the 4 buckets around its own (a 32-key
-//     odd-even merge sort) and merges the 8 buckets on either side of them
-//     straight in, 16 keys at a time (about half of them would pass a bound
+//     odd-even merge sort) and merges the 8 buckets around them (4 on
+//     either side) straight in, 16 keys at a time (about half of them would pass a bound
 //     test, so testing and buffering them costs more), which gives every
 //     lane a k-th distance.  It then walks the tree depth first, nearest
 //     child first: a round pops one node and its 8 lanes test the 8 children
