@@ -118,6 +118,25 @@ struct Box {
     double lo[3], hi[3];
 };
 
+// Workgroups go to the 8 XCDs round-robin (block b to XCD b % 8), and each
+// XCD has its own L2.  Neighbouring query waves read mostly the same
+// candidates, so each XCD is given runs of kXcdChunk consecutive logical
+// waves, the runs dealt to the XCDs in turn (contiguous ranges per XCD would
+// unbalance them where wave costs vary along the order, e.g. a batch sorted
+// by halo size).  Block b runs wave xcd_block(b, n), a bijection of [0, n):
+// blocks past the last whole round of runs keep their own index.
+#ifndef PGALF_KNN_XCD_CHUNK
+#define PGALF_KNN_XCD_CHUNK 64
+#endif
+constexpr unsigned kXcdChunk = PGALF_KNN_XCD_CHUNK;  // 0: no remap
+constexpr unsigned kXcds = 8;
+__host__ __device__ inline unsigned xcd_block(unsigned b, unsigned n) {
+    const unsigned round = kXcds * kXcdChunk;
+    if (kXcdChunk == 0 || b >= n / round * round) return b;
+    const unsigned x = b % kXcds, i = b / kXcds;  // the i-th block of XCD x
+    return ((i / kXcdChunk) * kXcds + x) * kXcdChunk + i % kXcdChunk;
+}
+
 // Implicit 8-ary box tree over nb leaf buckets: level l holds
 // ceil(nb / 8^l) nodes.  One tree per halo; the box of node j of level l is
 // boxes[loff[l] + j].
@@ -884,12 +903,13 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
     const int lane = threadIdx.x;
     const int grp = lane / kG, gl = lane % kG, gshift = grp * kG;
     // this wave's halo: its candidates, queries and tree
+    const unsigned wid = xcd_block(blockIdx.x, gridDim.x);
     KWave wv;
     if (waves) {
-        wv = waves[blockIdx.x];
+        wv = waves[wid];
     } else {
         wv.halo = 0;
-        wv.q0 = (long long)blockIdx.x * kWave;
+        wv.q0 = (long long)wid * kWave;
     }
     const KHalo &hd = halos ? halos[wv.halo] : halo0;
     const long long nc = hd.nc, nb = hd.nb, hq0 = hd.q0, hq1 = hd.q0 + hd.nq;
