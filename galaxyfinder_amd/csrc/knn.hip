@@ -1344,7 +1344,8 @@ knn_backward_k(const P *x, const P *y, const P *z, const P *m, long long np,
     __shared__ double win[4][kBwdWin];
     for (int c = threadIdx.x; c < 4 * kBwdWin; c += kBwdBlock) (&win[0][0])[c] = 0.0;
     __syncthreads();
-    const long long s0 = (long long)blockIdx.x * kBwdBlock;
+    // the same XCD-aware order as the query: neighbouring blocks share lists' targets
+    const long long s0 = (long long)xcd_block(blockIdx.x, gridDim.x) * kBwdBlock;
     const long long wb = s0 - kBwdHalo;  // sorted position of window slot 0
     const long long s = s0 + threadIdx.x;
     // adds v to component c of particle j (sorted position rank[j])
