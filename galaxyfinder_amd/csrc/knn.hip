@@ -23,8 +23,9 @@
 //     lane a k-th distance.  It then walks the tree depth first, nearest
 //     child first: a round pops one node and its 8 lanes test the 8 children
 //     against the group's box and the union of the lanes' current balls.
-//     Passing buckets go to a list, ordered by distance class and scanned in
-//     batches: each batch takes the first two buckets of an 8-entry window
+//     Passing buckets go to a list, ordered by distance class (the first
+//     list's 4 nearest buckets are merged straight in, like the seed) and
+//     scanned in batches: each batch takes the first two buckets of an 8-entry window
 //     that some lane's ball still reaches, stages their candidates in LDS,
 //     and every lane tests them against its own k-th key.  Accepted
 //     candidates go to a per-lane LDS buffer; when any lane's buffer may
@@ -87,6 +88,10 @@ constexpr int kSeedSpan = PGALF_KNN_SEEDSPAN;  // seed buckets in all (a window 
 #define PGALF_KNN_DIRECT 8
 #endif
 constexpr int kDirect = PGALF_KNN_DIRECT;  // window buckets merged without a bound test
+#ifndef PGALF_KNN_WALK_DIRECT
+#define PGALF_KNN_WALK_DIRECT 4
+#endif
+constexpr int kWalkDirect = PGALF_KNN_WALK_DIRECT;  // first-list buckets merged the same way
 constexpr int kFCap = PGALF_KNN_LISTCAP;  // bucket-list capacity per group
 constexpr int kStackCap = 96;  // depth-first walk stack per group
 constexpr int kFirstScan = PGALF_KNN_FIRSTSCAN;  // the first list is scanned early to tighten the bounds
@@ -108,6 +113,8 @@ static_assert(kFCap >= 32 + 8 && kFCap % 8 == 0, "list must take the first scan 
 static_assert(kSeedSpan >= kSeedFill && kFirstScan + kG <= kFCap, "seed window and first scan");
 static_assert(kDirect % 2 == 0 && kDirect <= kSeedSpan - kSeedFill && kBuf == 2 * kB,
               "direct merges take two buckets per 16-key chunk");
+static_assert(kWalkDirect % 2 == 0 && kWalkDirect <= kFirstScan,
+              "walk direct merges take two buckets per chunk, from the first list");
 // The walk keeps at most kFan entries of the deepest level and kFan-1 of every
 // level above it on the stack.
 constexpr int stack_need(int nlev) { return kFan + (kFan - 1) * (nlev > 3 ? nlev - 3 : 0); }
@@ -1126,8 +1133,37 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
             ring_sort(list, lcnt, reinterpret_cast<int *>(&sg) + grp * kFCap, gshift, gl);
             __syncthreads();
             if (dbg && lane == 0) atomicAdd(&dbg[4], 1ull);  // list scans
-            scan_list(st, list, lcnt, valid, k, px, py, pz, cx, cy, cz, cid, nc, leaves, sg, s_bc,
-                      s_qx, s_qy, s_qz, s_qk, lane, grp, gl, dbg);
+            int skip = 0;
+            if constexpr (kWalkDirect > 0) {
+                if (nscan == 1) {
+                    // the first list's nearest buckets merged straight in
+                    skip = lcnt < kWalkDirect ? lcnt : kWalkDirect;
+#pragma unroll 1
+                    for (int c = 0; c < kWalkDirect / 2; c++) {
+                        double bd[kBuf];
+                        unsigned bi[kBuf];
+#pragma unroll
+                        for (int t = 0; t < kBuf; t++) {
+                            const int j = 2 * c + t / kB;
+                            bd[t] = inf;
+                            bi[t] = 0xffffffffu;
+                            if (j < skip) {
+                                const long long e = (long long)(list[j] & kBucketMask) * kB + t % kB;
+                                if (e < nc) {
+                                    bd[t] = d2_ref(px, py, pz, cx[e], cy[e], cz[e]);
+                                    bi[t] = cid[e];
+                                }
+                            }
+                        }
+                        merge_top(st.td, st.ti, bd, bi);
+                    }
+                    st.kth = kth_of(st.td, st.ti, k);
+                    s_qk[lane] = valid ? st.kth.d : -1.0;
+                    __syncthreads();
+                }
+            }
+            scan_list(st, list + skip, lcnt - skip, valid, k, px, py, pz, cx, cy, cz, cid, nc,
+                      leaves, sg, s_bc, s_qx, s_qy, s_qz, s_qk, lane, grp, gl, dbg);
             lcnt = 0;
             R2 = grp_max_d(valid ? st.kth.d : 0.0);
         }
