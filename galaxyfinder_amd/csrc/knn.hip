@@ -354,13 +354,43 @@ __device__ inline unsigned hilbert_of(const double v[3], const unsigned long lon
 // Hilbert keys of particles idx[0..n) (idx == nullptr: identity).
 template <class P>
 __global__ void hilbert_k(const P *x, const P *y, const P *z, const unsigned *idx, long long n,
-                          const unsigned long long *mm, unsigned *keys, unsigned *vals) {
+                          const unsigned long long *mm, unsigned *keys, unsigned *vals,
+                          P *packed) {
     long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     unsigned i = idx ? idx[s] : (unsigned)s;
     double v[3] = {(double)x[i], (double)y[i], (double)z[i]};
     keys[s] = hilbert_of(v, mm);
     vals[s] = i;
+    if (packed) {  // (x, y, z, 0) per particle, for one-load gathers after the sort
+        P *d = packed + 4 * (size_t)i;
+        d[0] = x[i];
+        d[1] = y[i];
+        d[2] = z[i];
+        d[3] = P(0);
+    }
+}
+
+// Sorted FP64 copies from the packed (x, y, z, 0) records: one 16-byte load
+// per float particle (two for double).
+template <class P>
+__global__ void gather_packed_k(const P *packed, const unsigned *perm, long long n, double *sx,
+                                double *sy, double *sz) {
+    long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const size_t i = perm[s];
+    if constexpr (sizeof(P) == 4) {
+        const float4 v = *reinterpret_cast<const float4 *>(packed + 4 * i);
+        sx[s] = (double)v.x;
+        sy[s] = (double)v.y;
+        sz[s] = (double)v.z;
+    } else {
+        const double2 a = *reinterpret_cast<const double2 *>(packed + 4 * i);
+        const double2 b = *reinterpret_cast<const double2 *>(packed + 4 * i + 2);
+        sx[s] = (double)a.x;
+        sy[s] = (double)a.y;
+        sz[s] = (double)b.x;
+    }
 }
 
 // ---- batched preparation (one block per particle chunk of one halo) -------
@@ -1238,12 +1268,14 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
     unsigned *qk = ws.alloc_n<unsigned>(np), *qv = ws.alloc_n<unsigned>(np);
     unsigned *qk2 = ws.alloc_n<unsigned>(np), *qv2 = ws.alloc_n<unsigned>(np);
     if (!qk || !qv || !qk2 || !qv2) return hipErrorOutOfMemory;
-    hilbert_k<P><<<nblk(np, 256), 256, 0, s>>>(x, y, z, nullptr, np, mm, qk, qv);
+    P *packed = ws.alloc_n<P>(4 * (size_t)np);
+    if (!packed) return hipErrorOutOfMemory;
+    hilbert_k<P><<<nblk(np, 256), 256, 0, s>>>(x, y, z, nullptr, np, mm, qk, qv, packed);
     PGALF_HIP_TRY(radix_sort_pairs(&qk, &qv, qk2, qv2, np, 30, ws, s));
     double *qx = ws.alloc_n<double>(np), *qy = ws.alloc_n<double>(np),
            *qz = ws.alloc_n<double>(np);
     if (!qx || !qy || !qz) return hipErrorOutOfMemory;
-    gather_sorted_k<P><<<nblk(np, 256), 256, 0, s>>>(x, y, z, qv, np, qx, qy, qz);
+    gather_packed_k<P><<<nblk(np, 256), 256, 0, s>>>(packed, qv, np, qx, qy, qz);
 
     long long nc = np;
     const double *cx = qx, *cy = qy, *cz = qz;
@@ -1266,7 +1298,7 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
         double *sx = ws.alloc_n<double>(nc), *sy = ws.alloc_n<double>(nc),
                *sz = ws.alloc_n<double>(nc);
         if (!ck || !cv || !ck2 || !cv2 || !sx || !sy || !sz) return hipErrorOutOfMemory;
-        hilbert_k<P><<<nblk(nc, 256), 256, 0, s>>>(x, y, z, sel, nc, mm, ck, cv);
+        hilbert_k<P><<<nblk(nc, 256), 256, 0, s>>>(x, y, z, sel, nc, mm, ck, cv, nullptr);
         PGALF_HIP_TRY(radix_sort_pairs(&ck, &cv, ck2, cv2, nc, 30, ws, s));
         gather_sorted_k<P><<<nblk(nc, 256), 256, 0, s>>>(x, y, z, cv, nc, sx, sy, sz);
         cx = sx;
@@ -1510,7 +1542,7 @@ hipError_t knn_density_backward(const P *x, const P *y, const P *z, const P *m, 
     if (!mm || !hk || !hv || !hk2 || !hv2 || !rank) return hipErrorOutOfMemory;
     bbox_init_k<<<1, 64, 0, s>>>(mm, 1);
     bbox_k<P><<<256, 256, 0, s>>>(x, y, z, np, mm);
-    hilbert_k<P><<<nblk(np, 256), 256, 0, s>>>(x, y, z, nullptr, np, mm, hk, hv);
+    hilbert_k<P><<<nblk(np, 256), 256, 0, s>>>(x, y, z, nullptr, np, mm, hk, hv, nullptr);
     PGALF_HIP_TRY(radix_sort_pairs(&hk, &hv, hk2, hv2, np, 30, ws, s));
     inverse_perm_k<<<nblk(np, 256), 256, 0, s>>>(hv, np, rank);
     knn_backward_k<P><<<nblk(np, kBwdBlock), kBwdBlock, 0, s>>>(x, y, z, m, np, nbr, k, weighted,
