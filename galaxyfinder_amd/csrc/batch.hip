@@ -35,6 +35,7 @@ struct pgalf_batch {
     BatchDev bd;
     HaloSeg *d_segs = nullptr;
     PChunk *d_chunks = nullptr;
+    PChunk *d_fine = nullptr;
     unsigned *d_tile_halo = nullptr;
     unsigned *d_slab_tile = nullptr;
     long long *d_goffs = nullptr;  // smoothing groups' grid offsets, group after group
@@ -58,6 +59,7 @@ void free_batch(pgalf_batch *b) {
     if (b->gstream) (void)hipStreamDestroy(b->gstream);
     if (b->d_segs) (void)hipFree(b->d_segs);
     if (b->d_chunks) (void)hipFree(b->d_chunks);
+    if (b->d_fine) (void)hipFree(b->d_fine);
     if (b->d_tile_halo) (void)hipFree(b->d_tile_halo);
     if (b->d_slab_tile) (void)hipFree(b->d_slab_tile);
     if (b->d_goffs) (void)hipFree(b->d_goffs);
@@ -263,7 +265,7 @@ extern "C" pgalf_batch_t *pgalf_batch_create(int nhalo, const long long *off,
     b->nhalo = nhalo;
     b->k = k;
     std::vector<HaloSeg> segs((size_t)nhalo);
-    std::vector<PChunk> chunks;
+    std::vector<PChunk> chunks, fine;
     std::vector<unsigned> tile_halo, slab_tile;
     b->goff.assign((size_t)nhalo + 1, 0);
     bool too_wide = false;
@@ -296,6 +298,13 @@ extern "C" pgalf_batch_t *pgalf_batch_create(int nhalo, const long long *off,
             pc.b1 = std::min(c + (long long)kChunkParticles, off[h + 1]);
             chunks.push_back(pc);
         }
+        for (long long c = off[h]; c < off[h + 1]; c += kFineParticles) {
+            PChunk pc;
+            pc.halo = h;
+            pc.b0 = c;
+            pc.b1 = std::min(c + (long long)kFineParticles, off[h + 1]);
+            fine.push_back(pc);
+        }
     }
     // smoothing groups: one per (grid shape, smoothing factor)
     std::map<std::tuple<int, int, int, double>, std::vector<long long>> gmap;
@@ -321,6 +330,7 @@ extern "C" pgalf_batch_t *pgalf_batch_create(int nhalo, const long long *off,
     hipError_t e = too_wide ? hipErrorInvalidValue : hipSuccess;
     if (e == hipSuccess) e = upload(&b->d_segs, segs);
     if (e == hipSuccess) e = upload(&b->d_chunks, chunks);
+    if (e == hipSuccess) e = upload(&b->d_fine, fine);
     if (e == hipSuccess) e = upload(&b->d_tile_halo, tile_halo);
     if (e == hipSuccess) e = upload(&b->d_slab_tile, slab_tile);
     if (e == hipSuccess) e = upload(&b->d_goffs, goffs);
@@ -339,6 +349,8 @@ extern "C" pgalf_batch_t *pgalf_batch_create(int nhalo, const long long *off,
     b->bd.segs = b->d_segs;
     b->bd.chunks = b->d_chunks;
     b->bd.nchunks = (long long)chunks.size();
+    b->bd.fine = b->d_fine;
+    b->bd.nfine = (long long)fine.size();
     b->bd.tile_halo = b->d_tile_halo;
     b->bd.ntiles = toff;
     b->bd.max_halo_tiles = max_tiles;

@@ -45,6 +45,9 @@ struct PChunk {
     long long b0, b1;
 };
 constexpr int kChunkParticles = 8192;
+// Finer chunks (one workgroup each) for the per-particle gathers of a batch:
+// most halos of a catalogue are far smaller than kChunkParticles.
+constexpr int kFineParticles = 256;
 
 // Device view of a batch: descriptors built on the host once per catalogue.
 struct BatchDev {
@@ -53,6 +56,8 @@ struct BatchDev {
     const HaloSeg *segs = nullptr;  // [nhalo]
     const PChunk *chunks = nullptr; // [nchunks]
     long long nchunks = 0;
+    const PChunk *fine = nullptr;   // [nfine] chunks of <= kFineParticles
+    long long nfine = 0;
     const unsigned *tile_halo = nullptr;  // [ntiles] halo of each deposit tile
     long long ntiles = 0;
     long long max_halo_tiles = 0;

@@ -473,9 +473,7 @@ tsc_readback_k(const P *x, const P *y, const P *z, const HaloSeg *segs, HaloSeg 
     const long mx = (long)g.mx();
     // particle i's coordinates sit at p = i + sh, its result at order[p]
     const long long sh = shift ? shift[ch.halo] : 0;
-    // blockIdx.y: a share of the chunk (batch chunks are split over gridDim.y blocks)
-    for (long long i = ch.b0 + (long long)blockIdx.y * kBlock + threadIdx.x; i < ch.b1;
-         i += (long long)kBlock * gridDim.y) {
+    for (long long i = ch.b0 + threadIdx.x; i < ch.b1; i += kBlock) {
         const long long p = i + sh;
         const long long o = order ? (long long)order[p] : i;
         RAxis a[3];
@@ -550,9 +548,7 @@ tsc_gradback_k(const P *x, const P *y, const P *z, const P *m, const float *gin,
     const float *hF = F + sg.goff, *hA = A + sg.goff;
     const long mx = (long)g.mx();
     const long n[3] = {mx, g.ny, g.nz};
-    // blockIdx.y: a share of the chunk (batch chunks are split over gridDim.y blocks)
-    for (long long p = ch.b0 + (long long)blockIdx.y * kBlock + threadIdx.x; p < ch.b1;
-         p += (long long)kBlock * gridDim.y) {
+    for (long long p = ch.b0 + threadIdx.x; p < ch.b1; p += kBlock) {
         long id[3][3];
         double w[3][3], dw[3][3];
         grad_axis(((double)x[p] - g.x0) / g.h, id[0], w[0], dw[0]);
@@ -645,9 +641,7 @@ peak_climb_k(const P *x, const P *y, const P *z, const HaloSeg *segs, HaloSeg se
     const float *F = den + sg.goff;
     const long long mx = g.mx();
     const double lim[3] = {(double)(g.nx - 2), (double)(g.ny - 2), (double)(g.nz - 2)};
-    // blockIdx.y: a share of the chunk (batch chunks are split over gridDim.y blocks)
-    for (long long p = ch.b0 + (long long)blockIdx.y * kBlock + threadIdx.x; p < ch.b1;
-         p += (long long)kBlock * gridDim.y) {
+    for (long long p = ch.b0 + threadIdx.x; p < ch.b1; p += kBlock) {
         double q[3] = {((double)x[p] - g.x0) / g.h, ((double)y[p] - g.y0) / g.h,
                        ((double)z[p] - g.z0) / g.h};
 #pragma unroll
@@ -825,8 +819,8 @@ hipError_t tsc_deposit(const P *x, const P *y, const P *z, const P *m, const int
 template <class P>
 hipError_t tsc_deposit_batch(const P *x, const P *y, const P *z, const P *m, const BatchDev &b,
                              float *den, Workspace &ws, hipStream_t s) {
-    static_assert(kPerBlock == kChunkParticles && kChunkParticles % kBlock == 0,
-                  "batch chunks are deposit blocks and whole read-back blocks");
+    static_assert(kPerBlock == kChunkParticles && kFineParticles == kBlock,
+                  "batch chunks are deposit blocks; fine chunks are read-back blocks");
     const HaloSeg seg0{};
     return deposit_impl<P>(x, y, z, m, nullptr, PGALF_TYPE_ALL, b.nhalo, b.segs, seg0, b.chunks,
                            b.nchunks, b.tile_halo, b.ntiles, b.max_halo_tiles, b.slab_tile,
@@ -848,11 +842,11 @@ template <class P>
 hipError_t tsc_readback_batch(const P *x, const P *y, const P *z, const BatchDev &b,
                               const float *den, float *out, Workspace &ws, hipStream_t s,
                               const unsigned *order, const long long *shift) {
-    if (b.nchunks <= 0) return hipSuccess;
+    if (b.nfine <= 0) return hipSuccess;
     const HaloSeg seg0{};
-    const dim3 grid((unsigned)b.nchunks, kChunkParticles / kBlock);
-    tsc_readback_k<P><<<grid, kBlock, 0, s>>>(x, y, z, b.segs, seg0, b.chunks, kChunkParticles,
-                                               den, out, ws.status(), order, shift);
+    tsc_readback_k<P><<<(unsigned)b.nfine, kBlock, 0, s>>>(x, y, z, b.segs, seg0, b.fine,
+                                                          kFineParticles, den, out, ws.status(),
+                                                          order, shift);
     return hipGetLastError();
 }
 
@@ -871,11 +865,10 @@ template <class P>
 hipError_t tsc_grad_readback_batch(const P *x, const P *y, const P *z, const P *m,
                                    const float *gin, const BatchDev &b, const float *F,
                                    const float *A, double *acc, Workspace &ws, hipStream_t s) {
-    if (b.nchunks <= 0) return hipSuccess;
+    if (b.nfine <= 0) return hipSuccess;
     const HaloSeg seg0{};
-    const dim3 grid((unsigned)b.nchunks, kChunkParticles / kBlock);
-    tsc_gradback_k<P><<<grid, kBlock, 0, s>>>(
-        x, y, z, m, gin, b.segs, seg0, b.chunks, kChunkParticles, F, A, acc, b.np, ws.status());
+    tsc_gradback_k<P><<<(unsigned)b.nfine, kBlock, 0, s>>>(
+        x, y, z, m, gin, b.segs, seg0, b.fine, kFineParticles, F, A, acc, b.np, ws.status());
     return hipGetLastError();
 }
 
@@ -895,12 +888,11 @@ template <class P>
 hipError_t peak_climb_batch(const P *x, const P *y, const P *z, const BatchDev &b,
                             const float *den, int niter, double step, float *ox, float *oy,
                             float *oz, int *peak, hipStream_t s) {
-    if (b.nchunks <= 0) return hipSuccess;
+    if (b.nfine <= 0) return hipSuccess;
     const HaloSeg seg0{};
-    const dim3 grid((unsigned)b.nchunks, kChunkParticles / kBlock);
-    peak_climb_k<P><<<grid, kBlock, 0, s>>>(x, y, z, b.segs, seg0, b.chunks,
-                                                          kChunkParticles, den, niter, step, ox,
-                                                          oy, oz, peak);
+    peak_climb_k<P><<<(unsigned)b.nfine, kBlock, 0, s>>>(x, y, z, b.segs, seg0, b.fine,
+                                                        kFineParticles, den, niter, step, ox,
+                                                        oy, oz, peak);
     return hipGetLastError();
 }
 
