@@ -661,25 +661,54 @@ peak_climb_k(const P *x, const P *y, const P *z, const HaloSeg *segs, HaloSeg se
 #pragma unroll
         for (int a = 0; a < 3; a++) c[a] = (long)rint(q[a]);
         float fc = F[c[0] + mx * (c[1] + (long long)g.ny * c[2])];
+        const int nx = g.nx, ny = g.ny, nz = g.nz, mxi = (int)mx;
         for (int step_i = 0; step_i < kClimbMax; step_i++) {
             long best[3] = {c[0], c[1], c[2]};
             float fb = fc;
-            for (int dz = -1; dz <= 1; dz++)
-                for (int dy = -1; dy <= 1; dy++)
-                    for (int dx = -1; dx <= 1; dx++) {
-                        const long i = c[0] + dx, j = c[1] + dy, k = c[2] + dz;
-                        if (i < 0 || i >= g.nx || j < 0 || j >= g.ny || k < 0 || k >= g.nz)
-                            continue;
-                        const float f = F[i + mx * (j + (long long)g.ny * k)];
-                        // loop order visits neighbours by increasing flat index:
-                        // a strictly larger value wins, so ties keep the lowest
-                        if (f > fb) {
-                            fb = f;
-                            best[0] = i;
-                            best[1] = j;
-                            best[2] = k;
-                        }
+            if (c[0] >= 1 && c[0] <= nx - 2 && c[1] >= 1 && c[1] <= ny - 2 && c[2] >= 1 &&
+                c[2] <= nz - 2) {
+                // all 26 neighbours inside: their values first (independent
+                // loads, 32-bit offsets from the cell), then the same scan in
+                // flat-index order with the same strict comparison
+                const float *Fc = F + (c[0] + mx * (c[1] + (long long)ny * c[2]));
+                float f[27];
+#pragma unroll
+                for (int dz = -1; dz <= 1; dz++)
+#pragma unroll
+                    for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+                        for (int dx = -1; dx <= 1; dx++)
+                            f[(dz + 1) * 9 + (dy + 1) * 3 + dx + 1] =
+                                Fc[dx + mxi * (dy + ny * dz)];
+                int bi = 13;  // the cell itself
+#pragma unroll
+                for (int t = 0; t < 27; t++)
+                    if (f[t] > fb) {
+                        fb = f[t];
+                        bi = t;
                     }
+                best[0] = c[0] + bi % 3 - 1;
+                best[1] = c[1] + (bi / 3) % 3 - 1;
+                best[2] = c[2] + bi / 9 - 1;
+            } else {
+                for (int dz = -1; dz <= 1; dz++)
+                    for (int dy = -1; dy <= 1; dy++)
+                        for (int dx = -1; dx <= 1; dx++) {
+                            const long i = c[0] + dx, j = c[1] + dy, k = c[2] + dz;
+                            if (i < 0 || i >= nx || j < 0 || j >= ny || k < 0 || k >= nz)
+                                continue;
+                            const float f = F[i + mx * (j + (long long)ny * k)];
+                            // loop order visits neighbours by increasing flat
+                            // index: a strictly larger value wins, so ties keep
+                            // the lowest
+                            if (f > fb) {
+                                fb = f;
+                                best[0] = i;
+                                best[1] = j;
+                                best[2] = k;
+                            }
+                        }
+            }
             if (!(fb > fc)) break;
             c[0] = best[0];
             c[1] = best[1];
