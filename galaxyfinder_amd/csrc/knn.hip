@@ -29,8 +29,9 @@
 //     that some lane's ball still reaches, stages their candidates in LDS,
 //     and every lane tests them against its own k-th key.  Accepted
 //     candidates go to a per-lane LDS buffer; when any lane's buffer may
-//     overflow, all lanes merge it into their sorted top-32 list held in
-//     registers, 16 keys at a time (odd-even sort of 16 + bitonic merge);
+//     overflow, all lanes merge its first 16 keys into their sorted top-32
+//     list held in registers (odd-even sort of 16 + bitonic merge) and move
+//     the rest down; at the end of a list every key is merged;
 //   * keys are (FP64 dist2, uint32 index) compared lexicographically, so the
 //     lists are exact and deterministic regardless of visiting order.
 // Box pruning is conservative: box gaps are formed with the same monotone
@@ -92,6 +93,10 @@ constexpr int kDirect = PGALF_KNN_DIRECT;  // window buckets merged without a bo
 #define PGALF_KNN_WALK_DIRECT 4
 #endif
 constexpr int kWalkDirect = PGALF_KNN_WALK_DIRECT;  // first-list buckets merged the same way
+#ifndef PGALF_KNN_FLUSH_ONE
+#define PGALF_KNN_FLUSH_ONE 1
+#endif
+constexpr bool kFlushOne = PGALF_KNN_FLUSH_ONE != 0;  // mid-scan flushes merge one chunk
 constexpr int kFCap = PGALF_KNN_LISTCAP;  // bucket-list capacity per group
 constexpr int kStackCap = 96;  // depth-first walk stack per group
 constexpr int kFirstScan = PGALF_KNN_FIRSTSCAN;  // the first list is scanned early to tighten the bounds
@@ -693,9 +698,10 @@ __device__ inline Key kth_of(const double (&td)[kTop], const unsigned (&ti)[kTop
 __device__ inline void flush_buffer(double (&td)[kTop], unsigned (&ti)[kTop], int cnt,
                                     const unsigned (*s_bc)[kWave], int lane, double px,
                                     double py, double pz, const double *cx, const double *cy,
-                                    const double *cz, const unsigned *cid) {
+                                    const double *cz, const unsigned *cid,
+                                    int nch = kBufCap / kBuf) {
 #pragma unroll 1
-    for (int ch = 0; ch < kBufCap / kBuf; ch++) {  // not unrolled: bounds live loads
+    for (int ch = 0; ch < nch; ch++) {  // not unrolled: bounds live loads
         if (__ballot(cnt > ch * kBuf) == 0ull) break;
         double bd[kBuf];
         unsigned bi[kBuf];
@@ -859,9 +865,19 @@ __device__ inline void scan_list(QState &st, int *list, int count, bool valid, i
                 const int wm = dbg ? wave_max_i(st.cnt) : 0;
                 if (dbg && lane == 0) atomicAdd(&dbg[14], (unsigned long long)((wm + kBuf - 1) / kBuf));
                 const unsigned long long t_m = dbg_clock(dbg);
-                flush_buffer(st.td, st.ti, st.cnt, s_bc, lane, px, py, pz, cx, cy, cz, cid);
+                if constexpr (kFlushOne) {
+                    // one chunk for every lane; a lane's keys past it move down
+                    flush_buffer(st.td, st.ti, st.cnt, s_bc, lane, px, py, pz, cx, cy, cz, cid, 1);
+                    const int rest = st.cnt > kBuf ? st.cnt - kBuf : 0;
+                    const int wrest = wave_max_i(rest);
+                    for (int t = 0; t < wrest; t++)
+                        if (t < rest) s_bc[t][lane] = s_bc[kBuf + t][lane];
+                    st.cnt = rest;
+                } else {
+                    flush_buffer(st.td, st.ti, st.cnt, s_bc, lane, px, py, pz, cx, cy, cz, cid);
+                    st.cnt = 0;
+                }
                 dbg_cycles(dbg, 11, t_m, lane);
-                st.cnt = 0;
                 st.kth = kth_of(st.td, st.ti, k);
                 s_qk[lane] = valid ? st.kth.d : -1.0;
             }
