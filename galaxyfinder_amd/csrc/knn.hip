@@ -116,8 +116,9 @@ static_assert(kBufCap % kBuf == 0 && kBufCap % kChunkC == 0 && kBufCap >= 24,
 static_assert(kBatch >= 1 && kBatch <= kG, "one group lane tests each bucket of a batch");
 static_assert(kFCap >= 32 + 8 && kFCap % 8 == 0, "list must take the first scan and a round");
 static_assert(kSeedSpan >= kSeedFill && kFirstScan + kG <= kFCap, "seed window and first scan");
-static_assert(kDirect % 2 == 0 && kDirect <= kSeedSpan - kSeedFill && kBuf == 2 * kB,
-              "direct merges take two buckets per 16-key chunk");
+constexpr int kDM = 2 * kB;  // keys per direct-merge chunk (two buckets)
+static_assert(kDirect % 2 == 0 && kDirect <= kSeedSpan - kSeedFill,
+              "direct merges take two buckets per chunk");
 static_assert(kWalkDirect % 2 == 0 && kWalkDirect <= kFirstScan,
               "walk direct merges take two buckets per chunk, from the first list");
 // The walk keeps at most kFan entries of the deepest level and kFan-1 of every
@@ -654,17 +655,18 @@ __device__ inline void bitonic_clean(double (&d)[N], unsigned (&x)[N]) {
     if constexpr (J > 1) bitonic_clean<N, J / 2>(d, x);
 }
 
-// Merge a chunk of kBuf keys into the sorted top-32: sort the chunk, take
-// the elementwise minimum of the top against the chunk reversed and padded
-// with +inf to 32 -- the 32 smallest, as a bitonic sequence -- and clean.
-__device__ inline void merge_top(double (&td)[kTop], unsigned (&ti)[kTop],
-                                 double (&bd)[kBuf], unsigned (&bi)[kBuf]) {
-    oe_sort<kBuf>(bd, bi);
+// Merge a chunk of N keys into the sorted top-32: sort the chunk, take the
+// elementwise minimum of the top against the chunk reversed and padded with
+// +inf to 32 -- the 32 smallest, as a bitonic sequence -- and clean.
+template <int N>
+__device__ inline void merge_top(double (&td)[kTop], unsigned (&ti)[kTop], double (&bd)[N],
+                                 unsigned (&bi)[N]) {
+    oe_sort<N>(bd, bi);
 #pragma unroll
-    for (int j = 0; j < kBuf; j++) {
-        double cd = bd[kBuf - 1 - j];
-        unsigned ci = bi[kBuf - 1 - j];
-        constexpr int o = kTop - kBuf;  // reversed buffer meets the top's tail
+    for (int j = 0; j < N; j++) {
+        double cd = bd[N - 1 - j];
+        unsigned ci = bi[N - 1 - j];
+        constexpr int o = kTop - N;  // reversed buffer meets the top's tail
         bool take = klt(cd, ci, td[o + j], ti[o + j]);
         td[o + j] = take ? cd : td[o + j];
         ti[o + j] = take ? ci : ti[o + j];
@@ -1072,10 +1074,10 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
         const int ndir = nlo + (d1 > f0 + kSeedFill ? (int)(d1 - f0 - kSeedFill) : 0);
 #pragma unroll 1
         for (int c = 0; c < kDirect / 2; c++) {
-            double bd[kBuf];
-            unsigned bi[kBuf];
+            double bd[kDM];
+            unsigned bi[kDM];
 #pragma unroll
-            for (int t = 0; t < kBuf; t++) {
+            for (int t = 0; t < kDM; t++) {
                 const int j = 2 * c + t / kB;  // j-th direct bucket
                 const long long b = j < nlo ? d0 + j : f0 + kSeedFill + (j - nlo);
                 const long long e = b * kB + t % kB;
@@ -1186,10 +1188,10 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
                     skip = lcnt < kWalkDirect ? lcnt : kWalkDirect;
 #pragma unroll 1
                     for (int c = 0; c < kWalkDirect / 2; c++) {
-                        double bd[kBuf];
-                        unsigned bi[kBuf];
+                        double bd[kDM];
+                        unsigned bi[kDM];
 #pragma unroll
-                        for (int t = 0; t < kBuf; t++) {
+                        for (int t = 0; t < kDM; t++) {
                             const int j = 2 * c + t / kB;
                             bd[t] = inf;
                             bi[t] = 0xffffffffu;
