@@ -104,7 +104,14 @@ constexpr int kLevelShift = 5;  // stack entry = node << 5 | level
 constexpr int kBatch = PGALF_KNN_BATCH;  // buckets staged in LDS per scan batch
 constexpr int kStage = kBatch * kB;  // staged candidates per group
 constexpr int kStageStride = 2 * kStage + 1;  // 16-byte units per group: groups hit other banks
-constexpr int kRings = 8;    // distance classes of the near-first bucket order
+#ifndef PGALF_KNN_RINGS
+#define PGALF_KNN_RINGS 4
+#endif
+#ifndef PGALF_KNN_RING_STEPS
+#define PGALF_KNN_RING_STEPS 2
+#endif
+constexpr int kRings = PGALF_KNN_RINGS;  // distance classes of the near-first bucket order
+constexpr int kRingSteps = PGALF_KNN_RING_STEPS;  // classes per R (the group's largest ball)
 constexpr int kRingShift = 28;  // list entry = bucket | ring << 28
 constexpr int kBucketMask = (1 << kRingShift) - 1;
 static_assert(kG == kB, "a scan batch loads one candidate of each bucket per group lane");
@@ -1153,9 +1160,10 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
             if (dbg && lane == 0) atomicAdd(&dbg[2], 1ull);  // walk rounds
             if (cl == 0) {
                 int pos = lcnt + __popc(m & ((1u << gl) - 1u));
-                // distance class: gap from the group centre in R/4 steps,
-                // floor(4 sqrt(cg / R2)) capped at kRings - 1, by comparisons
-                const double c16 = 16.0 * cg;
+                // distance class: gap from the group centre in R/kRingSteps
+                // steps, floor(kRingSteps sqrt(cg / R2)) capped at kRings - 1,
+                // by comparisons
+                const double c16 = (double)(kRingSteps * kRingSteps) * cg;
                 int ring = 0;
 #pragma unroll
                 for (int j = 1; j < kRings; j++) ring += c16 >= (double)(j * j) * R2 ? 1 : 0;
