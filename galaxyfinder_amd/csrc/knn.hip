@@ -538,26 +538,38 @@ __global__ void gather_sorted_k(const P *x, const P *y, const P *z, const unsign
     sz[s] = (double)z[i];
 }
 
+// One lane per candidate slot (coalesced loads); the kB lanes of a bucket
+// reduce its box by shuffles.  fmin/fmax skip NaN pad slots.  Launch with a
+// multiple of kB threads per block covering nb * kB slots.
 __global__ void leaf_boxes_k(const double *sx, const double *sy, const double *sz, long long n,
                              long long nb, Box *boxes) {
-    long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb) return;
-    Box bx;
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        bx.lo[a] = DBL_MAX;
-        bx.hi[a] = -DBL_MAX;
-    }
-    long long e = min(n, (b + 1) * kB);
-    for (long long i = b * kB; i < e; i++) {
-        double v[3] = {sx[i], sy[i], sz[i]};
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    if (i < n) {
+        const double v[3] = {sx[i], sy[i], sz[i]};
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            bx.lo[a] = fmin(bx.lo[a], v[a]);
-            bx.hi[a] = fmax(bx.hi[a], v[a]);
+            lo[a] = fmin(lo[a], v[a]);
+            hi[a] = fmax(hi[a], v[a]);
         }
     }
-    boxes[b] = bx;
+#pragma unroll
+    for (int o = 1; o < kB; o <<= 1)
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            lo[a] = fmin(lo[a], __shfl_xor(lo[a], o, kB));
+            hi[a] = fmax(hi[a], __shfl_xor(hi[a], o, kB));
+        }
+    const long long b = i / kB;
+    if (i % kB == 0 && b < nb) {
+        Box bx;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            bx.lo[a] = lo[a];
+            bx.hi[a] = hi[a];
+        }
+        boxes[b] = bx;
+    }
 }
 
 __global__ void parent_boxes_k(const Box *child, long long nchild, Box *parent,
@@ -1352,7 +1364,7 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
     long long total = level_offset(nb, h0.nlev);
     Box *boxes = ws.alloc_n<Box>((size_t)total);
     if (!boxes) return hipErrorOutOfMemory;
-    leaf_boxes_k<<<nblk(nb, 256), 256, 0, s>>>(cx, cy, cz, nc, nb, boxes);
+    leaf_boxes_k<<<nblk(nb * kB, 256), 256, 0, s>>>(cx, cy, cz, nc, nb, boxes);
     for (int l = 1; l < h0.nlev; l++) {
         long long nl = level_count(nb, l);
         parent_boxes_k<<<nblk(nl, 256), 256, 0, s>>>(boxes + level_offset(nb, l - 1),
@@ -1746,7 +1758,7 @@ hipError_t knn_batch(const P *x, const P *y, const P *z, const P *m, const Batch
     gather_padded_k<P><<<nblk(np, 256), 256, 0, s>>>(x, y, z, hv, k2, halos, np, sx, sy, sz, sid);
     pad_slots_k<<<nblk((long long)plan.nhalo * kB, 256), 256, 0, s>>>(halos, plan.nhalo, sx, sy,
                                                                       sz, sid);
-    leaf_boxes_k<<<nblk(plan.nb_total, 256), 256, 0, s>>>(sx, sy, sz, npad, plan.nb_total,
+    leaf_boxes_k<<<nblk(plan.nb_total * kB, 256), 256, 0, s>>>(sx, sy, sz, npad, plan.nb_total,
                                                           boxes);
     long long lb = plan.nb_total;
     for (long long le : plan.level_begin) {
