@@ -14,6 +14,7 @@ constexpr int kThreads = 256;
 constexpr int kItems = 8;
 constexpr int kTile = kThreads * kItems;  // 2048
 constexpr int kWaves = kThreads / 64;
+constexpr long long kSmallScan = 8LL * kTile;  // scanned by one block in one launch
 
 __device__ inline unsigned wave_incl_scan(unsigned v, int lane) {
 #pragma unroll
@@ -58,9 +59,11 @@ __global__ void __launch_bounds__(kThreads) scan_reduce_k(const unsigned *in, lo
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-// Single block: exclusive scan of nb block sums in place, chunk by chunk.
-__global__ void __launch_bounds__(kThreads) scan_top_k(unsigned *bsum, long long nb,
-                                                       unsigned *total) {
+// Single block: exclusive scan of nb values, chunk by chunk (in may equal
+// out: each thread reads its values before writing them).  Scans the block
+// sums of a multi-block scan, and short arrays whole.
+__global__ void __launch_bounds__(kThreads) scan_top_k(const unsigned *in, unsigned *out,
+                                                       long long nb, unsigned *total) {
     unsigned carry = 0;
     for (long long c0 = 0; c0 < nb; c0 += kTile) {
         unsigned v[kItems];
@@ -68,7 +71,7 @@ __global__ void __launch_bounds__(kThreads) scan_top_k(unsigned *bsum, long long
 #pragma unroll
         for (int r = 0; r < kItems; r++) {
             long long i = c0 + (long long)threadIdx.x * kItems + r;
-            v[r] = i < nb ? bsum[i] : 0u;
+            v[r] = i < nb ? in[i] : 0u;
             s += v[r];
         }
         unsigned tot;
@@ -76,7 +79,7 @@ __global__ void __launch_bounds__(kThreads) scan_top_k(unsigned *bsum, long long
 #pragma unroll
         for (int r = 0; r < kItems; r++) {
             long long i = c0 + (long long)threadIdx.x * kItems + r;
-            if (i < nb) bsum[i] = off;
+            if (i < nb) out[i] = off;
             off += v[r];
         }
         carry += tot;
@@ -242,11 +245,15 @@ hipError_t fill_u64(unsigned long long *p, long long n, unsigned long long v, hi
 hipError_t exclusive_scan_u32(const unsigned *in, unsigned *out, long long n,
                               unsigned *total, Workspace &ws, hipStream_t s) {
     if (n <= 0) return total ? fill_u32(total, 1, 0u, s) : hipSuccess;
+    if (n <= kSmallScan) {  // one launch
+        scan_top_k<<<1, kThreads, 0, s>>>(in, out, n, total);
+        return hipGetLastError();
+    }
     long long nb = (n + kTile - 1) / kTile;
     unsigned *bsum = ws.alloc_n<unsigned>((size_t)nb);
     if (!bsum) return hipErrorOutOfMemory;
     scan_reduce_k<<<dim3((unsigned)nb), dim3(kThreads), 0, s>>>(in, n, bsum);
-    scan_top_k<<<1, kThreads, 0, s>>>(bsum, nb, total);
+    scan_top_k<<<1, kThreads, 0, s>>>(bsum, bsum, nb, total);
     scan_apply_k<<<dim3((unsigned)nb), dim3(kThreads), 0, s>>>(in, out, n, bsum);
     return hipGetLastError();
 }
