@@ -1298,8 +1298,7 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
     unsigned long long *dbg = debug ? ws.alloc_n<unsigned long long>(kDbgN) : nullptr;
     if (!mm || (debug && !dbg)) return hipErrorOutOfMemory;
     if (dbg) PGALF_HIP_TRY(hipMemsetAsync(dbg, 0, kDbgN * sizeof(unsigned long long), s));
-    PGALF_HIP_TRY(hipMemsetAsync(mm, 0xff, 3 * sizeof(unsigned long long), s));
-    PGALF_HIP_TRY(hipMemsetAsync(mm + 3, 0x00, 3 * sizeof(unsigned long long), s));
+    bbox_init_k<<<1, 64, 0, s>>>(mm, 1);
     bbox_k<P><<<256, 256, 0, s>>>(x, y, z, np, mm);
 
     // queries: all particles, Hilbert order

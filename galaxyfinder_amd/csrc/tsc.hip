@@ -357,6 +357,19 @@ __global__ void tsc_finalize_k(const unsigned *tile_halo, const unsigned *nunits
         dst[l] = (float)((double)(long long)src[l] * inv);
 }
 
+// Zeroes up to five word spans in one launch (the deposit's counters).
+struct ZeroSpans {
+    unsigned *p[5];
+    long long n[5];
+};
+__global__ void zero_spans_k(ZeroSpans z) {
+    const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long st = (long long)gridDim.x * blockDim.x;
+#pragma unroll
+    for (int k = 0; k < 5; k++)
+        for (long long i = i0; i < z.n[k]; i += st) z.p[k][i] = 0u;
+}
+
 // Tiles with bin entries: slot[t] = 1 (ranked by the exclusive scan after).
 __global__ void tsc_slot_flags_k(const unsigned *tcount, long long ntiles, unsigned *slot) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -764,11 +777,13 @@ hipError_t deposit_impl(const P *x, const P *y, const P *z, const P *m, const in
     if (!tcount || !tcursor || !tstart || !nunits || !ustart || !multi || !misc || !nsel ||
         !maxm || !scale)
         return hipErrorOutOfMemory;
-    PGALF_HIP_TRY(fill_u32(tcount, ntiles, 0u, s));
-    PGALF_HIP_TRY(fill_u32(tcursor, ntiles, 0u, s));
-    PGALF_HIP_TRY(fill_u32(misc, 8, 0u, s));
-    PGALF_HIP_TRY(fill_u32(nsel, nhalo, 0u, s));
-    PGALF_HIP_TRY(fill_u64(maxm, nhalo, 0ull, s));
+    {
+        ZeroSpans z{{tcount, tcursor, misc, nsel, reinterpret_cast<unsigned *>(maxm)},
+                    {ntiles, ntiles, 8, nhalo, 2LL * nhalo}};
+        const long long mx = std::max<long long>(ntiles, 2LL * nhalo);
+        const unsigned zb = (unsigned)std::min<long long>((mx + 255) / 256, 1024);
+        zero_spans_k<<<zb, 256, 0, s>>>(z);
+    }
     size_t lds_count = use_lds ? sizeof(unsigned) * max_halo_tiles : 0;
     if (nchunks > 0)
         tsc_count_k<P><<<dim3((unsigned)nchunks), dim3(kBlock), lds_count, s>>>(
