@@ -1175,10 +1175,10 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
                 // distance class: gap from the group centre in R/kRingSteps
                 // steps, floor(kRingSteps sqrt(cg / R2)) capped at kRings - 1,
                 // by comparisons
-                const double c16 = (double)(kRingSteps * kRingSteps) * cg;
+                const double cs = (double)(kRingSteps * kRingSteps) * cg;
                 int ring = 0;
 #pragma unroll
-                for (int j = 1; j < kRings; j++) ring += c16 >= (double)(j * j) * R2 ? 1 : 0;
+                for (int j = 1; j < kRings; j++) ring += cs >= (double)(j * j) * R2 ? 1 : 0;
                 if (pass) list[pos] = (int)child | (ring << kRingShift);
                 lcnt += npass;
             } else {
