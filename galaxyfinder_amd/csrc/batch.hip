@@ -20,6 +20,7 @@
 // can be captured into a HIP graph and replayed.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <tuple>
 #include <vector>
@@ -188,7 +189,8 @@ int run_maybe_graph(pgalf_batch *b, const std::vector<const void *> &key, int us
             (void)hipGraphExecDestroy(b->gexec);
             b->gexec = nullptr;
         }
-        e = b->ws->begin(s);
+        WsCall call(*b->ws, s);
+        e = call.e;
         if (e == hipSuccess) e = seq(s);
         return e == hipSuccess ? ok() : hip_err(e, where);
     }
@@ -227,6 +229,7 @@ int run_maybe_graph(pgalf_batch *b, const std::vector<const void *> &key, int us
         b->gkey = key;
     }
     if (e == hipSuccess) e = hipGraphLaunch(b->gexec, b->gstream);
+    if (e == hipSuccess) b->ws->end(b->gstream);  // the replay uses the arena
     if (e == hipSuccess) e = hipEventRecord(b->ev_out, b->gstream);
     if (e == hipSuccess) e = hipStreamWaitEvent(s, b->ev_out, 0);
     return e == hipSuccess ? ok() : hip_err(e, where);
@@ -395,12 +398,15 @@ extern "C" int pgalf_batch_run_grad(pgalf_batch_t *b, const float *x, const floa
     std::lock_guard<std::mutex> lk(g_api_mu);
     GradArgs a{x, y, z, m, w_tsc, w_knn, rho_mode, den, rho_tsc, rho_knn, nbr,
                gx, gy, gz, gm, climb_iters, climb_step, cx, cy, cz, peak};
-    // the iteration count and step are baked into a captured graph: part of the key
+    // the iteration count and step are baked into a captured graph: part of
+    // the key (the step by its bit pattern, so any two distinct steps differ)
+    unsigned step_bits = 0;
+    memcpy(&step_bits, &climb_step, sizeof(step_bits));
     const std::vector<const void *> key = {
         x, y, z, m, w_tsc, w_knn, den, rho_tsc, nbr, rho_knn, gx, gy, gz, gm, cx, cy, cz, peak,
         reinterpret_cast<const void *>((long long)rho_mode),
         reinterpret_cast<const void *>((long long)climb_iters),
-        reinterpret_cast<const void *>((long long)((double)climb_step * 1e6))};
+        reinterpret_cast<const void *>((unsigned long long)step_bits)};
     return run_maybe_graph(b, key, use_graph, (hipStream_t)stream, where,
                            [&](hipStream_t s) { return run_grad_sequence(b, a, s); });
 }
@@ -410,8 +416,8 @@ extern "C" int pgalf_batch_status(pgalf_batch_t *b, void *stream) {
     std::lock_guard<std::mutex> lk(g_api_mu);
     hipStream_t s = (hipStream_t)stream;
     if (!b->ws->status()) {
-        hipError_t e = b->ws->begin(s);
-        if (e != hipSuccess) return hip_err(e, "pgalf_batch_status");
+        WsCall call(*b->ws, s);
+        if (call.e != hipSuccess) return hip_err(call.e, "pgalf_batch_status");
     }
     return consume_status(*b->ws, s, "pgalf_batch_status");
 }

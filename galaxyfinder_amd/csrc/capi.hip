@@ -20,6 +20,21 @@ namespace pgalf {
 Workspace::~Workspace() {
     for (auto &c : chunks_) hipFree(c.ptr);
     if (status_) hipFree(status_);
+    if (done_) hipEventDestroy(done_);
+}
+
+static bool capturing(hipStream_t s) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
+void Workspace::end(hipStream_t s) {
+    // under stream capture the ordering is the graph's own
+    if (!done_ || capturing(s)) return;
+    if (hipEventRecord(done_, s) == hipSuccess) {
+        last_ = s;
+        has_last_ = true;
+    }
 }
 
 hipError_t Workspace::add_chunk(size_t bytes) {
@@ -33,6 +48,9 @@ hipError_t Workspace::add_chunk(size_t bytes) {
 }
 
 hipError_t Workspace::begin(hipStream_t s) {
+    if (!done_) PGALF_HIP_TRY(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
+    // the previous call ran on another stream: wait for its kernels
+    if (has_last_ && s != last_ && !capturing(s)) PGALF_HIP_TRY(hipStreamWaitEvent(s, done_, 0));
     if (!status_) {
         PGALF_HIP_TRY(hipMalloc(&status_, sizeof(unsigned)));
         PGALF_HIP_TRY(hipMemsetAsync(status_, 0, sizeof(unsigned), s));
@@ -294,7 +312,8 @@ extern "C" void assign_density_TSC(pgalf_particle_t *bp, int np, float *den, int
     std::lock_guard<std::mutex> lk(g_api_mu);
     hipStream_t s = nullptr;
     Workspace &ws = workspace_for_current_device();
-    hipError_t e = ws.begin(s);
+    WsCall call(ws, s);
+    hipError_t e = call.e;
     Grid g = make_grid(nx, ny, nz, Xmin, Ymin, Zmin, cellsize);
     DeviceParticles d{};
     float *dden = nullptr;
@@ -323,7 +342,8 @@ extern "C" void gaussian_Smoothing(float *denGrid, int nx, int ny, int nz, doubl
     std::lock_guard<std::mutex> lk(g_api_mu);
     hipStream_t s = nullptr;
     Workspace &ws = workspace_for_current_device();
-    hipError_t e = ws.begin(s);
+    WsCall call(ws, s);
+    hipError_t e = call.e;
     Grid g = make_grid(nx, ny, nz, 0, 0, 0, cellsize);
     float *dden = nullptr;
     if (e == hipSuccess) {
@@ -352,7 +372,8 @@ extern "C" void findDen(pgalf_particle_t *bp, int np, float *densph, double xmin
     std::lock_guard<std::mutex> lk(g_api_mu);
     hipStream_t s = nullptr;
     Workspace &ws = workspace_for_current_device();
-    hipError_t e = ws.begin(s);
+    WsCall call(ws, s);
+    hipError_t e = call.e;
     Grid g = make_grid(nx, ny, nz, xmin, ymin, zmin, cellsize);
     DeviceParticles d{};
     float *dden = nullptr, *dout = nullptr;
@@ -381,7 +402,8 @@ static int host_knn(pgalf_particle_t *bp, int np, int k, long long *nearindex, f
     if (k > np) return set_err(PGALF_ERR_SHORT_LIST, "%s: Numnear %d > np %d", where, k, np);
     hipStream_t s = nullptr;
     Workspace &ws = workspace_for_current_device();
-    hipError_t e = ws.begin(s);
+    WsCall call(ws, s);
+    hipError_t e = call.e;
     DeviceParticles d{};
     long long *dnbr = nullptr;
     float *drho = nullptr;
@@ -489,7 +511,8 @@ extern "C" int pgalf_lag_density_stage(pgalf_particle_t *bp, int np, int Numnear
     if (!check_grid(nx, ny, nz)) return set_err(PGALF_ERR_ARG, "%s: grid too large", where);
     hipStream_t s = nullptr;
     Workspace &ws = workspace_for_current_device();
-    hipError_t e = ws.begin(s);
+    WsCall call(ws, s);
+    hipError_t e = call.e;
     Grid g = make_grid(nx, ny, nz, xmin, ymin, zmin, cellsize);
     DeviceParticles d{};
     float *dden = nullptr, *drho = nullptr;
@@ -545,7 +568,8 @@ extern "C" int pgalf_dev_tsc(const float *x, const float *y, const float *z, con
     std::lock_guard<std::mutex> lk(g_api_mu);
     hipStream_t s = (hipStream_t)stream;
     Workspace &ws = workspace_for_current_device();
-    hipError_t e = ws.begin(s);
+    WsCall call(ws, s);
+    hipError_t e = call.e;
     timing_mark("tsc", s, false);
     if (e == hipSuccess) e = tsc_deposit<float>(x, y, z, m, type, np, ptype, grid_from(g), den, ws, s);
     timing_mark("tsc", s, true);
@@ -558,7 +582,8 @@ extern "C" int pgalf_dev_smooth(float *den, const pgalf_grid_t *g, float RG, voi
     std::lock_guard<std::mutex> lk(g_api_mu);
     hipStream_t s = (hipStream_t)stream;
     Workspace &ws = workspace_for_current_device();
-    hipError_t e = ws.begin(s);
+    WsCall call(ws, s);
+    hipError_t e = call.e;
     timing_mark("smooth", s, false);
     if (e == hipSuccess) e = gaussian_smooth(den, grid_from(g), RG, ws, s);
     timing_mark("smooth", s, true);
@@ -573,7 +598,8 @@ extern "C" int pgalf_dev_findden(const float *x, const float *y, const float *z,
     std::lock_guard<std::mutex> lk(g_api_mu);
     hipStream_t s = (hipStream_t)stream;
     Workspace &ws = workspace_for_current_device();
-    hipError_t e = ws.begin(s);
+    WsCall call(ws, s);
+    hipError_t e = call.e;
     timing_mark("findden", s, false);
     if (e == hipSuccess) e = tsc_readback<float>(x, y, z, np, grid_from(g), den, densph, ws, s);
     timing_mark("findden", s, true);
@@ -589,7 +615,8 @@ extern "C" int pgalf_dev_knn(const float *x, const float *y, const float *z, con
     std::lock_guard<std::mutex> lk(g_api_mu);
     hipStream_t s = (hipStream_t)stream;
     Workspace &ws = workspace_for_current_device();
-    hipError_t e = ws.begin(s);
+    WsCall call(ws, s);
+    hipError_t e = call.e;
     timing_mark("knn", s, false);
     if (e == hipSuccess)
         e = knn<float>(x, y, z, m, np, type, type ? cand_ptype : PGALF_TYPE_ALL, k, nbr,
@@ -609,7 +636,8 @@ extern "C" int pgalf_dev_density_stage(const float *x, const float *y, const flo
     std::lock_guard<std::mutex> lk(g_api_mu);
     hipStream_t s = (hipStream_t)stream;
     Workspace &ws = workspace_for_current_device();
-    hipError_t e = ws.begin(s);
+    WsCall call(ws, s);
+    hipError_t e = call.e;
     Grid gg = grid_from(g);
     timing_mark("stage", s, false);
     timing_mark("tsc", s, false);
@@ -650,7 +678,8 @@ extern "C" int pgalf_dev_density_stage_backward(
     std::lock_guard<std::mutex> lk(g_api_mu);
     hipStream_t s = (hipStream_t)stream;
     Workspace &ws = workspace_for_current_device();
-    hipError_t e = ws.begin(s);
+    WsCall call(ws, s);
+    hipError_t e = call.e;
     const Grid gg = grid_from(g);
     double *acc = ws.alloc_n<double>(4 * (size_t)np);
     if (!acc) e = hipErrorOutOfMemory;
@@ -690,7 +719,8 @@ extern "C" int pgalf_dev_peak_climb(const float *x, const float *y, const float 
     std::lock_guard<std::mutex> lk(g_api_mu);
     hipStream_t s = (hipStream_t)stream;
     Workspace &ws = workspace_for_current_device();
-    hipError_t e = ws.begin(s);
+    WsCall call(ws, s);
+    hipError_t e = call.e;
     if (e == hipSuccess)
         e = peak_climb<float>(x, y, z, np, grid_from(g), den, niter, (double)step, ox, oy, oz,
                               peak, s);
@@ -702,8 +732,8 @@ extern "C" int pgalf_dev_status(void *stream) {
     hipStream_t s = (hipStream_t)stream;
     Workspace &ws = workspace_for_current_device();
     if (!ws.status()) {
-        hipError_t e = ws.begin(s);
-        if (e != hipSuccess) return hip_err(e, "pgalf_dev_status");
+        WsCall call(ws, s);
+        if (call.e != hipSuccess) return hip_err(call.e, "pgalf_dev_status");
     }
     return consume_status(ws, s, "pgalf_dev_status");
 }

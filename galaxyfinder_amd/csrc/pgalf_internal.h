@@ -74,18 +74,26 @@ enum : unsigned {
 // Grow-only device arena.  begin() resets the bump pointer; alloc() never
 // moves earlier blocks (extra chunks are added and coalesced at the next
 // begin(), after the stream has drained).
+//
+// Calls may come on different streams.  Every call ends with end(s), which
+// records an event on its stream; a call on another stream first makes its
+// stream wait for that event, so its kernels reuse the arena only after the
+// previous call's kernels are done with it (calls on one stream are ordered
+// by the stream itself).
 class Workspace {
   public:
     // A bump position; release() returns the arena to it, so a stage's
-    // scratch is reused by the stages after it.  All users run on one stream
-    // (or one captured graph), so later kernels that reuse the memory run
-    // after the earlier ones have finished with it.
+    // scratch is reused by the stages after it.  Within one call every user
+    // runs on one stream (or one captured graph), so later kernels that reuse
+    // the memory run after the earlier ones have finished with it.
     struct Mark {
         size_t chunk, used;
     };
     explicit Workspace(int device) : device_(device) {}
     ~Workspace();
     hipError_t begin(hipStream_t s);
+    // The call on stream s has queued all its work (see the class comment).
+    void end(hipStream_t s);
     void *alloc(size_t bytes);
     template <class T> T *alloc_n(size_t n) { return static_cast<T *>(alloc(n * sizeof(T))); }
     Mark mark() const { return chunks_.empty() ? Mark{0, 0} : Mark{cur_, chunks_[cur_].used}; }
@@ -101,7 +109,21 @@ class Workspace {
     size_t high_water_ = 0;  // largest peak of any call
     unsigned *status_ = nullptr;
     int device_;
+    hipEvent_t done_ = nullptr;  // recorded by end(): the last call's work
+    hipStream_t last_ = nullptr;
+    bool has_last_ = false;
     hipError_t add_chunk(size_t bytes);
+};
+
+// begin() on construction, end() when the entry point returns.
+struct WsCall {
+    Workspace &ws;
+    hipStream_t s;
+    hipError_t e;
+    WsCall(Workspace &w, hipStream_t st) : ws(w), s(st), e(w.begin(st)) {}
+    ~WsCall() { ws.end(s); }
+    WsCall(const WsCall &) = delete;
+    WsCall &operator=(const WsCall &) = delete;
 };
 
 // Releases a stage's workspace scratch when the stage returns (its kernels

@@ -194,11 +194,16 @@ def dev_smooth(den, grid, RG):
     _lib.check(lib.pgalf_dev_smooth(_dptr(den), ctypes.byref(grid), float(RG), _stream()))
 
 
+def _grid_floats(grid):
+    return 2 * (grid.nx // 2 + 1) * grid.ny * grid.nz
+
+
 def dev_findden(x, y, z, grid, den, out):
     torch = _torch()
     n = x.numel()
     for t, w in ((x, "x"), (y, "y"), (z, "z"), (out, "out")):
         _check_dev(t, torch.float32, n, w)
+    _check_dev(den, torch.float32, _grid_floats(grid), "den")
     lib = _lib.load()
     _lib.check(lib.pgalf_dev_findden(_dptr(x), _dptr(y), _dptr(z), n, ctypes.byref(grid),
                                      _dptr(den), _dptr(out), _stream()))
@@ -214,6 +219,10 @@ def dev_knn(x, y, z, k, m=None, nbr=None, rho=None, rho_mode=0, types=None,
         _check_dev(nbr, torch.int64, n * k, "nbr")
     if rho is not None:
         _check_dev(rho, torch.float32, n, "rho")
+    if rho_mode == 2 or m is not None:
+        _check_dev(m, torch.float32, n, "m")
+    if types is not None:
+        _check_dev(types, torch.int32, n, "types")
     lib = _lib.load()
     _lib.check(lib.pgalf_dev_knn(_dptr(x), _dptr(y), _dptr(z), _dptr(m), _dptr(types), n,
                                  int(cand_ptype), int(k), _dptr(nbr), _dptr(rho),
@@ -226,6 +235,12 @@ def dev_density_stage(x, y, z, m, grid, RG, k, den, rho_tsc=None, nbr=None, rho_
     n = x.numel()
     for t, w in ((x, "x"), (y, "y"), (z, "z"), (m, "m")):
         _check_dev(t, torch.float32, n, w)
+    _check_dev(den, torch.float32, _grid_floats(grid), "den")
+    for t, w in ((rho_tsc, "rho_tsc"), (rho_knn, "rho_knn")):
+        if t is not None:
+            _check_dev(t, torch.float32, n, w)
+    if nbr is not None:
+        _check_dev(nbr, torch.int64, n * k, "nbr")
     lib = _lib.load()
     _lib.check(lib.pgalf_dev_density_stage(_dptr(x), _dptr(y), _dptr(z), _dptr(m), n,
                                            ctypes.byref(grid), float(RG), int(k), _dptr(den),
@@ -334,6 +349,9 @@ class Batch:
         for t, w in ((w_tsc, "w_tsc"), (w_knn, "w_knn")):
             if t is not None:
                 _check_dev(t, torch.float32, n, w)
+        if out is None and graph:
+            # a graph is captured per set of pointers: reuse one output set
+            out = getattr(self, "_grad_out", None)
         if out is None:
             f32 = dict(dtype=torch.float32, device=dev)
             out = dict(den=torch.empty(self.grid_floats, **f32),
@@ -344,6 +362,15 @@ class Batch:
                        cx=torch.empty(n, **f32), cy=torch.empty(n, **f32),
                        cz=torch.empty(n, **f32),
                        peak=torch.empty(n, dtype=torch.int32, device=dev))
+            if graph:
+                self._grad_out = out
+        shapes = dict(den=(torch.float32, self.grid_floats), rho_tsc=(torch.float32, n),
+                      rho_knn=(torch.float32, n), nbr=(torch.int64, n * self.k),
+                      gx=(torch.float32, n), gy=(torch.float32, n), gz=(torch.float32, n),
+                      gm=(torch.float32, n), cx=(torch.float32, n), cy=(torch.float32, n),
+                      cz=(torch.float32, n), peak=(torch.int32, n))
+        for key, (dt, cnt) in shapes.items():
+            _check_dev(out.get(key), dt, cnt, "out[%r]" % key)
         o = out
         lib = _lib.load()
         _lib.check(lib.pgalf_batch_run_grad(
@@ -378,7 +405,7 @@ def dev_peak_climb(x, y, z, grid, den, niter=5, step=0.5):
     n = x.numel()
     for t, w in ((x, "x"), (y, "y"), (z, "z")):
         _check_dev(t, torch.float32, n, w)
-    _check_dev(den, torch.float32, 2 * (grid.nx // 2 + 1) * grid.ny * grid.nz, "den")
+    _check_dev(den, torch.float32, _grid_floats(grid), "den")
     ox, oy, oz = (torch.empty(n, dtype=torch.float32, device=x.device) for _ in range(3))
     peak = torch.empty(n, dtype=torch.int32, device=x.device)
     lib = _lib.load()

@@ -110,7 +110,10 @@ typedef struct pgalf_grid {
 
 /* All pointers are device pointers; `stream` is a hipStream_t (NULL for the
  * default stream).  Calls are stream-ordered; their status is that of the
- * launch.  Kernel-detected conditions (out-of-grid particles, short lists)
+ * launch.  Calls share one scratch arena per device: a call on a stream other
+ * than the previous call's first makes its stream wait for the previous
+ * call's kernels (an event), so calls on different streams are serialised on
+ * the device rather than overlapped.  Kernel-detected conditions (out-of-grid particles, short lists)
  * are collected with pgalf_dev_status().  Grids that a call writes (`den` of
  * pgalf_dev_tsc, pgalf_dev_smooth, pgalf_dev_density_stage and the batch
  * runs) must be 8-byte aligned: they are written as float pairs

@@ -241,11 +241,17 @@ def test_sph_density_uniform_lattice():
 
 
 # ---- against the reference build (built here only) ----------------------------
-ref = O.ref_lib()
+@pytest.fixture(scope="module")
+def ref():
+    """The reference-source build oracle/_ref, loaded only by the tests that
+    use it (not at collection time)."""
+    R = O.ref_lib()
+    if R is None:
+        pytest.skip("reference build oracle/_ref absent")
+    return R
 
 
-@pytest.mark.skipif(ref is None, reason="reference build oracle/_ref absent")
-def test_oracle_vs_reference_build_random_halo():
+def test_oracle_vs_reference_build_random_halo(ref):
     R = ref
     R.ref_init(4 << 30)
     pos, mass = halos.small_halo(6000, seed=12)
@@ -273,8 +279,7 @@ def test_oracle_vs_reference_build_random_halo():
     assert np.array_equal(idx, oi) and np.array_equal(d2, od)
 
 
-@pytest.mark.skipif(ref is None, reason="reference build oracle/_ref absent")
-def test_reference_find_near_is_defective():
+def test_reference_find_near_is_defective(ref):
     """The reference findsphdensity (Find_Near, nnost.c:375) does not return
     the k nearest neighbours; the parity target is the exact kNN."""
     R = ref
