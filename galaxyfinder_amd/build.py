@@ -40,9 +40,13 @@ def _compile(src, out_dir, extra):
 
 def build(verbose=False, extra_flags=(), lib=LIB):
     """Compile every source and link `lib`.  `extra_flags` go to every hipcc
-    compile (e.g. -DPGALF_KNN_BUFCAP=32 for the kNN capacities)."""
-    tag = "" if lib == LIB else "_" + os.path.splitext(os.path.basename(lib))[0]
-    out_dir = os.path.join(HERE, "_build" + tag)
+    compile (e.g. -DPGALF_KNN_BUFCAP=32 for the kNN capacities).  Objects go
+    to _build/ beside the package library, or to _obj/ beside any other
+    `lib` (variant builds stay out of the package)."""
+    if os.path.abspath(lib) == LIB:
+        out_dir = os.path.join(HERE, "_build")
+    else:
+        out_dir = os.path.join(os.path.dirname(os.path.abspath(lib)), "_obj")
     os.makedirs(out_dir, exist_ok=True)
     with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
         objs = list(ex.map(lambda s: _compile(s, out_dir, extra_flags), SOURCES))

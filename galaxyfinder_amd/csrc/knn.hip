@@ -167,8 +167,17 @@ struct KHalo {
     long long idx_base;  // subtracted from candidate ids in the output lists
     long long nb;        // leaf buckets
     int nlev;
+    int kb;              // slot bits of the packed keys: nb * kB <= 2^kb
     long long loff[kMaxWalkLevels];
 };
+// Largest slot-bit count of the packed-key query (keys keep 51 - kb bits of
+// the distance); larger halos take the exact query only.
+constexpr int kMaxKeyBits = 30;
+inline int key_bits(long long nslots) {
+    int b = 1;
+    while ((1LL << b) < nslots) b++;
+    return b;
+}
 // One wavefront of queries: kWave consecutive query slots of one halo.
 struct KWave {
     int halo;
@@ -959,25 +968,21 @@ __device__ inline void ring_sort(int *list, int count, int *tmp, int gshift, int
     for (int j = gl; j < count; j += kG) list[j] = tmp[j];
 }
 
+// One wavefront of the exact query: logical wave `wid` (queries
+// [q0, q0 + 64) of a halo).
 template <class P>
-__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2)))
-knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned *qid,
-            const double *cxa, const double *cya, const double *cza, const unsigned *cida,
-            const Box *boxes, const KHalo *halos, KHalo halo0, const KWave *waves, int k,
-            long long *nbr, float *rho, int rho_mode, const P *mass,
-            int self_seed, unsigned long long *dbg) {
-    __shared__ unsigned s_bc[kBufCap][kWave];
-    __shared__ int s_list[kGroups][kFCap];
-    __shared__ int s_stk[kGroups][kStackCap];  // also the seed list before the walk
-    __shared__ long long s_off[kMaxLevels], s_n[kMaxLevels];
-    __shared__ double s_qx[kWave], s_qy[kWave], s_qz[kWave], s_qk[kWave];
-    __shared__ Stage sg;
-
+__device__ __forceinline__ void query_wave_exact(
+    unsigned wid, const double *qx, const double *qy, const double *qz, const unsigned *qid,
+    const double *cxa, const double *cya, const double *cza, const unsigned *cida,
+    const Box *boxes, const KHalo *halos, const KHalo &halo0, const KWave *waves, int k,
+    long long *nbr, float *rho, int rho_mode, const P *mass, int self_seed,
+    unsigned long long *dbg, unsigned (*s_bc)[kWave], int (*s_list)[kFCap],
+    int (*s_stk)[kStackCap], long long *s_off, long long *s_n, double *s_qx, double *s_qy,
+    double *s_qz, double *s_qk, Stage &sg) {
     const unsigned long long t_start = dbg_clock(dbg);
     const int lane = threadIdx.x;
     const int grp = lane / kG, gl = lane % kG, gshift = grp * kG;
     // this wave's halo: its candidates, queries and tree
-    const unsigned wid = xcd_block(blockIdx.x, gridDim.x);
     KWave wv;
     if (waves) {
         wv = waves[wid];
@@ -1280,7 +1285,706 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
     }
 }
 
+// The exact query kernel.  kList == false: block b runs logical wave
+// xcd_block(b, gridDim.x).  kList == true: the blocks run the waves listed in
+// wlist[0 .. *wcount) (those the packed-key kernel flagged), grid-stride --
+// a separate instantiation, so the loop does not cost the plain kernel
+// registers.
+template <class P, bool kList>
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2)))
+knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned *qid,
+            const double *cxa, const double *cya, const double *cza, const unsigned *cida,
+            const Box *boxes, const KHalo *halos, KHalo halo0, const KWave *waves, int k,
+            long long *nbr, float *rho, int rho_mode, const P *mass,
+            int self_seed, unsigned long long *dbg, const unsigned *wlist,
+            const unsigned *wcount) {
+    __shared__ unsigned s_bc[kBufCap][kWave];
+    __shared__ int s_list[kGroups][kFCap];
+    __shared__ int s_stk[kGroups][kStackCap];  // also the seed list before the walk
+    __shared__ long long s_off[kMaxLevels], s_n[kMaxLevels];
+    __shared__ double s_qx[kWave], s_qy[kWave], s_qz[kWave], s_qk[kWave];
+    __shared__ Stage sg;
+    if constexpr (kList) {
+        const unsigned n = *wcount;
+        for (unsigned it = blockIdx.x; it < n; it += gridDim.x) {
+            query_wave_exact<P>(wlist[it], qx, qy, qz, qid, cxa, cya, cza, cida, boxes, halos,
+                                halo0, waves, k, nbr, rho, rho_mode, mass, self_seed, dbg, s_bc,
+                                s_list, s_stk, s_off, s_n, s_qx, s_qy, s_qz, s_qk, sg);
+            __syncthreads();
+        }
+    } else {
+        query_wave_exact<P>(xcd_block(blockIdx.x, gridDim.x), qx, qy, qz, qid, cxa, cya, cza,
+                            cida, boxes, halos, halo0, waves, k, nbr, rho, rho_mode, mass,
+                            self_seed, dbg, s_bc, s_list, s_stk, s_off, s_n, s_qx, s_qy, s_qz,
+                            s_qk, sg);
+    }
+}
+
+// ---- packed-key query --------------------------------------------------------
+// The same search as the exact query (seed window, depth-first walk, batched
+// list scans, per-lane buffers merged into a sorted top-32 in registers),
+// with each candidate's (distance, id) pair packed into one double:
+//
+//   key bits = ((bits(d2) >> 1) + 2^52) with the low kb bits replaced by the
+//              candidate's slot e (nb * kB <= 2^kb).
+//
+// For d2 >= 0 the bits of d2 are monotone in d2, so the key orders
+// candidates by d2 truncated to 51 - kb bits (its "class") and, within a
+// class, by slot; every key is a positive normal double.  A compare-exchange
+// is then a v_min_f64 / v_max_f64 pair (no lexicographic select), and the
+// list is 32 doubles.
+//
+// Exactness.  Outside one class, key order is the exact (d2, id) order.  A
+// query's key-ordered top k can differ from the exact top k only if a
+// displaced candidate shares the k-th key's class, or two of the first k
+// entries share a class.  The scan accepts every candidate whose d2 is at
+// most the largest d2 of the current k-th key's class (class_ub), so every
+// candidate of the final k-th class that is ever seen passes through a merge;
+// merges record the smallest key they displace (L).  Candidates of a pruned
+// box or rejected by the scan lie in a larger class than the k-th.  A wave
+// with a lane that fails the check (same_class(L, k-th) or equal classes
+// among the first k) writes nothing and is listed in fix_list; the exact
+// query then recomputes those waves.  For distinct float positions such
+// lanes are about one in 10^6; duplicated points and lattices flag many.
+constexpr unsigned long long kKeyOff = 1ull << 52;
+__device__ inline double key_inf() { return __longlong_as_double(0x7fe0000000000000ll); }
+
+__device__ inline double pkey(double d2, unsigned e, unsigned long long keep) {
+    unsigned long long b = (unsigned long long)__double_as_longlong(d2);
+    b = (((b >> 1) + kKeyOff) & keep) | e;
+    return __longlong_as_double((long long)b);
+}
+__device__ inline unsigned pslot(double key, unsigned long long keep) {
+    return (unsigned)((unsigned long long)__double_as_longlong(key) & ~keep);
+}
+__device__ inline bool same_class(double a, double b, unsigned long long keep) {
+    return ((((unsigned long long)__double_as_longlong(a)) ^
+             ((unsigned long long)__double_as_longlong(b))) & keep) == 0ull;
+}
+// The largest d2 whose key has key's class: every d2 of a larger class
+// exceeds it (+inf for the empty-slot key).
+__device__ inline double class_ub(double key, unsigned long long keep) {
+    if (!(key < key_inf())) return __longlong_as_double(0x7ff0000000000000ll);
+    unsigned long long b = (unsigned long long)__double_as_longlong(key);
+    b = (((b - kKeyOff) | ~keep) << 1) | 1ull;
+    return __longlong_as_double((long long)b);
+}
+
+__device__ inline void pcas(double &a, double &b) {
+    const double lo = fmin(a, b), hi = fmax(a, b);
+    a = lo;
+    b = hi;
+}
+template <int N, int LO, int HI, int R>
+__device__ inline void oe_merge_p(double (&d)[N]) {
+    constexpr int STEP = 2 * R;
+    if constexpr (STEP < HI - LO) {
+        oe_merge_p<N, LO, HI, STEP>(d);
+        oe_merge_p<N, LO + R, HI, STEP>(d);
+#pragma unroll
+        for (int i = LO + R; i < HI - R; i += STEP) pcas(d[i], d[i + R]);
+    } else {
+        pcas(d[LO], d[LO + R]);
+    }
+}
+template <int N, int LO = 0, int HI = N - 1>
+__device__ inline void oe_sort_p(double (&d)[N]) {
+    if constexpr (HI - LO >= 1) {
+        constexpr int MID = LO + (HI - LO) / 2;
+        oe_sort_p<N, LO, MID>(d);
+        oe_sort_p<N, MID + 1, HI>(d);
+        oe_merge_p<N, LO, HI, 1>(d);
+    }
+}
+template <int N, int J>
+__device__ inline void bitonic_clean_p(double (&d)[N]) {
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const int l = i ^ J;
+        if (l > i) pcas(d[i], d[l]);
+    }
+    if constexpr (J > 1) bitonic_clean_p<N, J / 2>(d);
+}
+// Merge a chunk of N keys into the sorted top-32 (as merge_top); the keys
+// that leave the list lower L.
+template <int N>
+__device__ __forceinline__ void merge_top_p(double (&td)[kTop], double (&bd)[N], double &L) {
+    oe_sort_p<N>(bd);
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+        constexpr int o = kTop - N;
+        const double c = bd[N - 1 - j], t = td[o + j];
+        td[o + j] = fmin(t, c);
+        L = fmin(L, fmax(t, c));
+    }
+    bitonic_clean_p<kTop, kTop / 2>(td);
+}
+
+// Per-phase shader-clock sums of the packed-key query (PGALF_KNN_PROF=1; a
+// separate instantiation, so the plain kernel carries none of it).
+enum : int {
+    kPrSeed = 0, kPrWalk, kPrScan, kPrMerge, kPrTail, kPrTotal,
+    kPrWaves, kPrRounds, kPrBatches, kPrChunks, kPrFlagged, kPrSlots, kPrN = 16
+};
+template <bool kProf>
+__device__ inline unsigned long long pr_clock() {
+    if constexpr (kProf) return (unsigned long long)clock64();
+    return 0ull;
+}
+template <bool kProf>
+__device__ inline void pr_add(unsigned long long *prof, int i, unsigned long long v, int lane) {
+    if constexpr (kProf) {
+        if (lane == 0) atomicAdd(&prof[i], v);
+    }
+}
+
+struct PState {
+    double td[kTop];  // sorted keys
+    double kth;       // td[k - 1]
+    double kub;       // class_ub(kth); -1 for an invalid lane (accepts nothing)
+    double L;         // smallest key displaced from the list
+    int cnt;
+};
+
+__device__ __forceinline__ void p_bound(PState &st, int k, bool valid, unsigned long long keep) {
+    double r = st.td[kTop - 1];
+    if (k != kTop) {
+        // sorted: the (k-1)-th is the largest of the first k, taken as a
+        // masked maximum -- a masked select would be folded into a dynamic
+        // index, which puts the list in scratch memory
+        r = st.td[0];
+#pragma unroll
+        for (int t = 1; t < kTop; t++) r = t < k ? fmax(r, st.td[t]) : r;
+    }
+    st.kth = r;
+    st.kub = valid ? class_ub(r, keep) : -1.0;
+}
+
+// Gaps as max(lo - q, q - hi, 0): the same value as gap1 without branches.
+__device__ inline double gap1_p(double lo, double hi, double qlo, double qhi) {
+    return fmax(fmax(lo - qhi, qlo - hi), 0.0);
+}
+__device__ __forceinline__ double box_gap2_p(const Box &b, const double qlo[3], const double qhi[3]) {
+    const double g0 = gap1_p(b.lo[0], b.hi[0], qlo[0], qhi[0]);
+    const double g1 = gap1_p(b.lo[1], b.hi[1], qlo[1], qhi[1]);
+    const double g2 = gap1_p(b.lo[2], b.hi[2], qlo[2], qhi[2]);
+    const double a = g0 * g0;
+    const double bb = g1 * g1;
+    const double c = g2 * g2;
+    return (a + bb) + c;
+}
+// union_pass with the test folded into a minimum: g2 <= qk exactly when
+// g2 - qk <= 0 (IEEE subtraction keeps the sign and is zero only for equal
+// operands); invalid lanes hold qk = -1 and never pass.
+__device__ __forceinline__ bool union_pass_p(const Box &b, int grp, const double *s_qx,
+                                    const double *s_qy, const double *s_qz, const double *s_qk) {
+    double mn = __longlong_as_double(0x7ff0000000000000ll);
+#pragma unroll
+    for (int j = 0; j < kG; j++) {
+        const int L = grp * kG + j;
+        const double x = s_qx[L], y = s_qy[L], z = s_qz[L];
+        const double g0 = gap1_p(b.lo[0], b.hi[0], x, x);
+        const double g1 = gap1_p(b.lo[1], b.hi[1], y, y);
+        const double g2 = gap1_p(b.lo[2], b.hi[2], z, z);
+        const double a = g0 * g0;
+        const double bb = g1 * g1;
+        const double c = g2 * g2;
+        mn = fmin(mn, ((a + bb) + c) - s_qk[L]);
+    }
+    return mn <= 0.0;
+}
+
+// Merge a lane's buffer of cnt sorted-candidate slots into its top-32, nch
+// chunks of kBuf at most (wave uniform); keys recomputed from the slots (a
+// buffer of keys would be twice the LDS, or half the slots and twice the
+// flushes: both measured slower).
+template <bool kProf>
+__device__ __forceinline__ void flush_p(PState &st, int nch, const unsigned (*s_bc)[kWave], int lane,
+                               double px, double py, double pz, const double *cx,
+                               const double *cy, const double *cz, unsigned long long keep,
+                               unsigned long long *prof) {
+    const unsigned long long t0 = pr_clock<kProf>();
+#pragma unroll 1
+    for (int ch = 0; ch < nch; ch++) {
+        if (__ballot(st.cnt > ch * kBuf) == 0ull) break;
+        pr_add<kProf>(prof, kPrChunks, 1ull, lane);
+        double bd[kBuf];
+#pragma unroll
+        for (int t = 0; t < kBuf; t++) {
+            const int e = ch * kBuf + t;
+            bd[t] = key_inf();
+            if (e < st.cnt) {
+                const unsigned sidx = s_bc[e][lane];
+                bd[t] = pkey(d2_ref(px, py, pz, cx[sidx], cy[sidx], cz[sidx]), sidx, keep);
+            }
+        }
+        merge_top_p(st.td, bd, st.L);
+    }
+    pr_add<kProf>(prof, kPrMerge, pr_clock<kProf>() - t0, lane);
+}
+
+// scan_list of the packed-key query: a candidate goes to the buffer when its
+// d2 is at most the lane's kub.
+template <bool kProf>
+__device__ __forceinline__ void scan_list_p(PState &st, int *list, int count, bool valid, int k,
+                                   double px, double py, double pz, const double *cx,
+                                   const double *cy, const double *cz, long long nc,
+                                   const Box *leaves, Stage &sg, unsigned (*s_bc)[kWave],
+                                   double *s_qx, double *s_qy, double *s_qz, double *s_qk,
+                                   int lane, int grp, int gl, unsigned long long keep,
+                                   unsigned long long *prof) {
+    const double nan = __longlong_as_double(0x7ff8000000000000ll);
+    int ipos = 0;
+    const int nit = wave_max_i((count + kBatch - 1) / kBatch);
+    for (int it = 0; it < nit; it++) {
+        if (__ballot(ipos < count) == 0ull) break;
+        const int wend = ipos + kG < count ? ipos + kG : count;
+        int ent = 0;
+        bool pass = false;
+        if (ipos + gl < count) {
+            ent = list[ipos + gl];
+            pass = union_pass_p(leaves[ent & kBucketMask], grp, s_qx, s_qy, s_qz, s_qk);
+        }
+        const unsigned mm = (unsigned)((__ballot(pass) >> (grp * kG)) & kGMask);
+        const int npass = __popc(mm);
+        const int first = wend - npass;
+        if (pass) list[first + __popc(mm & ((1u << gl) - 1u))] = ent;
+        const int nsel = npass < kBatch ? npass : kBatch;
+        if (ipos < count) ipos = first + nsel;
+        double vx[kBatch], vy[kBatch], vz[kBatch];
+        unsigned ve[kBatch];
+#pragma unroll
+        for (int t = 0; t < kBatch; t++) {
+            long long e = t < nsel ? (long long)(list[first + t] & kBucketMask) * kB + gl : -1;
+            if (e >= nc) e = -1;
+            vx[t] = nan;
+            vy[t] = 0.0;
+            vz[t] = 0.0;
+            ve[t] = 0xffffffffu;
+            if (e >= 0) {
+                vx[t] = cx[e];
+                vy[t] = cy[e];
+                vz[t] = cz[e];
+                ve[t] = (unsigned)e;
+            }
+        }
+        const int wsel = wave_max_i(nsel);
+        pr_add<kProf>(prof, kPrBatches, 1ull, lane);
+        pr_add<kProf>(prof, kPrSlots, (unsigned long long)wsel, lane);
+#pragma unroll
+        for (int t = 0; t < kBatch; t++)
+            stage_put(sg, grp, t * kB + gl, vx[t], vy[t], vz[t], ve[t], ve[t]);
+        __syncthreads();
+        const int nslots = wsel * kB;
+        const uint4 *su = sg.u + grp * kStageStride;
+        for (int h = 0; h < nslots; h += kChunkC) {
+#pragma unroll
+            for (int c = h; c < h + kChunkC; c++) {
+                const uint4 a = su[2 * c], b = su[2 * c + 1];
+                const double d = d2_ref(px, py, pz, hilo_d(a.x, a.y), hilo_d(a.z, a.w),
+                                        hilo_d(b.x, b.y));
+                s_bc[st.cnt][lane] = b.z;
+                st.cnt += d <= st.kub ? 1 : 0;
+            }
+            if (__ballot(st.cnt > kBufCap - kChunkC) != 0ull) {
+                // one chunk for every lane; a lane's slots past it move down
+                flush_p<kProf>(st, 1, s_bc, lane, px, py, pz, cx, cy, cz, keep, prof);
+                const int rest = st.cnt > kBuf ? st.cnt - kBuf : 0;
+                const int wrest = wave_max_i(rest);
+                for (int t = 0; t < wrest; t++)
+                    if (t < rest) s_bc[t][lane] = s_bc[kBuf + t][lane];
+                st.cnt = rest;
+                p_bound(st, k, valid, keep);
+                s_qk[lane] = st.kub;
+            }
+        }
+        __syncthreads();
+    }
+    if (__ballot(st.cnt > 0) != 0ull) {
+        flush_p<kProf>(st, kBufCap / kBuf, s_bc, lane, px, py, pz, cx, cy, cz, keep, prof);
+        st.cnt = 0;
+        p_bound(st, k, valid, keep);
+    }
+    s_qk[lane] = st.kub;
+    __syncthreads();
+}
+
+template <class P, bool kProf>
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2)))
+knn_packed_k(const double *qx, const double *qy, const double *qz, const unsigned *qid,
+             const double *cxa, const double *cya, const double *cza, const unsigned *cida,
+             const Box *boxes, const KHalo *halos, KHalo halo0, const KWave *waves, int k,
+             long long *nbr, float *rho, int rho_mode, const P *mass, int self_seed,
+             unsigned *fix_count, unsigned *fix_list, unsigned long long *prof) {
+    __shared__ unsigned s_bc[kBufCap][kWave];
+    __shared__ int s_list[kGroups][kFCap];
+    __shared__ int s_stk[kGroups][kStackCap];  // also the seed list before the walk
+    __shared__ long long s_off[kMaxLevels], s_n[kMaxLevels];
+    __shared__ double s_qx[kWave], s_qy[kWave], s_qz[kWave], s_qk[kWave];
+    __shared__ Stage sg;
+
+    const unsigned long long t_start = pr_clock<kProf>();
+    const int lane = threadIdx.x;
+    const int grp = lane / kG, gl = lane % kG, gshift = grp * kG;
+    const unsigned wid = xcd_block(blockIdx.x, gridDim.x);
+    KWave wv;
+    if (waves) {
+        wv = waves[wid];
+    } else {
+        wv.halo = 0;
+        wv.q0 = (long long)wid * kWave;
+    }
+    const KHalo &hd = halos ? halos[wv.halo] : halo0;
+    const long long nc = hd.nc, nb = hd.nb, hq0 = hd.q0, hq1 = hd.q0 + hd.nq;
+    const long long idx_base = hd.idx_base;
+    const int nlev = hd.nlev;
+    const unsigned long long keep = ~((1ull << hd.kb) - 1ull);
+    const double *cx = cxa + hd.c0, *cy = cya + hd.c0, *cz = cza + hd.c0;
+    const unsigned *cid = cida + hd.c0;
+    if (lane < nlev) {
+        s_off[lane] = hd.loff[lane];
+        s_n[lane] = level_count(nb, lane);
+    }
+    const Box *leaves = boxes + hd.loff[0];
+    const long long qi = wv.q0 + lane;
+    const bool valid = qi < hq1;
+    double px = 0, py = 0, pz = 0;
+    if (valid) {
+        px = qx[qi];
+        py = qy[qi];
+        pz = qz[qi];
+    }
+    double qlo[3], qhi[3];
+    qlo[0] = grp_min_d(valid ? px : DBL_MAX);
+    qlo[1] = grp_min_d(valid ? py : DBL_MAX);
+    qlo[2] = grp_min_d(valid ? pz : DBL_MAX);
+    qhi[0] = grp_max_d(valid ? px : -DBL_MAX);
+    qhi[1] = grp_max_d(valid ? py : -DBL_MAX);
+    qhi[2] = grp_max_d(valid ? pz : -DBL_MAX);
+    const bool gvalid = ((__ballot(valid) >> gshift) & kGMask) != 0ull;
+
+    PState st;
+    const double kinf = key_inf();
+#pragma unroll
+    for (int j = 0; j < kTop; j++) st.td[j] = kinf;
+    st.L = kinf;
+    st.cnt = 0;
+    __syncthreads();
+    const double inf = __longlong_as_double(0x7ff0000000000000ll);
+
+    // ---- seed bucket (as the exact query)
+    long long node = 0;
+    if (self_seed) {
+        node = (wv.q0 - hq0 + grp * kG) / kB;
+    } else {
+        const double gc[3] = {0.5 * (qlo[0] + qhi[0]), 0.5 * (qlo[1] + qhi[1]),
+                              0.5 * (qlo[2] + qhi[2])};
+        for (int l = nlev - 1; l > 0; l--) {
+            long long child = kFan * node + (gl & (kFan - 1));
+            bool ok = gl < kFan && child < s_n[l - 1];
+            double g2 = inf, c2 = inf;
+            if (ok) {
+                Box bx = boxes[s_off[l - 1] + child];
+                g2 = box_gap2(bx, gc, gc);
+                double e0 = 0.5 * (bx.lo[0] + bx.hi[0]) - gc[0];
+                double e1 = 0.5 * (bx.lo[1] + bx.hi[1]) - gc[1];
+                double e2 = 0.5 * (bx.lo[2] + bx.hi[2]) - gc[2];
+                c2 = e0 * e0 + e1 * e1 + e2 * e2;
+            }
+            long long cbest = ok ? child : 0x7fffffffffffffffll;
+#pragma unroll
+            for (int o = kG / 2; o > 0; o >>= 1) {
+                double og = __shfl_xor(g2, o, 64);
+                double oc2 = __shfl_xor(c2, o, 64);
+                long long oc = __shfl_xor(cbest, o, 64);
+                bool take = (og < g2) | ((og == g2) & ((oc2 < c2) | ((oc2 == c2) & (oc < cbest))));
+                g2 = take ? og : g2;
+                c2 = take ? oc2 : c2;
+                cbest = take ? oc : cbest;
+            }
+            node = cbest;
+        }
+    }
+    if (node > nb - 1) node = nb - 1;
+    long long f0 = node - 1;
+    if (f0 > nb - kSeedFill) f0 = nb - kSeedFill;
+    if (f0 < 0) f0 = 0;
+#pragma unroll
+    for (int c = 0; c < kTop; c++) {
+        const long long e = f0 * kB + c;
+        if (gvalid && e < nc)
+            st.td[c] = pkey(d2_ref(px, py, pz, cx[e], cy[e], cz[e]), (unsigned)e, keep);
+    }
+    oe_sort_p<kTop>(st.td);
+    constexpr int kSeedBefore = (kSeedSpan - kSeedFill) / 2;
+    const long long slo = f0 - kSeedBefore < 0 ? 0 : f0 - kSeedBefore;
+    const long long shi = slo + kSeedSpan - 1 > nb - 1 ? nb - 1 : slo + kSeedSpan - 1;
+    const long long d0 = f0 - kDirect / 2 > slo ? f0 - kDirect / 2 : slo;
+    const long long d1 = f0 + kSeedFill + kDirect / 2 < shi + 1 ? f0 + kSeedFill + kDirect / 2
+                                                               : shi + 1;
+    if constexpr (kDirect > 0) {
+        const unsigned long long t_m = pr_clock<kProf>();
+        const int nlo = (int)(f0 - d0);
+        const int ndir = nlo + (d1 > f0 + kSeedFill ? (int)(d1 - f0 - kSeedFill) : 0);
+#pragma unroll 1
+        for (int c = 0; c < kDirect / 2; c++) {
+            double bd[kDM];
+#pragma unroll
+            for (int t = 0; t < kDM; t++) {
+                const int j = 2 * c + t / kB;
+                const long long b = j < nlo ? d0 + j : f0 + kSeedFill + (j - nlo);
+                const long long e = b * kB + t % kB;
+                bd[t] = kinf;
+                if (gvalid && j < ndir && e < nc)
+                    bd[t] = pkey(d2_ref(px, py, pz, cx[e], cy[e], cz[e]), (unsigned)e, keep);
+            }
+            merge_top_p(st.td, bd, st.L);
+        }
+        pr_add<kProf>(prof, kPrMerge, pr_clock<kProf>() - t_m, lane);
+    }
+    p_bound(st, k, valid, keep);
+    int scnt = 0;
+    if (gl == 0) {
+        for (long long b = slo; b <= shi; b++)
+            if (gvalid && (b < d0 || b >= d1)) s_stk[grp][scnt++] = (int)b;
+    }
+    scnt = __shfl(scnt, gshift, 64);
+    __syncthreads();
+    s_qx[lane] = px;
+    s_qy[lane] = py;
+    s_qz[lane] = pz;
+    s_qk[lane] = st.kub;
+    __syncthreads();
+    scan_list_p<kProf>(st, s_stk[grp], scnt, valid, k, px, py, pz, cx, cy, cz, nc, leaves, sg,
+                       s_bc, s_qx, s_qy, s_qz, s_qk, lane, grp, gl, keep, prof);
+    double R2 = grp_max_d(valid ? st.kub : 0.0);
+    pr_add<kProf>(prof, kPrSeed, pr_clock<kProf>() - t_start, lane);
+
+    // ---- depth-first walk, nearest child first (as the exact query)
+    const double gcen[3] = {0.5 * (qlo[0] + qhi[0]), 0.5 * (qlo[1] + qhi[1]),
+                            0.5 * (qlo[2] + qhi[2])};
+    int *list = s_list[grp];
+    int *stk = s_stk[grp];
+    int sp = 0, lcnt = 0, nscan = 0;
+    if (gvalid && nlev > 1) {
+        if (gl == 0) stk[0] = nlev - 1;
+        sp = 1;
+    }
+    __syncthreads();
+    for (;;) {
+        const bool more = sp > 0;
+        const bool any = __ballot(more) != 0ull;
+        if (any) {
+            const unsigned long long t_r = pr_clock<kProf>();
+            pr_add<kProf>(prof, kPrRounds, 1ull, lane);
+            int ent = 0;
+            if (more) ent = stk[sp - 1];
+            sp -= more ? 1 : 0;
+            const int cl = more ? (ent & ((1 << kLevelShift) - 1)) - 1 : 0;
+            const long long child = (long long)kFan * (ent >> kLevelShift) + gl;
+            bool pass = false;
+            double cg = inf;
+            if (more && child < s_n[cl] && !(cl == 0 && child >= slo && child <= shi)) {
+                Box cb = boxes[s_off[cl] + child];
+                pass = !(box_gap2_p(cb, qlo, qhi) > R2) &&
+                       union_pass_p(cb, grp, s_qx, s_qy, s_qz, s_qk);
+                cg = box_gap2_p(cb, gcen, gcen);
+            }
+            const unsigned m = (unsigned)((__ballot(pass) >> gshift) & kGMask);
+            const int npass = __popc(m);
+            if (cl == 0) {
+                int pos = lcnt + __popc(m & ((1u << gl) - 1u));
+                const double cs = (double)(kRingSteps * kRingSteps) * cg;
+                int ring = 0;
+#pragma unroll
+                for (int j = 1; j < kRings; j++) ring += cs >= (double)(j * j) * R2 ? 1 : 0;
+                if (pass) list[pos] = (int)child | (ring << kRingShift);
+                lcnt += npass;
+            } else {
+                int nearer = 0;
+#pragma unroll
+                for (int j = 0; j < kG; j++) {
+                    const double og = __shfl(cg, gshift + j, 64);
+                    nearer += (((m >> j) & 1u) != 0u) & ((og < cg) | ((og == cg) & (j < gl)));
+                }
+                if (pass) stk[sp + npass - 1 - nearer] = (int)(child << kLevelShift) | cl;
+                sp += npass;
+            }
+            __syncthreads();
+            pr_add<kProf>(prof, kPrWalk, pr_clock<kProf>() - t_r, lane);
+        }
+        const int lmax = wave_max_i(lcnt);
+        if (lmax > kFCap - kG || (!any && lmax > 0) || (nscan == 0 && lmax >= kFirstScan)) {
+            const unsigned long long t_s = pr_clock<kProf>();
+            nscan++;
+            ring_sort(list, lcnt, reinterpret_cast<int *>(&sg) + grp * kFCap, gshift, gl);
+            __syncthreads();
+            int skip = 0;
+            if constexpr (kWalkDirect > 0) {
+                if (nscan == 1) {
+                    const unsigned long long t_m = pr_clock<kProf>();
+                    skip = lcnt < kWalkDirect ? lcnt : kWalkDirect;
+#pragma unroll 1
+                    for (int c = 0; c < kWalkDirect / 2; c++) {
+                        double bd[kDM];
+#pragma unroll
+                        for (int t = 0; t < kDM; t++) {
+                            const int j = 2 * c + t / kB;
+                            bd[t] = kinf;
+                            if (j < skip) {
+                                const long long e = (long long)(list[j] & kBucketMask) * kB + t % kB;
+                                if (e < nc)
+                                    bd[t] = pkey(d2_ref(px, py, pz, cx[e], cy[e], cz[e]),
+                                                 (unsigned)e, keep);
+                            }
+                        }
+                        merge_top_p(st.td, bd, st.L);
+                    }
+                    p_bound(st, k, valid, keep);
+                    s_qk[lane] = st.kub;
+                    __syncthreads();
+                    pr_add<kProf>(prof, kPrMerge, pr_clock<kProf>() - t_m, lane);
+                }
+            }
+            scan_list_p<kProf>(st, list + skip, lcnt - skip, valid, k, px, py, pz, cx, cy, cz,
+                               nc, leaves, sg, s_bc, s_qx, s_qy, s_qz, s_qk, lane, grp, gl,
+                               keep, prof);
+            lcnt = 0;
+            R2 = grp_max_d(valid ? st.kub : 0.0);
+            pr_add<kProf>(prof, kPrScan, pr_clock<kProf>() - t_s, lane);
+        }
+        if (!any) break;
+    }
+    const unsigned long long t_tail = pr_clock<kProf>();
+
+    // ---- exactness check (see the block comment): list entries past k are
+    // displaced keys too
+    double Lf = st.L;
+#pragma unroll
+    for (int t = 1; t < kTop; t++) Lf = t >= k ? fmin(Lf, st.td[t]) : Lf;
+    bool flag = same_class(Lf, st.kth, keep);
+#pragma unroll
+    for (int t = 0; t + 1 < kTop; t++)
+        flag |= (t + 1 < k) & same_class(st.td[t], st.td[t + 1], keep);
+    flag &= valid;
+    pr_add<kProf>(prof, kPrWaves, 1ull, lane);
+    if (__ballot(flag) != 0ull) {
+        if (lane == 0) fix_list[atomicAdd(fix_count, 1u)] = wid;
+        pr_add<kProf>(prof, kPrFlagged, 1ull, lane);
+        return;
+    }
+    if constexpr (kProf) {
+        // the tail and total of unflagged waves, measured at their end
+        const unsigned long long t_end = pr_clock<kProf>();
+        pr_add<kProf>(prof, kPrTail, t_end - t_tail, lane);
+        pr_add<kProf>(prof, kPrTotal, t_end - t_start, lane);
+    }
+    if (!valid) return;
+    const long long q = qid[qi];
+    if (nbr) {
+        long long *row = nbr + q * k;
+#pragma unroll
+        for (int t = 0; t < kTop; t++)
+            if (t < k) row[t] = (long long)cid[pslot(st.td[t], keep)] - idx_base;
+    }
+    if (rho) {
+        // exact FP64 distances again from the slots, summed as the exact query does
+        double dk = 0.0;
+        {
+            const unsigned e = pslot(st.kth, keep);
+            dk = d2_ref(px, py, pz, cx[e], cy[e], cz[e]);
+        }
+        const float neardist = sqrtf((float)dk);
+        const float h = (float)((double)neardist / 2.0);
+        float sum = 0.0f;
+#pragma unroll
+        for (int t = 0; t < kTop; t++) {
+            if (t < k) {
+                const unsigned e = pslot(st.td[t], keep);
+                const float r = sqrtf((float)d2_ref(px, py, pz, cx[e], cy[e], cz[e]));
+                float w = W4(r, h);
+                if (rho_mode == 2) w = w * (float)mass[cid[e]];
+                sum += w;
+            }
+        }
+        rho[q] = sum;
+    }
+}
+
 inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// Blocks of the exact pass over the waves the packed-key kernel flagged
+// (grid-stride over the list; the list is short for distinct positions).
+constexpr unsigned kFixBlocks = 1024;
+
+// The packed-key query unless a halo needs more than kMaxKeyBits slot bits,
+// the debug counters (exact query only) are on, or PGALF_KNN_EXACT=1 asks for
+// the exact query alone.
+bool use_packed(int max_kb, bool debug) {
+    static const bool forced = [] {
+        const char *v = getenv("PGALF_KNN_EXACT");
+        return v && *v && *v != '0';
+    }();
+    return !forced && !debug && max_kb <= kMaxKeyBits;
+}
+
+// Runs the query over nwaves logical waves: the packed-key kernel and the
+// exact kernel over the waves it flags, or the exact kernel alone.
+template <class P>
+hipError_t run_query(bool packed, unsigned nwaves, const double *qx, const double *qy,
+                     const double *qz, const unsigned *qid, const double *cx, const double *cy,
+                     const double *cz, const unsigned *cid, const Box *boxes, const KHalo *halos,
+                     const KHalo &h0, const KWave *waves, int k, long long *nbr, float *rho,
+                     int rho_mode, const P *m, int self_seed, unsigned long long *dbg,
+                     Workspace &ws, hipStream_t s) {
+    if (nwaves == 0) return hipSuccess;
+    if (!packed) {
+        knn_query_k<P, false><<<nwaves, kWave, 0, s>>>(qx, qy, qz, qid, cx, cy, cz, cid, boxes,
+                                                       halos, h0, waves, k, nbr, rho, rho_mode,
+                                                       m, self_seed, dbg, nullptr, nullptr);
+        return hipGetLastError();
+    }
+    unsigned *fix = ws.alloc_n<unsigned>(1 + (size_t)nwaves);  // count, then wave ids
+    if (!fix) return hipErrorOutOfMemory;
+    PGALF_HIP_TRY(fill_u32(fix, 1, 0u, s));
+    static const bool prof_on = [] {
+        const char *v = getenv("PGALF_KNN_PROF");
+        return v && *v && *v != '0';
+    }();
+    unsigned long long *prof = nullptr;
+    if (prof_on) {
+        prof = ws.alloc_n<unsigned long long>(kPrN);
+        if (!prof) return hipErrorOutOfMemory;
+        PGALF_HIP_TRY(fill_u64(prof, kPrN, 0ull, s));
+        knn_packed_k<P, true><<<nwaves, kWave, 0, s>>>(qx, qy, qz, qid, cx, cy, cz, cid, boxes,
+                                                       halos, h0, waves, k, nbr, rho, rho_mode,
+                                                       m, self_seed, fix, fix + 1, prof);
+    } else {
+        knn_packed_k<P, false><<<nwaves, kWave, 0, s>>>(qx, qy, qz, qid, cx, cy, cz, cid,
+                                                        boxes, halos, h0, waves, k, nbr, rho,
+                                                        rho_mode, m, self_seed, fix, fix + 1,
+                                                        nullptr);
+    }
+    const unsigned nfix = nwaves < kFixBlocks ? nwaves : kFixBlocks;
+    knn_query_k<P, true><<<nfix, kWave, 0, s>>>(qx, qy, qz, qid, cx, cy, cz, cid, boxes, halos,
+                                                h0, waves, k, nbr, rho, rho_mode, m, self_seed,
+                                                nullptr, fix + 1, fix);
+    if (prof) {
+        unsigned long long d[kPrN];
+        unsigned nf = 0;
+        PGALF_HIP_TRY(hipMemcpyAsync(d, prof, sizeof(d), hipMemcpyDeviceToHost, s));
+        PGALF_HIP_TRY(hipMemcpyAsync(&nf, fix, sizeof(nf), hipMemcpyDeviceToHost, s));
+        PGALF_HIP_TRY(hipStreamSynchronize(s));
+        const double w = d[kPrWaves] ? (double)d[kPrWaves] : 1.0;
+        fprintf(stderr, "[pgalf] packed knn: waves %llu flagged %u | kcycles per wave: seed %.1f "
+                        "walk %.1f scan %.1f (merge %.1f) tail %.1f total %.1f | per wave: walk "
+                        "rounds %.1f scan batches %.1f staged buckets %.1f merged chunks %.1f\n",
+                d[kPrWaves], nf, d[kPrSeed] / w / 1e3, d[kPrWalk] / w / 1e3, d[kPrScan] / w / 1e3,
+                d[kPrMerge] / w / 1e3, d[kPrTail] / w / 1e3, d[kPrTotal] / w / 1e3,
+                d[kPrRounds] / w, d[kPrBatches] / w, d[kPrSlots] / w, d[kPrChunks] / w);
+    }
+    return hipGetLastError();
+}
 
 }  // namespace
 
@@ -1356,6 +2060,7 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
     h0.nq = np;
     h0.idx_base = 0;
     h0.nb = nb;
+    h0.kb = key_bits(nb * kB);
     h0.nlev = 1;
     while (level_count(nb, h0.nlev - 1) > 1) h0.nlev++;
     if (h0.nlev > kMaxLevels || h0.nlev > kMaxWalkLevels) return hipErrorInvalidValue;
@@ -1374,9 +2079,9 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
     if (sorted_hook && *sorted_hook) PGALF_HIP_TRY((*sorted_hook)(qx, qy, qz, qv, np));
     timing_mark("knn_query", s, false);
     long long nqb = (np + kWave - 1) / kWave;
-    knn_query_k<P><<<(unsigned)nqb, kWave, 0, s>>>(qx, qy, qz, qv, cx, cy, cz, cid, boxes,
-                                                   nullptr, h0, nullptr, k, nbr, rho, rho_mode,
-                                                   m, subset ? 0 : 1, dbg);
+    PGALF_HIP_TRY(run_query<P>(use_packed(h0.kb, debug), (unsigned)nqb, qx, qy, qz, qv, cx, cy,
+                               cz, cid, boxes, nullptr, h0, nullptr, k, nbr, rho, rho_mode, m,
+                               subset ? 0 : 1, dbg, ws, s));
     timing_mark("knn_query", s, true);
     if (debug) {
         unsigned long long d[kDbgN];
@@ -1627,6 +2332,8 @@ hipError_t knn_batch_plan(const long long *off, int nhalo, int k, KnnBatchPlan &
         hd.idx_base = off[h];
         hd.nb = (n + kB - 1) / kB;
         if (hd.nb >= (1LL << kRingShift)) return hipErrorInvalidValue;
+        hd.kb = key_bits(hd.nb * kB);
+        if (hd.kb > p.max_kb) p.max_kb = hd.kb;
         hd.nlev = 1;
         while (level_count(hd.nb, hd.nlev - 1) > 1) hd.nlev++;
         if (hd.nlev > kMaxWalkLevels) return hipErrorInvalidValue;
@@ -1769,9 +2476,10 @@ hipError_t knn_batch(const P *x, const P *y, const P *z, const P *m, const Batch
     if (sorted_hook && *sorted_hook) PGALF_HIP_TRY((*sorted_hook)(sx, sy, sz, sid, npad));
     timing_mark("knn_query", s, false);
     KHalo h0{};
-    knn_query_k<P><<<(unsigned)plan.nwaves, kWave, 0, s>>>(
-        sx, sy, sz, sid, sx, sy, sz, sid, boxes, halos, h0,
-        static_cast<const KWave *>(plan.waves), plan.k, nbr, rho, rho_mode, m, 1, nullptr);
+    PGALF_HIP_TRY(run_query<P>(use_packed(plan.max_kb, false), (unsigned)plan.nwaves, sx, sy, sz,
+                               sid, sx, sy, sz, sid, boxes, halos, h0,
+                               static_cast<const KWave *>(plan.waves), plan.k, nbr, rho,
+                               rho_mode, m, 1, nullptr, ws, s));
     timing_mark("knn_query", s, true);
     return hipGetLastError();
 }

@@ -252,6 +252,7 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np,
 // table, largest halos first.
 struct KnnBatchPlan {
     int nhalo = 0, k = 0, hbits = 0;
+    int max_kb = 0;  // slot bits of the packed keys of the largest halo
     long long np = 0, npad = 0, nb_total = 0, nboxes = 0, nwaves = 0;
     void *halos = nullptr;  // KHalo[nhalo] (knn.hip)
     void *waves = nullptr;  // KWave[nwaves] (knn.hip)
