@@ -11,8 +11,9 @@ Prints ONE JSON line (rank 0, N=1):
   value          particles/s through the stage (inputs already in HBM)
   roofline       dominant kernel (kNN query) algorithmic bytes per launch
                  / its mean launch time (HIP events on the launch stream)
-  cpu_baseline   the repo's C/OpenMP CPU restatement (oracle/, "port") on
-                 the same halo, timed on this host's cores
+  cpu_baseline   the reference's own C/OpenMP code compiled from its sources
+                 (oracle/_ref, "reference") on the same halo, timed on this
+                 host's cores (kNN list builder on a cell sample, scaled)
 """
 import argparse
 import json
@@ -56,53 +57,78 @@ def stage_bytes(n, g_cells, k):
     }
 
 
-def cpu_baseline(pos, mass, spec, k, knn_sample, reps=2):
-    """Time the CPU restatement of the stage on the same halo, `reps` times
-    (mean reported).  Deposit, smoothing and read-back run on the whole
-    halo; the kNN tree is built over every particle and queried for
-    knn_sample random particles (all by default), the query time scaled to
-    all particles."""
+def cpu_baseline(pos, mass, spec, k, knn_stride):
+    """Time the reference's own C/OpenMP code (oracle/_ref, compiled from the
+    reference sources) on the same halo, on every host core:
+    assign_density_TSC (tsc_omp2.c:166) and findDen (tsc_omp2.c:40) on the
+    whole halo; the serial octree build (new_Make_Tree_Near, nnost.c:199) and
+    lagFindStellarCore's warm-started Find_Near_PreCopied list builder
+    (nnost.c:1361-1529) on the cells whose flat index is a multiple of
+    knn_stride, its time scaled to every particle.  gaussian_Smoothing
+    (gsmooth.c) needs FFTW, which is absent, so the smoothing leg is the
+    oracle's FP64-FFT restatement.  Without the reference build, the whole
+    leg is the oracle's restatement ("port")."""
+    import ctypes
     from oracle import oracle as O
     n = len(pos)
-    g = (spec["nx"], spec["ny"], spec["nz"], spec["xmin"], spec["ymin"], spec["zmin"],
-         spec["cellsize"])
-    sel = np.random.default_rng(0).choice(n, min(knn_sample, n), replace=False)
-    st = {"tsc": 0.0, "smooth": 0.0, "findden": 0.0, "knn_build": 0.0, "knn_query_scaled": 0.0}
-    work = 0.0
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        den, _ = O.tsc_deposit(pos, mass, g, nthreads=1)
-        t1 = time.perf_counter()
-        sm = O.gsmooth(den, spec["nx"], spec["ny"], spec["nz"], spec["cellsize"], spec["RG"])
-        t2 = time.perf_counter()
+    nx, ny, nz = spec["nx"], spec["ny"], spec["nz"]
+    g = (nx, ny, nz, spec["xmin"], spec["ymin"], spec["zmin"], spec["cellsize"])
+    nth = O.num_threads()
+    R = O.ref_lib()
+    x, y, z = (np.ascontiguousarray(pos[:, a], dtype=np.float64) for a in range(3))
+    m = np.ascontiguousarray(mass, dtype=np.float64)
+    st = {}
+    t0 = time.perf_counter()
+    if R is not None:
+        R.ref_init(8 << 30)
+        den = np.zeros((nz, ny, 2 * (nx // 2 + 1)), dtype=np.float32)
+        R.ref_tsc(O._p(x, O._dp), O._p(y, O._dp), O._p(z, O._dp), O._p(m, O._dp), None, n,
+                  O._p(den, O._fp), *g, 5, nth)
+    else:
+        den, _ = O.tsc_deposit(pos, mass, g, nthreads=nth)
+    t1 = time.perf_counter()
+    sm = O.gsmooth(den, nx, ny, nz, spec["cellsize"], spec["RG"])
+    t2 = time.perf_counter()
+    if R is not None:
+        rb = np.zeros(n, dtype=np.float32)
+        R.ref_findden(O._p(x, O._dp), O._p(y, O._dp), O._p(z, O._dp), n, O._p(rb, O._fp),
+                      *g[3:], O._p(sm, O._fp), *g[:3], nth)
+    else:
         O.findden(pos, g, sm)
-        t3 = time.perf_counter()
+    t3 = time.perf_counter()
+    st["tsc"], st["smooth"], st["findden"] = t1 - t0, t2 - t1, t3 - t2
+    if R is not None:
+        idx = np.empty((n, k), dtype=np.int64)
+        nq, tt = ctypes.c_longlong(), ctypes.c_double()
+        R.ref_knn_lag(O._p(x, O._dp), O._p(y, O._dp), O._p(z, O._dp), n, k, *g[3:], *g[:3],
+                      O._p(idx, O._lp), nth, knn_stride, ctypes.byref(nq), ctypes.byref(tt))
+        t4 = time.perf_counter()
+        st["knn_tree"] = tt.value
+        st["knn_lists_scaled"] = (t4 - t3 - tt.value) * n / max(nq.value, 1)
+        kind = "reference"
+        sample = ("the reference's own C/OpenMP code (oracle/_ref, gcc -O2 -fopenmp from "
+                  "NewGalFinder/tsc_omp2.c and nnost.c) on the same %d-particle halo: "
+                  "assign_density_TSC + findDen on all particles, the serial octree build, "
+                  "and lagFindStellarCore's warm-started Find_Near_PreCopied list builder on "
+                  "every %d-th grid cell (%d queries, time scaled to all particles); "
+                  "smoothing by the oracle's FP64-FFT restatement (gsmooth.c needs FFTW, "
+                  "absent); %.1f s of CPU wall time" % (n, knn_stride, nq.value, t4 - t0))
+    else:
+        sel = np.random.default_rng(0).choice(n, min(1 << 18, n), replace=False)
         tb0 = time.perf_counter()
-        O.knn(pos[sel[:64]], k, cpos=pos)  # tree build (+64 queries)
+        O.knn(pos[sel[:64]], k, cpos=pos)
         tb1 = time.perf_counter()
-        idx, d2 = O.knn(pos[sel], k, cpos=pos)
-        O.sph_density(idx, d2)
-        tb2 = time.perf_counter()
-        build = tb1 - tb0
-        st["tsc"] += (t1 - t0) / reps
-        st["smooth"] += (t2 - t1) / reps
-        st["findden"] += (t3 - t2) / reps
-        st["knn_build"] += build / reps
-        st["knn_query_scaled"] += max(tb2 - tb1 - build, 0.0) * n / len(sel) / reps
-        work += (t3 - t0) + (tb2 - tb0)
+        O.knn(pos[sel], k, cpos=pos)
+        t4 = time.perf_counter()
+        st["knn_tree"] = tb1 - tb0
+        st["knn_lists_scaled"] = max(t4 - tb1 - (tb1 - tb0), 0.0) * n / len(sel)
+        kind = "port"
+        sample = ("the oracle's C/OpenMP restatement (reference build absent) on the same "
+                  "%d-particle halo, k-d tree kNN queried for %d particles (time scaled); "
+                  "%.1f s of CPU wall time" % (n, len(sel), t4 - t0))
     total = sum(st.values())
-    return {
-        "value": n / total,
-        "unit": "particles/s",
-        "cores": O.num_threads(),
-        "kind": "port",
-        "sample": ("CPU restatement (oracle/pgalf_oracle.c, OpenMP) on the same %d-particle "
-                   "halo, mean of %d runs: deposit (serial, reference order) + FP64 FFT "
-                   "smoothing + read-back on all particles, k-d tree kNN built on all and "
-                   "queried for %d particles (query time scaled to all); %.1f s of CPU work"
-                   % (n, reps, len(sel), work)),
-        "stages_s": st,
-    }
+    return {"value": n / total, "unit": "particles/s", "cores": nth, "kind": kind,
+            "sample": sample, "stages_s": {a: round(b, 4) for a, b in st.items()}}
 
 
 def batch_bench(steps, warmup, plain_pass=True):
@@ -353,7 +379,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 4])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--knn-sample", type=int, default=1 << 30)
+    ap.add_argument("--knn-stride", type=int, default=16,
+                    help="CPU baseline: run the reference list builder on every n-th cell")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched 10000-halo catalogue measurement")
     ap.add_argument("--batch-only", action="store_true",
@@ -361,8 +388,10 @@ def main():
     ap.add_argument("--no-grad", action="store_true",
                     help="skip the forward+backward measurement")
     ap.add_argument("--c5", action="store_true",
-                    help="only the 50000-halo batched fwd+bwd with peak climbing (about 1 min "
+                    help="only the 50000-halo batched fwd+bwd with peak climbing (about 20 s "
                          "of catalogue generation on the host)")
+    ap.add_argument("--no-c5", action="store_true",
+                    help="skip the 50000-halo batched fwd+bwd with peak climbing")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain %d-byte device copy (counter calibration)"
                     % CALIB_COPY_BYTES)
@@ -476,9 +505,11 @@ def main():
         out["batch"] = batch_bench(max(3, args.steps // 2), 2)
     if not args.no_grad:
         out["grad"] = grad_bench(max(3, args.steps // 2), 2)
+    if not args.no_c5:
+        out["c5"] = c5_bench(max(3, args.steps // 2), 2)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(pos.astype(np.float64), mass.astype(np.float64),
-                                           spec, k, args.knn_sample)
+                                           spec, k, args.knn_stride)
     print(json.dumps(out), flush=True)
 
 

@@ -64,6 +64,7 @@ SIGNATURES = {
                                     _c_i, _c_f, _c_p, _c_p, _c_p, _c_p, _c_i, _c_p]),
     "pgalf_dev_peak_climb": (_c_i, [_c_p, _c_p, _c_p, _c_ll, ctypes.POINTER(pgalf_grid_t),
                                     _c_p, _c_i, _c_f, _c_p, _c_p, _c_p, _c_p, _c_p]),
+    "pgalf_batch_workspace_bytes": (_c_ll, [_c_p, _c_i]),
     "pgalf_batch_destroy": (None, [_c_p]),
     "pgalf_timing_enable": (None, [_c_i]),
     "pgalf_timing_reset": (None, []),

@@ -422,6 +422,12 @@ extern "C" int pgalf_batch_status(pgalf_batch_t *b, void *stream) {
     return consume_status(*b->ws, s, "pgalf_batch_status");
 }
 
+extern "C" long long pgalf_batch_workspace_bytes(const pgalf_batch_t *b, int reserved) {
+    if (!b || !b->ws) return -1;
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    return (long long)(reserved ? b->ws->reserved() : b->ws->high_water());
+}
+
 extern "C" void pgalf_batch_destroy(pgalf_batch_t *b) {
     if (!b) return;
     std::lock_guard<std::mutex> lk(g_api_mu);

@@ -100,6 +100,13 @@ class Workspace {
     void release(const Mark &m);
     unsigned *status() { return status_; }  // device word of DEV_ERR_* bits
     int device() const { return device_; }
+    // Largest live scratch of any call so far (bytes), and the arena's size.
+    size_t high_water() const { return peak_ > high_water_ ? peak_ : high_water_; }
+    size_t reserved() const {
+        size_t t = 0;
+        for (const auto &c : chunks_) t += c.cap;
+        return t;
+    }
 
   private:
     struct Chunk { char *ptr; size_t cap, used; };

@@ -381,6 +381,11 @@ class Batch:
             _dptr(o["cy"]), _dptr(o["cz"]), _dptr(o["peak"]), 1 if graph else 0, _stream()))
         return out
 
+    def workspace_bytes(self, reserved=False):
+        """Largest device scratch any run of this batch has held at once (or,
+        with reserved=True, the bytes its arena holds now)."""
+        return int(_lib.load().pgalf_batch_workspace_bytes(self._b, 1 if reserved else 0))
+
     def status(self):
         """Synchronise torch's current stream; raise on kernel-detected errors."""
         lib = _lib.load()

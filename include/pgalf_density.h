@@ -213,6 +213,11 @@ int pgalf_batch_run_grad(pgalf_batch_t *b, const float *x, const float *y, const
 /* Synchronise `stream` and return the status of the batch's kernels since
  * the previous call (PGALF_ERR_OUT_OF_GRID or PGALF_OK). */
 int pgalf_batch_status(pgalf_batch_t *b, void *stream);
+/* Device scratch of the batch (no counterpart in the reference; for
+ * capacity planning): reserved == 0, the largest scratch any of its runs has
+ * held at once; reserved != 0, the bytes its arena holds now.  -1 for a
+ * null batch. */
+long long pgalf_batch_workspace_bytes(const pgalf_batch_t *b, int reserved);
 void pgalf_batch_destroy(pgalf_batch_t *b);
 
 /* Phase timing with HIP events on the launch stream (off by default). */

@@ -104,6 +104,10 @@ def ref_lib():
         R.ref_knn_warm.restype = _i64
         R.ref_knn_warm.argtypes = [_dp, _dp, _dp, _i64, _i, _d, _d, _d, _d, _i, _i,
                                    _i, _lp, _i]
+        R.ref_knn_lag.restype = _i64
+        R.ref_knn_lag.argtypes = [_dp, _dp, _dp, _i64, _i, _d, _d, _d, _d, _i, _i, _i, _lp,
+                                  _i, _i64, ctypes.POINTER(ctypes.c_longlong),
+                                  ctypes.POINTER(ctypes.c_double)]
         R.ref_findsphdensity.restype = None
         R.ref_findsphdensity.argtypes = [_dp, _dp, _dp, _dp, _i64, _i, _lp, _fp, _i]
         R.ref_sizeof_bp.restype = _i

@@ -210,3 +210,101 @@ void ref_findsphdensity(const double *x, const double *y, const double *z,
     findsphdensity(bp, (int)np, idx, k, dens);
     free(bp);
 }
+
+/* The np >= 1e5 neighbour-list builder of lagFindStellarCore as the reference
+ * runs it (nnost.c:1361-1529): particles linked to their nearest TSC node
+ * cell; shallow cells (fewer than DEEPSIZE members) one per thread with a
+ * warm-started list (re-measured, qsort-ed, Find_Near_PreCopied); then each
+ * deep cell by all threads, members dealt round-robin, every thread's list
+ * seeded with the cell's first k members.  Only cells whose flat index is a
+ * multiple of `stride` are run (1: all) -- a bounded sample of the same work
+ * for timing.  *nq gets the number of queries run, *t_tree the seconds of the
+ * serial tree build.  Returns the number of short lists. */
+long long ref_knn_lag(const double *x, const double *y, const double *z, long long np, int k,
+                      double xmin, double ymin, double zmin, double cellsize, int nx, int ny,
+                      int nz, long long *idx, int nthreads, long long stride, long long *nq,
+                      double *t_tree) {
+    double t0 = omp_get_wtime();
+    TPtlStruct *ptl;
+    TStruct *tree = ref_tree(x, y, z, np, &ptl);
+    if (t_tree) *t_tree = omp_get_wtime() - t0;
+    long long mx = 2 * (nx / 2 + 1);
+    long long ncells = mx * (long long)ny * nz;
+    long long *head = (long long *)malloc(sizeof(long long) * (size_t)ncells);
+    long long *cnt = (long long *)calloc((size_t)ncells, sizeof(long long));
+    long long *next = (long long *)malloc(sizeof(long long) * (size_t)np);
+    for (long long c = 0; c < ncells; c++) head[c] = -1;
+    for (long long i = 0; i < np; i++) {
+        long long ir = (long long)rint((x[i] - xmin) / cellsize);
+        long long jr = (long long)rint((y[i] - ymin) / cellsize);
+        long long kr = (long long)rint((z[i] - zmin) / cellsize);
+        long long c = ir + mx * (jr + (long long)ny * kr);
+        next[i] = head[c];
+        head[c] = i;
+        cnt[c]++;
+    }
+    if (stride < 1) stride = 1;
+    long long shortq = 0, nrun = 0;
+    omp_set_num_threads(nthreads);
+    ref_neighbor *tnb = (ref_neighbor *)malloc(sizeof(ref_neighbor) * (size_t)nthreads * MAX_NUM_NEAR);
+#pragma omp parallel reduction(+ : shortq, nrun)
+    {
+        ref_neighbor *nb = tnb + (size_t)omp_get_thread_num() * MAX_NUM_NEAR;
+        int npn = 0;
+#pragma omp for schedule(dynamic, 1)
+        for (long long c = 0; c < ncells; c += stride) {
+            if (cnt[c] == 0 || cnt[c] >= DEEPSIZE) continue;
+            for (long long i = head[c]; i >= 0; i = next[i]) {
+                particle pp;
+                pp.x = x[i]; pp.y = y[i]; pp.z = z[i];
+                for (int j = 0; j < npn; j++) {
+                    long long q = nb[j].indx;
+                    dptype tx = pp.x - x[q], ty = pp.y - y[q], tz = pp.z - z[q];
+                    nb[j].dist2 = tx * tx + ty * ty + tz * tz;
+                }
+                if (npn) qsort(nb, (size_t)npn, sizeof(ref_neighbor), ref_cmp_nb);
+                npn = Find_Near_PreCopied(&pp, k, tree, ptl, nb, npn);
+                if (npn != k) shortq++;
+                nrun++;
+                for (int j = 0; j < k; j++) idx[i * k + j] = j < npn ? nb[j].indx : -1;
+            }
+        }
+    }
+    for (long long c = 0; c < ncells; c += stride) {
+        if (cnt[c] < DEEPSIZE) continue;
+        int seed = 0;
+        for (long long i = head[c]; i >= 0 && seed < k; i = next[i]) tnb[seed++].indx = i;
+        for (int t = 1; t < nthreads; t++)
+            for (int j = 0; j < seed; j++) tnb[(size_t)t * MAX_NUM_NEAR + j] = tnb[j];
+#pragma omp parallel reduction(+ : shortq, nrun)
+        {
+            const int tid = omp_get_thread_num(), nth = omp_get_num_threads();
+            ref_neighbor *nb = tnb + (size_t)tid * MAX_NUM_NEAR;
+            int npn = seed;
+            long long icount = 0;
+            for (long long i = head[c]; i >= 0; i = next[i], icount++) {
+                if (icount % nth != tid) continue;
+                particle pp;
+                pp.x = x[i]; pp.y = y[i]; pp.z = z[i];
+                for (int j = 0; j < npn; j++) {
+                    long long q = nb[j].indx;
+                    dptype tx = pp.x - x[q], ty = pp.y - y[q], tz = pp.z - z[q];
+                    nb[j].dist2 = tx * tx + ty * ty + tz * tz;
+                }
+                if (npn) qsort(nb, (size_t)npn, sizeof(ref_neighbor), ref_cmp_nb);
+                npn = Find_Near_PreCopied(&pp, k, tree, ptl, nb, npn);
+                if (npn != k) shortq++;
+                nrun++;
+                for (int j = 0; j < k; j++) idx[i * k + j] = j < npn ? nb[j].indx : -1;
+            }
+        }
+    }
+    if (nq) *nq = nrun;
+    free(tnb);
+    free(head);
+    free(cnt);
+    free(next);
+    free(tree);
+    free(ptl);
+    return shortq;
+}
