@@ -46,6 +46,9 @@ SIGNATURES = {
                                  _c_p, _c_p]),
     "pgalf_dev_knn": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_ll, _c_i, _c_i, _c_p, _c_p,
                              _c_i, _c_p]),
+    "pgalf_dev_knn_noalloc": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_ll, _c_i, _c_i, _c_p,
+                                     _c_p, _c_i, _c_p]),
+    "pgalf_dev_knn_noalloc_max": (_c_ll, []),
     "pgalf_dev_density_stage": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_ll,
                                        ctypes.POINTER(pgalf_grid_t), _c_f, _c_i, _c_p, _c_p,
                                        _c_p, _c_p, _c_p]),

@@ -17,7 +17,17 @@
 namespace pgalf {
 
 // ---- workspace -------------------------------------------------------------
+Workspace::Workspace(int device, void *base, size_t bytes) : device_(device), fixed_(true) {
+    status_ = static_cast<unsigned *>(base);
+    Chunk c;
+    c.ptr = static_cast<char *>(base) + 256;
+    c.cap = bytes > 256 ? bytes - 256 : 0;
+    c.used = 0;
+    chunks_.push_back(c);
+}
+
 Workspace::~Workspace() {
+    if (fixed_) return;  // caller-owned memory
     for (auto &c : chunks_) hipFree(c.ptr);
     if (status_) hipFree(status_);
     if (done_) hipEventDestroy(done_);
@@ -30,7 +40,7 @@ static bool capturing(hipStream_t s) {
 
 void Workspace::end(hipStream_t s) {
     // under stream capture the ordering is the graph's own
-    if (!done_ || capturing(s)) return;
+    if (fixed_ || !done_ || capturing(s)) return;
     if (hipEventRecord(done_, s) == hipSuccess) {
         last_ = s;
         has_last_ = true;
@@ -38,6 +48,7 @@ void Workspace::end(hipStream_t s) {
 }
 
 hipError_t Workspace::add_chunk(size_t bytes) {
+    if (fixed_) return hipErrorOutOfMemory;
     Chunk c;
     c.cap = bytes;
     c.used = 0;
@@ -48,6 +59,13 @@ hipError_t Workspace::add_chunk(size_t bytes) {
 }
 
 hipError_t Workspace::begin(hipStream_t s) {
+    if (fixed_) {  // no events, no growth: reset the bump position only
+        if (peak_ > high_water_) high_water_ = peak_;
+        peak_ = 0;
+        chunks_[0].used = 0;
+        cur_ = 0;
+        return hipSuccess;
+    }
     if (!done_) PGALF_HIP_TRY(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
     // the previous call ran on another stream: wait for its kernels
     if (has_last_ && s != last_ && !capturing(s)) PGALF_HIP_TRY(hipStreamWaitEvent(s, done_, 0));
@@ -736,4 +754,67 @@ extern "C" int pgalf_dev_status(void *stream) {
         if (call.e != hipSuccess) return hip_err(call.e, "pgalf_dev_status");
     }
     return consume_status(ws, s, "pgalf_dev_status");
+}
+
+// ---- kNN with no allocation ------------------------------------------------------------
+// Scratch of pgalf_dev_knn_noalloc: a file-scope device array, one copy per
+// GPU, held until the process ends.  Host code never passes `scratch` itself
+// to a kernel (in host code the name is a stand-in, not a device address):
+// the first call on a device looks its address up with hipGetSymbolAddress
+// and keeps it per device.
+static constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_na_mu;
+std::map<int, Workspace *> g_na_ws;  // per device: a fixed arena over its scratch copy
+
+Workspace *noalloc_workspace(hipError_t &e) {
+    int dev = 0;
+    e = hipGetDevice(&dev);
+    if (e != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    auto it = g_na_ws.find(dev);
+    if (it != g_na_ws.end()) return it->second;
+    void *ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        if (e == hipSuccess) e = hipErrorInvalidSymbol;
+        return nullptr;
+    }
+    Workspace *w = new Workspace(dev, ptr, kNoAllocScratchBytes);
+    g_na_ws[dev] = w;
+    return w;
+}
+}  // namespace
+
+extern "C" long long pgalf_dev_knn_noalloc_max(void) { return PGALF_KNN_NOALLOC_MAX; }
+
+extern "C" int pgalf_dev_knn_noalloc(const float *x, const float *y, const float *z,
+                                     const float *m, const int *type, long long np,
+                                     int cand_ptype, int k, long long *nbr, float *rho,
+                                     int rho_mode, void *stream) {
+    const char *where = "pgalf_dev_knn_noalloc";
+    if (np < 1 || k < 1 || k > 32 || rho_mode < 0 || rho_mode > 2 || (rho_mode == 2 && !m))
+        return set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
+    if (np > PGALF_KNN_NOALLOC_MAX)
+        return set_err(PGALF_ERR_ARG, "%s: %lld particles exceed the %lld that its %zu-byte "
+                       "static scratch holds (use pgalf_dev_knn)", where, np,
+                       (long long)PGALF_KNN_NOALLOC_MAX, kNoAllocScratchBytes);
+    if (k > np) return set_err(PGALF_ERR_SHORT_LIST, "%s: k > np", where);
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    hipError_t e = hipSuccess;
+    Workspace *ws = noalloc_workspace(e);
+    if (!ws) return hip_err(e, where);
+    hipStream_t s = (hipStream_t)stream;
+    e = ws->begin(s);
+    timing_suspend(true);  // phase timing would create events
+    if (e == hipSuccess)
+        e = knn<float>(x, y, z, m, np, type, type ? cand_ptype : PGALF_TYPE_ALL, k, nbr,
+                       rho_mode ? rho : nullptr, rho_mode, *ws, s);
+    timing_suspend(false);
+    if (e == hipErrorOutOfMemory)
+        return set_err(PGALF_ERR_NOMEM, "%s: the %zu-byte static scratch is too small",
+                       where, kNoAllocScratchBytes);
+    return e == hipSuccess ? ok() : hip_err(e, where);
 }

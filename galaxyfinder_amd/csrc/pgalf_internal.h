@@ -90,7 +90,13 @@ class Workspace {
         size_t chunk, used;
     };
     explicit Workspace(int device) : device_(device) {}
+    // A fixed arena over caller-owned device memory [base, base + bytes)
+    // (256-byte aligned): it never allocates, frees or creates events; the
+    // status word is its first 256 bytes, and an allocation that does not
+    // fit returns nullptr.
+    Workspace(int device, void *base, size_t bytes);
     ~Workspace();
+    bool fixed() const { return fixed_; }
     hipError_t begin(hipStream_t s);
     // The call on stream s has queued all its work (see the class comment).
     void end(hipStream_t s);
@@ -119,6 +125,7 @@ class Workspace {
     hipEvent_t done_ = nullptr;  // recorded by end(): the last call's work
     hipStream_t last_ = nullptr;
     bool has_last_ = false;
+    bool fixed_ = false;
     hipError_t add_chunk(size_t bytes);
 };
 

@@ -131,6 +131,19 @@ int pgalf_dev_findden(const float *x, const float *y, const float *z, long long 
 int pgalf_dev_knn(const float *x, const float *y, const float *z, const float *m,
                   const int *type, long long np, int cand_ptype, int k,
                   long long *nbr, float *rho, int rho_mode, void *stream);
+/* pgalf_dev_knn without any allocation (no counterpart in the reference; for
+ * callers that cannot allocate: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and results as
+ * pgalf_dev_knn; its scratch is a static 64 MiB device array, one per GPU,
+ * held until the process ends, so np is limited to PGALF_KNN_NOALLOC_MAX
+ * (PGALF_ERR_ARG beyond it).  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (other streams, other host threads' streams)
+ * race and must be ordered by the caller. */
+#define PGALF_KNN_NOALLOC_MAX 524288LL
+int pgalf_dev_knn_noalloc(const float *x, const float *y, const float *z, const float *m,
+                          const int *type, long long np, int cand_ptype, int k,
+                          long long *nbr, float *rho, int rho_mode, void *stream);
+long long pgalf_dev_knn_noalloc_max(void);
 /* deposit -> smooth -> read-back -> kNN lists + W4 density, one call. */
 int pgalf_dev_density_stage(const float *x, const float *y, const float *z,
                             const float *m, long long np, const pgalf_grid_t *g,
