@@ -108,6 +108,8 @@ def ref_lib():
         R.ref_knn_lag.argtypes = [_dp, _dp, _dp, _i64, _i, _d, _d, _d, _d, _i, _i, _i, _lp,
                                   _i, _i64, ctypes.POINTER(ctypes.c_longlong),
                                   ctypes.POINTER(ctypes.c_double)]
+        R.ref_knn_query.restype = _i64
+        R.ref_knn_query.argtypes = [_dp, _dp, _dp, _i64, _i, _lp, _i64, _lp, _i]
         R.ref_findsphdensity.restype = None
         R.ref_findsphdensity.argtypes = [_dp, _dp, _dp, _dp, _i64, _i, _lp, _fp, _i]
         R.ref_sizeof_bp.restype = _i

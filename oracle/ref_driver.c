@@ -308,3 +308,30 @@ long long ref_knn_lag(const double *x, const double *y, const double *z, long lo
     free(ptl);
     return shortq;
 }
+
+/* Cold-start Find_Near_PreCopied for the nq particles qidx[] among all np
+ * (tree over all, as ref_knn_precopied).  Returns the number of short lists. */
+long long ref_knn_query(const double *x, const double *y, const double *z, long long np,
+                        int k, const long long *qidx, long long nq, long long *idx,
+                        int nthreads) {
+    TPtlStruct *ptl;
+    TStruct *tree = ref_tree(x, y, z, np, &ptl);
+    long long shortq = 0;
+    omp_set_num_threads(nthreads);
+#pragma omp parallel reduction(+ : shortq)
+    {
+        ref_neighbor nb[MAX_NUM_NEAR];
+#pragma omp for schedule(dynamic, 16)
+        for (long long t = 0; t < nq; t++) {
+            const long long i = qidx[t];
+            particle pp;
+            pp.x = x[i]; pp.y = y[i]; pp.z = z[i];
+            int n = Find_Near_PreCopied(&pp, k, tree, ptl, nb, 0);
+            if (n != k) shortq++;
+            for (int j = 0; j < k; j++) idx[t * k + j] = j < n ? nb[j].indx : -1;
+        }
+    }
+    free(tree);
+    free(ptl);
+    return shortq;
+}

@@ -16,7 +16,19 @@ Fixtures (inputs and the reference's outputs):
   knn_lattice.npz 6x6x6 lattice with duplicated points: reference lists
                   (tie order first-seen, nnost.c:315).
   w4_table.npz    W4(dist, h) of nnost.c:536 on a grid of (dist, h).
+
+With the argument ``config``, fixtures at the sizes of the two one-halo
+workloads instead (their particles are regenerated by
+galaxyfinder_amd.halos.single_halo_config; a CRC of the float32 positions is
+stored to catch a different draw):
+  config_c1.npz   131072 particles, 64^3: assign_density_TSC at 8 OpenMP
+                  threads (ptype 5); findDen of that grid at 32768 sampled
+                  particles; lagFindStellarCore's warm-started 32-NN lists
+                  (nnost.c:1361-1529) of 4096 sampled particles.
+  config_c2.npz   4194304 particles: cold-start Find_Near_PreCopied 32-NN
+                  lists over the full octree of 4096 sampled particles.
 """
+import zlib
 import os
 import sys
 
@@ -77,12 +89,58 @@ def ref_knn_warm(R, pos, k, grid, nth=4):
     return idx
 
 
+def pos_crc(pos):
+    return zlib.crc32(np.ascontiguousarray(pos, dtype=np.float32).tobytes())
+
+
+def config_fixtures(R):
+    import ctypes
+    k = 32
+    # C1
+    pos32, mass32, spec = halos.single_halo_config(1)
+    n, ng = len(pos32), spec["nx"]
+    grid = (ng, ng, ng, spec["xmin"], spec["ymin"], spec["zmin"], spec["cellsize"])
+    pos, mass = pos32.astype(np.float64), mass32.astype(np.float64)
+    types = np.ones(n, dtype=np.int32)
+    den = ref_tsc(R, pos, mass, types, grid, 5, 8)
+    rng = np.random.default_rng(20261203 + 101)
+    fsel = np.sort(rng.choice(n, 32768, replace=False))
+    dens = ref_findden(R, pos, grid, den, 8)[fsel]
+    x, y, z = _xyz(pos)
+    idx = np.zeros((n, k), dtype=np.int64)
+    nq, tt = ctypes.c_longlong(), ctypes.c_double()
+    assert R.ref_knn_lag(O._p(x, O._dp), O._p(y, O._dp), O._p(z, O._dp), n, k, *grid[3:],
+                         *grid[:3], O._p(idx, O._lp), 8, 1, ctypes.byref(nq),
+                         ctypes.byref(tt)) == 0
+    ksel = np.sort(rng.choice(n, 4096, replace=False))
+    np.savez_compressed(os.path.join(HERE, "config_c1.npz"), crc=pos_crc(pos32), n=n,
+                        grid=np.array(grid, dtype=np.float64), den_t8=den,
+                        findden_sel=fsel.astype(np.int32), findden=dens,
+                        knn_sel=ksel.astype(np.int32), knn_warm=idx[ksel].astype(np.int32))
+    # C2
+    pos32, mass32, spec = halos.single_halo_config(2)
+    n = len(pos32)
+    x, y, z = _xyz(pos32.astype(np.float64))
+    ksel = np.sort(np.random.default_rng(20261203 + 102).choice(n, 4096, replace=False))
+    out = np.zeros((len(ksel), k), dtype=np.int64)
+    assert R.ref_knn_query(O._p(x, O._dp), O._p(y, O._dp), O._p(z, O._dp), n, k,
+                           O._p(np.ascontiguousarray(ksel, dtype=np.int64), O._lp),
+                           len(ksel), O._p(out, O._lp), 8) == 0
+    np.savez_compressed(os.path.join(HERE, "config_c2.npz"), crc=pos_crc(pos32), n=n,
+                        knn_sel=ksel.astype(np.int32), knn_cold=out.astype(np.int32))
+
+
 def main():
     O.build(ref=True)
     R = O.ref_lib()
     if R is None:
         raise SystemExit("reference build missing (make -C oracle ref)")
     R.ref_init(4 << 30)
+    if len(sys.argv) > 1 and sys.argv[1] == "config":
+        config_fixtures(R)
+        for f in ("config_c1.npz", "config_c2.npz"):
+            print(f, os.path.getsize(os.path.join(HERE, f)))
+        return
 
     pos32, mass32 = halos.small_halo(4096, seed=1)
     pos = pos32.astype(np.float64)

@@ -1,0 +1,92 @@
+"""The HIP path against the reference's own outputs at the sizes of the two
+one-halo workloads (fixtures written by tests/golden/make_golden.py config
+from the reference sources compiled in place):
+
+  * 131072 particles, 64^3: the deposit against assign_density_TSC at 8
+    OpenMP threads (normwise <= 1e-6, pointwise <= 1e-5 on cells >= 1e-3 of
+    the maximum: the reference sums in FP32 in its slab order, this build
+    rounds the FP64 sum once); findDen of that grid bit-exact at 32768
+    sampled particles; 32-NN lists bit-exact against lagFindStellarCore's
+    warm-started builder at 4096 sampled particles;
+  * 4194304 particles: 32-NN lists bit-exact against cold-start
+    Find_Near_PreCopied over the full octree at 4096 sampled particles.
+"""
+import os
+import sys
+import zlib
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+from galaxyfinder_amd import density as D  # noqa: E402
+from galaxyfinder_amd import halos  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def say(msg):
+    print("[case] " + msg, flush=True)
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def _load(c):
+    f = np.load(os.path.join(GOLDEN, "config_c%d.npz" % c))
+    pos, mass, spec = halos.single_halo_config(c)
+    assert zlib.crc32(np.ascontiguousarray(pos).tobytes()) == int(f["crc"])
+    return f, pos, mass, spec
+
+
+def _lists(pos, k=32):
+    import torch
+    n = len(pos)
+    x, y, z = (_dev(pos[:, a]) for a in range(3))
+    nbr = torch.empty((n, k), dtype=torch.int64, device="cuda")
+    D.dev_knn(x, y, z, k, nbr=nbr)
+    D.dev_status()
+    return nbr.cpu().numpy()
+
+
+def test_workload_c1_against_reference_outputs():
+    import torch
+    f, pos, mass, spec = _load(1)
+    n = len(pos)
+    g = tuple(f["grid"])
+    grid = D.grid_struct(int(g[0]), int(g[1]), int(g[2]), *g[3:])
+    say("C1 deposit vs reference (8 threads), N=%d" % n)
+    x, y, z = (_dev(pos[:, a]) for a in range(3))
+    m = _dev(mass)
+    den = torch.empty(D.padded_shape(grid.nx, grid.ny, grid.nz), dtype=torch.float32,
+                      device="cuda")
+    D.dev_tsc(x, y, z, m, grid, den)
+    D.dev_status()
+    got = den.cpu().numpy()[..., :grid.nx].astype(np.float64)
+    ref = f["den_t8"][..., :grid.nx].astype(np.float64)
+    nrm = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    mask = np.abs(ref) >= 1e-3 * np.abs(ref).max()
+    pt = (np.abs(got - ref)[mask] / np.abs(ref)[mask]).max()
+    say("  normwise %.2e, pointwise %.2e" % (nrm, pt))
+    assert nrm <= 1e-6 and pt <= 1e-5
+    say("C1 findDen vs reference")
+    rb = torch.empty(n, dtype=torch.float32, device="cuda")
+    D.dev_findden(x, y, z, grid, _dev(f["den_t8"]), rb)
+    D.dev_status()
+    sel = f["findden_sel"].astype(np.int64)
+    assert np.array_equal(rb.cpu().numpy()[sel].view(np.int32), f["findden"].view(np.int32))
+    say("C1 lists vs lagFindStellarCore's warm-started builder")
+    nbr = _lists(pos)
+    assert np.array_equal(nbr[f["knn_sel"].astype(np.int64)], f["knn_warm"])
+
+
+def test_workload_c2_lists_against_reference():
+    f, pos, _, _ = _load(2)
+    say("C2 lists vs reference Find_Near_PreCopied, N=%d" % len(pos))
+    nbr = _lists(pos)
+    assert np.array_equal(nbr[f["knn_sel"].astype(np.int64)], f["knn_cold"])

@@ -294,3 +294,36 @@ def test_reference_find_near_is_defective(ref):
                          600, 32, O._p(idx, O._lp), O._p(dens, O._fp), 1)
     exact, _ = O.knn(p, 32)
     assert (idx == exact).all(axis=1).sum() < 600
+
+
+# ---- the reference's outputs at the one-halo workload sizes -------------------------
+def _config_fixture(c):
+    import zlib
+    f = np.load(os.path.join(os.path.dirname(__file__), "golden", "config_c%d.npz" % c))
+    pos, mass, spec = halos.single_halo_config(c)
+    assert zlib.crc32(np.ascontiguousarray(pos).tobytes()) == int(f["crc"]), \
+        "the workload generator drew different particles"
+    return f, pos, mass, spec
+
+
+def test_oracle_vs_reference_fixtures_at_workload_size():
+    """The oracle's exact kNN equals the reference's lists (warm-started
+    lagFindStellarCore builder at 131072 particles, cold full-octree queries
+    at 4194304), and its deposit and read-back equal the reference's."""
+    f, pos, mass, spec = _config_fixture(1)
+    p = pos.astype(np.float64)
+    sel = f["knn_sel"].astype(np.int64)
+    idx, _ = O.knn(p[sel], 32, cpos=p)
+    assert np.array_equal(idx, f["knn_warm"])
+    g = tuple(f["grid"])
+    g = (int(g[0]), int(g[1]), int(g[2])) + g[3:]
+    den, bad = O.tsc_deposit(p, mass.astype(np.float64), g, nthreads=8)
+    assert bad == 0 and np.array_equal(den.view(np.int32), f["den_t8"].view(np.int32))
+    rb, _ = O.findden(p, g, f["den_t8"])
+    fs = f["findden_sel"].astype(np.int64)
+    assert np.array_equal(rb[fs].view(np.int32), f["findden"].view(np.int32))
+    f2, pos2, _, _ = _config_fixture(2)
+    p2 = pos2.astype(np.float64)
+    sel2 = f2["knn_sel"].astype(np.int64)
+    idx2, _ = O.knn(p2[sel2], 32, cpos=p2)
+    assert np.array_equal(idx2, f2["knn_cold"])
