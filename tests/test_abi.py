@@ -123,3 +123,18 @@ def test_vectorised_catalogue():
         assert np.allclose(m[off[h]:off[h + 1]], 1.0 / c[h])
         # each halo is centred in its own box
         assert np.abs(p[off[h]:off[h + 1]].mean(axis=0) - 0.5).max() < 0.15
+
+
+@pytest.mark.skipif(not os.path.isdir("/root/reference/NewGalFinder"),
+                    reason="needs the reference sources (build container only)")
+def test_reference_nnost_links_against_the_library():
+    """the reference's own nnost.o, linked without
+    tsc_omp2.o and gsmooth.o, takes assign_density_TSC, gaussian_Smoothing
+    and findDen from libpgalf_density.so (link only, no call)."""
+    import subprocess
+    r = subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "-B", "linkcheck"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    for sym in ("assign_density_TSC", "gaussian_Smoothing", "findDen"):
+        assert "nnost.o: reference to %s" % sym in r.stderr
+        assert "libpgalf_density.so: definition of %s" % sym in r.stderr
