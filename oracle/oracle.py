@@ -79,6 +79,11 @@ def lib():
         L.orc_sph_grad_d.restype = None
         L.orc_sph_grad_d.argtypes = [_dp, _dp, _dp, _dp, _lp, _i64, _i, _dp, _dp, _dp, _dp,
                                      _dp]
+        L.orc_grid_cores.restype = _i64
+        L.orc_grid_cores.argtypes = [_dp, _dp, _dp, _ip, _i64, _i, _fp, _fp, _i, _i, _i, _d,
+                                     _d, _d, _d, _d, _i, _lp, _i64]
+        L.orc_finddenpeak.restype = _i64
+        L.orc_finddenpeak.argtypes = [_fp, _i, _lp, _i64, _ip, _i, _d, _lp, _i64]
         _LIB = L
     return _LIB
 
@@ -110,6 +115,9 @@ def ref_lib():
                                   ctypes.POINTER(ctypes.c_double)]
         R.ref_knn_query.restype = _i64
         R.ref_knn_query.argtypes = [_dp, _dp, _dp, _i64, _i, _lp, _i64, _lp, _i]
+        R.ref_finddenpeak.restype = _i64
+        R.ref_finddenpeak.argtypes = [_fp, _i, _lp, _i64, _dp, _dp, _dp, _ip, _i, _ip, _fp,
+                                      _fp, _fp, _fp, _i64]
         R.ref_findsphdensity.restype = None
         R.ref_findsphdensity.argtypes = [_dp, _dp, _dp, _dp, _i64, _i, _lp, _fp, _i]
         R.ref_sizeof_bp.restype = _i
@@ -390,3 +398,41 @@ def peak_climb(pos, grid, den_padded, niter=5, step=0.5):
         moving = moved
     peak = (c[:, 0] + nx * (c[:, 1] + ny * c[:, 2])).astype(np.int32)
     return out, peak
+
+
+# ---------------------------------------------------------------------------
+# integer peak stage
+# ---------------------------------------------------------------------------
+PEAKTHRESHOLD = 2000.0  # params.h:19
+NCELLBUFF = 10          # params.h:115
+
+
+def grid_cores(pos, types, ptype, densph, den_padded, grid, threshold=PEAKTHRESHOLD,
+               nbuff=NCELLBUFF):
+    """Grid cores of lagFindStellarCore (nnost.c:1115-1271): core particle
+    indices in (k, j, i) node order (orc_grid_cores)."""
+    nx, ny, nz, x0, y0, z0, h = grid
+    x, y, z = _xyz(pos)
+    t = np.ascontiguousarray(types if types is not None else np.ones(len(x)), dtype=np.int32)
+    d = np.ascontiguousarray(densph, dtype=np.float32)
+    g = np.ascontiguousarray(den_padded, dtype=np.float32)
+    cap = max(1, g.size // 27)
+    out = np.zeros(cap, dtype=np.int64)
+    n = lib().orc_grid_cores(_p(x, _dp), _p(y, _dp), _p(z, _dp), _p(t, _ip), len(x), ptype,
+                             _p(d, _fp), _p(g, _fp), nx, ny, nz, x0, y0, z0, h, threshold,
+                             nbuff, _p(out, _lp), cap)
+    assert n <= cap
+    return out[:n].copy()
+
+
+def finddenpeak(den, nbr, types=None, jflag=0, threshold=PEAKTHRESHOLD):
+    """finddenpeak (subhaloden.mod6.c:416-492): core particle indices in
+    index order (orc_finddenpeak)."""
+    d = np.ascontiguousarray(den, dtype=np.float32)
+    nb = np.ascontiguousarray(nbr, dtype=np.int64)
+    n, k = nb.shape
+    t = np.ascontiguousarray(types if types is not None else np.ones(n), dtype=np.int32)
+    out = np.zeros(n, dtype=np.int64)
+    c = lib().orc_finddenpeak(_p(d, _fp), k, _p(nb, _lp), n, _p(t, _ip), jflag, threshold,
+                              _p(out, _lp), n)
+    return out[:c].copy()

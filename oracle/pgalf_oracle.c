@@ -25,6 +25,14 @@
  *                            Find_Near_PreCopied lists in golden fixtures.
  *   orc_W4, orc_sph_*     -- vs the reference W4 (golden table) and closed
  *                            forms W(0,h)=1/(pi h^3), W(2h)=0.
+ *   orc_finddenpeak       -- bit-exact vs the reference build's finddenpeak
+ *                            (subhaloden.mod6.c compiled in place) on golden
+ *                            densities and lists.
+ *   orc_grid_cores        -- the reference block is inline in
+ *                            lagFindStellarCore (unbuildable: it calls the
+ *                            FFTW smoothing): pinned by an independent
+ *                            vectorised brute force and dedicated cases
+ *                            (plateaus, ties, the particle-0 skip).
  *
  * Build: oracle/Makefile (gcc -O2 -fopenmp, no FMA contraction).
  */
@@ -910,4 +918,100 @@ void orc_sph_grad_d(const double *x, const double *y, const double *z, const dou
             gx[K] -= ex; gy[K] -= ey; gz[K] -= ez;
         }
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Integer peak stage.                                                      */
+/* ------------------------------------------------------------------------ */
+#define ORC_TYPE_STAR 1
+
+/* Grid cores of lagFindStellarCore (nnost.c:1115-1271).  Particles (stars
+ * for ptype 1, all for ptype 5, none otherwise) are chained per nearest grid
+ * node (rint of the offset in cells) by head insertion in index order, so a
+ * node's chain runs from its largest index down (nnost.c:1121-1156).  A node
+ * of the interior [nbuff/2, n - nbuff/2) on every axis whose smoothed value
+ * exceeds `threshold` and every one of its 26 neighbours, and whose chain is
+ * not empty, gives a core: the first chain member of maximal densph
+ * (strictly greater replaces, nnost.c:1243-1249) -- unless the chain's last
+ * member, its smallest index, is particle 0 (`if(ibp > 0)`, nnost.c:1202,
+ * 1254).  Cores in (k, j, i) node order, as the OpenMP static schedule over k
+ * concatenates them (nnost.c:1216-1266).  Writes up to maxcore core
+ * particles; returns the number of cores. */
+long long orc_grid_cores(const double *x, const double *y, const double *z, const int *type,
+                         long long np, int ptype, const float *densph, const float *den,
+                         int nx, int ny, int nz, double xmin, double ymin, double zmin,
+                         double cellsize, double threshold, int nbuff, long long *core_peak,
+                         long long maxcore) {
+    const long long mx = 2LL * (nx / 2 + 1);
+    const long long ncells = mx * (long long)ny * nz;
+    long long *head = (long long *)malloc(sizeof(long long) * (size_t)ncells);
+    long long *next = (long long *)malloc(sizeof(long long) * (size_t)(np > 0 ? np : 1));
+    for (long long c = 0; c < ncells; c++) head[c] = -1;
+    if (ptype == ORC_TYPE_STAR || ptype == TYPE_ALL) {
+        for (long long i = 0; i < np; i++) {
+            if (ptype == ORC_TYPE_STAR && type[i] != ORC_TYPE_STAR) continue;
+            const long long a = (long long)rint((x[i] - xmin) / cellsize);
+            const long long b = (long long)rint((y[i] - ymin) / cellsize);
+            const long long c = (long long)rint((z[i] - zmin) / cellsize);
+            const long long cell = a + mx * (b + ny * c);
+            next[i] = head[cell];
+            head[cell] = i;
+        }
+    }
+    long long ncore = 0;
+    const int h = nbuff / 2;
+    for (int k = h; k < nz - h; k++)
+        for (int j = h; j < ny - h; j++)
+            for (int i = h; i < nx - h; i++) {
+                const long long cell = i + mx * ((long long)j + (long long)ny * k);
+                const float v = den[cell];
+                if (!((double)v > threshold) || head[cell] < 0) continue;
+                int peak = 1;
+                for (int dk = -1; dk <= 1 && peak; dk++)
+                    for (int dj = -1; dj <= 1 && peak; dj++)
+                        for (int di = -1; di <= 1 && peak; di++) {
+                            if (!di && !dj && !dk) continue;
+                            const long long o = cell + di + mx * (dj + (long long)ny * dk);
+                            if (!(v > den[o])) peak = 0;
+                        }
+                if (!peak) continue;
+                float best = -1.e20f;
+                long long arg = -1, last = -1;
+                for (long long p = head[cell]; p >= 0; p = next[p]) {
+                    if (densph[p] > best) {
+                        best = densph[p];
+                        arg = p;
+                    }
+                    last = p;
+                }
+                if (last > 0) {
+                    if (ncore < maxcore) core_peak[ncore] = arg;
+                    ncore++;
+                }
+            }
+    free(head);
+    free(next);
+    return ncore;
+}
+
+/* finddenpeak (subhaloden.mod6.c:416-492): particle i is a core when its
+ * density exceeds `threshold` (PEAKTHRESHOLD in the reference), it is a star
+ * if jflag == 1, and no particle of its neighbour list has a strictly larger
+ * density.  Cores in index order; writes up to maxcore; returns the count. */
+long long orc_finddenpeak(const float *den, int k, const long long *nbr, long long np,
+                          const int *type, int jflag, double threshold, long long *core_peak,
+                          long long maxcore) {
+    long long ncore = 0;
+    for (long long i = 0; i < np; i++) {
+        if (!((double)den[i] > threshold)) continue;
+        if (jflag == 1 && type[i] != ORC_TYPE_STAR) continue;
+        int peak = 1;
+        for (int t = 0; t < k && peak; t++)
+            if (den[nbr[i * k + t]] > den[i]) peak = 0;
+        if (peak) {
+            if (ncore < maxcore) core_peak[ncore] = i;
+            ncore++;
+        }
+    }
+    return ncore;
 }

@@ -335,3 +335,29 @@ long long ref_knn_query(const double *x, const double *y, const double *z, long 
     free(ptl);
     return shortq;
 }
+
+/* The reference's finddenpeak (subhaloden.mod6.c:416-492) on densities and
+ * neighbour lists given by the caller: cores (peak particle, its position as
+ * the record's floats, its density) copied out, at most maxout of them.
+ * Returns the number of cores. */
+int finddenpeak(float *, int, long long *, int, Coretype **, int, SimpleBasicParticleType *);
+
+long long ref_finddenpeak(const float *den, int k, const long long *nbr, long long np,
+                          const double *x, const double *y, const double *z, const int *type,
+                          int jflag, int *peak, float *cx, float *cy, float *cz, float *cden,
+                          long long maxout) {
+    ref_init(4LL << 30);
+    SimpleBasicParticleType *bp = ref_pack(x, y, z, NULL, type, np);
+    Coretype *core = (Coretype *)Malloc(sizeof(Coretype) * MAXNUMCORE, PPTR(core));
+    int n = finddenpeak((float *)den, k, (long long *)nbr, (int)np, &core, jflag, bp);
+    for (long long i = 0; i < n && i < maxout; i++) {
+        peak[i] = core[i].peak;
+        cx[i] = core[i].cx;
+        cy[i] = core[i].cy;
+        cz[i] = core[i].cz;
+        cden[i] = core[i].density;
+    }
+    Free(core);
+    free(bp);
+    return n;
+}

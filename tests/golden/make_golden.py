@@ -17,6 +17,14 @@ Fixtures (inputs and the reference's outputs):
                   (tie order first-seen, nnost.c:315).
   w4_table.npz    W4(dist, h) of nnost.c:536 on a grid of (dist, h).
 
+With the argument ``peaks``:
+  peaks_small.npz the 4096-particle halo's W4 kNN densities over its
+                  reference lists with seeded noise, scaled about
+                  PEAKTHRESHOLD and quantised so that neighbours tie, and
+                  uniform random densities quantised to 50 (more ties); the
+                  reference finddenpeak's cores (subhaloden.mod6.c:416-492)
+                  for jflag 0 and 1 on each.
+
 With the argument ``config``, fixtures at the sizes of the two one-halo
 workloads instead (their particles are regenerated by
 galaxyfinder_amd.halos.single_halo_config; a CRC of the float32 positions is
@@ -130,12 +138,51 @@ def config_fixtures(R):
                         knn_sel=ksel.astype(np.int32), knn_cold=out.astype(np.int32))
 
 
+def peaks_fixture(R):
+    kf = np.load(os.path.join(HERE, "knn_small.npz"))
+    tf = np.load(os.path.join(HERE, "tsc_small.npz"))
+    pos, types = kf["pos"], tf["types"]
+    idx = kf["idx_cold"].astype(np.int64)
+    n, k = idx.shape
+    _, d2 = O.knn(pos, k)
+    rho = O.sph_density(idx, d2).astype(np.float64)
+    # seeded multiplicative noise for many local maxima, about a third above
+    # the threshold, quantised to 4 units so that neighbouring densities tie
+    rho = rho * np.random.default_rng(7).uniform(0.7, 1.3, n)
+    scale = 2000.0 / np.quantile(rho, 0.66)
+    den = (np.round(rho * scale / 4.0) * 4.0).astype(np.float32)
+    # and uniform random densities in [0, 3000) quantised to 50: many ties
+    den_b = (np.floor(np.random.default_rng(8).uniform(0.0, 3000.0, n) / 50.0) *
+             50.0).astype(np.float32)
+    x, y, z = _xyz(pos)
+    out = dict(den_a=den, den_b=den_b, nbr=idx.astype(np.int32), types=types)
+    for tag, den, jflag in (("a", den, 0), ("a", den, 1), ("b", den_b, 0), ("b", den_b, 1)):
+        cap = n
+        pk = np.zeros(cap, dtype=np.int32)
+        cx, cy, cz, cd = (np.zeros(cap, dtype=np.float32) for _ in range(4))
+        nb = np.ascontiguousarray(idx)
+        c = R.ref_finddenpeak(O._p(den, O._fp), k, O._p(nb, O._lp), n, O._p(x, O._dp),
+                              O._p(y, O._dp), O._p(z, O._dp),
+                              O._p(np.ascontiguousarray(types, dtype=np.int32), O._ip), jflag,
+                              O._p(pk, O._ip), O._p(cx, O._fp), O._p(cy, O._fp), O._p(cz, O._fp),
+                              O._p(cd, O._fp), cap)
+        key = "%s_j%d" % (tag, jflag)
+        out["peak_" + key] = pk[:c]
+        out["cxyz_" + key] = np.stack([cx[:c], cy[:c], cz[:c]], axis=1)
+        out["cden_" + key] = cd[:c]
+    np.savez_compressed(os.path.join(HERE, "peaks_small.npz"), **out)
+
+
 def main():
     O.build(ref=True)
     R = O.ref_lib()
     if R is None:
         raise SystemExit("reference build missing (make -C oracle ref)")
     R.ref_init(4 << 30)
+    if len(sys.argv) > 1 and sys.argv[1] == "peaks":
+        peaks_fixture(R)
+        print("peaks_small.npz", os.path.getsize(os.path.join(HERE, "peaks_small.npz")))
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "config":
         config_fixtures(R)
         for f in ("config_c1.npz", "config_c2.npz"):

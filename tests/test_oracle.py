@@ -327,3 +327,98 @@ def test_oracle_vs_reference_fixtures_at_workload_size():
     sel2 = f2["knn_sel"].astype(np.int64)
     idx2, _ = O.knn(p2[sel2], 32, cpos=p2)
     assert np.array_equal(idx2, f2["knn_cold"])
+
+
+# ---- integer peak stage ---------------------------------------------------------------
+def test_finddenpeak_vs_reference_fixture():
+    """orc_finddenpeak equals the reference's finddenpeak (subhaloden.mod6.c
+    compiled in place) on densities with neighbour ties, jflag 0 and 1; the
+    reference's core record holds the particle's position and density."""
+    f = np.load(os.path.join(os.path.dirname(__file__), "golden", "peaks_small.npz"))
+    pos = np.load(os.path.join(os.path.dirname(__file__), "golden", "knn_small.npz"))["pos"]
+    for tag in "ab":
+        for jflag in (0, 1):
+            key = "%s_j%d" % (tag, jflag)
+            got = O.finddenpeak(f["den_" + tag], f["nbr"], f["types"], jflag)
+            assert np.array_equal(got, f["peak_" + key])
+            assert np.array_equal(pos[got].astype(np.float32), f["cxyz_" + key])
+            assert np.array_equal(f["den_" + tag][got], f["cden_" + key])
+
+
+def _brute_grid_cores(pos, types, ptype, densph, den, grid, thr, nbuff):
+    """Grid cores by array operations (independent of the oracle's chains)."""
+    nx, ny, nz, x0, y0, z0, h = grid
+    G = den[..., :nx].astype(np.float64)
+    hb = nbuff // 2
+    c = G[hb:nz - hb, hb:ny - hb, hb:nx - hb]
+    peak = c > thr
+    for dk in (-1, 0, 1):
+        for dj in (-1, 0, 1):
+            for di in (-1, 0, 1):
+                if dk == dj == di == 0:
+                    continue
+                peak &= c > G[hb + dk:nz - hb + dk, hb + dj:ny - hb + dj, hb + di:nx - hb + di]
+    sel = np.ones(len(pos), bool) if ptype == 5 else (types == 1 if ptype == 1 else
+                                                       np.zeros(len(pos), bool))
+    ijk = np.rint((pos - np.array([x0, y0, z0])) / h).astype(np.int64)
+    cell = ijk[:, 0] + nx * (ijk[:, 1] + ny * ijk[:, 2])
+    out = []
+    for k, j, i in zip(*np.nonzero(peak)):  # C order: (k, j, i) ascending
+        ci = (i + hb) + nx * ((j + hb) + ny * (k + hb))
+        mem = np.flatnonzero(sel & (cell == ci))
+        if len(mem) == 0 or mem.min() == 0:
+            continue
+        best = densph[mem].max()
+        out.append(mem[densph[mem] == best].max())
+    return np.array(out, dtype=np.int64)
+
+
+def _planted_grid(seed, n=4000, ng=24):
+    rng = np.random.default_rng(seed)
+    grid = (ng, ng + 2, ng - 1, -0.1, 0.05, 0.0, 0.05)
+    nx, ny, nz, x0, y0, z0, h = grid
+    den = np.zeros(O.padded_shape(nx, ny, nz), dtype=np.float32)
+    den[..., :nx] = np.round(rng.uniform(0, 3000, (nz, ny, nx)) / 100) * 100  # plateaus
+    ijk = np.stack([rng.integers(4, nx - 4, n), rng.integers(4, ny - 4, n),
+                    rng.integers(4, nz - 4, n)], axis=1)
+    pos = np.array([x0, y0, z0]) + (ijk + rng.uniform(-0.49, 0.49, (n, 3))) * h
+    densph = (np.round(rng.uniform(0, 50, n)) * 2).astype(np.float32)  # ties inside cells
+    types = rng.integers(1, 3, n).astype(np.int32)
+    return pos, types, densph, den, grid
+
+
+def test_grid_cores_vs_brute_force():
+    for seed in (1, 2, 3):
+        pos, types, densph, den, grid = _planted_grid(seed)
+        for ptype in (5, 1, 2):
+            got = O.grid_cores(pos, types, ptype, densph, den, grid, 2000.0, 10)
+            ref = _brute_grid_cores(pos, types, ptype, densph, den, grid, 2000.0, 10)
+            assert np.array_equal(got, ref), (seed, ptype)
+            if ptype != 2:
+                assert len(got) > 5
+
+
+def test_grid_cores_particle_zero_and_ties():
+    """The reference skips a peak cell whose chain ends at particle 0
+    (if(ibp>0), nnost.c:1202,1254) -- only when particle 0 is in the chain
+    (its type selected); ties of densph in a cell go to the higher index;
+    a cell equal to a neighbour, or to the threshold, is no peak."""
+    ng = 16
+    grid = (ng, ng, ng, 0.0, 0.0, 0.0, 1.0)
+    den = np.zeros(O.padded_shape(ng, ng, ng), dtype=np.float32)
+    den[8, 8, 8] = 5000.0           # peak A, holds particles 0, 3, 4
+    den[8, 8, 10] = 4000.0          # peak B, holds 1 and 2 (densph tie)
+    den[10, 10, 6] = den[10, 10, 7] = 4500.0  # plateau: no peak
+    den[6, 9, 9] = 2000.0           # at the threshold: no peak
+    pos = np.array([[8, 8, 8], [10, 8, 8], [10.2, 8, 8], [8.3, 8, 8], [7.7, 8.2, 8],
+                    [6, 10, 10], [7, 10, 10], [9, 9, 6]], dtype=np.float64)
+    densph = np.array([9, 5, 5, 7, 8, 1, 1, 1], dtype=np.float32)
+    types = np.array([1, 2, 2, 1, 1, 1, 1, 1], dtype=np.int32)
+    assert O.grid_cores(pos, types, 5, densph, den, grid).tolist() == [2]
+    assert _brute_grid_cores(pos, types, 5, densph, den, grid, 2000.0, 10).tolist() == [2]
+    # particle 0 made a DM particle: the star chains (ptype 1) do not hold it,
+    # so peak A gives its densest star; all-type chains (ptype 5) still do
+    types[0] = 2
+    assert O.grid_cores(pos, types, 1, densph, den, grid).tolist() == [4]
+    assert O.grid_cores(pos, types, 5, densph, den, grid).tolist() == [2]
+    assert _brute_grid_cores(pos, types, 1, densph, den, grid, 2000.0, 10).tolist() == [4]
