@@ -57,6 +57,14 @@ SIGNATURES = {
                                                 _c_p, _c_p, _c_i, _c_p, _c_p, _c_p, _c_p,
                                                 _c_p, _c_p, _c_p]),
     "pgalf_dev_status": (_c_i, [_c_p]),
+    "pgalf_grid_cores": (_c_i, [_c_p, _c_i, _c_p, _c_p, _c_i, _c_i, _c_i, _c_d, _c_d, _c_d,
+                                _c_d, _c_i, _c_f, _c_i, _c_p, _c_i, _c_p]),
+    "pgalf_finddenpeak": (_c_i, [_c_p, _c_i, _c_p, _c_i, _c_i, _c_p, _c_p, _c_i, _c_p]),
+    "pgalf_dev_grid_cores": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_ll, _c_i, _c_p,
+                                    ctypes.POINTER(pgalf_grid_t), _c_p, _c_f, _c_i, _c_p, _c_ll,
+                                    _c_p, _c_p]),
+    "pgalf_dev_finddenpeak": (_c_i, [_c_p, _c_i, _c_p, _c_ll, _c_p, _c_i, _c_f, _c_p, _c_ll,
+                                     _c_p, _c_p]),
     "pgalf_batch_create": (_c_p, [_c_i, _c_p, _c_p, _c_p, _c_i]),
     "pgalf_batch_grid_offset": (_c_ll, [_c_p, _c_i]),
     "pgalf_batch_run": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i,

@@ -818,3 +818,145 @@ extern "C" int pgalf_dev_knn_noalloc(const float *x, const float *y, const float
                        where, kNoAllocScratchBytes);
     return e == hipSuccess ? ok() : hip_err(e, where);
 }
+
+// ---- integer peak stage ------------------------------------------------------------------
+// Copies the cores' particle indices back and fills the reference-layout
+// records (peak, its position as floats, its density; other fields zero).
+static int fill_cores(const int *dcore, unsigned ncore_dev, int maxcore,
+                      const pgalf_particle_t *bp, const float *dens, pgalf_core_t *core,
+                      int *numcore, hipStream_t s, const char *where) {
+    const long long nout = ncore_dev < (unsigned)maxcore ? (long long)ncore_dev : maxcore;
+    std::vector<int> idx((size_t)(nout > 0 ? nout : 1));
+    if (nout > 0) {
+        hipError_t e = hipMemcpyAsync(idx.data(), dcore, sizeof(int) * nout,
+                                      hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_err(e, where);
+    }
+    for (long long c = 0; c < nout; c++) {
+        pgalf_core_t r;
+        memset(&r, 0, sizeof(r));
+        const int p = idx[c];
+        r.peak = p;
+        r.cx = (float)bp[p].x;
+        r.cy = (float)bp[p].y;
+        r.cz = (float)bp[p].z;
+        r.density = dens[p];
+        core[c] = r;
+    }
+    if (numcore) *numcore = (int)ncore_dev;
+    if (ncore_dev > (unsigned)maxcore)
+        return set_err(PGALF_ERR_CORE_OVERFLOW, "%s: %u cores, the array holds %d", where,
+                       ncore_dev, maxcore);
+    return ok();
+}
+
+extern "C" int pgalf_grid_cores(pgalf_particle_t *bp, int np, const float *densph,
+                                const float *denGrid, int nx, int ny, int nz, double xmin,
+                                double ymin, double zmin, double cellsize, int ptype,
+                                float threshold, int nbuff, pgalf_core_t *core, int maxcore,
+                                int *numcore) {
+    const char *where = "pgalf_grid_cores";
+    if (np < 0 || (np > 0 && (!bp || !densph)) || !denGrid || !core || maxcore < 0 ||
+        nbuff < 2 || !check_grid(nx, ny, nz) || !(cellsize > 0))
+        return set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    hipStream_t s = nullptr;
+    Workspace &ws = workspace_for_current_device();
+    WsCall call(ws, s);
+    hipError_t e = call.e;
+    const Grid g = make_grid(nx, ny, nz, xmin, ymin, zmin, cellsize);
+    DeviceParticles d{};
+    if (e == hipSuccess && np > 0) e = upload_particles(bp, np, ws, s, d);
+    float *dden = ws.alloc_n<float>(g.ncells()), *drho = ws.alloc_n<float>(np > 0 ? np : 1);
+    int *dcore = ws.alloc_n<int>(maxcore > 0 ? maxcore : 1);
+    unsigned *dn = ws.alloc_n<unsigned>(1);
+    if (e == hipSuccess && (!dden || !drho || !dcore || !dn)) e = hipErrorOutOfMemory;
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dden, denGrid, sizeof(float) * g.ncells(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && np > 0)
+        e = hipMemcpyAsync(drho, densph, sizeof(float) * np, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = grid_cores<double>(d.x, d.y, d.z, d.type, np, ptype, drho, g, dden,
+                               (double)threshold, nbuff, dcore, maxcore, dn, ws, s);
+    unsigned nc = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&nc, dn, sizeof(unsigned), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_err(e, where);
+    return fill_cores(dcore, nc, maxcore, bp, densph, core, numcore, s, where);
+}
+
+extern "C" int pgalf_finddenpeak(float *den, int numneigh, long long *neighbor, int np,
+                                 int jflag, pgalf_particle_t *bp, pgalf_core_t *core,
+                                 int maxcore, int *numcore) {
+    const char *where = "pgalf_finddenpeak";
+    if (np < 0 || (np > 0 && (!den || !neighbor || !bp)) || numneigh < 1 || !core ||
+        maxcore < 0)
+        return set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    hipStream_t s = nullptr;
+    Workspace &ws = workspace_for_current_device();
+    WsCall call(ws, s);
+    hipError_t e = call.e;
+    DeviceParticles d{};
+    if (e == hipSuccess && np > 0) e = upload_particles(bp, np, ws, s, d);
+    float *dden = ws.alloc_n<float>(np > 0 ? np : 1);
+    long long *dnb = ws.alloc_n<long long>((size_t)(np > 0 ? np : 1) * numneigh);
+    int *dcore = ws.alloc_n<int>(maxcore > 0 ? maxcore : 1);
+    unsigned *dn = ws.alloc_n<unsigned>(1);
+    if (e == hipSuccess && (!dden || !dnb || !dcore || !dn)) e = hipErrorOutOfMemory;
+    if (e == hipSuccess && np > 0) {
+        e = hipMemcpyAsync(dden, den, sizeof(float) * np, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(dnb, neighbor, sizeof(long long) * (size_t)np * numneigh,
+                               hipMemcpyHostToDevice, s);
+    }
+    if (e == hipSuccess)
+        e = find_den_peak(dden, numneigh, dnb, np, d.type, jflag, (double)PGALF_PEAKTHRESHOLD,
+                          dcore, maxcore, dn, ws, s);
+    unsigned nc = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&nc, dn, sizeof(unsigned), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_err(e, where);
+    return fill_cores(dcore, nc, maxcore, bp, den, core, numcore, s, where);
+}
+
+extern "C" int pgalf_dev_grid_cores(const float *x, const float *y, const float *z,
+                                    const int *type, long long np, int ptype,
+                                    const float *densph, const pgalf_grid_t *g, const float *den,
+                                    float threshold, int nbuff, int *core, long long maxcore,
+                                    int *ncore, void *stream) {
+    const char *where = "pgalf_dev_grid_cores";
+    if (!g || !den || !core || !ncore || np < 0 || maxcore < 0 || nbuff < 2 ||
+        (np > 0 && (!x || !y || !z || !densph)) || (ptype == PGALF_TYPE_STAR && !type) ||
+        !check_grid(g->nx, g->ny, g->nz))
+        return set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    hipStream_t s = (hipStream_t)stream;
+    Workspace &ws = workspace_for_current_device();
+    WsCall call(ws, s);
+    hipError_t e = call.e;
+    if (e == hipSuccess)
+        e = grid_cores<float>(x, y, z, type, np, ptype, densph, grid_from(g), den,
+                              (double)threshold, nbuff, core, maxcore,
+                              reinterpret_cast<unsigned *>(ncore), ws, s);
+    return e == hipSuccess ? ok() : hip_err(e, where);
+}
+
+extern "C" int pgalf_dev_finddenpeak(const float *den, int k, const long long *nbr,
+                                     long long np, const int *type, int jflag, float threshold,
+                                     int *core, long long maxcore, int *ncore, void *stream) {
+    const char *where = "pgalf_dev_finddenpeak";
+    if (np < 0 || k < 1 || !core || !ncore || maxcore < 0 || (np > 0 && (!den || !nbr)) ||
+        (jflag == 1 && !type))
+        return set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    hipStream_t s = (hipStream_t)stream;
+    Workspace &ws = workspace_for_current_device();
+    WsCall call(ws, s);
+    hipError_t e = call.e;
+    if (e == hipSuccess)
+        e = find_den_peak(den, k, nbr, np, type, jflag, (double)threshold, core, maxcore,
+                          reinterpret_cast<unsigned *>(ncore), ws, s);
+    return e == hipSuccess ? ok() : hip_err(e, where);
+}

@@ -307,6 +307,21 @@ hipError_t grad_finalize(const double *acc, long long np, float *gx, float *gy, 
 hipError_t unpack_particles(const void *d_bp, long long np, double *x, double *y,
                             double *z, double *m, int *type, hipStream_t s);
 
+// ---- integer peak stage (peaks.hip) --------------------------------------
+// Grid cores of lagFindStellarCore (nnost.c:1115-1271): core particle
+// indices, in (k, j, i) node order, to core[0 .. *ncore) (at most maxcore
+// written; *ncore is a device word and may exceed maxcore).
+template <class P>
+hipError_t grid_cores(const P *x, const P *y, const P *z, const int *type, long long np,
+                      int ptype, const float *densph, const Grid &g, const float *den,
+                      double threshold, int nbuff, int *core, long long maxcore,
+                      unsigned *ncore, Workspace &ws, hipStream_t s);
+// finddenpeak (subhaloden.mod6.c:416-492): core particle indices in index
+// order, as grid_cores.
+hipError_t find_den_peak(const float *den, int k, const long long *nbr, long long np,
+                         const int *type, int jflag, double threshold, int *core,
+                         long long maxcore, unsigned *ncore, Workspace &ws, hipStream_t s);
+
 // ---- phase timing (capi.hip) ---------------------------------------------
 void timing_mark(const char *phase, hipStream_t s, bool end);
 // Phase timing is suspended on this thread while a graph is being captured.

@@ -32,6 +32,19 @@ PARTICLE_DTYPE = np.dtype({
 })
 
 
+# Coretype (tree.h:204-213), 56 bytes.
+CORE_DTYPE = np.dtype({
+    "names": ["peak", "nummem", "numstar", "starmass", "coredensity", "cx", "cy", "cz",
+              "cvx", "cvy", "cvz", "Rtidal", "density", "flag"],
+    "formats": ["<i4", "<i4", "<i4", "<f4", "<f4", "<f4", "<f4", "<f4", "<f4", "<f4", "<f4",
+                "<f4", "<f4", "u1"],
+    "offsets": [0, 4, 8, 12, 16, 20, 24, 28, 32, 36, 40, 44, 48, 52],
+    "itemsize": 56,
+})
+PEAKTHRESHOLD = 2000.0  # params.h:19
+NCELLBUFF = 10          # params.h:115
+
+
 def make_particles(pos, mass=None, types=None):
     """Pack positions (n,3), masses and types into reference particle records."""
     pos = np.asarray(pos, dtype=np.float64)
@@ -142,6 +155,42 @@ def lag_density_stage(bp, Numnear, ptype, want_grid=False):
         libc.free(gptr)
     _lib.check(rc)
     return densph, nbr, grid, tuple(nxyz), tuple(x0)
+
+
+def _cores(fn, maxcore, *args):
+    n = ctypes.c_int(0)
+    core = np.zeros(max(int(maxcore), 1), dtype=CORE_DTYPE)
+    rc = fn(*args, _ptr(core), int(maxcore), ctypes.byref(n))
+    _lib.check(rc)
+    return core[:n.value]
+
+
+def grid_cores(bp, densph, denGrid, nx, ny, nz, xmin, ymin, zmin, cellsize, ptype,
+               threshold=PEAKTHRESHOLD, nbuff=NCELLBUFF, maxcore=None):
+    """Grid cores of lagFindStellarCore (nnost.c:1115-1271) as an array of
+    Coretype records in (k, j, i) node order."""
+    lib = _lib.load()
+    n = len(bp)
+    _need(bp, PARTICLE_DTYPE, n, "bp")
+    _need(densph, np.float32, n, "densph")
+    _need(denGrid, np.float32, 2 * (nx // 2 + 1) * ny * nz, "denGrid")
+    maxcore = max(1, nx * ny * nz // 8) if maxcore is None else maxcore
+    return _cores(lib.pgalf_grid_cores, maxcore, _ptr(bp), n, _ptr(densph), _ptr(denGrid),
+                  nx, ny, nz, xmin, ymin, zmin, cellsize, int(ptype), float(threshold),
+                  int(nbuff))
+
+
+def finddenpeak(den, numneigh, neighbor, jflag, bp, maxcore=None):
+    """finddenpeak (subhaloden.mod6.c:416-492) as an array of Coretype
+    records in index order."""
+    lib = _lib.load()
+    n = len(bp)
+    _need(bp, PARTICLE_DTYPE, n, "bp")
+    _need(den, np.float32, n, "den")
+    _need(neighbor, np.int64, n * numneigh, "neighbor")
+    maxcore = max(1, n) if maxcore is None else maxcore
+    return _cores(lib.pgalf_finddenpeak, maxcore, _ptr(den), int(numneigh), _ptr(neighbor), n,
+                  int(jflag), _ptr(bp))
 
 
 # ---------------------------------------------------------------------------
@@ -420,6 +469,49 @@ def dev_peak_climb(x, y, z, grid, den, niter=5, step=0.5):
     return ox, oy, oz, peak
 
 
+def _dev_cores(fn, maxcore, *args):
+    torch = _torch()
+    core = torch.empty(max(int(maxcore), 1), dtype=torch.int32, device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
+    _lib.check(fn(*args, _dptr(core), int(maxcore), _dptr(cnt), _stream()))
+    return core, cnt
+
+
+def dev_grid_cores(x, y, z, densph, grid, den, ptype=TYPE_ALL, types=None,
+                   threshold=PEAKTHRESHOLD, nbuff=NCELLBUFF, maxcore=None):
+    """Grid cores on device (pgalf_dev_grid_cores): returns (core particle
+    indices int32 tensor of capacity maxcore, count int32 tensor [1]); the
+    first min(count, maxcore) entries are the cores in (k, j, i) order."""
+    torch = _torch()
+    n = x.numel()
+    for t, w in ((x, "x"), (y, "y"), (z, "z"), (densph, "densph")):
+        _check_dev(t, torch.float32, n, w)
+    if types is not None:
+        _check_dev(types, torch.int32, n, "types")
+    _check_dev(den, torch.float32, _grid_floats(grid), "den")
+    maxcore = max(1, grid.nx * grid.ny * grid.nz // 8) if maxcore is None else maxcore
+    lib = _lib.load()
+    return _dev_cores(lib.pgalf_dev_grid_cores, maxcore, _dptr(x), _dptr(y), _dptr(z),
+                      _dptr(types), n, int(ptype), _dptr(densph), ctypes.byref(grid), _dptr(den),
+                      float(threshold), int(nbuff))
+
+
+def dev_finddenpeak(den, nbr, types=None, jflag=0, threshold=PEAKTHRESHOLD, maxcore=None):
+    """finddenpeak on device (pgalf_dev_finddenpeak): (core indices, count)
+    as dev_grid_cores, cores in index order."""
+    torch = _torch()
+    n = den.numel()
+    k = nbr.shape[1]
+    _check_dev(den, torch.float32, n, "den")
+    _check_dev(nbr, torch.int64, n * k, "nbr")
+    if types is not None:
+        _check_dev(types, torch.int32, n, "types")
+    maxcore = max(1, n) if maxcore is None else maxcore
+    lib = _lib.load()
+    return _dev_cores(lib.pgalf_dev_finddenpeak, maxcore, _dptr(den), int(k), _dptr(nbr), n,
+                      _dptr(types), int(jflag), float(threshold))
+
+
 def dev_status():
     """Synchronise torch's current stream and raise on kernel-detected errors."""
     lib = _lib.load()
@@ -447,4 +539,5 @@ __all__ = ["PgalfError", "PARTICLE_DTYPE", "make_particles", "padded_shape",
            "starfindsphdensity", "near_lists", "lag_density_stage", "grid_struct",
            "dev_tsc", "dev_smooth", "dev_findden", "dev_knn", "dev_density_stage",
            "dev_density_stage_backward", "dev_peak_climb", "dev_status", "Batch", "timing",
-           "timing_get"]
+           "timing_get", "CORE_DTYPE", "grid_cores", "finddenpeak", "dev_grid_cores",
+           "dev_finddenpeak"]

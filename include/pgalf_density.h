@@ -49,6 +49,26 @@ typedef struct pgalf_particle {
                                     reads/writes out of bounds there) */
 #define PGALF_ERR_NOMEM 4        /* device allocation failed */
 #define PGALF_ERR_SHORT_LIST 5   /* fewer than k candidates (reference: exit(99)) */
+#define PGALF_ERR_CORE_OVERFLOW 6 /* more cores than the caller's array holds
+                                     (reference: exit(999) or Realloc) */
+
+/* Same layout as the reference's Coretype (tree.h:204-213, 56 bytes).  The
+ * peak entry points fill peak, cx, cy, cz and density (as the reference does)
+ * and zero the other fields. */
+typedef struct pgalf_core {
+    int peak;
+    int nummem, numstar;
+    float starmass;
+    float coredensity;
+    float cx, cy, cz;
+    float cvx, cvy, cvz;
+    float Rtidal, density;
+    unsigned char flag;
+} pgalf_core_t;
+
+/* Reference constants of the peak stage (params.h:19,115). */
+#define PGALF_PEAKTHRESHOLD 2000.0f
+#define PGALF_NCELLBUFF 10
 
 /* Status of the last call on this host thread, and its message. */
 int pgalf_last_error(void);
@@ -100,6 +120,31 @@ int pgalf_near_lists(pgalf_particle_t *bp, int np, int Numnear, long long *neari
 int pgalf_lag_density_stage(pgalf_particle_t *bp, int np, int Numnear, int ptype,
                             float *densph, long long *nearindex, float **grid_out,
                             int *nxyz_out, double *xyzmin_out);
+
+/* Grid cores of lagFindStellarCore (nnost.c:1115-1271, the block between the
+ * read-back and MergingPeak): particles (stars for ptype PGALF_TYPE_STAR, all
+ * for PGALF_TYPE_ALL) on their nearest node of the smoothed grid denGrid;
+ * every node at least nbuff/2 cells inside the grid whose value exceeds
+ * `threshold` and its 26 neighbours and which holds particles gives a core:
+ * its particle of largest densph (the largest index among equal ones),
+ * skipped when particle 0 is among the node's particles (the reference's
+ * `if(ibp > 0)`).  Cores in (k, j, i) node order into core[0 .. *numcore);
+ * PGALF_ERR_CORE_OVERFLOW when more than maxcore (then *numcore is the
+ * count).  The reference uses threshold = PGALF_PEAKTHRESHOLD and
+ * nbuff = PGALF_NCELLBUFF.  Returns a status code. */
+int pgalf_grid_cores(pgalf_particle_t *bp, int np, const float *densph, const float *denGrid,
+                     int nx, int ny, int nz, double xmin, double ymin, double zmin,
+                     double cellsize, int ptype, float threshold, int nbuff,
+                     pgalf_core_t *core, int maxcore, int *numcore);
+
+/* finddenpeak (subhaloden.mod6.c:416-492) with a caller-sized core array in
+ * place of the reference's Realloc'ed one: particle i is a core when den[i] >
+ * PGALF_PEAKTHRESHOLD, it is a star if jflag == 1, and no particle of its
+ * list neighbor[i*numneigh ..] is strictly denser.  Cores in index order into
+ * core[0 .. *numcore); PGALF_ERR_CORE_OVERFLOW when more than maxcore.
+ * Returns a status code. */
+int pgalf_finddenpeak(float *den, int numneigh, long long *neighbor, int np, int jflag,
+                      pgalf_particle_t *bp, pgalf_core_t *core, int maxcore, int *numcore);
 
 /* ===================== Part 2: device-resident API ======================== */
 
@@ -175,6 +220,16 @@ int pgalf_dev_density_stage_backward(const float *x, const float *y, const float
 int pgalf_dev_peak_climb(const float *x, const float *y, const float *z, long long np,
                          const pgalf_grid_t *g, const float *den, int niter, float step,
                          float *ox, float *oy, float *oz, int *peak, void *stream);
+/* Device forms of pgalf_grid_cores and pgalf_finddenpeak: core particle
+ * indices to core[0 .. min(*ncore, maxcore)), the count *ncore left in device
+ * memory (stream-ordered, no synchronisation). */
+int pgalf_dev_grid_cores(const float *x, const float *y, const float *z, const int *type,
+                         long long np, int ptype, const float *densph, const pgalf_grid_t *g,
+                         const float *den, float threshold, int nbuff, int *core,
+                         long long maxcore, int *ncore, void *stream);
+int pgalf_dev_finddenpeak(const float *den, int k, const long long *nbr, long long np,
+                          const int *type, int jflag, float threshold, int *core,
+                          long long maxcore, int *ncore, void *stream);
 /* Synchronise `stream` and return the status of the kernels run since the
  * previous call (PGALF_ERR_OUT_OF_GRID / PGALF_ERR_SHORT_LIST or PGALF_OK). */
 int pgalf_dev_status(void *stream);

@@ -1,0 +1,159 @@
+"""GPU parity of the integer peak stage (galaxyfinder_amd/csrc/peaks.hip):
+
+  * finddenpeak (subhaloden.mod6.c:416-492): the cores, their positions and
+    densities bit-identical to the reference's own finddenpeak (fixture
+    tests/golden/peaks_small.npz, densities with neighbour ties, jflag 0
+    and 1), through the host entry point and the device one;
+  * the grid cores of lagFindStellarCore (nnost.c:1115-1271): core
+    particles bit-identical to the oracle (pinned by brute force and the
+    particle-0 / tie / plateau cases in tests/test_oracle.py) on planted
+    grids and on the 131072- and 4194304-particle halos' own smoothed grids,
+    read-back densities and lists.
+Integer outputs: the bar is equality.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+from galaxyfinder_amd import density as D  # noqa: E402
+from galaxyfinder_amd import halos  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def say(msg):
+    print("[case] " + msg, flush=True)
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def _take(pair):
+    core, cnt = pair
+    D.dev_status()
+    n = int(cnt.item())
+    return core.cpu().numpy()[:n].astype(np.int64)
+
+
+def test_finddenpeak_vs_reference_fixture():
+    f = np.load(os.path.join(GOLD, "peaks_small.npz"))
+    pos = np.load(os.path.join(GOLD, "knn_small.npz"))["pos"]
+    bp = D.make_particles(pos, None, f["types"])
+    nbr = np.ascontiguousarray(f["nbr"], dtype=np.int64)
+    for tag in "ab":
+        den = np.ascontiguousarray(f["den_" + tag])
+        for jflag in (0, 1):
+            key = "%s_j%d" % (tag, jflag)
+            say("finddenpeak fixture %s: %d reference cores" % (key, len(f["peak_" + key])))
+            core = D.finddenpeak(den, nbr.shape[1], nbr, jflag, bp)
+            assert np.array_equal(core["peak"], f["peak_" + key])
+            assert np.array_equal(np.stack([core["cx"], core["cy"], core["cz"]], axis=1),
+                                  f["cxyz_" + key])
+            assert np.array_equal(core["density"], f["cden_" + key])
+            got = _take(D.dev_finddenpeak(_dev(den), _dev(nbr), _dev(f["types"]), jflag))
+            assert np.array_equal(got, f["peak_" + key])
+
+
+def _planted(seed, n=6000, ng=28):
+    rng = np.random.default_rng(seed)
+    grid = (ng, ng + 3, ng - 2, -0.1, 0.05, 0.0, 0.05)
+    nx, ny, nz, x0, y0, z0, h = grid
+    den = np.zeros(D.padded_shape(nx, ny, nz), dtype=np.float32)
+    den[..., :nx] = np.round(rng.uniform(0, 3000, (nz, ny, nx)) / 100) * 100
+    ijk = np.stack([rng.integers(4, nx - 4, n), rng.integers(4, ny - 4, n),
+                    rng.integers(4, nz - 4, n)], axis=1)
+    pos = np.array([x0, y0, z0]) + (ijk + rng.uniform(-0.49, 0.49, (n, 3))) * h
+    pos = pos.astype(np.float32).astype(np.float64)
+    densph = (np.round(rng.uniform(0, 50, n)) * 2).astype(np.float32)
+    types = rng.integers(1, 3, n).astype(np.int32)
+    # a star, particle 0, on a planted peak node: that core must be skipped
+    i0, j0, k0 = nx // 2, ny // 2, nz // 2
+    den[k0, j0, i0] = 9999.0
+    pos[0] = np.array([x0 + i0 * h, y0 + j0 * h, z0 + k0 * h], dtype=np.float32)
+    types[0] = 1
+    return pos, types, densph, den, grid
+
+
+def test_grid_cores_planted_grids():
+    for seed in (11, 12):
+        pos, types, densph, den, grid = _planted(seed)
+        nx, ny, nz, x0, y0, z0, h = grid
+        g = D.grid_struct(*grid)
+        for ptype in (5, 1):
+            ref = O.grid_cores(pos, types, ptype, densph, den, grid, 2000.0, 10)
+            say("grid cores planted seed=%d ptype=%d: %d cores" % (seed, ptype, len(ref)))
+            core = D.grid_cores(D.make_particles(pos, None, types), densph, den, nx, ny, nz,
+                                x0, y0, z0, h, ptype, 2000.0, 10)
+            assert np.array_equal(core["peak"], ref)
+            assert np.array_equal(core["density"], densph[ref])
+            got = _take(D.dev_grid_cores(*(_dev(pos[:, a].astype(np.float32)) for a in range(3)),
+                                         _dev(densph), g, _dev(den), ptype=ptype,
+                                         types=_dev(types), threshold=2000.0, nbuff=10))
+            assert np.array_equal(got, ref)
+
+
+def test_grid_cores_particle_zero_and_ties():
+    ng = 16
+    grid = (ng, ng, ng, 0.0, 0.0, 0.0, 1.0)
+    den = np.zeros(D.padded_shape(ng, ng, ng), dtype=np.float32)
+    den[8, 8, 8] = 5000.0
+    den[8, 8, 10] = 4000.0
+    den[10, 10, 6] = den[10, 10, 7] = 4500.0
+    den[6, 9, 9] = 2000.0
+    pos = np.array([[8, 8, 8], [10, 8, 8], [10.2, 8, 8], [8.3, 8, 8], [7.7, 8.2, 8],
+                    [6, 10, 10], [7, 10, 10], [9, 9, 6]], dtype=np.float64)
+    densph = np.array([9, 5, 5, 7, 8, 1, 1, 1], dtype=np.float32)
+    for t0, ptype, want in ((1, 5, [2]), (2, 1, [4]), (2, 5, [2])):
+        types = np.array([t0, 2, 2, 1, 1, 1, 1, 1], dtype=np.int32)
+        core = D.grid_cores(D.make_particles(pos, None, types), densph, den, ng, ng, ng,
+                            0.0, 0.0, 0.0, 1.0, ptype)
+        assert core["peak"].tolist() == want
+        got = _take(D.dev_grid_cores(*(_dev(pos[:, a].astype(np.float32)) for a in range(3)),
+                                     _dev(densph), D.grid_struct(*grid), _dev(den), ptype=ptype,
+                                     types=_dev(types)))
+        assert got.tolist() == want
+
+
+@pytest.mark.parametrize("config", [1, 2])
+def test_peak_stage_on_workload_halos(config):
+    """Both peak finders on the halo's own stage outputs (smoothed grid,
+    read-back densities, W4 kNN densities and 32-NN lists), thresholds at a
+    quantile of the values so that the cores are many."""
+    import torch
+    pos, mass, spec = halos.single_halo_config(config)
+    n, ng, k = len(pos), spec["nx"], 32
+    grid = (ng, ng, ng, spec["xmin"], spec["ymin"], spec["zmin"], spec["cellsize"])
+    g = D.grid_struct(*grid)
+    x, y, z = (_dev(pos[:, a]) for a in range(3))
+    m = _dev(mass)
+    den = torch.empty(D.padded_shape(ng, ng, ng), dtype=torch.float32, device="cuda")
+    rt = torch.empty(n, dtype=torch.float32, device="cuda")
+    nbr = torch.empty((n, k), dtype=torch.int64, device="cuda")
+    rk = torch.empty(n, dtype=torch.float32, device="cuda")
+    D.dev_density_stage(x, y, z, m, g, spec["RG"], k, den, rt, nbr, rk)
+    D.dev_status()
+    den_h, rt_h = den.cpu().numpy(), rt.cpu().numpy()
+    thr = float(np.quantile(den_h[..., :ng][den_h[..., :ng] > 0], 0.5))
+    types = np.where(np.arange(n) % 4 == 0, 1, 2).astype(np.int32)
+    for ptype in (5, 1):
+        ref = O.grid_cores(pos.astype(np.float64), types, ptype, rt_h, den_h, grid, thr, 10)
+        got = _take(D.dev_grid_cores(x, y, z, rt, g, den, ptype=ptype, types=_dev(types),
+                                     threshold=thr, nbuff=10))
+        say("C%d grid cores ptype=%d threshold=%.4g: %d cores" % (config, ptype, thr, len(ref)))
+        assert len(ref) > 0 and np.array_equal(got, ref)
+    rk_h, nbr_h = rk.cpu().numpy(), nbr.cpu().numpy()
+    thr = float(np.quantile(rk_h, 0.5))
+    for jflag in (0, 1):
+        ref = O.finddenpeak(rk_h, nbr_h, types, jflag, thr)
+        got = _take(D.dev_finddenpeak(rk, nbr, _dev(types), jflag, thr))
+        say("C%d finddenpeak jflag=%d: %d cores" % (config, jflag, len(ref)))
+        assert len(ref) > 0 and np.array_equal(got, ref)
