@@ -9,8 +9,10 @@ RG = 1.5 cells, k = 32; synthetic seeded data.
 
 Prints ONE JSON line (rank 0, N=1):
   value          particles/s through the stage (inputs already in HBM)
-  roofline       dominant kernel (kNN query) algorithmic bytes per launch
-                 / its mean launch time (HIP events on the launch stream)
+  roofline       dominant kernel (knn_packed_k, the kNN query) algorithmic
+                 bytes per launch / its mean launch time (HIP events on the
+                 launch stream); its VALU issue fraction from the committed
+                 counters
   cpu_baseline   the reference's own C/OpenMP code compiled from its sources
                  (oracle/_ref, "reference") on the same halo, timed on this
                  host's cores (kNN list builder on a cell sample, scaled)
@@ -28,11 +30,32 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public peak
 CALIB_COPY_BYTES = 1 << 29  # plain device copy (negation) that scales the HBM byte counters
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "knn_query_k_traffic.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "knn_packed_k_traffic.txt")
+COUNTER_FILE = os.path.join(ROOT, "profiles", "knn_packed_k_counters.md")
+# wave64 VALU instructions the chip can issue per second: 256 CUs x 4 SIMDs x
+# 2.4 GHz, one wave64 instruction per SIMD every 4 cycles
+VALU_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4
+
+
+def measured_valu(n, k):
+    """VALU wave-instructions per knn_packed_k launch from the committed
+    counter summary, when it was taken on this workload (N and k in its
+    title line)."""
+    try:
+        text = open(COUNTER_FILE).read()
+    except OSError:
+        return None
+    if ("N=%s" % format(n, ",")) not in text.splitlines()[0] or ("k=%d" % k) not in text:
+        return None
+    for line in text.splitlines():
+        cells = [c.strip() for c in line.split("|")]
+        if len(cells) > 2 and cells[1] == "SQ_INSTS_VALU":
+            return float(cells[2])
+    return None
 
 
 def measured_traffic(n, k):
-    """HBM bytes per knn_query_k launch from the committed counter summary
+    """HBM bytes per knn_packed_k launch from the committed counter summary
     (FETCH_SIZE + WRITE_SIZE, each scaled by the same counter's reading for
     a plain copy of CALIB_COPY_BYTES), when it was taken on this workload."""
     try:
@@ -452,7 +475,8 @@ def main():
         step()
     torch.cuda.synchronize()
     phases = {}
-    for ph in ("stage", "tsc", "smooth", "findden", "knn", "knn_prep", "knn_query"):
+    for ph in ("stage", "tsc", "smooth", "findden", "knn", "knn_prep", "knn_query",
+               "knn_packed"):
         tot, cnt = D.timing_get(ph)
         if cnt:
             phases[ph] = tot / cnt
@@ -466,11 +490,12 @@ def main():
         del src, dst
     g_cells = spec["nx"] * spec["ny"] * spec["nz"]
     bts = stage_bytes(n, g_cells, k)
-    q_ms = phases.get("knn_query", float("nan"))
+    q_ms = phases.get("knn_packed", float("nan"))
     achieved = bts["knn"] / (q_ms * 1e-3) / 1e9
+    valu = measured_valu(n, k)
     roofline = {
         "bound": "hbm",
-        "kernel": "knn_query_k",
+        "kernel": "knn_packed_k",
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -478,6 +503,10 @@ def main():
         "traffic": measured_traffic(n, k),
         "bytes_per_launch": bts["knn"],
         "ms_per_launch": round(q_ms, 4),
+        "valu": None if valu is None else {
+            "insts_per_launch": valu,
+            "peak_insts_per_s": VALU_ISSUE_PER_S,
+            "frac": round(valu / VALU_ISSUE_PER_S / (q_ms * 1e-3), 4)},
     }
     stage_alg = sum(bts.values())
     out = {

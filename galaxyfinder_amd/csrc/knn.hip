@@ -1960,10 +1960,12 @@ hipError_t run_query(bool packed, unsigned nwaves, const double *qx, const doubl
                                                        halos, h0, waves, k, nbr, rho, rho_mode,
                                                        m, self_seed, fix, fix + 1, prof);
     } else {
+        timing_mark("knn_packed", s, false);
         knn_packed_k<P, false><<<nwaves, kWave, 0, s>>>(qx, qy, qz, qid, cx, cy, cz, cid,
                                                         boxes, halos, h0, waves, k, nbr, rho,
                                                         rho_mode, m, self_seed, fix, fix + 1,
                                                         nullptr);
+        timing_mark("knn_packed", s, true);
     }
     const unsigned nfix = nwaves < kFixBlocks ? nwaves : kFixBlocks;
     knn_query_k<P, true><<<nfix, kWave, 0, s>>>(qx, qy, qz, qid, cx, cy, cz, cid, boxes, halos,
