@@ -29,6 +29,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public peak
+# launch schedule measured on MI355X by python -m galaxyfinder_amd.tune
+TUNED_SCHEDULE = os.path.join(ROOT, "galaxyfinder_amd", "schedule_mi355x.json")
 CALIB_COPY_BYTES = 1 << 29  # plain device copy (negation) that scales the HBM byte counters
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "knn_packed_k_traffic.txt")
 COUNTER_FILE = os.path.join(ROOT, "profiles", "knn_packed_k_counters.md")
@@ -418,6 +420,9 @@ def main():
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain %d-byte device copy (counter calibration)"
                     % CALIB_COPY_BYTES)
+    ap.add_argument("--schedule", default=None,
+                    help="launch schedule (JSON of python -m galaxyfinder_amd.tune); default: "
+                         "%s when present, else the library defaults" % TUNED_SCHEDULE)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the density stage on a "
@@ -429,6 +434,11 @@ def main():
     from galaxyfinder_amd import density as D
     from galaxyfinder_amd import halos
 
+    sched_path = args.schedule or (TUNED_SCHEDULE if os.path.exists(TUNED_SCHEDULE) else None)
+    if sched_path:
+        from galaxyfinder_amd import tune
+        tune.load(sched_path)
+        print("[bench] launch schedule %s" % sched_path, file=sys.stderr, flush=True)
     if args.c5:
         print(json.dumps({"c5": c5_bench(args.steps, args.warmup)}), flush=True)
         return
@@ -525,7 +535,8 @@ def main():
         "data": "synthetic (seeded NFW+Plummer halo)",
         "config": {"workload": "one halo, N=%d, %d^3 grid, RG=%.1f cells, k=%d"
                                % (n, spec["nx"], spec["rg_cells"], k),
-                   "particles": n, "grid": [spec["nx"], spec["ny"], spec["nz"]], "k": k},
+                   "particles": n, "grid": [spec["nx"], spec["ny"], spec["nz"]], "k": k,
+                   "schedule": os.path.relpath(sched_path, ROOT) if sched_path else "defaults"},
         "phase_ms": {k_: round(v, 4) for k_, v in phases.items()},
         "stage_roofline_frac": round(stage_alg / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": roofline,

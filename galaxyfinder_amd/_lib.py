@@ -26,6 +26,12 @@ class pgalf_grid_t(ctypes.Structure):
                 ("xmin", _c_d), ("ymin", _c_d), ("zmin", _c_d), ("cellsize", _c_d)]
 
 
+class pgalf_schedule_t(ctypes.Structure):
+    """pgalf_schedule_t of include/pgalf_density.h."""
+    _fields_ = [("nmax", _c_ll), ("knn_packed", _c_i), ("knn_xcd_chunk", _c_i),
+                ("tsc_unit_entries", _c_i)]
+
+
 # name -> (restype, argtypes); every symbol of include/pgalf_density.h
 SIGNATURES = {
     "pgalf_last_error": (_c_i, []),
@@ -77,6 +83,9 @@ SIGNATURES = {
                                     _c_p, _c_i, _c_f, _c_p, _c_p, _c_p, _c_p, _c_p]),
     "pgalf_batch_workspace_bytes": (_c_ll, [_c_p, _c_i]),
     "pgalf_batch_destroy": (None, [_c_p]),
+    "pgalf_set_schedule": (_c_i, [ctypes.POINTER(pgalf_schedule_t), _c_i, _c_i]),
+    "pgalf_get_schedule": (_c_i, [_c_ll, ctypes.POINTER(pgalf_schedule_t),
+                                  ctypes.POINTER(_c_i)]),
     "pgalf_timing_enable": (None, [_c_i]),
     "pgalf_timing_reset": (None, []),
     "pgalf_timing_get": (_c_i, [ctypes.c_char_p, ctypes.POINTER(_c_d), ctypes.POINTER(_c_i)]),

@@ -359,6 +359,7 @@ extern "C" pgalf_batch_t *pgalf_batch_create(int nhalo, const long long *off,
     b->bd.max_halo_tiles = max_tiles;
     b->bd.slab_tile = b->d_slab_tile;
     b->bd.nslabs = (long long)slab_tile.size();
+    b->bd.sched = b->kplan.sched;  // the schedule of the batch's total, fixed here
     b->ws = new Workspace(b->device);
     ok();
     return b;

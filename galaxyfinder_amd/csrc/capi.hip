@@ -130,6 +130,23 @@ Workspace &workspace_for_current_device() {
     return *w;
 }
 
+// ---- schedule ------------------------------------------------------------------
+static std::mutex g_sched_mu;
+static std::vector<Schedule> g_sched;  // bins by ascending nmax
+static int g_smooth_direct_max = 56;
+
+Schedule schedule_for(long long n) {
+    std::lock_guard<std::mutex> lk(g_sched_mu);
+    for (const Schedule &b : g_sched)
+        if (n <= b.nmax) return b;
+    return Schedule();
+}
+
+int smooth_direct_max() {
+    std::lock_guard<std::mutex> lk(g_sched_mu);
+    return g_smooth_direct_max;
+}
+
 // ---- timing ------------------------------------------------------------------
 static std::mutex g_t_mu;
 static bool g_t_on = false;
@@ -959,4 +976,41 @@ extern "C" int pgalf_dev_finddenpeak(const float *den, int k, const long long *n
         e = find_den_peak(den, k, nbr, np, type, jflag, (double)threshold, core, maxcore,
                           reinterpret_cast<unsigned *>(ncore), ws, s);
     return e == hipSuccess ? ok() : hip_err(e, where);
+}
+
+// ---- launch schedule ---------------------------------------------------------------------
+extern "C" int pgalf_set_schedule(const pgalf_schedule_t *bins, int nbins,
+                                  int smooth_direct_max) {
+    if (nbins < 0 || (nbins > 0 && !bins) || smooth_direct_max < 0 || smooth_direct_max > 64)
+        return set_err(PGALF_ERR_ARG, "pgalf_set_schedule: bad arguments");
+    std::vector<Schedule> t;
+    for (int i = 0; i < nbins; i++) {
+        const pgalf_schedule_t &b = bins[i];
+        if ((i > 0 && b.nmax <= bins[i - 1].nmax) || b.knn_packed < 0 || b.knn_packed > 1 ||
+            b.knn_xcd_chunk < 0 || b.knn_xcd_chunk > 1024 || b.tsc_unit_entries < 256 ||
+            b.tsc_unit_entries > 65536)
+            return set_err(PGALF_ERR_ARG, "pgalf_set_schedule: bad bin %d", i);
+        Schedule c;
+        c.nmax = b.nmax;
+        c.knn_packed = b.knn_packed;
+        c.knn_xcd_chunk = b.knn_xcd_chunk;
+        c.tsc_unit_entries = b.tsc_unit_entries;
+        t.push_back(c);
+    }
+    std::lock_guard<std::mutex> lk(g_sched_mu);
+    g_sched = t;
+    g_smooth_direct_max = smooth_direct_max;
+    return ok();
+}
+
+extern "C" int pgalf_get_schedule(long long n, pgalf_schedule_t *out, int *smooth_direct_max_out) {
+    const Schedule c = schedule_for(n);
+    if (out) {
+        out->nmax = c.nmax;
+        out->knn_packed = c.knn_packed;
+        out->knn_xcd_chunk = c.knn_xcd_chunk;
+        out->tsc_unit_entries = c.tsc_unit_entries;
+    }
+    if (smooth_direct_max_out) *smooth_direct_max_out = smooth_direct_max();
+    return ok();
 }

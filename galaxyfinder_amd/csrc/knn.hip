@@ -140,21 +140,19 @@ struct Box {
 
 // Workgroups go to the 8 XCDs round-robin (block b to XCD b % 8), and each
 // XCD has its own L2.  Neighbouring query waves read mostly the same
-// candidates, so each XCD is given runs of kXcdChunk consecutive logical
-// waves, the runs dealt to the XCDs in turn (contiguous ranges per XCD would
-// unbalance them where wave costs vary along the order, e.g. a batch sorted
-// by halo size).  Block b runs wave xcd_block(b, n), a bijection of [0, n):
-// blocks past the last whole round of runs keep their own index.
-#ifndef PGALF_KNN_XCD_CHUNK
-#define PGALF_KNN_XCD_CHUNK 64
-#endif
-constexpr unsigned kXcdChunk = PGALF_KNN_XCD_CHUNK;  // 0: no remap
+// candidates, so each XCD is given runs of `chunk` consecutive logical waves
+// (the schedule's knn_xcd_chunk), the runs dealt to the XCDs in turn
+// (contiguous ranges per XCD would unbalance them where wave costs vary
+// along the order, e.g. a batch sorted by halo size).  Block b runs wave
+// xcd_block(b, n, chunk), a bijection of [0, n): blocks past the last whole
+// round of runs keep their own index.
 constexpr unsigned kXcds = 8;
-__host__ __device__ inline unsigned xcd_block(unsigned b, unsigned n) {
-    const unsigned round = kXcds * kXcdChunk;
-    if (kXcdChunk == 0 || b >= n / round * round) return b;
+__host__ __device__ inline unsigned xcd_block(unsigned b, unsigned n, unsigned chunk) {
+    if (chunk == 0) return b;
+    const unsigned round = kXcds * chunk;
+    if (b >= n / round * round) return b;
     const unsigned x = b % kXcds, i = b / kXcds;  // the i-th block of XCD x
-    return ((i / kXcdChunk) * kXcds + x) * kXcdChunk + i % kXcdChunk;
+    return ((i / chunk) * kXcds + x) * chunk + i % chunk;
 }
 
 // Implicit 8-ary box tree over nb leaf buckets: level l holds
@@ -1297,7 +1295,7 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
             const Box *boxes, const KHalo *halos, KHalo halo0, const KWave *waves, int k,
             long long *nbr, float *rho, int rho_mode, const P *mass,
             int self_seed, unsigned long long *dbg, const unsigned *wlist,
-            const unsigned *wcount) {
+            const unsigned *wcount, unsigned xcd_chunk) {
     __shared__ unsigned s_bc[kBufCap][kWave];
     __shared__ int s_list[kGroups][kFCap];
     __shared__ int s_stk[kGroups][kStackCap];  // also the seed list before the walk
@@ -1313,7 +1311,7 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
             __syncthreads();
         }
     } else {
-        query_wave_exact<P>(xcd_block(blockIdx.x, gridDim.x), qx, qy, qz, qid, cxa, cya, cza,
+        query_wave_exact<P>(xcd_block(blockIdx.x, gridDim.x, xcd_chunk), qx, qy, qz, qid, cxa, cya, cza,
                             cida, boxes, halos, halo0, waves, k, nbr, rho, rho_mode, mass,
                             self_seed, dbg, s_bc, s_list, s_stk, s_off, s_n, s_qx, s_qy, s_qz,
                             s_qk, sg);
@@ -1615,7 +1613,8 @@ knn_packed_k(const double *qx, const double *qy, const double *qz, const unsigne
              const double *cxa, const double *cya, const double *cza, const unsigned *cida,
              const Box *boxes, const KHalo *halos, KHalo halo0, const KWave *waves, int k,
              long long *nbr, float *rho, int rho_mode, const P *mass, int self_seed,
-             unsigned *fix_count, unsigned *fix_list, unsigned long long *prof) {
+             unsigned *fix_count, unsigned *fix_list, unsigned long long *prof,
+             unsigned xcd_chunk) {
     __shared__ unsigned s_bc[kBufCap][kWave];
     __shared__ int s_list[kGroups][kFCap];
     __shared__ int s_stk[kGroups][kStackCap];  // also the seed list before the walk
@@ -1626,7 +1625,7 @@ knn_packed_k(const double *qx, const double *qy, const double *qz, const unsigne
     const unsigned long long t_start = pr_clock<kProf>();
     const int lane = threadIdx.x;
     const int grp = lane / kG, gl = lane % kG, gshift = grp * kG;
-    const unsigned wid = xcd_block(blockIdx.x, gridDim.x);
+    const unsigned wid = xcd_block(blockIdx.x, gridDim.x, xcd_chunk);
     KWave wv;
     if (waves) {
         wv = waves[wid];
@@ -1917,15 +1916,11 @@ inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 // (grid-stride over the list; the list is short for distinct positions).
 constexpr unsigned kFixBlocks = 1024;
 
-// The packed-key query unless a halo needs more than kMaxKeyBits slot bits,
-// the debug counters (exact query only) are on, or PGALF_KNN_EXACT=1 asks for
-// the exact query alone.
-bool use_packed(int max_kb, bool debug) {
-    static const bool forced = [] {
-        const char *v = getenv("PGALF_KNN_EXACT");
-        return v && *v && *v != '0';
-    }();
-    return !forced && !debug && max_kb <= kMaxKeyBits;
+// The packed-key query unless the schedule asks for the exact query alone, a
+// halo needs more than kMaxKeyBits slot bits, or the debug counters (exact
+// query only) are on.
+bool use_packed(const Schedule &sc, int max_kb, bool debug) {
+    return sc.knn_packed != 0 && !debug && max_kb <= kMaxKeyBits;
 }
 
 // Runs the query over nwaves logical waves: the packed-key kernel and the
@@ -1936,12 +1931,13 @@ hipError_t run_query(bool packed, unsigned nwaves, const double *qx, const doubl
                      const double *cz, const unsigned *cid, const Box *boxes, const KHalo *halos,
                      const KHalo &h0, const KWave *waves, int k, long long *nbr, float *rho,
                      int rho_mode, const P *m, int self_seed, unsigned long long *dbg,
-                     Workspace &ws, hipStream_t s) {
+                     unsigned xcd_chunk, Workspace &ws, hipStream_t s) {
     if (nwaves == 0) return hipSuccess;
     if (!packed) {
         knn_query_k<P, false><<<nwaves, kWave, 0, s>>>(qx, qy, qz, qid, cx, cy, cz, cid, boxes,
                                                        halos, h0, waves, k, nbr, rho, rho_mode,
-                                                       m, self_seed, dbg, nullptr, nullptr);
+                                                       m, self_seed, dbg, nullptr, nullptr,
+                                                       xcd_chunk);
         return hipGetLastError();
     }
     unsigned *fix = ws.alloc_n<unsigned>(1 + (size_t)nwaves);  // count, then wave ids
@@ -1958,19 +1954,20 @@ hipError_t run_query(bool packed, unsigned nwaves, const double *qx, const doubl
         PGALF_HIP_TRY(fill_u64(prof, kPrN, 0ull, s));
         knn_packed_k<P, true><<<nwaves, kWave, 0, s>>>(qx, qy, qz, qid, cx, cy, cz, cid, boxes,
                                                        halos, h0, waves, k, nbr, rho, rho_mode,
-                                                       m, self_seed, fix, fix + 1, prof);
+                                                       m, self_seed, fix, fix + 1, prof,
+                                                       xcd_chunk);
     } else {
         timing_mark("knn_packed", s, false);
         knn_packed_k<P, false><<<nwaves, kWave, 0, s>>>(qx, qy, qz, qid, cx, cy, cz, cid,
                                                         boxes, halos, h0, waves, k, nbr, rho,
                                                         rho_mode, m, self_seed, fix, fix + 1,
-                                                        nullptr);
+                                                        nullptr, xcd_chunk);
         timing_mark("knn_packed", s, true);
     }
     const unsigned nfix = nwaves < kFixBlocks ? nwaves : kFixBlocks;
     knn_query_k<P, true><<<nfix, kWave, 0, s>>>(qx, qy, qz, qid, cx, cy, cz, cid, boxes, halos,
                                                 h0, waves, k, nbr, rho, rho_mode, m, self_seed,
-                                                nullptr, fix + 1, fix);
+                                                nullptr, fix + 1, fix, xcd_chunk);
     if (prof) {
         unsigned long long d[kPrN];
         unsigned nf = 0;
@@ -2081,9 +2078,10 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
     if (sorted_hook && *sorted_hook) PGALF_HIP_TRY((*sorted_hook)(qx, qy, qz, qv, np));
     timing_mark("knn_query", s, false);
     long long nqb = (np + kWave - 1) / kWave;
-    PGALF_HIP_TRY(run_query<P>(use_packed(h0.kb, debug), (unsigned)nqb, qx, qy, qz, qv, cx, cy,
-                               cz, cid, boxes, nullptr, h0, nullptr, k, nbr, rho, rho_mode, m,
-                               subset ? 0 : 1, dbg, ws, s));
+    const Schedule sc = schedule_for(np);
+    PGALF_HIP_TRY(run_query<P>(use_packed(sc, h0.kb, debug), (unsigned)nqb, qx, qy, qz, qv, cx,
+                               cy, cz, cid, boxes, nullptr, h0, nullptr, k, nbr, rho, rho_mode, m,
+                               subset ? 0 : 1, dbg, (unsigned)sc.knn_xcd_chunk, ws, s));
     timing_mark("knn_query", s, true);
     if (debug) {
         unsigned long long d[kDbgN];
@@ -2152,12 +2150,12 @@ __global__ void __launch_bounds__(kBwdBlock)
 knn_backward_k(const P *x, const P *y, const P *z, const P *m, long long np,
                const long long *nbr, int k, int weighted, const float *gin, double *acc,
                const unsigned *order, const unsigned *rank, const unsigned *halo_of,
-               const HaloSeg *segs) {
+               const HaloSeg *segs, unsigned xcd_chunk) {
     __shared__ double win[4][kBwdWin];
     for (int c = threadIdx.x; c < 4 * kBwdWin; c += kBwdBlock) (&win[0][0])[c] = 0.0;
     __syncthreads();
     // the same XCD-aware order as the query: neighbouring blocks share lists' targets
-    const long long s0 = (long long)xcd_block(blockIdx.x, gridDim.x) * kBwdBlock;
+    const long long s0 = (long long)xcd_block(blockIdx.x, gridDim.x, xcd_chunk) * kBwdBlock;
     const long long wb = s0 - kBwdHalo;  // sorted position of window slot 0
     const long long s = s0 + threadIdx.x;
     // adds v to component c of particle j (sorted position rank[j])
@@ -2289,9 +2287,9 @@ hipError_t knn_density_backward(const P *x, const P *y, const P *z, const P *m, 
     hilbert_k<P><<<nblk(np, 256), 256, 0, s>>>(x, y, z, nullptr, np, mm, hk, hv, nullptr);
     PGALF_HIP_TRY(radix_sort_pairs(&hk, &hv, hk2, hv2, np, 30, ws, s));
     inverse_perm_k<<<nblk(np, 256), 256, 0, s>>>(hv, np, rank);
-    knn_backward_k<P><<<nblk(np, kBwdBlock), kBwdBlock, 0, s>>>(x, y, z, m, np, nbr, k, weighted,
-                                                                gin, acc, hv, rank, nullptr,
-                                                                nullptr);
+    knn_backward_k<P><<<nblk(np, kBwdBlock), kBwdBlock, 0, s>>>(
+        x, y, z, m, np, nbr, k, weighted, gin, acc, hv, rank, nullptr, nullptr,
+        (unsigned)schedule_for(np).knn_xcd_chunk);
     return hipGetLastError();
 }
 
@@ -2314,6 +2312,7 @@ template hipError_t knn_density_backward<double>(const double *, const double *,
 hipError_t knn_batch_plan(const long long *off, int nhalo, int k, KnnBatchPlan &plan) {
     if (nhalo < 1 || k < 1 || k > kTop || off[0] != 0) return hipErrorInvalidValue;
     KnnBatchPlan p;
+    p.sched = schedule_for(off[nhalo]);
     p.nhalo = nhalo;
     p.k = k;
     p.np = off[nhalo];
@@ -2478,10 +2477,11 @@ hipError_t knn_batch(const P *x, const P *y, const P *z, const P *m, const Batch
     if (sorted_hook && *sorted_hook) PGALF_HIP_TRY((*sorted_hook)(sx, sy, sz, sid, npad));
     timing_mark("knn_query", s, false);
     KHalo h0{};
-    PGALF_HIP_TRY(run_query<P>(use_packed(plan.max_kb, false), (unsigned)plan.nwaves, sx, sy, sz,
-                               sid, sx, sy, sz, sid, boxes, halos, h0,
+    PGALF_HIP_TRY(run_query<P>(use_packed(plan.sched, plan.max_kb, false), (unsigned)plan.nwaves,
+                               sx, sy, sz, sid, sx, sy, sz, sid, boxes, halos, h0,
                                static_cast<const KWave *>(plan.waves), plan.k, nbr, rho,
-                               rho_mode, m, 1, nullptr, ws, s));
+                               rho_mode, m, 1, nullptr, (unsigned)plan.sched.knn_xcd_chunk, ws,
+                               s));
     timing_mark("knn_query", s, true);
     return hipGetLastError();
 }
@@ -2499,7 +2499,8 @@ hipError_t knn_batch_backward(const P *x, const P *y, const P *z, const P *m, co
     if (!rank) return hipErrorOutOfMemory;
     inverse_perm_k<<<nblk(np, 256), 256, 0, s>>>(order, np, rank);
     knn_backward_k<P><<<nblk(np, kBwdBlock), kBwdBlock, 0, s>>>(
-        x, y, z, m, np, nbr, plan.k, weighted, gin, acc, order, rank, halo_of, b.segs);
+        x, y, z, m, np, nbr, plan.k, weighted, gin, acc, order, rank, halo_of, b.segs,
+        (unsigned)plan.sched.knn_xcd_chunk);
     return hipGetLastError();
 }
 

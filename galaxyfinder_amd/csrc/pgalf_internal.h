@@ -49,8 +49,24 @@ constexpr int kChunkParticles = 8192;
 // most halos of a catalogue are far smaller than kChunkParticles.
 constexpr int kFineParticles = 256;
 
+// Launch schedule of a workload (knobs that an autotuner sets; none of them
+// changes a result bit except smooth_direct_max, whose methods agree within
+// the smoothing tolerance).  schedule_for(n): the first bin of the active
+// table with n <= nmax, the defaults beyond the last bin or with no table.
+struct Schedule {
+    long long nmax = -1;
+    int knn_packed = 1;           // packed-key query (+ exact re-run of flagged waves)
+    int knn_xcd_chunk = 64;       // consecutive query waves per XCD run (0: no remap)
+    int tsc_unit_entries = 4096;  // bin entries per deposit work unit
+};
+Schedule schedule_for(long long n);
+// Strided smoothing lines of length n (a multiple of 8) up to this use the
+// direct convolution; rows up to min(this, 32).  Global, not binned.
+int smooth_direct_max();
+
 // Device view of a batch: descriptors built on the host once per catalogue.
 struct BatchDev {
+    Schedule sched;                 // fixed when the batch is created
     int nhalo = 0;
     long long np = 0;               // particles in all halos
     const HaloSeg *segs = nullptr;  // [nhalo]
@@ -265,6 +281,7 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np,
 // candidate order, then the internal levels level by level) and the wave
 // table, largest halos first.
 struct KnnBatchPlan {
+    Schedule sched;  // fixed when the batch is created
     int nhalo = 0, k = 0, hbits = 0;
     int max_kb = 0;  // slot bits of the packed keys of the largest halo
     long long np = 0, npad = 0, nb_total = 0, nboxes = 0, nwaves = 0;

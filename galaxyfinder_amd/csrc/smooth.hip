@@ -723,14 +723,13 @@ hipError_t get_plan(int n, double expfac, AxisPlan &out) {
     return hipSuccess;
 }
 
-// Kernel choice per line length (compile-time switches for A/B timing):
-// the register FFT for n = 64, 128, 256 unless the direct convolution takes
-// the pass (strided lines up to PGALF_DIRECT_STRIDED_MAX, rows up to 32).
+// Kernel choice per line length: the direct convolution for lengths that
+// are multiples of 8 up to the schedule's smooth_direct_max (rows: up to 32
+// as well; longer rows need ~255 VGPRs there), else the register FFT for
+// n = 64, 128, 256, else the Stockham kernel.  The default 56 takes 8..56
+// directly and 64 by the register FFT, which measured faster in the batch.
 #ifndef PGALF_SMOOTH_FFT2
 #define PGALF_SMOOTH_FFT2 1
-#endif
-#ifndef PGALF_DIRECT_STRIDED_MAX
-#define PGALF_DIRECT_STRIDED_MAX 32  // 40..56 direct; 64 takes the register FFT
 #endif
 #ifndef PGALF_FFT2_NL256
 #define PGALF_FFT2_NL256 16  // complex lines per block at L = 256 (16: 128-byte rows)
@@ -788,10 +787,8 @@ hipError_t run_pass(float *g, const AxisPlan &pl, long long sa, int nb, long lon
     const bool fft2 = PGALF_SMOOTH_FFT2 && pl.L == pl.n &&
                       (pl.n == 64 || pl.n == 128 || pl.n == 256) && gcells > 0 &&
                       gcells < (1LL << 29);
-    // direct convolution: strided passes up to PGALF_DIRECT_STRIDED_MAX,
-    // contiguous rows up to 32 (longer rows need ~255 VGPRs there)
-    const bool direct = pl.direct && (sa == 1 ? pl.n <= 32
-                                              : pl.n <= PGALF_DIRECT_STRIDED_MAX || !fft2);
+    const int dmax = smooth_direct_max();
+    const bool direct = pl.direct && pl.n <= dmax && (sa != 1 || pl.n <= 32);
     if (fft2 && !direct) {
         AxisPass ap{};
         ap.n = pl.n;
@@ -810,7 +807,7 @@ hipError_t run_pass(float *g, const AxisPlan &pl, long long sa, int nb, long lon
         ap.gcells = gcells;
         return run_fft2(g, pl, ap, nc, s, goffs, ngrids);
     }
-    if (pl.direct && !(sa == 1 && pl.n > 32)) {
+    if (direct) {
         DirectPass dp;
         dp.na = nb;
         dp.nlines = (long long)nb * nc;

@@ -4,8 +4,9 @@
 //   1. count:   every selected particle lists the 16^3 node tiles its 27-node
 //               stencil touches (1..8 tiles) -> per-tile counts (block-local
 //               LDS histograms, one global add per block and tile);
-//   2. scan:    tile offsets; tiles are cut into work units of <= kChunk
-//               entries so a cusp tile is shared by many workgroups;
+//   2. scan:    tile offsets; tiles are cut into work units of <= `unit`
+//               entries (the schedule's tsc_unit_entries) so a cusp tile is
+//               shared by many workgroups;
 //   3. scatter: particle ids into per-tile bins (ranges reserved per block);
 //   4. deposit: one workgroup per unit accumulates its particles'
 //               contributions that land inside the tile in an LDS int64
@@ -37,7 +38,6 @@ namespace {
 
 constexpr int kT = 16;               // tile edge (nodes)
 constexpr int kTileNodes = kT * kT * kT;
-constexpr int kChunk = 4096;         // entries per deposit unit
 constexpr int kLdsHistMax = 8192;    // tiles handled with LDS histograms
 constexpr int kBlock = 256;
 constexpr int kPerBlock = 8192;      // particles per count/scatter block
@@ -235,12 +235,12 @@ tsc_scatter_k(const P *x, const P *y, const P *z, const P *m, const int *type, i
 
 // Units per tile (none for a tile without bin entries: the de-tiling pass
 // writes its zeros) and whether the tile needs the multi-unit side buffer.
-__global__ void tsc_units_k(const unsigned *tcount, long long ntiles, unsigned *nunits,
-                            unsigned *multi) {
+__global__ void tsc_units_k(const unsigned *tcount, long long ntiles, unsigned unit,
+                            unsigned *nunits, unsigned *multi) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntiles) return;
     unsigned c = tcount[t];
-    unsigned u = (c + kChunk - 1) / kChunk;
+    unsigned u = (c + unit - 1) / unit;
     nunits[t] = u;
     multi[t] = u > 1 ? 1u : 0u;
 }
@@ -279,7 +279,7 @@ tsc_deposit_k(const P *x, const P *y, const P *z, const P *m, const HaloSeg *seg
               const unsigned *tcount, const unsigned *ustart, const unsigned *nunits,
               const unsigned *unit_tile, const unsigned *total_units,
               const unsigned *multi_slot, const unsigned *bins, const double *scale,
-              const unsigned *tslot, float *tbuf, unsigned long long *side) {
+              const unsigned *tslot, float *tbuf, unsigned long long *side, unsigned unit) {
     __shared__ unsigned long long acc[kTileNodes];
     const unsigned nunit_total = *total_units;
     for (unsigned u = blockIdx.x; u < nunit_total; u += gridDim.x) {
@@ -298,8 +298,8 @@ tsc_deposit_k(const P *x, const P *y, const P *z, const P *m, const HaloSeg *seg
         long long ox = (long long)tx * kT, oy = (long long)ty * kT, oz = (long long)tz * kT;
         for (int i = threadIdx.x; i < kTileNodes; i += kBlock) acc[i] = 0ull;
         __syncthreads();
-        unsigned e0 = tstart[t] + chunk * kChunk;
-        unsigned e1 = min(e0 + kChunk, tstart[t] + tcount[t]);
+        unsigned e0 = tstart[t] + chunk * unit;
+        unsigned e1 = min(e0 + unit, tstart[t] + tcount[t]);
         for (unsigned e = e0 + threadIdx.x; e < e1; e += kBlock) {
             long long i = bins[e];
             Axis ax[3];
@@ -760,8 +760,9 @@ hipError_t deposit_impl(const P *x, const P *y, const P *z, const P *m, const in
                         int ptype, int nhalo, const HaloSeg *segs, const HaloSeg &seg0,
                         const PChunk *chunks, long long nchunks, const unsigned *tile_halo,
                         long long ntiles, long long max_halo_tiles, const unsigned *slab_tile,
-                        long long nslabs, long long np, float *den, Workspace &ws,
-                        hipStream_t s) {
+                        long long nslabs, long long np, float *den, const Schedule &sc,
+                        Workspace &ws, hipStream_t s) {
+    const unsigned unit = (unsigned)sc.tsc_unit_entries;
     WsScope scope(ws);  // bins, tile-major buffer and side buffer end with the deposit
     int use_lds = max_halo_tiles <= kLdsHistMax ? 1 : 0;
     unsigned *tcount = ws.alloc_n<unsigned>(ntiles);
@@ -797,15 +798,15 @@ hipError_t deposit_impl(const P *x, const P *y, const P *z, const P *m, const in
         tsc_scatter_k<P><<<dim3((unsigned)nchunks), dim3(kBlock), 2 * lds_count, s>>>(
             x, y, z, m, type, ptype, segs, seg0, chunks, tstart, tcursor, bins, use_lds);
     unsigned tb = (unsigned)((ntiles + 255) / 256);
-    tsc_units_k<<<tb, 256, 0, s>>>(tcount, ntiles, nunits, multi);
+    tsc_units_k<<<tb, 256, 0, s>>>(tcount, ntiles, unit, nunits, multi);
     PGALF_HIP_TRY(exclusive_scan_u32(nunits, ustart, ntiles, &misc[2], ws, s));
     PGALF_HIP_TRY(exclusive_scan_u32(multi, multi, ntiles, &misc[3], ws, s));
-    // unit -> tile map: the unit count is bounded by ntiles + 8*np/kChunk
-    size_t max_units = (size_t)ntiles + (size_t)(8 * np) / kChunk + 1;
+    // unit -> tile map: the unit count is bounded by ntiles + 8*np/unit
+    size_t max_units = (size_t)ntiles + (size_t)(8 * np) / unit + 1;
     unsigned *unit_tile = ws.alloc_n<unsigned>(max_units);
-    // multi-unit side buffer: a tile with > kChunk entries needs one slot;
-    // there are at most 8*np/kChunk of them.
-    size_t max_multi = (size_t)(8 * np) / kChunk + 1;
+    // multi-unit side buffer: a tile with > unit entries needs one slot;
+    // there are at most 8*np/unit of them.
+    size_t max_multi = (size_t)(8 * np) / unit + 1;
     if ((long long)max_multi > ntiles) max_multi = (size_t)ntiles;
     unsigned long long *side = ws.alloc_n<unsigned long long>(max_multi * kTileNodes);
     if (!unit_tile || !side) return hipErrorOutOfMemory;
@@ -823,7 +824,7 @@ hipError_t deposit_impl(const P *x, const P *y, const P *z, const P *m, const in
     PGALF_HIP_TRY(exclusive_scan_u32(tslot, tslot, ntiles, &misc[4], ws, s));
     tsc_deposit_k<P><<<(unsigned)ncu * 4, kBlock, 0, s>>>(
         x, y, z, m, segs, seg0, tile_halo, tstart, tcount, ustart, nunits, unit_tile, &misc[2],
-        multi, bins, scale, tslot, tbuf, side);
+        multi, bins, scale, tslot, tbuf, side, unit);
     tsc_finalize_k<<<(unsigned)ntiles, 256, 0, s>>>(tile_halo, nunits, multi, tslot, side, scale,
                                                     tbuf);
     long long db = nslabs < (long long)ncu * 8 ? nslabs : (long long)ncu * 8;
@@ -857,7 +858,8 @@ hipError_t tsc_deposit(const P *x, const P *y, const P *z, const P *m, const int
     const TileGeom tg = tile_geom(g);
     if (tg.ntx > kMaxTilesX) return hipErrorInvalidValue;
     return deposit_impl<P>(x, y, z, m, type, ptype, 1, nullptr, seg0, nullptr, nchunks, nullptr,
-                           ntiles, ntiles, nullptr, (long long)tg.nty * tg.ntz, np, den, ws, s);
+                           ntiles, ntiles, nullptr, (long long)tg.nty * tg.ntz, np, den,
+                           schedule_for(np), ws, s);
 }
 
 template <class P>
@@ -868,7 +870,7 @@ hipError_t tsc_deposit_batch(const P *x, const P *y, const P *z, const P *m, con
     const HaloSeg seg0{};
     return deposit_impl<P>(x, y, z, m, nullptr, PGALF_TYPE_ALL, b.nhalo, b.segs, seg0, b.chunks,
                            b.nchunks, b.tile_halo, b.ntiles, b.max_halo_tiles, b.slab_tile,
-                           b.nslabs, b.np, den, ws, s);
+                           b.nslabs, b.np, den, b.sched, ws, s);
 }
 
 template <class P>

@@ -518,6 +518,36 @@ def dev_status():
     _lib.check(lib.pgalf_dev_status(_stream()))
 
 
+SCHEDULE_KEYS = ("knn_packed", "knn_xcd_chunk", "tsc_unit_entries")
+SMOOTH_DIRECT_MAX = 56  # default of the library
+
+
+def set_schedule(bins=(), smooth_direct_max=SMOOTH_DIRECT_MAX):
+    """Install a launch schedule (pgalf_set_schedule): ``bins`` is a list of
+    dicts {nmax, knn_packed, knn_xcd_chunk, tsc_unit_entries} by ascending
+    nmax; a workload of n particles takes the first bin with n <= nmax, the
+    library defaults beyond the last.  An empty list restores the defaults.
+    A batch keeps the schedule it was created under."""
+    lib = _lib.load()
+    arr = (_lib.pgalf_schedule_t * max(1, len(bins)))()
+    for i, b in enumerate(bins):
+        arr[i].nmax = int(b["nmax"])
+        for key in SCHEDULE_KEYS:
+            setattr(arr[i], key, int(b[key]))
+    _lib.check(lib.pgalf_set_schedule(arr, len(bins), int(smooth_direct_max)))
+
+
+def get_schedule(n):
+    """(schedule dict, smooth_direct_max) that a workload of n particles gets."""
+    lib = _lib.load()
+    out = _lib.pgalf_schedule_t()
+    dm = ctypes.c_int()
+    _lib.check(lib.pgalf_get_schedule(int(n), ctypes.byref(out), ctypes.byref(dm)))
+    d = {"nmax": out.nmax}
+    d.update({key: getattr(out, key) for key in SCHEDULE_KEYS})
+    return d, dm.value
+
+
 def timing(enable=None, reset=False):
     lib = _lib.load()
     if enable is not None:
@@ -540,4 +570,4 @@ __all__ = ["PgalfError", "PARTICLE_DTYPE", "make_particles", "padded_shape",
            "dev_tsc", "dev_smooth", "dev_findden", "dev_knn", "dev_density_stage",
            "dev_density_stage_backward", "dev_peak_climb", "dev_status", "Batch", "timing",
            "timing_get", "CORE_DTYPE", "grid_cores", "finddenpeak", "dev_grid_cores",
-           "dev_finddenpeak"]
+           "dev_finddenpeak", "set_schedule", "get_schedule"]

@@ -1,0 +1,232 @@
+"""Autotuner of the launch schedule (pgalf_set_schedule).
+
+The knobs are the ones the library reads at launch time:
+  * knn_packed        1: packed-key kNN query plus the exact re-run of the
+                      waves it flags; 0: the exact query alone;
+  * knn_xcd_chunk     consecutive kNN query waves kept on one XCD (0: the
+                      hardware's round-robin order);
+  * tsc_unit_entries  bin entries per deposit work unit;
+and, global rather than per bin, smooth_direct_max: the longest strided line
+smoothed by direct convolution instead of an FFT.
+
+For each bin of halo sizes the tuner builds one seeded halo of the bin's
+representative size (NFW plus two Plummer subhalos, a power-of-two grid of
+about 4 n^(1/3) cells, RG = 2 cells, k = 32), times the full density stage
+with HIP events on the launch stream under each candidate (coordinate
+descent: one knob at a time, the others held at the best so far), and keeps a
+candidate only if its outputs are bit-identical to the library defaults'
+(none of the three bin knobs may change a result bit).  smooth_direct_max is
+then timed on a batch of small halos on grids of 32, 40, 48, 56 and 64
+cells per side and accepted only if the smoothed grids agree with the
+default's within 1e-5 normwise per halo.
+
+The result is a JSON document {"device", "bins": [...], "smooth_direct_max",
+"timings"}; ``apply(doc)`` (or ``load(path)``) installs it.  Usage:
+    python -m galaxyfinder_amd.tune --out schedule.json
+"""
+import argparse
+import json
+import os
+import sys
+
+import numpy as np
+
+from . import density as D
+from . import halos
+
+K = 32
+# (upper bound of the bin in particles, representative halo size)
+DEFAULT_BINS = ((16384, 8192), (262144, 131072), (2097152, 1048576), (1 << 62, 4194304))
+CANDIDATES = {
+    "knn_packed": (1, 0),
+    "knn_xcd_chunk": (64, 0, 16, 256),
+    "tsc_unit_entries": (4096, 1024, 16384),
+}
+DIRECT_MAX = (56, 32, 48, 64)
+GRID_SIDES = (32, 40, 48, 56, 64)
+DEFAULT = {"knn_packed": 1, "knn_xcd_chunk": 64, "tsc_unit_entries": 4096}
+
+
+def validate(doc):
+    """Check a schedule document; returns (bins, smooth_direct_max)."""
+    bins = doc.get("bins", [])
+    last = None
+    for b in bins:
+        for key in ("nmax",) + D.SCHEDULE_KEYS:
+            if key not in b:
+                raise ValueError("schedule bin lacks %r: %r" % (key, b))
+        if last is not None and int(b["nmax"]) <= last:
+            raise ValueError("schedule bins must ascend in nmax")
+        last = int(b["nmax"])
+    dm = int(doc.get("smooth_direct_max", D.SMOOTH_DIRECT_MAX))
+    return bins, dm
+
+
+def apply(doc):
+    bins, dm = validate(doc)
+    D.set_schedule(bins, dm)
+
+
+def load(path):
+    with open(path) as f:
+        doc = json.load(f)
+    apply(doc)
+    return doc
+
+
+def _time_ms(fn, reps):
+    import torch
+    fn()
+    st = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(reps)]
+    for a, b in ev:
+        a.record(st)
+        fn()
+        b.record(st)
+    torch.cuda.synchronize()
+    return float(np.median([a.elapsed_time(b) for a, b in ev]))
+
+
+class _Stage:
+    """One seeded halo and its output buffers on the device."""
+
+    def __init__(self, n, seed):
+        import torch
+        pos, mass = halos.small_halo(n, seed=seed)
+        ng = halos.grid_for(n)
+        self.n, self.ng = n, ng
+        self.grid = D.grid_struct(ng, ng, ng, 0.0, 0.0, 0.0, 1.0 / ng)
+        self.rg = 2.0 / ng
+        dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")  # noqa: E731
+        self.x, self.y, self.z = (dev(pos[:, a]) for a in range(3))
+        self.m = dev(mass)
+        f32 = dict(dtype=torch.float32, device="cuda")
+        self.den = torch.empty(D.padded_shape(ng, ng, ng), **f32)
+        self.rt = torch.empty(n, **f32)
+        self.rk = torch.empty(n, **f32)
+        self.nbr = torch.empty((n, K), dtype=torch.int64, device="cuda")
+
+    def run(self):
+        D.dev_density_stage(self.x, self.y, self.z, self.m, self.grid, self.rg, K, self.den,
+                            self.rt, self.nbr, self.rk)
+
+    def outputs(self):
+        D.dev_status()
+        return [t.cpu().numpy().view(np.uint8).copy()
+                for t in (self.den, self.rt, self.nbr, self.rk)]
+
+
+def tune_bin(nmax, n, reps=5, seed=0, log=print):
+    """Best bin knobs for halos of about n particles (coordinate descent)."""
+    st = _Stage(n, seed)
+    D.set_schedule([], D.SMOOTH_DIRECT_MAX)
+    st.run()
+    ref = st.outputs()
+    best = dict(DEFAULT)
+    times = {}
+
+    def measure(cand):
+        key = tuple(sorted(cand.items()))
+        if key in times:
+            return times[key]
+        D.set_schedule([dict(nmax=nmax, **cand)], D.SMOOTH_DIRECT_MAX)
+        t = _time_ms(st.run, reps)
+        got = st.outputs()
+        same = all(np.array_equal(a, b) for a, b in zip(got, ref))
+        if not same:
+            raise RuntimeError("schedule %r changed the outputs at n=%d" % (cand, n))
+        times[key] = t
+        log("  n=%d %s: %.3f ms" % (n, cand, t))
+        return t
+
+    tbest = measure(best)
+    for knob, values in CANDIDATES.items():
+        for v in values:
+            cand = dict(best, **{knob: v})
+            t = measure(cand)
+            if t < tbest * 0.98:  # a change must beat noise by 2%
+                best, tbest = cand, t
+    D.set_schedule([], D.SMOOTH_DIRECT_MAX)
+    return dict(nmax=nmax, **best), tbest, times[tuple(sorted(DEFAULT.items()))]
+
+
+def tune_smoothing(nhalo=2000, reps=5, log=print):
+    """smooth_direct_max timed on a batch of small halos (grids of 32..64
+    cells per side, every multiple of 8)."""
+    import torch
+    pos, mass, off, _, _ = halos.catalogue_vec(5, nhalo=nhalo, nmin=64, nmax=8192)
+    # grids of every length the threshold can move between the two methods
+    sides = [GRID_SIDES[h % len(GRID_SIDES)] for h in range(nhalo)]
+    grids = [(g, g, g, 0.0, 0.0, 0.0, 1.0 / g) for g in sides]
+    rg = np.array([2.0 / g for g in sides], dtype=np.float32)
+    n = len(pos)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")  # noqa: E731
+    x, y, z = (dev(pos[:, a]) for a in range(3))
+    m = dev(mass)
+    res = {}
+    ref = None
+    for dm in DIRECT_MAX:
+        D.set_schedule([], dm)
+        b = D.Batch(off, grids, rg, K)
+        den = torch.empty(b.grid_floats, dtype=torch.float32, device="cuda")
+        rt = torch.empty(n, dtype=torch.float32, device="cuda")
+        t = _time_ms(lambda: b.run(x, y, z, m, den, rho_tsc=rt), reps)
+        b.status()
+        g = den.cpu().numpy().astype(np.float64)
+        if ref is None:
+            ref = g
+        else:
+            for h in range(nhalo):
+                a, e = b.grid_offsets[h], b.grid_offsets[h + 1]
+                r = ref[a:e]
+                err = np.linalg.norm(g[a:e] - r) / max(np.linalg.norm(r), 1e-300)
+                if err > 1e-5:
+                    raise RuntimeError("smooth_direct_max=%d: halo %d off by %.2e" % (dm, h, err))
+        b.close()
+        res[dm] = t
+        log("  smooth_direct_max=%d: batch of %d halos %.3f ms" % (dm, nhalo, t))
+    D.set_schedule([], D.SMOOTH_DIRECT_MAX)
+    best = D.SMOOTH_DIRECT_MAX
+    for dm, t in res.items():
+        if t < res[best] * 0.98:
+            best = dm
+    return best, res
+
+
+def tune(bins=DEFAULT_BINS, reps=5, smooth_halos=2000, log=print):
+    import torch
+    doc = {"device": torch.cuda.get_device_name(0), "bins": [], "timings": {}}
+    for nmax, n in bins:
+        b, t, t0 = tune_bin(nmax, n, reps=reps, log=log)
+        doc["bins"].append(b)
+        doc["timings"]["n=%d" % n] = {"best_ms": t, "default_ms": t0}
+    if smooth_halos > 0:
+        dm, res = tune_smoothing(smooth_halos, reps=reps, log=log)
+        doc["smooth_direct_max"] = dm
+        doc["timings"]["smooth_direct_max"] = {str(k): v for k, v in res.items()}
+    else:
+        doc["smooth_direct_max"] = D.SMOOTH_DIRECT_MAX
+    validate(doc)
+    return doc
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--out", default="schedule.json")
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--smooth-halos", type=int, default=2000)
+    ap.add_argument("--max-n", type=int, default=1 << 62,
+                    help="skip bins whose representative halo is larger")
+    a = ap.parse_args(argv)
+    bins = [b for b in DEFAULT_BINS if b[1] <= a.max_n]
+    doc = tune(bins, reps=a.reps, smooth_halos=a.smooth_halos,
+               log=lambda s: print(s, flush=True))
+    with open(a.out, "w") as f:
+        json.dump(doc, f, indent=1)
+    print("wrote %s" % os.path.abspath(a.out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

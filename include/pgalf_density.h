@@ -288,6 +288,27 @@ int pgalf_batch_status(pgalf_batch_t *b, void *stream);
 long long pgalf_batch_workspace_bytes(const pgalf_batch_t *b, int reserved);
 void pgalf_batch_destroy(pgalf_batch_t *b);
 
+/* Launch schedule (no counterpart in the reference): the knobs an autotuner
+ * sets, per bin of workloads up to nmax particles (bins by ascending nmax;
+ * a single-halo call picks the bin of its particle count, a batch the bin of
+ * its total when it is created; beyond the last bin, the defaults).  Only
+ * smooth_direct_max changes result bits, and its methods agree within the
+ * smoothing tolerance; the kNN lists and the deposit are identical under
+ * every schedule. */
+typedef struct pgalf_schedule {
+    long long nmax;        /* bin: workloads of at most nmax particles */
+    int knn_packed;        /* 1 (default): packed-key kNN query; 0: exact query only */
+    int knn_xcd_chunk;     /* consecutive kNN query waves per XCD run, 0..1024 (64) */
+    int tsc_unit_entries;  /* bin entries per deposit work unit, 256..65536 (4096) */
+} pgalf_schedule_t;
+/* Replace the table (nbins 0: the defaults) and the global smoothing
+ * threshold: lines of a length that is a multiple of 8 up to
+ * smooth_direct_max (0..64, default 56; rows also <= 32) are smoothed by
+ * direct convolution, longer ones by FFT. */
+int pgalf_set_schedule(const pgalf_schedule_t *bins, int nbins, int smooth_direct_max);
+/* The schedule a workload of n particles gets, and the smoothing threshold. */
+int pgalf_get_schedule(long long n, pgalf_schedule_t *out, int *smooth_direct_max);
+
 /* Phase timing with HIP events on the launch stream (off by default). */
 void pgalf_timing_enable(int on);
 void pgalf_timing_reset(void);
