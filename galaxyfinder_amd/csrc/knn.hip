@@ -1607,25 +1607,19 @@ __device__ __forceinline__ void scan_list_p(PState &st, int *list, int count, bo
     __syncthreads();
 }
 
+// One wavefront of the packed-key query: logical wave `wid`.
 template <class P, bool kProf>
-__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2)))
-knn_packed_k(const double *qx, const double *qy, const double *qz, const unsigned *qid,
-             const double *cxa, const double *cya, const double *cza, const unsigned *cida,
-             const Box *boxes, const KHalo *halos, KHalo halo0, const KWave *waves, int k,
-             long long *nbr, float *rho, int rho_mode, const P *mass, int self_seed,
-             unsigned *fix_count, unsigned *fix_list, unsigned long long *prof,
-             unsigned xcd_chunk) {
-    __shared__ unsigned s_bc[kBufCap][kWave];
-    __shared__ int s_list[kGroups][kFCap];
-    __shared__ int s_stk[kGroups][kStackCap];  // also the seed list before the walk
-    __shared__ long long s_off[kMaxLevels], s_n[kMaxLevels];
-    __shared__ double s_qx[kWave], s_qy[kWave], s_qz[kWave], s_qk[kWave];
-    __shared__ Stage sg;
-
+__device__ __forceinline__ void packed_wave(
+    unsigned wid, const double *qx, const double *qy, const double *qz, const unsigned *qid,
+    const double *cxa, const double *cya, const double *cza, const unsigned *cida,
+    const Box *boxes, const KHalo *halos, const KHalo &halo0, const KWave *waves, int k,
+    long long *nbr, float *rho, int rho_mode, const P *mass, int self_seed,
+    unsigned *fix_count, unsigned *fix_list, unsigned long long *prof,
+    unsigned (*s_bc)[kWave], int (*s_list)[kFCap], int (*s_stk)[kStackCap], long long *s_off,
+    long long *s_n, double *s_qx, double *s_qy, double *s_qz, double *s_qk, Stage &sg) {
     const unsigned long long t_start = pr_clock<kProf>();
     const int lane = threadIdx.x;
     const int grp = lane / kG, gl = lane % kG, gshift = grp * kG;
-    const unsigned wid = xcd_block(blockIdx.x, gridDim.x, xcd_chunk);
     KWave wv;
     if (waves) {
         wv = waves[wid];
@@ -1909,6 +1903,34 @@ knn_packed_k(const double *qx, const double *qy, const double *qz, const unsigne
         rho[q] = sum;
     }
 }
+
+#define PGALF_PACKED_ARGS                                                                    \
+    qx, qy, qz, qid, cxa, cya, cza, cida, boxes, halos, halo0, waves, k, nbr, rho, rho_mode,  \
+        mass, self_seed, fix_count, fix_list, prof, s_bc, s_list, s_stk, s_off, s_n, s_qx,   \
+        s_qy, s_qz, s_qk, sg
+#define PGALF_PACKED_SHARED                                                                  \
+    __shared__ unsigned s_bc[kBufCap][kWave];                                                \
+    __shared__ int s_list[kGroups][kFCap];                                                   \
+    __shared__ int s_stk[kGroups][kStackCap]; /* also the seed list before the walk */       \
+    __shared__ long long s_off[kMaxLevels], s_n[kMaxLevels];                                 \
+    __shared__ double s_qx[kWave], s_qy[kWave], s_qz[kWave], s_qk[kWave];                    \
+    __shared__ Stage sg;
+
+// The packed-key query kernel: block b runs logical wave xcd_block(b, gridDim.x).
+template <class P, bool kProf>
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2)))
+knn_packed_k(const double *qx, const double *qy, const double *qz, const unsigned *qid,
+             const double *cxa, const double *cya, const double *cza, const unsigned *cida,
+             const Box *boxes, const KHalo *halos, KHalo halo0, const KWave *waves, int k,
+             long long *nbr, float *rho, int rho_mode, const P *mass, int self_seed,
+             unsigned *fix_count, unsigned *fix_list, unsigned long long *prof,
+             unsigned xcd_chunk) {
+    PGALF_PACKED_SHARED
+    packed_wave<P, kProf>(xcd_block(blockIdx.x, gridDim.x, xcd_chunk), PGALF_PACKED_ARGS);
+}
+
+#undef PGALF_PACKED_ARGS
+#undef PGALF_PACKED_SHARED
 
 inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
