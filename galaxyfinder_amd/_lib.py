@@ -71,6 +71,9 @@ SIGNATURES = {
                                     _c_p, _c_p]),
     "pgalf_dev_finddenpeak": (_c_i, [_c_p, _c_i, _c_p, _c_ll, _c_p, _c_i, _c_f, _c_p, _c_ll,
                                      _c_p, _c_p]),
+    "pgalf_merging_peak": (_c_i, [_c_p, _c_i, _c_p, _c_i, _c_i]),
+    "pgalf_dev_merging_peak": (_c_i, [_c_p, _c_p, _c_p, _c_ll, _c_p, _c_p, _c_i, _c_f, _c_p,
+                                      _c_p, _c_p, _c_p]),
     "pgalf_batch_create": (_c_p, [_c_i, _c_p, _c_p, _c_p, _c_i]),
     "pgalf_batch_grid_offset": (_c_ll, [_c_p, _c_i]),
     "pgalf_batch_run": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i,

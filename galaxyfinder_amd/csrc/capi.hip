@@ -294,6 +294,9 @@ int consume_status(Workspace &ws, hipStream_t s, const char *where) {
         return set_err(PGALF_ERR_OUT_OF_GRID, "%s: particles outside the grid were skipped", where);
     if (st & DEV_ERR_SHORT_LIST)
         return set_err(PGALF_ERR_SHORT_LIST, "%s: short neighbour list", where);
+    if (st & DEV_ERR_BAD_INDEX)
+        return set_err(PGALF_ERR_ARG, "%s: an index outside the particle range was skipped",
+                       where);
     return ok();
 }
 
@@ -975,6 +978,91 @@ extern "C" int pgalf_dev_finddenpeak(const float *den, int k, const long long *n
     if (e == hipSuccess)
         e = find_den_peak(den, k, nbr, np, type, jflag, (double)threshold, core, maxcore,
                           reinterpret_cast<unsigned *>(ncore), ws, s);
+    return e == hipSuccess ? ok() : hip_err(e, where);
+}
+
+extern "C" int pgalf_merging_peak(pgalf_particle_t *bp, int np, pgalf_core_t *core, int numcore,
+                                  int iflag) {
+    const char *where = "pgalf_merging_peak";
+    if (numcore <= 0) {
+        ok();
+        return 0;
+    }
+    if (!bp || !core || np < 1 || numcore >= (1 << 30) || (iflag != 0 && iflag != 1)) {
+        set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
+        return -1;
+    }
+    // the cores' peak positions and keys, gathered on the host (FP64, as
+    // the reference reads bp[core[i].peak])
+    std::vector<double> pos(3 * (size_t)numcore);
+    std::vector<float> key(numcore);
+    for (int i = 0; i < numcore; i++) {
+        const int p = core[i].peak;
+        if (p < 0 || p >= np) {
+            set_err(PGALF_ERR_ARG, "%s: core %d has peak %d outside [0, %d)", where, i, p, np);
+            return -1;
+        }
+        pos[i] = bp[p].x;
+        pos[numcore + i] = bp[p].y;
+        pos[2 * (size_t)numcore + i] = bp[p].z;
+        key[i] = iflag == 0 ? core[i].density : (float)core[i].numstar;
+    }
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    hipStream_t s = nullptr;
+    Workspace &ws = workspace_for_current_device();
+    WsCall call(ws, s);
+    hipError_t e = call.e;
+    double *dpos = ws.alloc_n<double>(3 * (size_t)numcore);
+    float *dkey = ws.alloc_n<float>(numcore);
+    int *dkeep = ws.alloc_n<int>(numcore), *dorder = ws.alloc_n<int>(numcore);
+    unsigned *dn = ws.alloc_n<unsigned>(1);
+    if (e == hipSuccess && (!dpos || !dkey || !dkeep || !dorder || !dn)) e = hipErrorOutOfMemory;
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dpos, pos.data(), sizeof(double) * pos.size(), hipMemcpyHostToDevice,
+                           s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dkey, key.data(), sizeof(float) * numcore, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = merging_peak<double>(dpos, dpos + numcore, dpos + 2 * (size_t)numcore, numcore,
+                                 nullptr, dkey, numcore, (double)PGALF_MERGINGPEAKLENGTH, dkeep,
+                                 dn, dorder, ws, s);
+    std::vector<int> keep(numcore), order(numcore);
+    unsigned nk = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&nk, dn, sizeof(unsigned), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(keep.data(), dkeep, sizeof(int) * numcore, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(order.data(), dorder, sizeof(int) * numcore, hipMemcpyDeviceToHost,
+                           s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        hip_err(e, where);
+        return -1;
+    }
+    // kept cores in rank order, then the ranked array past them
+    std::vector<pgalf_core_t> src(core, core + numcore);
+    for (unsigned j = 0; j < nk; j++) core[j] = src[keep[j]];
+    for (int r = (int)nk; r < numcore; r++) core[r] = src[order[r]];
+    ok();
+    return (int)nk;
+}
+
+extern "C" int pgalf_dev_merging_peak(const float *x, const float *y, const float *z,
+                                      long long np, const int *peak, const float *key,
+                                      int numcore, float link, int *keep, int *order,
+                                      int *nkeep, void *stream) {
+    const char *where = "pgalf_dev_merging_peak";
+    if (np < 1 || !x || !y || !z || !peak || !key || !keep || !nkeep || numcore < 0 ||
+        numcore >= (1 << 30) || !(link >= 0.0f))
+        return set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    hipStream_t s = (hipStream_t)stream;
+    Workspace &ws = workspace_for_current_device();
+    WsCall call(ws, s);
+    hipError_t e = call.e;
+    if (e == hipSuccess)
+        e = merging_peak<float>(x, y, z, np, peak, key, numcore, (double)link, keep,
+                                reinterpret_cast<unsigned *>(nkeep), order, ws, s);
     return e == hipSuccess ? ok() : hip_err(e, where);
 }
 

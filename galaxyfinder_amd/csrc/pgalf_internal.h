@@ -85,6 +85,7 @@ struct BatchDev {
 enum : unsigned {
     DEV_ERR_OUT_OF_GRID = 1u,
     DEV_ERR_SHORT_LIST = 2u,
+    DEV_ERR_BAD_INDEX = 4u,  // an index argument outside the particle range
 };
 
 // Grow-only device arena.  begin() resets the bump pointer; alloc() never
@@ -338,6 +339,15 @@ hipError_t grid_cores(const P *x, const P *y, const P *z, const int *type, long 
 hipError_t find_den_peak(const float *den, int k, const long long *nbr, long long np,
                          const int *type, int jflag, double threshold, int *core,
                          long long maxcore, unsigned *ncore, Workspace &ws, hipStream_t s);
+// MergingPeak (subhaloden.mod6.c:341-415) over n cores: core i sits at the
+// particle peak[i] of np (peak == nullptr: at x[i], y[i], z[i]) and is ranked by
+// key[i], largest first.  keep[0 .. *nkeep): the input indices of the kept
+// cores in rank order (*nkeep a device word); order_out (nullable): the input
+// index of each rank.
+template <class P>
+hipError_t merging_peak(const P *x, const P *y, const P *z, long long np, const int *peak,
+                        const float *key, int n, double link, int *keep, unsigned *nkeep,
+                        int *order_out, Workspace &ws, hipStream_t s);
 
 // ---- phase timing (capi.hip) ---------------------------------------------
 void timing_mark(const char *phase, hipStream_t s, bool end);

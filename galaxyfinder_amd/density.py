@@ -193,6 +193,22 @@ def finddenpeak(den, numneigh, neighbor, jflag, bp, maxcore=None):
                   int(jflag), _ptr(bp))
 
 
+MERGINGPEAKLENGTH = float(np.float32(4.0e-3))  # params.h:49 (PGALF_MERGINGPEAKLENGTH)
+
+
+def MergingPeak(bp, np_, core, numcore, iflag=0):
+    """MergingPeak (subhaloden.mod6.c:341-415) on a Coretype record array, in
+    place, as the reference: returns the new core count; core[:count] are the
+    kept cores, largest key first (density for iflag 0, numstar for 1)."""
+    lib = _lib.load()
+    _need(bp, PARTICLE_DTYPE, np_, "bp")
+    _need(core, CORE_DTYPE, numcore, "core")
+    n = lib.pgalf_merging_peak(_ptr(bp), int(np_), _ptr(core), int(numcore), int(iflag))
+    if n < 0:
+        _lib.check()
+    return n
+
+
 # ---------------------------------------------------------------------------
 # device-resident API (torch tensors on the GPU)
 # ---------------------------------------------------------------------------
@@ -512,6 +528,28 @@ def dev_finddenpeak(den, nbr, types=None, jflag=0, threshold=PEAKTHRESHOLD, maxc
                       _dptr(types), int(jflag), float(threshold))
 
 
+def dev_merging_peak(x, y, z, peak, key, link=MERGINGPEAKLENGTH, order=False):
+    """MergingPeak on device (pgalf_dev_merging_peak): cores at particles
+    peak (int32), ranked by key (float32), largest first.  Returns (keep,
+    count): the input indices of the kept cores in rank order in keep[:count]
+    (int32 tensors); with order=True also the input index of every rank."""
+    torch = _torch()
+    n = x.numel()
+    for t, w in ((x, "x"), (y, "y"), (z, "z")):
+        _check_dev(t, torch.float32, n, w)
+    nc = peak.numel()
+    _check_dev(peak, torch.int32, nc, "peak")
+    _check_dev(key, torch.float32, nc, "key")
+    keep = torch.empty(max(nc, 1), dtype=torch.int32, device=x.device)
+    cnt = torch.zeros(1, dtype=torch.int32, device=x.device)
+    od = torch.empty(max(nc, 1), dtype=torch.int32, device=x.device) if order else None
+    lib = _lib.load()
+    _lib.check(lib.pgalf_dev_merging_peak(_dptr(x), _dptr(y), _dptr(z), n, _dptr(peak),
+                                          _dptr(key), nc, float(link), _dptr(keep), _dptr(od),
+                                          _dptr(cnt), _stream()))
+    return (keep, cnt, od) if order else (keep, cnt)
+
+
 def dev_status():
     """Synchronise torch's current stream and raise on kernel-detected errors."""
     lib = _lib.load()
@@ -570,4 +608,5 @@ __all__ = ["PgalfError", "PARTICLE_DTYPE", "make_particles", "padded_shape",
            "dev_tsc", "dev_smooth", "dev_findden", "dev_knn", "dev_density_stage",
            "dev_density_stage_backward", "dev_peak_climb", "dev_status", "Batch", "timing",
            "timing_get", "CORE_DTYPE", "grid_cores", "finddenpeak", "dev_grid_cores",
-           "dev_finddenpeak", "set_schedule", "get_schedule"]
+           "dev_finddenpeak", "set_schedule", "get_schedule", "MergingPeak",
+           "dev_merging_peak"]

@@ -146,6 +146,20 @@ int pgalf_grid_cores(pgalf_particle_t *bp, int np, const float *densph, const fl
 int pgalf_finddenpeak(float *den, int numneigh, long long *neighbor, int np, int jflag,
                       pgalf_particle_t *bp, pgalf_core_t *core, int maxcore, int *numcore);
 
+/* MergingPeak (subhaloden.mod6.c:341-415, same arguments and return value):
+ * the cores are ordered by density (iflag 0) or by numstar (iflag 1),
+ * largest first, equal keys in their input order; of each
+ * friends-of-friends group of their peak particles (link length
+ * PGALF_MERGINGPEAKLENGTH, ost.c:306-356) the first core is kept.  core[0 ..
+ * return) holds the kept cores in that order, core[return .. numcore) the
+ * rest of the ordered array, as the reference leaves it.  iflag 1 does not
+ * touch the reference's static peak flags (subhaloden.mod6.c:199-212, which
+ * its only caller with iflag 1 maintains, :1465).  numcore <= 0 returns 0;
+ * on error -1 (pgalf_last_error). */
+#define PGALF_MERGINGPEAKLENGTH 4.0e-3f /* params.h:49 */
+int pgalf_merging_peak(pgalf_particle_t *bp, int np, pgalf_core_t *core, int numcore,
+                       int iflag);
+
 /* ===================== Part 2: device-resident API ======================== */
 
 typedef struct pgalf_grid {
@@ -230,8 +244,18 @@ int pgalf_dev_grid_cores(const float *x, const float *y, const float *z, const i
 int pgalf_dev_finddenpeak(const float *den, int k, const long long *nbr, long long np,
                           const int *type, int jflag, float threshold, int *core,
                           long long maxcore, int *ncore, void *stream);
+/* Device form of pgalf_merging_peak: core i at particle peak[i], ranked by
+ * key[i] (largest first, ties in input order); keep[0 .. *nkeep) receives
+ * the input indices of the kept cores in rank order, order (nullable) the
+ * input index of every rank; *nkeep is left in device memory.  link: the
+ * reference uses PGALF_MERGINGPEAKLENGTH.  numcore < 2^30.  A peak outside
+ * [0, np) links to nothing and is reported by pgalf_dev_status. */
+int pgalf_dev_merging_peak(const float *x, const float *y, const float *z, long long np,
+                           const int *peak, const float *key, int numcore, float link,
+                           int *keep, int *order, int *nkeep, void *stream);
 /* Synchronise `stream` and return the status of the kernels run since the
- * previous call (PGALF_ERR_OUT_OF_GRID / PGALF_ERR_SHORT_LIST or PGALF_OK). */
+ * previous call (PGALF_ERR_OUT_OF_GRID / PGALF_ERR_SHORT_LIST / PGALF_ERR_ARG
+ * for an index outside the particle range, or PGALF_OK). */
 int pgalf_dev_status(void *stream);
 
 /* ===================== Part 3: batched catalogue API ====================== */

@@ -84,6 +84,8 @@ def lib():
                                      _d, _d, _d, _d, _i, _lp, _i64]
         L.orc_finddenpeak.restype = _i64
         L.orc_finddenpeak.argtypes = [_fp, _i, _lp, _i64, _ip, _i, _d, _lp, _i64]
+        L.orc_merging_peak.restype = _i64
+        L.orc_merging_peak.argtypes = [_dp, _dp, _dp, _fp, _i64, _d, _ip, _ip]
         _LIB = L
     return _LIB
 
@@ -118,6 +120,8 @@ def ref_lib():
         R.ref_finddenpeak.restype = _i64
         R.ref_finddenpeak.argtypes = [_fp, _i, _lp, _i64, _dp, _dp, _dp, _ip, _i, _ip, _fp,
                                       _fp, _fp, _fp, _i64]
+        R.ref_merging_peak.restype = _i
+        R.ref_merging_peak.argtypes = [_dp, _dp, _dp, _i64, _ip, _fp, _i, _ip]
         R.ref_findsphdensity.restype = None
         R.ref_findsphdensity.argtypes = [_dp, _dp, _dp, _dp, _i64, _i, _lp, _fp, _i]
         R.ref_sizeof_bp.restype = _i
@@ -436,3 +440,37 @@ def finddenpeak(den, nbr, types=None, jflag=0, threshold=PEAKTHRESHOLD):
     c = lib().orc_finddenpeak(_p(d, _fp), k, _p(nb, _lp), n, _p(t, _ip), jflag, threshold,
                               _p(out, _lp), n)
     return out[:c].copy()
+
+
+MERGINGPEAKLENGTH = float(np.float32(4.0e-3))  # params.h:49, a float in MergingPeak
+
+
+def merging_peak(pos, peak, key, link=MERGINGPEAKLENGTH):
+    """MergingPeak (subhaloden.mod6.c:341-415) on cores given by their peak
+    particle indices and ordering key (density): returns (order, keep), the
+    input indices in rank order and those of the kept cores, in rank order
+    (orc_merging_peak)."""
+    pk = np.asarray(peak, dtype=np.int64)
+    p = _f64(pos)[pk]
+    x, y, z = (np.ascontiguousarray(p[:, a]) for a in range(3))
+    kf = np.ascontiguousarray(key, dtype=np.float32)
+    n = len(pk)
+    order = np.zeros(max(1, n), dtype=np.int32)
+    keep = np.zeros(max(1, n), dtype=np.int32)
+    c = lib().orc_merging_peak(_p(x, _dp), _p(y, _dp), _p(z, _dp), _p(kf, _fp), n, link,
+                               _p(order, _ip), _p(keep, _ip))
+    return order[:n].astype(np.int64), keep[:c].astype(np.int64)
+
+
+def ref_merging_peak(pos, peak, density):
+    """The reference's own MergingPeak (iflag 0) through oracle/_ref: returns
+    (numcore, peaks of the whole core array after the call)."""
+    R = ref_lib()
+    pos = _f64(pos)
+    x, y, z = (np.ascontiguousarray(pos[:, a]) for a in range(3))
+    pk = np.ascontiguousarray(peak, dtype=np.int32)
+    d = np.ascontiguousarray(density, dtype=np.float32)
+    out = np.zeros(len(pk), dtype=np.int32)
+    n = R.ref_merging_peak(_p(x, _dp), _p(y, _dp), _p(z, _dp), len(x), _p(pk, _ip), _p(d, _fp),
+                           len(pk), _p(out, _ip))
+    return n, out.astype(np.int64)

@@ -28,6 +28,10 @@
  *   orc_finddenpeak       -- bit-exact vs the reference build's finddenpeak
  *                            (subhaloden.mod6.c compiled in place) on golden
  *                            densities and lists.
+ *   orc_merging_peak      -- vs the reference build's MergingPeak (with
+ *                            ost.c's friends-of-friends tree, compiled in
+ *                            place) on golden core sets; brute-force
+ *                            component invariants.
  *   orc_grid_cores        -- the reference block is inline in
  *                            lagFindStellarCore (unbuildable: it calls the
  *                            FFTW smoothing): pinned by an independent
@@ -1014,4 +1018,69 @@ long long orc_finddenpeak(const float *den, int k, const long long *nbr, long lo
         }
     }
     return ncore;
+}
+
+/* ------------------------------------------------------------------------ */
+/* MergingPeak (subhaloden.mod6.c:341-415; friends-of-friends walk of       */
+/* ost.c:306-356).  Cores are ordered by key, largest first (the qsort with */
+/* sortcoreden, :323-330; equal keys keep their input order, as glibc's     */
+/* merge sort does).  In that order, each core not yet claimed claims every */
+/* unclaimed core reachable through links and stays a core itself; two      */
+/* cores link when (double)sqrtf((float)d2) <= link, d2 the FP64 squared   */
+/* distance of their peak particles formed dx*dx+dy*dy+dz*dz (ost.c:       */
+/* 328-333).  x, y, z: each core's peak position.  order[r]: the input      */
+/* index of the core of rank r.  keep[0 .. return): the kept cores' input   */
+/* indices in rank order.  A breadth-first claim over all unclaimed cores  */
+/* per step: O(n^2), for test sizes.                                        */
+/* ------------------------------------------------------------------------ */
+static void orc_stable_desc(const float *key, int *idx, int *tmp, long long n) {
+    for (long long w = 1; w < n; w *= 2) {
+        for (long long lo = 0; lo < n; lo += 2 * w) {
+            long long mid = lo + w < n ? lo + w : n, hi = lo + 2 * w < n ? lo + 2 * w : n;
+            long long a = lo, b = mid, t = lo;
+            while (a < mid && b < hi) tmp[t++] = key[idx[b]] > key[idx[a]] ? idx[b++] : idx[a++];
+            while (a < mid) tmp[t++] = idx[a++];
+            while (b < hi) tmp[t++] = idx[b++];
+        }
+        memcpy(idx, tmp, sizeof(int) * (size_t)n);
+    }
+}
+
+static int orc_fof_linked(const double *x, const double *y, const double *z, int a, int b,
+                          double link) {
+    double dx = x[a] - x[b], dy = y[a] - y[b], dz = z[a] - z[b];
+    double d2 = dx * dx + dy * dy + dz * dz;
+    float d = sqrtf((float)d2);
+    return (double)d <= link;
+}
+
+long long orc_merging_peak(const double *x, const double *y, const double *z, const float *key,
+                           long long n, double link, int *order, int *keep) {
+    if (n <= 0) return 0;
+    int *tmp = (int *)malloc(sizeof(int) * (size_t)n);
+    int *queue = (int *)malloc(sizeof(int) * (size_t)n);
+    unsigned char *claimed = (unsigned char *)calloc((size_t)n, 1);
+    for (long long i = 0; i < n; i++) order[i] = (int)i;
+    orc_stable_desc(key, order, tmp, n);
+    long long nkeep = 0;
+    for (long long r = 0; r < n; r++) {
+        const int s = order[r];
+        if (claimed[s]) continue;
+        long long head = 0, tail = 0;
+        claimed[s] = 1;
+        queue[tail++] = s;
+        while (head < tail) {
+            const int a = queue[head++];
+            for (long long b = 0; b < n; b++)
+                if (!claimed[b] && orc_fof_linked(x, y, z, a, (int)b, link)) {
+                    claimed[b] = 1;
+                    queue[tail++] = (int)b;
+                }
+        }
+        keep[nkeep++] = s;
+    }
+    free(tmp);
+    free(queue);
+    free(claimed);
+    return nkeep;
 }

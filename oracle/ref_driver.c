@@ -361,3 +361,26 @@ long long ref_finddenpeak(const float *den, int k, const long long *nbr, long lo
     free(bp);
     return n;
 }
+
+/* MergingPeak (subhaloden.mod6.c:341-415, friends-of-friends of ost.c) on
+ * cores given by their peak particle and density (iflag 0: ordered by
+ * density); returns the new core count.  order_peak[0 .. numcore) receives
+ * the peaks of the whole core array after the call (the kept cores first,
+ * then what the reference leaves behind them). */
+int MergingPeak(SimpleBasicParticleType *, int, Coretype *, int, int);
+
+int ref_merging_peak(const double *x, const double *y, const double *z, long long np,
+                     const int *peak, const float *density, int numcore, int *order_peak) {
+    ref_init(4LL << 30);
+    SimpleBasicParticleType *bp = ref_pack(x, y, z, NULL, NULL, np);
+    Coretype *core = (Coretype *)calloc((size_t)numcore, sizeof(Coretype));
+    for (int i = 0; i < numcore; i++) {
+        core[i].peak = peak[i];
+        core[i].density = density[i];
+    }
+    int n = MergingPeak(bp, (int)np, core, numcore, 0);
+    for (int i = 0; i < numcore; i++) order_peak[i] = core[i].peak;
+    free(core);
+    free(bp);
+    return n;
+}

@@ -25,6 +25,17 @@ With the argument ``peaks``:
                   reference finddenpeak's cores (subhaloden.mod6.c:416-492)
                   for jflag 0 and 1 on each.
 
+With the argument ``merge``:
+  merge_small.npz core sets and the reference MergingPeak's result
+                  (subhaloden.mod6.c:341-415 with ost.c's friends-of-friends
+                  tree, iflag 0): the peaks of the whole core array after
+                  the call and the kept count.  Cases: "clumps" (40 clumps
+                  of 200 particles of width 4e-3, 1500 cores, densities
+                  quantised so that they tie), "halo" (every particle of
+                  the 4096-particle halo a core, its finddenpeak fixture
+                  densities), "chain" (cores on lines spaced just under and
+                  just over the link length, and coincident cores).
+
 With the argument ``config``, fixtures at the sizes of the two one-halo
 workloads instead (their particles are regenerated by
 galaxyfinder_amd.halos.single_halo_config; a CRC of the float32 positions is
@@ -173,6 +184,41 @@ def peaks_fixture(R):
     np.savez_compressed(os.path.join(HERE, "peaks_small.npz"), **out)
 
 
+def merge_cases():
+    """Inputs of merge_small.npz: (name, positions, peak indices, densities)."""
+    rng = np.random.default_rng(11)
+    cent = rng.uniform(0.3, 0.7, (40, 3))
+    pos = np.concatenate([c + rng.normal(0.0, 0.004, (200, 3)) for c in cent])
+    pos = pos.astype(np.float32).astype(np.float64)
+    peak = rng.choice(len(pos), 1500, replace=False)
+    den = np.round(rng.uniform(0.0, 100.0, 1500)).astype(np.float32)
+    yield "clumps", pos, peak, den
+    kf = np.load(os.path.join(HERE, "knn_small.npz"))
+    pf = np.load(os.path.join(HERE, "peaks_small.npz"))
+    hp = kf["pos"]
+    yield "halo", hp, np.arange(len(hp)), pf["den_a"]
+    lines = []
+    for i, step in enumerate((0.0039, 0.0041, 0.00399, 0.0040001)):
+        t = np.arange(30)[:, None] * np.array([[step, 0.0, 0.0]])
+        lines.append(t + np.array([0.2, 0.1 + 0.05 * i, 0.5]))
+    lines.append(np.repeat(np.array([[0.8, 0.8, 0.8]]), 5, axis=0))  # coincident
+    cp = np.concatenate(lines).astype(np.float32).astype(np.float64)
+    cden = np.random.default_rng(12).permutation(len(cp)).astype(np.float32)
+    yield "chain", cp, np.arange(len(cp)), cden
+
+
+def merge_fixture(R):
+    out = {}
+    for name, pos, peak, den in merge_cases():
+        n, order_peak = O.ref_merging_peak(pos, peak, den)
+        out[name + "_pos"] = pos
+        out[name + "_peak"] = np.asarray(peak, dtype=np.int32)
+        out[name + "_den"] = den
+        out[name + "_n"] = np.int64(n)
+        out[name + "_out"] = order_peak.astype(np.int32)
+    np.savez_compressed(os.path.join(HERE, "merge_small.npz"), **out)
+
+
 def main():
     O.build(ref=True)
     R = O.ref_lib()
@@ -182,6 +228,10 @@ def main():
     if len(sys.argv) > 1 and sys.argv[1] == "peaks":
         peaks_fixture(R)
         print("peaks_small.npz", os.path.getsize(os.path.join(HERE, "peaks_small.npz")))
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "merge":
+        merge_fixture(R)
+        print("merge_small.npz", os.path.getsize(os.path.join(HERE, "merge_small.npz")))
         return
     if len(sys.argv) > 1 and sys.argv[1] == "config":
         config_fixtures(R)

@@ -138,3 +138,17 @@ def test_reference_nnost_links_against_the_library():
     for sym in ("assign_density_TSC", "gaussian_Smoothing", "findDen"):
         assert "nnost.o: reference to %s" % sym in r.stderr
         assert "libpgalf_density.so: definition of %s" % sym in r.stderr
+
+
+def test_merging_peak_host_checks_without_gpu_calls():
+    """pgalf_merging_peak returns 0 for no cores and -1 (PGALF_ERR_ARG) for
+    a peak outside the particles, before any device work."""
+    bp = D.make_particles(np.array([[0.1, 0.2, 0.3], [0.4, 0.5, 0.6]]))
+    core = np.zeros(2, dtype=D.CORE_DTYPE)
+    assert D.MergingPeak(bp, 2, core, 0) == 0
+    core["peak"] = [0, 7]
+    with pytest.raises(_lib.PgalfError) as e:
+        D.MergingPeak(bp, 2, core, 2)
+    assert e.value.code == 1
+    with pytest.raises(_lib.PgalfError):
+        D.MergingPeak(bp, 2, core, 2, iflag=3)

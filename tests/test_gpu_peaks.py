@@ -8,7 +8,14 @@
     particles bit-identical to the oracle (pinned by brute force and the
     particle-0 / tie / plateau cases in tests/test_oracle.py) on planted
     grids and on the 131072- and 4194304-particle halos' own smoothed grids,
-    read-back densities and lists.
+    read-back densities and lists;
+  * MergingPeak (subhaloden.mod6.c:341-415): the whole core array after the
+    call bit-identical to the reference's own MergingPeak (fixture
+    tests/golden/merge_small.npz: clumps, the 4096-particle halo, chains at
+    the link length, coincident cores), through the host and the device
+    entry points; kept cores and rank order equal to the oracle's (pinned
+    by that fixture and by scipy connected components) on 20000 dense cores
+    with tied keys and on the 4194304-particle halo's grid cores.
 Integer outputs: the bar is equality.
 """
 import os
@@ -157,3 +164,85 @@ def test_peak_stage_on_workload_halos(config):
         got = _take(D.dev_finddenpeak(rk, nbr, _dev(types), jflag, thr))
         say("C%d finddenpeak jflag=%d: %d cores" % (config, jflag, len(ref)))
         assert len(ref) > 0 and np.array_equal(got, ref)
+
+
+def _core_records(peak, den):
+    core = np.zeros(len(peak), dtype=D.CORE_DTYPE)
+    core["peak"] = peak
+    core["density"] = den
+    return core
+
+
+@pytest.mark.parametrize("name", ["clumps", "halo", "chain"])
+def test_merging_peak_vs_reference_fixture(name):
+    """MergingPeak (subhaloden.mod6.c:341-415) against the reference build's
+    own result: the whole core array after the call and the kept count,
+    through the host entry point; the kept cores and the rank order through
+    the device one."""
+    f = np.load(os.path.join(GOLD, "merge_small.npz"))
+    pos, peak, den = f[name + "_pos"], f[name + "_peak"], f[name + "_den"]
+    n, out = int(f[name + "_n"]), f[name + "_out"].astype(np.int64)
+    say("MergingPeak fixture %s: %d cores, %d kept" % (name, len(peak), n))
+    bp = D.make_particles(pos)
+    core = _core_records(peak, den)
+    assert D.MergingPeak(bp, len(bp), core, len(core), 0) == n
+    assert np.array_equal(core["peak"].astype(np.int64), out)
+    keep, cnt, order = D.dev_merging_peak(*(_dev(pos[:, a].astype(np.float32)) for a in range(3)),
+                                          _dev(peak.astype(np.int32)), _dev(den), order=True)
+    D.dev_status()
+    c = int(cnt.item())
+    assert c == n
+    pk = peak.astype(np.int64)
+    assert np.array_equal(pk[keep.cpu().numpy()[:c]], out[:n])
+    assert np.array_equal(pk[order.cpu().numpy()][n:], out[n:])
+
+
+def test_merging_peak_dense_random_vs_oracle():
+    """20000 cores packed so that groups chain across many cells, with tied
+    keys: kept cores and rank order equal to the oracle's."""
+    rng = np.random.default_rng(31)
+    pos = rng.uniform(0.45, 0.55, (60000, 3)).astype(np.float32)
+    peak = rng.choice(len(pos), 20000, replace=False).astype(np.int32)
+    key = np.round(rng.uniform(0, 50, len(peak))).astype(np.float32)
+    order_o, keep_o = O.merging_peak(pos.astype(np.float64), peak, key)
+    say("MergingPeak dense random: %d cores, %d kept (oracle)" % (len(peak), len(keep_o)))
+    keep, cnt, order = D.dev_merging_peak(*(_dev(pos[:, a]) for a in range(3)), _dev(peak),
+                                          _dev(key), order=True)
+    D.dev_status()
+    c = int(cnt.item())
+    assert np.array_equal(order.cpu().numpy().astype(np.int64), order_o)
+    assert c == len(keep_o) and np.array_equal(keep.cpu().numpy()[:c].astype(np.int64), keep_o)
+
+
+def test_merging_peak_on_workload_grid_cores():
+    """The grid cores of the 4194304-particle halo (threshold at the median
+    of the smoothed grid, ranked by their read-back densities) merged on the
+    GPU and by the oracle.  Grid cores are strict 26-neighbour maxima, two
+    or more cells (2/256 of the box) apart, so the reference link length
+    (4e-3) merges none of them here; a link of 4 cells does."""
+    import torch
+    pos, mass, spec = halos.single_halo_config(2)
+    n, ng = len(pos), spec["nx"]
+    grid = (ng, ng, ng, spec["xmin"], spec["ymin"], spec["zmin"], spec["cellsize"])
+    g = D.grid_struct(*grid)
+    x, y, z = (_dev(pos[:, a]) for a in range(3))
+    den = torch.empty(D.padded_shape(ng, ng, ng), dtype=torch.float32, device="cuda")
+    rt = torch.empty(n, dtype=torch.float32, device="cuda")
+    D.dev_tsc(x, y, z, _dev(mass), g, den)
+    D.dev_smooth(den, g, spec["RG"])
+    D.dev_findden(x, y, z, g, den, rt)
+    D.dev_status()
+    den_h = den.cpu().numpy()
+    thr = float(np.quantile(den_h[..., :ng][den_h[..., :ng] > 0], 0.5))
+    core = _take(D.dev_grid_cores(x, y, z, rt, g, den, threshold=thr, nbuff=10))
+    key = rt.cpu().numpy()[core]
+    for link in (D.MERGINGPEAKLENGTH, float(np.float32(4.0 / ng))):
+        order_o, keep_o = O.merging_peak(pos.astype(np.float64), core, key, link)
+        say("MergingPeak of C2 grid cores, link %.4g: %d cores, %d kept"
+            % (link, len(core), len(keep_o)))
+        keep, cnt = D.dev_merging_peak(x, y, z, _dev(core.astype(np.int32)), _dev(key), link)
+        D.dev_status()
+        c = int(cnt.item())
+        assert c == len(keep_o) and np.array_equal(keep.cpu().numpy()[:c].astype(np.int64),
+                                                   keep_o)
+    assert len(keep_o) < len(core)

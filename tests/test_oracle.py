@@ -422,3 +422,64 @@ def test_grid_cores_particle_zero_and_ties():
     assert O.grid_cores(pos, types, 1, densph, den, grid).tolist() == [4]
     assert O.grid_cores(pos, types, 5, densph, den, grid).tolist() == [2]
     assert _brute_grid_cores(pos, types, 1, densph, den, grid, 2000.0, 10).tolist() == [4]
+
+
+def _merge_expected(f, name):
+    n = int(f[name + "_n"])
+    return n, f[name + "_out"].astype(np.int64)
+
+
+@pytest.mark.parametrize("name", ["clumps", "halo", "chain"])
+def test_merging_peak_vs_reference_fixture(name):
+    """orc_merging_peak against the reference's MergingPeak (fixture written
+    from the reference build): the kept cores in order, and the whole core
+    array after the call (kept cores, then the rest of the sorted array)."""
+    f = np.load(os.path.join(GOLD, "merge_small.npz"))
+    pos, peak, den = f[name + "_pos"], f[name + "_peak"].astype(np.int64), f[name + "_den"]
+    n, out = _merge_expected(f, name)
+    order, keep = O.merging_peak(pos, peak, den)
+    assert len(keep) == n
+    assert np.array_equal(peak[keep], out[:n])
+    assert np.array_equal(peak[order][n:], out[n:])
+
+
+def _fof_components(p, link):
+    """Connected components of the link graph by an independent method
+    (scipy's connected_components over brute-force pairs)."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    d2 = ((p[:, None, :] - p[None, :, :]) ** 2).sum(axis=2)
+    adj = np.sqrt(d2.astype(np.float32)).astype(np.float64) <= link
+    i, j = np.nonzero(adj)
+    g = coo_matrix((np.ones(len(i)), (i, j)), shape=(len(p), len(p)))
+    return connected_components(g, directed=False)[1]
+
+
+def test_merging_peak_is_first_of_each_fof_group():
+    rng = np.random.default_rng(21)
+    cent = rng.uniform(0.2, 0.8, (30, 3))
+    pos = np.concatenate([c + rng.normal(0.0, 0.003, (40, 3)) for c in cent])
+    pos = pos.astype(np.float32).astype(np.float64)
+    peak = rng.permutation(len(pos))[:900]
+    den = np.round(rng.uniform(0, 30, len(peak))).astype(np.float32)  # many ties
+    order, keep = O.merging_peak(pos, peak, den)
+    assert np.array_equal(order, np.argsort(-den, kind="stable"))
+    lab = _fof_components(pos[peak], O.MERGINGPEAKLENGTH)
+    rank = np.empty(len(peak), dtype=np.int64)
+    rank[order] = np.arange(len(peak))
+    first = {}
+    for r, i in enumerate(order):
+        first.setdefault(lab[i], i)
+    want = sorted(first.values(), key=lambda i: rank[i])
+    assert keep.tolist() == want
+    assert len(set(lab[keep])) == len(keep) == lab.max() + 1
+
+
+def test_merging_peak_vs_reference_build_random(ref):
+    rng = np.random.default_rng(22)
+    pos = rng.uniform(0.4, 0.6, (3000, 3)).astype(np.float32).astype(np.float64)
+    peak = rng.choice(3000, 2000, replace=False)
+    den = rng.uniform(0, 1, 2000).astype(np.float32)
+    n, out = O.ref_merging_peak(pos, peak, den)
+    order, keep = O.merging_peak(pos, peak, den)
+    assert len(keep) == n and np.array_equal(peak[keep], out[:n])
