@@ -156,6 +156,103 @@ def cpu_baseline(pos, mass, spec, k, knn_stride):
             "sample": sample, "stages_s": {a: round(b, 4) for a, b in st.items()}}
 
 
+def peaks_bench(x, y, z, g, den, rho_t, nbr, rho_k, spec, reps=10, cpu=True):
+    """The integer peak stage on the headline halo's own stage outputs: grid
+    cores (nnost.c:1115-1271; densph = the read-back densities), finddenpeak
+    (subhaloden.mod6.c:416-492; the W4 kNN densities and 32-NN lists),
+    MergingPeak of the grid cores (subhaloden.mod6.c:341-415, link 4e-3).
+    The synthetic halo's densities are not in the reference's units, so the
+    thresholds are the medians of the positive grid values and of the kNN
+    densities (the reference's PEAKTHRESHOLD = 2000 would select nothing;
+    for the reference's finddenpeak, whose threshold is compiled in, the
+    densities are scaled so that the same share passes it).
+    GPU times: HIP events on the launch stream, median of `reps`.  CPU: the
+    reference's own finddenpeak and MergingPeak (oracle/_ref) and the
+    oracle's grid cores, on the same inputs."""
+    import torch
+    from galaxyfinder_amd import density as D
+    n, k = x.numel(), nbr.shape[1]
+    ng = spec["nx"]
+    # every 4th particle a star
+    types = torch.from_numpy(np.where(np.arange(n) % 4 == 0, 1, 2).astype(np.int32)).cuda()
+    den_h = den.cpu().numpy()
+    cells = den_h[..., :ng]
+    thr_g = float(np.quantile(cells[cells > 0], 0.5))
+    thr_k = float(np.quantile(rho_k.cpu().numpy(), 0.5))
+    st = torch.cuda.current_stream()
+
+    def timed(fn):
+        fn()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(reps)]
+        for a, b in ev:
+            a.record(st)
+            fn()
+            b.record(st)
+        torch.cuda.synchronize()
+        D.dev_status()
+        return float(np.median([a.elapsed_time(b) for a, b in ev]))
+
+    out = {"thresholds": {"grid": thr_g, "knn": thr_k}}
+    res = {}
+
+    def gc():
+        res["gc"] = D.dev_grid_cores(x, y, z, rho_t, g, den, threshold=thr_g, nbuff=10)
+    t = timed(gc)
+    core = res["gc"][0][:int(res["gc"][1].item())].clone()
+    gb = (4 * den.numel() + 20 * n) / (t * 1e-3) / 1e9
+    out["grid_cores"] = {"ms": round(t, 4), "cores": int(core.numel()),
+                         "alg_gbs": round(gb, 1)}
+
+    def fdp():
+        res["fd"] = D.dev_finddenpeak(rho_k, nbr, types, 0, thr_k)
+    t = timed(fdp)
+    gb = (8 + 8 * k) * n / (t * 1e-3) / 1e9
+    out["finddenpeak"] = {"ms": round(t, 4), "cores": int(res["fd"][1].item()),
+                          "alg_gbs": round(gb, 1)}
+    key = rho_t[core.long()].contiguous()
+
+    def mp():
+        res["mp"] = D.dev_merging_peak(x, y, z, core, key)
+    t = timed(mp)
+    out["merging_peak"] = {"ms": round(t, 4), "cores_in": int(core.numel()),
+                           "cores_kept": int(res["mp"][1].item())}
+    if cpu:
+        from oracle import oracle as O
+        R = O.ref_lib()
+        pos = np.stack([a.cpu().numpy() for a in (x, y, z)], axis=1).astype(np.float64)
+        rt_h, rk_h, nb_h = rho_t.cpu().numpy(), rho_k.cpu().numpy(), nbr.cpu().numpy()
+        ty_h = types.cpu().numpy()
+        grid = (ng, spec["ny"], spec["nz"], spec["xmin"], spec["ymin"], spec["zmin"],
+                spec["cellsize"])
+        c = {}
+        t0 = time.perf_counter()
+        gcore = O.grid_cores(pos, ty_h, 5, rt_h, den_h, grid, thr_g, 10)
+        c["grid_cores_oracle_s"] = round(time.perf_counter() - t0, 4)
+        if R is not None:
+            x_, y_, z_ = (np.ascontiguousarray(pos[:, a]) for a in range(3))
+            cap = n
+            pk = np.zeros(cap, dtype=np.int32)
+            cx, cy, cz, cd = (np.zeros(cap, dtype=np.float32) for _ in range(4))
+            nbc = np.ascontiguousarray(nb_h)
+            R.ref_init(8 << 30)
+            # the reference's threshold is compiled in (PEAKTHRESHOLD): the
+            # densities are scaled so that the same share passes it
+            rk_s = (rk_h.astype(np.float64) * (O.PEAKTHRESHOLD / thr_k)).astype(np.float32)
+            t0 = time.perf_counter()
+            R.ref_finddenpeak(O._p(rk_s, O._fp), k, O._p(nbc, O._lp), n, O._p(x_, O._dp),
+                              O._p(y_, O._dp), O._p(z_, O._dp), O._p(ty_h, O._ip), 0,
+                              O._p(pk, O._ip), O._p(cx, O._fp), O._p(cy, O._fp),
+                              O._p(cz, O._fp), O._p(cd, O._fp), cap)
+            c["finddenpeak_reference_s"] = round(time.perf_counter() - t0, 4)
+            t0 = time.perf_counter()
+            O.ref_merging_peak(pos, gcore, rt_h[gcore])
+            c["merging_peak_reference_s"] = round(time.perf_counter() - t0, 4)
+        c["threads"] = O.num_threads()
+        out["cpu"] = c
+    return out
+
+
 def batch_bench(steps, warmup, plain_pass=True):
     """Batched forward over the 10000-halo catalogue (dN/dn ~ n^-2 on
     [100, 1048576], grids 32^3-256^3 from n, RG = 2 cells, k = 32): eager
@@ -410,6 +507,8 @@ def main():
                     help="skip the batched 10000-halo catalogue measurement")
     ap.add_argument("--batch-only", action="store_true",
                     help="only the batched catalogue measurement (no per-halo pass)")
+    ap.add_argument("--no-peaks", action="store_true",
+                    help="skip the integer peak stage measurement")
     ap.add_argument("--no-grad", action="store_true",
                     help="skip the forward+backward measurement")
     ap.add_argument("--c5", action="store_true",
@@ -541,6 +640,9 @@ def main():
         "stage_roofline_frac": round(stage_alg / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": roofline,
     }
+    if not args.no_peaks:
+        out["peaks"] = peaks_bench(x, y, z, g, den, rho_t, nbr, rho_k, spec,
+                                   cpu=not args.no_cpu_baseline)
     if not args.no_batch:
         out["batch"] = batch_bench(max(3, args.steps // 2), 2)
     if not args.no_grad:

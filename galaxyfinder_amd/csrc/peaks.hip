@@ -13,18 +13,19 @@
 // is particle 0 (`if(ibp > 0)`).  Cores come in (k, j, i) node order.  Here:
 //   1. peak_flag_k   one thread per interior node: the strict 26-neighbour
 //                    maximum above the threshold -> flag;
-//   2. an exclusive scan of the flags gives the peak nodes in (k, j, i)
-//      order (peak_list_k);
+//   2. an exclusive scan of the flags ranks the peak nodes in (k, j, i)
+//      order; peak_list_k clears their state;
 //   3. peak_member_k one thread per selected particle: its node (rint of the
-//      FP64 offset in cells, as the reference), a binary search among the
-//      peak nodes, then atomicMax of (ordered densph << 32 | index) -- the
+//      FP64 offset in cells, as the reference), the node's flag and rank
+//      read directly, then atomicMax of (ordered densph << 32 | index) -- the
 //      maximal density, ties to the larger index -- and a flag when the
 //      particle is particle 0;
 //   4. core_compact_k one workgroup: the peaks whose chain is not empty and
 //      does not hold particle 0, compacted in order.
 // finddenpeak: one thread per particle (its density above the threshold, a
-// star when jflag == 1, no neighbour strictly denser), then an exclusive scan
-// compacts the cores in index order.
+// star when jflag == 1, no neighbour strictly denser; lists of k <= 32 staged
+// in LDS by coalesced loads), then an exclusive scan compacts the cores in
+// index order.
 #include <cfloat>
 #include <cstdint>
 
@@ -70,21 +71,24 @@ __global__ void peak_flag_k(const float *den, Interior in, double threshold, uns
     flag[t] = peak ? 1u : 0u;
 }
 
-// Peak nodes in (k, j, i) order; their per-peak state cleared.
+// Per-peak state cleared (peak p is the p-th flagged node in (k, j, i) order).
 __global__ void peak_list_k(const unsigned *flag, const unsigned *pos, Interior in,
-                            long long *peak_cell, unsigned long long *best, unsigned *has0) {
+                            unsigned long long *best, unsigned *has0) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= in.count() || !flag[t]) return;
     const unsigned p = pos[t];
-    peak_cell[p] = in.cell(t);
     best[p] = 0ull;
     has0[p] = 0u;
 }
 
+// The particle's node (rint of the FP64 offset in cells, as the reference);
+// only interior nodes can be peaks, and their flags and scan positions give
+// the peak's rank directly.
 template <class P>
 __global__ void peak_member_k(const P *x, const P *y, const P *z, const int *type, long long np,
-                              int ptype, const float *densph, Grid g, const long long *peak_cell,
-                              const unsigned *npeak, unsigned long long *best, unsigned *has0) {
+                              int ptype, const float *densph, Grid g, Interior in,
+                              const unsigned *flag, const unsigned *pos,
+                              unsigned long long *best, unsigned *has0) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     if (ptype == PGALF_TYPE_STAR) {
@@ -92,21 +96,15 @@ __global__ void peak_member_k(const P *x, const P *y, const P *z, const int *typ
     } else if (ptype != PGALF_TYPE_ALL) {
         return;  // the reference chains stars or all particles only
     }
-    const long long a = (long long)rint(((double)x[i] - g.x0) / g.h);
-    const long long b = (long long)rint(((double)y[i] - g.y0) / g.h);
-    const long long c = (long long)rint(((double)z[i] - g.z0) / g.h);
-    if (a < 0 || a >= g.nx || b < 0 || b >= g.ny || c < 0 || c >= g.nz) return;
-    const long long cell = a + g.mx() * (b + (long long)g.ny * c);
-    // lower bound among the sorted peak nodes
-    long long lo = 0, hi = (long long)*npeak;
-    while (lo < hi) {
-        const long long mid = (lo + hi) >> 1;
-        if (peak_cell[mid] < cell) lo = mid + 1;
-        else hi = mid;
-    }
-    if (lo >= (long long)*npeak || peak_cell[lo] != cell) return;
-    atomicMax(&best[lo], ((unsigned long long)ord_f(densph[i]) << 32) | (unsigned long long)i);
-    if (i == 0) has0[lo] = 1u;
+    const long long a = (long long)rint(((double)x[i] - g.x0) / g.h) - in.i0;
+    const long long b = (long long)rint(((double)y[i] - g.y0) / g.h) - in.j0;
+    const long long c = (long long)rint(((double)z[i] - g.z0) / g.h) - in.k0;
+    if (a < 0 || a >= in.ni || b < 0 || b >= in.nj || c < 0 || c >= in.nk) return;
+    const long long t = a + (long long)in.ni * (b + (long long)in.nj * c);
+    if (!flag[t]) return;
+    const unsigned p = pos[t];
+    atomicMax(&best[p], ((unsigned long long)ord_f(densph[i]) << 32) | (unsigned long long)i);
+    if (i == 0) has0[p] = 1u;
 }
 
 // One workgroup of 1024: the peaks with a non-empty chain not holding
@@ -140,6 +138,32 @@ __global__ void __launch_bounds__(1024) core_compact_k(const unsigned *npeak,
         __syncthreads();
     }
     if (threadIdx.x == 0) *ncore = s_base;
+}
+
+// k <= 32: a block of 256 particles stages its 256 lists (low 32 bits of
+// each index; particle indices are below 2^31) in LDS with coalesced loads,
+// two list rows per wave instruction, rows padded to 33 entries.
+constexpr int kDenPeakRows = 256;
+__global__ void __launch_bounds__(kDenPeakRows)
+denpeak_flag_lds_k(const float *den, int k, const long long *nbr, long long np, const int *type,
+                   int jflag, double threshold, unsigned *flag) {
+    __shared__ int s_idx[kDenPeakRows * 33];
+    const long long base = (long long)blockIdx.x * kDenPeakRows;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int t = lane & 31;
+    for (int r = wave * 2 + (lane >> 5); r < kDenPeakRows; r += 2 * (kDenPeakRows / 64)) {
+        if (t < k && base + r < np) s_idx[r * 33 + t] = (int)nbr[(base + r) * k + t];
+    }
+    __syncthreads();
+    const long long i = base + threadIdx.x;
+    if (i >= np) return;
+    const float v = den[i];
+    bool peak = (double)v > threshold && (jflag != 1 || type[i] == PGALF_TYPE_STAR);
+    if (peak) {
+        const int *li = s_idx + threadIdx.x * 33;
+        for (int u = 0; u < k; u++) peak &= !(den[li[u]] > v);
+    }
+    flag[i] = peak ? 1u : 0u;
 }
 
 __global__ void denpeak_flag_k(const float *den, int k, const long long *nbr, long long np,
@@ -414,16 +438,15 @@ hipError_t grid_cores(const P *x, const P *y, const P *z, const int *type, long 
     unsigned *flag = ws.alloc_n<unsigned>(nint), *pos = ws.alloc_n<unsigned>(nint);
     unsigned *npeak = ws.alloc_n<unsigned>(1);
     const long long cap = nint;  // peak nodes: at most one per interior node
-    long long *peak_cell = ws.alloc_n<long long>(cap);
     unsigned long long *best = ws.alloc_n<unsigned long long>(cap);
     unsigned *has0 = ws.alloc_n<unsigned>(cap);
-    if (!flag || !pos || !npeak || !peak_cell || !best || !has0) return hipErrorOutOfMemory;
+    if (!flag || !pos || !npeak || !best || !has0) return hipErrorOutOfMemory;
     peak_flag_k<<<nblk(nint, 256), 256, 0, s>>>(den, in, threshold, flag);
     PGALF_HIP_TRY(exclusive_scan_u32(flag, pos, nint, npeak, ws, s));
-    peak_list_k<<<nblk(nint, 256), 256, 0, s>>>(flag, pos, in, peak_cell, best, has0);
+    peak_list_k<<<nblk(nint, 256), 256, 0, s>>>(flag, pos, in, best, has0);
     if (np > 0)
-        peak_member_k<P><<<nblk(np, 256), 256, 0, s>>>(x, y, z, type, np, ptype, densph, g,
-                                                       peak_cell, npeak, best, has0);
+        peak_member_k<P><<<nblk(np, 256), 256, 0, s>>>(x, y, z, type, np, ptype, densph, g, in,
+                                                       flag, pos, best, has0);
     core_compact_k<<<1, 1024, 0, s>>>(npeak, best, has0, core, maxcore, ncore);
     return hipGetLastError();
 }
@@ -435,7 +458,12 @@ hipError_t find_den_peak(const float *den, int k, const long long *nbr, long lon
     WsScope scope(ws);
     unsigned *flag = ws.alloc_n<unsigned>(np), *pos = ws.alloc_n<unsigned>(np);
     if (!flag || !pos) return hipErrorOutOfMemory;
-    denpeak_flag_k<<<nblk(np, 256), 256, 0, s>>>(den, k, nbr, np, type, jflag, threshold, flag);
+    if (k <= 32 && np < (1LL << 31))
+        denpeak_flag_lds_k<<<nblk(np, kDenPeakRows), kDenPeakRows, 0, s>>>(den, k, nbr, np, type,
+                                                                          jflag, threshold, flag);
+    else
+        denpeak_flag_k<<<nblk(np, 256), 256, 0, s>>>(den, k, nbr, np, type, jflag, threshold,
+                                                     flag);
     PGALF_HIP_TRY(exclusive_scan_u32(flag, pos, np, ncore, ws, s));
     denpeak_scatter_k<<<nblk(np, 256), 256, 0, s>>>(flag, pos, np, core, maxcore);
     return hipGetLastError();
