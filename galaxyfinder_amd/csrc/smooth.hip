@@ -629,6 +629,55 @@ smooth_direct_x_k(float *grid, DirectPass dp, Taps taps, const long long *goffs)
     }
 }
 
+// Whole cubic grids of side N <= 32 (N a multiple of 8), one workgroup per
+// grid: the padded grid (N * N * (N + 2) floats, 139 KB at N = 32) is loaded
+// into LDS once, the x, y and z passes run there, one line per thread, with
+// the same conv_line as the per-pass kernels (so every value is the same
+// bit for bit: the grid is FP32 between passes in both), and it is stored
+// once -- a third of the HBM traffic of three passes.
+template <int N>
+__global__ void __launch_bounds__(1024) smooth_cube_k(float *grid, Taps taps,
+                                                      const long long *goffs) {
+    constexpr int W = N + 2;
+    constexpr int CELLS = N * N * W;
+    static_assert(CELLS % 2 == 0, "float pairs");
+    __shared__ float c[CELLS];
+    float *g = grid + (goffs ? goffs[blockIdx.x] : 0LL);
+    // grid offsets and rows are even: 8-byte aligned float pairs
+    const float2 *g2 = reinterpret_cast<const float2 *>(g);
+    float2 *c2 = reinterpret_cast<float2 *>(c);
+    for (int e = threadIdx.x; e < CELLS / 2; e += 1024) c2[e] = g2[e];
+    __syncthreads();
+    for (int r = threadIdx.x; r < N * N; r += 1024) {  // x: rows (y, z)
+        float *p = c + r * W;
+        float v[N];
+#pragma unroll
+        for (int j = 0; j < N; j++) v[j] = p[j];
+        conv_line<N>(v, taps, p, 1);
+        p[N] = 0.0f;
+        p[N + 1] = 0.0f;
+    }
+    __syncthreads();
+    for (int l = threadIdx.x; l < N * N; l += 1024) {  // y: lines (x, z)
+        float *p = c + l % N + (l / N) * (N * W);
+        float v[N];
+#pragma unroll
+        for (int j = 0; j < N; j++) v[j] = p[j * W];
+        conv_line<N>(v, taps, p, W);
+    }
+    __syncthreads();
+    for (int l = threadIdx.x; l < N * N; l += 1024) {  // z: lines (x, y)
+        float *p = c + l % N + (l / N) * W;
+        float v[N];
+#pragma unroll
+        for (int j = 0; j < N; j++) v[j] = p[j * (N * W)];
+        conv_line<N>(v, taps, p, N * W);
+    }
+    __syncthreads();
+    float2 *o2 = reinterpret_cast<float2 *>(g);
+    for (int e = threadIdx.x; e < CELLS / 2; e += 1024) o2[e] = c2[e];
+}
+
 // ---- host planning ----------------------------------------------------------
 struct AxisPlan {
     int n = 0, L = 0;
@@ -730,6 +779,9 @@ hipError_t get_plan(int n, double expfac, AxisPlan &out) {
 // directly and 64 by the register FFT, which measured faster in the batch.
 #ifndef PGALF_SMOOTH_FFT2
 #define PGALF_SMOOTH_FFT2 1
+#endif
+#ifndef PGALF_SMOOTH_CUBE
+#define PGALF_SMOOTH_CUBE 1  // small cubes in one LDS-resident kernel
 #endif
 #ifndef PGALF_FFT2_NL256
 #define PGALF_FFT2_NL256 16  // complex lines per block at L = 256 (16: 128-byte rows)
@@ -904,6 +956,22 @@ hipError_t smooth_passes(float *den, int nx, int ny, int nz, double expfac, hipS
     PGALF_HIP_TRY(get_plan(nz, expfac, pz));
     const long long mx = 2LL * (nx / 2 + 1);
     const long long gc = mx * ny * nz;
+    // small cubes whose three passes are all direct: one LDS-resident kernel
+    if (PGALF_SMOOTH_CUBE && nx == ny && ny == nz && px.direct && nx <= 32 &&
+        nx <= smooth_direct_max()) {
+        for (int z0 = 0; z0 < ngrids; z0 += 1 << 30) {
+            const int cnt = ngrids - z0 < (1 << 30) ? ngrids - z0 : (1 << 30);
+            const long long *gz = goffs ? goffs + z0 : nullptr;
+            switch (nx) {
+            case 8: smooth_cube_k<8><<<cnt, 1024, 0, s>>>(den, px.taps, gz); break;
+            case 16: smooth_cube_k<16><<<cnt, 1024, 0, s>>>(den, px.taps, gz); break;
+            case 24: smooth_cube_k<24><<<cnt, 1024, 0, s>>>(den, px.taps, gz); break;
+            case 32: smooth_cube_k<32><<<cnt, 1024, 0, s>>>(den, px.taps, gz); break;
+            default: return hipErrorInvalidValue;
+            }
+        }
+        return hipGetLastError();
+    }
     // x: lines are rows (contiguous), batch over ny*nz rows
     PGALF_HIP_TRY(run_pass(den, px, 1, ny * nz, mx, 1, 0, 1, 1, (int)mx, gc, s, goffs, ngrids));
     // y: lines along y (stride mx), batch over x (stride 1), one z per block row

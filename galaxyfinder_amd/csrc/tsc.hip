@@ -7,7 +7,9 @@
 //   2. scan:    tile offsets; tiles are cut into work units of <= `unit`
 //               entries (the schedule's tsc_unit_entries) so a cusp tile is
 //               shared by many workgroups;
-//   3. scatter: particle ids into per-tile bins (ranges reserved per block);
+//   3. scatter: the particles (position and mass) into per-tile bins
+//               (ranges reserved per block), so the deposit reads its
+//               entries coalesced;
 //   4. deposit: one workgroup per unit accumulates its particles'
 //               contributions that land inside the tile in an LDS int64
 //               fixed-point tile (ds_add_u64).  Integer addition is
@@ -90,6 +92,22 @@ __device__ inline bool particle_axes(const P *x, const P *y, const P *z, const P
     // unchecked writes do (memory of the padded row); y/z must stay inside.
     return ax[0].nr >= 1 && ax[0].nr <= g.mx() - 2 && ax[1].nr >= 1 &&
            ax[1].nr <= g.ny - 2 && ax[2].nr >= 1 && ax[2].nr <= g.nz - 2;
+}
+
+// Bin entry: the particle itself (position and mass), so that the deposit
+// reads its entries with coalesced loads instead of gathering particles in
+// input order.
+template <class P>
+struct alignas(4 * sizeof(P)) BinRec {
+    P x, y, z, m;
+};
+
+template <class P>
+__device__ inline void rec_axes(const BinRec<P> &r, const Grid &g, Axis ax[3]) {
+    const double pm = (double)r.m;
+    deposit_axis(((double)r.x - g.x0) / g.h, pm, true, ax[0]);
+    deposit_axis(((double)r.y - g.y0) / g.h, pm, false, ax[1]);
+    deposit_axis(((double)r.z - g.z0) / g.h, pm, false, ax[2]);
 }
 
 __device__ inline bool selected(const int *type, long long i, int ptype) {
@@ -180,7 +198,7 @@ template <class P>
 __global__ void __launch_bounds__(kBlock)
 tsc_scatter_k(const P *x, const P *y, const P *z, const P *m, const int *type, int ptype,
               const HaloSeg *segs, HaloSeg seg0, const PChunk *chunks, const unsigned *tstart,
-              unsigned *tcursor, unsigned *bins, int use_lds) {
+              unsigned *tcursor, BinRec<P> *bins, int use_lds) {
     extern __shared__ unsigned sh[];
     const PChunk c = chunk_at(chunks, seg0, kPerBlock);
     const HaloSeg sg = seg_at(segs, seg0, c.halo);
@@ -217,6 +235,11 @@ tsc_scatter_k(const P *x, const P *y, const P *z, const P *m, const int *type, i
         if (!selected(type, i, ptype)) continue;
         Axis ax[3];
         if (!particle_axes(x, y, z, m, i, g, ax)) continue;
+        BinRec<P> rec;
+        rec.x = x[i];
+        rec.y = y[i];
+        rec.z = z[i];
+        rec.m = m[i];
         int tx0, tx1, ty0, ty1, tz0, tz1;
         tile_span(ax[0].nr, tx0, tx1);
         tile_span(ax[1].nr, ty0, ty1);
@@ -228,7 +251,7 @@ tsc_scatter_k(const P *x, const P *y, const P *z, const P *m, const int *type, i
                     unsigned slot;
                     if (use_lds) slot = base[t] + atomicAdd(&hist[t], 1u);
                     else slot = tstart[sg.toff + t] + atomicAdd(&tcursor[sg.toff + t], 1u);
-                    bins[slot] = (unsigned)i;
+                    bins[slot] = rec;
                 }
     }
 }
@@ -278,7 +301,7 @@ tsc_deposit_k(const P *x, const P *y, const P *z, const P *m, const HaloSeg *seg
               HaloSeg seg0, const unsigned *tile_halo, const unsigned *tstart,
               const unsigned *tcount, const unsigned *ustart, const unsigned *nunits,
               const unsigned *unit_tile, const unsigned *total_units,
-              const unsigned *multi_slot, const unsigned *bins, const double *scale,
+              const unsigned *multi_slot, const BinRec<P> *bins, const double *scale,
               const unsigned *tslot, float *tbuf, unsigned long long *side, unsigned unit) {
     __shared__ unsigned long long acc[kTileNodes];
     const unsigned nunit_total = *total_units;
@@ -301,9 +324,9 @@ tsc_deposit_k(const P *x, const P *y, const P *z, const P *m, const HaloSeg *seg
         unsigned e0 = tstart[t] + chunk * unit;
         unsigned e1 = min(e0 + unit, tstart[t] + tcount[t]);
         for (unsigned e = e0 + threadIdx.x; e < e1; e += kBlock) {
-            long long i = bins[e];
+            const BinRec<P> rec = bins[e];
             Axis ax[3];
-            particle_axes(x, y, z, m, i, g, ax);
+            rec_axes(rec, g, ax);
 #pragma unroll
             for (int c = -1; c <= 1; c++) {
                 long long nzn = ax[2].nr + c - oz;
@@ -792,7 +815,7 @@ hipError_t deposit_impl(const P *x, const P *y, const P *z, const P *m, const in
             use_lds);
     PGALF_HIP_TRY(exclusive_scan_u32(tcount, tstart, ntiles, &misc[1], ws, s));
     // the bin list can hold at most 8 entries per particle
-    unsigned *bins = ws.alloc_n<unsigned>((size_t)np * 8 + 1);
+    BinRec<P> *bins = ws.alloc_n<BinRec<P>>((size_t)np * 8 + 1);
     if (!bins) return hipErrorOutOfMemory;
     if (nchunks > 0)
         tsc_scatter_k<P><<<dim3((unsigned)nchunks), dim3(kBlock), 2 * lds_count, s>>>(
