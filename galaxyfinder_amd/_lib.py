@@ -32,7 +32,8 @@ class pgalf_schedule_t(ctypes.Structure):
                 ("tsc_unit_entries", _c_i)]
 
 
-# name -> (restype, argtypes); every symbol of include/pgalf_density.h
+# name -> (restype, argtypes); every symbol of include/pgalf_density.h and
+# include/pgalf_catalog.h
 SIGNATURES = {
     "pgalf_last_error": (_c_i, []),
     "pgalf_last_error_string": (ctypes.c_char_p, []),
@@ -89,6 +90,15 @@ SIGNATURES = {
     "pgalf_set_schedule": (_c_i, [ctypes.POINTER(pgalf_schedule_t), _c_i, _c_i]),
     "pgalf_get_schedule": (_c_i, [_c_ll, ctypes.POINTER(pgalf_schedule_t),
                                   ctypes.POINTER(_c_i)]),
+    "pgalf_catalog_open": (_c_p, [ctypes.c_char_p, ctypes.c_char_p, _c_i]),
+    "pgalf_catalog_header": (_c_i, [_c_p, _c_p]),
+    "pgalf_catalog_next": (_c_ll, [_c_p, _c_ll, _c_ll, _c_i, ctypes.POINTER(_c_ll)]),
+    "pgalf_catalog_fetch": (_c_i, [_c_p] + [_c_p] * 14),
+    "pgalf_catalog_records": (_c_p, [_c_p, ctypes.POINTER(_c_ll)]),
+    "pgalf_catalog_close": (None, [_c_p]),
+    "pgalf_writer_open": (_c_p, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]),
+    "pgalf_writer_halo": (_c_i, [_c_p, _c_i, _c_p, _c_p, _c_p, _c_i]),
+    "pgalf_writer_close": (_c_i, [_c_p]),
     "pgalf_timing_enable": (None, [_c_i]),
     "pgalf_timing_reset": (None, []),
     "pgalf_timing_get": (_c_i, [ctypes.c_char_p, ctypes.POINTER(_c_d), ctypes.POINTER(_c_i)]),

@@ -220,6 +220,24 @@ hipError_t unpack_particles(const void *d_bp, long long np, double *x, double *y
     return hipGetLastError();
 }
 
+// Grid box of nnost.c:1040-1056: min/max of all particles, 1e-4 extent
+// buffer on the max side, NCELLBUFF = 10 cells, origin shifted by 5 cells.
+void lag_grid_box(const double *x, const double *y, const double *z, long long n,
+                  long long stride, int nxyz[3], double xyzmin[3]) {
+    const int nbuff = 10;  // NCELLBUFF (params.h:115)
+    const double *c[3] = {x, y, z};
+    for (int d = 0; d < 3; d++) {
+        double lo = 1e20, hi = -1e20;
+        for (long long i = 0; i < n; i++) {
+            lo = fmin(lo, c[d][i * stride]);
+            hi = fmax(hi, c[d][i * stride]);
+        }
+        hi = hi + (hi - lo) * 0.0001;
+        nxyz[d] = (int)((hi - lo) / kLagCellSize + nbuff);
+        xyzmin[d] = lo - kLagCellSize * nbuff / 2.;
+    }
+}
+
 }  // namespace pgalf
 
 using namespace pgalf;
@@ -527,25 +545,14 @@ extern "C" int pgalf_lag_density_stage(pgalf_particle_t *bp, int np, int Numnear
         return set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
     if (nearindex && Numnear > np)
         return set_err(PGALF_ERR_SHORT_LIST, "%s: Numnear %d > np %d", where, Numnear, np);
-    // Grid box of nnost.c:1040-1056: min/max of all particles, 1e-4 extent
-    // buffer on the max side, NCELLBUFF = 10 cells, origin shifted by 5 cells.
-    const double cellsize = 0.004;  // TSC_CELL_SIZE (params.h:92)
-    const float RG = 0.008f;        // Gaussian_Smoothing_Length (params.h:104)
-    const int nbuff = 10;           // NCELLBUFF (params.h:115)
-    double xmin = 1e20, ymin = 1e20, zmin = 1e20, xmax = -1e20, ymax = -1e20, zmax = -1e20;
-    for (int i = 0; i < np; i++) {
-        xmin = fmin(xmin, bp[i].x); ymin = fmin(ymin, bp[i].y); zmin = fmin(zmin, bp[i].z);
-        xmax = fmax(xmax, bp[i].x); ymax = fmax(ymax, bp[i].y); zmax = fmax(zmax, bp[i].z);
-    }
-    xmax = xmax + (xmax - xmin) * 0.0001;
-    ymax = ymax + (ymax - ymin) * 0.0001;
-    zmax = zmax + (zmax - zmin) * 0.0001;
-    int nx = (int)((xmax - xmin) / cellsize + nbuff);
-    int ny = (int)((ymax - ymin) / cellsize + nbuff);
-    int nz = (int)((zmax - zmin) / cellsize + nbuff);
-    xmin -= cellsize * nbuff / 2.;
-    ymin -= cellsize * nbuff / 2.;
-    zmin -= cellsize * nbuff / 2.;
+    const double cellsize = kLagCellSize;
+    const float RG = 0.008f;  // Gaussian_Smoothing_Length (params.h:104)
+    constexpr long long kStride = sizeof(pgalf_particle_t) / sizeof(double);
+    int nn[3];
+    double lo[3];
+    lag_grid_box(&bp[0].x, &bp[0].y, &bp[0].z, np, kStride, nn, lo);
+    const int nx = nn[0], ny = nn[1], nz = nn[2];
+    const double xmin = lo[0], ymin = lo[1], zmin = lo[2];
     if (!check_grid(nx, ny, nz)) return set_err(PGALF_ERR_ARG, "%s: grid too large", where);
     hipStream_t s = nullptr;
     Workspace &ws = workspace_for_current_device();

@@ -349,6 +349,13 @@ hipError_t merging_peak(const P *x, const P *y, const P *z, long long np, const 
                         const float *key, int n, double link, int *keep, unsigned *nkeep,
                         int *order_out, Workspace &ws, hipStream_t s);
 
+// ---- lagFindStellarCore's grid (capi.hip) ---------------------------------
+constexpr double kLagCellSize = 0.004;  // TSC_CELL_SIZE (params.h:92)
+// The TSC grid nnost.c:1040-1058 sizes for n particles (coordinates
+// x[i * stride], ...): node counts and origin; the cell size is kLagCellSize.
+void lag_grid_box(const double *x, const double *y, const double *z, long long n,
+                  long long stride, int nxyz[3], double xyzmin[3]);
+
 // ---- phase timing (capi.hip) ---------------------------------------------
 void timing_mark(const char *phase, hipStream_t s, bool end);
 // Phase timing is suspended on this thread while a graph is being captured.

@@ -36,6 +36,18 @@ With the argument ``merge``:
                   densities), "chain" (cores on lines spaced just under and
                   just over the link length, and coincident cores).
 
+With the argument ``catalog``:
+  catalog_small.npz  the reference's record layouts (sizeof / offsetof of
+                  HaloQ, DmType, GasType, SinkType, StarType, HaloInfo,
+                  SubInfo); for the synthetic catalogue of catalog_gen.py
+                  (written again by the tests) the reference's FREAD +
+                  Dump2BP + mklocalize output of every halo (localised
+                  positions, masses, types, link02, Dump2BP origins), and
+                  the files its write_data produces with catalog_gen's
+                  seeded memberships (SHA-256 and size of GALFIND.DATA,
+                  GALCATALOG.LIST and the background file, and the
+                  GALCATALOG.LIST bytes).
+
 With the argument ``config``, fixtures at the sizes of the two one-halo
 workloads instead (their particles are regenerated by
 galaxyfinder_amd.halos.single_halo_config; a CRC of the float32 positions is
@@ -55,6 +67,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+sys.path.insert(0, HERE)
 
 from oracle import oracle as O  # noqa: E402
 from galaxyfinder_amd import halos  # noqa: E402
@@ -219,6 +232,46 @@ def merge_fixture(R):
     np.savez_compressed(os.path.join(HERE, "merge_small.npz"), **out)
 
 
+def catalog_fixture(R):
+    import ctypes
+    import hashlib
+    import tempfile
+    import catalog_gen as C
+    lay = np.zeros(22, dtype=np.int64)
+    R.ref_catalog_layouts(lay.ctypes.data_as(ctypes.c_void_p))
+    d = tempfile.mkdtemp()
+    h, m = C.write(d)
+    cap = 1 << 16
+    x, y, z, ms = (np.zeros(cap) for _ in range(4))
+    ty = np.zeros(cap, dtype=np.int32)
+    l2 = np.zeros(cap, dtype=np.float32)
+    off = np.zeros(64, dtype=np.int64)
+    org = np.zeros(3 * 64)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    R.ref_read_catalog.restype = ctypes.c_longlong
+    nh = R.ref_read_catalog(h.encode(), m.encode(), 63, ctypes.c_longlong(cap), p(x), p(y),
+                            p(z), p(ms), p(ty), p(l2), p(off), p(org))
+    n = int(off[nh])
+    p2all, mp = [], []
+    for i in range(nh):
+        k, p2 = C.membership(int(off[i + 1] - off[i]))
+        mp.append(k)
+        p2all.append(p2)
+    p2all = np.concatenate(p2all).astype(np.int32)
+    mp = np.array(mp, dtype=np.int32)
+    outs = [os.path.join(d, f) for f in ("GALFIND.DATA", "GALCATALOG.LIST", "background")]
+    R.ref_write_catalog.restype = ctypes.c_longlong
+    R.ref_write_catalog(h.encode(), m.encode(), p(p2all), p(mp), *(f.encode() for f in outs))
+    data = [open(f, "rb").read() for f in outs]
+    np.savez_compressed(
+        os.path.join(HERE, "catalog_small.npz"), layouts=lay, x=x[:n], y=y[:n], z=z[:n],
+        mass=ms[:n], type=ty[:n], link02=l2[:n], offsets=off[:nh + 1],
+        origin=org[:3 * nh].reshape(nh, 3), mpeak=mp,
+        sha256=np.array([hashlib.sha256(b).hexdigest() for b in data]),
+        nbytes=np.array([len(b) for b in data], dtype=np.int64),
+        catalog_list=np.frombuffer(data[1], dtype=np.uint8))
+
+
 def main():
     O.build(ref=True)
     R = O.ref_lib()
@@ -232,6 +285,10 @@ def main():
     if len(sys.argv) > 1 and sys.argv[1] == "merge":
         merge_fixture(R)
         print("merge_small.npz", os.path.getsize(os.path.join(HERE, "merge_small.npz")))
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "catalog":
+        catalog_fixture(R)
+        print("catalog_small.npz", os.path.getsize(os.path.join(HERE, "catalog_small.npz")))
         return
     if len(sys.argv) > 1 and sys.argv[1] == "config":
         config_fixtures(R)

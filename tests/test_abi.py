@@ -15,11 +15,11 @@ sys.path.insert(0, ROOT)
 
 from galaxyfinder_amd import _lib, density as D, halos  # noqa: E402
 
-HEADER = os.path.join(ROOT, "include", "pgalf_density.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("pgalf_density.h", "pgalf_catalog.h")]
 
 
 def declared_functions():
-    text = open(HEADER).read()
+    text = "".join(open(h).read() for h in HEADERS)
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text)
     skip = {"if", "sizeof", "defined"}
