@@ -253,6 +253,117 @@ def peaks_bench(x, y, z, g, den, rho_t, nbr, rho_k, spec, reps=10, cpu=True):
     return out
 
 
+def catalog_bench(nhalo=2000, seed=3, cpu=True):
+    """The catalogue path on a synthetic FoF catalogue written to a scratch
+    directory: nhalo halos of halos.catalogue_vec (dN/dn ~ n^-2 on
+    [100, 65536]) of DM (1e8 Msun/h) and one star in five (1e6 Msun/h), each
+    scaled to 0.2 cMpc/h across in a 100 cMpc/h box.  Times, on the host,
+    the loader (read, Dump2BP, mklocalize, grid sizing) and the writer
+    (every particle in subhalo 0); on the GPU, the batched density stage over
+    the loaded catalogue with its lagFindStellarCore grids.  CPU: the
+    reference's own FREAD + Dump2BP + mklocalize and write_data (oracle/_ref)
+    on the same files."""
+    import ctypes
+    import shutil
+    import tempfile
+    import torch
+    from galaxyfinder_amd import catalog as CAT, density as D, halos
+    pos, mass, off, _, _ = halos.catalogue_vec(seed, nhalo=nhalo, alpha=2.0, nmin=100,
+                                                nmax=65536)
+    rng = np.random.default_rng(seed)
+    d = tempfile.mkdtemp(prefix="pgalf_cat_")
+    try:
+        centres = rng.uniform(5.0, 95.0, (nhalo, 3))
+        recs = []
+        for h in range(nhalo):
+            a, e = int(off[h]), int(off[h + 1])
+            r = np.zeros(e - a, dtype=CAT.PM_DTYPE)
+            p = (pos[a:e].astype(np.float64) - 0.5) * 0.2 + centres[h]
+            r["x"], r["y"], r["z"] = p[:, 0], p[:, 1], p[:, 2]
+            r["mass"] = 1.0e8
+            star = np.arange(e - a) % 5 == 0
+            r["mass"][star] = 1.0e6
+            recs.append((r[~star], r[:0], np.zeros(0, dtype=np.uint8), r[star]))
+        hp, mp = os.path.join(d, "FoF_halo_cat.00001"), os.path.join(d, "FoF_member_particle.00001")
+        CAT.write_fof_catalog(hp, mp, [100.0, 0.7, 0.3, 0.048, 0.7, 1.0, 1.0], recs)
+        mbytes = os.path.getsize(mp)
+        t0 = time.perf_counter()
+        cat = CAT.Catalog(hp, mp)
+        b = cat.next_batch(copy_records=False)
+        t_load = time.perf_counter() - t0
+        n = int(b["offsets"][-1])
+        grids = CAT.Catalog.grid_tuples(b)
+        dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+        x, y, z = (dev(b[k].astype(np.float32)) for k in "xyz")
+        m = dev(b["mass"].astype(np.float32))
+        bt = D.Batch(b["offsets"], grids, CAT.GAUSSIAN_SMOOTHING_LENGTH, 32)
+        den = torch.empty(bt.grid_floats, dtype=torch.float32, device="cuda")
+        rt = torch.empty(n, dtype=torch.float32, device="cuda")
+        nbr = torch.empty((n, 32), dtype=torch.int64, device="cuda")
+        rk = torch.empty(n, dtype=torch.float32, device="cuda")
+        bt.run(x, y, z, m, den, rt, nbr, rk)
+        bt.status()
+        reps = 3
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            bt.run(x, y, z, m, den, rt, nbr, rk)
+        torch.cuda.synchronize()
+        t_gpu = (time.perf_counter() - t0) / reps
+        bt.status()
+        bt.close()
+        outs = [os.path.join(d, f) for f in ("GALFIND.DATA", "GALCATALOG.LIST", "background")]
+        t0 = time.perf_counter()
+        w = CAT.Writer(*outs)
+        offs, roffs = b["offsets"], b["record_offsets"]
+        for h in range(len(offs) - 1):
+            a, e = offs[h], offs[h + 1]
+            w.halo(b["type"][a:e], b["records"][roffs[h]:roffs[h + 1]],
+                   np.zeros(e - a, dtype=np.int32), 1)
+        w.close()
+        t_write = time.perf_counter() - t0
+        cat.close()
+        out = {"workload": "%d halos, %d particles (dN/dn ~ n^-2 on [100, 65536]), member "
+                           "file %.1f MB, lagFindStellarCore grids, RG = 0.008, k = 32"
+                           % (nhalo, n, mbytes / 1e6),
+               "halos": len(offs) - 1, "particles": n,
+               "grid_cells": int(sum(g[0] * g[1] * g[2] for g in grids)),
+               "load_s": round(t_load, 4), "load_mb_per_s": round(mbytes / 1e6 / t_load, 1),
+               "gpu_stage_ms": round(t_gpu * 1e3, 3),
+               "gpu_particles_per_s": round(n / t_gpu, 1),
+               "write_s": round(t_write, 4)}
+        if cpu:
+            from oracle import oracle as O
+            R = O.ref_lib()
+            if R is not None:
+                R.ref_init(8 << 30)
+                xs, ys, zs, ms = (np.zeros(n) for _ in range(4))
+                ty = np.zeros(n, dtype=np.int32)
+                l2 = np.zeros(n, dtype=np.float32)
+                ro = np.zeros(nhalo + 2, dtype=np.int64)
+                org = np.zeros(3 * (nhalo + 2))
+                pp = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+                R.ref_read_catalog.restype = ctypes.c_longlong
+                t0 = time.perf_counter()
+                R.ref_read_catalog(hp.encode(), mp.encode(), nhalo + 1, ctypes.c_longlong(n),
+                                   pp(xs), pp(ys), pp(zs), pp(ms), pp(ty), pp(l2), pp(ro),
+                                   pp(org))
+                out["load_reference_s"] = round(time.perf_counter() - t0, 4)
+                p2 = np.zeros(n, dtype=np.int32)
+                mpk = np.ones(nhalo, dtype=np.int32)
+                refo = [f + ".ref" for f in outs]
+                R.ref_write_catalog.restype = ctypes.c_longlong
+                t0 = time.perf_counter()
+                R.ref_write_catalog(hp.encode(), mp.encode(), pp(p2), pp(mpk),
+                                    *(f.encode() for f in refo))
+                out["write_reference_s"] = round(time.perf_counter() - t0, 4)
+                out["files_identical_to_reference"] = all(
+                    open(a_, "rb").read() == open(b_, "rb").read() for a_, b_ in zip(outs, refo))
+        return out
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def batch_bench(steps, warmup, plain_pass=True):
     """Batched forward over the 10000-halo catalogue (dN/dn ~ n^-2 on
     [100, 1048576], grids 32^3-256^3 from n, RG = 2 cells, k = 32): eager
@@ -507,6 +618,10 @@ def main():
                     help="skip the batched 10000-halo catalogue measurement")
     ap.add_argument("--batch-only", action="store_true",
                     help="only the batched catalogue measurement (no per-halo pass)")
+    ap.add_argument("--catalog-only", action="store_true",
+                    help="only the FoF catalogue measurement")
+    ap.add_argument("--no-catalog", action="store_true",
+                    help="skip the FoF catalogue load / batch / write measurement")
     ap.add_argument("--no-peaks", action="store_true",
                     help="skip the integer peak stage measurement")
     ap.add_argument("--no-grad", action="store_true",
@@ -538,6 +653,9 @@ def main():
         from galaxyfinder_amd import tune
         tune.load(sched_path)
         print("[bench] launch schedule %s" % sched_path, file=sys.stderr, flush=True)
+    if args.catalog_only:
+        print(json.dumps({"catalog": catalog_bench(cpu=not args.no_cpu_baseline)}), flush=True)
+        return
     if args.c5:
         print(json.dumps({"c5": c5_bench(args.steps, args.warmup)}), flush=True)
         return
@@ -643,6 +761,8 @@ def main():
     if not args.no_peaks:
         out["peaks"] = peaks_bench(x, y, z, g, den, rho_t, nbr, rho_k, spec,
                                    cpu=not args.no_cpu_baseline)
+    if not args.no_catalog:
+        out["catalog"] = catalog_bench(cpu=not args.no_cpu_baseline)
     if not args.no_batch:
         out["batch"] = batch_bench(max(3, args.steps // 2), 2)
     if not args.no_grad:

@@ -27,8 +27,41 @@ GRID_DTYPE = np.dtype({"names": ["nx", "ny", "nz", "xmin", "ymin", "zmin", "cell
                        "offsets": [0, 4, 8, 16, 24, 32, 40], "itemsize": 48})
 
 
+PM_DTYPE = np.dtype({"names": ["x", "y", "z", "vx", "vy", "vz", "mass", "id", "levelp",
+                               "family", "tag", "tp", "zp", "chem", "mass0", "birth_d",
+                               "partp"],
+                     "formats": ["<f8"] * 7 + ["<i4", "<i4", "i1", "i1", "<f8", "<f8",
+                                               ("<f8", 3), "<f8", "<f8", "<i4"],
+                     "offsets": [0, 8, 16, 24, 32, 40, 48, 56, 60, 64, 65, 72, 80, 88, 112,
+                                 120, 128],
+                     "itemsize": 136})  # DmType = StarType (ramses.h:66-92)
+
+
 def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def write_fof_catalog(halo_cat, members, header, halos):
+    """Writes a catalogue in the FoF format the loader reads: header (7
+    floats: size, hubble, omep, omepb, omeplam, amax, anow) and, per halo,
+    a HaloQ record and its records.  halos: list of (dm, gas, sink, star)
+    structured arrays (PM_DTYPE for dm and star; gas and sink of the
+    reference's GasType / SinkType layouts, or empty)."""
+    hq = np.zeros(len(halos), dtype=HALOQ_DTYPE)
+    for h, (dm, gas, sink, star) in enumerate(halos):
+        n = len(dm) + len(gas) + len(sink) + len(star)
+        hq[h]["np"], hq[h]["npstar"], hq[h]["npgas"] = n, len(star), len(gas)
+        hq[h]["npdm"], hq[h]["npsink"] = len(dm), len(sink)
+        if len(dm):
+            hq[h]["x"], hq[h]["y"], hq[h]["z"] = dm["x"][0], dm["y"][0], dm["z"][0]
+    with open(halo_cat, "wb") as f:
+        f.write(np.asarray(header, dtype="<f4").tobytes())
+        f.write(hq.tobytes())
+    with open(members, "wb") as f:
+        for recs in halos:
+            for r in recs:
+                if len(r):
+                    f.write(r.tobytes())
 
 
 class Catalog:
@@ -44,12 +77,14 @@ class Catalog:
         self.header = dict(zip(("size", "hubble", "omep", "omepb", "omeplam", "amax", "anow"),
                                list(h)))
 
-    def next_batch(self, max_halos=1 << 30, max_particles=1 << 40, min_np=MIN_HALO_NP):
+    def next_batch(self, max_halos=1 << 30, max_particles=1 << 40, min_np=MIN_HALO_NP,
+                   copy_records=True):
         """The next batch of halos as a dict of numpy arrays (None at the
         end of the catalogue): x, y, z (localised, FP64), mass, type, link02,
         vx, vy, vz per particle; offsets, record_offsets (bytes), haloq,
         origin (nhalo x 3), grids (pgalf_grid_t fields) per halo; records
-        (the member-file bytes of the batch)."""
+        (the member-file bytes of the batch; with copy_records=False a view
+        of the library's buffer, valid until the next call or close)."""
         npart = ctypes.c_longlong()
         nh = self._lib.pgalf_catalog_next(self._c, int(max_halos), int(max_particles),
                                           int(min_np), ctypes.byref(npart))
@@ -72,8 +107,11 @@ class Catalog:
             _p(out["record_offsets"]), _p(out["haloq"]), _p(out["origin"]), _p(out["grids"])))
         nb = ctypes.c_longlong()
         ptr = self._lib.pgalf_catalog_records(self._c, ctypes.byref(nb))
-        out["records"] = (np.ctypeslib.as_array((ctypes.c_ubyte * nb.value).from_address(ptr))
-                          .copy() if nb.value else np.zeros(0, dtype=np.uint8))
+        if nb.value:
+            view = np.ctypeslib.as_array((ctypes.c_ubyte * nb.value).from_address(ptr))
+            out["records"] = view.copy() if copy_records else view
+        else:
+            out["records"] = np.zeros(0, dtype=np.uint8)
         return out
 
     @staticmethod

@@ -41,6 +41,7 @@ struct pgalf_batch {
     unsigned *d_slab_tile = nullptr;
     long long *d_goffs = nullptr;  // smoothing groups' grid offsets, group after group
     std::vector<SmoothGroup> groups;
+    SmoothBatchPlan *splan = nullptr;
     KnnBatchPlan kplan;
     Workspace *ws = nullptr;
     // graph of the launch sequence for one set of pointers
@@ -64,6 +65,7 @@ void free_batch(pgalf_batch *b) {
     if (b->d_tile_halo) (void)hipFree(b->d_tile_halo);
     if (b->d_slab_tile) (void)hipFree(b->d_slab_tile);
     if (b->d_goffs) (void)hipFree(b->d_goffs);
+    smooth_batch_plan_free(b->splan);
     knn_batch_plan_free(b->kplan);
     delete b->ws;
     delete b;
@@ -93,7 +95,8 @@ hipError_t run_sequence(pgalf_batch *b, const float *x, const float *y, const fl
     PGALF_HIP_TRY(tsc_deposit_batch<float>(x, y, z, m, b->bd, den, ws, s));
     timing_mark("batch_tsc", s, true);
     timing_mark("batch_smooth", s, false);
-    PGALF_HIP_TRY(gaussian_smooth_batch(den, b->groups.data(), (int)b->groups.size(), s));
+    PGALF_HIP_TRY(
+        gaussian_smooth_batch(den, b->groups.data(), (int)b->groups.size(), b->splan, s));
     timing_mark("batch_smooth", s, true);
     // with a kNN to run, the read-back goes over its sorted slots (coherent
     // gathers, the same FP64 coordinates) and is timed inside "batch_knn"
@@ -157,7 +160,8 @@ hipError_t run_grad_sequence(pgalf_batch *b, const GradArgs &a, hipStream_t s) {
         float *A = ws.alloc_n<float>(b->goff[b->nhalo]);
         if (!A) return hipErrorOutOfMemory;
         PGALF_HIP_TRY(tsc_deposit_batch<float>(a.x, a.y, a.z, g_t, b->bd, A, ws, s));
-        PGALF_HIP_TRY(gaussian_smooth_batch(A, b->groups.data(), (int)b->groups.size(), s));
+        PGALF_HIP_TRY(
+            gaussian_smooth_batch(A, b->groups.data(), (int)b->groups.size(), b->splan, s));
         PGALF_HIP_TRY(tsc_grad_readback_batch<float>(a.x, a.y, a.z, a.m, g_t, b->bd, a.den, A,
                                                      acc, ws, s));
     }
@@ -326,6 +330,7 @@ extern "C" pgalf_batch_t *pgalf_batch_create(int nhalo, const long long *off,
         grp.expfac = std::get<3>(kv.first);
         grp.count = (int)kv.second.size();
         grp.goffs = nullptr;
+        grp.hgoffs = nullptr;
         gfirst.push_back(goffs.size());
         goffs.insert(goffs.end(), kv.second.begin(), kv.second.end());
         b->groups.push_back(grp);
@@ -339,8 +344,12 @@ extern "C" pgalf_batch_t *pgalf_batch_create(int nhalo, const long long *off,
     if (e == hipSuccess) e = upload(&b->d_goffs, goffs);
     for (size_t i = 0; i < b->groups.size(); i++) {
         b->groups[i].goffs = b->d_goffs + gfirst[i];
+        b->groups[i].hgoffs = goffs.data() + gfirst[i];
         if (e == hipSuccess) e = smooth_group_prepare(b->groups[i]);
     }
+    if (e == hipSuccess) b->splan = smooth_batch_plan_create(b->groups.data(),
+                                                             (int)b->groups.size(), e);
+    for (auto &grp : b->groups) grp.hgoffs = nullptr;
     if (e == hipSuccess) e = knn_batch_plan(off, nhalo, k, b->kplan);
     if (e != hipSuccess) {
         hip_err(e, where);

@@ -29,6 +29,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
+#include <new>
 #include <thread>
 #include <vector>
 
@@ -92,7 +94,8 @@ struct pgalf_catalog {
     // the last batch
     std::vector<pgalf_haloq_t> haloq;
     std::vector<long long> off, roff;  // particle and record-byte offsets per halo
-    std::vector<unsigned char> rec;
+    std::unique_ptr<unsigned char[]> rec;  // records of the batch (not zero-filled)
+    size_t rec_bytes = 0, rec_cap = 0;
     std::vector<double> x, y, z, m, vx, vy, vz, origin;
     std::vector<int> type;
     std::vector<float> link02;
@@ -203,11 +206,13 @@ extern "C" long long pgalf_catalog_next(pgalf_catalog_t *c, long long max_halos,
     c->haloq.clear();
     c->off.assign(1, 0);
     c->roff.assign(1, 0);
-    c->rec.clear();
     long long tot = 0;
     size_t rbytes = 0;
-    // choose the batch's halos: read HaloQ records, skip the members of
-    // halos below min_np (gfind.c:388 reads them and passes them by)
+    // choose the batch's halos from their HaloQ records; a halo below
+    // min_np is passed by (gfind.c:388 reads it and hands it to no one):
+    // skip[h] bytes of the member file lie before chosen halo h
+    std::vector<size_t> skip;
+    size_t gap = 0;
     while ((long long)c->haloq.size() < max_halos) {
         pgalf_haloq_t q;
         if (c->have_pending) {
@@ -223,10 +228,7 @@ extern "C" long long pgalf_catalog_next(pgalf_catalog_t *c, long long max_halos,
             return -1;
         }
         if ((long long)q.np < min_np) {
-            if (std::fseek(c->mf, (long)qb, SEEK_CUR) != 0) {
-                set_err(PGALF_ERR_ARG, "%s: member file shorter than the catalogue", where);
-                return -1;
-            }
+            gap += qb;
             continue;
         }
         if (!c->haloq.empty() && tot + (long long)q.np > max_particles) {
@@ -234,16 +236,42 @@ extern "C" long long pgalf_catalog_next(pgalf_catalog_t *c, long long max_halos,
             c->have_pending = true;
             break;
         }
-        c->rec.resize(rbytes + qb);
-        if (qb && std::fread(c->rec.data() + rbytes, 1, qb, c->mf) != qb) {
-            set_err(PGALF_ERR_ARG, "%s: member file shorter than the catalogue", where);
-            return -1;
-        }
         c->haloq.push_back(q);
+        skip.push_back(gap);
+        gap = 0;
         tot += (long long)q.np;
         rbytes += qb;
         c->off.push_back(tot);
         c->roff.push_back((long long)rbytes);
+    }
+    // the chosen halos' records: one read per run of consecutive halos
+    if (rbytes > c->rec_cap) {
+        c->rec.reset(new (std::nothrow) unsigned char[rbytes]);
+        c->rec_cap = c->rec ? rbytes : 0;
+        if (!c->rec) {
+            set_err(PGALF_ERR_NOMEM, "%s: host allocation failed", where);
+            return -1;
+        }
+    }
+    c->rec_bytes = rbytes;
+    for (size_t h = 0; h < c->haloq.size();) {
+        if (skip[h] && std::fseek(c->mf, (long)skip[h], SEEK_CUR) != 0) {
+            set_err(PGALF_ERR_ARG, "%s: member file shorter than the catalogue", where);
+            return -1;
+        }
+        size_t e = h + 1;
+        while (e < c->haloq.size() && skip[e] == 0) e++;
+        const size_t a = (size_t)c->roff[h], n = (size_t)c->roff[e] - a;
+        if (n && std::fread(c->rec.get() + a, 1, n, c->mf) != n) {
+            set_err(PGALF_ERR_ARG, "%s: member file shorter than the catalogue", where);
+            return -1;
+        }
+        h = e;
+    }
+    // the records of small halos after the last chosen one
+    if (gap && std::fseek(c->mf, (long)gap, SEEK_CUR) != 0) {
+        set_err(PGALF_ERR_ARG, "%s: member file shorter than the catalogue", where);
+        return -1;
     }
     const long long nh = (long long)c->haloq.size();
     for (auto *v : {&c->x, &c->y, &c->z, &c->m, &c->vx, &c->vy, &c->vz}) v->resize(tot);
@@ -255,7 +283,7 @@ extern "C" long long pgalf_catalog_next(pgalf_catalog_t *c, long long max_halos,
     auto unpack = [&](long long h0, long long h1) {
         for (long long h = h0; h < h1; h++) {
             const pgalf_haloq_t &q = c->haloq[h];
-            const unsigned char *r = c->rec.data() + c->roff[h];
+            const unsigned char *r = c->rec.get() + c->roff[h];
             long long p = c->off[h];
             const struct {
                 unsigned long long n;
@@ -334,8 +362,8 @@ extern "C" int pgalf_catalog_fetch(const pgalf_catalog_t *c, double *x, double *
 
 extern "C" const void *pgalf_catalog_records(const pgalf_catalog_t *c, long long *bytes) {
     if (!c) return nullptr;
-    if (bytes) *bytes = (long long)c->rec.size();
-    return c->rec.data();
+    if (bytes) *bytes = (long long)c->rec_bytes;
+    return c->rec.get();
 }
 
 extern "C" void pgalf_catalog_close(pgalf_catalog_t *c) {

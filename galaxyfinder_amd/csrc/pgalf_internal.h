@@ -251,13 +251,20 @@ double smooth_expfac(float RG, double cellsize);
 struct SmoothGroup {
     int nx, ny, nz;
     double expfac;
-    const long long *goffs;  // device [count]: first float of each grid
+    const long long *goffs;   // device [count]: first float of each grid
+    const long long *hgoffs;  // host copy (read while a plan is made)
     int count;
 };
 // Builds the group's FFT plans (host work and uploads; call before capture).
 hipError_t smooth_group_prepare(const SmoothGroup &grp);
+// The launch plan of a catalogue's smoothing (smooth.hip): axis passes of
+// grids of different shapes with the same line length share one launch.
+struct SmoothBatchPlan;
+SmoothBatchPlan *smooth_batch_plan_create(const SmoothGroup *groups, int ngroups,
+                                          hipError_t &err);
+void smooth_batch_plan_free(SmoothBatchPlan *plan);
 hipError_t gaussian_smooth_batch(float *den, const SmoothGroup *groups, int ngroups,
-                                 hipStream_t s);
+                                 const SmoothBatchPlan *plan, hipStream_t s);
 
 // ---- kNN (knn.hip) -------------------------------------------------------
 // Exact k nearest candidates of every query (self included when the query

@@ -32,6 +32,7 @@
 // scratch there).
 #include <cmath>
 #include <cstring>
+#include <array>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -65,6 +66,16 @@ struct AxisPass {
     int tables;         // 1: twiddles and spectral factor staged in LDS
     int pairs;          // 1: adjacent lines may be read as one float2 (8-byte aligned)
     long long gcells;   // floats of one grid (register FFT: byte offsets fit 31 bits)
+};
+
+// One grid's lines in a pass that covers grids of several shapes (a batch's
+// axis pass over every grid with the same line length): its offset, the
+// stride along the line, the fast batch dimension (nb lines, stride sb) and
+// the slow one (nc, stride sc).  Blocks beyond a grid's own lines exit.
+struct GridLines {
+    long long goff;
+    long long sa, sb, sc;
+    int nb, nc;
 };
 
 __device__ inline double2 cmul(double2 a, double2 b) {
@@ -155,11 +166,22 @@ constexpr int kLoadBatch = 16;  // global loads in flight per thread in the load
 template <int LC>
 __global__ void __launch_bounds__(kBlock)
 smooth_axis_k(float *g, AxisPass ap, const double2 *tw_g, const double *H_g,
-              const long long *goffs) {
+              const long long *goffs, const GridLines *gl) {
     extern __shared__ double2 buf[];
     const int L = ap.L, P = ap.P, n = ap.n, LS = ap.LS;
     const long long ib0 = (long long)blockIdx.x * 2 * P;
-    float *base = g + (goffs ? goffs[blockIdx.z] : 0LL) + (long long)blockIdx.y * ap.sc;
+    long long nbl = ap.nb, sbl = ap.sb, sal = ap.sa;
+    float *base;
+    if (gl) {
+        const GridLines d = gl[blockIdx.z];
+        if ((int)blockIdx.y >= d.nc || ib0 >= d.nb) return;  // the whole block
+        nbl = d.nb;
+        sbl = d.sb;
+        sal = d.sa;
+        base = g + d.goff + (long long)blockIdx.y * d.sc;
+    } else {
+        base = g + (goffs ? goffs[blockIdx.z] : 0LL) + (long long)blockIdx.y * ap.sc;
+    }
     const bool tables = ap.tables != 0;
     double2 *s_tw = buf + P * LS;
     double *s_H = reinterpret_cast<double *>(s_tw + L);
@@ -185,7 +207,7 @@ smooth_axis_k(float *g, AxisPass ap, const double2 *tw_g, const double *H_g,
             if (ap.contiguous) { lb = idx / n; j = idx - lb * n; }
             else { j = idx / lines; lb = idx - j * lines; }
             const long long ib = ib0 + lb;
-            v[u] = (idx < tot && ib < ap.nb) ? base[ib * ap.sb + (long long)j * ap.sa] : 0.0f;
+            v[u] = (idx < tot && ib < nbl) ? base[ib * sbl + (long long)j * sal] : 0.0f;
         }
 #pragma unroll
         for (int u = 0; u < kLoadBatch; u++) {
@@ -218,9 +240,9 @@ smooth_axis_k(float *g, AxisPass ap, const double2 *tw_g, const double *H_g,
         if (ap.contiguous) { lb = idx / n; j = idx - lb * n; }
         else { j = idx / lines; lb = idx - j * lines; }
         long long ib = ib0 + lb;
-        if (ib < ap.nb) {
+        if (ib < nbl) {
             const double *src = reinterpret_cast<const double *>(&buf[(lb >> 1) * LS + j]);
-            base[ib * ap.sb + (long long)j * ap.sa] = (float)src[lb & 1];
+            base[ib * sbl + (long long)j * sal] = (float)src[lb & 1];
         }
     }
     if (ap.zero_pads) {
@@ -228,7 +250,7 @@ smooth_axis_k(float *g, AxisPass ap, const double2 *tw_g, const double *H_g,
         for (int idx = threadIdx.x; idx < lines * pw; idx += kBlock) {
             int lb = idx / pw, j = n + idx % pw;
             long long ib = ib0 + lb;
-            if (ib < ap.nb) base[ib * ap.sb + j] = 0.0f;
+            if (ib < nbl) base[ib * sbl + j] = 0.0f;
         }
     }
 }
@@ -564,11 +586,21 @@ __device__ inline void conv_line(const float (&v)[N], const Taps &taps, float *o
 // consecutive x, so every load and store of the wave is coalesced.
 template <int N>
 __global__ void __launch_bounds__(256)
-smooth_direct_k(float *grid, DirectPass dp, Taps taps, const long long *goffs) {
+smooth_direct_k(float *grid, DirectPass dp, Taps taps, const long long *goffs,
+                const GridLines *gl) {
     const long long line = (long long)blockIdx.x * 256 + threadIdx.x;
+    float *g0 = grid + (goffs ? goffs[blockIdx.y] : 0LL);
+    if (gl) {  // this grid's lines of a pass over several shapes
+        const GridLines d = gl[blockIdx.y];
+        g0 = grid + d.goff;
+        dp.na = d.nb;
+        dp.nlines = (long long)d.nb * d.nc;
+        dp.sa = d.sb;
+        dp.sb = d.sc;
+        dp.step = d.sa;
+    }
     if (line >= dp.nlines) return;
-    float *p = grid + (goffs ? goffs[blockIdx.y] : 0LL) + (line % dp.na) * dp.sa +
-               (line / dp.na) * dp.sb;
+    float *p = g0 + (line % dp.na) * dp.sa + (line / dp.na) * dp.sb;
     float v[N];
 #pragma unroll
     for (int j = 0; j < N; j++) v[j] = p[(long long)j * dp.step];
@@ -581,13 +613,21 @@ smooth_direct_k(float *grid, DirectPass dp, Taps taps, const long long *goffs) {
 // global loads and stores are coalesced; pad columns are written as zero.
 template <int N, int BLK>
 __global__ void __launch_bounds__(BLK)
-smooth_direct_x_k(float *grid, DirectPass dp, Taps taps, const long long *goffs) {
+smooth_direct_x_k(float *grid, DirectPass dp, Taps taps, const long long *goffs,
+                  const GridLines *gl) {
     extern __shared__ float tile[];
     // row width mx = 2 (n/2 + 1) = N + 2 for even N (checked by the host)
     constexpr int W = N + 2, WS = N + 3;
     const long long row0 = (long long)blockIdx.x * BLK;
+    float *g0 = grid + (goffs ? goffs[blockIdx.y] : 0LL);
+    if (gl) {  // this grid's rows of a pass over several shapes
+        const GridLines d = gl[blockIdx.y];
+        g0 = grid + d.goff;
+        dp.nlines = (long long)d.nb * d.nc;
+        if (row0 >= dp.nlines) return;  // the whole block
+    }
     const int nrows = (int)(dp.nlines - row0 < BLK ? dp.nlines - row0 : BLK);
-    float *base = grid + (goffs ? goffs[blockIdx.y] : 0LL) + row0 * W;
+    float *base = g0 + row0 * W;
     const int tot = nrows * W;
     // kLB independent loads per thread in flight before their LDS stores
     constexpr int kLB = 12;
@@ -833,14 +873,33 @@ hipError_t run_fft2(float *g, const AxisPlan &pl, const AxisPass &ap, int nc, hi
     return hipGetLastError();
 }
 
+// A pass over several grids of different shapes (a batch's grids with this
+// line length): per-grid lines in gl[0 .. count), the launch sized by the
+// largest grid; sa == 1 marks the x pass, nb is the smallest grid's line
+// count (it caps the lines per block).
+struct MixedLines {
+    const GridLines *gl = nullptr;
+    int count = 0;
+    int min_nb = 0, max_nb = 0, max_nc = 0;
+    long long max_lines = 0;
+};
+
+bool fft2_length(const AxisPlan &pl, long long gcells) {
+    return PGALF_SMOOTH_FFT2 && pl.L == pl.n && (pl.n == 64 || pl.n == 128 || pl.n == 256) &&
+           gcells > 0 && gcells < (1LL << 29);
+}
+bool direct_length(const AxisPlan &pl, bool rows, int dmax) {
+    return pl.direct && pl.n <= dmax && (!rows || pl.n <= 32);
+}
+
 hipError_t run_pass(float *g, const AxisPlan &pl, long long sa, int nb, long long sb, int nc,
                     long long sc, int contiguous, int zero_pads, int padw, long long gcells,
-                    hipStream_t s, const long long *goffs = nullptr, int ngrids = 1) {
-    const bool fft2 = PGALF_SMOOTH_FFT2 && pl.L == pl.n &&
-                      (pl.n == 64 || pl.n == 128 || pl.n == 256) && gcells > 0 &&
-                      gcells < (1LL << 29);
-    const int dmax = smooth_direct_max();
-    const bool direct = pl.direct && pl.n <= dmax && (sa != 1 || pl.n <= 32);
+                    hipStream_t s, const long long *goffs = nullptr, int ngrids = 1,
+                    const MixedLines *mix = nullptr, int dmax = -1) {
+    if (dmax < 0) dmax = smooth_direct_max();
+    const bool fft2 = !mix && fft2_length(pl, gcells);
+    const bool direct = direct_length(pl, sa == 1, dmax);
+    if (mix) ngrids = mix->count;
     if (fft2 && !direct) {
         AxisPass ap{};
         ap.n = pl.n;
@@ -873,22 +932,24 @@ hipError_t run_pass(float *g, const AxisPlan &pl, long long sa, int nb, long lon
         const bool rows = sa == 1;
         if (rows && padw != pl.n + 2) return hipErrorInvalidValue;
         const int blk = 256;
-        const unsigned gx = (unsigned)((dp.nlines + blk - 1) / blk);
+        const long long lines = mix ? mix->max_lines : dp.nlines;
+        const unsigned gx = (unsigned)((lines + blk - 1) / blk);
         const size_t lds = rows ? sizeof(float) * (size_t)blk * (padw + 1) : 0;
         for (int z0 = 0; z0 < ngrids; z0 += kMaxY) {
             const int nz = ngrids - z0 < kMaxY ? ngrids - z0 : kMaxY;
-            const long long *gz = goffs ? goffs + z0 : nullptr;
+            const long long *gz = goffs && !mix ? goffs + z0 : nullptr;
+            const GridLines *gl = mix ? mix->gl + z0 : nullptr;
             const dim3 grid(gx, (unsigned)nz);
 #define PGALF_DIRECT_ROWS(NN)                                                                  \
     case NN:                                                                                   \
         if (rows)                                                                              \
-            smooth_direct_x_k<NN, 256><<<grid, 256, lds, s>>>(g, dp, pl.taps, gz);             \
+            smooth_direct_x_k<NN, 256><<<grid, 256, lds, s>>>(g, dp, pl.taps, gz, gl);         \
         else                                                                                   \
-            smooth_direct_k<NN><<<grid, 256, 0, s>>>(g, dp, pl.taps, gz);                      \
+            smooth_direct_k<NN><<<grid, 256, 0, s>>>(g, dp, pl.taps, gz, gl);                  \
         break;
 #define PGALF_DIRECT_STRIDED(NN)                                                               \
     case NN:                                                                                   \
-        smooth_direct_k<NN><<<grid, 256, 0, s>>>(g, dp, pl.taps, gz);                          \
+        smooth_direct_k<NN><<<grid, 256, 0, s>>>(g, dp, pl.taps, gz, gl);                      \
         break;
             switch (pl.n) {
                 PGALF_DIRECT_ROWS(8)
@@ -935,14 +996,57 @@ hipError_t run_pass(float *g, const AxisPlan &pl, long long sa, int nb, long lon
     ap.padw = padw;
     const size_t lds = sizeof(double2) * (size_t)P * ap.LS + tbytes;
     constexpr int kMaxZ = 65535;  // grid dimension limit: larger batches in several launches
+    const int gnb = mix ? mix->max_nb : nb, gnc = mix ? mix->max_nc : nc;
     for (int z0 = 0; z0 < ngrids; z0 += kMaxZ) {
         const int nz = ngrids - z0 < kMaxZ ? ngrids - z0 : kMaxZ;
-        dim3 grid((unsigned)((nb + 2 * P - 1) / (2 * P)), (unsigned)nc, (unsigned)nz);
-        const long long *gz = goffs ? goffs + z0 : nullptr;
+        dim3 grid((unsigned)((gnb + 2 * P - 1) / (2 * P)), (unsigned)gnc, (unsigned)nz);
+        const long long *gz = goffs && !mix ? goffs + z0 : nullptr;
+        const GridLines *gl = mix ? mix->gl + z0 : nullptr;
         if (LC == kLdsComplex)
-            smooth_axis_k<kLdsComplex><<<grid, kBlock, lds, s>>>(g, ap, pl.tw, pl.H, gz);
+            smooth_axis_k<kLdsComplex><<<grid, kBlock, lds, s>>>(g, ap, pl.tw, pl.H, gz, gl);
         else
-            smooth_axis_k<kLdsComplexMax><<<grid, kBlock, lds, s>>>(g, ap, pl.tw, pl.H, gz);
+            smooth_axis_k<kLdsComplexMax><<<grid, kBlock, lds, s>>>(g, ap, pl.tw, pl.H, gz, gl);
+    }
+    return hipGetLastError();
+}
+
+// The axis pass `axis` (0: x rows, 1: y, 2: z) of grids of shape nx, ny, nz
+// at goffs (or, with mix, of the grids in mix->gl, of any shape with this
+// line length; nx, ny, nz then only give the line length and row width).
+hipError_t axis_pass(float *den, int axis, const AxisPlan &pl, int nx, int ny, int nz,
+                     hipStream_t s, const long long *goffs, int ngrids,
+                     const MixedLines *mix = nullptr, int dmax = -1) {
+    const long long mx = 2LL * (nx / 2 + 1);
+    const long long gc = mx * ny * nz;
+    switch (axis) {
+    case 0:  // lines are rows (contiguous), batch over ny*nz rows
+        return run_pass(den, pl, 1, mix ? mix->min_nb : ny * nz, mx, 1, 0, 1, 1, (int)mx, gc,
+                        s, goffs, ngrids, mix, dmax);
+    case 1:  // lines along y (stride mx), batch over x (stride 1), one z per block row
+        return run_pass(den, pl, mx, mix ? mix->min_nb : nx, 1, nz, mx * ny, 0, 0, 0, gc, s,
+                        goffs, ngrids, mix, dmax);
+    default:  // lines along z (stride mx*ny), batch over x, one y per block row
+        return run_pass(den, pl, mx * ny, mix ? mix->min_nb : nx, 1, ny, mx, 0, 0, 0, gc, s,
+                        goffs, ngrids, mix, dmax);
+    }
+}
+
+// Small cubes whose three passes are all direct: one LDS-resident kernel.
+bool cube_shape(int nx, int ny, int nz, const AxisPlan &px, int dmax) {
+    return PGALF_SMOOTH_CUBE && nx == ny && ny == nz && px.direct && nx <= 32 && nx <= dmax;
+}
+hipError_t cube_pass(float *den, int n, const Taps &taps, hipStream_t s, const long long *goffs,
+                     int ngrids) {
+    for (int z0 = 0; z0 < ngrids; z0 += 1 << 30) {
+        const int cnt = ngrids - z0 < (1 << 30) ? ngrids - z0 : (1 << 30);
+        const long long *gz = goffs ? goffs + z0 : nullptr;
+        switch (n) {
+        case 8: smooth_cube_k<8><<<cnt, 1024, 0, s>>>(den, taps, gz); break;
+        case 16: smooth_cube_k<16><<<cnt, 1024, 0, s>>>(den, taps, gz); break;
+        case 24: smooth_cube_k<24><<<cnt, 1024, 0, s>>>(den, taps, gz); break;
+        case 32: smooth_cube_k<32><<<cnt, 1024, 0, s>>>(den, taps, gz); break;
+        default: return hipErrorInvalidValue;
+        }
     }
     return hipGetLastError();
 }
@@ -950,34 +1054,14 @@ hipError_t run_pass(float *g, const AxisPlan &pl, long long sa, int nb, long lon
 // The three axis passes of one grid shape over `ngrids` grids.
 hipError_t smooth_passes(float *den, int nx, int ny, int nz, double expfac, hipStream_t s,
                          const long long *goffs, int ngrids) {
-    AxisPlan px, py, pz;
-    PGALF_HIP_TRY(get_plan(nx, expfac, px));
-    PGALF_HIP_TRY(get_plan(ny, expfac, py));
-    PGALF_HIP_TRY(get_plan(nz, expfac, pz));
-    const long long mx = 2LL * (nx / 2 + 1);
-    const long long gc = mx * ny * nz;
-    // small cubes whose three passes are all direct: one LDS-resident kernel
-    if (PGALF_SMOOTH_CUBE && nx == ny && ny == nz && px.direct && nx <= 32 &&
-        nx <= smooth_direct_max()) {
-        for (int z0 = 0; z0 < ngrids; z0 += 1 << 30) {
-            const int cnt = ngrids - z0 < (1 << 30) ? ngrids - z0 : (1 << 30);
-            const long long *gz = goffs ? goffs + z0 : nullptr;
-            switch (nx) {
-            case 8: smooth_cube_k<8><<<cnt, 1024, 0, s>>>(den, px.taps, gz); break;
-            case 16: smooth_cube_k<16><<<cnt, 1024, 0, s>>>(den, px.taps, gz); break;
-            case 24: smooth_cube_k<24><<<cnt, 1024, 0, s>>>(den, px.taps, gz); break;
-            case 32: smooth_cube_k<32><<<cnt, 1024, 0, s>>>(den, px.taps, gz); break;
-            default: return hipErrorInvalidValue;
-            }
-        }
-        return hipGetLastError();
-    }
-    // x: lines are rows (contiguous), batch over ny*nz rows
-    PGALF_HIP_TRY(run_pass(den, px, 1, ny * nz, mx, 1, 0, 1, 1, (int)mx, gc, s, goffs, ngrids));
-    // y: lines along y (stride mx), batch over x (stride 1), one z per block row
-    PGALF_HIP_TRY(run_pass(den, py, mx, nx, 1, nz, mx * ny, 0, 0, 0, gc, s, goffs, ngrids));
-    // z: lines along z (stride mx*ny), batch over x, one y per block row
-    PGALF_HIP_TRY(run_pass(den, pz, mx * ny, nx, 1, ny, mx, 0, 0, 0, gc, s, goffs, ngrids));
+    AxisPlan p[3];
+    PGALF_HIP_TRY(get_plan(nx, expfac, p[0]));
+    PGALF_HIP_TRY(get_plan(ny, expfac, p[1]));
+    PGALF_HIP_TRY(get_plan(nz, expfac, p[2]));
+    const int dmax = smooth_direct_max();
+    if (cube_shape(nx, ny, nz, p[0], dmax)) return cube_pass(den, nx, p[0].taps, s, goffs, ngrids);
+    for (int a = 0; a < 3; a++)
+        PGALF_HIP_TRY(axis_pass(den, a, p[a], nx, ny, nz, s, goffs, ngrids, nullptr, dmax));
     return hipSuccess;
 }
 
@@ -1001,13 +1085,138 @@ hipError_t smooth_group_prepare(const SmoothGroup &grp) {
     return get_plan(grp.nz, grp.expfac, p);
 }
 
-hipError_t gaussian_smooth_batch(float *den, const SmoothGroup *groups, int ngroups,
-                                 hipStream_t s) {
+// A catalogue's smoothing: small cubes by the LDS-resident kernel per shape;
+// the axis passes of lengths that take the register FFT per shape (its
+// launch geometry is per shape); every other axis pass -- direct
+// convolution or the Stockham FFT -- as one launch per (axis, line length,
+// smoothing factor) over the grids of all shapes that have it, so that a
+// catalogue of many shapes does not pay one small launch per shape and
+// axis.  Passes run axis by axis (all x, then all y, then all z), which
+// keeps each grid's x -> y -> z order.  The direct-convolution threshold is
+// the one in force when the plan is made.
+struct SmoothBatchPlan {
+    int dmax = 56;
+    std::vector<int> cubes;      // groups
+    std::vector<int> whole[3];   // groups whose axis pass runs per shape
+    struct Mixed {
+        int axis, nx, ny, nz;  // a representative shape (line length, row width)
+        double expfac;
+        size_t first;
+        MixedLines ml;
+    };
+    std::vector<Mixed> mixed;
+    GridLines *d_lines = nullptr;
+};
+
+SmoothBatchPlan *smooth_batch_plan_create(const SmoothGroup *groups, int ngroups,
+                                          hipError_t &err) {
+    err = hipSuccess;
+    auto *pl = new SmoothBatchPlan;
+    pl->dmax = smooth_direct_max();
+    std::map<std::tuple<int, int, unsigned long long>, std::vector<GridLines>> mix;
+    std::map<std::tuple<int, int, unsigned long long>, std::array<int, 3>> rep;
     for (int i = 0; i < ngroups; i++) {
-        const SmoothGroup &grp = groups[i];
-        if (grp.count > 0)
-            PGALF_HIP_TRY(smooth_passes(den, grp.nx, grp.ny, grp.nz, grp.expfac, s, grp.goffs,
-                                        grp.count));
+        const SmoothGroup &g = groups[i];
+        AxisPlan p[3];
+        const int dims[3] = {g.nx, g.ny, g.nz};
+        for (int a = 0; a < 3 && err == hipSuccess; a++) err = get_plan(dims[a], g.expfac, p[a]);
+        if (err != hipSuccess) break;
+        if (cube_shape(g.nx, g.ny, g.nz, p[0], pl->dmax)) {
+            pl->cubes.push_back(i);
+            continue;
+        }
+        const long long mx = 2LL * (g.nx / 2 + 1), gc = mx * g.ny * g.nz;
+        unsigned long long eb;
+        std::memcpy(&eb, &g.expfac, sizeof(eb));
+        for (int a = 0; a < 3; a++) {
+            if (fft2_length(p[a], gc) && !direct_length(p[a], a == 0, pl->dmax)) {
+                pl->whole[a].push_back(i);
+                continue;
+            }
+            const auto key = std::make_tuple(a, dims[a], eb);
+            rep[key] = {g.nx, g.ny, g.nz};
+            auto &v = mix[key];
+            for (int k = 0; k < g.count; k++) {
+                GridLines d;
+                d.goff = g.hgoffs[k];
+                if (a == 0) {
+                    d.sa = 1; d.sb = mx; d.sc = 0; d.nb = g.ny * g.nz; d.nc = 1;
+                } else if (a == 1) {
+                    d.sa = mx; d.sb = 1; d.sc = mx * g.ny; d.nb = g.nx; d.nc = g.nz;
+                } else {
+                    d.sa = mx * g.ny; d.sb = 1; d.sc = mx; d.nb = g.nx; d.nc = g.ny;
+                }
+                v.push_back(d);
+            }
+        }
+    }
+    std::vector<GridLines> all;
+    for (auto &kv : mix) {
+        SmoothBatchPlan::Mixed m;
+        m.axis = std::get<0>(kv.first);
+        const auto &r = rep[kv.first];
+        m.nx = r[0];
+        m.ny = r[1];
+        m.nz = r[2];
+        std::memcpy(&m.expfac, &std::get<2>(kv.first), sizeof(m.expfac));
+        m.first = all.size();
+        m.ml.count = (int)kv.second.size();
+        m.ml.min_nb = 1 << 30;
+        for (const GridLines &d : kv.second) {
+            m.ml.min_nb = std::min(m.ml.min_nb, d.nb);
+            m.ml.max_nb = std::max(m.ml.max_nb, d.nb);
+            m.ml.max_nc = std::max(m.ml.max_nc, d.nc);
+            m.ml.max_lines = std::max(m.ml.max_lines, (long long)d.nb * d.nc);
+        }
+        all.insert(all.end(), kv.second.begin(), kv.second.end());
+        pl->mixed.push_back(m);
+    }
+    if (err == hipSuccess && !all.empty()) {
+        err = hipMalloc(&pl->d_lines, sizeof(GridLines) * all.size());
+        if (err == hipSuccess)
+            err = hipMemcpy(pl->d_lines, all.data(), sizeof(GridLines) * all.size(),
+                            hipMemcpyHostToDevice);
+    }
+    for (auto &m : pl->mixed) m.ml.gl = pl->d_lines + m.first;
+    if (err != hipSuccess) {
+        smooth_batch_plan_free(pl);
+        return nullptr;
+    }
+    return pl;
+}
+
+void smooth_batch_plan_free(SmoothBatchPlan *pl) {
+    if (!pl) return;
+    if (pl->d_lines) (void)hipFree(pl->d_lines);
+    delete pl;
+}
+
+hipError_t gaussian_smooth_batch(float *den, const SmoothGroup *groups, int ngroups,
+                                 const SmoothBatchPlan *pl, hipStream_t s) {
+    (void)ngroups;
+    for (int i : pl->cubes) {
+        const SmoothGroup &g = groups[i];
+        AxisPlan p;
+        PGALF_HIP_TRY(get_plan(g.nx, g.expfac, p));
+        PGALF_HIP_TRY(cube_pass(den, g.nx, p.taps, s, g.goffs, g.count));
+    }
+    for (int a = 0; a < 3; a++) {
+        for (int i : pl->whole[a]) {
+            const SmoothGroup &g = groups[i];
+            const int dims[3] = {g.nx, g.ny, g.nz};
+            AxisPlan p;
+            PGALF_HIP_TRY(get_plan(dims[a], g.expfac, p));
+            PGALF_HIP_TRY(axis_pass(den, a, p, g.nx, g.ny, g.nz, s, g.goffs, g.count, nullptr,
+                                    pl->dmax));
+        }
+        for (const auto &m : pl->mixed) {
+            if (m.axis != a) continue;
+            const int dims[3] = {m.nx, m.ny, m.nz};
+            AxisPlan p;
+            PGALF_HIP_TRY(get_plan(dims[a], m.expfac, p));
+            PGALF_HIP_TRY(axis_pass(den, a, p, m.nx, m.ny, m.nz, s, nullptr, m.ml.count, &m.ml,
+                                    pl->dmax));
+        }
     }
     return hipSuccess;
 }
