@@ -38,7 +38,6 @@ struct pgalf_batch {
     PChunk *d_chunks = nullptr;
     PChunk *d_fine = nullptr;
     unsigned *d_tile_halo = nullptr;
-    unsigned *d_slab_tile = nullptr;
     long long *d_goffs = nullptr;  // smoothing groups' grid offsets, group after group
     std::vector<SmoothGroup> groups;
     SmoothBatchPlan *splan = nullptr;
@@ -63,7 +62,6 @@ void free_batch(pgalf_batch *b) {
     if (b->d_chunks) (void)hipFree(b->d_chunks);
     if (b->d_fine) (void)hipFree(b->d_fine);
     if (b->d_tile_halo) (void)hipFree(b->d_tile_halo);
-    if (b->d_slab_tile) (void)hipFree(b->d_slab_tile);
     if (b->d_goffs) (void)hipFree(b->d_goffs);
     smooth_batch_plan_free(b->splan);
     knn_batch_plan_free(b->kplan);
@@ -273,9 +271,10 @@ extern "C" pgalf_batch_t *pgalf_batch_create(int nhalo, const long long *off,
     b->k = k;
     std::vector<HaloSeg> segs((size_t)nhalo);
     std::vector<PChunk> chunks, fine;
-    std::vector<unsigned> tile_halo, slab_tile;
+    std::vector<unsigned> tile_halo;
     b->goff.assign((size_t)nhalo + 1, 0);
     bool too_wide = false;
+    int max_entries = 1;
     long long toff = 0, max_tiles = 0;
     for (int h = 0; h < nhalo; h++) {
         const pgalf_grid_t &gg = grids[h];
@@ -294,8 +293,8 @@ extern "C" pgalf_batch_t *pgalf_batch_create(int nhalo, const long long *off,
         b->goff[h + 1] = b->goff[h] + sg.g.ncells();
         const long long nt = tsc_tile_count(sg.g);
         tile_halo.insert(tile_halo.end(), (size_t)nt, (unsigned)h);
-        tsc_slab_tiles(sg.g, toff, slab_tile);
-        too_wide |= tsc_tiles_x(sg.g) > tsc_max_tiles_x();
+        too_wide |= !tsc_grid_fits(sg.g);
+        max_entries = std::max(max_entries, tsc_tile_entries(sg.g));
         toff += nt;
         max_tiles = std::max(max_tiles, nt);
         for (long long c = off[h]; c < off[h + 1]; c += kChunkParticles) {
@@ -340,7 +339,6 @@ extern "C" pgalf_batch_t *pgalf_batch_create(int nhalo, const long long *off,
     if (e == hipSuccess) e = upload(&b->d_chunks, chunks);
     if (e == hipSuccess) e = upload(&b->d_fine, fine);
     if (e == hipSuccess) e = upload(&b->d_tile_halo, tile_halo);
-    if (e == hipSuccess) e = upload(&b->d_slab_tile, slab_tile);
     if (e == hipSuccess) e = upload(&b->d_goffs, goffs);
     for (size_t i = 0; i < b->groups.size(); i++) {
         b->groups[i].goffs = b->d_goffs + gfirst[i];
@@ -366,8 +364,7 @@ extern "C" pgalf_batch_t *pgalf_batch_create(int nhalo, const long long *off,
     b->bd.tile_halo = b->d_tile_halo;
     b->bd.ntiles = toff;
     b->bd.max_halo_tiles = max_tiles;
-    b->bd.slab_tile = b->d_slab_tile;
-    b->bd.nslabs = (long long)slab_tile.size();
+    b->bd.max_tile_entries = max_entries;
     b->bd.sched = b->kplan.sched;  // the schedule of the batch's total, fixed here
     b->ws = new Workspace(b->device);
     ok();

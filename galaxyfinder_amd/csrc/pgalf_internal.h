@@ -77,8 +77,7 @@ struct BatchDev {
     const unsigned *tile_halo = nullptr;  // [ntiles] halo of each deposit tile
     long long ntiles = 0;
     long long max_halo_tiles = 0;
-    const unsigned *slab_tile = nullptr;  // [nslabs] first tile of each (halo, ty, tz) slab
-    long long nslabs = 0;
+    int max_tile_entries = 4;       // deposit tiles a particle can touch, at most
 };
 
 // Device error bits raised by kernels (read back with the status word).
@@ -212,11 +211,10 @@ hipError_t tsc_readback_batch(const P *x, const P *y, const P *z, const BatchDev
                               const long long *shift = nullptr);
 // Deposit tiles of one grid (16^3 nodes over the padded mx x ny x nz box).
 long long tsc_tile_count(const Grid &g);
-// Appends the first tile id (tx = 0) of every (ty, tz) slab of a grid whose
-// tiles start at toff; tiles along x of a grid and their limit.
-void tsc_slab_tiles(const Grid &g, long long toff, std::vector<unsigned> &out);
-int tsc_tiles_x(const Grid &g);
-int tsc_max_tiles_x();
+// Whether the deposit can take the grid (a tile holds at least one padded row).
+bool tsc_grid_fits(const Grid &g);
+// Deposit tiles one particle can touch in the grid, at most.
+int tsc_tile_entries(const Grid &g);
 // Adjoint of read-back and deposit (see tsc.hip): adds dL/dx, dL/dy, dL/dz,
 // dL/dm of L(rho_tsc) to acc = [gx | gy | gz | gm] (FP64, np each) from
 // gin = dL/drho_tsc, the forward smoothed grid F and the adjoint grid A.

@@ -1,9 +1,11 @@
 // TSC mass deposit and TSC read-back for gfx950.
 //
 // Deposit (assign_density_TSC, NewGalFinder/tsc_omp2.c:166-506):
-//   1. count:   every selected particle lists the 16^3 node tiles its 27-node
-//               stencil touches (1..8 tiles) -> per-tile counts (block-local
-//               LDS histograms, one global add per block and tile);
+//   Tiles are TY x TZ whole padded rows of a grid (TY * TZ * mx <= kTileCap
+//   nodes), so a particle's 27-node stencil touches at most 2 x 2 of them.
+//   1. count:   every selected particle lists the tiles its stencil touches
+//               -> per-tile counts (block-local LDS histograms, one global
+//               add per block and tile);
 //   2. scan:    tile offsets; tiles are cut into work units of <= `unit`
 //               entries (the schedule's tsc_unit_entries) so a cusp tile is
 //               shared by many workgroups;
@@ -14,17 +16,12 @@
 //               contributions that land inside the tile in an LDS int64
 //               fixed-point tile (ds_add_u64).  Integer addition is
 //               associative, so the grid is bit-identical run to run
-//               whatever the order of the atomics; single-unit tiles are
-//               written once, whole, into a tile-major buffer (one aligned
-//               16 KB slot per tile with entries), multi-unit tiles are
-//               summed in a global int64 side buffer and converted into
-//               their slot by a finalize pass;
-//   5. de-tile: the padded row-major grid is written row by row from the
-//               slots, zeros for tiles without entries.  Every row is one
-//               run of consecutive stores: tile-shaped stores into the
-//               padded grid (rows of mx floats) leave partly written lines
-//               at every tile edge, which the memory system completes at a
-//               fraction of its streaming rate.
+//               whatever the order of the atomics.  A single-unit tile is
+//               written straight into the grid: TZ runs of TY * mx
+//               consecutive floats; multi-unit tiles are summed in a global
+//               int64 side buffer;
+//   5. rows:    the tiles the deposit did not write -- zeros for tiles
+//               without entries, the side sums for multi-unit tiles.
 //   The per-contribution value is the reference's FP32 product
 //   (wx*wy)*wz with its float/double conversions (tsc_omp2.c:303-382); the
 //   fixed-point scale 2^S is chosen from max|m| * count so no sum overflows.
@@ -38,15 +35,21 @@
 namespace pgalf {
 namespace {
 
-constexpr int kT = 16;               // tile edge (nodes)
-constexpr int kTileNodes = kT * kT * kT;
+// A deposit tile is TY x TZ whole padded rows (mx floats each) of one grid:
+// its output is TZ runs of TY * mx consecutive floats of the grid, written
+// straight from the workgroup's LDS accumulator, so no tile-shaped stores
+// (partly written lines at every tile edge) and no second pass.  TY and TZ
+// are powers of two with TY * TZ * mx <= kTileCap, as square as they come.
+constexpr int kTileCap = 4352;       // nodes of one tile: 34 KB of int64 in LDS
 constexpr int kLdsHistMax = 8192;    // tiles handled with LDS histograms
 constexpr int kBlock = 256;
 constexpr int kPerBlock = 8192;      // particles per count/scatter block
 
 struct TileGeom {
-    int ntx, nty, ntz;
-    __host__ __device__ long long ntiles() const { return (long long)ntx * nty * ntz; }
+    int mx, TY, TZ;  // row width, rows of a tile along y and z
+    int nty, ntz;    // tiles along y and z
+    __host__ __device__ long long ntiles() const { return (long long)nty * ntz; }
+    __host__ __device__ int nodes() const { return mx * TY * TZ; }
 };
 
 struct Axis {
@@ -114,10 +117,10 @@ __device__ inline bool selected(const int *type, long long i, int ptype) {
     return ptype == PGALF_TYPE_ALL || type == nullptr || type[i] == ptype;
 }
 
-// Tiles touched by nodes [nr-1, nr+1] along one axis.
-__device__ inline void tile_span(long long nr, int &t0, int &t1) {
-    t0 = (int)((nr - 1) / kT);
-    t1 = (int)((nr + 1) / kT);
+// Tiles of extent `ext` touched by nodes [nr-1, nr+1] along one axis.
+__device__ inline void tile_span(long long nr, int ext, int &t0, int &t1) {
+    t0 = (int)((nr - 1) / ext);
+    t1 = (int)((nr + 1) / ext);
 }
 
 // Halo `h` of a batch, or the single halo `seg0` of a one-halo call.
@@ -137,9 +140,17 @@ __device__ inline PChunk chunk_at(const PChunk *chunks, const HaloSeg &seg0, int
 
 __host__ __device__ inline TileGeom tile_geom(const Grid &g) {
     TileGeom tg;
-    tg.ntx = (int)((g.mx() + kT - 1) / kT);
-    tg.nty = (g.ny + kT - 1) / kT;
-    tg.ntz = (g.nz + kT - 1) / kT;
+    tg.mx = (int)g.mx();
+    const int rows = kTileCap / tg.mx;  // >= 1: checked by the host (tsc_grid_fits)
+    int TY = 1, TZ = 1;
+    while (2 * TY * TZ <= rows && TY * TZ < 256) {
+        if (TZ <= TY) TZ *= 2;
+        else TY *= 2;
+    }
+    tg.TY = TY;
+    tg.TZ = TZ;
+    tg.nty = (g.ny + TY - 1) / TY;
+    tg.ntz = (g.nz + TZ - 1) / TZ;
     return tg;
 }
 
@@ -172,17 +183,15 @@ tsc_count_k(const P *x, const P *y, const P *z, const P *m, const int *type, int
         }
         local_sel++;
         local_max = fmax(local_max, fabs((double)m[i]));
-        int tx0, tx1, ty0, ty1, tz0, tz1;
-        tile_span(ax[0].nr, tx0, tx1);
-        tile_span(ax[1].nr, ty0, ty1);
-        tile_span(ax[2].nr, tz0, tz1);
+        int ty0, ty1, tz0, tz1;
+        tile_span(ax[1].nr, tg.TY, ty0, ty1);
+        tile_span(ax[2].nr, tg.TZ, tz0, tz1);
         for (int tz = tz0; tz <= tz1; tz++)
-            for (int ty = ty0; ty <= ty1; ty++)
-                for (int tx = tx0; tx <= tx1; tx++) {
-                    long long t = tx + (long long)tg.ntx * (ty + (long long)tg.nty * tz);
-                    if (use_lds) atomicAdd(&hist[t], 1u);
-                    else atomicAdd(&tcount[sg.toff + t], 1u);
-                }
+            for (int ty = ty0; ty <= ty1; ty++) {
+                const long long t = ty + (long long)tg.nty * tz;
+                if (use_lds) atomicAdd(&hist[t], 1u);
+                else atomicAdd(&tcount[sg.toff + t], 1u);
+            }
     }
     if (local_sel) atomicAdd(&nsel[c.halo], local_sel);
     if (local_max > 0.0)
@@ -214,14 +223,12 @@ tsc_scatter_k(const P *x, const P *y, const P *z, const P *m, const int *type, i
             if (!selected(type, i, ptype)) continue;
             Axis ax[3];
             if (!particle_axes(x, y, z, m, i, g, ax)) continue;
-            int tx0, tx1, ty0, ty1, tz0, tz1;
-            tile_span(ax[0].nr, tx0, tx1);
-            tile_span(ax[1].nr, ty0, ty1);
-            tile_span(ax[2].nr, tz0, tz1);
+            int ty0, ty1, tz0, tz1;
+            tile_span(ax[1].nr, tg.TY, ty0, ty1);
+            tile_span(ax[2].nr, tg.TZ, tz0, tz1);
             for (int tz = tz0; tz <= tz1; tz++)
                 for (int ty = ty0; ty <= ty1; ty++)
-                    for (int tx = tx0; tx <= tx1; tx++)
-                        atomicAdd(&hist[tx + (long long)tg.ntx * (ty + (long long)tg.nty * tz)], 1u);
+                    atomicAdd(&hist[ty + (long long)tg.nty * tz], 1u);
         }
         __syncthreads();
         for (long long t = threadIdx.x; t < ntiles; t += kBlock) {
@@ -240,24 +247,22 @@ tsc_scatter_k(const P *x, const P *y, const P *z, const P *m, const int *type, i
         rec.y = y[i];
         rec.z = z[i];
         rec.m = m[i];
-        int tx0, tx1, ty0, ty1, tz0, tz1;
-        tile_span(ax[0].nr, tx0, tx1);
-        tile_span(ax[1].nr, ty0, ty1);
-        tile_span(ax[2].nr, tz0, tz1);
+        int ty0, ty1, tz0, tz1;
+        tile_span(ax[1].nr, tg.TY, ty0, ty1);
+        tile_span(ax[2].nr, tg.TZ, tz0, tz1);
         for (int tz = tz0; tz <= tz1; tz++)
-            for (int ty = ty0; ty <= ty1; ty++)
-                for (int tx = tx0; tx <= tx1; tx++) {
-                    long long t = tx + (long long)tg.ntx * (ty + (long long)tg.nty * tz);
-                    unsigned slot;
-                    if (use_lds) slot = base[t] + atomicAdd(&hist[t], 1u);
-                    else slot = tstart[sg.toff + t] + atomicAdd(&tcursor[sg.toff + t], 1u);
-                    bins[slot] = rec;
-                }
+            for (int ty = ty0; ty <= ty1; ty++) {
+                const long long t = ty + (long long)tg.nty * tz;
+                unsigned slot;
+                if (use_lds) slot = base[t] + atomicAdd(&hist[t], 1u);
+                else slot = tstart[sg.toff + t] + atomicAdd(&tcursor[sg.toff + t], 1u);
+                bins[slot] = rec;
+            }
     }
 }
 
-// Units per tile (none for a tile without bin entries: the de-tiling pass
-// writes its zeros) and whether the tile needs the multi-unit side buffer.
+// Units per tile (none for a tile without bin entries: tsc_rows_k writes
+// its zeros) and whether the tile needs the multi-unit side buffer.
 __global__ void tsc_units_k(const unsigned *tcount, long long ntiles, unsigned unit,
                             unsigned *nunits, unsigned *multi) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -295,15 +300,33 @@ __global__ void tsc_scale_k(const unsigned *nsel, const unsigned long long *maxm
     scale[2 * h + 1] = ldexp(1.0, -S) * vf;
 }
 
+// The rows of tile t (TZ runs of TY * mx consecutive floats, clipped to the
+// grid) from `src` (the tile's node values in x + mx (ly + TY lz) order; a
+// null src writes zeros).
+template <class F>
+__device__ inline void write_tile_rows(float *den, const HaloSeg &sg, const TileGeom &tg,
+                                       unsigned tl, F value) {
+    const int ty = (int)(tl % (unsigned)tg.nty), tz = (int)(tl / (unsigned)tg.nty);
+    const long long mx = tg.mx;
+    const int nrows = min(tg.TY, sg.g.ny - ty * tg.TY);
+    const int run = nrows * tg.mx;
+    for (int lz = 0; lz < tg.TZ; lz++) {
+        const long long z = (long long)tz * tg.TZ + lz;
+        if (z >= sg.g.nz) break;
+        float *dst = den + sg.goff + mx * ((long long)ty * tg.TY + (long long)sg.g.ny * z);
+        const int l0 = lz * tg.TY * tg.mx;
+        for (int i = threadIdx.x; i < run; i += blockDim.x) dst[i] = value(l0 + i);
+    }
+}
+
 template <class P>
 __global__ void __launch_bounds__(kBlock)
-tsc_deposit_k(const P *x, const P *y, const P *z, const P *m, const HaloSeg *segs,
-              HaloSeg seg0, const unsigned *tile_halo, const unsigned *tstart,
-              const unsigned *tcount, const unsigned *ustart, const unsigned *nunits,
-              const unsigned *unit_tile, const unsigned *total_units,
+tsc_deposit_k(const HaloSeg *segs, HaloSeg seg0, const unsigned *tile_halo,
+              const unsigned *tstart, const unsigned *tcount, const unsigned *ustart,
+              const unsigned *nunits, const unsigned *unit_tile, const unsigned *total_units,
               const unsigned *multi_slot, const BinRec<P> *bins, const double *scale,
-              const unsigned *tslot, float *tbuf, unsigned long long *side, unsigned unit) {
-    __shared__ unsigned long long acc[kTileNodes];
+              float *den, unsigned long long *side, unsigned unit) {
+    __shared__ unsigned long long acc[kTileCap];
     const unsigned nunit_total = *total_units;
     for (unsigned u = blockIdx.x; u < nunit_total; u += gridDim.x) {
         unsigned t = unit_tile[u];
@@ -315,11 +338,10 @@ tsc_deposit_k(const P *x, const P *y, const P *z, const P *m, const HaloSeg *seg
         unsigned chunk = u - ustart[t];
         unsigned nu = nunits[t];
         const unsigned tl = (unsigned)(t - sg.toff);
-        int tx = (int)(tl % tg.ntx);
-        int ty = (int)((tl / tg.ntx) % tg.nty);
-        int tz = (int)(tl / ((unsigned)tg.ntx * tg.nty));
-        long long ox = (long long)tx * kT, oy = (long long)ty * kT, oz = (long long)tz * kT;
-        for (int i = threadIdx.x; i < kTileNodes; i += kBlock) acc[i] = 0ull;
+        const long long oy = (long long)(tl % (unsigned)tg.nty) * tg.TY;
+        const long long oz = (long long)(tl / (unsigned)tg.nty) * tg.TZ;
+        const int nodes = tg.nodes();
+        for (int i = threadIdx.x; i < nodes; i += kBlock) acc[i] = 0ull;
         __syncthreads();
         unsigned e0 = tstart[t] + chunk * unit;
         unsigned e1 = min(e0 + unit, tstart[t] + tcount[t]);
@@ -330,54 +352,64 @@ tsc_deposit_k(const P *x, const P *y, const P *z, const P *m, const HaloSeg *seg
 #pragma unroll
             for (int c = -1; c <= 1; c++) {
                 long long nzn = ax[2].nr + c - oz;
-                if (nzn < 0 || nzn >= kT) continue;
+                if (nzn < 0 || nzn >= tg.TZ) continue;
                 float wz = axis_weight(ax[2], c);
 #pragma unroll
                 for (int b = -1; b <= 1; b++) {
                     long long nyn = ax[1].nr + b - oy;
-                    if (nyn < 0 || nyn >= kT) continue;
+                    if (nyn < 0 || nyn >= tg.TY) continue;
                     float wy = axis_weight(ax[1], b);
+                    const long long row = (long long)tg.mx * (nyn + (long long)tg.TY * nzn);
 #pragma unroll
                     for (int a = -1; a <= 1; a++) {
-                        long long nxn = ax[0].nr + a - ox;
-                        if (nxn < 0 || nxn >= kT) continue;
+                        // x nodes stay in [0, mx) (particle_axes checked at count)
+                        const long long nxn = ax[0].nr + a;
                         float wx = axis_weight(ax[0], a);
                         float wxy = wx * wy;
                         float cval = wxy * wz;
                         long long q = (long long)rint((double)cval * sc);
-                        atomicAdd(&acc[nxn + kT * (nyn + kT * nzn)], (unsigned long long)q);
+                        atomicAdd(&acc[nxn + row], (unsigned long long)q);
                     }
                 }
             }
         }
         __syncthreads();
         if (nu == 1) {
-            // the whole tile, tile-major: aligned full-line stores
-            float *dst = tbuf + (size_t)tslot[t] * kTileNodes;
-            for (int l = threadIdx.x; l < kTileNodes; l += kBlock)
-                dst[l] = (float)((double)(long long)acc[l] * inv);
+            // the tile's rows, straight into the grid
+            write_tile_rows(den, sg, tg, tl,
+                            [&](int l) { return (float)((double)(long long)acc[l] * inv); });
         } else {
-            unsigned long long *dst = side + (size_t)multi_slot[t] * kTileNodes;
-            for (int l = threadIdx.x; l < kTileNodes; l += kBlock)
+            unsigned long long *dst = side + (size_t)multi_slot[t] * kTileCap;
+            for (int l = threadIdx.x; l < nodes; l += kBlock)
                 if (acc[l]) atomicAdd(&dst[l], acc[l]);
         }
         __syncthreads();
     }
 }
 
-// Multi-unit tiles: the side-buffer sums into the tile-major buffer.
-__global__ void tsc_finalize_k(const unsigned *tile_halo, const unsigned *nunits,
-                               const unsigned *multi_slot, const unsigned *tslot,
-                               const unsigned long long *side, const double *scale,
-                               float *tbuf) {
-    const unsigned t = blockIdx.x;
-    if (nunits[t] <= 1) return;
-    const int h = tile_halo ? (int)tile_halo[t] : 0;
-    const double inv = scale[2 * h + 1];
-    const unsigned long long *src = side + (size_t)multi_slot[t] * kTileNodes;
-    float *dst = tbuf + (size_t)tslot[t] * kTileNodes;
-    for (int l = threadIdx.x; l < kTileNodes; l += blockDim.x)
-        dst[l] = (float)((double)(long long)src[l] * inv);
+// The rows of the tiles the deposit did not write: zeros for tiles without
+// bin entries, the side-buffer sums for multi-unit tiles.  Grid-stride over
+// the tiles, one workgroup per tile.
+__global__ void __launch_bounds__(256)
+tsc_rows_k(const HaloSeg *segs, HaloSeg seg0, const unsigned *tile_halo, long long ntiles,
+           const unsigned *nunits, const unsigned *multi_slot,
+           const unsigned long long *side, const double *scale, float *den) {
+    for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const unsigned nu = nunits[t];
+        if (nu == 1) continue;
+        const int h = tile_halo ? (int)tile_halo[t] : 0;
+        const HaloSeg sg = seg_at(segs, seg0, h);
+        const TileGeom tg = tile_geom(sg.g);
+        const unsigned tl = (unsigned)(t - sg.toff);
+        if (nu == 0) {
+            write_tile_rows(den, sg, tg, tl, [](int) { return 0.0f; });
+        } else {
+            const double inv = scale[2 * h + 1];
+            const unsigned long long *src = side + (size_t)multi_slot[t] * kTileCap;
+            write_tile_rows(den, sg, tg, tl,
+                            [&](int l) { return (float)((double)(long long)src[l] * inv); });
+        }
+    }
 }
 
 // Zeroes up to five word spans in one launch (the deposit's counters).
@@ -393,87 +425,9 @@ __global__ void zero_spans_k(ZeroSpans z) {
         for (long long i = i0; i < z.n[k]; i += st) z.p[k][i] = 0u;
 }
 
-// Tiles with bin entries: slot[t] = 1 (ranked by the exclusive scan after).
-__global__ void tsc_slot_flags_k(const unsigned *tcount, long long ntiles, unsigned *slot) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < ntiles) slot[t] = tcount[t] ? 1u : 0u;
-}
-
-// The padded row-major grid from the tile-major buffer.  A block takes one
-// slab (the tile row ty, layer tz of one halo); each wave writes, for four of
-// its kT layers z, the rows y in [kT ty, kT ty + kT), which are one
-// contiguous run of floats, with consecutive stores (zeros where the tile
-// has no entries).  Tile-shaped stores into the padded grid (rows of mx
-// floats) would leave partly written lines at every tile edge, which the
-// memory system completes at a fraction of its streaming rate; the reads
-// here are aligned 16-float pieces of the tiles, four per lane in flight.
-constexpr int kMaxTilesX = 1024;  // tiles along x (mx <= 16384)
-__global__ void __launch_bounds__(256)
-tsc_detile_k(const HaloSeg *segs, HaloSeg seg0, const unsigned *tile_halo,
-             const unsigned *slab_tile, long long nslabs, const unsigned *tcount,
-             const unsigned *tslot, const float *tbuf, float *den) {
-    __shared__ int s_slot[kMaxTilesX];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    constexpr int kU = 4;  // loads in flight per lane
-    for (long long sl = blockIdx.x; sl < nslabs; sl += gridDim.x) {
-        const int h0 = slab_tile ? (int)tile_halo[slab_tile[sl]] : 0;
-        const HaloSeg sg = seg_at(segs, seg0, h0);
-        const Grid g = sg.g;
-        const TileGeom tg = tile_geom(g);
-        const long long mx = g.mx();
-        const long long t0 = slab_tile ? (long long)slab_tile[sl] : sl * tg.ntx;
-        const long long tl = t0 - sg.toff;
-        const long long ty = (tl / tg.ntx) % tg.nty, tz = tl / ((long long)tg.ntx * tg.nty);
-        __syncthreads();  // the previous unit's reads of s_slot are done
-        for (int i = threadIdx.x; i < tg.ntx; i += blockDim.x)
-            s_slot[i] = tcount[t0 + i] ? (int)tslot[t0 + i] : -1;
-        __syncthreads();
-        const int mxi = (int)mx;
-        const int nrows = (int)(g.ny - ty * kT < kT ? g.ny - ty * kT : kT);
-        const int n = nrows * mxi;  // floats of one layer's run (< 2^18)
-        const float rmx = 1.0f / (float)mxi;
-        for (int lz = wave; lz < kT; lz += 4) {
-            const long long gz = tz * kT + lz;
-            if (gz >= g.nz) break;
-            float *base = den + sg.goff + mx * (ty * kT + (long long)g.ny * gz);
-            const int lzo = kT * kT * lz;
-            // pairs of floats: mx and every grid offset are even, so an even
-            // element e and e + 1 share a row and a 16-float tile piece
-            for (int e0 = 0; e0 < n; e0 += 128 * kU) {
-                float2 v[kU];
-#pragma unroll
-                for (int j = 0; j < kU; j++) {
-                    // element e of the run: row ly = e / mx (float estimate,
-                    // exact after one correction), column x
-                    const int e = e0 + 2 * (lane + 64 * j);
-                    int ly = (int)((float)e * rmx);
-                    int x = e - ly * mxi;
-                    if (x < 0) {
-                        x += mxi;
-                        ly--;
-                    } else if (x >= mxi) {
-                        x -= mxi;
-                        ly++;
-                    }
-                    const int slot = e < n ? s_slot[x >> 4] : -1;
-                    v[j] = slot >= 0 ? *reinterpret_cast<const float2 *>(
-                                           tbuf + (size_t)slot * kTileNodes +
-                                           (unsigned)((x & (kT - 1)) + kT * ly + lzo))
-                                     : make_float2(0.0f, 0.0f);
-                }
-#pragma unroll
-                for (int j = 0; j < kU; j++) {
-                    const int e = e0 + 2 * (lane + 64 * j);
-                    if (e < n) *reinterpret_cast<float2 *>(base + e) = v[j];
-                }
-            }
-        }
-    }
-}
-
 // Zero the side slots actually used (their count is known on device only).
 __global__ void fill_side_k(unsigned long long *p, const unsigned *nmulti) {
-    long long n = (long long)(*nmulti) * kTileNodes;
+    long long n = (long long)(*nmulti) * kTileCap;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x)
         p[i] = 0ull;
@@ -782,11 +736,10 @@ template <class P>
 hipError_t deposit_impl(const P *x, const P *y, const P *z, const P *m, const int *type,
                         int ptype, int nhalo, const HaloSeg *segs, const HaloSeg &seg0,
                         const PChunk *chunks, long long nchunks, const unsigned *tile_halo,
-                        long long ntiles, long long max_halo_tiles, const unsigned *slab_tile,
-                        long long nslabs, long long np, float *den, const Schedule &sc,
-                        Workspace &ws, hipStream_t s) {
+                        long long ntiles, long long max_halo_tiles, int max_entries, long long np,
+                        float *den, const Schedule &sc, Workspace &ws, hipStream_t s) {
     const unsigned unit = (unsigned)sc.tsc_unit_entries;
-    WsScope scope(ws);  // bins, tile-major buffer and side buffer end with the deposit
+    WsScope scope(ws);  // bins and side buffer end with the deposit
     int use_lds = max_halo_tiles <= kLdsHistMax ? 1 : 0;
     unsigned *tcount = ws.alloc_n<unsigned>(ntiles);
     unsigned *tcursor = ws.alloc_n<unsigned>(ntiles);
@@ -814,8 +767,8 @@ hipError_t deposit_impl(const P *x, const P *y, const P *z, const P *m, const in
             x, y, z, m, type, ptype, segs, seg0, chunks, tcount, nsel, maxm, ws.status(),
             use_lds);
     PGALF_HIP_TRY(exclusive_scan_u32(tcount, tstart, ntiles, &misc[1], ws, s));
-    // the bin list can hold at most 8 entries per particle
-    BinRec<P> *bins = ws.alloc_n<BinRec<P>>((size_t)np * 8 + 1);
+    // at most max_entries tiles per particle (2 x 2 unless a tile is one row)
+    BinRec<P> *bins = ws.alloc_n<BinRec<P>>((size_t)np * max_entries + 1);
     if (!bins) return hipErrorOutOfMemory;
     if (nchunks > 0)
         tsc_scatter_k<P><<<dim3((unsigned)nchunks), dim3(kBlock), 2 * lds_count, s>>>(
@@ -824,36 +777,27 @@ hipError_t deposit_impl(const P *x, const P *y, const P *z, const P *m, const in
     tsc_units_k<<<tb, 256, 0, s>>>(tcount, ntiles, unit, nunits, multi);
     PGALF_HIP_TRY(exclusive_scan_u32(nunits, ustart, ntiles, &misc[2], ws, s));
     PGALF_HIP_TRY(exclusive_scan_u32(multi, multi, ntiles, &misc[3], ws, s));
-    // unit -> tile map: the unit count is bounded by ntiles + 8*np/unit
-    size_t max_units = (size_t)ntiles + (size_t)(8 * np) / unit + 1;
+    // unit -> tile map: the unit count is bounded by ntiles + entries/unit
+    const size_t max_ent = (size_t)np * max_entries;
+    size_t max_units = (size_t)ntiles + max_ent / unit + 1;
     unsigned *unit_tile = ws.alloc_n<unsigned>(max_units);
     // multi-unit side buffer: a tile with > unit entries needs one slot;
-    // there are at most 8*np/unit of them.
-    size_t max_multi = (size_t)(8 * np) / unit + 1;
+    // there are at most entries/unit of them.
+    size_t max_multi = max_ent / unit + 1;
     if ((long long)max_multi > ntiles) max_multi = (size_t)ntiles;
-    unsigned long long *side = ws.alloc_n<unsigned long long>(max_multi * kTileNodes);
+    unsigned long long *side = ws.alloc_n<unsigned long long>(max_multi * kTileCap);
     if (!unit_tile || !side) return hipErrorOutOfMemory;
     tsc_unit_map_k<<<tb, 256, 0, s>>>(ustart, nunits, ntiles, unit_tile);
     tsc_scale_k<<<(unsigned)((nhalo + 255) / 256), 256, 0, s>>>(nsel, maxm, nhalo, scale);
     const int ncu = cu_count(ws.device());
     fill_side_k<<<(unsigned)ncu * 4, 256, 0, s>>>(side, &misc[3]);
-    // tile-major buffer: one kTileNodes slot per tile with entries (at most
-    // one per bin entry)
-    unsigned *tslot = ws.alloc_n<unsigned>(ntiles);
-    const long long ncap = std::max(1LL, std::min(ntiles, 8 * np));
-    float *tbuf = ws.alloc_n<float>((size_t)ncap * kTileNodes);
-    if (!tslot || !tbuf) return hipErrorOutOfMemory;
-    tsc_slot_flags_k<<<tb, 256, 0, s>>>(tcount, ntiles, tslot);
-    PGALF_HIP_TRY(exclusive_scan_u32(tslot, tslot, ntiles, &misc[4], ws, s));
     tsc_deposit_k<P><<<(unsigned)ncu * 4, kBlock, 0, s>>>(
-        x, y, z, m, segs, seg0, tile_halo, tstart, tcount, ustart, nunits, unit_tile, &misc[2],
-        multi, bins, scale, tslot, tbuf, side, unit);
-    tsc_finalize_k<<<(unsigned)ntiles, 256, 0, s>>>(tile_halo, nunits, multi, tslot, side, scale,
-                                                    tbuf);
-    long long db = nslabs < (long long)ncu * 8 ? nslabs : (long long)ncu * 8;
-    if (db > 0)
-        tsc_detile_k<<<(unsigned)db, 256, 0, s>>>(segs, seg0, tile_halo, slab_tile, nslabs, tcount,
-                                                  tslot, tbuf, den);
+        segs, seg0, tile_halo, tstart, tcount, ustart, nunits, unit_tile, &misc[2], multi, bins,
+        scale, den, side, unit);
+    const long long rb = ntiles < (long long)ncu * 8 ? ntiles : (long long)ncu * 8;
+    if (rb > 0)
+        tsc_rows_k<<<(unsigned)rb, 256, 0, s>>>(segs, seg0, tile_halo, ntiles, nunits, multi, side,
+                                                scale, den);
     return hipGetLastError();
 }
 
@@ -861,28 +805,23 @@ hipError_t deposit_impl(const P *x, const P *y, const P *z, const P *m, const in
 
 long long tsc_tile_count(const Grid &g) { return tile_geom(g).ntiles(); }
 
-void tsc_slab_tiles(const Grid &g, long long toff, std::vector<unsigned> &out) {
-    const TileGeom tg = tile_geom(g);
-    for (int tz = 0; tz < tg.ntz; tz++)
-        for (int ty = 0; ty < tg.nty; ty++)
-            out.push_back((unsigned)(toff + (long long)tg.ntx * (ty + (long long)tg.nty * tz)));
-}
+bool tsc_grid_fits(const Grid &g) { return g.mx() <= kTileCap; }
 
-int tsc_tiles_x(const Grid &g) { return tile_geom(g).ntx; }
-int tsc_max_tiles_x() { return kMaxTilesX; }
+int tsc_tile_entries(const Grid &g) {
+    const TileGeom tg = tile_geom(g);
+    return (tg.TY == 1 ? 3 : 2) * (tg.TZ == 1 ? 3 : 2);
+}
 
 template <class P>
 hipError_t tsc_deposit(const P *x, const P *y, const P *z, const P *m, const int *type,
                        long long np, int ptype, const Grid &g, float *den, Workspace &ws,
                        hipStream_t s) {
     const HaloSeg seg0 = single_seg(g, np);
+    if (!tsc_grid_fits(g)) return hipErrorInvalidValue;
     const long long ntiles = tile_geom(g).ntiles();
     const long long nchunks = (np + kPerBlock - 1) / kPerBlock;
-    const TileGeom tg = tile_geom(g);
-    if (tg.ntx > kMaxTilesX) return hipErrorInvalidValue;
     return deposit_impl<P>(x, y, z, m, type, ptype, 1, nullptr, seg0, nullptr, nchunks, nullptr,
-                           ntiles, ntiles, nullptr, (long long)tg.nty * tg.ntz, np, den,
-                           schedule_for(np), ws, s);
+                           ntiles, ntiles, tsc_tile_entries(g), np, den, schedule_for(np), ws, s);
 }
 
 template <class P>
@@ -892,8 +831,8 @@ hipError_t tsc_deposit_batch(const P *x, const P *y, const P *z, const P *m, con
                   "batch chunks are deposit blocks; fine chunks are read-back blocks");
     const HaloSeg seg0{};
     return deposit_impl<P>(x, y, z, m, nullptr, PGALF_TYPE_ALL, b.nhalo, b.segs, seg0, b.chunks,
-                           b.nchunks, b.tile_halo, b.ntiles, b.max_halo_tiles, b.slab_tile,
-                           b.nslabs, b.np, den, b.sched, ws, s);
+                           b.nchunks, b.tile_halo, b.ntiles, b.max_halo_tiles,
+                           b.max_tile_entries, b.np, den, b.sched, ws, s);
 }
 
 template <class P>
