@@ -531,7 +531,7 @@ smooth_fft2_k(float *g, AxisPass ap, const double2 *tw, const double *H,
 }
 
 // ---- direct circular convolution for short lines ------------------------------
-// For n a multiple of 8 up to kDirectMax, an axis pass is the circular
+// For n up to kDirectMax, an axis pass is the circular
 // convolution out[i] = sum_j g[(i - j) mod n] in[j] with the exact periodic
 // kernel g[m] = (1/n) sum_k f(k) cos(2 pi k m / n) of the spectral factor f
 // (identical in exact arithmetic to the FFT -> multiply -> inverse FFT of the
@@ -561,8 +561,7 @@ struct DirectPass {
 template <int N>
 __device__ inline void conv_line(const float (&v)[N], const Taps &taps, float *out,
                                  long long step) {
-    constexpr int kB = 8;
-    static_assert(N % kB == 0, "whole blocks of outputs");
+    constexpr int kB = 8;  // the last block of outputs may be partial
 #pragma unroll
     for (int i0 = 0; i0 < N; i0 += kB) {
         double acc[kB];
@@ -573,12 +572,15 @@ __device__ inline void conv_line(const float (&v)[N], const Taps &taps, float *o
             const double vj = (double)v[j];
 #pragma unroll
             for (int b = 0; b < kB; b++) {
-                const int d = (i0 + b - j + N) % N;
-                acc[b] = fma(taps.g[d <= N / 2 ? d : N - d], vj, acc[b]);
+                if (i0 + b < N) {
+                    const int d = (i0 + b - j + N) % N;
+                    acc[b] = fma(taps.g[d <= N / 2 ? d : N - d], vj, acc[b]);
+                }
             }
         }
 #pragma unroll
-        for (int b = 0; b < kB; b++) out[(long long)(i0 + b) * step] = (float)acc[b];
+        for (int b = 0; b < kB; b++)
+            if (i0 + b < N) out[(long long)(i0 + b) * step] = (float)acc[b];
     }
 }
 
@@ -616,8 +618,8 @@ __global__ void __launch_bounds__(BLK)
 smooth_direct_x_k(float *grid, DirectPass dp, Taps taps, const long long *goffs,
                   const GridLines *gl) {
     extern __shared__ float tile[];
-    // row width mx = 2 (n/2 + 1) = N + 2 for even N (checked by the host)
-    constexpr int W = N + 2, WS = N + 3;
+    // row width mx = 2 (N/2 + 1) (checked by the host)
+    constexpr int W = 2 * (N / 2 + 1), WS = W + 1;
     const long long row0 = (long long)blockIdx.x * BLK;
     float *g0 = grid + (goffs ? goffs[blockIdx.y] : 0LL);
     if (gl) {  // this grid's rows of a pass over several shapes
@@ -723,7 +725,7 @@ struct AxisPlan {
     int n = 0, L = 0;
     double2 *tw = nullptr;
     double *H = nullptr;
-    bool direct = false;  // n a multiple of 8 <= kDirectMax: direct convolution
+    bool direct = false;  // n <= kDirectMax: direct convolution possible
     Taps taps{};
 };
 
@@ -767,7 +769,7 @@ hipError_t get_plan(int n, double expfac, AxisPlan &out) {
     for (int k = 0; k < n; k++) f[k] = axis_factor(k, n, expfac);
     std::vector<double> H(L);
     // periodic kernel g[m] = (1/n) sum_k f(k) cos(2 pi k m / n)
-    const bool direct = n % 8 == 0 && n <= kDirectMax;
+    const bool direct = n <= kDirectMax;
     std::vector<long double> gk;
     if (direct || !pow2) {
         gk.resize(n);
@@ -930,7 +932,7 @@ hipError_t run_pass(float *g, const AxisPlan &pl, long long sa, int nb, long lon
         constexpr int kMaxY = 65535;
         // x pass: rows staged through LDS, 256 rows per block
         const bool rows = sa == 1;
-        if (rows && padw != pl.n + 2) return hipErrorInvalidValue;
+        if (rows && padw != 2 * (pl.n / 2 + 1)) return hipErrorInvalidValue;
         const int blk = 256;
         const long long lines = mix ? mix->max_lines : dp.nlines;
         const unsigned gx = (unsigned)((lines + blk - 1) / blk);
@@ -952,14 +954,28 @@ hipError_t run_pass(float *g, const AxisPlan &pl, long long sa, int nb, long lon
         smooth_direct_k<NN><<<grid, 256, 0, s>>>(g, dp, pl.taps, gz, gl);                      \
         break;
             switch (pl.n) {
-                PGALF_DIRECT_ROWS(8)
-                PGALF_DIRECT_ROWS(16)
-                PGALF_DIRECT_ROWS(24)
-                PGALF_DIRECT_ROWS(32)
-                PGALF_DIRECT_STRIDED(40)
-                PGALF_DIRECT_STRIDED(48)
-                PGALF_DIRECT_STRIDED(56)
-                PGALF_DIRECT_STRIDED(64)
+                PGALF_DIRECT_ROWS(1) PGALF_DIRECT_ROWS(2) PGALF_DIRECT_ROWS(3)
+                PGALF_DIRECT_ROWS(4) PGALF_DIRECT_ROWS(5) PGALF_DIRECT_ROWS(6)
+                PGALF_DIRECT_ROWS(7) PGALF_DIRECT_ROWS(8) PGALF_DIRECT_ROWS(9)
+                PGALF_DIRECT_ROWS(10) PGALF_DIRECT_ROWS(11) PGALF_DIRECT_ROWS(12)
+                PGALF_DIRECT_ROWS(13) PGALF_DIRECT_ROWS(14) PGALF_DIRECT_ROWS(15)
+                PGALF_DIRECT_ROWS(16) PGALF_DIRECT_ROWS(17) PGALF_DIRECT_ROWS(18)
+                PGALF_DIRECT_ROWS(19) PGALF_DIRECT_ROWS(20) PGALF_DIRECT_ROWS(21)
+                PGALF_DIRECT_ROWS(22) PGALF_DIRECT_ROWS(23) PGALF_DIRECT_ROWS(24)
+                PGALF_DIRECT_ROWS(25) PGALF_DIRECT_ROWS(26) PGALF_DIRECT_ROWS(27)
+                PGALF_DIRECT_ROWS(28) PGALF_DIRECT_ROWS(29) PGALF_DIRECT_ROWS(30)
+                PGALF_DIRECT_ROWS(31) PGALF_DIRECT_ROWS(32)
+                PGALF_DIRECT_STRIDED(33) PGALF_DIRECT_STRIDED(34) PGALF_DIRECT_STRIDED(35)
+                PGALF_DIRECT_STRIDED(36) PGALF_DIRECT_STRIDED(37) PGALF_DIRECT_STRIDED(38)
+                PGALF_DIRECT_STRIDED(39) PGALF_DIRECT_STRIDED(40) PGALF_DIRECT_STRIDED(41)
+                PGALF_DIRECT_STRIDED(42) PGALF_DIRECT_STRIDED(43) PGALF_DIRECT_STRIDED(44)
+                PGALF_DIRECT_STRIDED(45) PGALF_DIRECT_STRIDED(46) PGALF_DIRECT_STRIDED(47)
+                PGALF_DIRECT_STRIDED(48) PGALF_DIRECT_STRIDED(49) PGALF_DIRECT_STRIDED(50)
+                PGALF_DIRECT_STRIDED(51) PGALF_DIRECT_STRIDED(52) PGALF_DIRECT_STRIDED(53)
+                PGALF_DIRECT_STRIDED(54) PGALF_DIRECT_STRIDED(55) PGALF_DIRECT_STRIDED(56)
+                PGALF_DIRECT_STRIDED(57) PGALF_DIRECT_STRIDED(58) PGALF_DIRECT_STRIDED(59)
+                PGALF_DIRECT_STRIDED(60) PGALF_DIRECT_STRIDED(61) PGALF_DIRECT_STRIDED(62)
+                PGALF_DIRECT_STRIDED(63) PGALF_DIRECT_STRIDED(64)
             default: return hipErrorInvalidValue;
             }
 #undef PGALF_DIRECT_ROWS
@@ -1033,7 +1049,8 @@ hipError_t axis_pass(float *den, int axis, const AxisPlan &pl, int nx, int ny, i
 
 // Small cubes whose three passes are all direct: one LDS-resident kernel.
 bool cube_shape(int nx, int ny, int nz, const AxisPlan &px, int dmax) {
-    return PGALF_SMOOTH_CUBE && nx == ny && ny == nz && px.direct && nx <= 32 && nx <= dmax;
+    return PGALF_SMOOTH_CUBE && nx == ny && ny == nz && nx % 8 == 0 && px.direct && nx <= 32 &&
+           nx <= dmax;
 }
 hipError_t cube_pass(float *den, int n, const Taps &taps, hipStream_t s, const long long *goffs,
                      int ngrids) {

@@ -153,7 +153,7 @@ def tune_bin(nmax, n, reps=5, seed=0, log=print):
 
 def tune_smoothing(nhalo=2000, reps=5, log=print):
     """smooth_direct_max timed on a batch of small halos (grids of 32..64
-    cells per side, every multiple of 8)."""
+    cells per side in steps of 8)."""
     import torch
     pos, mass, off, _, _ = halos.catalogue_vec(5, nhalo=nhalo, nmin=64, nmax=8192)
     # grids of every length the threshold can move between the two methods

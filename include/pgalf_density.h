@@ -326,9 +326,8 @@ typedef struct pgalf_schedule {
     int tsc_unit_entries;  /* bin entries per deposit work unit, 256..65536 (4096) */
 } pgalf_schedule_t;
 /* Replace the table (nbins 0: the defaults) and the global smoothing
- * threshold: lines of a length that is a multiple of 8 up to
- * smooth_direct_max (0..64, default 56; rows also <= 32) are smoothed by
- * direct convolution, longer ones by FFT. */
+ * threshold: lines up to smooth_direct_max long (0..64, default 56; rows
+ * also <= 32) are smoothed by direct convolution, longer ones by FFT. */
 int pgalf_set_schedule(const pgalf_schedule_t *bins, int nbins, int smooth_direct_max);
 /* The schedule a workload of n particles gets, and the smoothing threshold. */
 int pgalf_get_schedule(long long n, pgalf_schedule_t *out, int *smooth_direct_max);
