@@ -542,6 +542,7 @@ smooth_fft2_k(float *g, AxisPass ap, const double2 *tw, const double *H,
 // memory speed, and its n FMAs per point match the FFT's flop count at
 // these lengths.
 constexpr int kDirectMax = 64;
+constexpr int kDirectRowsMax = 56;  // x rows staged through LDS (rows kernel instances)
 struct Taps {
     double g[kDirectMax / 2 + 1];
 };
@@ -891,7 +892,7 @@ bool fft2_length(const AxisPlan &pl, long long gcells) {
            gcells > 0 && gcells < (1LL << 29);
 }
 bool direct_length(const AxisPlan &pl, bool rows, int dmax) {
-    return pl.direct && pl.n <= dmax && (!rows || pl.n <= 32);
+    return pl.direct && pl.n <= dmax && (!rows || pl.n <= kDirectRowsMax);
 }
 
 hipError_t run_pass(float *g, const AxisPlan &pl, long long sa, int nb, long long sb, int nc,
@@ -964,15 +965,15 @@ hipError_t run_pass(float *g, const AxisPlan &pl, long long sa, int nb, long lon
                 PGALF_DIRECT_ROWS(22) PGALF_DIRECT_ROWS(23) PGALF_DIRECT_ROWS(24)
                 PGALF_DIRECT_ROWS(25) PGALF_DIRECT_ROWS(26) PGALF_DIRECT_ROWS(27)
                 PGALF_DIRECT_ROWS(28) PGALF_DIRECT_ROWS(29) PGALF_DIRECT_ROWS(30)
-                PGALF_DIRECT_ROWS(31) PGALF_DIRECT_ROWS(32)
-                PGALF_DIRECT_STRIDED(33) PGALF_DIRECT_STRIDED(34) PGALF_DIRECT_STRIDED(35)
-                PGALF_DIRECT_STRIDED(36) PGALF_DIRECT_STRIDED(37) PGALF_DIRECT_STRIDED(38)
-                PGALF_DIRECT_STRIDED(39) PGALF_DIRECT_STRIDED(40) PGALF_DIRECT_STRIDED(41)
-                PGALF_DIRECT_STRIDED(42) PGALF_DIRECT_STRIDED(43) PGALF_DIRECT_STRIDED(44)
-                PGALF_DIRECT_STRIDED(45) PGALF_DIRECT_STRIDED(46) PGALF_DIRECT_STRIDED(47)
-                PGALF_DIRECT_STRIDED(48) PGALF_DIRECT_STRIDED(49) PGALF_DIRECT_STRIDED(50)
-                PGALF_DIRECT_STRIDED(51) PGALF_DIRECT_STRIDED(52) PGALF_DIRECT_STRIDED(53)
-                PGALF_DIRECT_STRIDED(54) PGALF_DIRECT_STRIDED(55) PGALF_DIRECT_STRIDED(56)
+                PGALF_DIRECT_ROWS(31) PGALF_DIRECT_ROWS(32) PGALF_DIRECT_ROWS(33)
+                PGALF_DIRECT_ROWS(34) PGALF_DIRECT_ROWS(35) PGALF_DIRECT_ROWS(36)
+                PGALF_DIRECT_ROWS(37) PGALF_DIRECT_ROWS(38) PGALF_DIRECT_ROWS(39)
+                PGALF_DIRECT_ROWS(40) PGALF_DIRECT_ROWS(41) PGALF_DIRECT_ROWS(42)
+                PGALF_DIRECT_ROWS(43) PGALF_DIRECT_ROWS(44) PGALF_DIRECT_ROWS(45)
+                PGALF_DIRECT_ROWS(46) PGALF_DIRECT_ROWS(47) PGALF_DIRECT_ROWS(48)
+                PGALF_DIRECT_ROWS(49) PGALF_DIRECT_ROWS(50) PGALF_DIRECT_ROWS(51)
+                PGALF_DIRECT_ROWS(52) PGALF_DIRECT_ROWS(53) PGALF_DIRECT_ROWS(54)
+                PGALF_DIRECT_ROWS(55) PGALF_DIRECT_ROWS(56)
                 PGALF_DIRECT_STRIDED(57) PGALF_DIRECT_STRIDED(58) PGALF_DIRECT_STRIDED(59)
                 PGALF_DIRECT_STRIDED(60) PGALF_DIRECT_STRIDED(61) PGALF_DIRECT_STRIDED(62)
                 PGALF_DIRECT_STRIDED(63) PGALF_DIRECT_STRIDED(64)

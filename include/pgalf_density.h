@@ -327,7 +327,7 @@ typedef struct pgalf_schedule {
 } pgalf_schedule_t;
 /* Replace the table (nbins 0: the defaults) and the global smoothing
  * threshold: lines up to smooth_direct_max long (0..64, default 56; rows
- * also <= 32) are smoothed by direct convolution, longer ones by FFT. */
+ * also <= 56) are smoothed by direct convolution, longer ones by FFT. */
 int pgalf_set_schedule(const pgalf_schedule_t *bins, int nbins, int smooth_direct_max);
 /* The schedule a workload of n particles gets, and the smoothing threshold. */
 int pgalf_get_schedule(long long n, pgalf_schedule_t *out, int *smooth_direct_max);
