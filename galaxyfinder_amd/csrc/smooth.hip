@@ -10,8 +10,9 @@
 // lines packed as one complex line, with one of three kernels:
 //   * lines of 64, 128, 256: a four-step FFT with each thread's DFT-8/16 in
 //     FP64 registers and two LDS transposes per line (smooth_fft2_k);
-//   * short lines (multiples of 8 up to 56 strided, 32 as rows): direct
-//     circular convolution with the periodic kernel (smooth_direct_k);
+//   * short lines (up to the direct threshold, 56 by default): direct
+//     circular convolution with the periodic kernel (smooth_direct_k,
+//     smooth_direct_x_k for rows; small cubes in one LDS-resident kernel);
 //   * other lengths: a workgroup loads a batch of lines into LDS, runs a
 //     Stockham radix-4/2 FFT, multiplies by the real spectral factor H, runs
 //     the inverse FFT and stores the lines back (smooth_axis_k).
@@ -815,11 +816,11 @@ hipError_t get_plan(int n, double expfac, AxisPlan &out) {
     return hipSuccess;
 }
 
-// Kernel choice per line length: the direct convolution for lengths that
-// are multiples of 8 up to the schedule's smooth_direct_max (rows: up to 32
-// as well; longer rows need ~255 VGPRs there), else the register FFT for
-// n = 64, 128, 256, else the Stockham kernel.  The default 56 takes 8..56
-// directly and 64 by the register FFT, which measured faster in the batch.
+// Kernel choice per line length: the direct convolution for lengths up to
+// the schedule's smooth_direct_max (rows: up to kDirectRowsMax as well),
+// else the register FFT for n = 64, 128, 256, else the Stockham kernel.
+// The default 56 takes 1..56 directly and 64 by the register FFT, which
+// measured faster in the batch.
 #ifndef PGALF_SMOOTH_FFT2
 #define PGALF_SMOOTH_FFT2 1
 #endif
