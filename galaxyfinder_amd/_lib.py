@@ -32,6 +32,12 @@ class pgalf_schedule_t(ctypes.Structure):
                 ("tsc_unit_entries", _c_i)]
 
 
+class pgalf_shells_t(ctypes.Structure):
+    """pgalf_shells_t of include/pgalf_density.h (CSR shells)."""
+    _fields_ = [("part_ptr", _c_p), ("part", _c_p), ("core_ptr", _c_p), ("cores", _c_p),
+                ("maxshell", _c_ll), ("maxcores", _c_ll), ("nshell", _c_ll)]
+
+
 # name -> (restype, argtypes); every symbol of include/pgalf_density.h and
 # include/pgalf_catalog.h
 SIGNATURES = {
@@ -75,6 +81,15 @@ SIGNATURES = {
     "pgalf_merging_peak": (_c_i, [_c_p, _c_i, _c_p, _c_i, _c_i]),
     "pgalf_dev_merging_peak": (_c_i, [_c_p, _c_p, _c_p, _c_ll, _c_p, _c_p, _c_i, _c_f, _c_p,
                                       _c_p, _c_p, _c_p]),
+    "pgalf_find_core_density": (_c_i, [_c_p, _c_i, _c_p, _c_i, _c_p, _c_i, _c_p, _c_p, _c_p]),
+    "pgalf_shell_thresholds": (_c_i, [_c_p, _c_i, _c_p, _c_i, _c_p]),
+    "pgalf_shell_particles_peaks": (_c_i, [_c_i, _c_p, _c_i, _c_p, _c_i, _c_p, _c_p, _c_p, _c_p,
+                                           _c_i, _c_p]),
+    "pgalf_dev_find_core_density": (_c_i, [_c_p, _c_ll, _c_p, _c_i, _c_p, _c_p, _c_p, _c_i,
+                                           _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p]),
+    "pgalf_dev_shell_thresholds": (_c_i, [_c_p, _c_ll, _c_p, _c_i, _c_p, _c_p, _c_p]),
+    "pgalf_dev_shell_particles_peaks": (_c_i, [_c_p, _c_ll, _c_p, _c_i, _c_p, _c_p, _c_i, _c_p,
+                                               _c_i, _c_p, _c_p, _c_p, _c_p]),
     "pgalf_batch_create": (_c_p, [_c_i, _c_p, _c_p, _c_p, _c_i]),
     "pgalf_batch_grid_offset": (_c_ll, [_c_p, _c_i]),
     "pgalf_batch_run": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i,

@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpgalf_density.so")
 SOURCES = ["scan_sort.hip", "tsc.hip", "smooth.hip", "knn.hip", "peaks.hip", "capi.hip",
-           "batch.hip", "catalog.hip"]
+           "batch.hip", "catalog.hip", "watershed.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PGALF_ARCH", "gfx950")
 

@@ -1073,6 +1073,304 @@ extern "C" int pgalf_dev_merging_peak(const float *x, const float *y, const floa
     return e == hipSuccess ? ok() : hip_err(e, where);
 }
 
+// ---- watershed of subhalo_den ------------------------------------------------------------
+namespace {
+
+int shell_err(int err, const char *where) {
+    if (err == 2)
+        return set_err(PGALF_ERR_ARG,
+                       "%s: a root's own core was never enclosed (two cores share a peak)",
+                       where);
+    if (err == 1)
+        return set_err(PGALF_ERR_CORE_OVERFLOW, "%s: more shells than maxshell", where);
+    return set_err(PGALF_ERR_CORE_OVERFLOW, "%s: more shell core entries than maxcores", where);
+}
+
+template <class T>
+hipError_t upload(T **d, const T *h, size_t n, Workspace &ws, hipStream_t s) {
+    *d = ws.alloc_n<T>(n > 0 ? n : 1);
+    if (!*d) return hipErrorOutOfMemory;
+    return n > 0 ? hipMemcpyAsync(*d, h, sizeof(T) * n, hipMemcpyHostToDevice, s) : hipSuccess;
+}
+
+}  // namespace
+
+extern "C" int pgalf_find_core_density(pgalf_particle_t *bp, int np, long long *neighbor,
+                                       int NumNeighbor, pgalf_core_t *core, int numcore,
+                                       const float *den, int *haloid, unsigned char *flag) {
+    const char *where = "pgalf_find_core_density";
+    if (np < 0 || numcore < 0 || NumNeighbor < 1 || NumNeighbor > 32 ||
+        (np > 0 && (!bp || !neighbor || !den || !haloid || !flag)) || (numcore > 0 && !core)) {
+        set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
+        return -1;
+    }
+    std::vector<int> peak(numcore);
+    for (int c = 0; c < numcore; c++) {
+        peak[c] = core[c].peak;
+        if (peak[c] < 0 || peak[c] >= np) {
+            set_err(PGALF_ERR_ARG, "%s: core %d has peak %d outside [0, %d)", where, c, peak[c],
+                    np);
+            return -1;
+        }
+    }
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    hipStream_t s = nullptr;
+    Workspace &ws = workspace_for_current_device();
+    WsCall call(ws, s);
+    hipError_t e = call.e;
+    DeviceParticles d{};
+    float *dden = nullptr;
+    long long *dnb = nullptr;
+    int *dpeak = nullptr;
+    if (e == hipSuccess && np > 0) e = upload_particles(bp, np, ws, s, d);
+    if (e == hipSuccess) e = upload(&dden, den, np, ws, s);
+    if (e == hipSuccess) e = upload(&dnb, neighbor, (size_t)np * NumNeighbor, ws, s);
+    if (e == hipSuccess) e = upload(&dpeak, peak.data(), numcore, ws, s);
+    const int nc1 = numcore > 0 ? numcore : 1;
+    CoreOut o;
+    o.coredensity = ws.alloc_n<float>(nc1);
+    o.nummem = ws.alloc_n<int>(nc1);
+    o.numstar = ws.alloc_n<int>(nc1);
+    o.starmass = ws.alloc_n<float>(nc1);
+    o.haloid = ws.alloc_n<int>(np > 0 ? np : 1);
+    o.flag = ws.alloc_n<unsigned char>(np > 0 ? np : 1);
+    if (e == hipSuccess && (!o.coredensity || !o.nummem || !o.numstar || !o.starmass ||
+                            !o.haloid || !o.flag))
+        e = hipErrorOutOfMemory;
+    int fail = 0;
+    if (e == hipSuccess)
+        e = find_core_density(dden, np, dnb, NumNeighbor, d.type, d.m, dpeak, numcore, o, &fail,
+                              ws, s);
+    std::vector<float> cd(nc1), sm(nc1);
+    std::vector<int> nm(nc1), ns(nc1);
+    if (e == hipSuccess && numcore > 0) {
+        e = hipMemcpyAsync(cd.data(), o.coredensity, sizeof(float) * numcore,
+                           hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(sm.data(), o.starmass, sizeof(float) * numcore,
+                               hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(nm.data(), o.nummem, sizeof(int) * numcore, hipMemcpyDeviceToHost,
+                               s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(ns.data(), o.numstar, sizeof(int) * numcore,
+                               hipMemcpyDeviceToHost, s);
+    }
+    if (e == hipSuccess && np > 0) {
+        e = hipMemcpyAsync(haloid, o.haloid, sizeof(int) * np, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(flag, o.flag, np, hipMemcpyDeviceToHost, s);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        hip_err(e, where);
+        return -1;
+    }
+    if (consume_status(ws, s, where) != PGALF_OK) return -1;
+    if (fail) {
+        set_err(PGALF_ERR_ARG, "%s: a core-density search did not converge", where);
+        return -1;
+    }
+    for (int c = 0; c < numcore; c++) {
+        core[c].coredensity = cd[c];
+        core[c].nummem = nm[c];
+        core[c].numstar = ns[c];
+        core[c].starmass = sm[c];
+    }
+    ok();
+    return numcore;
+}
+
+extern "C" int pgalf_shell_thresholds(const float *den, int np, const pgalf_core_t *core,
+                                      int numcore, float *thr) {
+    const char *where = "pgalf_shell_thresholds";
+    if (np < 0 || numcore < 0 || (np > 0 && !den) || (numcore > 0 && !core) || !thr) {
+        set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
+        return -1;
+    }
+    std::vector<float> cd(numcore);
+    for (int c = 0; c < numcore; c++) cd[c] = core[c].coredensity;
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    hipStream_t s = nullptr;
+    Workspace &ws = workspace_for_current_device();
+    WsCall call(ws, s);
+    hipError_t e = call.e;
+    float *dden = nullptr, *dcd = nullptr;
+    if (e == hipSuccess) e = upload(&dden, den, np, ws, s);
+    if (e == hipSuccess) e = upload(&dcd, cd.data(), numcore, ws, s);
+    int n = 0;
+    if (e == hipSuccess) e = shell_thresholds(dden, np, dcd, numcore, thr, &n, ws, s);
+    if (e != hipSuccess) {
+        hip_err(e, where);
+        return -1;
+    }
+    ok();
+    return n;
+}
+
+namespace {
+
+int shells_run(const float *dden, long long np, const long long *dnb, int k, const int *dpeak,
+               const float *dcd, int numcore, const float *thr, int nthr, unsigned char *dflag,
+               int *dhalo, pgalf_shells_t *sh, bool host_shells, Workspace &ws, hipStream_t s,
+               const char *where) {
+    ShellOut o;
+    o.maxshell = sh->maxshell;
+    o.maxcores = sh->maxcores;
+    hipError_t e = hipSuccess;
+    if (host_shells) {
+        o.part_ptr = ws.alloc_n<long long>(sh->maxshell + 1);
+        o.core_ptr = ws.alloc_n<long long>(sh->maxshell + 1);
+        o.part = ws.alloc_n<int>(np > 0 ? np : 1);
+        o.cores = ws.alloc_n<int>(sh->maxcores > 0 ? sh->maxcores : 1);
+        if (!o.part_ptr || !o.core_ptr || !o.part || !o.cores) e = hipErrorOutOfMemory;
+    } else {
+        o.part_ptr = sh->part_ptr;
+        o.core_ptr = sh->core_ptr;
+        o.part = sh->part;
+        o.cores = sh->cores;
+    }
+    long long nshell = 0;
+    int err = 0;
+    if (e == hipSuccess)
+        e = core_shells(dden, np, dnb, k, dpeak, dcd, numcore, thr, nthr, dflag, dhalo, o,
+                        &nshell, &err, ws, s);
+    if (e == hipSuccess && err == 0 && host_shells) {
+        e = hipMemcpyAsync(sh->part_ptr, o.part_ptr, sizeof(long long) * (nshell + 1),
+                           hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(sh->core_ptr, o.core_ptr, sizeof(long long) * (nshell + 1),
+                               hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e == hipSuccess && sh->part_ptr[nshell] > 0)
+            e = hipMemcpyAsync(sh->part, o.part, sizeof(int) * sh->part_ptr[nshell],
+                               hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess && sh->core_ptr[nshell] > 0)
+            e = hipMemcpyAsync(sh->cores, o.cores, sizeof(int) * sh->core_ptr[nshell],
+                               hipMemcpyDeviceToHost, s);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_err(e, where);
+    const int st = consume_status(ws, s, where);
+    if (st != PGALF_OK) return st;
+    if (err) return shell_err(err, where);
+    sh->nshell = nshell;
+    return ok();
+}
+
+}  // namespace
+
+extern "C" int pgalf_shell_particles_peaks(int np, long long *neighbor, int NumNeighbor,
+                                           pgalf_core_t *core, int numcore, const float *den,
+                                           int *haloid, unsigned char *flag, const float *thr,
+                                           int nthr, pgalf_shells_t *shells) {
+    const char *where = "pgalf_shell_particles_peaks";
+    if (np < 0 || numcore < 0 || NumNeighbor < 1 || NumNeighbor > 32 || nthr < 0 ||
+        (nthr > 0 && !thr) || !shells || !shells->part_ptr || !shells->core_ptr ||
+        !shells->part || !shells->cores || shells->maxshell < 0 || shells->maxcores < 0 ||
+        (np > 0 && (!neighbor || !den || !haloid || !flag)) || (numcore > 0 && !core))
+        return set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
+    std::vector<int> peak(numcore);
+    std::vector<float> cd(numcore);
+    for (int c = 0; c < numcore; c++) {
+        peak[c] = core[c].peak;
+        cd[c] = core[c].coredensity;
+        if (peak[c] < 0 || peak[c] >= np)
+            return set_err(PGALF_ERR_ARG, "%s: core %d has peak %d outside [0, %d)", where, c,
+                           peak[c], np);
+    }
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    hipStream_t s = nullptr;
+    Workspace &ws = workspace_for_current_device();
+    WsCall call(ws, s);
+    hipError_t e = call.e;
+    float *dden = nullptr, *dcd = nullptr;
+    long long *dnb = nullptr;
+    int *dpeak = nullptr, *dhalo = nullptr;
+    unsigned char *dflag = nullptr;
+    if (e == hipSuccess) e = upload(&dden, den, np, ws, s);
+    if (e == hipSuccess) e = upload(&dnb, neighbor, (size_t)np * NumNeighbor, ws, s);
+    if (e == hipSuccess) e = upload(&dpeak, peak.data(), numcore, ws, s);
+    if (e == hipSuccess) e = upload(&dcd, cd.data(), numcore, ws, s);
+    if (e == hipSuccess) e = upload(&dhalo, haloid, np, ws, s);
+    if (e == hipSuccess) e = upload(&dflag, flag, np, ws, s);
+    if (e != hipSuccess) return hip_err(e, where);
+    const int r = shells_run(dden, np, dnb, NumNeighbor, dpeak, dcd, numcore, thr, nthr, dflag,
+                             dhalo, shells, true, ws, s, where);
+    if (r != PGALF_OK) return r;
+    if (np > 0) {
+        e = hipMemcpyAsync(haloid, dhalo, sizeof(int) * np, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(flag, dflag, np, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_err(e, where);
+    }
+    return ok();
+}
+
+extern "C" int pgalf_dev_find_core_density(const float *den, long long np, const long long *nbr,
+                                           int k, const int *type, const double *mass,
+                                           const int *peak, int numcore, float *coredensity,
+                                           int *nummem, int *numstar, float *starmass,
+                                           int *haloid, unsigned char *flag, void *stream) {
+    const char *where = "pgalf_dev_find_core_density";
+    if (np < 0 || np >= (1LL << 31) || numcore < 0 || k < 1 || k > 32 ||
+        (np > 0 && (!den || !nbr || !type || !mass || !haloid || !flag)) ||
+        (numcore > 0 && (!peak || !coredensity || !nummem || !numstar || !starmass)))
+        return set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    hipStream_t s = (hipStream_t)stream;
+    Workspace &ws = workspace_for_current_device();
+    WsCall call(ws, s);
+    hipError_t e = call.e;
+    CoreOut o{coredensity, nummem, numstar, starmass, haloid, flag};
+    int fail = 0;
+    if (e == hipSuccess)
+        e = find_core_density(den, np, nbr, k, type, mass, peak, numcore, o, &fail, ws, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_err(e, where);
+    const int st = consume_status(ws, s, where);
+    if (st != PGALF_OK) return st;
+    if (fail) return set_err(PGALF_ERR_ARG, "%s: a core-density search did not converge", where);
+    return ok();
+}
+
+extern "C" int pgalf_dev_shell_thresholds(const float *den, long long np,
+                                          const float *coredensity, int numcore, float *thr,
+                                          int *nthr, void *stream) {
+    const char *where = "pgalf_dev_shell_thresholds";
+    if (np < 0 || numcore < 0 || (np > 0 && !den) || (numcore > 0 && !coredensity) || !thr ||
+        !nthr)
+        return set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    hipStream_t s = (hipStream_t)stream;
+    Workspace &ws = workspace_for_current_device();
+    WsCall call(ws, s);
+    hipError_t e = call.e;
+    if (e == hipSuccess) e = shell_thresholds(den, np, coredensity, numcore, thr, nthr, ws, s);
+    return e == hipSuccess ? ok() : hip_err(e, where);
+}
+
+extern "C" int pgalf_dev_shell_particles_peaks(const float *den, long long np,
+                                               const long long *nbr, int k, const int *peak,
+                                               const float *coredensity, int numcore,
+                                               const float *thr, int nthr, unsigned char *flag,
+                                               int *haloid, pgalf_shells_t *shells,
+                                               void *stream) {
+    const char *where = "pgalf_dev_shell_particles_peaks";
+    if (np < 0 || np >= (1LL << 31) || numcore < 0 || k < 1 || k > 32 || nthr < 0 ||
+        (nthr > 0 && !thr) || !shells || !shells->part_ptr || !shells->core_ptr ||
+        !shells->part || !shells->cores || shells->maxshell < 0 || shells->maxcores < 0 ||
+        (np > 0 && (!den || !nbr || !flag || !haloid)) ||
+        (numcore > 0 && (!peak || !coredensity)))
+        return set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    hipStream_t s = (hipStream_t)stream;
+    Workspace &ws = workspace_for_current_device();
+    WsCall call(ws, s);
+    if (call.e != hipSuccess) return hip_err(call.e, where);
+    return shells_run(den, np, nbr, k, peak, coredensity, numcore, thr, nthr, flag, haloid,
+                      shells, false, ws, s, where);
+}
+
 // ---- launch schedule ---------------------------------------------------------------------
 extern "C" int pgalf_set_schedule(const pgalf_schedule_t *bins, int nbins,
                                   int smooth_direct_max) {

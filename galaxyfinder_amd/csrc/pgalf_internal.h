@@ -184,6 +184,41 @@ hipError_t radix_sort_pairs(unsigned **keys, unsigned **vals, unsigned *keys_tmp
 hipError_t fill_u32(unsigned *p, long long n, unsigned v, hipStream_t s);
 hipError_t fill_u64(unsigned long long *p, long long n, unsigned long long v, hipStream_t s);
 
+// ---- watershed of subhalo_den (watershed.hip) ---------------------------
+// FindCoreDensity outputs (device arrays): per core, per particle.
+struct CoreOut {
+    float *coredensity;
+    int *nummem, *numstar;
+    float *starmass;
+    int *haloid;
+    unsigned char *flag;
+};
+// Shells as CSR: particles part[part_ptr[s] .. part_ptr[s+1]), cores
+// cores[core_ptr[s] .. core_ptr[s+1]); at most maxshell shells, maxcores
+// core entries.
+struct ShellOut {
+    long long *part_ptr;
+    int *part;
+    long long *core_ptr;
+    int *cores;
+    long long maxshell, maxcores;
+};
+// Host-synchronising (reads counts back between stages); not capturable.
+// *search_fail: a threshold search that never converged (the reference loops
+// for ever there).
+hipError_t find_core_density(const float *den, long long np, const long long *nbr, int k,
+                             const int *type, const double *mass, const int *peak, int numcore,
+                             const CoreOut &o, int *search_fail, Workspace &ws, hipStream_t s);
+// The shell thresholds (host array thr[15], *nthr of them).
+hipError_t shell_thresholds(const float *den, long long np, const float *coredensity,
+                            int numcore, float *thr, int *nthr, Workspace &ws, hipStream_t s);
+// *err: 1 more shells than maxshell, 2 a root's own core never enclosed (two
+// cores sharing a peak), 3 more core entries than maxcores.
+hipError_t core_shells(const float *den, long long np, const long long *nbr, int k,
+                       const int *peak, const float *coredensity, int numcore, const float *thr,
+                       int nthr, unsigned char *flag, int *haloid, const ShellOut &o,
+                       long long *nshell, int *err, Workspace &ws, hipStream_t s);
+
 // ---- TSC deposit / read-back (tsc.hip) ----------------------------------
 template <class P>
 hipError_t tsc_deposit(const P *x, const P *y, const P *z, const P *m,

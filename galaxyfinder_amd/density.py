@@ -15,7 +15,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import PgalfError, pgalf_grid_t
+from ._lib import PgalfError, pgalf_grid_t, pgalf_shells_t
 
 TYPE_STAR = 1
 TYPE_DM = 2
@@ -207,6 +207,85 @@ def MergingPeak(bp, np_, core, numcore, iflag=0):
     if n < 0:
         _lib.check()
     return n
+
+
+# Watershed of subhalo_den (subhaloden.mod6.c:1167-1475, 572-678, 2995-3024).
+# The reference's working particles `wp` are three arrays here: den
+# (wp[].den), haloid (wp[].haloid) and flag (wp[].flag).
+WP_PEAK, WP_VISITED, WP_CORE, WP_SHELL, WP_BOUND, WP_REMAINING = 1, 2, 4, 8, 16, 32
+NOT_HALO_MEMBER = -9999  # defs.h:29
+MAX_SHELL_DIVIDE = 15
+
+
+def FindCoreDensity(bp, np_, neighbor, NumNeighbor, core, numcore, den, haloid, flag):
+    """FindCoreDensity (subhaloden.mod6.c:1167-1475) with wp as den, haloid
+    (int32, out) and flag (uint8, out): fills core[:numcore]'s coredensity,
+    nummem, numstar and starmass in place; returns numcore."""
+    lib = _lib.load()
+    _need(bp, PARTICLE_DTYPE, np_, "bp")
+    _need(neighbor, np.int64, np_ * NumNeighbor, "neighbor")
+    _need(core, CORE_DTYPE, numcore, "core")
+    _need(den, np.float32, np_, "den")
+    _need(haloid, np.int32, np_, "haloid")
+    _need(flag, np.uint8, np_, "flag")
+    n = lib.pgalf_find_core_density(_ptr(bp), int(np_), _ptr(neighbor), int(NumNeighbor),
+                                    _ptr(core), int(numcore), _ptr(den), _ptr(haloid),
+                                    _ptr(flag))
+    if n < 0:
+        _lib.check()
+    return n
+
+
+def shell_thresholds(den, np_, core, numcore):
+    """The shell thresholds of subhalo_den (subhaloden.mod6.c:2995-3024),
+    highest first (float32 array)."""
+    lib = _lib.load()
+    _need(den, np.float32, np_, "den")
+    _need(core, CORE_DTYPE, numcore, "core")
+    thr = np.zeros(MAX_SHELL_DIVIDE, dtype=np.float32)
+    n = lib.pgalf_shell_thresholds(_ptr(den), int(np_), _ptr(core), int(numcore), _ptr(thr))
+    if n < 0:
+        _lib.check()
+    return thr[:n].copy()
+
+
+def _shell_lists(n, part_ptr, part, core_ptr, cores):
+    return [(part[part_ptr[i]:part_ptr[i + 1]].copy(), cores[core_ptr[i]:core_ptr[i + 1]].copy())
+            for i in range(n)]
+
+
+def GetShellParticlesPeaks(np_, neighbor, NumNeighbor, core, numcore, den, haloid, flag, thr,
+                           maxcores=None):
+    """GetShellParticlesPeaks (subhaloden.mod6.c:612-678) for every threshold
+    of thr in order, then SaveRemainingParticles2LastShell (:572-611); flag
+    and haloid are updated in place.  Returns the shells as a list of
+    (particles, cores) arrays (SHELL2P / SHELL2C)."""
+    lib = _lib.load()
+    _need(neighbor, np.int64, np_ * NumNeighbor, "neighbor")
+    _need(core, CORE_DTYPE, numcore, "core")
+    _need(den, np.float32, np_, "den")
+    _need(haloid, np.int32, np_, "haloid")
+    _need(flag, np.uint8, np_, "flag")
+    thr = np.ascontiguousarray(thr, dtype=np.float32)
+    maxshell = numcore * max(len(thr), 1) + 1
+    maxcores = numcore * (len(thr) + 1) + 1 if maxcores is None else maxcores
+    while True:
+        pp = np.zeros(maxshell + 1, np.int64)
+        cp = np.zeros(maxshell + 1, np.int64)
+        part = np.zeros(max(np_, 1), np.int32)
+        cores = np.zeros(maxcores, np.int32)
+        sh = pgalf_shells_t(_ptr(pp), _ptr(part), _ptr(cp), _ptr(cores), maxshell, maxcores, 0)
+        fl, hid = flag.copy(), haloid.copy()
+        rc = lib.pgalf_shell_particles_peaks(int(np_), _ptr(neighbor), int(NumNeighbor),
+                                             _ptr(core), int(numcore), _ptr(den), _ptr(hid),
+                                             _ptr(fl), _ptr(thr), len(thr), ctypes.byref(sh))
+        if rc == 6 and maxcores < 64 * (numcore + 1) * (len(thr) + 1):  # core entries overflow
+            maxcores *= 4
+            continue
+        _lib.check(rc)
+        flag[:] = fl
+        haloid[:] = hid
+        return _shell_lists(sh.nshell, pp, part, cp, cores)
 
 
 # ---------------------------------------------------------------------------
@@ -610,3 +689,95 @@ __all__ = ["PgalfError", "PARTICLE_DTYPE", "make_particles", "padded_shape",
            "timing_get", "CORE_DTYPE", "grid_cores", "finddenpeak", "dev_grid_cores",
            "dev_finddenpeak", "set_schedule", "get_schedule", "MergingPeak",
            "dev_merging_peak"]
+
+
+def dev_find_core_density(den, nbr, peak, types, mass):
+    """FindCoreDensity on device (pgalf_dev_find_core_density).  den float32
+    [n], nbr int64 [n, k], peak int32 [numcore], types int32 [n], mass
+    float64 [n].  Returns a dict of device tensors: coredensity, nummem,
+    numstar, starmass (per core), haloid, flag (per particle)."""
+    torch = _torch()
+    n = den.numel()
+    k = nbr.shape[1]
+    nc = peak.numel()
+    _check_dev(den, torch.float32, n, "den")
+    _check_dev(nbr, torch.int64, n * k, "nbr")
+    _check_dev(peak, torch.int32, nc, "peak")
+    _check_dev(types, torch.int32, n, "types")
+    _check_dev(mass, torch.float64, n, "mass")
+    dev = den.device
+    o = dict(coredensity=torch.empty(max(nc, 1), dtype=torch.float32, device=dev),
+             nummem=torch.empty(max(nc, 1), dtype=torch.int32, device=dev),
+             numstar=torch.empty(max(nc, 1), dtype=torch.int32, device=dev),
+             starmass=torch.empty(max(nc, 1), dtype=torch.float32, device=dev),
+             haloid=torch.empty(max(n, 1), dtype=torch.int32, device=dev),
+             flag=torch.empty(max(n, 1), dtype=torch.uint8, device=dev))
+    lib = _lib.load()
+    rc = lib.pgalf_dev_find_core_density(_dptr(den), n, _dptr(nbr), int(k), _dptr(types),
+                                         _dptr(mass), _dptr(peak), nc, _dptr(o["coredensity"]),
+                                         _dptr(o["nummem"]), _dptr(o["numstar"]),
+                                         _dptr(o["starmass"]), _dptr(o["haloid"]),
+                                         _dptr(o["flag"]), _stream())
+    _lib.check(rc)
+    for key in ("coredensity", "nummem", "numstar", "starmass"):
+        o[key] = o[key][:nc]
+    for key in ("haloid", "flag"):
+        o[key] = o[key][:n]
+    return o
+
+
+def dev_shell_thresholds(den, coredensity):
+    """The shell thresholds from device densities (pgalf_dev_shell_thresholds);
+    a float32 numpy array."""
+    torch = _torch()
+    n, nc = den.numel(), coredensity.numel()
+    _check_dev(den, torch.float32, n, "den")
+    _check_dev(coredensity, torch.float32, nc, "coredensity")
+    thr = np.zeros(MAX_SHELL_DIVIDE, dtype=np.float32)
+    cnt = ctypes.c_int(0)
+    lib = _lib.load()
+    _lib.check(lib.pgalf_dev_shell_thresholds(_dptr(den), n, _dptr(coredensity), nc, _ptr(thr),
+                                              ctypes.byref(cnt), _stream()))
+    return thr[:cnt.value].copy()
+
+
+def dev_shell_particles_peaks(den, nbr, peak, coredensity, thr, flag, haloid, maxcores=None):
+    """Shells on device (pgalf_dev_shell_particles_peaks): flag and haloid
+    (device tensors from dev_find_core_density) are updated in place.
+    Returns (nshell, part_ptr, part, core_ptr, cores) as device tensors."""
+    torch = _torch()
+    n = den.numel()
+    k = nbr.shape[1]
+    nc = peak.numel()
+    _check_dev(den, torch.float32, n, "den")
+    _check_dev(nbr, torch.int64, n * k, "nbr")
+    _check_dev(peak, torch.int32, nc, "peak")
+    _check_dev(coredensity, torch.float32, nc, "coredensity")
+    _check_dev(flag, torch.uint8, n, "flag")
+    _check_dev(haloid, torch.int32, n, "haloid")
+    thr = np.ascontiguousarray(thr, dtype=np.float32)
+    dev = den.device
+    maxshell = nc * max(len(thr), 1) + 1
+    maxcores = 4 * nc * (len(thr) + 1) + 16 if maxcores is None else maxcores
+    pp = torch.zeros(maxshell + 1, dtype=torch.int64, device=dev)
+    cp = torch.zeros(maxshell + 1, dtype=torch.int64, device=dev)
+    part = torch.zeros(max(n, 1), dtype=torch.int32, device=dev)
+    flag0, haloid0 = flag.clone(), haloid.clone()
+    lib = _lib.load()
+    while True:
+        cores = torch.zeros(max(maxcores, 1), dtype=torch.int32, device=dev)
+        sh = pgalf_shells_t(_dptr(pp), _dptr(part), _dptr(cp), _dptr(cores), maxshell, maxcores,
+                            0)
+        rc = lib.pgalf_dev_shell_particles_peaks(_dptr(den), n, _dptr(nbr), int(k), _dptr(peak),
+                                                 _dptr(coredensity), nc, _ptr(thr), len(thr),
+                                                 _dptr(flag), _dptr(haloid), ctypes.byref(sh),
+                                                 _stream())
+        # a core listed by several shells of one threshold can outgrow the
+        # core entries: start again from the input state with more room
+        if rc == 6 and maxcores < 64 * (nc + 1) * (len(thr) + 1):
+            flag.copy_(flag0)
+            haloid.copy_(haloid0)
+            maxcores *= 4
+            continue
+        _lib.check(rc)
+        return sh.nshell, pp, part, cp, cores

@@ -160,6 +160,64 @@ int pgalf_finddenpeak(float *den, int numneigh, long long *neighbor, int np, int
 int pgalf_merging_peak(pgalf_particle_t *bp, int np, pgalf_core_t *core, int numcore,
                        int iflag);
 
+/* Watershed of subhalo_den.  The reference keeps the working particles in a
+ * global array `wp` (subhaloden.mod6.c:237-243); here its fields are three
+ * caller arrays of np entries: den (wp[].den as Dump2WorkingParticle sets
+ * it, :493-500), haloid (wp[].haloid) and flag (wp[].flag, bits below).
+ * Flag bits (subhaloden.mod6.c:180-186) and the non-member id (defs.h:29): */
+#define PGALF_WP_PEAK 0x01
+#define PGALF_WP_VISITED 0x02
+#define PGALF_WP_CORE 0x04
+#define PGALF_WP_SHELL 0x08
+#define PGALF_WP_BOUND 0x10
+#define PGALF_WP_REMAINING 0x20
+#define PGALF_NOT_HALO_MEMBER (-9999)
+#define PGALF_MAX_SHELL_DIVIDE 15 /* the most shell thresholds subhalo_den uses */
+
+/* FindCoreDensity (subhaloden.mod6.c:1167-1475; same arguments, same return
+ * value numcore) with wp as den/haloid/flag.  Reads core[c].peak and
+ * bp[].type/mass; writes core[c].coredensity, nummem, numstar, starmass (the
+ * FP32 sum of the member stars' masses in sweep order), flag (PEAK, CORE,
+ * BOUND) and haloid (the core of each member, PGALF_NOT_HALO_MEMBER
+ * elsewhere).  The reference spreads the cores over OpenMP threads; this is
+ * the one-thread result: a particle among the members of several cores has
+ * the last core's id.  NumNeighbor <= 32.  On error -1 (pgalf_last_error). */
+int pgalf_find_core_density(pgalf_particle_t *bp, int np, long long *neighbor, int NumNeighbor,
+                            pgalf_core_t *core, int numcore, const float *den, int *haloid,
+                            unsigned char *flag);
+
+/* Shells as CSR arrays owned by the caller: shell s holds particles
+ * part[part_ptr[s] .. part_ptr[s+1]) and cores cores[core_ptr[s] ..
+ * core_ptr[s+1]) (the reference's SHELL2P(s) / SHELL2C(s),
+ * subhaloden.mod6.c:285-295).  part holds np entries, part_ptr and core_ptr
+ * maxshell + 1; a core listed by several shells of one threshold appears in
+ * each, so cores may need more than numcore * nthr entries. */
+typedef struct pgalf_shells {
+    long long *part_ptr;
+    int *part;
+    long long *core_ptr;
+    int *cores;
+    long long maxshell, maxcores;
+    long long nshell; /* out */
+} pgalf_shells_t;
+
+/* The shell thresholds of subhalo_den (subhaloden.mod6.c:2995-3024):
+ * nshelldivide values (10, or up to 15 for np > 1000), highest first, into
+ * thr[0 .. return).  On error -1. */
+int pgalf_shell_thresholds(const float *den, int np, const pgalf_core_t *core, int numcore,
+                           float *thr);
+
+/* GetShellParticlesPeaks (subhaloden.mod6.c:612-678) for each of thr[0 ..
+ * nthr) in order, then SaveRemainingParticles2LastShell (:572-611), on the
+ * state FindCoreDensity left (flag, haloid updated in place; core[c].peak
+ * and coredensity read).  The shells go to *shells (nshell set).  Returns a
+ * status code: PGALF_ERR_CORE_OVERFLOW when the shells or their core entries
+ * exceed maxshell / maxcores, PGALF_ERR_ARG when two cores share a peak
+ * (the reference never ends there). */
+int pgalf_shell_particles_peaks(int np, long long *neighbor, int NumNeighbor, pgalf_core_t *core,
+                                int numcore, const float *den, int *haloid, unsigned char *flag,
+                                const float *thr, int nthr, pgalf_shells_t *shells);
+
 /* ===================== Part 2: device-resident API ======================== */
 
 typedef struct pgalf_grid {
@@ -253,6 +311,20 @@ int pgalf_dev_finddenpeak(const float *den, int k, const long long *nbr, long lo
 int pgalf_dev_merging_peak(const float *x, const float *y, const float *z, long long np,
                            const int *peak, const float *key, int numcore, float link,
                            int *keep, int *order, int *nkeep, void *stream);
+/* Device forms of the watershed (den, nbr, peak, type, mass, outputs and
+ * the shell arrays in device memory; thr in host memory).  They read counts
+ * back between stages, so they synchronise `stream` and cannot be captured
+ * in a graph.  mass: the particles' FP64 masses (starmass sums them). */
+int pgalf_dev_find_core_density(const float *den, long long np, const long long *nbr, int k,
+                                const int *type, const double *mass, const int *peak,
+                                int numcore, float *coredensity, int *nummem, int *numstar,
+                                float *starmass, int *haloid, unsigned char *flag, void *stream);
+int pgalf_dev_shell_thresholds(const float *den, long long np, const float *coredensity,
+                               int numcore, float *thr, int *nthr, void *stream);
+int pgalf_dev_shell_particles_peaks(const float *den, long long np, const long long *nbr, int k,
+                                    const int *peak, const float *coredensity, int numcore,
+                                    const float *thr, int nthr, unsigned char *flag,
+                                    int *haloid, pgalf_shells_t *shells, void *stream);
 /* Synchronise `stream` and return the status of the kernels run since the
  * previous call (PGALF_ERR_OUT_OF_GRID / PGALF_ERR_SHORT_LIST / PGALF_ERR_ARG
  * for an index outside the particle range, or PGALF_OK). */

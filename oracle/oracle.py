@@ -86,6 +86,15 @@ def lib():
         L.orc_finddenpeak.argtypes = [_fp, _i, _lp, _i64, _ip, _i, _d, _lp, _i64]
         L.orc_merging_peak.restype = _i64
         L.orc_merging_peak.argtypes = [_dp, _dp, _dp, _fp, _i64, _d, _ip, _ip]
+        _ucp = ctypes.POINTER(ctypes.c_ubyte)
+        L.orc_find_core_density.restype = _i64
+        L.orc_find_core_density.argtypes = [_fp, _i64, _lp, _i, _ip, _dp, _ip, _i, _fp, _ip,
+                                            _ip, _fp, _ip, _ucp]
+        L.orc_shell_thresholds.restype = _i
+        L.orc_shell_thresholds.argtypes = [_fp, _i64, _fp, _i, _fp]
+        L.orc_core_shells.restype = _i64
+        L.orc_core_shells.argtypes = [_fp, _i64, _lp, _i, _ip, _fp, _i, _fp, _i, _ucp, _ip,
+                                      _lp, _ip, _lp, _ip, _i64, _i64]
         _LIB = L
     return _LIB
 
@@ -125,6 +134,10 @@ def ref_lib():
         R.ref_findsphdensity.restype = None
         R.ref_findsphdensity.argtypes = [_dp, _dp, _dp, _dp, _i64, _i, _lp, _fp, _i]
         R.ref_sizeof_bp.restype = _i
+        R.ref_core_shells.restype = _i64
+        R.ref_core_shells.argtypes = [_ip, _dp, _i64, _fp, _lp, _i, _ip, _i, _i, _fp, _ip, _ip,
+                                      _fp, _ip, ctypes.POINTER(ctypes.c_ubyte), _i, _fp, _i,
+                                      _lp, _ip, _lp, _ip, _i64, _i64]
         R.ref_init.restype = _i
         R.ref_init.argtypes = [_i64]
         _REF = R
@@ -474,3 +487,116 @@ def ref_merging_peak(pos, peak, density):
     n = R.ref_merging_peak(_p(x, _dp), _p(y, _dp), _p(z, _dp), len(x), _p(pk, _ip), _p(d, _fp),
                            len(pk), _p(out, _ip))
     return n, out.astype(np.int64)
+
+
+# Watershed of subhalo_den (subhaloden.mod6.c:1167-1475, 572-678, 2978-3024).
+WP_PEAK, WP_VISITED, WP_CORE, WP_SHELL, WP_BOUND, WP_REMAINING = 1, 2, 4, 8, 16, 32
+NOT_HALO_MEMBER = -9999
+MAX_SHELL_DIVIDE = 15
+
+
+def _core_out(ncore, np_):
+    return dict(coredensity=np.zeros(ncore, np.float32), nummem=np.zeros(ncore, np.int32),
+                numstar=np.zeros(ncore, np.int32), starmass=np.zeros(ncore, np.float32),
+                haloid=np.zeros(np_, np.int32), flag=np.zeros(np_, np.uint8))
+
+
+def _shell_bufs(np_, ncore, nthr, maxcores=None):
+    maxshell = ncore * max(nthr, 1) + 1
+    if maxcores is None:
+        maxcores = ncore * (max(nthr, 1) + 1) + 1
+    return (np.zeros(maxshell + 1, np.int64), np.zeros(max(np_, 1), np.int32),
+            np.zeros(maxshell + 1, np.int64), np.zeros(maxcores, np.int32), maxshell)
+
+
+def _shell_lists(n, part_ptr, part, core_ptr, cores):
+    return [(part[part_ptr[s]:part_ptr[s + 1]].copy(), cores[core_ptr[s]:core_ptr[s + 1]].copy())
+            for s in range(n)]
+
+
+def find_core_density(den, nbr, peak, types, mass):
+    """FindCoreDensity (orc_find_core_density): dict of per-core and
+    per-particle outputs."""
+    d = np.ascontiguousarray(den, np.float32)
+    nb = np.ascontiguousarray(nbr, np.int64)
+    n, k = nb.shape
+    pk = np.ascontiguousarray(peak, np.int32)
+    t = np.ascontiguousarray(types, np.int32)
+    m = _f64(mass)
+    o = _core_out(len(pk), n)
+    _ucp = ctypes.POINTER(ctypes.c_ubyte)
+    lib().orc_find_core_density(_p(d, _fp), n, _p(nb, _lp), k, _p(t, _ip), _p(m, _dp),
+                                _p(pk, _ip), len(pk), _p(o["coredensity"], _fp),
+                                _p(o["nummem"], _ip), _p(o["numstar"], _ip),
+                                _p(o["starmass"], _fp), _p(o["haloid"], _ip),
+                                _p(o["flag"], _ucp))
+    return o
+
+
+def shell_thresholds(den, coredensity):
+    """The shell thresholds of subhalo_den (orc_shell_thresholds), float32."""
+    d = np.ascontiguousarray(den, np.float32)
+    cd = np.ascontiguousarray(coredensity, np.float32)
+    thr = np.zeros(MAX_SHELL_DIVIDE, np.float32)
+    n = lib().orc_shell_thresholds(_p(d, _fp), len(d), _p(cd, _fp), len(cd), _p(thr, _fp))
+    return thr[:n].copy()
+
+
+def core_shells(den, nbr, peak, coredensity, thr, flag, haloid):
+    """GetShellParticlesPeaks over thr, then SaveRemainingParticles2LastShell
+    (orc_core_shells), starting from FindCoreDensity's flag and haloid
+    (updated copies returned).  Returns (shells, flag, haloid) with shells a
+    list of (particles, cores) arrays, or raises on the reference's
+    non-terminating case."""
+    d = np.ascontiguousarray(den, np.float32)
+    nb = np.ascontiguousarray(nbr, np.int64)
+    n, k = nb.shape
+    pk = np.ascontiguousarray(peak, np.int32)
+    cd = np.ascontiguousarray(coredensity, np.float32)
+    th = np.ascontiguousarray(thr, np.float32)
+    _ucp = ctypes.POINTER(ctypes.c_ubyte)
+    maxcores = None
+    while True:
+        fl = np.array(flag, np.uint8)
+        hid = np.array(haloid, np.int32)
+        pp, part, cp, cores, maxshell = _shell_bufs(n, len(pk), len(th), maxcores)
+        ns = lib().orc_core_shells(_p(d, _fp), n, _p(nb, _lp), k, _p(pk, _ip), _p(cd, _fp),
+                                   len(pk), _p(th, _fp), len(th), _p(fl, _ucp), _p(hid, _ip),
+                                   _p(pp, _lp), _p(part, _ip), _p(cp, _lp), _p(cores, _ip),
+                                   maxshell, len(cores))
+        if ns != -3:
+            break
+        maxcores = 4 * len(cores)
+    if ns < 0:
+        raise RuntimeError("orc_core_shells: %d" % ns)
+    return _shell_lists(ns, pp, part, cp, cores), fl, hid
+
+
+def ref_core_shells(den, nbr, peak, types, mass, thr=None, nthreads=1):
+    """The reference's own FindCoreDensity (and, with thr, the shells)
+    through oracle/_ref.  Returns (core dict, shells or None)."""
+    R = ref_lib()
+    d = np.ascontiguousarray(den, np.float32)
+    nb = np.ascontiguousarray(nbr, np.int64)
+    n, k = nb.shape
+    pk = np.ascontiguousarray(peak, np.int32)
+    t = np.ascontiguousarray(types, np.int32)
+    m = _f64(mass)
+    th = np.ascontiguousarray(thr if thr is not None else [], np.float32)
+    _ucp = ctypes.POINTER(ctypes.c_ubyte)
+    maxcores = None
+    while True:
+        o = _core_out(len(pk), n)
+        pp, part, cp, cores, maxshell = _shell_bufs(n, len(pk), len(th), maxcores)
+        ns = R.ref_core_shells(_p(t, _ip), _p(m, _dp), n, _p(d, _fp), _p(nb, _lp), k,
+                               _p(pk, _ip), len(pk), nthreads, _p(o["coredensity"], _fp),
+                               _p(o["nummem"], _ip), _p(o["numstar"], _ip),
+                               _p(o["starmass"], _fp), _p(o["haloid"], _ip), _p(o["flag"], _ucp),
+                               1 if thr is not None else 0, _p(th, _fp), len(th), _p(pp, _lp),
+                               _p(part, _ip), _p(cp, _lp), _p(cores, _ip), maxshell, len(cores))
+        if ns != -3:
+            break
+        maxcores = 4 * len(cores)
+    if ns < 0:
+        raise RuntimeError("ref_core_shells: %d" % ns)
+    return o, (_shell_lists(ns, pp, part, cp, cores) if thr is not None else None)

@@ -1084,3 +1084,270 @@ long long orc_merging_peak(const double *x, const double *y, const double *z, co
     free(claimed);
     return nkeep;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Watershed of subhalo_den: FindCoreDensity, the shell thresholds and      */
+/* GetShellParticlesPeaks + SaveRemainingParticles2LastShell.               */
+/* Working-particle flag bits (subhaloden.mod6.c:180-186) and the          */
+/* non-member id (defs.h:29).                                               */
+/* ------------------------------------------------------------------------ */
+#define ORC_WP_PEAK 0x01
+#define ORC_WP_VISITED 0x02
+#define ORC_WP_CORE 0x04
+#define ORC_WP_SHELL 0x08
+#define ORC_WP_BOUND 0x10
+#define ORC_WP_REMAINING 0x20
+#define ORC_NOT_HALO_MEMBER (-9999)
+#define ORC_CORE_ENCLOSED 0x01
+
+/* One breadth-first sweep of FindCoreDensity from `root` over particles
+ * denser than thr (subhaloden.mod6.c:1243-1266 for the trial sweep,
+ * :1276-1300 for the final one).  A neighbour that is an unvisited peak
+ * sets the break flag; the trial sweep (stop_all) then ends at once, the
+ * final sweep only leaves the rest of that particle's list.  vis[] is the
+ * thread's visited bit; the pushed particles are q[0 .. return). */
+static long long orc_core_sweep(const float *den, const long long *nbr, int k,
+                                const unsigned char *flag, unsigned char *vis, int *q,
+                                int root, float thr, int stop_all, int *brk) {
+    long long n = 0, now = 0;
+    *brk = 0;
+    q[n++] = root;
+    vis[root] = 1;
+    while (now < n && !(stop_all && *brk)) {
+        const long long base = (long long)q[now] * k;
+        for (int s = 0; s < k; s++) {
+            const long long v = nbr[base + s];
+            if (!(den[v] > thr)) continue;
+            if (!vis[v] && !(flag[v] & ORC_WP_PEAK)) {
+                vis[v] = 1;
+                q[n++] = (int)v;
+            } else if (!vis[v]) {
+                *brk = 1;
+                break;
+            }
+        }
+        now++;
+    }
+    return n;
+}
+
+/* FindCoreDensity (subhaloden.mod6.c:1167-1475) as one OpenMP thread runs
+ * it (the reference parallelises over cores with per-thread visited bits;
+ * with one thread its result is the schedule-free one).  den: wp[].den as
+ * Dump2WorkingParticle sets it (:493-500); peak[c]: core[c].peak.
+ * Per core: coredensity, nummem, numstar, starmass (the FP32 sum of the
+ * stars' FP64 masses in sweep order); per particle: flag (PEAK, CORE,
+ * BOUND) and haloid (the last core whose final sweep holds it).
+ * The threshold search (:1229-1269) halves [downden, upden] from
+ * [minden, den[peak]] while |(upden-downden)/denthr| > 1e-3
+ * (COREDENRESOLUTION, params.h:12; the definition at
+ * subhaloden.mod6.c:22 is commented out).  A final sweep that breaks discards the core's
+ * members (nummem 0) without clearing their visited bits (:1301-1310), so
+ * they stay visited for every later core: restated here with one vis[]
+ * kept across cores. */
+long long orc_find_core_density(const float *den, long long np, const long long *nbr, int k,
+                                const int *type, const double *mass, const int *peak,
+                                int numcore, float *coredensity, int *nummem, int *numstar,
+                                float *starmass, int *haloid, unsigned char *flag) {
+    for (long long i = 0; i < np; i++) haloid[i] = ORC_NOT_HALO_MEMBER;
+    float minden = 1.e23f;
+    for (long long i = 0; i < np; i++) minden = den[i] < minden ? den[i] : minden;
+    minden = (float)(1.0 > (double)minden ? 1.0 : (double)minden); /* DENFLOOR, params.h:109 */
+    unsigned char *vis = (unsigned char *)calloc((size_t)(np > 0 ? np : 1), 1);
+    int *q = (int *)malloc(sizeof(int) * (size_t)(np > 0 ? np : 1));
+    for (long long i = 0; i < np; i++) flag[i] = 0;
+    for (int c = 0; c < numcore; c++) flag[peak[c]] |= ORC_WP_PEAK;
+    for (int c = 0; c < numcore; c++) {
+        float upden = den[peak[c]], downden = minden, denthr;
+        long long n = 0;
+        int brk;
+        do {
+            for (long long j = 0; j < n; j++) vis[q[j]] = 0;
+            denthr = (float)(0.5 * (double)(upden + downden));
+            n = orc_core_sweep(den, nbr, k, flag, vis, q, peak[c], denthr, 1, &brk);
+            if (!brk) upden = denthr;
+            else downden = denthr;
+        } while (fabs((double)((upden - downden) / denthr)) > 1.e-3);
+        coredensity[c] = denthr = upden;
+        for (long long j = 0; j < n; j++) vis[q[j]] = 0;
+        n = orc_core_sweep(den, nbr, k, flag, vis, q, peak[c], denthr, 0, &brk);
+        if (brk) n = 0;
+        for (long long j = 0; j < n; j++) vis[q[j]] = 0;
+        int nstar = 0;
+        float smass = 0.0f;
+        for (long long j = 0; j < n; j++)
+            if (type[q[j]] == ORC_TYPE_STAR) {
+                nstar++;
+                smass = (float)((double)smass + mass[q[j]]);
+            }
+        for (long long j = 0; j < n; j++) {
+            flag[q[j]] = (unsigned char)((flag[q[j]] | ORC_WP_CORE | ORC_WP_BOUND) &
+                                         ~ORC_WP_REMAINING);
+            haloid[q[j]] = c;
+        }
+        nummem[c] = (int)n;
+        numstar[c] = nstar;
+        starmass[c] = smass;
+    }
+    free(vis);
+    free(q);
+    return numcore;
+}
+
+/* The shell thresholds of subhalo_den (subhaloden.mod6.c:2995-3024): the
+ * log10(1 + density) range from the floored particle minimum to the largest
+ * core density, divided into nshelldivide steps (10, or
+ * min(15, 10 log10(1.5 np)) when np > 1000; NSHELLDIVIDE params.h:43),
+ * highest first.  Writes thr[0 .. return). */
+int orc_shell_thresholds(const float *den, long long np, const float *coredensity, int numcore,
+                         float *thr) {
+    float minshellden = 1.e27f, maxshellden = -1.e27f;
+    for (int c = 0; c < numcore; c++)
+        maxshellden = maxshellden > coredensity[c] ? maxshellden : coredensity[c];
+    for (long long i = 0; i < np; i++) minshellden = minshellden < den[i] ? minshellden : den[i];
+    minshellden = (float)(1.0 > (double)minshellden ? 1.0 : (double)minshellden);
+    minshellden = (float)log10((double)minshellden + 1.0);
+    maxshellden = (float)log10((double)maxshellden + 1.0);
+    int nsd = 10;
+    if (np > 1000) {
+        double a = 10.0 * log10((double)np * 1.5);
+        nsd = (int)(10.0 > a ? 10.0 : a);
+        nsd = nsd < 15 ? nsd : 15;
+    }
+    for (int i = 0; i < nsd; i++) {
+        float t = minshellden + (float)(nsd - 1 - i) * (maxshellden - minshellden) / (float)nsd;
+        thr[i] = (float)(pow(10.0, (double)t) - 1.0);
+    }
+    return nsd;
+}
+
+/* GetShellParticlesPeaks (subhaloden.mod6.c:612-678) once per threshold,
+ * then SaveRemainingParticles2LastShell (:572-611).  For each threshold, in
+ * core order, every core above it that no earlier sweep of this threshold
+ * reached roots a sweep over particles denser than the threshold (peaks
+ * included, no barrier); the sweep's particles that are neither shell nor
+ * core particles yet form a new shell (flag SHELL, haloid non-member), and
+ * the cores whose peak it reached (the first core of each such peak) are
+ * the shell's cores.  Particles left outside every core and shell join the
+ * last shell (or a shell of all cores when there is none).  Shells go to
+ * CSR arrays: particles part[part_ptr[s] .. part_ptr[s+1]), cores
+ * cores[core_ptr[s] .. core_ptr[s+1]).  flag/haloid are updated in place
+ * (as FindCoreDensity left them).  A core reached by several sweeps of one
+ * threshold is listed in each of their shells, as the reference does.
+ * Returns nshell, -1 when more than maxshell shells, -2 when a root's own
+ * core is never enclosed (two cores sharing a peak: the reference loops for
+ * ever), -3 when the shell cores exceed maxcores. */
+long long orc_core_shells(const float *den, long long np, const long long *nbr, int k,
+                          const int *peak, const float *coredensity, int numcore,
+                          const float *thr, int nthr, unsigned char *flag, int *haloid,
+                          long long *part_ptr, int *part, long long *core_ptr, int *cores,
+                          long long maxshell, long long maxcores) {
+    int *q = (int *)malloc(sizeof(int) * (size_t)(np > 0 ? np : 1));
+    unsigned char *cflag = (unsigned char *)calloc((size_t)(numcore > 0 ? numcore : 1), 1);
+    int *first_core = (int *)malloc(sizeof(int) * (size_t)(np > 0 ? np : 1));
+    for (long long i = 0; i < np; i++) first_core[i] = -1;
+    for (int c = numcore - 1; c >= 0; c--) first_core[peak[c]] = c;
+    long long nshell = 0, npart = 0, ncores = 0, n = 0;
+    part_ptr[0] = core_ptr[0] = 0;
+    for (int t = 0; t < nthr; t++) {
+        const float dthr = thr[t];
+        memset(cflag, 0, (size_t)(numcore > 0 ? numcore : 1));
+        for (long long i = 0; i < np; i++) flag[i] &= (unsigned char)~ORC_WP_VISITED;
+        n = 0;
+        /* GET_NONENCLOSED_ID (:555-563) scans from core 0 each time; the
+         * cores before the last root are enclosed or below the threshold
+         * and stay so within one threshold, so the scan resumes there */
+        int cursor = 0;
+        for (;;) {
+            int root = -1;
+            for (int c = cursor; c < numcore; c++)
+                if (!(cflag[c] & ORC_CORE_ENCLOSED) && coredensity[c] > dthr) {
+                    root = c;
+                    break;
+                }
+            if (root < 0) break;
+            cursor = root;
+            for (long long j = 0; j < n; j++) flag[q[j]] &= (unsigned char)~ORC_WP_VISITED;
+            long long now = 0;
+            n = 0;
+            q[n++] = peak[root];
+            flag[peak[root]] |= ORC_WP_VISITED;
+            while (now < n) {
+                const long long base = (long long)q[now] * k;
+                for (int s = 0; s < k; s++) {
+                    const long long v = nbr[base + s];
+                    if (den[v] > dthr && !(flag[v] & ORC_WP_VISITED)) {
+                        flag[v] |= ORC_WP_VISITED;
+                        q[n++] = (int)v;
+                    }
+                }
+                now++;
+            }
+            if (nshell >= maxshell) {
+                nshell = -1;
+                goto done;
+            }
+            for (long long j = 0; j < n; j++) {
+                const int v = q[j];
+                if (!(flag[v] & ORC_WP_SHELL) && !(flag[v] & ORC_WP_CORE)) {
+                    part[npart++] = v;
+                    flag[v] = (unsigned char)((flag[v] | ORC_WP_SHELL) & ~ORC_WP_REMAINING);
+                    haloid[v] = ORC_NOT_HALO_MEMBER;
+                }
+            }
+            for (long long j = 0; j < n; j++) {
+                const int v = q[j];
+                if (!(flag[v] & ORC_WP_PEAK)) continue;
+                /* the reference scans the cores for the first with this peak
+                 * (:664-672); first_core[] holds that core */
+                const int c = first_core[v];
+                if (c < 0) continue;
+                if (ncores >= maxcores) {
+                    nshell = -3;
+                    goto done;
+                }
+                cflag[c] |= ORC_CORE_ENCLOSED;
+                cores[ncores++] = c;
+            }
+            if (!(cflag[root] & ORC_CORE_ENCLOSED)) {
+                nshell = -2;
+                goto done;
+            }
+            nshell++;
+            part_ptr[nshell] = npart;
+            core_ptr[nshell] = ncores;
+        }
+    }
+    {
+        long long nrest = 0;
+        for (long long i = 0; i < np; i++)
+            if (!(flag[i] & ORC_WP_CORE) && !(flag[i] & ORC_WP_SHELL)) nrest++;
+        if (nrest > 0) {
+            if (nshell == 0) {
+                if (maxshell < 1) {
+                    nshell = -1;
+                    goto done;
+                }
+                if (ncores + numcore > maxcores) {
+                    nshell = -3;
+                    goto done;
+                }
+                for (int c = 0; c < numcore; c++) cores[ncores++] = c;
+                nshell = 1;
+                core_ptr[1] = ncores;
+            }
+            for (long long i = 0; i < np; i++)
+                if (!(flag[i] & ORC_WP_CORE) && !(flag[i] & ORC_WP_SHELL)) {
+                    flag[i] = (unsigned char)((flag[i] | ORC_WP_SHELL) & ~ORC_WP_REMAINING);
+                    haloid[i] = ORC_NOT_HALO_MEMBER;
+                    part[npart++] = (int)i;
+                }
+            part_ptr[nshell] = npart;
+        }
+    }
+done:
+    free(q);
+    free(cflag);
+    free(first_core);
+    return nshell;
+}

@@ -384,3 +384,103 @@ int ref_merging_peak(const double *x, const double *y, const double *z, long lon
     free(bp);
     return n;
 }
+
+/* The watershed of subhalo_den (subhaloden.mod6.c:2978-3024) with the
+ * reference's own Dump2WorkingParticle, FindCoreDensity,
+ * GetShellParticlesPeaks and SaveRemainingParticles2LastShell.  They keep
+ * their state in file-scope globals of subhaloden.mod6.c: the working
+ * particles `wp`, the shells `shell[]` and their count `nshell`.  The two
+ * record types are file-local there (:237-242, :306-311) and restated here
+ * with the same members, since this driver fills and reads them.
+ * Cores are given by their peak particle (core order as given); den is
+ * wp[].den.  FindCoreDensity runs on `nthreads` OpenMP threads.  With
+ * with_shells, the shell thresholds thr[0 .. nthr) are applied in order and
+ * the shells copied out as CSR (particles, cores); returns nshell, -1
+ * when more than maxshell, -3 when the shell cores exceed maxcores.
+ * Without, returns 0. */
+typedef struct {
+    float den;
+    int haloid;
+    unsigned char flag;
+    unsigned char flag2[MAXTHREADS];
+} ref_working_particle;
+typedef struct {
+    int nlinkedcore;
+    int *linkedcore;
+    int nlinkedparticle;
+    int *linkedparticle;
+} ref_shell_type;
+extern ref_working_particle *wp;
+extern ref_shell_type shell[];
+extern int nshell;
+void Dump2WorkingParticle(float *, int, Coretype *, int);
+int FindCoreDensity(SimpleBasicParticleType *, int, long long *, int, Coretype *, int);
+void GetShellParticlesPeaks(int, long long *, int, Coretype *, int, float);
+void SaveRemainingParticles2LastShell(int, Coretype *, int);
+
+long long ref_core_shells(const int *type, const double *mass, long long np, const float *den,
+                          const long long *nbr, int k, const int *peak, int numcore,
+                          int nthreads, float *coredensity, int *nummem, int *numstar,
+                          float *starmass, int *haloid, unsigned char *flag, int with_shells,
+                          const float *thr, int nthr, long long *part_ptr, int *part,
+                          long long *core_ptr, int *cores, long long maxshell,
+                          long long maxcores) {
+    ref_init(4LL << 30);
+    SimpleBasicParticleType *bp =
+        (SimpleBasicParticleType *)calloc((size_t)np, sizeof(SimpleBasicParticleType));
+    for (long long i = 0; i < np; i++) {
+        bp[i].type = (unsigned int)type[i];
+        bp[i].mass = mass[i];
+        bp[i].indx = (idtype)i;
+    }
+    Coretype *core = (Coretype *)calloc((size_t)(numcore > 0 ? numcore : 1), sizeof(Coretype));
+    for (int c = 0; c < numcore; c++) core[c].peak = peak[c];
+    wp = (ref_working_particle *)Malloc(sizeof(ref_working_particle) * np, PPTR(wp));
+    Dump2WorkingParticle((float *)den, (int)np, core, numcore);
+    omp_set_num_threads(nthreads);
+    FindCoreDensity(bp, (int)np, (long long *)nbr, k, core, numcore);
+    for (int c = 0; c < numcore; c++) {
+        coredensity[c] = core[c].coredensity;
+        nummem[c] = core[c].nummem;
+        numstar[c] = core[c].numstar;
+        starmass[c] = core[c].starmass;
+    }
+    long long ret = 0;
+    if (with_shells) {
+        nshell = 0;
+        for (int t = 0; t < nthr; t++)
+            GetShellParticlesPeaks((int)np, (long long *)nbr, k, core, numcore, thr[t]);
+        SaveRemainingParticles2LastShell((int)np, core, numcore);
+        ret = nshell <= maxshell ? nshell : -1;
+        long long np_out = 0, nc_out = 0, nc_all = 0;
+        for (int s = 0; s < nshell; s++) nc_all += shell[s].nlinkedcore;
+        if (ret >= 0 && nc_all > maxcores) ret = -3;
+        if (ret >= 0) {
+            part_ptr[0] = core_ptr[0] = 0;
+            for (int s = 0; s < nshell; s++) {
+                for (int j = 0; j < shell[s].nlinkedparticle; j++)
+                    part[np_out++] = shell[s].linkedparticle[j];
+                for (int j = 0; j < shell[s].nlinkedcore; j++)
+                    cores[nc_out++] = shell[s].linkedcore[j];
+                part_ptr[s + 1] = np_out;
+                core_ptr[s + 1] = nc_out;
+            }
+        }
+        for (int s = nshell - 1; s >= 0; s--) {
+            Free(shell[s].linkedcore);
+            Free(shell[s].linkedparticle);
+            shell[s].linkedcore = NULL;
+            shell[s].linkedparticle = NULL;
+            shell[s].nlinkedcore = shell[s].nlinkedparticle = 0;
+        }
+        nshell = 0;
+    }
+    for (long long i = 0; i < np; i++) {
+        haloid[i] = wp[i].haloid;
+        flag[i] = wp[i].flag;
+    }
+    Free(wp);
+    free(core);
+    free(bp);
+    return ret;
+}
