@@ -1099,7 +1099,7 @@ extern "C" int pgalf_find_core_density(pgalf_particle_t *bp, int np, long long *
                                        int NumNeighbor, pgalf_core_t *core, int numcore,
                                        const float *den, int *haloid, unsigned char *flag) {
     const char *where = "pgalf_find_core_density";
-    if (np < 0 || numcore < 0 || NumNeighbor < 1 || NumNeighbor > 32 ||
+    if (np < 0 || np >= (1 << 26) || numcore < 0 || NumNeighbor < 1 || NumNeighbor > 32 ||
         (np > 0 && (!bp || !neighbor || !den || !haloid || !flag)) || (numcore > 0 && !core)) {
         set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
         return -1;
@@ -1264,7 +1264,8 @@ extern "C" int pgalf_shell_particles_peaks(int np, long long *neighbor, int NumN
                                            int *haloid, unsigned char *flag, const float *thr,
                                            int nthr, pgalf_shells_t *shells) {
     const char *where = "pgalf_shell_particles_peaks";
-    if (np < 0 || numcore < 0 || NumNeighbor < 1 || NumNeighbor > 32 || nthr < 0 ||
+    if (np < 0 || np >= (1 << 26) || numcore < 0 || NumNeighbor < 1 || NumNeighbor > 32 ||
+        nthr < 0 ||
         (nthr > 0 && !thr) || !shells || !shells->part_ptr || !shells->core_ptr ||
         !shells->part || !shells->cores || shells->maxshell < 0 || shells->maxcores < 0 ||
         (np > 0 && (!neighbor || !den || !haloid || !flag)) || (numcore > 0 && !core))
@@ -1312,7 +1313,7 @@ extern "C" int pgalf_dev_find_core_density(const float *den, long long np, const
                                            int *nummem, int *numstar, float *starmass,
                                            int *haloid, unsigned char *flag, void *stream) {
     const char *where = "pgalf_dev_find_core_density";
-    if (np < 0 || np >= (1LL << 31) || numcore < 0 || k < 1 || k > 32 ||
+    if (np < 0 || np >= (1LL << 26) || numcore < 0 || k < 1 || k > 32 ||
         (np > 0 && (!den || !nbr || !type || !mass || !haloid || !flag)) ||
         (numcore > 0 && (!peak || !coredensity || !nummem || !numstar || !starmass)))
         return set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
@@ -1356,7 +1357,7 @@ extern "C" int pgalf_dev_shell_particles_peaks(const float *den, long long np,
                                                int *haloid, pgalf_shells_t *shells,
                                                void *stream) {
     const char *where = "pgalf_dev_shell_particles_peaks";
-    if (np < 0 || np >= (1LL << 31) || numcore < 0 || k < 1 || k > 32 || nthr < 0 ||
+    if (np < 0 || np >= (1LL << 26) || numcore < 0 || k < 1 || k > 32 || nthr < 0 ||
         (nthr > 0 && !thr) || !shells || !shells->part_ptr || !shells->core_ptr ||
         !shells->part || !shells->cores || shells->maxshell < 0 || shells->maxcores < 0 ||
         (np > 0 && (!den || !nbr || !flag || !haloid)) ||

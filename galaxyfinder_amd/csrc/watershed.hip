@@ -69,7 +69,9 @@ static_assert((1 << kHashBits) == kWaveH, "hash size");
 enum Mode { kTrial = 0, kFinal = 1, kShell = 2 };
 // kBig: too large for the LDS sweep; kMid: swept by a workgroup; kGiant: left
 // to the device-wide sweep.
-enum : unsigned char { kOk = 0, kQuirk = 1, kBig = 2, kSearchFail = 3, kMid = 4, kGiant = 5 };
+// kHuge: a shell root swept device-wide with others (multi_sweep).
+enum : unsigned char { kOk = 0, kQuirk = 1, kBig = 2, kSearchFail = 3, kMid = 4, kGiant = 5,
+                       kHuge = 6 };
 enum : int { kSweepHit = -1, kSweepBig = -2 };
 
 inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -425,6 +427,12 @@ __global__ void core_scalar_k(int c, float cd, int nm, int *nummem, int *numstar
         starmass[c] = 0.0f;
     }
     status[c] = st;
+}
+
+template <class T>
+hipError_t to_host(T *h, const T *d, size_t n, hipStream_t s) {
+    PGALF_HIP_TRY(hipMemcpyAsync(h, d, sizeof(T) * n, hipMemcpyDeviceToHost, s));
+    return hipStreamSynchronize(s);
 }
 
 // ---- device-wide level-synchronous sweep ----------------------------------
@@ -865,6 +873,244 @@ __global__ __launch_bounds__(kBlkT) void shell_block_k(const float *den, const l
     }
 }
 
+// ---- device-wide shell sweeps of several roots at once -------------------
+// Up to kMaxRoots shell sweeps advance level by level together: the level's
+// frontier nodes of all roots form one index space (root r's nodes at
+// off[r] .. off[r+1]); a candidate claims its particle in its root's claim
+// array with its (node, slot) rank; each node counts its winners, an
+// exclusive scan of the counts places them, and the winners are appended to
+// their root's queue in (node, slot) order -- each root's FIFO order.  The
+// host queues levels in groups without waiting; every kernel reads the
+// level's size from device memory and exits when all sweeps are done.
+
+constexpr int kMaxRoots = 16;
+constexpr int kMlGrid = 2048;  // blocks of the grid-stride level kernels
+constexpr int kMlScanBlocks = 1024;
+constexpr int kMlLevelsPerSync = 16;
+
+struct MultiSweep {
+    int M;
+    long long np, nwords;
+    int *bq;          // [M][np]
+    unsigned *vb;     // [M][nwords]
+    int *claim;       // [M][np], INT_MAX idle
+    long long *lo, *hi, *off;  // [M], [M], [M + 1]
+    unsigned *cnt, *base;      // [M * np] per frontier node
+    unsigned *bsum;            // [kMlScanBlocks]
+    int *done;
+    const int *roots;          // [M] root particles
+};
+
+__global__ void ml_init_k(MultiSweep a) {
+    const int r = threadIdx.x;
+    if (r < a.M) {
+        const int root = a.roots[r];
+        a.bq[(long long)r * a.np] = root;
+        atomicOr(&a.vb[(long long)r * a.nwords + (root >> 5)], 1u << (root & 31));
+        a.lo[r] = 0;
+        a.hi[r] = 1;
+    }
+    if (r == 0) *a.done = 0;
+}
+
+__global__ void ml_plan_k(MultiSweep a) {
+    if (*a.done) return;
+    long long o = 0;
+    for (int r = 0; r < a.M; r++) {
+        a.off[r] = o;
+        o += a.hi[r] - a.lo[r];
+    }
+    a.off[a.M] = o;
+    if (o == 0) *a.done = 1;
+}
+
+__device__ __forceinline__ int ml_root_of(const MultiSweep &a, long long g) {
+    int r = 0;
+    while (r + 1 < a.M && a.off[r + 1] <= g) r++;
+    return r;
+}
+
+// Candidate i = node * 32 + slot of the level: its particle (or -1).
+__device__ __forceinline__ int ml_cand(const MultiSweep &a, long long i, float t, const float *den,
+                                       const long long *nbr, int k, int &r, long long &f) {
+    const long long g = i >> 5;
+    const int s = (int)(i & 31);
+    r = ml_root_of(a, g);
+    f = g - a.off[r];
+    if (s >= k) return -1;
+    const int u = a.bq[(long long)r * a.np + a.lo[r] + f];
+    const int v = (int)nbr[(long long)u * k + s];
+    if (!(den[v] > t)) return -1;
+    if ((a.vb[(long long)r * a.nwords + (v >> 5)] >> (v & 31)) & 1u) return -1;
+    return v;
+}
+
+__global__ void ml_claim_k(MultiSweep a, float t, const float *den, const long long *nbr, int k) {
+    if (*a.done) return;
+    const long long n = a.off[a.M] * 32;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        int r;
+        long long f;
+        const int v = ml_cand(a, i, t, den, nbr, k, r, f);
+        if (v >= 0) atomicMin(&a.claim[(long long)r * a.np + v], (int)(f * 32 + (i & 31)));
+    }
+}
+
+// Winners of each node: count (count pass) or append (push pass).  The grid
+// stride is a multiple of 64, so a wave always holds two whole nodes.
+template <bool kPush>
+__global__ void ml_win_k(MultiSweep a, float t, const float *den, const long long *nbr, int k) {
+    if (*a.done) return;
+    const long long n = a.off[a.M] * 32;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i0 = (long long)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
+        const long long i = i0 + threadIdx.x;
+        int r = 0, v = -1;
+        long long f = 0;
+        bool win = false;
+        if (i < n) {
+            v = ml_cand(a, i, t, den, nbr, k, r, f);
+            win = v >= 0 && a.claim[(long long)r * a.np + v] == (int)(f * 32 + (i & 31));
+        }
+        const unsigned long long m = __ballot(win);
+        const int lane = threadIdx.x & 63;
+        const unsigned half = (unsigned)(m >> (lane & 32));
+        if (!kPush) {
+            if (i < n && (lane & 31) == 0) a.cnt[i >> 5] = __popc(half);
+        } else if (win) {
+            const long long pos = a.base[i >> 5] - a.base[a.off[r]] +
+                                  __popc(half & ((1u << (lane & 31)) - 1u));
+            a.bq[(long long)r * a.np + a.hi[r] + pos] = v;
+            atomicOr(&a.vb[(long long)r * a.nwords + (v >> 5)], 1u << (v & 31));
+            a.claim[(long long)r * a.np + v] = INT_MAX;
+        }
+    }
+}
+
+// Exclusive scan of cnt[0 .. off[M]) into base, sized on the device.
+__global__ __launch_bounds__(256) void ml_scan1_k(MultiSweep a) {
+    if (*a.done) return;
+    const long long n = a.off[a.M];
+    const long long seg = (n + kMlScanBlocks - 1) / kMlScanBlocks;
+    const long long b0 = blockIdx.x * seg, b1 = b0 + seg < n ? b0 + seg : n;
+    unsigned acc = 0;
+    for (long long j = b0 + threadIdx.x; j < b1; j += 256) acc += a.cnt[j];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o);
+    __shared__ unsigned w[4];
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) a.bsum[blockIdx.x] = w[0] + w[1] + w[2] + w[3];
+}
+
+__global__ __launch_bounds__(1024) void ml_scan2_k(MultiSweep a) {
+    if (*a.done) return;
+    __shared__ unsigned sh[1024];
+    const unsigned v = a.bsum[threadIdx.x];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const unsigned add = threadIdx.x >= (unsigned)o ? sh[threadIdx.x - o] : 0u;
+        __syncthreads();
+        sh[threadIdx.x] += add;
+        __syncthreads();
+    }
+    a.bsum[threadIdx.x] = sh[threadIdx.x] - v;
+}
+
+__global__ __launch_bounds__(256) void ml_scan3_k(MultiSweep a) {
+    if (*a.done) return;
+    const long long n = a.off[a.M];
+    const long long seg = (n + kMlScanBlocks - 1) / kMlScanBlocks;
+    const long long b0 = blockIdx.x * seg, b1 = b0 + seg < n ? b0 + seg : n;
+    __shared__ unsigned w[4];
+    unsigned carry = a.bsum[blockIdx.x];
+    for (long long c0 = b0; c0 < b1; c0 += 256) {
+        const long long j = c0 + threadIdx.x;
+        const unsigned x = j < b1 ? a.cnt[j] : 0u;
+        // inclusive scan within the wave, then across the 4 waves
+        unsigned inc = x;
+        const int lane = threadIdx.x & 63;
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned y = __shfl_up(inc, o);
+            if (lane >= o) inc += y;
+        }
+        if (lane == 63) w[threadIdx.x >> 6] = inc;
+        __syncthreads();
+        unsigned before = 0, tot = 0;
+        for (int q = 0; q < 4; q++) {
+            if (q < (int)(threadIdx.x >> 6)) before += w[q];
+            tot += w[q];
+        }
+        if (j < b1) a.base[j] = carry + before + inc - x;
+        carry += tot;
+        __syncthreads();
+    }
+}
+
+__global__ void ml_advance_k(MultiSweep a) {
+    if (*a.done) return;
+    const int r = threadIdx.x;
+    if (r >= a.M) return;
+    const long long f0 = a.off[r], f1 = a.off[r + 1];
+    const long long c = f1 > f0 ? (long long)a.base[f1 - 1] + a.cnt[f1 - 1] - a.base[f0] : 0;
+    a.lo[r] = a.hi[r];
+    a.hi[r] += c;
+}
+
+hipError_t multi_bufs(MultiSweep &a, long long np, Workspace &ws, hipStream_t s) {
+    a.np = np;
+    a.nwords = (np + 31) / 32;
+    a.bq = ws.alloc_n<int>((size_t)kMaxRoots * np);
+    a.vb = ws.alloc_n<unsigned>((size_t)kMaxRoots * a.nwords);
+    a.claim = ws.alloc_n<int>((size_t)kMaxRoots * np);
+    a.lo = ws.alloc_n<long long>(kMaxRoots);
+    a.hi = ws.alloc_n<long long>(kMaxRoots);
+    a.off = ws.alloc_n<long long>(kMaxRoots + 1);
+    a.cnt = ws.alloc_n<unsigned>((size_t)kMaxRoots * np);
+    a.base = ws.alloc_n<unsigned>((size_t)kMaxRoots * np);
+    a.bsum = ws.alloc_n<unsigned>(kMlScanBlocks);
+    a.done = ws.alloc_n<int>(1);
+    int *roots = ws.alloc_n<int>(kMaxRoots);
+    a.roots = roots;
+    if (!a.bq || !a.vb || !a.claim || !a.lo || !a.hi || !a.off || !a.cnt || !a.base || !a.bsum ||
+        !a.done || !roots)
+        return hipErrorOutOfMemory;
+    PGALF_HIP_TRY(fill_u32(a.vb, kMaxRoots * a.nwords, 0u, s));
+    return fill_u32(reinterpret_cast<unsigned *>(a.claim), kMaxRoots * np, 0x7fffffffu, s);
+}
+
+// Shell sweeps (threshold t) of the M <= kMaxRoots roots; on return the
+// queues hold n[r] particles each (host array) and the visited bits are
+// clear again.
+hipError_t multi_sweep(MultiSweep &a, const int *roots_h, int M, float t, const float *den,
+                       const long long *nbr, int k, long long *n_h, hipStream_t s) {
+    a.M = M;
+    PGALF_HIP_TRY(hipMemcpyAsync(const_cast<int *>(a.roots), roots_h, sizeof(int) * M,
+                                 hipMemcpyHostToDevice, s));
+    ml_init_k<<<1, 64, 0, s>>>(a);
+    for (;;) {
+        for (int l = 0; l < kMlLevelsPerSync; l++) {
+            ml_plan_k<<<1, 1, 0, s>>>(a);
+            ml_claim_k<<<kMlGrid, 256, 0, s>>>(a, t, den, nbr, k);
+            ml_win_k<false><<<kMlGrid, 256, 0, s>>>(a, t, den, nbr, k);
+            ml_scan1_k<<<kMlScanBlocks, 256, 0, s>>>(a);
+            ml_scan2_k<<<1, 1024, 0, s>>>(a);
+            ml_scan3_k<<<kMlScanBlocks, 256, 0, s>>>(a);
+            ml_win_k<true><<<kMlGrid, 256, 0, s>>>(a, t, den, nbr, k);
+            ml_advance_k<<<1, 64, 0, s>>>(a);
+        }
+        int done = 0;
+        PGALF_HIP_TRY(to_host(&done, a.done, 1, s));
+        if (done) break;
+    }
+    PGALF_HIP_TRY(to_host(n_h, a.hi, M, s));
+    for (int r = 0; r < M; r++)
+        bits_clear_k<<<nblk(n_h[r], 256), 256, 0, s>>>(a.bq + (long long)r * a.np, n_h[r],
+                                                       a.vb + (long long)r * a.nwords);
+    return hipGetLastError();
+}
+
 // ---- shells ---------------------------------------------------------------
 
 struct ShellState {
@@ -1115,6 +1361,11 @@ __global__ void visited_mark_k(const int *list, long long n, unsigned char *flag
 
 __global__ void lastroot_reset_k(ShellState *st) { st->lastroot = -1; }
 
+__global__ void set_status_k(unsigned char *status, const int *ent, int n, unsigned char v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) status[ent[i]] = v;
+}
+
 __global__ void shell_state_init_k(ShellState *st, long long *part_ptr, long long *core_ptr) {
     st->nshell = st->npart = st->ncores = 0;
     st->stop = 0;
@@ -1129,11 +1380,6 @@ __global__ void fill_u8_k(unsigned char *p, long long n, unsigned char v) {
     if (i < n) p[i] = v;
 }
 
-template <class T>
-hipError_t to_host(T *h, const T *d, size_t n, hipStream_t s) {
-    PGALF_HIP_TRY(hipMemcpyAsync(h, d, sizeof(T) * n, hipMemcpyDeviceToHost, s));
-    return hipStreamSynchronize(s);
-}
 
 }  // namespace
 
@@ -1462,6 +1708,14 @@ hipError_t core_shells(const float *den, long long np, const long long *nbr, int
     };
     std::vector<unsigned char> hst(B), henc(nc1);
     std::vector<int> hbatch(B), hent;
+    // once a sweep has outgrown the workgroup tier, the lower thresholds'
+    // roots too large for the LDS go straight to the device-wide sweep (the
+    // sweeps only grow as the threshold falls)
+    bool giant_mode = false;
+    MultiSweep ms{};
+    bool have_ms = false;
+    std::vector<long long> hn(kMaxRoots);
+    std::vector<int> hslot(B);
     int last_root = -1;
     float last_thr = 0.0f;
     for (int ti = 0; ti < nthr; ti++) {
@@ -1469,7 +1723,8 @@ hipError_t core_shells(const float *den, long long np, const long long *nbr, int
         fill_u8_k<<<nblk(nc1, 256), 256, 0, s>>>(enclosed, nc1, 0);
         lastroot_reset_k<<<1, 1, 0, s>>>(st);
         int next = 0;
-        int spec = 1;  // workgroup sweeps launched at a time (grows while none is giant)
+        int spec = 1;    // workgroup sweeps launched at a time (grows while none is giant)
+        int mgroup = 1;  // device-wide sweeps run together (grows after the first)
         while (next < numcore) {
             const int nr = numcore - next;
             shell_cand_k<<<nblk(nr, 256), 256, 0, s>>>(coredensity, enclosed, next, numcore, t,
@@ -1505,8 +1760,25 @@ hipError_t core_shells(const float *den, long long np, const long long *nbr, int
                     return hipSuccess;
                 }
                 if (hs.stop >= nb) break;
-                PGALF_HIP_TRY(to_host(hst.data() + hs.stop, bstatus + hs.stop, nb - hs.stop, s));
-                if (hst[hs.stop] == kBig) {
+                const int i = hs.stop;
+                PGALF_HIP_TRY(to_host(hst.data() + i, bstatus + i, nb - i, s));
+                if (hst[i] == kHuge) {
+                    // a root swept device-wide: applied by scans
+                    const int c = hbatch[i];
+                    const long long n = hn[hslot[i]];
+                    const int *list = ms.bq + (long long)hslot[i] * np;
+                    big_flags_k<<<nblk(n, 256), 256, 0, s>>>(list, n, pk, core_of, flag, fp, ff);
+                    {
+                        WsScope sc(ws);
+                        PGALF_HIP_TRY(exclusive_scan_u32(fp, pp, n, tot, ws, s));
+                        PGALF_HIP_TRY(exclusive_scan_u32(ff, pf, n, tot + 1, ws, s));
+                    }
+                    big_write_k<<<nblk(n, 256), 256, 0, s>>>(a, list, n, fp, pp, ff, pf, tot);
+                    big_fin_k<<<1, 1, 0, s>>>(a, c, tot);
+                    startpos = i + 1;
+                    continue;
+                }
+                if (hst[i] == kBig && !giant_mode) {
                     // workgroup sweeps of this root and of the next not yet
                     // enclosed candidates too large for the LDS
                     if (!have_pool) {
@@ -1516,38 +1788,48 @@ hipError_t core_shells(const float *den, long long np, const long long *nbr, int
                     PGALF_HIP_TRY(to_host(henc.data(), enclosed, nc1, s));
                     const int want = spec < pool.slots ? spec : pool.slots;
                     hent.clear();
-                    for (int j = hs.stop; j < nb && (int)hent.size() < want; j++)
+                    for (int j = i; j < nb && (int)hent.size() < want; j++)
                         if (hst[j] == kBig && !henc[hbatch[j]]) hent.push_back(j);
                     PGALF_HIP_TRY(hipMemcpyAsync(ent, hent.data(), sizeof(int) * hent.size(),
                                                  hipMemcpyHostToDevice, s));
                     shell_block_k<<<(unsigned)hent.size(), kBlkT, 0, s>>>(
                         den, nbr, k, pk, peak, batch, ent, (int)hent.size(), t, pool, bstatus,
                         count, slot_of);
-                    PGALF_HIP_TRY(to_host(hst.data() + hs.stop, bstatus + hs.stop,
-                                          nb - hs.stop, s));
+                    PGALF_HIP_TRY(to_host(hst.data() + i, bstatus + i, nb - i, s));
                     bool giant = false;
                     for (int j : hent) giant |= hst[j] == kGiant;
+                    giant_mode |= giant;
                     spec = giant ? 1 : (spec * 4 < pool.slots ? spec * 4 : pool.slots);
-                    startpos = hs.stop;
+                    startpos = i;
                     continue;
                 }
-                // kGiant: swept and applied device-wide
-                const int c = hbatch[hs.stop];
-                PGALF_HIP_TRY(get_hpeak());
-                PGALF_HIP_TRY(get_lb());
-                long long n;
-                bool hit;
-                PGALF_HIP_TRY(level_sweep(lb, kShell, hpeak[c], t, den, nbr, k, pk, nullptr, &n,
-                                          &hit, ws, s));
-                big_flags_k<<<nblk(n, 256), 256, 0, s>>>(lb.bq, n, pk, core_of, flag, fp, ff);
-                {
-                    WsScope sc(ws);
-                    PGALF_HIP_TRY(exclusive_scan_u32(fp, pp, n, tot, ws, s));
-                    PGALF_HIP_TRY(exclusive_scan_u32(ff, pf, n, tot + 1, ws, s));
+                // kGiant, or kBig past the first giant: device-wide sweeps of
+                // this root and of the next not yet enclosed such candidates
+                giant_mode = true;
+                if (!have_ms) {
+                    PGALF_HIP_TRY(multi_bufs(ms, np, ws, s));
+                    have_ms = true;
                 }
-                big_write_k<<<nblk(n, 256), 256, 0, s>>>(a, lb.bq, n, fp, pp, ff, pf, tot);
-                big_fin_k<<<1, 1, 0, s>>>(a, c, tot);
-                startpos = hs.stop + 1;
+                PGALF_HIP_TRY(get_hpeak());
+                PGALF_HIP_TRY(to_host(henc.data(), enclosed, nc1, s));
+                hent.clear();
+                std::vector<int> roots;
+                for (int j = i; j < nb && (int)hent.size() < mgroup; j++)
+                    if ((hst[j] == kGiant || hst[j] == kBig) && !henc[hbatch[j]]) {
+                        hent.push_back(j);
+                        roots.push_back(hpeak[hbatch[j]]);
+                    }
+                PGALF_HIP_TRY(multi_sweep(ms, roots.data(), (int)roots.size(), t, den, nbr, k,
+                                          hn.data(), s));
+                for (size_t r = 0; r < hent.size(); r++) {
+                    hslot[hent[r]] = (int)r;
+                    hst[hent[r]] = kHuge;
+                }
+                PGALF_HIP_TRY(hipMemcpyAsync(ent, hent.data(), sizeof(int) * hent.size(),
+                                             hipMemcpyHostToDevice, s));
+                set_status_k<<<1, 64, 0, s>>>(bstatus, ent, (int)hent.size(), kHuge);
+                mgroup = mgroup * 4 < kMaxRoots ? mgroup * 4 : kMaxRoots;
+                startpos = i;
             }
         }
         ShellState hs;

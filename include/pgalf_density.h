@@ -181,7 +181,8 @@ int pgalf_merging_peak(pgalf_particle_t *bp, int np, pgalf_core_t *core, int num
  * BOUND) and haloid (the core of each member, PGALF_NOT_HALO_MEMBER
  * elsewhere).  The reference spreads the cores over OpenMP threads; this is
  * the one-thread result: a particle among the members of several cores has
- * the last core's id.  NumNeighbor <= 32.  On error -1 (pgalf_last_error). */
+ * the last core's id.  NumNeighbor <= 32, np < 2^26.  On error -1
+ * (pgalf_last_error). */
 int pgalf_find_core_density(pgalf_particle_t *bp, int np, long long *neighbor, int NumNeighbor,
                             pgalf_core_t *core, int numcore, const float *den, int *haloid,
                             unsigned char *flag);
