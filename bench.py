@@ -156,11 +156,13 @@ def cpu_baseline(pos, mass, spec, k, knn_stride):
             "sample": sample, "stages_s": {a: round(b, 4) for a, b in st.items()}}
 
 
-def peaks_bench(x, y, z, g, den, rho_t, nbr, rho_k, spec, reps=10, cpu=True):
+def peaks_bench(x, y, z, m, g, den, rho_t, nbr, rho_k, spec, reps=10, cpu=True):
     """The integer peak stage on the headline halo's own stage outputs: grid
     cores (nnost.c:1115-1271; densph = the read-back densities), finddenpeak
     (subhaloden.mod6.c:416-492; the W4 kNN densities and 32-NN lists),
-    MergingPeak of the grid cores (subhaloden.mod6.c:341-415, link 4e-3).
+    MergingPeak of the grid cores (subhaloden.mod6.c:341-415, link 4e-3),
+    and the watershed on the finddenpeak cores (FindCoreDensity, :1167-1475;
+    the shells, :572-678 and :2995-3024; FP64 masses).
     The synthetic halo's densities are not in the reference's units, so the
     thresholds are the medians of the positive grid values and of the kNN
     densities (the reference's PEAKTHRESHOLD = 2000 would select nothing;
@@ -217,6 +219,43 @@ def peaks_bench(x, y, z, g, den, rho_t, nbr, rho_k, spec, reps=10, cpu=True):
     t = timed(mp)
     out["merging_peak"] = {"ms": round(t, 4), "cores_in": int(core.numel()),
                            "cores_kept": int(res["mp"][1].item())}
+    # the watershed of subhalo_den on the finddenpeak cores: FindCoreDensity,
+    # the shell thresholds, GetShellParticlesPeaks over all of them and
+    # SaveRemainingParticles2LastShell (host-orchestrated: wall time around
+    # synchronised calls, median of 3)
+    peak = res["fd"][0][:int(res["fd"][1].item())].to(torch.int32).contiguous()
+    m64 = m.double()
+
+    def wall(fn, n=3):
+        ts = []
+        for _ in range(n):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        return float(np.median(ts))
+
+    def fcd():
+        res["fcd"] = D.dev_find_core_density(rho_k, nbr, peak, types, m64)
+    t_fcd = wall(fcd)
+    o = res["fcd"]
+    thr = D.dev_shell_thresholds(rho_k, o["coredensity"])
+
+    def shells(nthr):
+        def run():
+            fl, hid = o["flag"].clone(), o["haloid"].clone()
+            res["sh"] = D.dev_shell_particles_peaks(rho_k, nbr, peak, o["coredensity"],
+                                                    thr[:nthr], fl, hid)
+        return run
+    t_sh = wall(shells(len(thr)))
+    nshell = int(res["sh"][0])
+    t_sh6 = wall(shells(6))
+    out["watershed"] = {
+        "cores": int(peak.numel()), "find_core_density_ms": round(t_fcd, 3),
+        "members": int(o["nummem"].sum().item()), "thresholds": len(thr),
+        "shells_ms": round(t_sh, 2), "shells": nshell,
+        "shells_upper6_ms": round(t_sh6, 2)}
     if cpu:
         from oracle import oracle as O
         R = O.ref_lib()
@@ -248,6 +287,19 @@ def peaks_bench(x, y, z, g, den, rho_t, nbr, rho_k, spec, reps=10, cpu=True):
             t0 = time.perf_counter()
             O.ref_merging_peak(pos, gcore, rt_h[gcore])
             c["merging_peak_reference_s"] = round(time.perf_counter() - t0, 4)
+            # the reference's FindCoreDensity on all threads, and with its
+            # shells over the six upper thresholds (the lower ones take the
+            # reference hours: every sweep scans the core array per peak)
+            pk_h = peak.cpu().numpy()
+            m_h = m64.cpu().numpy()
+            t0 = time.perf_counter()
+            O.ref_core_shells(rk_h, nb_h, pk_h, ty_h, m_h, nthreads=O.num_threads())
+            c["find_core_density_reference_s"] = round(time.perf_counter() - t0, 4)
+            t0 = time.perf_counter()
+            O.ref_core_shells(rk_h, nb_h, pk_h, ty_h, m_h, thr=thr[:6],
+                              nthreads=O.num_threads())
+            c["find_core_density_shells_upper6_reference_s"] = round(
+                time.perf_counter() - t0, 4)
         c["threads"] = O.num_threads()
         out["cpu"] = c
     return out
@@ -622,6 +674,9 @@ def main():
                     help="only the FoF catalogue measurement")
     ap.add_argument("--no-catalog", action="store_true",
                     help="skip the FoF catalogue load / batch / write measurement")
+    ap.add_argument("--peaks-only", action="store_true",
+                    help="one density stage on the headline halo, then only the peak stage "
+                         "and the watershed (the 'peaks' object; for profiling)")
     ap.add_argument("--no-peaks", action="store_true",
                     help="skip the integer peak stage measurement")
     ap.add_argument("--no-grad", action="store_true",
@@ -681,6 +736,13 @@ def main():
     def step():
         D.dev_density_stage(x, y, z, m, g, spec["RG"], k, den, rho_t, nbr, rho_k)
 
+    if args.peaks_only:
+        step()
+        D.dev_status()
+        print(json.dumps({"peaks": peaks_bench(x, y, z, m, g, den, rho_t, nbr, rho_k, spec,
+                                               reps=3, cpu=not args.no_cpu_baseline)}),
+              flush=True)
+        return
     for _ in range(args.warmup):
         step()
     D.dev_status()
@@ -759,7 +821,7 @@ def main():
         "roofline": roofline,
     }
     if not args.no_peaks:
-        out["peaks"] = peaks_bench(x, y, z, g, den, rho_t, nbr, rho_k, spec,
+        out["peaks"] = peaks_bench(x, y, z, m, g, den, rho_t, nbr, rho_k, spec,
                                    cpu=not args.no_cpu_baseline)
     if not args.no_catalog:
         out["catalog"] = catalog_bench(cpu=not args.no_cpu_baseline)

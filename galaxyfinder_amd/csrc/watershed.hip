@@ -1388,10 +1388,11 @@ __global__ void fill_u8_k(unsigned char *p, long long n, unsigned char v) {
 namespace {
 
 // Pool of workgroup sweeps: at most `want` slots, bitmaps + queues within
-// about 1 GiB.
+// about 1 GiB.  A workgroup sweep holds up to 64K particles: past that the
+// device-wide sweeps are faster.
 hipError_t make_pool(Pool &p, long long np, int want, Workspace &ws, hipStream_t s) {
     p.nwords = (np + 31) / 32;
-    p.cap = np + kBlkT < (1LL << 18) ? np + kBlkT : (1LL << 18);
+    p.cap = np + kBlkT < (1LL << 16) ? np + kBlkT : (1LL << 16);
     const long long per = p.nwords * 4 + p.cap * 4;
     long long slots = (1LL << 30) / (per > 0 ? per : 1);
     slots = slots < want ? slots : want;

@@ -198,7 +198,7 @@ def test_watershed_sweeps_beyond_the_lds_and_workgroup_tiers():
     """Two clumps far apart, one core each at its densest particle: no sweep
     reaches the other core, so every threshold trial passes and the final
     sweeps hold whole clumps (280000 particles: past the LDS tier of 448 and
-    the workgroup tier of 261120); the shells' lower thresholds sweep them
+    the workgroup tier of 64512); the shells' lower thresholds sweep them
     too.  Plus 30 extra cores at random dense particles of the first clump."""
     rng = np.random.default_rng(77)
     m = 280000
