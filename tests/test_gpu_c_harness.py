@@ -1,4 +1,4 @@
-"""The library called from C exactly as NewGalFinder would call it: the
+"""The library called from C exactly as NewGalFinder would call it.  The
 harness tests/c_harness/pgalf_harness.c includes include/pgalf_density.h,
 links -lpgalf_density and calls assign_density_TSC, findDen,
 gaussian_Smoothing, findsphdensity and starfindsphdensity with host arrays
@@ -11,6 +11,11 @@ reference's outputs and the oracle:
   * findsphdensity lists bit-exact against the reference's cold-start
     Find_Near_PreCopied lists, densities bit-exact against the oracle;
   * starfindsphdensity densities and lists bit-exact against the oracle.
+
+The second harness, tests/c_harness/pgalf_watershed_harness.c, runs the
+watershed of subhalo_den through pgalf_find_core_density,
+pgalf_shell_thresholds and pgalf_shell_particles_peaks; its outputs are
+checked bit-exact against the reference's (watershed_small.npz).
 """
 import os
 import subprocess
@@ -71,3 +76,53 @@ def test_c_caller_on_reference_fixtures(tmp_path):
     sidx, sden = O.starfindsphdensity(pos, mass, types, k)
     assert np.array_equal(rd("star_near.bin", np.int64).reshape(n, k), sidx)
     assert np.array_equal(rd("star_den.bin", np.float32).view(np.int32), sden.view(np.int32))
+
+
+WS_HARNESS = os.path.join(ROOT, "tests", "c_harness", "pgalf_watershed_harness")
+
+
+@pytest.mark.parametrize("name", ["fdp", "rand"])
+def test_c_caller_watershed_on_reference_fixture(tmp_path, name):
+    """tests/c_harness/pgalf_watershed_harness.c runs subhalo_den's watershed
+    block (subhaloden.mod6.c:2978-3024) through pgalf_find_core_density,
+    pgalf_shell_thresholds and pgalf_shell_particles_peaks; its outputs equal
+    the reference's (fixture tests/golden/watershed_small.npz, written by the
+    reference's FindCoreDensity and GetShellParticlesPeaks built in place)."""
+    import zlib
+    if not os.path.exists(WS_HARNESS):
+        pytest.fail("C harness not built (make -C tests/c_harness, run by build())")
+    f = np.load(os.path.join(GOLD, "watershed_small.npz"))
+    pos = f["pos"].astype(np.float64)
+    idx, _ = O.knn(pos, int(f["k"]))
+    assert zlib.crc32(np.ascontiguousarray(idx, dtype=np.int64).tobytes()) == int(f["knn_crc"])
+    n, k = idx.shape
+    peak = f[name + "_peak"].astype(np.int32)
+    D.make_particles(pos, f["mass"], f["types"]).tofile(tmp_path / "particles.bin")
+    np.ascontiguousarray(idx, dtype=np.int64).tofile(tmp_path / "neighbor.bin")
+    np.ascontiguousarray(f[name + "_den"], dtype=np.float32).tofile(tmp_path / "den.bin")
+    peak.tofile(tmp_path / "peak.bin")
+    (tmp_path / "params.txt").write_text("%d %d %d\n" % (n, k, len(peak)))
+    print("[case] C watershed harness %s: N=%d, %d cores" % (name, n, len(peak)), flush=True)
+    r = subprocess.run([WS_HARNESS, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    core = np.fromfile(tmp_path / "core.bin", dtype=D.CORE_DTYPE)
+    for key in ("coredensity", "nummem", "numstar", "starmass"):
+        assert np.array_equal(np.ascontiguousarray(core[key]).view(np.int32),
+                              np.ascontiguousarray(f[name + "_" + key]).view(np.int32)), key
+    assert np.array_equal(np.fromfile(tmp_path / "haloid.bin", dtype=np.int32),
+                          f[name + "_haloid2"])
+    assert np.array_equal(np.fromfile(tmp_path / "flag.bin", dtype=np.uint8), f[name + "_flag2"])
+    assert np.array_equal(np.fromfile(tmp_path / "thr.bin", dtype=np.float32), f[name + "_thr"])
+    raw = (tmp_path / "shells.bin").read_bytes()
+    ns = int(np.frombuffer(raw, np.int64, 1)[0])
+    o = 8
+    pp = np.frombuffer(raw, np.int64, ns + 1, o)
+    o += 8 * (ns + 1)
+    part = np.frombuffer(raw, np.int32, int(pp[-1]), o)
+    o += 4 * int(pp[-1])
+    cp = np.frombuffer(raw, np.int64, ns + 1, o)
+    o += 8 * (ns + 1)
+    cores = np.frombuffer(raw, np.int32, int(cp[-1]), o)
+    assert np.array_equal(pp, f[name + "_part_ptr"]) and np.array_equal(part, f[name + "_part"])
+    assert np.array_equal(cp, f[name + "_core_ptr"]) and np.array_equal(cores, f[name + "_cores"])
