@@ -884,9 +884,10 @@ __global__ __launch_bounds__(kBlkT) void shell_block_k(const float *den, const l
 // level's size from device memory and exits when all sweeps are done.
 
 constexpr int kMaxRoots = 16;
-constexpr int kMlGrid = 2048;  // blocks of the grid-stride level kernels
+constexpr int kMlBlocksPerRoot = 256;  // grid x of the per-root level kernels
 constexpr int kMlScanBlocks = 1024;
 constexpr int kMlLevelsPerSync = 16;
+constexpr long long kMlCandCap = 1LL << 27;  // candidates of a level kept between passes
 
 struct MultiSweep {
     int M;
@@ -897,6 +898,8 @@ struct MultiSweep {
     long long *lo, *hi, *off;  // [M], [M], [M + 1]
     unsigned *cnt, *base;      // [M * np] per frontier node
     unsigned *bsum;            // [kMlScanBlocks]
+    int *cv;                   // [kMlCandCap]: a level's candidate particles (-1 none)
+    int *keep;                 // 1 when this level's candidates fit in cv
     int *done;
     const int *roots;          // [M] root particles
 };
@@ -921,69 +924,69 @@ __global__ void ml_plan_k(MultiSweep a) {
         o += a.hi[r] - a.lo[r];
     }
     a.off[a.M] = o;
+    *a.keep = o * 32 <= kMlCandCap ? 1 : 0;
     if (o == 0) *a.done = 1;
 }
 
-__device__ __forceinline__ int ml_root_of(const MultiSweep &a, long long g) {
-    int r = 0;
-    while (r + 1 < a.M && a.off[r + 1] <= g) r++;
-    return r;
-}
-
-// Candidate i = node * 32 + slot of the level: its particle (or -1).
-__device__ __forceinline__ int ml_cand(const MultiSweep &a, long long i, float t, const float *den,
-                                       const long long *nbr, int k, int &r, long long &f) {
-    const long long g = i >> 5;
-    const int s = (int)(i & 31);
-    r = ml_root_of(a, g);
-    f = g - a.off[r];
+// Candidate j = node * 32 + slot of root r's level: its particle, or -1.
+__device__ __forceinline__ int ml_cand(const MultiSweep &a, int r, long long j, float t,
+                                       const float *den, const long long *nbr, int k) {
+    const int s = (int)(j & 31);
     if (s >= k) return -1;
-    const int u = a.bq[(long long)r * a.np + a.lo[r] + f];
+    const int u = a.bq[(long long)r * a.np + a.lo[r] + (j >> 5)];
     const int v = (int)nbr[(long long)u * k + s];
     if (!(den[v] > t)) return -1;
     if ((a.vb[(long long)r * a.nwords + (v >> 5)] >> (v & 31)) & 1u) return -1;
     return v;
 }
 
+// Root r = blockIdx.y; its candidates j in [0, nf_r * 32), global index
+// off[r] * 32 + j.  The claim pass keeps each candidate's particle in cv
+// when the level fits, so the two later passes read it back coalesced.
 __global__ void ml_claim_k(MultiSweep a, float t, const float *den, const long long *nbr, int k) {
     if (*a.done) return;
-    const long long n = a.off[a.M] * 32;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x) {
-        int r;
-        long long f;
-        const int v = ml_cand(a, i, t, den, nbr, k, r, f);
-        if (v >= 0) atomicMin(&a.claim[(long long)r * a.np + v], (int)(f * 32 + (i & 31)));
+    const int r = blockIdx.y;
+    if (r >= a.M) return;
+    const long long n = (a.hi[r] - a.lo[r]) * 32, g0 = a.off[r] * 32;
+    const bool keep = *a.keep != 0;
+    for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < n;
+         j += (long long)gridDim.x * blockDim.x) {
+        const int v = ml_cand(a, r, j, t, den, nbr, k);
+        if (keep) a.cv[g0 + j] = v;
+        if (v >= 0) atomicMin(&a.claim[(long long)r * a.np + v], (int)j);
     }
 }
 
-// Winners of each node: count (count pass) or append (push pass).  The grid
+// Winners of each node: count (count pass) or append (push pass).  The
 // stride is a multiple of 64, so a wave always holds two whole nodes.
 template <bool kPush>
 __global__ void ml_win_k(MultiSweep a, float t, const float *den, const long long *nbr, int k) {
     if (*a.done) return;
-    const long long n = a.off[a.M] * 32;
+    const int r = blockIdx.y;
+    if (r >= a.M) return;
+    const long long n = (a.hi[r] - a.lo[r]) * 32, g0 = a.off[r] * 32;
+    const bool keep = *a.keep != 0;
     const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long i0 = (long long)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
-        const long long i = i0 + threadIdx.x;
-        int r = 0, v = -1;
-        long long f = 0;
+    int *claim = a.claim + (long long)r * a.np;
+    for (long long j0 = (long long)blockIdx.x * blockDim.x; j0 < n; j0 += stride) {
+        const long long j = j0 + threadIdx.x;
+        int v = -1;
         bool win = false;
-        if (i < n) {
-            v = ml_cand(a, i, t, den, nbr, k, r, f);
-            win = v >= 0 && a.claim[(long long)r * a.np + v] == (int)(f * 32 + (i & 31));
+        if (j < n) {
+            v = keep ? a.cv[g0 + j] : ml_cand(a, r, j, t, den, nbr, k);
+            win = v >= 0 && claim[v] == (int)j;
         }
         const unsigned long long m = __ballot(win);
         const int lane = threadIdx.x & 63;
         const unsigned half = (unsigned)(m >> (lane & 32));
         if (!kPush) {
-            if (i < n && (lane & 31) == 0) a.cnt[i >> 5] = __popc(half);
+            if (j < n && (lane & 31) == 0) a.cnt[a.off[r] + (j >> 5)] = __popc(half);
         } else if (win) {
-            const long long pos = a.base[i >> 5] - a.base[a.off[r]] +
+            const long long pos = a.base[a.off[r] + (j >> 5)] - a.base[a.off[r]] +
                                   __popc(half & ((1u << (lane & 31)) - 1u));
             a.bq[(long long)r * a.np + a.hi[r] + pos] = v;
             atomicOr(&a.vb[(long long)r * a.nwords + (v >> 5)], 1u << (v & 31));
-            a.claim[(long long)r * a.np + v] = INT_MAX;
+            claim[v] = INT_MAX;
         }
     }
 }
@@ -1070,11 +1073,13 @@ hipError_t multi_bufs(MultiSweep &a, long long np, Workspace &ws, hipStream_t s)
     a.cnt = ws.alloc_n<unsigned>((size_t)kMaxRoots * np);
     a.base = ws.alloc_n<unsigned>((size_t)kMaxRoots * np);
     a.bsum = ws.alloc_n<unsigned>(kMlScanBlocks);
+    a.cv = ws.alloc_n<int>(kMlCandCap);
+    a.keep = ws.alloc_n<int>(1);
     a.done = ws.alloc_n<int>(1);
     int *roots = ws.alloc_n<int>(kMaxRoots);
     a.roots = roots;
     if (!a.bq || !a.vb || !a.claim || !a.lo || !a.hi || !a.off || !a.cnt || !a.base || !a.bsum ||
-        !a.done || !roots)
+        !a.cv || !a.keep || !a.done || !roots)
         return hipErrorOutOfMemory;
     PGALF_HIP_TRY(fill_u32(a.vb, kMaxRoots * a.nwords, 0u, s));
     return fill_u32(reinterpret_cast<unsigned *>(a.claim), kMaxRoots * np, 0x7fffffffu, s);
@@ -1092,12 +1097,13 @@ hipError_t multi_sweep(MultiSweep &a, const int *roots_h, int M, float t, const 
     for (;;) {
         for (int l = 0; l < kMlLevelsPerSync; l++) {
             ml_plan_k<<<1, 1, 0, s>>>(a);
-            ml_claim_k<<<kMlGrid, 256, 0, s>>>(a, t, den, nbr, k);
-            ml_win_k<false><<<kMlGrid, 256, 0, s>>>(a, t, den, nbr, k);
+            const dim3 grid(kMlBlocksPerRoot, M);
+            ml_claim_k<<<grid, 256, 0, s>>>(a, t, den, nbr, k);
+            ml_win_k<false><<<grid, 256, 0, s>>>(a, t, den, nbr, k);
             ml_scan1_k<<<kMlScanBlocks, 256, 0, s>>>(a);
             ml_scan2_k<<<1, 1024, 0, s>>>(a);
             ml_scan3_k<<<kMlScanBlocks, 256, 0, s>>>(a);
-            ml_win_k<true><<<kMlGrid, 256, 0, s>>>(a, t, den, nbr, k);
+            ml_win_k<true><<<grid, 256, 0, s>>>(a, t, den, nbr, k);
             ml_advance_k<<<1, 64, 0, s>>>(a);
         }
         int done = 0;
