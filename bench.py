@@ -305,6 +305,66 @@ def peaks_bench(x, y, z, m, g, den, rho_t, nbr, rho_k, spec, reps=10, cpu=True):
     return out
 
 
+def error_report():
+    """Observed errors of the GPU stage against FP64 on the one-halo
+    workloads: the deposit and the smoothed grid (normwise over the grid,
+    pointwise over cells >= 1e-3 of the maximum), the read-back and the W4
+    kNN densities (normwise and pointwise over all particles).  Two FP64
+    references: the reference's own arithmetic carried in FP64 (FP32 TSC
+    weights as tsc_omp2.c forms them, sums and transforms in FP64) and a
+    stage computed in FP64 throughout (FP64 weights)."""
+    import torch
+    from galaxyfinder_amd import density as D, halos
+    from oracle import oracle as O
+
+    def err(got, ref, mask_rel=None):
+        got = np.asarray(got, np.float64).ravel()
+        ref = np.asarray(ref, np.float64).ravel()
+        nrm = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+        m = np.abs(ref) >= (mask_rel or 0.0) * np.abs(ref).max()
+        m &= ref != 0
+        return {"normwise": nrm, "pointwise": float((np.abs(got - ref)[m] / np.abs(ref)[m]).max())}
+
+    out = {}
+    for cfg in (1, 4, 2):
+        pos, mass, spec = halos.single_halo_config(cfg)
+        n, nx, k = len(pos), spec["nx"], spec["k"]
+        grid = (nx, spec["ny"], spec["nz"], spec["xmin"], spec["ymin"], spec["zmin"],
+                spec["cellsize"])
+        g = D.grid_struct(*grid)
+        dev = torch.device("cuda")
+        x, y, z = (torch.from_numpy(np.ascontiguousarray(pos[:, a])).to(dev) for a in range(3))
+        m = torch.from_numpy(np.ascontiguousarray(mass)).to(dev)
+        shape = D.padded_shape(nx, spec["ny"], spec["nz"])
+        dep = torch.empty(shape, dtype=torch.float32, device=dev)
+        D.dev_tsc(x, y, z, m, g, dep)
+        dep_h = dep.cpu().numpy()[..., :nx].astype(np.float64)
+        den = torch.empty(shape, dtype=torch.float32, device=dev)
+        rt = torch.empty(n, dtype=torch.float32, device=dev)
+        nbr = torch.empty((n, k), dtype=torch.int64, device=dev)
+        rk = torch.empty(n, dtype=torch.float32, device=dev)
+        D.dev_density_stage(x, y, z, m, g, spec["RG"], k, den, rt, nbr, rk)
+        D.dev_status()
+        den_h = den.cpu().numpy()[..., :nx].astype(np.float64)
+        rt_h, rk_h, idx = rt.cpu().numpy(), rk.cpu().numpy(), nbr.cpu().numpy()
+        pos64 = pos.astype(np.float64)
+        # the reference's arithmetic in FP64: its FP32 weights, FP64 sums
+        dref, _ = O.tsc_deposit(pos64, mass, grid, fp64=True)
+        dref = np.ascontiguousarray(dref[..., :nx])
+        sref = O.gsmooth_f64(dref, spec["cellsize"], spec["RG"])
+        # FP64 throughout
+        rt64, rk64, F64 = O.stage_forward_d(pos64, mass, grid, spec["RG"], idx, weighted=False)
+        out["halo_%d" % n] = {
+            "deposit_vs_fp64_sums": err(dep_h, dref, 1e-3),
+            "deposit_vs_fp64_weights": err(dep_h, O.tsc_deposit_d(pos64, mass, grid), 1e-3),
+            "smoothed_vs_fp64_sums": err(den_h, sref, 1e-3),
+            "smoothed_vs_fp64_stage": err(den_h, F64, 1e-3),
+            "readback_vs_fp64_stage": err(rt_h, rt64),
+            "knn_density_vs_fp64": err(rk_h, rk64)}
+        print("[bench] errors: %d-particle halo done" % n, file=sys.stderr, flush=True)
+    return out
+
+
 def catalog_bench(nhalo=2000, seed=3, cpu=True):
     """The catalogue path on a synthetic FoF catalogue written to a scratch
     directory: nhalo halos of halos.catalogue_vec (dN/dn ~ n^-2 on
@@ -674,6 +734,9 @@ def main():
                     help="only the FoF catalogue measurement")
     ap.add_argument("--no-catalog", action="store_true",
                     help="skip the FoF catalogue load / batch / write measurement")
+    ap.add_argument("--errors", action="store_true",
+                    help="observed errors of the stage's outputs against the FP64 oracle on "
+                         "the 131072-, 4194304- and 1048576-particle halos (one JSON line)")
     ap.add_argument("--peaks-only", action="store_true",
                     help="one density stage on the headline halo, then only the peak stage "
                          "and the watershed (the 'peaks' object; for profiling)")
@@ -717,6 +780,9 @@ def main():
     if args.batch_only:
         print(json.dumps({"batch": batch_bench(args.steps, args.warmup, plain_pass=False)}),
               flush=True)
+        return
+    if args.errors:
+        print(json.dumps({"errors": error_report()}), flush=True)
         return
     pos, mass, spec = halos.single_halo_config(args.config)
     n, k = len(pos), spec["k"]
