@@ -90,6 +90,10 @@ SIGNATURES = {
     "pgalf_dev_shell_thresholds": (_c_i, [_c_p, _c_ll, _c_p, _c_i, _c_p, _c_p, _c_p]),
     "pgalf_dev_shell_particles_peaks": (_c_i, [_c_p, _c_ll, _c_p, _c_i, _c_p, _c_p, _c_i, _c_p,
                                                _c_i, _c_p, _c_p, _c_p, _c_p]),
+    "pgalf_catalog_shell_layout": (_c_ll, [_c_i, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p]),
+    "pgalf_dev_catalog_watershed": (_c_i, [_c_i, _c_p, _c_p, _c_p, _c_p, _c_i, _c_p, _c_p, _c_p,
+                                           _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p,
+                                           _c_p, _c_p, _c_ll, _c_p, _c_p, _c_p]),
     "pgalf_batch_create": (_c_p, [_c_i, _c_p, _c_p, _c_p, _c_i]),
     "pgalf_batch_grid_offset": (_c_ll, [_c_p, _c_i]),
     "pgalf_batch_run": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i,

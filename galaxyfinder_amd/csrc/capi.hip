@@ -1372,6 +1372,71 @@ extern "C" int pgalf_dev_shell_particles_peaks(const float *den, long long np,
                       shells, false, ws, s, where);
 }
 
+extern "C" long long pgalf_catalog_shell_layout(int nhalo, const long long *off,
+                                                const long long *core_off, long long *shell_base,
+                                                long long *core_base, long long *core_cap,
+                                                long long *total_cores) {
+    if (nhalo < 0 || !off || !core_off || (nhalo > 0 && (!shell_base || !core_base || !core_cap)) ||
+        !total_cores) {
+        set_err(PGALF_ERR_ARG, "pgalf_catalog_shell_layout: bad arguments");
+        return -1;
+    }
+    long long tot_s = 0;
+    catalog_shell_layout(nhalo, off, core_off, shell_base, core_base, core_cap, &tot_s,
+                         total_cores);
+    ok();
+    return tot_s;
+}
+
+extern "C" int pgalf_dev_catalog_watershed(int nhalo, const long long *off,
+                                           const long long *core_off, const float *den,
+                                           const long long *nbr, int k, const int *type,
+                                           const double *mass, const int *peak,
+                                           float *coredensity, int *nummem, int *numstar,
+                                           float *starmass, int *haloid, unsigned char *flag,
+                                           long long *part_ptr, int *part, long long *core_ptr,
+                                           int *cores, long long small_max, long long *nshell,
+                                           int *halo_status, void *stream) {
+    const char *where = "pgalf_dev_catalog_watershed";
+    if (nhalo < 0 || !off || !core_off || k < 1 || k > 32 || small_max < 0 ||
+        (nhalo > 0 && (!nshell || !halo_status)))
+        return set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
+    if (off[0] != 0 || core_off[0] != 0)
+        return set_err(PGALF_ERR_ARG, "%s: off[0] and core_off[0] must be 0", where);
+    for (int h = 0; h < nhalo; h++)
+        if (off[h + 1] < off[h] || core_off[h + 1] < core_off[h] ||
+            off[h + 1] - off[h] >= (1LL << 26))
+            return set_err(PGALF_ERR_ARG, "%s: halo %d has bad offsets or 2^26 particles", where,
+                           h);
+    const long long np = nhalo > 0 ? off[nhalo] : 0, nc = nhalo > 0 ? core_off[nhalo] : 0;
+    if ((np > 0 && (!den || !nbr || !type || !mass || !haloid || !flag || !part)) ||
+        (nc > 0 && (!peak || !coredensity || !nummem || !numstar || !starmass || !cores)) ||
+        (nhalo > 0 && (!part_ptr || !core_ptr)))
+        return set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
+    std::lock_guard<std::mutex> lk(g_api_mu);
+    hipStream_t s = (hipStream_t)stream;
+    Workspace &ws = workspace_for_current_device();
+    WsCall call(ws, s);
+    hipError_t e = call.e;
+    CoreOut o{coredensity, nummem, numstar, starmass, haloid, flag};
+    CatShellOut so{part_ptr, part, core_ptr, cores};
+    if (e == hipSuccess)
+        e = catalog_watershed(nhalo, off, core_off, den, nbr, k, type, mass, peak, o, so,
+                              small_max > 0 ? small_max : 65536, nshell, halo_status, ws, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_err(e, where);
+    const int st = consume_status(ws, s, where);
+    if (st != PGALF_OK) return st;
+    for (int h = 0; h < nhalo; h++) {
+        if (halo_status[h] == 13)
+            return set_err(PGALF_ERR_CORE_OVERFLOW, "%s: halo %d's core entries outgrew its room",
+                           where, h);
+        if (halo_status[h])
+            return set_err(PGALF_ERR_ARG, "%s: halo %d failed (%d)", where, h, halo_status[h]);
+    }
+    return ok();
+}
+
 // ---- launch schedule ---------------------------------------------------------------------
 extern "C" int pgalf_set_schedule(const pgalf_schedule_t *bins, int nbins,
                                   int smooth_direct_max) {

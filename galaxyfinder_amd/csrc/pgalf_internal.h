@@ -212,6 +212,29 @@ hipError_t find_core_density(const float *den, long long np, const long long *nb
 // The shell thresholds (host array thr[15], *nthr of them).
 hipError_t shell_thresholds(const float *den, long long np, const float *coredensity,
                             int numcore, float *thr, int *nthr, Workspace &ws, hipStream_t s);
+// Catalogue watershed: halo h's particles [off[h], off[h+1]) and cores
+// [coff[h], coff[h+1]) (host arrays), lists and core peaks local to the
+// halo.  Shells of halo h: part_ptr / core_ptr entries at sbase[h] (nc*nsd +
+// 2 of them), its particles at off[h] of part, core entries at cbase[h]
+// (ccap[h] of them) -- the layout catalog_shell_layout gives.
+struct CatShellOut {
+    long long *part_ptr;
+    int *part;
+    long long *core_ptr;
+    int *cores;
+};
+void catalog_shell_layout(int nhalo, const long long *off, const long long *coff,
+                          long long *sbase, long long *cbase, long long *ccap,
+                          long long *tot_shell, long long *tot_cores);
+// Halos of at most small_max particles run one per workgroup; the others
+// (and any that outgrow it) through the one-halo tiers.  herr_h[h]: 0, or a
+// non-zero code for a halo whose results are not valid (a search that did
+// not converge: 2; shells: 10 + the core_shells error).
+hipError_t catalog_watershed(int nhalo, const long long *off_h, const long long *coff_h,
+                             const float *den, const long long *nbr, int k, const int *type,
+                             const double *mass, const int *peak, const CoreOut &o,
+                             const CatShellOut &so, long long small_max, long long *nshell_h,
+                             int *herr_h, Workspace &ws, hipStream_t s);
 // *err: 1 more shells than maxshell, 2 a root's own core never enclosed (two
 // cores sharing a peak), 3 more core entries than maxcores.
 hipError_t core_shells(const float *den, long long np, const long long *nbr, int k,

@@ -679,8 +679,13 @@ __device__ long long block_sweep(BlockLds &L, int mode, int root, float t, const
         if (f < nodes && s < k) {
             const int u = q[head + f];
             v = (int)nbr[(long long)u * k + s];
-            entry_test(mode, v, root, t, den, pk, stale, cand, hit);
-            // the bits are set by atomics of other waves: read them past L1
+            entry_test(mode, v, root, t, den, pk, nullptr, cand, hit);
+            // visited and stale bits may be set by atomics of other waves of
+            // this workgroup: read them past L1
+            if (mode != kShell && stale && (cand || hit) &&
+                ((__hip_atomic_load(&stale[v >> 5], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT) >> (v & 31)) & 1u))
+                cand = hit = false;
             if (cand && ((__hip_atomic_load(&vb[v >> 5], __ATOMIC_RELAXED,
                                             __HIP_MEMORY_SCOPE_AGENT) >>
                           (v & 31)) & 1u))
@@ -1387,6 +1392,366 @@ __global__ void fill_u8_k(unsigned char *p, long long n, unsigned char v) {
 }
 
 
+
+// ---- catalogue watershed --------------------------------------------------
+// A catalogue's halos up to a size limit: one 1024-thread workgroup per halo
+// (halos taken from a counter, largest first) runs FindCoreDensity, then the
+// shells, in the reference's own order -- cores one after another, roots one
+// after another -- with the workgroup sweeps; so the dropped cores' visited
+// marks and the last-core member ids come out as one thread leaves them.
+// Every index of a halo is local to it: its particles [off[h], off[h+1]),
+// its cores [coff[h], coff[h+1]) with local peak indices, its lists local
+// (as the batched stage writes them).
+
+constexpr int kMaxShellDivide = 15;
+
+struct CatArgs {
+    const long long *off, *coff;  // [nhalo + 1]
+    const int *order;             // the halos run here, largest first
+    int ntask;
+    int *next;                    // work counter
+    const float *den;
+    const long long *nbr;
+    int k;
+    const int *type;
+    const double *mass;
+    const int *peak;              // per core: local peak particle
+    float *coredensity;
+    int *nummem, *numstar;
+    float *starmass;
+    int *haloid;                  // per particle: local core or -9999
+    unsigned char *flag;
+    unsigned char *pk;            // per particle: 1 for a peak
+    int *core_of;                 // per particle: first local core of that peak
+    unsigned *bits;               // [slots][2 * nw]: visited bits, stale bits
+    int *q;                       // [slots][qcap]
+    long long nw, qcap;
+    int *herr;                    // per halo: 0 done, else left to the one-halo path
+    const float *thr;             // [nhalo][15]
+    const int *nthr;              // [nhalo]
+    unsigned char *enclosed;      // per core
+    long long *part_ptr, *core_ptr;  // halo h at sbase[h]
+    const long long *sbase;
+    int *part;                    // halo h at off[h]
+    int *cores;                   // halo h at cbase[h], ccap[h] entries
+    const long long *cbase, *ccap;
+    long long *nshell;            // [nhalo]
+};
+
+enum : int { kCatBig = 1, kCatSearch = 2, kCatBadPeak = 3, kCatCores = 4, kCatEnclose = 5 };
+
+__device__ __forceinline__ int cat_task(int *next, int *s_task) {
+    if (threadIdx.x == 0) *s_task = atomicAdd(next, 1);
+    __syncthreads();
+    const int t = *s_task;
+    __syncthreads();
+    return t;
+}
+
+__global__ __launch_bounds__(kBlkT) void cat_fcd_k(CatArgs a) {
+    __shared__ BlockLds L;
+    __shared__ int s_task, s_bad;
+    __shared__ unsigned s_min;
+    unsigned *vb = a.bits + (long long)blockIdx.x * 2 * a.nw;
+    unsigned *stale = vb + a.nw;
+    int *q = a.q + (long long)blockIdx.x * a.qcap;
+    const int tid = threadIdx.x;
+    for (;;) {
+        const int ti = cat_task(a.next, &s_task);
+        if (ti >= a.ntask) break;
+        const int h = a.order[ti];
+        const long long p0 = a.off[h], c0 = a.coff[h];
+        const int n = (int)(a.off[h + 1] - p0), nc = (int)(a.coff[h + 1] - c0);
+        const float *den = a.den + p0;
+        const long long *nbr = a.nbr + p0 * a.k;
+        unsigned char *flag = a.flag + p0, *pk = a.pk + p0;
+        int *haloid = a.haloid + p0;
+        if (tid == 0) {
+            s_bad = 0;
+            s_min = 0xffffffffu;
+        }
+        for (int i = tid; i < n; i += kBlkT) {
+            flag[i] = 0;
+            haloid[i] = kNotMember;
+            pk[i] = 0;
+        }
+        for (long long w = tid; w < (n + 31) / 32; w += kBlkT) stale[w] = 0u;
+        __syncthreads();
+        unsigned lo = 0xffffffffu;
+        for (int i = tid; i < n; i += kBlkT) {
+            const unsigned kk = ord_key(den[i]);
+            lo = kk < lo ? kk : lo;
+        }
+        atomicMin(&s_min, lo);
+        for (int c = tid; c < nc; c += kBlkT) {
+            const int p = a.peak[c0 + c];
+            if (p < 0 || p >= n) {
+                s_bad = 1;
+            } else {
+                pk[p] = 1;
+                flag[p] = kWpPeak;
+            }
+        }
+        __syncthreads();
+        if (s_bad) {
+            if (tid == 0) a.herr[h] = kCatBadPeak;
+            continue;
+        }
+        float gmin = ord_val(s_min);
+        gmin = gmin < 1.e23f ? gmin : 1.e23f;
+        const float minden = (float)(kDenFloor > (double)gmin ? kDenFloor : (double)gmin);
+        int err = 0;
+        for (int c = 0; c < nc && !err; c++) {
+            const int root = a.peak[c0 + c];
+            float up = den[root], dn = minden, t;
+            int steps = 0;
+            do {
+                t = core_trial(up, dn);
+                const long long r = block_sweep(L, kTrial, root, t, den, nbr, a.k, pk, stale, vb,
+                                                q, a.qcap, nullptr);
+                if (r == kSweepBig) {
+                    err = kCatBig;
+                    break;
+                }
+                if (r == kSweepHit) dn = t;
+                else up = t;
+                if (++steps > kMaxSearch) {
+                    err = kCatSearch;
+                    break;
+                }
+            } while (search_more(up, dn, t));
+            if (err) break;
+            bool fhit = false;
+            const long long nf = block_sweep(L, kFinal, root, up, den, nbr, a.k, pk, stale, vb, q,
+                                             a.qcap, &fhit);
+            if (nf == kSweepBig) {
+                err = kCatBig;
+                break;
+            }
+            int ns = 0;
+            float acc = 0.0f;
+            if (fhit) {
+                // the dropped core's sweep stays visited for the later cores
+                for (long long j = tid; j < nf; j += kBlkT) {
+                    const int v = q[j];
+                    atomicOr(&stale[v >> 5], 1u << (v & 31));
+                }
+            } else {
+                for (long long j = tid; j < nf; j += kBlkT) {
+                    const int v = q[j];
+                    flag[v] = (unsigned char)((flag[v] | kWpCore | kWpBound) & ~kWpRemaining);
+                    haloid[v] = c;
+                }
+                block_star_sum(L, q, nf, a.type + p0, a.mass + p0, &ns, &acc);
+            }
+            if (tid == 0) {
+                a.coredensity[c0 + c] = up;
+                a.nummem[c0 + c] = fhit ? 0 : (int)nf;
+                a.numstar[c0 + c] = ns;
+                a.starmass[c0 + c] = acc;
+            }
+            __syncthreads();
+        }
+        if (tid == 0) a.herr[h] = err;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlkT) void cat_shells_k(CatArgs a) {
+    __shared__ BlockLds L;
+    __shared__ int s_task, s_root, s_err;
+    __shared__ long long s_nshell, s_npart, s_ncores;
+    unsigned *vb = a.bits + (long long)blockIdx.x * 2 * a.nw;
+    int *q = a.q + (long long)blockIdx.x * a.qcap;
+    const int tid = threadIdx.x;
+    for (;;) {
+        const int ti = cat_task(a.next, &s_task);
+        if (ti >= a.ntask) break;
+        const int h = a.order[ti];
+        if (a.herr[h]) continue;  // uniform: every thread reads the same word
+        const long long p0 = a.off[h], c0 = a.coff[h];
+        const int n = (int)(a.off[h + 1] - p0), nc = (int)(a.coff[h + 1] - c0);
+        const float *den = a.den + p0;
+        const long long *nbr = a.nbr + p0 * a.k;
+        unsigned char *flag = a.flag + p0, *pk = a.pk + p0, *enc = a.enclosed + c0;
+        int *haloid = a.haloid + p0, *core_of = a.core_of + p0, *part = a.part + p0;
+        int *cores = a.cores + a.cbase[h];
+        long long *pp = a.part_ptr + a.sbase[h], *cp = a.core_ptr + a.sbase[h];
+        const float *cd = a.coredensity + c0;
+        const long long ccap = a.ccap[h];
+        for (int i = tid; i < n; i += kBlkT) core_of[i] = INT_MAX;
+        if (tid == 0) {
+            s_nshell = s_npart = s_ncores = 0;
+            s_err = 0;
+            pp[0] = cp[0] = 0;
+        }
+        __syncthreads();
+        for (int c = tid; c < nc; c += kBlkT) atomicMin(&core_of[a.peak[c0 + c]], c);
+        __syncthreads();
+        const int nsd = a.nthr[h];
+        int lastroot = -1;
+        float last_t = 0.0f;
+        for (int tix = 0; tix < nsd && !s_err; tix++) {
+            const float t = a.thr[(long long)h * kMaxShellDivide + tix];
+            for (int c = tid; c < nc; c += kBlkT) enc[c] = 0;
+            __syncthreads();
+            int cursor = 0, thr_root = -1;
+            for (;;) {
+                // GET_NONENCLOSED_ID (:555-563): the first core above t that
+                // no sweep of this threshold reached (the cores before the
+                // last root stay enclosed or below t)
+                if (tid == 0) s_root = INT_MAX;
+                __syncthreads();
+                for (int c0b = cursor; c0b < nc; c0b += kBlkT) {
+                    const int c = c0b + tid;
+                    if (c < nc && cd[c] > t && !enc[c]) atomicMin(&s_root, c);
+                    __syncthreads();
+                    const int found = s_root;
+                    __syncthreads();
+                    if (found != INT_MAX) break;
+                }
+                const int root = s_root;
+                __syncthreads();
+                if (root == INT_MAX) break;
+                cursor = root;
+                const long long ns = block_sweep(L, kShell, a.peak[c0 + root], t, den, nbr, a.k,
+                                                 pk, nullptr, vb, q, a.qcap, nullptr);
+                if (ns == kSweepBig) {
+                    if (tid == 0) s_err = kCatBig;
+                    __syncthreads();
+                    break;
+                }
+                long long nco = s_ncores, npa = s_npart;
+                __syncthreads();
+                bool over = false;
+                for (long long b = 0; b < ns; b += kBlkT) {
+                    const long long j = b + tid;
+                    const int v = j < ns ? q[j] : -1;
+                    const bool isp = v >= 0 && pk[v] && core_of[v] != INT_MAX;
+                    const bool isf = v >= 0 && !(flag[v] & (kWpShell | kWpCore));
+                    int tp, tf;
+                    const int ppos = block_prefix_1k(isp, L.wtot, tp);
+                    const int fpos = block_prefix_1k(isf, L.wtot, tf);
+                    if (nco + tp > ccap) {
+                        over = true;
+                        break;
+                    }
+                    if (isp) {
+                        const int cc = core_of[v];
+                        cores[nco + ppos] = cc;
+                        enc[cc] = 1;
+                    }
+                    if (isf) {
+                        part[npa + fpos] = v;
+                        flag[v] = (unsigned char)((flag[v] | kWpShell) & ~kWpRemaining);
+                        haloid[v] = kNotMember;
+                    }
+                    nco += tp;
+                    npa += tf;
+                    __syncthreads();
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    if (over) {
+                        s_err = kCatCores;
+                    } else if (!enc[root]) {
+                        s_err = kCatEnclose;
+                    } else {
+                        s_nshell++;
+                        s_ncores = nco;
+                        s_npart = npa;
+                        pp[s_nshell] = npa;
+                        cp[s_nshell] = nco;
+                    }
+                }
+                __syncthreads();
+                if (s_err) break;
+                thr_root = root;
+            }
+            lastroot = thr_root;
+            last_t = t;
+        }
+        if (!s_err) {
+            // SaveRemainingParticles2LastShell (:572-611)
+            long long npa = s_npart;
+            for (int b = 0; b < n; b += kBlkT) {
+                const int i = b + tid;
+                const bool isr = i < n && !(flag[i] & (kWpCore | kWpShell));
+                int tr;
+                const int rpos = block_prefix_1k(isr, L.wtot, tr);
+                if (isr) {
+                    part[npa + rpos] = i;
+                    flag[i] = (unsigned char)((flag[i] | kWpShell) & ~kWpRemaining);
+                    haloid[i] = kNotMember;
+                }
+                npa += tr;
+                __syncthreads();
+            }
+            if (tid == 0 && npa > s_npart) {
+                if (s_nshell == 0) {
+                    if (s_ncores + nc > ccap) {
+                        s_err = kCatCores;
+                    } else {
+                        for (int c = 0; c < nc; c++) cores[s_ncores + c] = c;
+                        s_ncores += nc;
+                        s_nshell = 1;
+                        cp[1] = s_ncores;
+                    }
+                }
+                s_npart = npa;
+                pp[s_nshell] = npa;
+            }
+            __syncthreads();
+        }
+        if (!s_err && lastroot >= 0) {
+            // the last sweep of the last threshold leaves its marks
+            const long long ns = block_sweep(L, kShell, a.peak[c0 + lastroot], last_t, den, nbr,
+                                             a.k, pk, nullptr, vb, q, a.qcap, nullptr);
+            for (long long j = tid; j < ns; j += kBlkT) flag[q[j]] |= kWpVisited;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            a.herr[h] = s_err;
+            a.nshell[h] = s_nshell;
+        }
+        __syncthreads();
+    }
+}
+
+// Per-halo minimum density and largest core density (ordered keys).
+__global__ void cat_minmax_k(const long long *off, const long long *coff, const float *den,
+                             const float *cd, unsigned *kmin, unsigned *kmax) {
+    const int h = blockIdx.x;
+    unsigned lo = 0xffffffffu, hi = 0u;
+    for (long long i = off[h] + threadIdx.x; i < off[h + 1]; i += blockDim.x) {
+        const unsigned kk = ord_key(den[i]);
+        lo = kk < lo ? kk : lo;
+    }
+    for (long long c = coff[h] + threadIdx.x; c < coff[h + 1]; c += blockDim.x) {
+        const unsigned kk = ord_key(cd[c]);
+        hi = kk > hi ? kk : hi;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned a2 = __shfl_down(lo, o), b2 = __shfl_down(hi, o);
+        lo = a2 < lo ? a2 : lo;
+        hi = b2 > hi ? b2 : hi;
+    }
+    __shared__ unsigned slo[4], shi[4];
+    if ((threadIdx.x & 63) == 0) {
+        slo[threadIdx.x >> 6] = lo;
+        shi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+            lo = slo[w] < lo ? slo[w] : lo;
+            hi = shi[w] > hi ? shi[w] : hi;
+        }
+        kmin[h] = lo;
+        kmax[h] = hi;
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1601,6 +1966,37 @@ hipError_t find_core_density(const float *den, long long np, const long long *nb
     return hipGetLastError();
 }
 
+// nshelldivide of subhalo_den: 10, or min(15, 10 log10(1.5 np)) for np > 1000.
+static int shell_divide(long long np) {
+    int nsd = 10;
+    if (np > 1000) {
+        const double a2 = 10.0 * log10((double)np * 1.5);
+        nsd = (int)(10.0 > a2 ? 10.0 : a2);
+        nsd = nsd < 15 ? nsd : 15;
+    }
+    return nsd;
+}
+
+// The shell thresholds of subhalo_den (subhaloden.mod6.c:2995-3024) from the
+// smallest particle density pmin and the largest core density cmax (1e27 /
+// -1e27 when there are none), in FP32 variables with glibc's log10 and pow
+// as the reference forms them; thr[0 .. return), highest first.
+static int thresholds_from(float pmin, float cmax, long long np, float *thr) {
+    float minshellden = 1.e27f, maxshellden = -1.e27f;
+    maxshellden = maxshellden > cmax ? maxshellden : cmax;
+    minshellden = minshellden < pmin ? minshellden : pmin;
+    minshellden = (float)(kDenFloor > (double)minshellden ? kDenFloor : (double)minshellden);
+    minshellden = (float)log10((double)minshellden + 1.0);
+    maxshellden = (float)log10((double)maxshellden + 1.0);
+    const int nsd = shell_divide(np);
+    for (int i = 0; i < nsd; i++) {
+        const float step = (float)(nsd - 1 - i) * (maxshellden - minshellden);
+        const float tl = minshellden + step / (float)nsd;
+        thr[i] = (float)(pow(10.0, (double)tl) - 1.0);
+    }
+    return nsd;
+}
+
 hipError_t shell_thresholds(const float *den, long long np, const float *coredensity,
                             int numcore, float *thr, int *nthr, Workspace &ws, hipStream_t s) {
     WsScope scope(ws);
@@ -1621,33 +2017,12 @@ hipError_t shell_thresholds(const float *den, long long np, const float *coreden
         memcpy(&f, &b, 4);
         return f;
     };
-    // subhaloden.mod6.c:2995-3024 with FP32 variables and glibc's log10/pow
-    float minshellden = 1.e27f, maxshellden = -1.e27f;
-    if (numcore > 0) {
-        const float cmax = val(h[3]);
-        maxshellden = maxshellden > cmax ? maxshellden : cmax;
-    }
-    if (np > 0) {
-        const float pmin = val(h[0]);
-        minshellden = minshellden < pmin ? minshellden : pmin;
-    }
-    minshellden = (float)(kDenFloor > (double)minshellden ? kDenFloor : (double)minshellden);
-    minshellden = (float)log10((double)minshellden + 1.0);
-    maxshellden = (float)log10((double)maxshellden + 1.0);
-    int nsd = 10;
-    if (np > 1000) {
-        const double a2 = 10.0 * log10((double)np * 1.5);
-        nsd = (int)(10.0 > a2 ? 10.0 : a2);
-        nsd = nsd < 15 ? nsd : 15;
-    }
-    for (int i = 0; i < nsd; i++) {
-        const float step = (float)(nsd - 1 - i) * (maxshellden - minshellden);
-        const float tl = minshellden + step / (float)nsd;
-        thr[i] = (float)(pow(10.0, (double)tl) - 1.0);
-    }
-    *nthr = nsd;
+    const float pmin = np > 0 ? val(h[0]) : 1.e27f;
+    const float cmax = numcore > 0 ? val(h[3]) : -1.e27f;
+    *nthr = thresholds_from(pmin, cmax, np, thr);
     return hipSuccess;
 }
+
 
 hipError_t core_shells(const float *den, long long np, const long long *nbr, int k,
                        const int *peak, const float *coredensity, int numcore, const float *thr,
@@ -1873,6 +2248,214 @@ hipError_t core_shells(const float *den, long long np, const long long *nbr, int
     PGALF_HIP_TRY(to_host(&hs, st, 1, s));
     *err_out = hs.err;
     *nshell_out = hs.nshell;
+    return hipGetLastError();
+}
+
+
+void catalog_shell_layout(int nhalo, const long long *off, const long long *coff,
+                          long long *sbase, long long *cbase, long long *ccap,
+                          long long *tot_shell, long long *tot_cores) {
+    long long sb = 0, cb = 0;
+    for (int h = 0; h < nhalo; h++) {
+        const long long n = off[h + 1] - off[h], nc = coff[h + 1] - coff[h];
+        const int nsd = shell_divide(n);
+        sbase[h] = sb;
+        cbase[h] = cb;
+        ccap[h] = 4 * nc * (nsd + 1) + 16;
+        sb += nc * nsd + 2;  // maxshell + 1 pointer entries
+        cb += ccap[h];
+    }
+    *tot_shell = sb;
+    *tot_cores = cb;
+}
+
+hipError_t catalog_watershed(int nhalo, const long long *off_h, const long long *coff_h,
+                             const float *den, const long long *nbr, int k, const int *type,
+                             const double *mass, const int *peak, const CoreOut &o,
+                             const CatShellOut &so, long long small_max, long long *nshell_h,
+                             int *herr_h, Workspace &ws, hipStream_t s) {
+    WsScope scope(ws);
+    std::vector<long long> sbase(nhalo), cbase(nhalo), ccap(nhalo);
+    long long tot_s = 0, tot_c = 0;
+    catalog_shell_layout(nhalo, off_h, coff_h, sbase.data(), cbase.data(), ccap.data(), &tot_s,
+                         &tot_c);
+    const long long np = off_h[nhalo], ncore = coff_h[nhalo];
+    // the halos for the workgroups (largest first), the rest one by one
+    std::vector<int> small, big;
+    long long nmax_small = 0;
+    for (int h = 0; h < nhalo; h++) {
+        const long long n = off_h[h + 1] - off_h[h];
+        if (n <= 0) continue;
+        if (n <= small_max) {
+            small.push_back(h);
+            nmax_small = n > nmax_small ? n : nmax_small;
+        } else {
+            big.push_back(h);
+        }
+    }
+    std::stable_sort(small.begin(), small.end(), [&](int a, int b) {
+        return off_h[a + 1] - off_h[a] > off_h[b + 1] - off_h[b];
+    });
+    long long *doff = ws.alloc_n<long long>(nhalo + 1), *dcoff = ws.alloc_n<long long>(nhalo + 1);
+    long long *dsbase = ws.alloc_n<long long>(nhalo > 0 ? nhalo : 1);
+    long long *dcbase = ws.alloc_n<long long>(nhalo > 0 ? nhalo : 1);
+    long long *dccap = ws.alloc_n<long long>(nhalo > 0 ? nhalo : 1);
+    long long *dnshell = ws.alloc_n<long long>(nhalo > 0 ? nhalo : 1);
+    int *order = ws.alloc_n<int>(small.size() > 0 ? small.size() : 1);
+    int *next = ws.alloc_n<int>(2), *herr = ws.alloc_n<int>(nhalo > 0 ? nhalo : 1);
+    unsigned char *pk = ws.alloc_n<unsigned char>(np > 0 ? np : 1);
+    int *core_of = ws.alloc_n<int>(np > 0 ? np : 1);
+    unsigned char *enclosed = ws.alloc_n<unsigned char>(ncore > 0 ? ncore : 1);
+    float *thr = ws.alloc_n<float>((size_t)(nhalo > 0 ? nhalo : 1) * kMaxShellDivide);
+    int *nthr = ws.alloc_n<int>(nhalo > 0 ? nhalo : 1);
+    unsigned *kmin = ws.alloc_n<unsigned>(nhalo > 0 ? nhalo : 1);
+    unsigned *kmax = ws.alloc_n<unsigned>(nhalo > 0 ? nhalo : 1);
+    if (!doff || !dcoff || !dsbase || !dcbase || !dccap || !dnshell || !order || !next || !herr ||
+        !pk || !core_of || !enclosed || !thr || !nthr || !kmin || !kmax)
+        return hipErrorOutOfMemory;
+    PGALF_HIP_TRY(hipMemcpyAsync(doff, off_h, sizeof(long long) * (nhalo + 1),
+                                 hipMemcpyHostToDevice, s));
+    PGALF_HIP_TRY(hipMemcpyAsync(dcoff, coff_h, sizeof(long long) * (nhalo + 1),
+                                 hipMemcpyHostToDevice, s));
+    if (nhalo > 0) {
+        PGALF_HIP_TRY(hipMemcpyAsync(dsbase, sbase.data(), sizeof(long long) * nhalo,
+                                     hipMemcpyHostToDevice, s));
+        PGALF_HIP_TRY(hipMemcpyAsync(dcbase, cbase.data(), sizeof(long long) * nhalo,
+                                     hipMemcpyHostToDevice, s));
+        PGALF_HIP_TRY(hipMemcpyAsync(dccap, ccap.data(), sizeof(long long) * nhalo,
+                                     hipMemcpyHostToDevice, s));
+        PGALF_HIP_TRY(fill_u32(reinterpret_cast<unsigned *>(herr), nhalo, 0u, s));
+        PGALF_HIP_TRY(fill_u64(reinterpret_cast<unsigned long long *>(dnshell), nhalo, 0ull, s));
+    }
+    if (!small.empty())
+        PGALF_HIP_TRY(hipMemcpyAsync(order, small.data(), sizeof(int) * small.size(),
+                                     hipMemcpyHostToDevice, s));
+    PGALF_HIP_TRY(fill_u32(reinterpret_cast<unsigned *>(next), 2, 0u, s));
+    CatArgs a{};
+    a.off = doff;
+    a.coff = dcoff;
+    a.order = order;
+    a.ntask = (int)small.size();
+    a.next = next;
+    a.den = den;
+    a.nbr = nbr;
+    a.k = k;
+    a.type = type;
+    a.mass = mass;
+    a.peak = peak;
+    a.coredensity = o.coredensity;
+    a.nummem = o.nummem;
+    a.numstar = o.numstar;
+    a.starmass = o.starmass;
+    a.haloid = o.haloid;
+    a.flag = o.flag;
+    a.pk = pk;
+    a.core_of = core_of;
+    a.herr = herr;
+    a.thr = thr;
+    a.nthr = nthr;
+    a.enclosed = enclosed;
+    a.part_ptr = so.part_ptr;
+    a.core_ptr = so.core_ptr;
+    a.sbase = dsbase;
+    a.part = so.part;
+    a.cores = so.cores;
+    a.cbase = dcbase;
+    a.ccap = dccap;
+    a.nshell = dnshell;
+    int slots = 0;
+    if (!small.empty()) {
+        a.nw = (nmax_small + 31) / 32;
+        a.qcap = nmax_small + kBlkT;
+        const long long per = 2 * a.nw * 4 + a.qcap * 4;
+        long long want = (1LL << 30) / per;
+        want = want < 1024 ? want : 1024;
+        want = want < (long long)small.size() ? want : (long long)small.size();
+        slots = (int)(want > 1 ? want : 1);
+        a.bits = ws.alloc_n<unsigned>((size_t)slots * 2 * a.nw);
+        a.q = ws.alloc_n<int>((size_t)slots * a.qcap);
+        if (!a.bits || !a.q) return hipErrorOutOfMemory;
+        PGALF_HIP_TRY(fill_u32(a.bits, (long long)slots * 2 * a.nw, 0u, s));
+        cat_fcd_k<<<slots, kBlkT, 0, s>>>(a);
+        PGALF_HIP_TRY(hipGetLastError());
+    }
+    std::vector<int> herr_v(nhalo > 0 ? nhalo : 1, 0);
+    if (nhalo > 0) PGALF_HIP_TRY(to_host(herr_v.data(), herr, nhalo, s));
+    // halos left to the one-halo tiers: FindCoreDensity now, shells below
+    std::vector<char> single(nhalo > 0 ? nhalo : 1, 0);
+    for (int h : big) single[h] = 1;
+    for (int h : small)
+        if (herr_v[h]) single[h] = 1;
+    auto one_fcd = [&](int h) -> hipError_t {
+        const long long p0 = off_h[h], c0 = coff_h[h];
+        CoreOut oh{o.coredensity + c0, o.nummem + c0, o.numstar + c0, o.starmass + c0,
+                   o.haloid + p0, o.flag + p0};
+        int fail = 0;
+        PGALF_HIP_TRY(find_core_density(den + p0, off_h[h + 1] - p0, nbr + p0 * k, k, type + p0,
+                                        mass + p0, peak + c0, (int)(coff_h[h + 1] - c0), oh,
+                                        &fail, ws, s));
+        if (fail) herr_v[h] = kCatSearch;
+        return hipSuccess;
+    };
+    for (int h = 0; h < nhalo; h++)
+        if (single[h]) {
+            herr_v[h] = 0;
+            PGALF_HIP_TRY(one_fcd(h));
+        }
+    // thresholds per halo, formed on the host as the reference forms them
+    if (nhalo > 0) {
+        cat_minmax_k<<<nhalo, 256, 0, s>>>(doff, dcoff, den, o.coredensity, kmin, kmax);
+        std::vector<unsigned> lo(nhalo), hi(nhalo);
+        PGALF_HIP_TRY(to_host(lo.data(), kmin, nhalo, s));
+        PGALF_HIP_TRY(to_host(hi.data(), kmax, nhalo, s));
+        std::vector<float> th((size_t)nhalo * kMaxShellDivide, 0.0f);
+        std::vector<int> nt(nhalo);
+        for (int h = 0; h < nhalo; h++) {
+            const long long n = off_h[h + 1] - off_h[h], nc = coff_h[h + 1] - coff_h[h];
+            const float pmin = n > 0 ? host_val(lo[h]) : 1.e27f;
+            const float cmax = nc > 0 ? host_val(hi[h]) : -1.e27f;
+            nt[h] = thresholds_from(pmin, cmax, n, th.data() + (size_t)h * kMaxShellDivide);
+        }
+        PGALF_HIP_TRY(hipMemcpyAsync(thr, th.data(), sizeof(float) * th.size(),
+                                     hipMemcpyHostToDevice, s));
+        PGALF_HIP_TRY(hipMemcpyAsync(nthr, nt.data(), sizeof(int) * nhalo,
+                                     hipMemcpyHostToDevice, s));
+        // shells of the workgroup halos (halos marked single are skipped)
+        if (!small.empty()) {
+            std::vector<int> mark(nhalo);
+            for (int h = 0; h < nhalo; h++) mark[h] = single[h] ? 1 : 0;
+            PGALF_HIP_TRY(hipMemcpyAsync(herr, mark.data(), sizeof(int) * nhalo,
+                                         hipMemcpyHostToDevice, s));
+            PGALF_HIP_TRY(fill_u32(reinterpret_cast<unsigned *>(next), 1, 0u, s));
+            cat_shells_k<<<slots, kBlkT, 0, s>>>(a);
+            PGALF_HIP_TRY(hipGetLastError());
+            std::vector<int> e2(nhalo);
+            PGALF_HIP_TRY(to_host(e2.data(), herr, nhalo, s));
+            for (int h : small)
+                if (!single[h] && e2[h]) {
+                    // a sweep or the core entries outgrew the workgroup
+                    // path: the halo again through the one-halo tiers
+                    single[h] = 1;
+                    PGALF_HIP_TRY(one_fcd(h));
+                }
+        }
+        PGALF_HIP_TRY(to_host(nshell_h, dnshell, nhalo, s));
+        for (int h = 0; h < nhalo; h++) {
+            if (!single[h]) continue;
+            const long long p0 = off_h[h], c0 = coff_h[h];
+            ShellOut oh{so.part_ptr + sbase[h], so.part + p0, so.core_ptr + sbase[h],
+                        so.cores + cbase[h], (coff_h[h + 1] - c0) * nt[h] + 1, ccap[h]};
+            long long ns = 0;
+            int err = 0;
+            PGALF_HIP_TRY(core_shells(den + p0, off_h[h + 1] - p0, nbr + p0 * k, k, peak + c0,
+                                      o.coredensity + c0, (int)(coff_h[h + 1] - c0),
+                                      th.data() + (size_t)h * kMaxShellDivide, nt[h],
+                                      o.flag + p0, o.haloid + p0, oh, &ns, &err, ws, s));
+            nshell_h[h] = ns;
+            if (err && !herr_v[h]) herr_v[h] = 10 + err;
+        }
+    }
+    for (int h = 0; h < nhalo; h++) herr_h[h] = herr_v[h];
     return hipGetLastError();
 }
 

@@ -781,3 +781,63 @@ def dev_shell_particles_peaks(den, nbr, peak, coredensity, thr, flag, haloid, ma
             continue
         _lib.check(rc)
         return sh.nshell, pp, part, cp, cores
+
+
+def dev_catalog_watershed(off, core_off, den, nbr, peak, types, mass, small_max=0):
+    """FindCoreDensity + thresholds + shells of every halo of a catalogue
+    (pgalf_dev_catalog_watershed).  off / core_off: int64 numpy arrays
+    (nhalo + 1); den float32, nbr int64 [n, k] with halo-local indices,
+    peak int32 (halo-local), types int32, mass float64: device tensors.
+    Returns a dict of device tensors (coredensity, nummem, numstar, starmass,
+    haloid, flag, part_ptr, part, core_ptr, cores) and host arrays
+    (shell_base, core_base, nshell, status)."""
+    torch = _torch()
+    off = np.ascontiguousarray(off, dtype=np.int64)
+    core_off = np.ascontiguousarray(core_off, dtype=np.int64)
+    nhalo = len(off) - 1
+    n, nc = int(off[-1]), int(core_off[-1])
+    k = nbr.shape[1]
+    _check_dev(den, torch.float32, n, "den")
+    _check_dev(nbr, torch.int64, n * k, "nbr")
+    _check_dev(peak, torch.int32, nc, "peak")
+    _check_dev(types, torch.int32, n, "types")
+    _check_dev(mass, torch.float64, n, "mass")
+    lib = _lib.load()
+    sbase = np.zeros(max(nhalo, 1), np.int64)
+    cbase = np.zeros(max(nhalo, 1), np.int64)
+    ccap = np.zeros(max(nhalo, 1), np.int64)
+    tot_c = ctypes.c_longlong(0)
+    tot_s = lib.pgalf_catalog_shell_layout(nhalo, _ptr(off), _ptr(core_off), _ptr(sbase),
+                                           _ptr(cbase), _ptr(ccap), ctypes.byref(tot_c))
+    if tot_s < 0:
+        _lib.check()
+    dev = den.device
+    e = lambda m, dt: torch.empty(max(m, 1), dtype=dt, device=dev)  # noqa: E731
+    o = dict(coredensity=e(nc, torch.float32), nummem=e(nc, torch.int32),
+             numstar=e(nc, torch.int32), starmass=e(nc, torch.float32),
+             haloid=e(n, torch.int32), flag=e(n, torch.uint8),
+             part_ptr=e(tot_s, torch.int64), part=e(n, torch.int32),
+             core_ptr=e(tot_s, torch.int64), cores=e(tot_c.value, torch.int32))
+    nshell = np.zeros(max(nhalo, 1), np.int64)
+    status = np.zeros(max(nhalo, 1), np.int32)
+    rc = lib.pgalf_dev_catalog_watershed(
+        nhalo, _ptr(off), _ptr(core_off), _dptr(den), _dptr(nbr), int(k), _dptr(types),
+        _dptr(mass), _dptr(peak), _dptr(o["coredensity"]), _dptr(o["nummem"]),
+        _dptr(o["numstar"]), _dptr(o["starmass"]), _dptr(o["haloid"]), _dptr(o["flag"]),
+        _dptr(o["part_ptr"]), _dptr(o["part"]), _dptr(o["core_ptr"]), _dptr(o["cores"]),
+        int(small_max), _ptr(nshell), _ptr(status), _stream())
+    o.update(shell_base=sbase[:nhalo], core_base=cbase[:nhalo], nshell=nshell[:nhalo],
+             status=status[:nhalo])
+    _lib.check(rc)
+    return o
+
+
+def catalog_shells_of(o, off, h):
+    """Halo h's shells from dev_catalog_watershed's result (host copies in
+    o, numpy): a list of (particles, cores) arrays, local indices."""
+    sb, cb, ns = int(o["shell_base"][h]), int(o["core_base"][h]), int(o["nshell"][h])
+    pp = o["part_ptr"][sb:sb + ns + 1]
+    cp = o["core_ptr"][sb:sb + ns + 1]
+    part = o["part"][off[h]:off[h + 1]]
+    cores = o["cores"][cb:]
+    return [(part[pp[i]:pp[i + 1]], cores[cp[i]:cp[i + 1]]) for i in range(ns)]

@@ -383,6 +383,44 @@ int pgalf_batch_status(pgalf_batch_t *b, void *stream);
  * held at once; reserved != 0, the bytes its arena holds now.  -1 for a
  * null batch. */
 long long pgalf_batch_workspace_bytes(const pgalf_batch_t *b, int reserved);
+
+/* The watershed of subhalo_den over a whole catalogue: FindCoreDensity, the
+ * shell thresholds and the shells (GetShellParticlesPeaks for each,
+ * SaveRemainingParticles2LastShell) of every halo, each exactly as
+ * pgalf_dev_find_core_density / pgalf_dev_shell_particles_peaks give them on
+ * the halo alone.  The reference runs these per halo in each MPI worker's
+ * subhalo_den (subhaloden.mod6.c:2978-3024); there is no one reference
+ * function for the catalogue.  Halo h: particles [off[h], off[h+1]) of the
+ * concatenated device arrays den, type, mass (FP64) and of the per-particle
+ * outputs haloid (a core index local to the halo, or PGALF_NOT_HALO_MEMBER),
+ * flag; its k-entry lists in nbr hold indices local to the halo (as
+ * pgalf_batch_run writes them); its cores [core_off[h], core_off[h+1]) of
+ * peak (local particle indices) and of the per-core outputs.  off and
+ * core_off are host arrays of nhalo + 1 entries.
+ * Shells of halo h: shell_base[h], core_base[h] and core_cap[h] from
+ * pgalf_catalog_shell_layout; its nshell[h] shells are part_ptr[shell_base[h]
+ * .. + nshell[h] + 1) (offsets into its particles part[off[h] ..), local
+ * indices) and core_ptr (the same range; offsets into cores[core_base[h]
+ * ..), local core indices).  Halos of at most small_max particles (0: 65536)
+ * run one per workgroup in one launch per phase; larger ones through the
+ * one-halo path.  nshell and halo_status are host arrays of nhalo entries;
+ * halo_status[h] != 0 marks a halo whose results are not valid (2: a
+ * core-density search did not converge; 13: its core entries outgrew
+ * core_cap[h]); the call then returns PGALF_ERR_CORE_OVERFLOW or
+ * PGALF_ERR_ARG.  Synchronises `stream`.
+ * pgalf_catalog_shell_layout fills the three host arrays (nhalo entries)
+ * and returns the number of part_ptr / core_ptr entries; *total_cores gets
+ * the size of cores. */
+long long pgalf_catalog_shell_layout(int nhalo, const long long *off, const long long *core_off,
+                                     long long *shell_base, long long *core_base,
+                                     long long *core_cap, long long *total_cores);
+int pgalf_dev_catalog_watershed(int nhalo, const long long *off, const long long *core_off,
+                                const float *den, const long long *nbr, int k, const int *type,
+                                const double *mass, const int *peak, float *coredensity,
+                                int *nummem, int *numstar, float *starmass, int *haloid,
+                                unsigned char *flag, long long *part_ptr, int *part,
+                                long long *core_ptr, int *cores, long long small_max,
+                                long long *nshell, int *halo_status, void *stream);
 void pgalf_batch_destroy(pgalf_batch_t *b);
 
 /* Launch schedule (no counterpart in the reference): the knobs an autotuner
