@@ -476,7 +476,7 @@ def catalog_bench(nhalo=2000, seed=3, cpu=True):
         shutil.rmtree(d, ignore_errors=True)
 
 
-def batch_bench(steps, warmup, plain_pass=True):
+def batch_bench(steps, warmup, plain_pass=True, watershed=True, cpu=True):
     """Batched forward over the 10000-halo catalogue (dN/dn ~ n^-2 on
     [100, 1048576], grids 32^3-256^3 from n, RG = 2 cells, k = 32): eager
     and HIP-graph runs of the batch, and one pass of the single-halo stage
@@ -560,7 +560,180 @@ def batch_bench(steps, warmup, plain_pass=True):
         out["speedup_vs_per_halo"] = round(plain_ms / min(eager_ms, graph_ms), 2)
         print("[bench] per-halo plain pass: %.1f ms (%.1fx)" % (plain_ms, out["speedup_vs_per_halo"]),
               file=sys.stderr, flush=True)
+    if watershed:
+        out["watershed"] = catalog_watershed_bench(off, rho_k, nbr, m, cpu=cpu)
     b.close()
+    return out
+
+
+def catalog_cores(off, rho_k, nbr):
+    """Cores for the catalogue watershed: each halo's finddenpeak cores above
+    its median W4 density (device finddenpeak on the densities divided by
+    the halo's median, threshold 1, over the lists in catalogue indices);
+    a halo left without one gets its densest particle.  Every 4th particle
+    a star.  Returns (core offsets, per-halo local core arrays, types)."""
+    import torch
+    from galaxyfinder_amd import density as D
+    nh, n = len(off) - 1, int(off[-1])
+    dev = rho_k.device
+    cnt = np.diff(off)
+    rk_h = rho_k.cpu().numpy()
+    med = np.array([np.median(rk_h[off[h]:off[h + 1]]) for h in range(nh)], np.float32)
+    hof = torch.from_numpy(np.repeat(np.arange(nh), cnt)).to(dev)
+    scaled = (rho_k / torch.from_numpy(med).to(dev)[hof]).contiguous()
+    base = torch.from_numpy(off[:-1]).to(dev)[hof]
+    nbr_g = nbr + base[:, None]
+    types = torch.from_numpy(np.where(np.arange(n) % 4 == 0, 1, 2).astype(np.int32)).to(dev)
+    pk, npk = D.dev_finddenpeak(scaled, nbr_g, types, 0, 1.0)
+    del nbr_g, scaled
+    pk = pk[:int(npk.item())].cpu().numpy().astype(np.int64)
+    hk = np.searchsorted(off, pk, side="right") - 1
+    ncore = np.bincount(hk, minlength=nh)
+    peaks = []
+    for h, loc in enumerate(np.split(pk, np.cumsum(ncore)[:-1])):
+        loc = loc - off[h]
+        if len(loc) == 0:
+            loc = np.array([int(np.argmax(rk_h[off[h]:off[h + 1]]))])
+        peaks.append(loc.astype(np.int32))
+    coff = np.zeros(nh + 1, np.int64)
+    coff[1:] = np.cumsum([len(p) for p in peaks])
+    return coff, peaks, types
+
+
+def catalog_watershed_bench(off, rho_k, nbr, m, cpu=True, stride=10, ref_max=20000):
+    """The watershed of subhalo_den (FindCoreDensity, the shell thresholds,
+    GetShellParticlesPeaks + SaveRemainingParticles2LastShell) of every halo
+    of the batched catalogue in one call (pgalf_dev_catalog_watershed), on
+    the batch's own W4 kNN densities and 32-NN lists.  Cores: each halo's
+    finddenpeak cores above its median density (device finddenpeak on the
+    densities divided by the halo's median, threshold 1); every 4th particle
+    a star; FP64 masses.
+      * one_call_ms: the catalogue call (wall time around the synchronised
+        call, median of 3);
+      * per_halo: the one-halo device entry points (dev_find_core_density,
+        dev_shell_thresholds, dev_shell_particles_peaks) called halo by halo
+        on every `stride`-th halo in size order, and that time times
+        `stride` as the estimate for the catalogue;
+      * reference: the halos of that sample with at most `ref_max`
+        particles, gathered into one sub-catalogue: the catalogue call on
+        it, the reference's own functions (oracle/_ref; FindCoreDensity on
+        all threads, the shells serial) halo by halo, and whether every
+        output is equal."""
+    import torch
+    from galaxyfinder_amd import density as D
+    off = np.asarray(off, np.int64)
+    nh, n = len(off) - 1, int(off[-1])
+    dev = rho_k.device
+    cnt = np.diff(off)
+    coff, peaks, types = catalog_cores(off, rho_k, nbr)
+    peak_t = torch.from_numpy(np.concatenate(peaks)).to(dev)
+    m64 = m.double()
+
+    def wall(fn, reps=3):
+        ts = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        return float(np.median(ts))
+
+    res = {}
+
+    def cat_call():
+        res["o"] = D.dev_catalog_watershed(off, coff, rho_k, nbr, peak_t, types, m64)
+    t_cat = wall(cat_call)
+    o = res["o"]
+    assert not o["status"].any(), "catalogue watershed: halo status %s" % (
+        np.unique(o["status"]),)
+    out = {"workload": "the batched catalogue's %d halos, %d particles; finddenpeak cores "
+                       "above each halo's median W4 density (%d cores); every 4th particle "
+                       "a star; FP64 masses" % (nh, n, int(coff[-1])),
+           "cores": int(coff[-1]), "shells": int(o["nshell"].sum()),
+           "dropped_cores": int((o["nummem"][:int(coff[-1])] == 0).sum().item()),
+           "one_call_ms": round(t_cat, 2)}
+    print("[bench] catalogue watershed: %d cores, %d shells, %.1f ms" % (
+        out["cores"], out["shells"], t_cat), file=sys.stderr, flush=True)
+    # the one-halo path on every stride-th halo in size order
+    sample = np.argsort(cnt, kind="stable")[::stride]
+
+    def one(h):
+        a, e = int(off[h]), int(off[h + 1])
+        ca, ce = int(coff[h]), int(coff[h + 1])
+        den, nb, p = rho_k[a:e], nbr[a:e], peak_t[ca:ce]
+        oh = D.dev_find_core_density(den, nb, p, types[a:e], m64[a:e])
+        thr = D.dev_shell_thresholds(den, oh["coredensity"])
+        D.dev_shell_particles_peaks(den, nb, p, oh["coredensity"], thr, oh["flag"],
+                                    oh["haloid"])
+    for h in sample[:20]:
+        one(h)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for h in sample:
+        one(h)
+    torch.cuda.synchronize()
+    t_one = (time.perf_counter() - t0) * 1e3
+    out["per_halo"] = {"sample_halos": len(sample),
+                       "sample_particles": int(cnt[sample].sum()),
+                       "sample_ms": round(t_one, 1),
+                       "estimate_ms": round(t_one * stride, 1),
+                       "one_call_speedup": round(t_one * stride / t_cat, 2)}
+    print("[bench] one-halo path on %d halos: %.1f ms (x%d = %.0f ms)" % (
+        len(sample), t_one, stride, t_one * stride), file=sys.stderr, flush=True)
+    if cpu:
+        from oracle import oracle as O
+        if O.ref_lib() is not None:
+            sub = [int(h) for h in sorted(sample) if cnt[h] <= ref_max]
+            idx = np.concatenate([np.arange(off[h], off[h + 1]) for h in sub])
+            soff = np.zeros(len(sub) + 1, np.int64)
+            soff[1:] = np.cumsum(cnt[sub])
+            scoff = np.zeros(len(sub) + 1, np.int64)
+            scoff[1:] = np.cumsum([len(peaks[h]) for h in sub])
+            it = torch.from_numpy(idx).to(dev)
+            s_den, s_nbr = rho_k[it].contiguous(), nbr[it].contiguous()
+            s_ty, s_m = types[it].contiguous(), m64[it].contiguous()
+            s_pk = torch.from_numpy(np.concatenate([peaks[h] for h in sub])).to(dev)
+
+            def sub_call():
+                res["s"] = D.dev_catalog_watershed(soff, scoff, s_den, s_nbr, s_pk, s_ty, s_m)
+            t_sub = wall(sub_call)
+            so = {k_: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k_, v in res["s"].items()}
+            den_h, nb_h = s_den.cpu().numpy(), s_nbr.cpu().numpy()
+            ty_h, m_h = s_ty.cpu().numpy(), s_m.cpu().numpy()
+            same = True
+            t_ref = 0.0
+            for j in range(len(sub)):
+                a, e = int(soff[j]), int(soff[j + 1])
+                ca, ce = int(scoff[j]), int(scoff[j + 1])
+                p = peaks[sub[j]]
+                # thresholds from an untimed first FindCoreDensity (the
+                # reference forms them inline in subhalo_den)
+                r, _ = O.ref_core_shells(den_h[a:e], nb_h[a:e], p, ty_h[a:e], m_h[a:e],
+                                         nthreads=O.num_threads())
+                thr = O.shell_thresholds(den_h[a:e], r["coredensity"])
+                t0 = time.perf_counter()
+                r, rsh = O.ref_core_shells(den_h[a:e], nb_h[a:e], p, ty_h[a:e], m_h[a:e],
+                                           thr=thr, nthreads=O.num_threads())
+                t_ref += time.perf_counter() - t0
+                for key in ("coredensity", "nummem", "numstar", "starmass"):
+                    same &= np.array_equal(np.ascontiguousarray(so[key][ca:ce]).view(np.int32),
+                                           np.ascontiguousarray(r[key]).view(np.int32))
+                same &= np.array_equal(so["flag"][a:e], r["flag"])
+                same &= np.array_equal(so["haloid"][a:e], r["haloid"])
+                gs = D.catalog_shells_of(so, soff, j)
+                same &= len(gs) == len(rsh) and all(
+                    np.array_equal(g0, r0) and np.array_equal(g1, r1)
+                    for (g0, g1), (r0, r1) in zip(gs, rsh))
+            out["reference"] = {
+                "sample": "%d halos of the size-ordered sample with <= %d particles, %d "
+                          "particles, %d cores" % (len(sub), ref_max, len(idx), int(scoff[-1])),
+                "gpu_one_call_ms": round(t_sub, 2),
+                "reference_s": round(t_ref, 3),
+                "reference_threads": O.num_threads(),
+                "equal": bool(same)}
+            print("[bench] reference watershed sample: %.2f s (GPU one call %.1f ms), equal=%s"
+                  % (t_ref, t_sub, same), file=sys.stderr, flush=True)
     return out
 
 
@@ -778,7 +951,8 @@ def main():
         print(json.dumps({"c5": c5_bench(args.steps, args.warmup)}), flush=True)
         return
     if args.batch_only:
-        print(json.dumps({"batch": batch_bench(args.steps, args.warmup, plain_pass=False)}),
+        print(json.dumps({"batch": batch_bench(args.steps, args.warmup, plain_pass=False,
+                                               cpu=not args.no_cpu_baseline)}),
               flush=True)
         return
     if args.errors:
@@ -892,7 +1066,7 @@ def main():
     if not args.no_catalog:
         out["catalog"] = catalog_bench(cpu=not args.no_cpu_baseline)
     if not args.no_batch:
-        out["batch"] = batch_bench(max(3, args.steps // 2), 2)
+        out["batch"] = batch_bench(max(3, args.steps // 2), 2, cpu=not args.no_cpu_baseline)
     if not args.no_grad:
         out["grad"] = grad_bench(max(3, args.steps // 2), 2)
     if not args.no_c5:

@@ -1422,7 +1422,7 @@ extern "C" int pgalf_dev_catalog_watershed(int nhalo, const long long *off,
     CatShellOut so{part_ptr, part, core_ptr, cores};
     if (e == hipSuccess)
         e = catalog_watershed(nhalo, off, core_off, den, nbr, k, type, mass, peak, o, so,
-                              small_max > 0 ? small_max : 65536, nshell, halo_status, ws, s);
+                              small_max > 0 ? small_max : 131072, nshell, halo_status, ws, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_err(e, where);
     const int st = consume_status(ws, s, where);

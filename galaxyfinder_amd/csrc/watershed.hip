@@ -40,11 +40,15 @@
 // shell.  A root whose sweep outgrew the LDS is swept device-wide and
 // applied by scans.
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "pgalf_internal.h"
@@ -1719,9 +1723,11 @@ __global__ __launch_bounds__(kBlkT) void cat_shells_k(CatArgs a) {
 }
 
 // Per-halo minimum density and largest core density (ordered keys).
-__global__ void cat_minmax_k(const long long *off, const long long *coff, const float *den,
-                             const float *cd, unsigned *kmin, unsigned *kmax) {
-    const int h = blockIdx.x;
+// Minimum density and largest core density of the halos list[0..gridDim.x)
+// (ordered keys).
+__global__ void cat_minmax_k(const int *list, const long long *off, const long long *coff,
+                             const float *den, const float *cd, unsigned *kmin, unsigned *kmax) {
+    const int h = list[blockIdx.x];
     unsigned lo = 0xffffffffu, hi = 0u;
     for (long long i = off[h] + threadIdx.x; i < off[h + 1]; i += blockDim.x) {
         const unsigned kk = ord_key(den[i]);
@@ -2252,6 +2258,36 @@ hipError_t core_shells(const float *den, long long np, const long long *nbr, int
 }
 
 
+// Helper streams of the catalogue watershed, per device (default
+// priority): one for the workgroup halos, and kHelpers more whose threads
+// take large halos through the one-halo tiers beside the caller's thread,
+// each with a workspace of its own.  An event orders them after the
+// caller's stream.
+constexpr int kHelpers = 3;
+struct SideSet {
+    hipStream_t wg = nullptr;
+    hipEvent_t ev = nullptr;
+    hipStream_t s[kHelpers] = {};
+    Workspace *ws[kHelpers] = {};
+};
+
+static hipError_t side_set(int dev, SideSet **out) {
+    static std::mutex mu;
+    static std::map<int, SideSet> sets;
+    std::lock_guard<std::mutex> lk(mu);
+    SideSet &x = sets[dev];
+    if (!x.wg) PGALF_HIP_TRY(hipStreamCreateWithFlags(&x.wg, hipStreamNonBlocking));
+    if (!x.ev) PGALF_HIP_TRY(hipEventCreateWithFlags(&x.ev, hipEventDisableTiming));
+    for (int i = 0; i < kHelpers; i++) {
+        if (!x.s[i]) PGALF_HIP_TRY(hipStreamCreateWithFlags(&x.s[i], hipStreamNonBlocking));
+        if (!x.ws[i]) x.ws[i] = new Workspace(dev);
+    }
+    *out = &x;
+    return hipSuccess;
+}
+
+__global__ void or_status_k(unsigned *status, unsigned bits) { *status |= bits; }
+
 void catalog_shell_layout(int nhalo, const long long *off, const long long *coff,
                           long long *sbase, long long *cbase, long long *ccap,
                           long long *tot_shell, long long *tot_cores) {
@@ -2376,86 +2412,148 @@ hipError_t catalog_watershed(int nhalo, const long long *off_h, const long long 
         a.q = ws.alloc_n<int>((size_t)slots * a.qcap);
         if (!a.bits || !a.q) return hipErrorOutOfMemory;
         PGALF_HIP_TRY(fill_u32(a.bits, (long long)slots * 2 * a.nw, 0u, s));
-        cat_fcd_k<<<slots, kBlkT, 0, s>>>(a);
-        PGALF_HIP_TRY(hipGetLastError());
     }
     std::vector<int> herr_v(nhalo > 0 ? nhalo : 1, 0);
-    if (nhalo > 0) PGALF_HIP_TRY(to_host(herr_v.data(), herr, nhalo, s));
-    // halos left to the one-halo tiers: FindCoreDensity now, shells below
-    std::vector<char> single(nhalo > 0 ? nhalo : 1, 0);
-    for (int h : big) single[h] = 1;
-    for (int h : small)
-        if (herr_v[h]) single[h] = 1;
-    auto one_fcd = [&](int h) -> hipError_t {
-        const long long p0 = off_h[h], c0 = coff_h[h];
+    std::vector<long long> nshell_v(nhalo > 0 ? nhalo : 1, 0);
+    // FindCoreDensity, thresholds and shells of halo h by the one-halo tiers
+    // on stream s with scratch from ws (each thread its own)
+    auto one_halo = [&](int h, Workspace &ws, hipStream_t s) -> hipError_t {
+        const long long p0 = off_h[h], c0 = coff_h[h], n = off_h[h + 1] - p0;
+        const int nc = (int)(coff_h[h + 1] - c0);
         CoreOut oh{o.coredensity + c0, o.nummem + c0, o.numstar + c0, o.starmass + c0,
                    o.haloid + p0, o.flag + p0};
         int fail = 0;
-        PGALF_HIP_TRY(find_core_density(den + p0, off_h[h + 1] - p0, nbr + p0 * k, k, type + p0,
-                                        mass + p0, peak + c0, (int)(coff_h[h + 1] - c0), oh,
-                                        &fail, ws, s));
-        if (fail) herr_v[h] = kCatSearch;
+        PGALF_HIP_TRY(find_core_density(den + p0, n, nbr + p0 * k, k, type + p0, mass + p0,
+                                        peak + c0, nc, oh, &fail, ws, s));
+        herr_v[h] = fail ? kCatSearch : 0;
+        float th[kMaxShellDivide];
+        int nt = 0;
+        PGALF_HIP_TRY(shell_thresholds(den + p0, n, o.coredensity + c0, nc, th, &nt, ws, s));
+        ShellOut sh{so.part_ptr + sbase[h], so.part + p0, so.core_ptr + sbase[h],
+                    so.cores + cbase[h], (long long)nc * nt + 1, ccap[h]};
+        long long ns = 0;
+        int err = 0;
+        PGALF_HIP_TRY(core_shells(den + p0, n, nbr + p0 * k, k, peak + c0, o.coredensity + c0, nc,
+                                  th, nt, o.flag + p0, o.haloid + p0, sh, &ns, &err, ws, s));
+        nshell_v[h] = ns;
+        if (err && !herr_v[h]) herr_v[h] = 10 + err;
         return hipSuccess;
     };
-    for (int h = 0; h < nhalo; h++)
-        if (single[h]) {
-            herr_v[h] = 0;
-            PGALF_HIP_TRY(one_fcd(h));
+    // The workgroup halos run on a side stream while this thread and
+    // kHelpers helper threads take the large halos (largest first) through
+    // the one-halo tiers: the workgroups take one halo each from a queue,
+    // and the one-halo tiers' kernels fill the compute units it leaves.  The
+    // workgroup thread allocates nothing (its buffers are above); every
+    // thread writes only its own halos' outputs and host entries.
+    std::vector<int> fcd_err(nhalo > 0 ? nhalo : 1, 0), shell_err(nhalo > 0 ? nhalo : 1, 0);
+    std::vector<long long> nshell_small(nhalo > 0 ? nhalo : 1, 0);
+    std::stable_sort(big.begin(), big.end(), [&](int a2, int b2) {
+        return off_h[a2 + 1] - off_h[a2] > off_h[b2 + 1] - off_h[b2];
+    });
+    std::atomic<int> next_big(0);
+    std::atomic<unsigned> helper_status(0);
+    hipError_t side_e = hipSuccess, helper_e[kHelpers];
+    for (int i = 0; i < kHelpers; i++) helper_e[i] = hipSuccess;
+    std::vector<std::thread> threads;
+    struct Joiner {
+        std::vector<std::thread> &t;
+        ~Joiner() {
+            for (auto &x : t)
+                if (x.joinable()) x.join();
         }
-    // thresholds per halo, formed on the host as the reference forms them
-    if (nhalo > 0) {
-        cat_minmax_k<<<nhalo, 256, 0, s>>>(doff, dcoff, den, o.coredensity, kmin, kmax);
-        std::vector<unsigned> lo(nhalo), hi(nhalo);
-        PGALF_HIP_TRY(to_host(lo.data(), kmin, nhalo, s));
-        PGALF_HIP_TRY(to_host(hi.data(), kmax, nhalo, s));
-        std::vector<float> th((size_t)nhalo * kMaxShellDivide, 0.0f);
-        std::vector<int> nt(nhalo);
-        for (int h = 0; h < nhalo; h++) {
-            const long long n = off_h[h + 1] - off_h[h], nc = coff_h[h + 1] - coff_h[h];
-            const float pmin = n > 0 ? host_val(lo[h]) : 1.e27f;
-            const float cmax = nc > 0 ? host_val(hi[h]) : -1.e27f;
-            nt[h] = thresholds_from(pmin, cmax, n, th.data() + (size_t)h * kMaxShellDivide);
-        }
-        PGALF_HIP_TRY(hipMemcpyAsync(thr, th.data(), sizeof(float) * th.size(),
-                                     hipMemcpyHostToDevice, s));
-        PGALF_HIP_TRY(hipMemcpyAsync(nthr, nt.data(), sizeof(int) * nhalo,
-                                     hipMemcpyHostToDevice, s));
-        // shells of the workgroup halos (halos marked single are skipped)
-        if (!small.empty()) {
-            std::vector<int> mark(nhalo);
-            for (int h = 0; h < nhalo; h++) mark[h] = single[h] ? 1 : 0;
-            PGALF_HIP_TRY(hipMemcpyAsync(herr, mark.data(), sizeof(int) * nhalo,
-                                         hipMemcpyHostToDevice, s));
-            PGALF_HIP_TRY(fill_u32(reinterpret_cast<unsigned *>(next), 1, 0u, s));
-            cat_shells_k<<<slots, kBlkT, 0, s>>>(a);
-            PGALF_HIP_TRY(hipGetLastError());
-            std::vector<int> e2(nhalo);
-            PGALF_HIP_TRY(to_host(e2.data(), herr, nhalo, s));
-            for (int h : small)
-                if (!single[h] && e2[h]) {
-                    // a sweep or the core entries outgrew the workgroup
-                    // path: the halo again through the one-halo tiers
-                    single[h] = 1;
-                    PGALF_HIP_TRY(one_fcd(h));
+    } joiner{threads};
+    int dev = 0;
+    PGALF_HIP_TRY(hipGetDevice(&dev));
+    SideSet *ss = nullptr;
+    const int nhelp = (int)(big.size() > 1 ? (big.size() - 1 < (size_t)kHelpers ? big.size() - 1
+                                                                                 : kHelpers)
+                                           : 0);
+    if (!small.empty() || nhelp > 0) {
+        PGALF_HIP_TRY(side_set(dev, &ss));
+        PGALF_HIP_TRY(hipEventRecord(ss->ev, s));
+    }
+    if (!small.empty()) {
+        const hipStream_t s2 = ss->wg;
+        PGALF_HIP_TRY(hipStreamWaitEvent(s2, ss->ev, 0));
+        threads.emplace_back([&, dev, s2]() {
+            side_e = [&]() -> hipError_t {
+                PGALF_HIP_TRY(hipSetDevice(dev));
+                cat_fcd_k<<<slots, kBlkT, 0, s2>>>(a);
+                PGALF_HIP_TRY(hipGetLastError());
+                PGALF_HIP_TRY(to_host(fcd_err.data(), herr, nhalo, s2));
+                // thresholds, formed on the host as the reference forms them
+                cat_minmax_k<<<(unsigned)small.size(), 256, 0, s2>>>(order, doff, dcoff, den,
+                                                                     o.coredensity, kmin, kmax);
+                std::vector<unsigned> lo(nhalo), hi(nhalo);
+                PGALF_HIP_TRY(to_host(lo.data(), kmin, nhalo, s2));
+                PGALF_HIP_TRY(to_host(hi.data(), kmax, nhalo, s2));
+                std::vector<float> th((size_t)nhalo * kMaxShellDivide, 0.0f);
+                std::vector<int> nt(nhalo, 0), mark(nhalo, 1);
+                for (int h : small) {
+                    const long long n = off_h[h + 1] - off_h[h], nc = coff_h[h + 1] - coff_h[h];
+                    const float pmin = host_val(lo[h]);
+                    const float cmax = nc > 0 ? host_val(hi[h]) : -1.e27f;
+                    nt[h] = thresholds_from(pmin, cmax, n, th.data() + (size_t)h * kMaxShellDivide);
+                    mark[h] = fcd_err[h] ? 1 : 0;  // halos marked 1 are skipped
                 }
-        }
-        PGALF_HIP_TRY(to_host(nshell_h, dnshell, nhalo, s));
-        for (int h = 0; h < nhalo; h++) {
-            if (!single[h]) continue;
-            const long long p0 = off_h[h], c0 = coff_h[h];
-            ShellOut oh{so.part_ptr + sbase[h], so.part + p0, so.core_ptr + sbase[h],
-                        so.cores + cbase[h], (coff_h[h + 1] - c0) * nt[h] + 1, ccap[h]};
-            long long ns = 0;
-            int err = 0;
-            PGALF_HIP_TRY(core_shells(den + p0, off_h[h + 1] - p0, nbr + p0 * k, k, peak + c0,
-                                      o.coredensity + c0, (int)(coff_h[h + 1] - c0),
-                                      th.data() + (size_t)h * kMaxShellDivide, nt[h],
-                                      o.flag + p0, o.haloid + p0, oh, &ns, &err, ws, s));
-            nshell_h[h] = ns;
-            if (err && !herr_v[h]) herr_v[h] = 10 + err;
+                PGALF_HIP_TRY(hipMemcpyAsync(thr, th.data(), sizeof(float) * th.size(),
+                                             hipMemcpyHostToDevice, s2));
+                PGALF_HIP_TRY(hipMemcpyAsync(nthr, nt.data(), sizeof(int) * nhalo,
+                                             hipMemcpyHostToDevice, s2));
+                PGALF_HIP_TRY(hipMemcpyAsync(herr, mark.data(), sizeof(int) * nhalo,
+                                             hipMemcpyHostToDevice, s2));
+                PGALF_HIP_TRY(fill_u32(reinterpret_cast<unsigned *>(next), 1, 0u, s2));
+                cat_shells_k<<<slots, kBlkT, 0, s2>>>(a);
+                PGALF_HIP_TRY(hipGetLastError());
+                PGALF_HIP_TRY(to_host(shell_err.data(), herr, nhalo, s2));
+                return to_host(nshell_small.data(), dnshell, nhalo, s2);
+            }();
+        });
+    }
+    for (int i = 0; i < nhelp; i++) {
+        const hipStream_t hs = ss->s[i];
+        Workspace *hw = ss->ws[i];
+        PGALF_HIP_TRY(hipStreamWaitEvent(hs, ss->ev, 0));
+        threads.emplace_back([&, i, dev, hs, hw]() {
+            helper_e[i] = [&]() -> hipError_t {
+                PGALF_HIP_TRY(hipSetDevice(dev));
+                WsCall call(*hw, hs);
+                PGALF_HIP_TRY(call.e);
+                for (int j; (j = next_big.fetch_add(1)) < (int)big.size();)
+                    PGALF_HIP_TRY(one_halo(big[j], *hw, hs));
+                // the helper's device error bits go to the caller's status
+                unsigned st = 0;
+                PGALF_HIP_TRY(to_host(&st, hw->status(), 1, hs));
+                if (st) {
+                    helper_status.fetch_or(st);
+                    PGALF_HIP_TRY(fill_u32(hw->status(), 1, 0u, hs));
+                }
+                return hipStreamSynchronize(hs);
+            }();
+        });
+    }
+    for (int j; (j = next_big.fetch_add(1)) < (int)big.size();)
+        PGALF_HIP_TRY(one_halo(big[j], ws, s));
+    for (auto &x : threads) x.join();
+    PGALF_HIP_TRY(side_e);
+    for (int i = 0; i < kHelpers; i++) PGALF_HIP_TRY(helper_e[i]);
+    if (helper_status.load()) {
+        or_status_k<<<1, 1, 0, s>>>(ws.status(), helper_status.load());
+        PGALF_HIP_TRY(hipGetLastError());
+    }
+    for (int h : small) {
+        if (fcd_err[h] || shell_err[h]) {
+            // a sweep or the core entries outgrew the workgroup path: the
+            // halo again through the one-halo tiers
+            PGALF_HIP_TRY(one_halo(h, ws, s));
+        } else {
+            nshell_v[h] = nshell_small[h];
         }
     }
-    for (int h = 0; h < nhalo; h++) herr_h[h] = herr_v[h];
+    for (int h = 0; h < nhalo; h++) {
+        herr_h[h] = herr_v[h];
+        nshell_h[h] = nshell_v[h];
+    }
     return hipGetLastError();
 }
 

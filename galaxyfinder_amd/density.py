@@ -788,6 +788,8 @@ def dev_catalog_watershed(off, core_off, den, nbr, peak, types, mass, small_max=
     (pgalf_dev_catalog_watershed).  off / core_off: int64 numpy arrays
     (nhalo + 1); den float32, nbr int64 [n, k] with halo-local indices,
     peak int32 (halo-local), types int32, mass float64: device tensors.
+    Halos of at most small_max particles (0: the library's 131072) run one
+    per workgroup, the rest through the one-halo path on four host threads.
     Returns a dict of device tensors (coredensity, nummem, numstar, starmass,
     haloid, flag, part_ptr, part, core_ptr, cores) and host arrays
     (shell_base, core_base, nshell, status)."""

@@ -401,9 +401,13 @@ long long pgalf_batch_workspace_bytes(const pgalf_batch_t *b, int reserved);
  * pgalf_catalog_shell_layout; its nshell[h] shells are part_ptr[shell_base[h]
  * .. + nshell[h] + 1) (offsets into its particles part[off[h] ..), local
  * indices) and core_ptr (the same range; offsets into cores[core_base[h]
- * ..), local core indices).  Halos of at most small_max particles (0: 65536)
- * run one per workgroup in one launch per phase; larger ones through the
- * one-halo path.  nshell and halo_status are host arrays of nhalo entries;
+ * ..), local core indices).  Halos of at most small_max particles (0:
+ * 131072) run one per workgroup in one launch per phase, on a stream of the
+ * library's own; larger ones (largest first) through the one-halo path, on
+ * `stream` and on three more streams of the library's, each driven by a
+ * host thread with its own scratch, all at the same time.  The results do
+ * not depend on small_max or on that split.  nshell and halo_status are
+ * host arrays of nhalo entries;
  * halo_status[h] != 0 marks a halo whose results are not valid (2: a
  * core-density search did not converge; 13: its core entries outgrew
  * core_cap[h]); the call then returns PGALF_ERR_CORE_OVERFLOW or

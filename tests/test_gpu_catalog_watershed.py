@@ -15,8 +15,11 @@ batched stage (halo-local 32-NN lists, W4 densities), finddenpeak cores above
 each halo's 30% density quantile (plus, in every 8th halo, three dense
 particles that are not local maxima, so some cores are dropped), FP64
 masses, one particle in three a star.  The
-workgroup path takes the halos up to small_max (3,000 here, so both paths
-run; and the default).  Integer and FP32 outputs: the bar is equality.
+workgroup path takes the halos up to small_max, the one-halo path the rest
+on four host threads at once: small_max 3,000 (both paths), the default
+(every halo on the workgroup path) and 1 (every halo on the one-halo path,
+160 halos over the four threads).  Integer and FP32 outputs: the bar is
+equality.
 """
 import os
 import sys
@@ -118,7 +121,7 @@ def _same_halo(got, c, h, core, fl, hid, shells):
         assert np.array_equal(p, r) and np.array_equal(q, s), h
 
 
-@pytest.mark.parametrize("small_max", [3000, 0])
+@pytest.mark.parametrize("small_max", [3000, 0, 1])
 def test_catalog_watershed_equals_the_one_halo_path(cat, small_max):
     got = _run_catalog(cat, small_max)
     nh = len(cat["off"]) - 1
