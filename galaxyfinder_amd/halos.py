@@ -232,3 +232,38 @@ def small_halo(n, seed=0, nsub=2):
     radii = rng.uniform(0.1, 0.25, nsub)
     subs = [(0.05, 0.02, r) for r in radii]
     return halo(rng, n, c=6.0, r200=0.35, subs=subs)
+
+
+def catalogue_cores(off, rho_k, nbr):
+    """Cores for the catalogue watershed: each halo's finddenpeak cores above
+    its median W4 density (device finddenpeak on the densities divided by
+    the halo's median, threshold 1, over the lists in catalogue indices);
+    a halo left without one gets its densest particle.  Every 4th particle
+    a star.  Returns (core offsets, per-halo local core arrays, types)."""
+    import torch
+    from galaxyfinder_amd import density as D
+    off = np.asarray(off, np.int64)
+    nh, n = len(off) - 1, int(off[-1])
+    dev = rho_k.device
+    cnt = np.diff(off)
+    rk_h = rho_k.cpu().numpy()
+    med = np.array([np.median(rk_h[off[h]:off[h + 1]]) for h in range(nh)], np.float32)
+    hof = torch.from_numpy(np.repeat(np.arange(nh), cnt)).to(dev)
+    scaled = (rho_k / torch.from_numpy(med).to(dev)[hof]).contiguous()
+    base = torch.from_numpy(off[:-1]).to(dev)[hof]
+    nbr_g = nbr + base[:, None]
+    types = torch.from_numpy(np.where(np.arange(n) % 4 == 0, 1, 2).astype(np.int32)).to(dev)
+    pk, npk = D.dev_finddenpeak(scaled, nbr_g, types, 0, 1.0)
+    del nbr_g, scaled
+    pk = pk[:int(npk.item())].cpu().numpy().astype(np.int64)
+    hk = np.searchsorted(off, pk, side="right") - 1
+    ncore = np.bincount(hk, minlength=nh)
+    peaks = []
+    for h, loc in enumerate(np.split(pk, np.cumsum(ncore)[:-1])):
+        loc = loc - off[h]
+        if len(loc) == 0:
+            loc = np.array([int(np.argmax(rk_h[off[h]:off[h + 1]]))])
+        peaks.append(loc.astype(np.int32))
+    coff = np.zeros(nh + 1, np.int64)
+    coff[1:] = np.cumsum([len(p) for p in peaks])
+    return coff, peaks, types

@@ -149,3 +149,42 @@ def test_catalog_watershed_against_the_oracle(cat):
                                         o["haloid"])
         say("halo %d: %d particles, %d cores, %d shells" % (h, e - a, ce - ca, len(shells)))
         _same_halo(got, cat, h, o, fl, hid, shells)
+
+
+def test_catalog_watershed_on_the_10000_halo_catalogue():
+    """The 10,000-halo catalogue (9.15M particles, 100 to 451,434 per halo)
+    through the batched stage; each halo's finddenpeak cores above its
+    median density (42k cores).  One call at the default split (halos over
+    131,072 particles through the one-halo path on four host threads, the
+    rest on the workgroup path); against the one-halo entry points on the
+    eight largest halos, the eight around the split and 24 seeded ones."""
+    import torch
+    pos, mass, off, grids, rg = halos.catalogue(config_id=3)
+    n, nh = int(off[-1]), len(off) - 1
+    b = D.Batch(off, grids, rg, 32)
+    x, y, z = (_dev(pos[:, a]) for a in range(3))
+    m = _dev(mass)
+    den = torch.empty(b.grid_floats, dtype=torch.float32, device="cuda")
+    nbr = torch.empty((n, 32), dtype=torch.int64, device="cuda")
+    rk = torch.empty(n, dtype=torch.float32, device="cuda")
+    b.run(x, y, z, m, den, None, nbr, rk)
+    b.status()
+    del den
+    b.close()
+    coff, peaks, types = halos.catalogue_cores(off, rk, nbr)
+    mass64 = m.double()
+    peak = np.concatenate(peaks)
+    c = dict(off=off, coff=coff, rk=rk, nbr=nbr, types=types.cpu().numpy(),
+             mass64=mass64.cpu().numpy(), peak=peak)
+    got = _run_catalog(c, 0)
+    assert not got["status"].any()
+    sizes = np.diff(off)
+    order = np.argsort(-sizes, kind="stable")
+    split = int(np.searchsorted(-sizes[order], -131072, side="left"))
+    pick = list(order[:8]) + list(order[max(0, split - 4):split + 4])
+    pick += list(np.random.default_rng(3).choice(nh, 24, replace=False))
+    say("10k catalogue: %d cores, %d shells; checking %d halos" % (
+        int(coff[-1]), int(got["nshell"].sum()), len(pick)))
+    for h in dict.fromkeys(int(v) for v in pick):
+        core, fl, hid, shells = _one_halo(c, h)
+        _same_halo(got, c, h, core, fl, hid, shells)
