@@ -29,7 +29,7 @@ Workspace::Workspace(int device, void *base, size_t bytes) : device_(device), fi
 Workspace::~Workspace() {
     if (fixed_) return;  // caller-owned memory
     for (auto &c : chunks_) hipFree(c.ptr);
-    if (status_) hipFree(status_);
+    if (status_ && own_status_) hipFree(status_);
     if (done_) hipEventDestroy(done_);
 }
 
@@ -670,6 +670,29 @@ extern "C" int pgalf_dev_knn(const float *x, const float *y, const float *z, con
     return e == hipSuccess ? ok() : hip_err(e, "pgalf_dev_knn");
 }
 
+// Side stream of the one-halo density stage, per device (default
+// priority), with a workspace of its own that reports into the device
+// workspace's status word, and the events that order the two streams.
+struct StageSide {
+    hipStream_t s = nullptr;
+    Workspace *ws = nullptr;
+    hipEvent_t start = nullptr, sorted = nullptr, done = nullptr;
+};
+
+static hipError_t stage_side(Workspace &main_ws, StageSide **out) {
+    static std::map<int, StageSide> sides;  // under g_api_mu
+    StageSide &x = sides[main_ws.device()];
+    if (!x.s) PGALF_HIP_TRY(hipStreamCreateWithFlags(&x.s, hipStreamNonBlocking));
+    for (hipEvent_t *ev : {&x.start, &x.sorted, &x.done})
+        if (!*ev) PGALF_HIP_TRY(hipEventCreateWithFlags(ev, hipEventDisableTiming));
+    if (!x.ws) {
+        x.ws = new Workspace(main_ws.device());
+        x.ws->borrow_status(main_ws.status());
+    }
+    *out = &x;
+    return hipSuccess;
+}
+
 extern "C" int pgalf_dev_density_stage(const float *x, const float *y, const float *z,
                                        const float *m, long long np, const pgalf_grid_t *g,
                                        float RG, int k, float *den, float *rho_tsc,
@@ -684,6 +707,50 @@ extern "C" int pgalf_dev_density_stage(const float *x, const float *y, const flo
     WsCall call(ws, s);
     hipError_t e = call.e;
     Grid gg = grid_from(g);
+    const bool with_knn = nbr || rho_knn;
+    StageSide *side = nullptr;
+    if (e == hipSuccess && with_knn && !capturing(s)) e = stage_side(ws, &side);
+    if (e == hipSuccess && side) {
+        // the grid chain on the side stream beside the kNN: the deposit and
+        // smoothing while the kNN sorts and builds its tree, the read-back
+        // (over the kNN's sorted coordinates) while it queries
+        timing_mark("stage", s, false);
+        e = hipEventRecord(side->start, s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(side->s, side->start, 0);
+        WsCall call2(*side->ws, side->s);
+        if (e == hipSuccess) e = call2.e;
+        timing_mark("tsc", side->s, false);
+        if (e == hipSuccess)
+            e = tsc_deposit<float>(x, y, z, m, nullptr, np, PGALF_TYPE_ALL, gg, den, *side->ws,
+                                   side->s);
+        timing_mark("tsc", side->s, true);
+        timing_mark("smooth", side->s, false);
+        if (e == hipSuccess) e = gaussian_smooth(den, gg, RG, *side->ws, side->s);
+        timing_mark("smooth", side->s, true);
+        SortedHook rb;
+        if (rho_tsc) {
+            const SortedHook on_side = sorted_readback(gg, den, rho_tsc, *side->ws, side->s);
+            rb = [side, on_side, s](const double *sx, const double *sy, const double *sz,
+                                    const unsigned *order, long long n) -> hipError_t {
+                PGALF_HIP_TRY(hipEventRecord(side->sorted, s));
+                PGALF_HIP_TRY(hipStreamWaitEvent(side->s, side->sorted, 0));
+                return on_side(sx, sy, sz, order, n);
+            };
+        }
+        timing_mark("knn", s, false);
+        if (e == hipSuccess)
+            e = knn<float>(x, y, z, m, np, nullptr, PGALF_TYPE_ALL, k, nbr, rho_knn,
+                           rho_knn ? 1 : 0, ws, s, &rb);
+        timing_mark("knn", s, true);
+        // the caller's stream waits for the grid chain (also on an error:
+        // the side stream's work must end before its scratch is reused)
+        const hipError_t e2 = hipEventRecord(side->done, side->s);
+        const hipError_t e3 = e2 == hipSuccess ? hipStreamWaitEvent(s, side->done, 0) : e2;
+        if (e == hipSuccess) e = e3;
+        if (e3 != hipSuccess) hipStreamSynchronize(side->s);
+        timing_mark("stage", s, true);
+        return e == hipSuccess ? ok() : hip_err(e, "pgalf_dev_density_stage");
+    }
     timing_mark("stage", s, false);
     timing_mark("tsc", s, false);
     if (e == hipSuccess) e = tsc_deposit<float>(x, y, z, m, nullptr, np, PGALF_TYPE_ALL, gg, den, ws, s);
@@ -693,7 +760,6 @@ extern "C" int pgalf_dev_density_stage(const float *x, const float *y, const flo
     timing_mark("smooth", s, true);
     // with a kNN to run, the read-back waits for its Hilbert order (and is
     // timed inside the "knn" interval)
-    const bool with_knn = nbr || rho_knn;
     if (e == hipSuccess && rho_tsc && !with_knn) {
         timing_mark("findden", s, false);
         e = tsc_readback<float>(x, y, z, np, gg, den, rho_tsc, ws, s);

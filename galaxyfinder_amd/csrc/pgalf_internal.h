@@ -121,6 +121,13 @@ class Workspace {
     Mark mark() const { return chunks_.empty() ? Mark{0, 0} : Mark{cur_, chunks_[cur_].used}; }
     void release(const Mark &m);
     unsigned *status() { return status_; }  // device word of DEV_ERR_* bits
+    // Kernels run with this workspace report into another workspace's status
+    // word (a side stream's scratch beside that workspace's own call).
+    void borrow_status(unsigned *st) {
+        if (own_status_ && status_) return;  // already has a word of its own
+        status_ = st;
+        own_status_ = false;
+    }
     int device() const { return device_; }
     // Largest live scratch of any call so far (bytes), and the arena's size.
     size_t high_water() const { return peak_ > high_water_ ? peak_ : high_water_; }
@@ -142,6 +149,7 @@ class Workspace {
     hipStream_t last_ = nullptr;
     bool has_last_ = false;
     bool fixed_ = false;
+    bool own_status_ = true;
     hipError_t add_chunk(size_t bytes);
 };
 
