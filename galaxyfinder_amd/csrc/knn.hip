@@ -1937,6 +1937,40 @@ inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 // Blocks of the exact pass over the waves the packed-key kernel flagged
 // (grid-stride over the list; the list is short for distinct positions).
 constexpr unsigned kFixBlocks = 1024;
+// One-halo queries: up to kSplitWaves flagged waves are re-run as one-query
+// waves (the exact lists do not depend on which queries share a wave, and a
+// wave with one valid lane runs that query's search alone, so the waves'
+// searches run side by side instead of in one wave's lockstep); more go
+// whole.
+constexpr unsigned kSplitWaves = 256;
+
+// From the flag list (count, wave ids): per query of a flagged wave a wave
+// whose query range ends after that query (lane 0 valid only; past the
+// halo's last query the last query again), its own halo record, and the
+// list 0..; split[0] = the split waves, split[1] = the whole waves left (all
+// of them when the list is longer than kSplitWaves).
+__global__ void split_fix_k(const unsigned *fix, KHalo h0, KHalo *sh, KWave *sw, unsigned *wl,
+                            unsigned *split) {
+    const unsigned n = fix[0];
+    const bool part = n <= kSplitWaves;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        split[0] = part ? n * kWave : 0u;
+        split[1] = part ? 0u : n;
+    }
+    if (!part) return;
+    const long long end = h0.q0 + h0.nq;
+    for (unsigned j = blockIdx.x * blockDim.x + threadIdx.x; j < n * kWave;
+         j += gridDim.x * blockDim.x) {
+        long long q = (long long)fix[1 + j / kWave] * kWave + (j % kWave);
+        q = q < end ? q : end - 1;
+        KHalo h = h0;
+        h.nq = q + 1 - h0.q0;
+        sh[j] = h;
+        sw[j].halo = (int)j;
+        sw[j].q0 = q;
+        wl[j] = j;
+    }
+}
 
 // The packed-key query unless the schedule asks for the exact query alone, a
 // halo needs more than kMaxKeyBits slot bits, or the debug counters (exact
@@ -1987,9 +2021,25 @@ hipError_t run_query(bool packed, unsigned nwaves, const double *qx, const doubl
         timing_mark("knn_packed", s, true);
     }
     const unsigned nfix = nwaves < kFixBlocks ? nwaves : kFixBlocks;
-    knn_query_k<P, true><<<nfix, kWave, 0, s>>>(qx, qy, qz, qid, cx, cy, cz, cid, boxes, halos,
-                                                h0, waves, k, nbr, rho, rho_mode, m, self_seed,
-                                                nullptr, fix + 1, fix, xcd_chunk);
+    if (!halos && !waves) {
+        const size_t ns = (size_t)kSplitWaves * kWave;
+        KHalo *sh = ws.alloc_n<KHalo>(ns);
+        KWave *sw = ws.alloc_n<KWave>(ns);
+        unsigned *wl = ws.alloc_n<unsigned>(ns), *split = ws.alloc_n<unsigned>(2);
+        if (!sh || !sw || !wl || !split) return hipErrorOutOfMemory;
+        split_fix_k<<<64, 256, 0, s>>>(fix, h0, sh, sw, wl, split);
+        knn_query_k<P, true><<<kFixBlocks, kWave, 0, s>>>(
+            qx, qy, qz, qid, cx, cy, cz, cid, boxes, sh, h0, sw, k, nbr, rho, rho_mode, m,
+            self_seed, nullptr, wl, split, xcd_chunk);
+        knn_query_k<P, true><<<nfix, kWave, 0, s>>>(qx, qy, qz, qid, cx, cy, cz, cid, boxes,
+                                                    halos, h0, waves, k, nbr, rho, rho_mode, m,
+                                                    self_seed, nullptr, fix + 1, split + 1,
+                                                    xcd_chunk);
+    } else {
+        knn_query_k<P, true><<<nfix, kWave, 0, s>>>(qx, qy, qz, qid, cx, cy, cz, cid, boxes,
+                                                    halos, h0, waves, k, nbr, rho, rho_mode, m,
+                                                    self_seed, nullptr, fix + 1, fix, xcd_chunk);
+    }
     if (prof) {
         unsigned long long d[kPrN];
         unsigned nf = 0;
