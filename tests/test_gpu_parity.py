@@ -178,6 +178,31 @@ def test_out_of_grid_is_reported():
     assert abs(den.sum() * 64.0 - 1.0) < 1e-6
 
 
+def test_out_of_grid_is_reported_by_the_device_stage():
+    """The device stage with a kNN runs its grid chain on a second stream
+    with scratch of its own; the deposit's status still reaches
+    pgalf_dev_status, and the kNN lists are unaffected."""
+    torch = _torch()
+    say("out-of-grid status, device stage with kNN")
+    rng = np.random.default_rng(4)
+    pos = np.concatenate([rng.uniform(0.3, 0.7, (200, 3)), [[0.5, 0.99, 0.5]]])
+    n, k = len(pos), 8
+    x, y, z = (torch.from_numpy(np.ascontiguousarray(pos[:, a], dtype=np.float32)).cuda()
+               for a in range(3))
+    m = torch.ones(n, dtype=torch.float32, device="cuda")
+    g = D.grid_struct(16, 16, 16, 0.0, 0.0, 0.0, 1.0 / 16)
+    den = torch.empty(D.padded_shape(16, 16, 16), dtype=torch.float32, device="cuda")
+    rho_t = torch.empty(n, dtype=torch.float32, device="cuda")
+    nbr = torch.empty((n, k), dtype=torch.int64, device="cuda")
+    D.dev_density_stage(x, y, z, m, g, 2.0 / 16, k, den, rho_t, nbr, None)
+    with pytest.raises(D.PgalfError) as ei:
+        D.dev_status()
+    assert ei.value.code == 3
+    D.dev_status()  # the status word was cleared
+    want, _ = O.knn(pos.astype(np.float32).astype(np.float64), k)
+    assert np.array_equal(nbr.cpu().numpy(), want)
+
+
 def _stage_on_device(pos, mass, spec, k=32):
     torch = _torch()
     dev = torch.device("cuda")
