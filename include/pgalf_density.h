@@ -262,7 +262,13 @@ int pgalf_dev_knn_noalloc(const float *x, const float *y, const float *z, const 
                           const int *type, long long np, int cand_ptype, int k,
                           long long *nbr, float *rho, int rho_mode, void *stream);
 long long pgalf_dev_knn_noalloc_max(void);
-/* deposit -> smooth -> read-back -> kNN lists + W4 density, one call. */
+/* deposit -> smooth -> read-back -> kNN lists + W4 density, one call.  With
+ * a kNN to run and `stream` not under graph capture, the deposit, smoothing
+ * and read-back run on a stream of the library's own (default priority)
+ * beside the kNN on `stream`, ordered by events: they start after the work
+ * queued on `stream` before the call, and `stream` waits for them before
+ * anything queued after it.  Under capture the stages run one after the
+ * other on `stream`.  The results are the same either way. */
 int pgalf_dev_density_stage(const float *x, const float *y, const float *z,
                             const float *m, long long np, const pgalf_grid_t *g,
                             float RG, int k, float *den, float *rho_tsc,
