@@ -41,6 +41,7 @@
 // applied by scans.
 #include <algorithm>
 #include <atomic>
+#include <system_error>
 #include <cfloat>
 #include <climits>
 #include <cmath>
@@ -2416,25 +2417,25 @@ hipError_t catalog_watershed(int nhalo, const long long *off_h, const long long 
     std::vector<int> herr_v(nhalo > 0 ? nhalo : 1, 0);
     std::vector<long long> nshell_v(nhalo > 0 ? nhalo : 1, 0);
     // FindCoreDensity, thresholds and shells of halo h by the one-halo tiers
-    // on stream s with scratch from ws (each thread its own)
-    auto one_halo = [&](int h, Workspace &ws, hipStream_t s) -> hipError_t {
+    // on stream hs with scratch from hw (each thread its own)
+    auto one_halo = [&](int h, Workspace &hw, hipStream_t hs) -> hipError_t {
         const long long p0 = off_h[h], c0 = coff_h[h], n = off_h[h + 1] - p0;
         const int nc = (int)(coff_h[h + 1] - c0);
         CoreOut oh{o.coredensity + c0, o.nummem + c0, o.numstar + c0, o.starmass + c0,
                    o.haloid + p0, o.flag + p0};
         int fail = 0;
         PGALF_HIP_TRY(find_core_density(den + p0, n, nbr + p0 * k, k, type + p0, mass + p0,
-                                        peak + c0, nc, oh, &fail, ws, s));
+                                        peak + c0, nc, oh, &fail, hw, hs));
         herr_v[h] = fail ? kCatSearch : 0;
         float th[kMaxShellDivide];
         int nt = 0;
-        PGALF_HIP_TRY(shell_thresholds(den + p0, n, o.coredensity + c0, nc, th, &nt, ws, s));
+        PGALF_HIP_TRY(shell_thresholds(den + p0, n, o.coredensity + c0, nc, th, &nt, hw, hs));
         ShellOut sh{so.part_ptr + sbase[h], so.part + p0, so.core_ptr + sbase[h],
                     so.cores + cbase[h], (long long)nc * nt + 1, ccap[h]};
         long long ns = 0;
         int err = 0;
         PGALF_HIP_TRY(core_shells(den + p0, n, nbr + p0 * k, k, peak + c0, o.coredensity + c0, nc,
-                                  th, nt, o.flag + p0, o.haloid + p0, sh, &ns, &err, ws, s));
+                                  th, nt, o.flag + p0, o.haloid + p0, sh, &ns, &err, hw, hs));
         nshell_v[h] = ns;
         if (err && !herr_v[h]) herr_v[h] = 10 + err;
         return hipSuccess;
@@ -2472,11 +2473,13 @@ hipError_t catalog_watershed(int nhalo, const long long *off_h, const long long 
         PGALF_HIP_TRY(side_set(dev, &ss));
         PGALF_HIP_TRY(hipEventRecord(ss->ev, s));
     }
-    if (!small.empty()) {
-        const hipStream_t s2 = ss->wg;
-        PGALF_HIP_TRY(hipStreamWaitEvent(s2, ss->ev, 0));
-        threads.emplace_back([&, dev, s2]() {
-            side_e = [&]() -> hipError_t {
+    // The threads copy their work closures, whose references are to objects
+    // declared before `joiner` (so they outlive the threads whichever way
+    // this function returns); a thread that cannot start leaves its work to
+    // this thread.
+    bool wg_inline = false;
+    const hipStream_t s2 = ss ? ss->wg : nullptr;
+    auto wg_work = [&, dev, s2]() -> hipError_t {
                 PGALF_HIP_TRY(hipSetDevice(dev));
                 cat_fcd_k<<<slots, kBlkT, 0, s2>>>(a);
                 PGALF_HIP_TRY(hipGetLastError());
@@ -2507,15 +2510,20 @@ hipError_t catalog_watershed(int nhalo, const long long *off_h, const long long 
                 PGALF_HIP_TRY(hipGetLastError());
                 PGALF_HIP_TRY(to_host(shell_err.data(), herr, nhalo, s2));
                 return to_host(nshell_small.data(), dnshell, nhalo, s2);
-            }();
-        });
+    };
+    if (!small.empty()) {
+        PGALF_HIP_TRY(hipStreamWaitEvent(s2, ss->ev, 0));
+        try {
+            threads.emplace_back([&side_e, wg_work]() { side_e = wg_work(); });
+        } catch (const std::system_error &) {
+            wg_inline = true;
+        }
     }
     for (int i = 0; i < nhelp; i++) {
         const hipStream_t hs = ss->s[i];
         Workspace *hw = ss->ws[i];
         PGALF_HIP_TRY(hipStreamWaitEvent(hs, ss->ev, 0));
-        threads.emplace_back([&, i, dev, hs, hw]() {
-            helper_e[i] = [&]() -> hipError_t {
+        auto helper_work = [&, dev, hs, hw]() -> hipError_t {
                 PGALF_HIP_TRY(hipSetDevice(dev));
                 WsCall call(*hw, hs);
                 PGALF_HIP_TRY(call.e);
@@ -2529,11 +2537,16 @@ hipError_t catalog_watershed(int nhalo, const long long *off_h, const long long 
                     PGALF_HIP_TRY(fill_u32(hw->status(), 1, 0u, hs));
                 }
                 return hipStreamSynchronize(hs);
-            }();
-        });
+        };
+        try {
+            threads.emplace_back([&helper_e, i, helper_work]() { helper_e[i] = helper_work(); });
+        } catch (const std::system_error &) {
+            break;  // the large halos left go to this thread
+        }
     }
     for (int j; (j = next_big.fetch_add(1)) < (int)big.size();)
         PGALF_HIP_TRY(one_halo(big[j], ws, s));
+    if (wg_inline) side_e = wg_work();
     for (auto &x : threads) x.join();
     PGALF_HIP_TRY(side_e);
     for (int i = 0; i < kHelpers; i++) PGALF_HIP_TRY(helper_e[i]);
