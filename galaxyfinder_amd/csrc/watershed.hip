@@ -912,7 +912,19 @@ struct MultiSweep {
     int *keep;                 // 1 when this level's candidates fit in cv
     int *done;
     const int *roots;          // [M] root particles
+    unsigned *above;           // [nwords]: bit v set when den[v] > the sweep's threshold
+    float above_t;             // the threshold `above` holds (host copy), NaN: none
 };
+
+__global__ void ml_above_k(const float *den, long long np, float t, unsigned *above) {
+    const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w * 32 >= np) return;
+    unsigned bits = 0u;
+    const long long v0 = w * 32;
+    const int nv = np - v0 < 32 ? (int)(np - v0) : 32;
+    for (int b = 0; b < nv; b++) bits |= (den[v0 + b] > t ? 1u : 0u) << b;
+    above[w] = bits;
+}
 
 __global__ void ml_init_k(MultiSweep a) {
     const int r = threadIdx.x;
@@ -945,7 +957,7 @@ __device__ __forceinline__ int ml_cand(const MultiSweep &a, int r, long long j, 
     if (s >= k) return -1;
     const int u = a.bq[(long long)r * a.np + a.lo[r] + (j >> 5)];
     const int v = (int)nbr[(long long)u * k + s];
-    if (!(den[v] > t)) return -1;
+    if (!((a.above[v >> 5] >> (v & 31)) & 1u)) return -1;  // den[v] > t
     if ((a.vb[(long long)r * a.nwords + (v >> 5)] >> (v & 31)) & 1u) return -1;
     return v;
 }
@@ -1088,8 +1100,10 @@ hipError_t multi_bufs(MultiSweep &a, long long np, Workspace &ws, hipStream_t s)
     a.done = ws.alloc_n<int>(1);
     int *roots = ws.alloc_n<int>(kMaxRoots);
     a.roots = roots;
+    a.above = ws.alloc_n<unsigned>((size_t)a.nwords);
+    a.above_t = __builtin_nanf("");
     if (!a.bq || !a.vb || !a.claim || !a.lo || !a.hi || !a.off || !a.cnt || !a.base || !a.bsum ||
-        !a.cv || !a.keep || !a.done || !roots)
+        !a.cv || !a.keep || !a.done || !roots || !a.above)
         return hipErrorOutOfMemory;
     PGALF_HIP_TRY(fill_u32(a.vb, kMaxRoots * a.nwords, 0u, s));
     return fill_u32(reinterpret_cast<unsigned *>(a.claim), kMaxRoots * np, 0x7fffffffu, s);
@@ -1103,6 +1117,11 @@ hipError_t multi_sweep(MultiSweep &a, const int *roots_h, int M, float t, const 
     a.M = M;
     PGALF_HIP_TRY(hipMemcpyAsync(const_cast<int *>(a.roots), roots_h, sizeof(int) * M,
                                  hipMemcpyHostToDevice, s));
+    if (!(a.above_t == t)) {  // the threshold's bitmap (a 512 KB bit set at 4M
+                              // particles, where den is 16 MB), once per threshold
+        ml_above_k<<<nblk(a.nwords, 256), 256, 0, s>>>(den, a.np, t, a.above);
+        a.above_t = t;
+    }
     ml_init_k<<<1, 64, 0, s>>>(a);
     for (;;) {
         for (int l = 0; l < kMlLevelsPerSync; l++) {
@@ -2298,7 +2317,7 @@ void catalog_shell_layout(int nhalo, const long long *off, const long long *coff
         const int nsd = shell_divide(n);
         sbase[h] = sb;
         cbase[h] = cb;
-        ccap[h] = 4 * nc * (nsd + 1) + 16;
+        ccap[h] = 16 * nc * (nsd + 1) + 16;  // a core listed by every sweep that reaches it
         sb += nc * nsd + 2;  // maxshell + 1 pointer entries
         cb += ccap[h];
     }

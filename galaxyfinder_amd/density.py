@@ -758,7 +758,10 @@ def dev_shell_particles_peaks(den, nbr, peak, coredensity, thr, flag, haloid, ma
     thr = np.ascontiguousarray(thr, dtype=np.float32)
     dev = den.device
     maxshell = nc * max(len(thr), 1) + 1
-    maxcores = 4 * nc * (len(thr) + 1) + 16 if maxcores is None else maxcores
+    # room for a core listed by every sweep that reaches it (the 4M-particle
+    # halo's 38,248 cores need 5.8M entries over 15 thresholds); more on
+    # demand below
+    maxcores = 16 * nc * (len(thr) + 1) + 16 if maxcores is None else maxcores
     pp = torch.zeros(maxshell + 1, dtype=torch.int64, device=dev)
     cp = torch.zeros(maxshell + 1, dtype=torch.int64, device=dev)
     part = torch.zeros(max(n, 1), dtype=torch.int32, device=dev)
@@ -774,7 +777,7 @@ def dev_shell_particles_peaks(den, nbr, peak, coredensity, thr, flag, haloid, ma
                                                  _stream())
         # a core listed by several shells of one threshold can outgrow the
         # core entries: start again from the input state with more room
-        if rc == 6 and maxcores < 64 * (nc + 1) * (len(thr) + 1):
+        if rc == 6 and maxcores < 1024 * (nc + 1) * (len(thr) + 1):
             flag.copy_(flag0)
             haloid.copy_(haloid0)
             maxcores *= 4

@@ -420,7 +420,9 @@ long long pgalf_batch_workspace_bytes(const pgalf_batch_t *b, int reserved);
  * PGALF_ERR_ARG.  Synchronises `stream`.
  * pgalf_catalog_shell_layout fills the three host arrays (nhalo entries)
  * and returns the number of part_ptr / core_ptr entries; *total_cores gets
- * the size of cores. */
+ * the size of cores.  Halo h has cores x nshelldivide + 2 pointer entries
+ * and core_cap[h] = 16 x cores x (nshelldivide + 1) + 16 core entries (a
+ * core is listed by every shell sweep that reaches it). */
 long long pgalf_catalog_shell_layout(int nhalo, const long long *off, const long long *core_off,
                                      long long *shell_base, long long *core_base,
                                      long long *core_cap, long long *total_cores);

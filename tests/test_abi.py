@@ -157,7 +157,7 @@ def test_merging_peak_host_checks_without_gpu_calls():
 def test_catalog_shell_layout():
     """pgalf_catalog_shell_layout: halo h's pointer entries (cores x
     nshelldivide + 2, nshelldivide 10 up to 1000 particles, else
-    min(15, 10 log10(1.5 n))) and core-entry room (4 x cores x (nsd + 1) +
+    min(15, 10 log10(1.5 n))) and core-entry room (16 x cores x (nsd + 1) +
     16), laid out halo after halo."""
     import ctypes
     lib = _lib.load()
@@ -170,7 +170,7 @@ def test_catalog_shell_layout():
                                         ctypes.byref(tot))
     nsd = [10, int(10 * np.log10(1000 * 1.5)), 15]
     want_s = [3 * nsd[0] + 2, 0 * nsd[1] + 2, 7 * nsd[2] + 2]
-    want_c = [4 * c * (d + 1) + 16 for c, d in zip((3, 0, 7), nsd)]
+    want_c = [16 * c * (d + 1) + 16 for c, d in zip((3, 0, 7), nsd)]
     assert list(sb) == [0, want_s[0], want_s[0] + want_s[1]] and ns == sum(want_s)
     assert list(cc) == want_c and list(cb) == [0, want_c[0], want_c[0] + want_c[1]]
     assert tot.value == sum(want_c)
