@@ -268,7 +268,7 @@ def GetShellParticlesPeaks(np_, neighbor, NumNeighbor, core, numcore, den, haloi
     _need(flag, np.uint8, np_, "flag")
     thr = np.ascontiguousarray(thr, dtype=np.float32)
     maxshell = numcore * max(len(thr), 1) + 1
-    maxcores = numcore * (len(thr) + 1) + 1 if maxcores is None else maxcores
+    maxcores = 16 * numcore * (len(thr) + 1) + 16 if maxcores is None else maxcores
     while True:
         pp = np.zeros(maxshell + 1, np.int64)
         cp = np.zeros(maxshell + 1, np.int64)
@@ -279,7 +279,7 @@ def GetShellParticlesPeaks(np_, neighbor, NumNeighbor, core, numcore, den, haloi
         rc = lib.pgalf_shell_particles_peaks(int(np_), _ptr(neighbor), int(NumNeighbor),
                                              _ptr(core), int(numcore), _ptr(den), _ptr(hid),
                                              _ptr(fl), _ptr(thr), len(thr), ctypes.byref(sh))
-        if rc == 6 and maxcores < 64 * (numcore + 1) * (len(thr) + 1):  # core entries overflow
+        if rc == 6 and maxcores < 1024 * (numcore + 1) * (len(thr) + 1):  # core entries overflow
             maxcores *= 4
             continue
         _lib.check(rc)
