@@ -1345,6 +1345,9 @@ knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned
 // query then recomputes those waves.  For distinct float positions such
 // lanes are about one in 10^6; duplicated points and lattices flag many.
 constexpr unsigned long long kKeyOff = 1ull << 52;
+// A wave with at most kRadLanes flagged lanes writes its other lanes and
+// lists the flagged ones for knn_radius_k; a wave with more is listed whole.
+constexpr int kRadLanes = 4;
 __device__ inline double key_inf() { return __longlong_as_double(0x7fe0000000000000ll); }
 
 __device__ inline double pkey(double d2, unsigned e, unsigned long long keep) {
@@ -1614,7 +1617,8 @@ __device__ __forceinline__ void packed_wave(
     const double *cxa, const double *cya, const double *cza, const unsigned *cida,
     const Box *boxes, const KHalo *halos, const KHalo &halo0, const KWave *waves, int k,
     long long *nbr, float *rho, int rho_mode, const P *mass, int self_seed,
-    unsigned *fix_count, unsigned *fix_list, unsigned long long *prof,
+    unsigned *fix_count, unsigned *fix_list, unsigned *rad, double *rad_r2,
+    unsigned long long *prof,
     unsigned (*s_bc)[kWave], int (*s_list)[kFCap], int (*s_stk)[kStackCap], long long *s_off,
     long long *s_n, double *s_qx, double *s_qy, double *s_qz, double *s_qk, Stage &sg) {
     const unsigned long long t_start = pr_clock<kProf>();
@@ -1861,10 +1865,21 @@ __device__ __forceinline__ void packed_wave(
         flag |= (t + 1 < k) & same_class(st.td[t], st.td[t + 1], keep);
     flag &= valid;
     pr_add<kProf>(prof, kPrWaves, 1ull, lane);
-    if (__ballot(flag) != 0ull) {
-        if (lane == 0) fix_list[atomicAdd(fix_count, 1u)] = wid;
+    const unsigned long long fm = __ballot(flag);
+    if (fm != 0ull) {
         pr_add<kProf>(prof, kPrFlagged, 1ull, lane);
-        return;
+        if (__popcll(fm) > kRadLanes) {
+            // many lanes (duplicated points, lattices): the whole wave again
+            if (lane == 0) fix_list[atomicAdd(fix_count, 1u)] = wid;
+            return;
+        }
+        if (flag) {
+            // this lane alone, by the radius of its k-th key's class
+            const unsigned i = atomicAdd(rad, 1u);
+            rad[1 + i] = wid * kWave + lane;
+            rad_r2[i] = class_ub(st.kth, keep);
+            return;
+        }
     }
     if constexpr (kProf) {
         // the tail and total of unflagged waves, measured at their end
@@ -1906,7 +1921,8 @@ __device__ __forceinline__ void packed_wave(
 
 #define PGALF_PACKED_ARGS                                                                    \
     qx, qy, qz, qid, cxa, cya, cza, cida, boxes, halos, halo0, waves, k, nbr, rho, rho_mode,  \
-        mass, self_seed, fix_count, fix_list, prof, s_bc, s_list, s_stk, s_off, s_n, s_qx,   \
+        mass, self_seed, fix_count, fix_list, rad, rad_r2, prof, s_bc, s_list, s_stk, s_off,  \
+        s_n, s_qx,                                                                           \
         s_qy, s_qz, s_qk, sg
 #define PGALF_PACKED_SHARED                                                                  \
     __shared__ unsigned s_bc[kBufCap][kWave];                                                \
@@ -1923,8 +1939,8 @@ knn_packed_k(const double *qx, const double *qy, const double *qz, const unsigne
              const double *cxa, const double *cya, const double *cza, const unsigned *cida,
              const Box *boxes, const KHalo *halos, KHalo halo0, const KWave *waves, int k,
              long long *nbr, float *rho, int rho_mode, const P *mass, int self_seed,
-             unsigned *fix_count, unsigned *fix_list, unsigned long long *prof,
-             unsigned xcd_chunk) {
+             unsigned *fix_count, unsigned *fix_list, unsigned *rad, double *rad_r2,
+             unsigned long long *prof, unsigned xcd_chunk) {
     PGALF_PACKED_SHARED
     packed_wave<P, kProf>(xcd_block(blockIdx.x, gridDim.x, xcd_chunk), PGALF_PACKED_ARGS);
 }
@@ -1937,38 +1953,153 @@ inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 // Blocks of the exact pass over the waves the packed-key kernel flagged
 // (grid-stride over the list; the list is short for distinct positions).
 constexpr unsigned kFixBlocks = 1024;
-// One-halo queries: up to kSplitWaves flagged waves are re-run as one-query
-// waves (the exact lists do not depend on which queries share a wave, and a
-// wave with one valid lane runs that query's search alone, so the waves'
-// searches run side by side instead of in one wave's lockstep); more go
-// whole.
-constexpr unsigned kSplitWaves = 256;
+// ---- exact re-run of single flagged queries ------------------------------
+// A lane of the packed-key query fails its check only through the class of
+// its k-th key (a displaced candidate in that class, or two of its first k
+// sharing a class).  Its list's k keys all have d2 at most R2 = class_ub(k-th
+// key), so the exact top k -- the k smallest (d2, id) -- lie within R2.  One
+// wave per such query walks the box tree level by level from the root,
+// keeping the boxes whose gap (the walk's own monotone FP64 gap) does not
+// exceed R2, gathers the candidates of the buckets it reaches with d2 <= R2,
+// sorts them by (d2, id) in LDS and writes the list and the W4 density with
+// the exact query's arithmetic.  A query whose frontier or ball outgrows the
+// LDS arrays (or, which cannot happen, holds fewer than k) goes to the
+// whole-wave exact re-run.
+constexpr int kRadFront = 1024;  // frontier nodes of one level
+constexpr int kRadCand = 512;    // candidates within R2
 
-// From the flag list (count, wave ids): per query of a flagged wave a wave
-// whose query range ends after that query (lane 0 valid only; past the
-// halo's last query the last query again), its own halo record, and the
-// list 0..; split[0] = the split waves, split[1] = the whole waves left (all
-// of them when the list is longer than kSplitWaves).
-__global__ void split_fix_k(const unsigned *fix, KHalo h0, KHalo *sh, KWave *sw, unsigned *wl,
-                            unsigned *split) {
-    const unsigned n = fix[0];
-    const bool part = n <= kSplitWaves;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        split[0] = part ? n * kWave : 0u;
-        split[1] = part ? 0u : n;
-    }
-    if (!part) return;
-    const long long end = h0.q0 + h0.nq;
-    for (unsigned j = blockIdx.x * blockDim.x + threadIdx.x; j < n * kWave;
-         j += gridDim.x * blockDim.x) {
-        long long q = (long long)fix[1 + j / kWave] * kWave + (j % kWave);
-        q = q < end ? q : end - 1;
-        KHalo h = h0;
-        h.nq = q + 1 - h0.q0;
-        sh[j] = h;
-        sw[j].halo = (int)j;
-        sw[j].q0 = q;
-        wl[j] = j;
+template <class P>
+__global__ void __launch_bounds__(kWave)
+knn_radius_k(const double *qx, const double *qy, const double *qz, const unsigned *qid,
+             const double *cxa, const double *cya, const double *cza, const unsigned *cida,
+             const Box *boxes, const KHalo *halos, KHalo halo0, const KWave *waves, int k,
+             long long *nbr, float *rho, int rho_mode, const P *mass, const unsigned *rad,
+             const double *rad_r2, unsigned *whole_count, unsigned *whole_list) {
+    __shared__ int s_front[2][kRadFront];
+    __shared__ double s_d[kRadCand];
+    __shared__ unsigned s_id[kRadCand];
+    const int lane = threadIdx.x;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const unsigned n = rad[0];
+    for (unsigned it = blockIdx.x; it < n; it += gridDim.x) {
+        const unsigned ent = rad[1 + it];
+        const unsigned w = ent / kWave;
+        KWave wv;
+        if (waves) {
+            wv = waves[w];
+        } else {
+            wv.halo = 0;
+            wv.q0 = (long long)w * kWave;
+        }
+        const KHalo &hd = halos ? halos[wv.halo] : halo0;
+        const long long qi = wv.q0 + (long long)(ent % kWave);
+        const double p[3] = {qx[qi], qy[qi], qz[qi]};
+        const double R2 = rad_r2[it];
+        const double *cx = cxa + hd.c0, *cy = cya + hd.c0, *cz = cza + hd.c0;
+        const unsigned *cid = cida + hd.c0;
+        // the boxes of each level within R2, from the root (node 0 of the
+        // top level) down to the leaf buckets
+        int cur = 0, nf = 1;
+        bool over = false;
+        if (lane == 0) s_front[0][0] = 0;
+        __syncthreads();
+        for (int lev = hd.nlev - 1; lev >= 1 && !over; lev--) {
+            const long long nchild = level_count(hd.nb, lev - 1);
+            const Box *bl = boxes + hd.loff[lev - 1];
+            int cnt = 0;
+            for (int c0 = 0; c0 < nf * kFan; c0 += kWave) {
+                const int c = c0 + lane;
+                bool pass = false;
+                long long child = 0;
+                if (c < nf * kFan) {
+                    child = (long long)s_front[cur][c / kFan] * kFan + c % kFan;
+                    if (child < nchild) pass = !(box_gap2(bl[child], p, p) > R2);
+                }
+                const unsigned long long m = __ballot(pass);
+                const int pos = cnt + __popcll(m & below);
+                if (pass && pos < kRadFront) s_front[cur ^ 1][pos] = (int)child;
+                cnt += __popcll(m);
+            }
+            over = cnt > kRadFront;
+            cur ^= 1;
+            nf = cnt;
+            __syncthreads();
+        }
+        // the candidates of those buckets within R2
+        int nk = 0;
+        if (!over) {
+            for (int c0 = 0; c0 < nf * kB; c0 += kWave) {
+                const int c = c0 + lane;
+                bool in = false;
+                double d = 0.0;
+                unsigned id = 0u;
+                if (c < nf * kB) {
+                    const long long e = (long long)s_front[cur][c / kB] * kB + c % kB;
+                    if (e < hd.nc) {
+                        d = d2_ref(p[0], p[1], p[2], cx[e], cy[e], cz[e]);
+                        in = !(d > R2);
+                        id = cid[e];
+                    }
+                }
+                const unsigned long long m = __ballot(in);
+                const int pos = nk + __popcll(m & below);
+                if (in && pos < kRadCand) {
+                    s_d[pos] = d;
+                    s_id[pos] = id;
+                }
+                nk += __popcll(m);
+            }
+            over = nk > kRadCand || nk < k;
+        }
+        if (over) {
+            if (lane == 0) whole_list[atomicAdd(whole_count, 1u)] = w;
+            __syncthreads();
+            continue;
+        }
+        // bitonic sort by (d2, id), padded to a power of two
+        int N = 1;
+        while (N < nk) N <<= 1;
+        for (int c = nk + lane; c < N; c += kWave) {
+            s_d[c] = __longlong_as_double(0x7ff0000000000000ll);
+            s_id[c] = 0xffffffffu;
+        }
+        __syncthreads();
+        for (int size = 2; size <= N; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int c = lane; c < N / 2; c += kWave) {
+                    const int lo = 2 * c - (c & (stride - 1)), hi = lo + stride;
+                    const double da = s_d[lo], db = s_d[hi];
+                    const unsigned ia = s_id[lo], ib = s_id[hi];
+                    const bool gt = (da > db) | ((da == db) & (ia > ib));
+                    if (gt == ((lo & size) == 0)) {
+                        s_d[lo] = db;
+                        s_d[hi] = da;
+                        s_id[lo] = ib;
+                        s_id[hi] = ia;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        const long long q = qid[qi];
+        if (nbr)
+            for (int t = lane; t < k; t += kWave)
+                nbr[q * k + t] = (long long)s_id[t] - hd.idx_base;
+        if (rho && lane == 0) {
+            // the exact query's density: h from the k-th distance, the terms
+            // in list order
+            const float neardist = sqrtf((float)s_d[k - 1]);
+            const float h = (float)((double)neardist / 2.0);
+            float sum = 0.0f;
+            for (int t = 0; t < k; t++) {
+                const float r = sqrtf((float)s_d[t]);
+                float wt = W4(r, h);
+                if (rho_mode == 2) wt = wt * (float)mass[s_id[t]];
+                sum += wt;
+            }
+            rho[q] = sum;
+        }
+        __syncthreads();
     }
 }
 
@@ -1996,9 +2127,15 @@ hipError_t run_query(bool packed, unsigned nwaves, const double *qx, const doubl
                                                        xcd_chunk);
         return hipGetLastError();
     }
-    unsigned *fix = ws.alloc_n<unsigned>(1 + (size_t)nwaves);  // count, then wave ids
-    if (!fix) return hipErrorOutOfMemory;
+    // whole waves to re-run (count, then wave ids: the packed kernel's and the
+    // radius kernel's), and single flagged queries (count, then wave * 64 +
+    // lane) with their radii
+    unsigned *fix = ws.alloc_n<unsigned>(1 + (size_t)(1 + kRadLanes) * nwaves);
+    unsigned *rad = ws.alloc_n<unsigned>(1 + (size_t)kRadLanes * nwaves);
+    double *rad_r2 = ws.alloc_n<double>((size_t)kRadLanes * nwaves);
+    if (!fix || !rad || !rad_r2) return hipErrorOutOfMemory;
     PGALF_HIP_TRY(fill_u32(fix, 1, 0u, s));
+    PGALF_HIP_TRY(fill_u32(rad, 1, 0u, s));
     static const bool prof_on = [] {
         const char *v = getenv("PGALF_KNN_PROF");
         return v && *v && *v != '0';
@@ -2010,47 +2147,36 @@ hipError_t run_query(bool packed, unsigned nwaves, const double *qx, const doubl
         PGALF_HIP_TRY(fill_u64(prof, kPrN, 0ull, s));
         knn_packed_k<P, true><<<nwaves, kWave, 0, s>>>(qx, qy, qz, qid, cx, cy, cz, cid, boxes,
                                                        halos, h0, waves, k, nbr, rho, rho_mode,
-                                                       m, self_seed, fix, fix + 1, prof,
-                                                       xcd_chunk);
+                                                       m, self_seed, fix, fix + 1, rad, rad_r2,
+                                                       prof, xcd_chunk);
     } else {
         timing_mark("knn_packed", s, false);
         knn_packed_k<P, false><<<nwaves, kWave, 0, s>>>(qx, qy, qz, qid, cx, cy, cz, cid,
                                                         boxes, halos, h0, waves, k, nbr, rho,
                                                         rho_mode, m, self_seed, fix, fix + 1,
-                                                        nullptr, xcd_chunk);
+                                                        rad, rad_r2, nullptr, xcd_chunk);
         timing_mark("knn_packed", s, true);
     }
     const unsigned nfix = nwaves < kFixBlocks ? nwaves : kFixBlocks;
-    if (!halos && !waves) {
-        const size_t ns = (size_t)kSplitWaves * kWave;
-        KHalo *sh = ws.alloc_n<KHalo>(ns);
-        KWave *sw = ws.alloc_n<KWave>(ns);
-        unsigned *wl = ws.alloc_n<unsigned>(ns), *split = ws.alloc_n<unsigned>(2);
-        if (!sh || !sw || !wl || !split) return hipErrorOutOfMemory;
-        split_fix_k<<<64, 256, 0, s>>>(fix, h0, sh, sw, wl, split);
-        knn_query_k<P, true><<<kFixBlocks, kWave, 0, s>>>(
-            qx, qy, qz, qid, cx, cy, cz, cid, boxes, sh, h0, sw, k, nbr, rho, rho_mode, m,
-            self_seed, nullptr, wl, split, xcd_chunk);
-        knn_query_k<P, true><<<nfix, kWave, 0, s>>>(qx, qy, qz, qid, cx, cy, cz, cid, boxes,
-                                                    halos, h0, waves, k, nbr, rho, rho_mode, m,
-                                                    self_seed, nullptr, fix + 1, split + 1,
-                                                    xcd_chunk);
-    } else {
-        knn_query_k<P, true><<<nfix, kWave, 0, s>>>(qx, qy, qz, qid, cx, cy, cz, cid, boxes,
-                                                    halos, h0, waves, k, nbr, rho, rho_mode, m,
-                                                    self_seed, nullptr, fix + 1, fix, xcd_chunk);
-    }
+    knn_radius_k<P><<<nfix, kWave, 0, s>>>(qx, qy, qz, qid, cx, cy, cz, cid, boxes, halos, h0,
+                                           waves, k, nbr, rho, rho_mode, m, rad, rad_r2, fix,
+                                           fix + 1);
+    knn_query_k<P, true><<<nfix, kWave, 0, s>>>(qx, qy, qz, qid, cx, cy, cz, cid, boxes, halos,
+                                                h0, waves, k, nbr, rho, rho_mode, m, self_seed,
+                                                nullptr, fix + 1, fix, xcd_chunk);
     if (prof) {
         unsigned long long d[kPrN];
-        unsigned nf = 0;
+        unsigned nf = 0, nr = 0;
         PGALF_HIP_TRY(hipMemcpyAsync(d, prof, sizeof(d), hipMemcpyDeviceToHost, s));
         PGALF_HIP_TRY(hipMemcpyAsync(&nf, fix, sizeof(nf), hipMemcpyDeviceToHost, s));
+        PGALF_HIP_TRY(hipMemcpyAsync(&nr, rad, sizeof(nr), hipMemcpyDeviceToHost, s));
         PGALF_HIP_TRY(hipStreamSynchronize(s));
         const double w = d[kPrWaves] ? (double)d[kPrWaves] : 1.0;
-        fprintf(stderr, "[pgalf] packed knn: waves %llu flagged %u | kcycles per wave: seed %.1f "
+        fprintf(stderr, "[pgalf] packed knn: waves %llu flagged %llu (single queries %u, whole "
+                        "waves %u) | kcycles per wave: seed %.1f "
                         "walk %.1f scan %.1f (merge %.1f) tail %.1f total %.1f | per wave: walk "
                         "rounds %.1f scan batches %.1f staged buckets %.1f merged chunks %.1f\n",
-                d[kPrWaves], nf, d[kPrSeed] / w / 1e3, d[kPrWalk] / w / 1e3, d[kPrScan] / w / 1e3,
+                d[kPrWaves], d[kPrFlagged], nr, nf, d[kPrSeed] / w / 1e3, d[kPrWalk] / w / 1e3, d[kPrScan] / w / 1e3,
                 d[kPrMerge] / w / 1e3, d[kPrTail] / w / 1e3, d[kPrTotal] / w / 1e3,
                 d[kPrRounds] / w, d[kPrBatches] / w, d[kPrSlots] / w, d[kPrChunks] / w);
     }

@@ -8,7 +8,11 @@ The cases:
   * too few candidates (status 5, where the reference exits);
   * empty deposits and read-backs;
   * a batch of tiny ragged halos;
-  * a device grid that is not 8-byte aligned (refused).
+  * a device grid that is not 8-byte aligned (refused);
+  * queries with two neighbours at exactly equal distance, spread so that
+    each wave of the packed-key query flags at most a few lanes: those
+    queries are re-run alone by the radius kernel (the rest of the wave
+    keeps the packed results).
 
 Expected results:
   * lists come from the oracle's brute-force exact search and are bit-exact;
@@ -161,3 +165,44 @@ def test_unaligned_grid_is_rejected():
     assert bad == 0
     tot, want = float(big[:nfl].double().sum()), 64.0 * float(unit.sum())
     assert abs(tot - want) <= 1e-5 * want
+
+
+@pytest.mark.parametrize("k,rho_mode", [(32, 1), (20, 2)])
+def test_single_flagged_queries_rerun_by_radius(k, rho_mode):
+    """20,000 float32 points in [0.1, 0.9)^3, plus, for 60 queries q on a
+    coarse grid of the box, the two points q +- v (v a multiple of 2^-14 per
+    axis, so both are exact in float32 and exactly equidistant from q): each
+    such q has two of its first k at the same distance, which the packed
+    keys cannot order, so its lane is flagged (and, the queries lying far
+    apart, few lanes per wave).  Every list and density against the exact
+    search of the oracle."""
+    import torch
+    rng = np.random.default_rng(77 + k)
+    base = (0.1 + 0.8 * rng.random((20000, 3))).astype(np.float32)
+    g = np.linspace(0.15, 0.85, 4)
+    targets = np.stack(np.meshgrid(g, g, g[:4]), -1).reshape(-1, 3)[:60]
+    picks = [int(np.argmin(((base - t) ** 2).sum(1))) for t in targets]
+    v = np.array([3.0, -2.0, 1.0], dtype=np.float32) * np.float32(2.0 ** -14)
+    extra = np.concatenate([np.stack([base[i] + v, base[i] - v]) for i in picks])
+    pos = np.concatenate([base, extra]).astype(np.float32)
+    # the planted pairs are exact and exactly equidistant from their query
+    for j, i in enumerate(picks):
+        a, b = pos[20000 + 2 * j].astype(np.float64), pos[20001 + 2 * j].astype(np.float64)
+        q = pos[i].astype(np.float64)
+        assert ((q - a) ** 2).sum() == ((q - b) ** 2).sum()
+    n = len(pos)
+    say("planted ties: %d queries, n=%d k=%d rho_mode=%d" % (len(picks), n, k, rho_mode))
+    mass = (0.5 + rng.random(n)).astype(np.float32)
+    x, y, z = (torch.from_numpy(np.ascontiguousarray(pos[:, a])).cuda() for a in range(3))
+    m = torch.from_numpy(mass).cuda()
+    nbr = torch.empty((n, k), dtype=torch.int64, device="cuda")
+    rho = torch.empty(n, dtype=torch.float32, device="cuda")
+    D.dev_knn(x, y, z, k, m=m, nbr=nbr, rho=rho, rho_mode=rho_mode)
+    D.dev_status()
+    idx, d2 = O.knn(pos.astype(np.float64), k)
+    got = nbr.cpu().numpy()
+    assert np.array_equal(got[picks], idx[picks])
+    assert np.array_equal(got, idx)
+    want = O.sph_density(idx, d2, mass if rho_mode == 2 else None)
+    assert np.array_equal(rho.cpu().numpy().view(np.int32), want.view(np.int32))
+
