@@ -1950,9 +1950,12 @@ knn_packed_k(const double *qx, const double *qy, const double *qz, const unsigne
 
 inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
-// Blocks of the exact pass over the waves the packed-key kernel flagged
-// (grid-stride over the list; the list is short for distinct positions).
-constexpr unsigned kFixBlocks = 1024;
+// Blocks of the re-runs, grid-stride over their lists (short for distinct
+// positions).  Few blocks: the launches are made whatever the lists hold,
+// and an empty block of the exact kernel (256 VGPRs) still waits for a free
+// SIMD while the grid chain's kernels occupy the chip.
+constexpr unsigned kFixBlocks = 64;
+constexpr unsigned kRadBlocks = 256;
 // ---- exact re-run of single flagged queries ------------------------------
 // A lane of the packed-key query fails its check only through the class of
 // its k-th key (a displaced candidate in that class, or two of its first k
@@ -2158,7 +2161,8 @@ hipError_t run_query(bool packed, unsigned nwaves, const double *qx, const doubl
         timing_mark("knn_packed", s, true);
     }
     const unsigned nfix = nwaves < kFixBlocks ? nwaves : kFixBlocks;
-    knn_radius_k<P><<<nfix, kWave, 0, s>>>(qx, qy, qz, qid, cx, cy, cz, cid, boxes, halos, h0,
+    const unsigned nrad = nwaves < kRadBlocks ? nwaves : kRadBlocks;
+    knn_radius_k<P><<<nrad, kWave, 0, s>>>(qx, qy, qz, qid, cx, cy, cz, cid, boxes, halos, h0,
                                            waves, k, nbr, rho, rho_mode, m, rad, rad_r2, fix,
                                            fix + 1);
     knn_query_k<P, true><<<nfix, kWave, 0, s>>>(qx, qy, qz, qid, cx, cy, cz, cid, boxes, halos,
