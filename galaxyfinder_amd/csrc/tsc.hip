@@ -301,22 +301,36 @@ __global__ void tsc_scale_k(const unsigned *nsel, const unsigned long long *maxm
 }
 
 // The rows of tile t (TZ runs of TY * mx consecutive floats, clipped to the
-// grid) from `src` (the tile's node values in x + mx (ly + TY lz) order; a
-// null src writes zeros).
-template <class F>
+// grid): value2(l) gives the tile's nodes l and l + 1 (x + mx (ly + TY lz)
+// order).  Written as float pairs: mx and every grid offset are even and the
+// grid is 8-byte aligned, so each pair is an aligned 8-byte store.
+template <class F2>
 __device__ inline void write_tile_rows(float *den, const HaloSeg &sg, const TileGeom &tg,
-                                       unsigned tl, F value) {
+                                       unsigned tl, F2 value2) {
     const int ty = (int)(tl % (unsigned)tg.nty), tz = (int)(tl / (unsigned)tg.nty);
     const long long mx = tg.mx;
     const int nrows = min(tg.TY, sg.g.ny - ty * tg.TY);
-    const int run = nrows * tg.mx;
+    const int run2 = nrows * tg.mx / 2;
     for (int lz = 0; lz < tg.TZ; lz++) {
         const long long z = (long long)tz * tg.TZ + lz;
         if (z >= sg.g.nz) break;
-        float *dst = den + sg.goff + mx * ((long long)ty * tg.TY + (long long)sg.g.ny * z);
+        float2 *dst = reinterpret_cast<float2 *>(
+            den + sg.goff + mx * ((long long)ty * tg.TY + (long long)sg.g.ny * z));
         const int l0 = lz * tg.TY * tg.mx;
-        for (int i = threadIdx.x; i < run; i += blockDim.x) dst[i] = value(l0 + i);
+        for (int i = threadIdx.x; i < run2; i += blockDim.x) dst[i] = value2(l0 + 2 * i);
     }
+}
+
+// Nodes l, l + 1 of a fixed-point tile as floats; a pair of empty nodes
+// (most of a sparse tile) skips the conversion.
+__device__ inline float2 fixed_pair(const unsigned long long *acc, int l, double inv) {
+    const unsigned long long a0 = acc[l], a1 = acc[l + 1];
+    float2 v = make_float2(0.0f, 0.0f);
+    if ((a0 | a1) != 0ull) {
+        v.x = (float)((double)(long long)a0 * inv);
+        v.y = (float)((double)(long long)a1 * inv);
+    }
+    return v;
 }
 
 template <class P>
@@ -376,8 +390,7 @@ tsc_deposit_k(const HaloSeg *segs, HaloSeg seg0, const unsigned *tile_halo,
         __syncthreads();
         if (nu == 1) {
             // the tile's rows, straight into the grid
-            write_tile_rows(den, sg, tg, tl,
-                            [&](int l) { return (float)((double)(long long)acc[l] * inv); });
+            write_tile_rows(den, sg, tg, tl, [&](int l) { return fixed_pair(acc, l, inv); });
         } else {
             unsigned long long *dst = side + (size_t)multi_slot[t] * kTileCap;
             for (int l = threadIdx.x; l < nodes; l += kBlock)
@@ -402,12 +415,11 @@ tsc_rows_k(const HaloSeg *segs, HaloSeg seg0, const unsigned *tile_halo, long lo
         const TileGeom tg = tile_geom(sg.g);
         const unsigned tl = (unsigned)(t - sg.toff);
         if (nu == 0) {
-            write_tile_rows(den, sg, tg, tl, [](int) { return 0.0f; });
+            write_tile_rows(den, sg, tg, tl, [](int) { return make_float2(0.0f, 0.0f); });
         } else {
             const double inv = scale[2 * h + 1];
             const unsigned long long *src = side + (size_t)multi_slot[t] * kTileCap;
-            write_tile_rows(den, sg, tg, tl,
-                            [&](int l) { return (float)((double)(long long)src[l] * inv); });
+            write_tile_rows(den, sg, tg, tl, [&](int l) { return fixed_pair(src, l, inv); });
         }
     }
 }
