@@ -188,3 +188,30 @@ def test_catalog_watershed_on_the_10000_halo_catalogue():
     for h in dict.fromkeys(int(v) for v in pick):
         core, fl, hid, shells = _one_halo(c, h)
         _same_halo(got, c, h, core, fl, hid, shells)
+
+
+def test_catalog_watershed_with_halos_without_cores(cat):
+    """Halos with no cores (the reference's subhalo_den never gets here with
+    none; a caller of the catalogue call may pass them): no core outputs,
+    one shell holding every particle in index order, the same as the
+    one-halo path; the other halos as before.  Both paths."""
+    import torch
+    off, coff, peak = cat["off"], cat["coff"], cat["peak"]
+    keep = np.ones(len(off) - 1, bool)
+    keep[[1, 7, 50, 100]] = False
+    peaks = [peak[coff[h]:coff[h + 1]] if keep[h] else peak[:0] for h in range(len(off) - 1)]
+    c2 = dict(cat)
+    c2["peak"] = np.concatenate(peaks).astype(np.int32)
+    c2["coff"] = np.concatenate([[0], np.cumsum([len(p) for p in peaks])]).astype(np.int64)
+    for small_max in (0, 1):
+        got = _run_catalog(c2, small_max)
+        assert not got["status"].any()
+        for h in (0, 1, 7, 8, 50, 100):
+            core, fl, hid, shells = _one_halo(c2, h)
+            _same_halo(got, c2, h, core, fl, hid, shells)
+            if not keep[h]:
+                n = int(off[h + 1] - off[h])
+                gs = D.catalog_shells_of(got, off, h)
+                assert len(gs) == 1 and np.array_equal(gs[0][0], np.arange(n))
+                assert len(gs[0][1]) == 0
+    say("halos without cores: equal to the one-halo path")
