@@ -926,6 +926,21 @@ __global__ void ml_above_k(const float *den, long long np, float t, unsigned *ab
     above[w] = bits;
 }
 
+// The next level's plan (thread 0): each root's frontier [lo, hi) at
+// off[r] of the level's node space, whether its candidates fit in cv, and
+// whether every sweep is done.
+__device__ void ml_plan(MultiSweep &a) {
+    long long o = 0;
+    for (int r = 0; r < a.M; r++) {
+        a.off[r] = o;
+        o += a.hi[r] - a.lo[r];
+    }
+    a.off[a.M] = o;
+    *a.keep = o * 32 <= kMlCandCap ? 1 : 0;
+    if (o == 0) *a.done = 1;
+}
+
+// The roots (one lane each), then the first level's plan.
 __global__ void ml_init_k(MultiSweep a) {
     const int r = threadIdx.x;
     if (r < a.M) {
@@ -936,18 +951,8 @@ __global__ void ml_init_k(MultiSweep a) {
         a.hi[r] = 1;
     }
     if (r == 0) *a.done = 0;
-}
-
-__global__ void ml_plan_k(MultiSweep a) {
-    if (*a.done) return;
-    long long o = 0;
-    for (int r = 0; r < a.M; r++) {
-        a.off[r] = o;
-        o += a.hi[r] - a.lo[r];
-    }
-    a.off[a.M] = o;
-    *a.keep = o * 32 <= kMlCandCap ? 1 : 0;
-    if (o == 0) *a.done = 1;
+    __syncthreads();
+    if (r == 0) ml_plan(a);
 }
 
 // Candidate j = node * 32 + slot of root r's level: its particle, or -1.
@@ -1073,14 +1078,20 @@ __global__ __launch_bounds__(256) void ml_scan3_k(MultiSweep a) {
     }
 }
 
+// The level's appended winners close each root's frontier (one lane per
+// root), then the next level's plan.
 __global__ void ml_advance_k(MultiSweep a) {
     if (*a.done) return;
     const int r = threadIdx.x;
-    if (r >= a.M) return;
-    const long long f0 = a.off[r], f1 = a.off[r + 1];
-    const long long c = f1 > f0 ? (long long)a.base[f1 - 1] + a.cnt[f1 - 1] - a.base[f0] : 0;
-    a.lo[r] = a.hi[r];
-    a.hi[r] += c;
+    if (r < a.M) {
+        const long long f0 = a.off[r], f1 = a.off[r + 1];
+        const long long c =
+            f1 > f0 ? (long long)a.base[f1 - 1] + a.cnt[f1 - 1] - a.base[f0] : 0;
+        a.lo[r] = a.hi[r];
+        a.hi[r] += c;
+    }
+    __syncthreads();
+    if (r == 0) ml_plan(a);
 }
 
 hipError_t multi_bufs(MultiSweep &a, long long np, Workspace &ws, hipStream_t s) {
@@ -1125,7 +1136,6 @@ hipError_t multi_sweep(MultiSweep &a, const int *roots_h, int M, float t, const 
     ml_init_k<<<1, 64, 0, s>>>(a);
     for (;;) {
         for (int l = 0; l < kMlLevelsPerSync; l++) {
-            ml_plan_k<<<1, 1, 0, s>>>(a);
             const dim3 grid(kMlBlocksPerRoot, M);
             ml_claim_k<<<grid, 256, 0, s>>>(a, t, den, nbr, k);
             ml_win_k<false><<<grid, 256, 0, s>>>(a, t, den, nbr, k);
