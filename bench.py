@@ -305,6 +305,50 @@ def peaks_bench(x, y, z, m, g, den, rho_t, nbr, rho_k, spec, reps=10, cpu=True):
     return out
 
 
+def host_api_bench(pos, mass, spec, reps=3):
+    """The drop-in entry points with host buffers (the reference's own call
+    sequence, level A of the integration notes) on the headline halo:
+    assign_density_TSC -> gaussian_Smoothing -> findDen -> findsphdensity,
+    each copying its inputs to the device and its outputs back (PCIe
+    included), 88-byte particle records in FP64; and pgalf_lag_density_stage
+    (level B: the same stage in one device residency, with the reference's
+    own grid sizing).  Wall time per call, median of `reps`."""
+    from galaxyfinder_amd import density as D
+    n, k = len(pos), spec["k"]
+    nx, ny, nz = spec["nx"], spec["ny"], spec["nz"]
+    g0 = (spec["xmin"], spec["ymin"], spec["zmin"], spec["cellsize"])
+    bp = D.make_particles(pos.astype(np.float64), mass.astype(np.float64))
+    den = np.zeros(D.padded_shape(nx, ny, nz), dtype=np.float32)
+    densph = np.zeros(n, dtype=np.float32)
+    nbr = np.zeros((n, k), dtype=np.int64)
+    rho = np.zeros(n, dtype=np.float32)
+    calls = {
+        "assign_density_TSC": lambda: D.assign_density_TSC(bp, n, den, nx, ny, nz, *g0, 5),
+        "gaussian_Smoothing": lambda: D.gaussian_Smoothing(den, nx, ny, nz, spec["cellsize"],
+                                                           spec["RG"]),
+        "findDen": lambda: D.findDen(bp, n, densph, *g0, den, nx, ny, nz),
+        "findsphdensity": lambda: D.findsphdensity(bp, n, nbr, k, rho),
+        "pgalf_lag_density_stage": lambda: D.lag_density_stage(bp, k, 5),
+    }
+    ms = {}
+    for name, fn in calls.items():
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        ms[name] = round(float(np.median(ts)), 2)
+    seq = sum(ms[a] for a in ("assign_density_TSC", "gaussian_Smoothing", "findDen",
+                              "findsphdensity"))
+    return {"workload": "one halo, N=%d, %d^3 grid, RG=%.1f cells, k=%d; host buffers "
+                        "(88-byte records)" % (n, nx, spec["rg_cells"], k),
+            "ms": ms, "sequence_ms": round(seq, 2),
+            "sequence_particles_per_s": round(n / (seq * 1e-3), 1),
+            "lag_density_stage_particles_per_s": round(
+                n / (ms["pgalf_lag_density_stage"] * 1e-3), 1)}
+
+
 def error_report():
     """Observed errors of the GPU stage against FP64 on the one-halo
     workloads: the deposit and the smoothed grid (normwise over the grid,
@@ -876,6 +920,11 @@ def main():
     ap.add_argument("--errors", action="store_true",
                     help="observed errors of the stage's outputs against the FP64 oracle on "
                          "the 131072-, 4194304- and 1048576-particle halos (one JSON line)")
+    ap.add_argument("--host-api", action="store_true",
+                    help="only the drop-in entry points with host buffers (PCIe-inclusive "
+                         "times) on the headline halo")
+    ap.add_argument("--no-host-api", action="store_true",
+                    help="skip the host-buffer (PCIe-inclusive) measurement")
     ap.add_argument("--peaks-only", action="store_true",
                     help="one density stage on the headline halo, then only the peak stage "
                          "and the watershed (the 'peaks' object; for profiling)")
@@ -926,6 +975,9 @@ def main():
         return
     pos, mass, spec = halos.single_halo_config(args.config)
     n, k = len(pos), spec["k"]
+    if args.host_api:
+        print(json.dumps({"host_api": host_api_bench(pos, mass, spec)}), flush=True)
+        return
     print("[bench] workload config %d: N=%d grid=%d^3 RG=%.2f cells k=%d"
           % (args.config, n, spec["nx"], spec["rg_cells"], k), file=sys.stderr, flush=True)
     dev = torch.device("cuda")
@@ -1026,6 +1078,8 @@ def main():
         "stage_roofline_frac": round(stage_alg / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": roofline,
     }
+    if not args.no_host_api:
+        out["host_api"] = host_api_bench(pos, mass, spec)
     if not args.no_peaks:
         out["peaks"] = peaks_bench(x, y, z, m, g, den, rho_t, nbr, rho_k, spec,
                                    cpu=not args.no_cpu_baseline)

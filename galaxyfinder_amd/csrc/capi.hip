@@ -10,6 +10,8 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <system_error>
+#include <thread>
 #include <vector>
 
 #include "pgalf_internal.h"
@@ -318,6 +320,100 @@ int consume_status(Workspace &ws, hipStream_t s, const char *where) {
     return ok();
 }
 
+// ---- host copies through pinned staging ------------------------------------
+// The host-array entry points move the caller's pageable arrays (the particle
+// records in, grids and neighbour lists out).  A copy from or to pageable
+// memory runs well below the link's speed, so large copies go through two
+// pinned staging buffers per device: the DMA of one chunk overlaps host
+// threads copying the neighbouring chunk between the caller's memory and the
+// other buffer.  Small copies go directly.  Both return with the caller's
+// memory no longer needed (host to device) or filled (device to host).
+constexpr size_t kStageBytes = (size_t)32 << 20;
+constexpr size_t kStageMin = (size_t)8 << 20;
+struct Staging {
+    void *buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+static hipError_t staging_for_current_device(Staging **out) {
+    static std::map<int, Staging> st;  // under g_api_mu
+    int dev = 0;
+    PGALF_HIP_TRY(hipGetDevice(&dev));
+    Staging &x = st[dev];
+    for (int b = 0; b < 2; b++) {
+        if (!x.buf[b]) PGALF_HIP_TRY(hipHostMalloc(&x.buf[b], kStageBytes, hipHostMallocDefault));
+        if (!x.ev[b]) PGALF_HIP_TRY(hipEventCreateWithFlags(&x.ev[b], hipEventDisableTiming));
+    }
+    *out = &x;
+    return hipSuccess;
+}
+
+// memcpy on up to 8 host threads (the calling one included)
+static void par_memcpy(void *dst, const void *src, size_t n) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const unsigned nt = hw < 2 ? 1u : (hw < 8 ? hw : 8u);
+    if (nt == 1 || n < ((size_t)1 << 20)) {
+        memcpy(dst, src, n);
+        return;
+    }
+    const size_t per = (n + nt - 1) / nt;
+    std::vector<std::thread> th;
+    for (unsigned i = 1; i < nt && i * per < n; i++) {
+        char *d = static_cast<char *>(dst) + i * per;
+        const char *s = static_cast<const char *>(src) + i * per;
+        const size_t len = n - i * per < per ? n - i * per : per;
+        try {
+            th.emplace_back([d, s, len] { memcpy(d, s, len); });
+        } catch (const std::system_error &) {
+            memcpy(d, s, len);
+        }
+    }
+    memcpy(dst, src, per < n ? per : n);
+    for (auto &t : th) t.join();
+}
+
+static hipError_t host_to_dev(void *d, const void *h, size_t n, hipStream_t s) {
+    if (n < kStageMin) return hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, s);
+    Staging *st = nullptr;
+    PGALF_HIP_TRY(staging_for_current_device(&st));
+    for (size_t o = 0, i = 0; o < n; o += kStageBytes, i++) {
+        const int b = (int)(i & 1);
+        const size_t len = n - o < kStageBytes ? n - o : kStageBytes;
+        // the buffer's previous DMA (of this call or an earlier one) is done
+        PGALF_HIP_TRY(hipEventSynchronize(st->ev[b]));
+        par_memcpy(st->buf[b], static_cast<const char *>(h) + o, len);
+        PGALF_HIP_TRY(hipMemcpyAsync(static_cast<char *>(d) + o, st->buf[b], len,
+                                     hipMemcpyHostToDevice, s));
+        PGALF_HIP_TRY(hipEventRecord(st->ev[b], s));
+    }
+    return hipSuccess;
+}
+
+static hipError_t dev_to_host(void *h, const void *d, size_t n, hipStream_t s) {
+    if (n < kStageMin) {
+        PGALF_HIP_TRY(hipMemcpyAsync(h, d, n, hipMemcpyDeviceToHost, s));
+        return hipStreamSynchronize(s);
+    }
+    Staging *st = nullptr;
+    PGALF_HIP_TRY(staging_for_current_device(&st));
+    const size_t nch = (n + kStageBytes - 1) / kStageBytes;
+    for (size_t i = 0; i <= nch; i++) {
+        if (i < nch) {  // chunk i into buffer i % 2 (after that buffer's earlier DMAs: stream order)
+            const size_t o = i * kStageBytes, len = n - o < kStageBytes ? n - o : kStageBytes;
+            PGALF_HIP_TRY(hipMemcpyAsync(st->buf[i & 1], static_cast<const char *>(d) + o, len,
+                                         hipMemcpyDeviceToHost, s));
+            PGALF_HIP_TRY(hipEventRecord(st->ev[i & 1], s));
+        }
+        if (i > 0) {  // chunk i - 1 out of its buffer while chunk i moves
+            const size_t j = i - 1, o = j * kStageBytes;
+            const size_t len = n - o < kStageBytes ? n - o : kStageBytes;
+            PGALF_HIP_TRY(hipEventSynchronize(st->ev[j & 1]));
+            par_memcpy(static_cast<char *>(h) + o, st->buf[j & 1], len);
+        }
+    }
+    return hipSuccess;
+}
+
 static bool check_grid(int nx, int ny, int nz) {
     return nx >= 3 && ny >= 3 && nz >= 3 && (long long)nx * ny * nz < (1LL << 31);
 }
@@ -339,8 +435,7 @@ hipError_t upload_particles(const pgalf_particle_t *bp, long long np, Workspace 
     d.m = ws.alloc_n<double>(np);
     d.type = ws.alloc_n<int>(np);
     if (!raw || !d.x || !d.y || !d.z || !d.m || !d.type) return hipErrorOutOfMemory;
-    PGALF_HIP_TRY(hipMemcpyAsync(raw, bp, sizeof(pgalf_particle_t) * (size_t)np,
-                                 hipMemcpyHostToDevice, s));
+    PGALF_HIP_TRY(host_to_dev(raw, bp, sizeof(pgalf_particle_t) * (size_t)np, s));
     return unpack_particles(raw, np, d.x, d.y, d.z, d.m, d.type, s);
 }
 
@@ -381,7 +476,7 @@ extern "C" void assign_density_TSC(pgalf_particle_t *bp, int np, float *den, int
     if (e == hipSuccess)
         e = tsc_deposit<double>(d.x, d.y, d.z, d.m, d.type, np, ptype, g, dden, ws, s);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(den, dden, sizeof(float) * g.ncells(), hipMemcpyDeviceToHost, s);
+        e = dev_to_host(den, dden, sizeof(float) * g.ncells(), s);
     if (e != hipSuccess) {
         hip_err(e, "assign_density_TSC");
         return;
@@ -407,10 +502,9 @@ extern "C" void gaussian_Smoothing(float *denGrid, int nx, int ny, int nz, doubl
         if (!dden) e = hipErrorOutOfMemory;
     }
     if (e == hipSuccess)
-        e = hipMemcpyAsync(dden, denGrid, sizeof(float) * g.ncells(), hipMemcpyHostToDevice, s);
+        e = host_to_dev(dden, denGrid, sizeof(float) * g.ncells(), s);
     if (e == hipSuccess) e = gaussian_smooth(dden, g, RG, ws, s);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(denGrid, dden, sizeof(float) * g.ncells(), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = dev_to_host(denGrid, dden, sizeof(float) * g.ncells(), s);
     if (e != hipSuccess) {
         hip_err(e, "gaussian_Smoothing");
         return;
@@ -440,10 +534,9 @@ extern "C" void findDen(pgalf_particle_t *bp, int np, float *densph, double xmin
         if (!dden || !dout) e = hipErrorOutOfMemory;
     }
     if (e == hipSuccess)
-        e = hipMemcpyAsync(dden, den, sizeof(float) * g.ncells(), hipMemcpyHostToDevice, s);
+        e = host_to_dev(dden, den, sizeof(float) * g.ncells(), s);
     if (e == hipSuccess) e = tsc_readback<double>(d.x, d.y, d.z, np, g, dden, dout, ws, s);
-    if (e == hipSuccess && np > 0)
-        e = hipMemcpyAsync(densph, dout, sizeof(float) * np, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && np > 0) e = dev_to_host(densph, dout, sizeof(float) * np, s);
     if (e != hipSuccess) {
         hip_err(e, "findDen");
         return;
@@ -476,10 +569,8 @@ static int host_knn(pgalf_particle_t *bp, int np, int k, long long *nearindex, f
         e = knn<double>(d.x, d.y, d.z, d.m, np, cand_ptype == PGALF_TYPE_ALL ? nullptr : d.type,
                         cand_ptype, k, dnbr, drho, rho_mode, ws, s);
     if (e == hipSuccess && dnbr)
-        e = hipMemcpyAsync(nearindex, dnbr, sizeof(long long) * (size_t)np * k,
-                           hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess && drho)
-        e = hipMemcpyAsync(dens, drho, sizeof(float) * np, hipMemcpyDeviceToHost, s);
+        e = dev_to_host(nearindex, dnbr, sizeof(long long) * (size_t)np * k, s);
+    if (e == hipSuccess && drho) e = dev_to_host(dens, drho, sizeof(float) * np, s);
     if (e != hipSuccess) return hip_err(e, where);
     return consume_status(ws, s, where);
 }
@@ -578,16 +669,14 @@ extern "C" int pgalf_lag_density_stage(pgalf_particle_t *bp, int np, int Numnear
     } else if (e == hipSuccess) {
         e = tsc_readback<double>(d.x, d.y, d.z, np, g, dden, drho, ws, s);
     }
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(densph, drho, sizeof(float) * np, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = dev_to_host(densph, drho, sizeof(float) * np, s);
     if (e == hipSuccess && nearindex)
-        e = hipMemcpyAsync(nearindex, dnbr, sizeof(long long) * (size_t)np * Numnear,
-                           hipMemcpyDeviceToHost, s);
+        e = dev_to_host(nearindex, dnbr, sizeof(long long) * (size_t)np * Numnear, s);
     float *hgrid = nullptr;
     if (e == hipSuccess && grid_out) {
         hgrid = (float *)malloc(sizeof(float) * g.ncells());
         if (!hgrid) return set_err(PGALF_ERR_NOMEM, "%s: host allocation failed", where);
-        e = hipMemcpyAsync(hgrid, dden, sizeof(float) * g.ncells(), hipMemcpyDeviceToHost, s);
+        e = dev_to_host(hgrid, dden, sizeof(float) * g.ncells(), s);
     }
     if (e != hipSuccess) {
         free(hgrid);
