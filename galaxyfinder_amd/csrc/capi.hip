@@ -2,6 +2,7 @@
 // (assign_density_TSC, gaussian_Smoothing, findDen, findsphdensity,
 // starfindsphdensity; tsc_omp2.c / gsmooth.c / nnost.c) as synchronous
 // host-array calls, plus the device-resident API used from Python.
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -222,17 +223,57 @@ hipError_t unpack_particles(const void *d_bp, long long np, double *x, double *y
     return hipGetLastError();
 }
 
+// fn(lo, hi) over [0, n) split across up to 8 host threads (the calling
+// one included; a thread that cannot start leaves its part to the caller)
+template <class F>
+static void par_for(size_t n, size_t min_serial, F fn) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const unsigned nt = hw < 2 ? 1u : (hw < 8 ? hw : 8u);
+    if (nt == 1 || n < min_serial) {
+        fn((size_t)0, n);
+        return;
+    }
+    const size_t per = (n + nt - 1) / nt;
+    std::vector<std::thread> th;
+    for (unsigned i = 1; i < nt && i * per < n; i++) {
+        const size_t lo = i * per, hi = n - lo < per ? n : lo + per;
+        try {
+            th.emplace_back([&fn, lo, hi] { fn(lo, hi); });
+        } catch (const std::system_error &) {
+            fn(lo, hi);
+        }
+    }
+    fn((size_t)0, per < n ? per : n);
+    for (auto &t : th) t.join();
+}
+
 // Grid box of nnost.c:1040-1056: min/max of all particles, 1e-4 extent
 // buffer on the max side, NCELLBUFF = 10 cells, origin shifted by 5 cells.
 void lag_grid_box(const double *x, const double *y, const double *z, long long n,
                   long long stride, int nxyz[3], double xyzmin[3]) {
     const int nbuff = 10;  // NCELLBUFF (params.h:115)
     const double *c[3] = {x, y, z};
+    // min and max over host threads (order-free: the same values)
+    double plo[8][3], phi[8][3];
+    std::atomic<int> part(0);
+    par_for((size_t)(n > 0 ? n : 0), (size_t)1 << 16, [&](size_t a, size_t b) {
+        double l[3] = {1e20, 1e20, 1e20}, h[3] = {-1e20, -1e20, -1e20};
+        for (size_t i = a; i < b; i++)
+            for (int d = 0; d < 3; d++) {
+                l[d] = fmin(l[d], c[d][(long long)i * stride]);
+                h[d] = fmax(h[d], c[d][(long long)i * stride]);
+            }
+        const int q = part.fetch_add(1);
+        for (int d = 0; d < 3; d++) {
+            plo[q][d] = l[d];
+            phi[q][d] = h[d];
+        }
+    });
     for (int d = 0; d < 3; d++) {
         double lo = 1e20, hi = -1e20;
-        for (long long i = 0; i < n; i++) {
-            lo = fmin(lo, c[d][i * stride]);
-            hi = fmax(hi, c[d][i * stride]);
+        for (int q = 0; q < part.load(); q++) {
+            lo = fmin(lo, plo[q][d]);
+            hi = fmax(hi, phi[q][d]);
         }
         hi = hi + (hi - lo) * 0.0001;
         nxyz[d] = (int)((hi - lo) / kLagCellSize + nbuff);
@@ -348,28 +389,10 @@ static hipError_t staging_for_current_device(Staging **out) {
     return hipSuccess;
 }
 
-// memcpy on up to 8 host threads (the calling one included)
 static void par_memcpy(void *dst, const void *src, size_t n) {
-    const unsigned hw = std::thread::hardware_concurrency();
-    const unsigned nt = hw < 2 ? 1u : (hw < 8 ? hw : 8u);
-    if (nt == 1 || n < ((size_t)1 << 20)) {
-        memcpy(dst, src, n);
-        return;
-    }
-    const size_t per = (n + nt - 1) / nt;
-    std::vector<std::thread> th;
-    for (unsigned i = 1; i < nt && i * per < n; i++) {
-        char *d = static_cast<char *>(dst) + i * per;
-        const char *s = static_cast<const char *>(src) + i * per;
-        const size_t len = n - i * per < per ? n - i * per : per;
-        try {
-            th.emplace_back([d, s, len] { memcpy(d, s, len); });
-        } catch (const std::system_error &) {
-            memcpy(d, s, len);
-        }
-    }
-    memcpy(dst, src, per < n ? per : n);
-    for (auto &t : th) t.join();
+    par_for(n, (size_t)1 << 20, [dst, src](size_t lo, size_t hi) {
+        memcpy(static_cast<char *>(dst) + lo, static_cast<const char *>(src) + lo, hi - lo);
+    });
 }
 
 static hipError_t host_to_dev(void *d, const void *h, size_t n, hipStream_t s) {
@@ -426,17 +449,54 @@ struct DeviceParticles {
     int *type;
 };
 
+// The fields the stage reads (x, y, z, mass, type: 36 of the record's 88
+// bytes) into device SoA arrays.  Large arrays are gathered on host threads
+// into the pinned staging buffers, one chunk of SoA sections at a time, so
+// the link carries only those fields; small ones go as records and are
+// unpacked on the device.
 hipError_t upload_particles(const pgalf_particle_t *bp, long long np, Workspace &ws,
                             hipStream_t s, DeviceParticles &d) {
-    void *raw = ws.alloc(sizeof(pgalf_particle_t) * (size_t)np);
     d.x = ws.alloc_n<double>(np);
     d.y = ws.alloc_n<double>(np);
     d.z = ws.alloc_n<double>(np);
     d.m = ws.alloc_n<double>(np);
     d.type = ws.alloc_n<int>(np);
-    if (!raw || !d.x || !d.y || !d.z || !d.m || !d.type) return hipErrorOutOfMemory;
-    PGALF_HIP_TRY(host_to_dev(raw, bp, sizeof(pgalf_particle_t) * (size_t)np, s));
-    return unpack_particles(raw, np, d.x, d.y, d.z, d.m, d.type, s);
+    if (!d.x || !d.y || !d.z || !d.m || !d.type) return hipErrorOutOfMemory;
+    if (sizeof(pgalf_particle_t) * (size_t)np < kStageMin) {
+        void *raw = ws.alloc(sizeof(pgalf_particle_t) * (size_t)np);
+        if (!raw) return hipErrorOutOfMemory;
+        PGALF_HIP_TRY(hipMemcpyAsync(raw, bp, sizeof(pgalf_particle_t) * (size_t)np,
+                                     hipMemcpyHostToDevice, s));
+        return unpack_particles(raw, np, d.x, d.y, d.z, d.m, d.type, s);
+    }
+    Staging *st = nullptr;
+    PGALF_HIP_TRY(staging_for_current_device(&st));
+    const size_t per = kStageBytes / (4 * sizeof(double) + sizeof(int)) / 64 * 64;
+    for (size_t o = 0, i = 0; o < (size_t)np; o += per, i++) {
+        const int b = (int)(i & 1);
+        const size_t c = (size_t)np - o < per ? (size_t)np - o : per;
+        double *hx = static_cast<double *>(st->buf[b]);
+        double *hy = hx + per, *hz = hy + per, *hm = hz + per;
+        int *ht = reinterpret_cast<int *>(hm + per);
+        PGALF_HIP_TRY(hipEventSynchronize(st->ev[b]));  // the buffer's previous DMAs are done
+        const pgalf_particle_t *src = bp + o;
+        par_for(c, (size_t)1 << 16, [=](size_t lo, size_t hi) {
+            for (size_t j = lo; j < hi; j++) {
+                hx[j] = src[j].x;
+                hy[j] = src[j].y;
+                hz[j] = src[j].z;
+                hm[j] = src[j].mass;
+                ht[j] = (int)src[j].type;
+            }
+        });
+        PGALF_HIP_TRY(hipMemcpyAsync(d.x + o, hx, sizeof(double) * c, hipMemcpyHostToDevice, s));
+        PGALF_HIP_TRY(hipMemcpyAsync(d.y + o, hy, sizeof(double) * c, hipMemcpyHostToDevice, s));
+        PGALF_HIP_TRY(hipMemcpyAsync(d.z + o, hz, sizeof(double) * c, hipMemcpyHostToDevice, s));
+        PGALF_HIP_TRY(hipMemcpyAsync(d.m + o, hm, sizeof(double) * c, hipMemcpyHostToDevice, s));
+        PGALF_HIP_TRY(hipMemcpyAsync(d.type + o, ht, sizeof(int) * c, hipMemcpyHostToDevice, s));
+        PGALF_HIP_TRY(hipEventRecord(st->ev[b], s));
+    }
+    return hipSuccess;
 }
 
 Grid make_grid(int nx, int ny, int nz, double x0, double y0, double z0, double h) {
