@@ -3,9 +3,9 @@
 // Loader (gfind.c:25-91,221-241,363-371; subhaloden.mod6.c:79-177,
 // 2810-2837): the halo catalogue file holds a 7-float header and one HaloQ
 // record per halo, the member file each halo's DM, gas, sink and star
-// records in that order.  A batch of halos is read with one sequential read
-// per halo (smaller halos are skipped over); each halo is then localised on
-// host threads as subhalo_den does
+// records in that order.  A batch of halos is chosen from the HaloQ records
+// (smaller halos are skipped over); host threads then read their halos'
+// records with positional reads and localise each halo as subhalo_den does
 // before the density stage: Dump2BP subtracts the per-axis minimum
 // (FP64), mklocalize moves the particles below the widest gap of a halo
 // that spans the periodic box by one box size.  The reference's gap search
@@ -24,13 +24,16 @@
 // to GALFIND.DATA; the particles of no subhalo go to the background file.
 // Sums run in FP64 in particle order, as the reference's.
 #include <pthread.h>
+#include <unistd.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
 #include <new>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -83,6 +86,33 @@ void record_mv(const unsigned char *r, int type, double &m, double &vx, double &
     }
 }
 
+// A host array that grows without zero-filling: the loader's threads write
+// every element, so the pages are first touched by those threads.
+template <class T>
+struct HostArray {
+    std::unique_ptr<T[]> p;
+    size_t n = 0, cap = 0;
+    bool resize(size_t m) {
+        if (m > cap) {
+            p.reset(new (std::nothrow) T[m]);
+            cap = p ? m : 0;
+        }
+        n = p ? m : 0;
+        return p || m == 0;
+    }
+    T *data() { return p.get(); }
+    const T *data() const { return p.get(); }
+    T &operator[](size_t i) { return p[i]; }
+    const T &operator[](size_t i) const { return p[i]; }
+    size_t size() const { return n; }
+    bool empty() const { return n == 0; }
+};
+
+// Host threads for the loader: at most 16, one if the machine has one.
+unsigned loader_threads() {
+    return std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+}
+
 }  // namespace
 
 struct pgalf_catalog {
@@ -96,9 +126,10 @@ struct pgalf_catalog {
     std::vector<long long> off, roff;  // particle and record-byte offsets per halo
     std::unique_ptr<unsigned char[]> rec;  // records of the batch (not zero-filled)
     size_t rec_bytes = 0, rec_cap = 0;
-    std::vector<double> x, y, z, m, vx, vy, vz, origin;
-    std::vector<int> type;
-    std::vector<float> link02;
+    HostArray<double> x, y, z, m, vx, vy, vz;
+    HostArray<int> type;
+    HostArray<float> link02;
+    std::vector<double> origin;
     std::vector<pgalf_grid_t> grids;
 };
 
@@ -244,7 +275,6 @@ extern "C" long long pgalf_catalog_next(pgalf_catalog_t *c, long long max_halos,
         c->off.push_back(tot);
         c->roff.push_back((long long)rbytes);
     }
-    // the chosen halos' records: one read per run of consecutive halos
     if (rbytes > c->rec_cap) {
         c->rec.reset(new (std::nothrow) unsigned char[rbytes]);
         c->rec_cap = c->rec ? rbytes : 0;
@@ -254,33 +284,59 @@ extern "C" long long pgalf_catalog_next(pgalf_catalog_t *c, long long max_halos,
         }
     }
     c->rec_bytes = rbytes;
-    for (size_t h = 0; h < c->haloq.size();) {
-        if (skip[h] && std::fseek(c->mf, (long)skip[h], SEEK_CUR) != 0) {
-            set_err(PGALF_ERR_ARG, "%s: member file shorter than the catalogue", where);
-            return -1;
-        }
-        size_t e = h + 1;
-        while (e < c->haloq.size() && skip[e] == 0) e++;
-        const size_t a = (size_t)c->roff[h], n = (size_t)c->roff[e] - a;
-        if (n && std::fread(c->rec.get() + a, 1, n, c->mf) != n) {
-            set_err(PGALF_ERR_ARG, "%s: member file shorter than the catalogue", where);
-            return -1;
-        }
-        h = e;
-    }
-    // the records of small halos after the last chosen one
-    if (gap && std::fseek(c->mf, (long)gap, SEEK_CUR) != 0) {
-        set_err(PGALF_ERR_ARG, "%s: member file shorter than the catalogue", where);
+    // member-file position of each chosen halo's records
+    const long long nh = (long long)c->haloq.size();
+    const off_t base = ftello(c->mf);
+    if (base < 0) {
+        set_err(PGALF_ERR_ARG, "%s: cannot tell the member file's position", where);
         return -1;
     }
-    const long long nh = (long long)c->haloq.size();
-    for (auto *v : {&c->x, &c->y, &c->z, &c->m, &c->vx, &c->vy, &c->vz}) v->resize(tot);
-    c->type.resize(tot);
-    c->link02.resize(tot);
+    std::vector<long long> fpos(nh);
+    long long at = (long long)base;
+    for (long long h = 0; h < nh; h++) {
+        at += (long long)skip[h];
+        fpos[h] = at;
+        at += c->roff[h + 1] - c->roff[h];
+    }
+    const long long end = at + (long long)gap;  // after the small halos that follow
+    bool grown = true;
+    for (auto *v : {&c->x, &c->y, &c->z, &c->m, &c->vx, &c->vy, &c->vz}) grown &= v->resize(tot);
+    grown = grown && c->type.resize(tot) && c->link02.resize(tot);
+    if (!grown) {
+        set_err(PGALF_ERR_NOMEM, "%s: host allocation failed", where);
+        return -1;
+    }
     c->origin.resize(3 * (size_t)nh);
     c->grids.resize(nh);
     const double omep = (double)c->hdr.omep;
+    const int fd = fileno(c->mf);
+    std::atomic<bool> short_file(false);
+    // halos [h0, h1): one positional read per run of consecutive records,
+    // then the particles' fields, link length and localisation
     auto unpack = [&](long long h0, long long h1) {
+        for (long long h = h0; h < h1;) {
+            long long e = h + 1;
+            while (e < h1 && skip[e] == 0) e++;
+            unsigned char *dst = c->rec.get() + c->roff[h];
+            size_t left = (size_t)(c->roff[e] - c->roff[h]);
+            off_t pos = (off_t)fpos[h];
+            while (left) {
+                const ssize_t got = pread(fd, dst, left, pos);
+                if (got <= 0) {
+                    short_file = true;
+                    return;
+                }
+                dst += got;
+                pos += got;
+                left -= (size_t)got;
+            }
+            h = e;
+        }
+        // FREAD's link length of the last mass seen (consecutive particles
+        // mostly share one mass): the same value the formula gives
+        double last_m = 0.0;
+        float last_l = 0.0f;
+        bool have = false;
         for (long long h = h0; h < h1; h++) {
             const pgalf_haloq_t &q = c->haloq[h];
             const unsigned char *r = c->rec.get() + c->roff[h];
@@ -298,18 +354,22 @@ extern "C" long long pgalf_catalog_next(pgalf_catalog_t *c, long long max_halos,
                     c->z[p] = rd_d(r, 16);
                     record_mv(r, pt.type, c->m[p], c->vx[p], c->vy[p], c->vz[p]);
                     c->type[p] = pt.type;
-                    // gfind.c:51: the quotient in long double, C's pow (double)
-                    const double q3 = (double)((long double)c->m[p] / 2.7755e11L /
-                                               (long double)omep);
-                    c->link02[p] =
-                        (float)(0.2 * std::pow(q3, (double)0.33333333333333333333L));
+                    if (!have || !(c->m[p] == last_m)) {
+                        // gfind.c:51: the quotient in long double, C's pow (double)
+                        const double q3 = (double)((long double)c->m[p] / 2.7755e11L /
+                                                   (long double)omep);
+                        last_l = (float)(0.2 * std::pow(q3, (double)0.33333333333333333333L));
+                        last_m = c->m[p];
+                        have = true;
+                    }
+                    c->link02[p] = last_l;
                 }
             }
             localise(c, h);
         }
     };
     // halos split over host threads by particle count
-    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const unsigned nt = loader_threads();
     if (nh > 1 && tot > 200000 && nt > 1) {
         std::vector<std::thread> th;
         long long h = 0;
@@ -318,13 +378,21 @@ extern "C" long long pgalf_catalog_next(pgalf_catalog_t *c, long long max_halos,
             long long e = h;
             while (e < nh && (c->off[e + 1] <= target || e == h)) e++;
             if (t + 1 == nt) e = nh;
-            th.emplace_back(unpack, h, e);
+            try {
+                th.emplace_back(unpack, h, e);
+            } catch (const std::system_error &) {
+                unpack(h, e);
+            }
             h = e;
         }
-        if (h < nh) th.emplace_back(unpack, h, nh);
+        if (h < nh) unpack(h, nh);
         for (auto &t : th) t.join();
     } else {
         unpack(0, nh);
+    }
+    if (short_file || fseeko(c->mf, (off_t)end, SEEK_SET) != 0) {
+        set_err(PGALF_ERR_ARG, "%s: member file shorter than the catalogue", where);
+        return -1;
     }
     if (np_out) *np_out = tot;
     ok();
@@ -337,9 +405,15 @@ extern "C" int pgalf_catalog_fetch(const pgalf_catalog_t *c, double *x, double *
                                    long long *record_offsets, pgalf_haloq_t *haloq,
                                    double *origin, pgalf_grid_t *grids) {
     if (!c) return set_err(PGALF_ERR_ARG, "pgalf_catalog_fetch: bad arguments");
-    const size_t n = c->x.size(), nh = c->haloq.size();
-    auto cp = [](auto *dst, const auto &src) {
-        if (dst && !src.empty()) std::memcpy(dst, src.data(), sizeof(src[0]) * src.size());
+    // the copies, split over host threads by bytes
+    struct Copy {
+        void *dst;
+        const void *src;
+        size_t bytes;
+    };
+    std::vector<Copy> jobs;
+    auto cp = [&jobs](auto *dst, const auto &src) {
+        if (dst && !src.empty()) jobs.push_back({dst, src.data(), sizeof(src[0]) * src.size()});
     };
     cp(x, c->x);
     cp(y, c->y);
@@ -355,8 +429,30 @@ extern "C" int pgalf_catalog_fetch(const pgalf_catalog_t *c, double *x, double *
     cp(haloq, c->haloq);
     cp(origin, c->origin);
     cp(grids, c->grids);
-    (void)n;
-    (void)nh;
+    size_t total = 0;
+    for (const Copy &j : jobs) total += j.bytes;
+    // bytes [lo, hi) of the jobs laid end to end
+    auto run = [&jobs](size_t lo, size_t hi) {
+        size_t at = 0;
+        for (const Copy &j : jobs) {
+            const size_t a = std::max(lo, at), b = std::min(hi, at + j.bytes);
+            if (a < b)
+                std::memcpy((char *)j.dst + (a - at), (const char *)j.src + (a - at), b - a);
+            at += j.bytes;
+        }
+    };
+    const unsigned nt = total >= ((size_t)8 << 20) ? loader_threads() : 1u;
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt; t++) {
+        const size_t lo = total * t / nt, hi = total * (t + 1) / nt;
+        try {
+            th.emplace_back(run, lo, hi);
+        } catch (const std::system_error &) {
+            run(lo, hi);
+        }
+    }
+    run(0, total / nt);
+    for (auto &t : th) t.join();
     return ok();
 }
 

@@ -199,3 +199,97 @@ def test_one_batch_skipping_small_halos(files):
         fa, fe = full["offsets"][h], full["offsets"][h + 1]
         assert np.array_equal(b["x"][a:e], full["x"][fa:fe])
         assert b["grids"][i] == full["grids"][h]
+
+
+def _large_catalogue(d, seed=11):
+    """60 halos (20 below 30 particles, between large ones), 240k particles:
+    DM of two masses in runs, gas of random masses, stars of one mass;
+    three halos span the periodic box."""
+    rng = np.random.default_rng(seed)
+    halos = []
+    for h in range(60):
+        n = int(rng.integers(5, 29)) if h % 3 == 1 else int(rng.integers(2000, 10000))
+        c = rng.uniform(1.0, 99.0, 3)
+        if h % 17 == 0:
+            c[h % 3] = 99.95
+        nd, ng = n * 3 // 5, n // 5
+        ns = n - nd - ng
+
+        def put(r):
+            p = c + rng.normal(0.0, 0.05, (len(r), 3))
+            p %= 100.0
+            r["x"], r["y"], r["z"] = p[:, 0], p[:, 1], p[:, 2]
+            r["vx"] = rng.normal(0.0, 100.0, len(r))
+            return r
+
+        dm = put(np.zeros(nd, dtype=CAT.PM_DTYPE))
+        dm["mass"] = np.where((np.arange(nd) // 50) % 2 == 0, 1.0e8, 2.5e8)
+        gas = put(np.zeros(ng, dtype=C.GAS))
+        gas["mass"] = rng.uniform(1.0e6, 1.0e7, ng).astype(np.float32)
+        star = put(np.zeros(ns, dtype=CAT.PM_DTYPE))
+        star["mass"] = 1.0e6
+        halos.append((dm, gas, np.zeros(0, dtype=np.uint8), star))
+    hp, mp = os.path.join(d, "FoF_halo_cat.00002"), os.path.join(d, "FoF_member_particle.00002")
+    CAT.write_fof_catalog(hp, mp, [100.0, 0.7, 0.3, 0.048, 0.7, 1.0, 1.0], halos)
+    return hp, mp
+
+
+def test_threaded_batch_equals_small_batches():
+    """A batch above the loader's threading size (host threads read their
+    halos' records with positional reads) equals the same halos loaded in
+    batches of at most 60000 particles (read on the calling thread), field
+    by field and record by record, and the reference build's reader where
+    it is present; a member file cut short is an error, not a batch."""
+    d = tempfile.mkdtemp()
+    files = _large_catalogue(d)
+    cat = CAT.Catalog(*files)
+    big = cat.next_batch()
+    assert cat.next_batch() is None
+    cat.close()
+    n = int(big["offsets"][-1])
+    assert n > 200000 and len(big["offsets"]) - 1 == 40
+    cat = CAT.Catalog(*files)
+    parts = []
+    while True:
+        b = cat.next_batch(max_particles=60000)
+        if b is None:
+            break
+        assert b["offsets"][-1] <= 60000 or len(b["offsets"]) == 2
+        parts.append(b)
+    cat.close()
+    assert len(parts) > 4
+    for k in ("x", "y", "z", "mass", "vx", "vy", "vz", "type", "link02", "records"):
+        assert np.array_equal(big[k], np.concatenate([p[k] for p in parts])), k
+    for k in ("origin", "grids", "haloq"):
+        assert np.array_equal(big[k], np.concatenate([p[k] for p in parts])), k
+    sizes = np.concatenate([np.diff(p["offsets"]) for p in parts])
+    assert np.array_equal(np.diff(big["offsets"]), sizes)
+    from oracle import oracle as O
+    import ctypes
+    R = O.ref_lib()
+    if R is not None:
+        R.ref_init(4 << 30)
+        cat = CAT.Catalog(*files)
+        allh = cat.next_batch(min_np=0)
+        cat.close()
+        na = int(allh["offsets"][-1])
+        x, y, z, m = (np.zeros(na) for _ in range(4))
+        ty = np.zeros(na, dtype=np.int32)
+        l2 = np.zeros(na, dtype=np.float32)
+        off = np.zeros(62, dtype=np.int64)
+        org = np.zeros(186)
+        p = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+        R.ref_read_catalog.restype = ctypes.c_longlong
+        assert R.ref_read_catalog(files[0].encode(), files[1].encode(), 61,
+                                  ctypes.c_longlong(na), p(x), p(y), p(z), p(m), p(ty), p(l2),
+                                  p(off), p(org)) == 60
+        for k, v in (("x", x), ("y", y), ("z", z), ("mass", m), ("type", ty), ("link02", l2)):
+            assert np.array_equal(allh[k], v), k
+        assert np.array_equal(allh["origin"].ravel(), org[:180])
+    with open(files[1], "r+b") as f:
+        f.truncate(os.path.getsize(files[1]) - 1000)
+    cat = CAT.Catalog(*files)
+    from galaxyfinder_amd._lib import PgalfError
+    with pytest.raises(PgalfError):
+        cat.next_batch()
+    cat.close()
