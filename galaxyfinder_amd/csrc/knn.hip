@@ -1287,9 +1287,12 @@ __device__ __forceinline__ void query_wave_exact(
 // xcd_block(b, gridDim.x).  kList == true: the blocks run the waves listed in
 // wlist[0 .. *wcount) (those the packed-key kernel flagged), grid-stride --
 // a separate instantiation, so the loop does not cost the plain kernel
-// registers.
+// registers.  The list instantiation runs a few blocks at most and asks for
+// one wave per SIMD, so what does not fit 256 VGPRs goes to AGPRs rather
+// than to scratch memory (a dispatch that needs scratch costs far more than
+// this kernel's own work when its list is empty).
 template <class P, bool kList>
-__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kList ? 1 : 2)))
 knn_query_k(const double *qx, const double *qy, const double *qz, const unsigned *qid,
             const double *cxa, const double *cya, const double *cza, const unsigned *cida,
             const Box *boxes, const KHalo *halos, KHalo halo0, const KWave *waves, int k,

@@ -91,15 +91,20 @@ __device__ __forceinline__ bool stale_bit(const unsigned *stale, int v) {
 __device__ __forceinline__ void entry_test(int mode, int v, int root, float t, const float *den,
                                            const unsigned char *pk, const unsigned *stale,
                                            bool &cand, bool &hit) {
-    cand = hit = false;
-    if (!(den[v] > t)) return;
-    if (mode == kShell) {
-        cand = true;
-        return;
+    // both flags are computed as values and stored once each: a store to
+    // one flag or the other would keep them in scratch memory
+    bool c = false, h = false;
+    if (den[v] > t) {
+        if (mode == kShell) {
+            c = true;
+        } else if (!stale_bit(stale, v)) {
+            const bool peak = pk[v] != 0;
+            c = !peak;
+            h = peak && v != root;
+        }
     }
-    if (stale_bit(stale, v)) return;
-    if (!pk[v]) cand = true;
-    else if (v != root) hit = true;
+    cand = c;
+    hit = h;
 }
 
 struct WaveLds {
