@@ -579,6 +579,55 @@ __global__ void leaf_boxes_k(const double *sx, const double *sy, const double *s
     }
 }
 
+// gather_packed_k and leaf_boxes_k in one pass, for a tree whose candidates
+// are all the sorted queries: thread s gathers sorted particle s and the kB
+// lanes of a bucket reduce its box as leaf_boxes_k does (threads past n
+// hold empty extents).
+template <class P>
+__global__ void gather_leaf_k(const P *packed, const unsigned *perm, long long n, double *sx,
+                              double *sy, double *sz, long long nb, Box *boxes) {
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    if (s < n) {
+        const size_t i = perm[s];
+        double v[3];
+        if constexpr (sizeof(P) == 4) {
+            const float4 f = *reinterpret_cast<const float4 *>(packed + 4 * i);
+            v[0] = (double)f.x;
+            v[1] = (double)f.y;
+            v[2] = (double)f.z;
+        } else {
+            const double2 a = *reinterpret_cast<const double2 *>(packed + 4 * i);
+            const double2 b = *reinterpret_cast<const double2 *>(packed + 4 * i + 2);
+            v[0] = (double)a.x;
+            v[1] = (double)a.y;
+            v[2] = (double)b.x;
+        }
+        sx[s] = v[0];
+        sy[s] = v[1];
+        sz[s] = v[2];
+#pragma unroll
+        for (int a = 0; a < 3; a++) lo[a] = hi[a] = v[a];
+    }
+#pragma unroll
+    for (int o = 1; o < kB; o <<= 1)
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            lo[a] = fmin(lo[a], __shfl_xor(lo[a], o, kB));
+            hi[a] = fmax(hi[a], __shfl_xor(hi[a], o, kB));
+        }
+    const long long b = s / kB;
+    if (s % kB == 0 && b < nb) {
+        Box bx;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            bx.lo[a] = lo[a];
+            bx.hi[a] = hi[a];
+        }
+        boxes[b] = bx;
+    }
+}
+
 __global__ void parent_boxes_k(const Box *child, long long nchild, Box *parent,
                                long long nparent) {
     long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2220,7 +2269,19 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
     double *qx = ws.alloc_n<double>(np), *qy = ws.alloc_n<double>(np),
            *qz = ws.alloc_n<double>(np);
     if (!qx || !qy || !qz) return hipErrorOutOfMemory;
-    gather_packed_k<P><<<nblk(np, 256), 256, 0, s>>>(packed, qv, np, qx, qy, qz);
+    // every particle a candidate: the leaf boxes come with the gather
+    Box *leaf_boxes = nullptr;
+    if (!subset) {
+        const long long nb = (np + kB - 1) / kB;
+        int nlev = 1;
+        while (level_count(nb, nlev - 1) > 1 && nlev <= kMaxLevels) nlev++;
+        leaf_boxes = ws.alloc_n<Box>((size_t)level_offset(nb, nlev));
+        if (!leaf_boxes) return hipErrorOutOfMemory;
+        gather_leaf_k<P><<<nblk(nb * kB, 256), 256, 0, s>>>(packed, qv, np, qx, qy, qz, nb,
+                                                             leaf_boxes);
+    } else {
+        gather_packed_k<P><<<nblk(np, 256), 256, 0, s>>>(packed, qv, np, qx, qy, qz);
+    }
 
     long long nc = np;
     const double *cx = qx, *cy = qy, *cz = qz;
@@ -2270,9 +2331,12 @@ hipError_t knn(const P *x, const P *y, const P *z, const P *m, long long np, con
     if (h0.nlev > kMaxLevels || h0.nlev > kMaxWalkLevels) return hipErrorInvalidValue;
     for (int l = 0; l < h0.nlev; l++) h0.loff[l] = level_offset(nb, l);
     long long total = level_offset(nb, h0.nlev);
-    Box *boxes = ws.alloc_n<Box>((size_t)total);
-    if (!boxes) return hipErrorOutOfMemory;
-    leaf_boxes_k<<<nblk(nb * kB, 256), 256, 0, s>>>(cx, cy, cz, nc, nb, boxes);
+    Box *boxes = leaf_boxes;
+    if (!boxes) {
+        boxes = ws.alloc_n<Box>((size_t)total);
+        if (!boxes) return hipErrorOutOfMemory;
+        leaf_boxes_k<<<nblk(nb * kB, 256), 256, 0, s>>>(cx, cy, cz, nc, nb, boxes);
+    }
     for (int l = 1; l < h0.nlev; l++) {
         long long nl = level_count(nb, l);
         parent_boxes_k<<<nblk(nl, 256), 256, 0, s>>>(boxes + level_offset(nb, l - 1),
