@@ -604,35 +604,70 @@ extern "C" void findDen(pgalf_particle_t *bp, int np, float *densph, double xmin
     consume_status(ws, s, "findDen");
 }
 
-// kNN lists (+ optional density) for host particles.
-static int host_knn(pgalf_particle_t *bp, int np, int k, long long *nearindex, float *dens,
-                    int rho_mode, int cand_ptype, const char *where) {
-    if (np <= 0 || !bp || k < 1 || k > 32) return set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
-    if (k > np) return set_err(PGALF_ERR_SHORT_LIST, "%s: Numnear %d > np %d", where, k, np);
+// One kNN pass of host_knn: k neighbours among the candidates of type
+// cand_ptype (PGALF_TYPE_ALL: every particle), lists and/or density.
+struct HostKnnJob {
+    int k;
+    long long *nearindex;
+    float *dens;
+    int rho_mode;
+    int cand_ptype;
+};
+
+// kNN lists (+ optional density) for host particles: the particles are
+// uploaded once, then the jobs run in order, each one's outputs copied back
+// before the next starts (a job that fails ends the call with its error).
+static int host_knn(pgalf_particle_t *bp, int np, const HostKnnJob *jobs, int njobs,
+                    const char *where) {
+    auto check = [&](const HostKnnJob &jb) {
+        if (np <= 0 || !bp || jb.k < 1 || jb.k > 32)
+            return set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
+        if (jb.k > np)
+            return set_err(PGALF_ERR_SHORT_LIST, "%s: Numnear %d > np %d", where, jb.k, np);
+        return (int)PGALF_OK;
+    };
+    if (njobs < 1) return set_err(PGALF_ERR_ARG, "%s: bad arguments", where);
+    const int first = check(jobs[0]);
+    if (first != PGALF_OK) return first;
     hipStream_t s = nullptr;
     Workspace &ws = workspace_for_current_device();
     WsCall call(ws, s);
     hipError_t e = call.e;
     DeviceParticles d{};
-    long long *dnbr = nullptr;
-    float *drho = nullptr;
     if (e == hipSuccess) e = upload_particles(bp, np, ws, s, d);
-    if (e == hipSuccess && nearindex) {
-        dnbr = ws.alloc_n<long long>((size_t)np * k);
-        if (!dnbr) e = hipErrorOutOfMemory;
-    }
-    if (e == hipSuccess && dens) {
-        drho = ws.alloc_n<float>(np);
-        if (!drho) e = hipErrorOutOfMemory;
-    }
-    if (e == hipSuccess)
-        e = knn<double>(d.x, d.y, d.z, d.m, np, cand_ptype == PGALF_TYPE_ALL ? nullptr : d.type,
-                        cand_ptype, k, dnbr, drho, rho_mode, ws, s);
-    if (e == hipSuccess && dnbr)
-        e = dev_to_host(nearindex, dnbr, sizeof(long long) * (size_t)np * k, s);
-    if (e == hipSuccess && drho) e = dev_to_host(dens, drho, sizeof(float) * np, s);
     if (e != hipSuccess) return hip_err(e, where);
-    return consume_status(ws, s, where);
+    for (int j = 0; j < njobs; j++) {
+        const HostKnnJob &jb = jobs[j];
+        const int ok_job = check(jb);
+        if (ok_job != PGALF_OK) return ok_job;
+        long long *dnbr = nullptr;
+        float *drho = nullptr;
+        if (jb.nearindex) {
+            dnbr = ws.alloc_n<long long>((size_t)np * jb.k);
+            if (!dnbr) e = hipErrorOutOfMemory;
+        }
+        if (e == hipSuccess && jb.dens) {
+            drho = ws.alloc_n<float>(np);
+            if (!drho) e = hipErrorOutOfMemory;
+        }
+        if (e == hipSuccess)
+            e = knn<double>(d.x, d.y, d.z, d.m, np,
+                            jb.cand_ptype == PGALF_TYPE_ALL ? nullptr : d.type, jb.cand_ptype,
+                            jb.k, dnbr, drho, jb.rho_mode, ws, s);
+        if (e == hipSuccess && dnbr)
+            e = dev_to_host(jb.nearindex, dnbr, sizeof(long long) * (size_t)np * jb.k, s);
+        if (e == hipSuccess && drho) e = dev_to_host(jb.dens, drho, sizeof(float) * np, s);
+        if (e != hipSuccess) return hip_err(e, where);
+        const int st = consume_status(ws, s, where);
+        if (st != PGALF_OK) return st;
+    }
+    return PGALF_OK;
+}
+
+static int host_knn(pgalf_particle_t *bp, int np, int k, long long *nearindex, float *dens,
+                    int rho_mode, int cand_ptype, const char *where) {
+    const HostKnnJob job{k, nearindex, dens, rho_mode, cand_ptype};
+    return host_knn(bp, np, &job, 1, where);
 }
 
 extern "C" void findsphdensity(pgalf_particle_t *bp, int np, long long *nearindex, int Numnear,
@@ -657,12 +692,12 @@ extern "C" void starfindsphdensity(pgalf_particle_t *bp, int np, long long *near
     // NUMNEARDEN = 10 nearest stars (params.h:73); fewer when fewer stars exist,
     // as Find_Near returns them (nnost.c:698-703).
     int kden = nstar < 10 ? nstar : 10;
-    if (densph &&
-        host_knn(bp, np, kden, nullptr, densph, 2, PGALF_TYPE_STAR, "starfindsphdensity") !=
-            PGALF_OK)
-        return;
-    if (nearindex)
-        host_knn(bp, np, Numnear, nearindex, nullptr, 0, PGALF_TYPE_ALL, "starfindsphdensity");
+    // one upload for both passes: the star densities, then the lists
+    HostKnnJob jobs[2];
+    int nj = 0;
+    if (densph) jobs[nj++] = {kden, nullptr, densph, 2, PGALF_TYPE_STAR};
+    if (nearindex) jobs[nj++] = {Numnear, nearindex, nullptr, 0, PGALF_TYPE_ALL};
+    if (nj) host_knn(bp, np, jobs, nj, "starfindsphdensity");
 }
 
 extern "C" int pgalf_near_lists(pgalf_particle_t *bp, int np, int Numnear, long long *nearindex) {
